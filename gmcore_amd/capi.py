@@ -1,0 +1,113 @@
+"""ctypes mirror of include/gmcore_hip.h (structs, enums, signatures).
+
+The C ABI is the drop-in boundary of the dynamics path; this module only
+describes it.  ``load_hip()`` opens the in-tree ``libgmcore_hip.so`` and fails
+loudly when it has not been built -- there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_HIP = os.path.join(HERE, "lib", "libgmcore_hip.so")
+LIB_HOST = os.path.join(HERE, "lib", "libgmcore_host.so")
+
+# error codes
+GMC_OK, GMC_EINVAL, GMC_EDEVICE, GMC_ENOMEM, GMC_ENUMERIC, GMC_ESTATE = range(6)
+
+MESH_FIELDS = [
+    "full_lat", "half_lat", "full_lat_deg", "half_lat_deg",
+    "full_cos_lat", "half_cos_lat", "full_sin_lat", "half_sin_lat",
+    "area_cell", "area_lon", "area_lon_west", "area_lon_east",
+    "area_lon_north", "area_lon_south",
+    "area_lat", "area_lat_west", "area_lat_east", "area_lat_north",
+    "area_lat_south", "area_vtx", "area_subcell1", "area_subcell2",
+    "de_lon", "le_lon", "de_lat", "le_lat",
+    "full_f", "half_f",
+    "full_tangent_wgt1", "full_tangent_wgt2",
+    "half_tangent_wgt1", "half_tangent_wgt2",
+]
+MESH = {name: k for k, name in enumerate(MESH_FIELDS)}
+GMC_MESH_NFIELDS = len(MESH_FIELDS)
+
+GMC_TIME_PC2, GMC_TIME_WRFRK3, GMC_TIME_EULER = 0, 1, 2
+GMC_PV_MIDPOINT, GMC_PV_UPWIND = 0, 1
+
+FIELDS = ["u_lon", "v_lat", "gz", "u_f", "v_f", "gz_f", "gzs", "du", "dv",
+          "dgz", "pv", "ke", "div", "m"]
+FIELD = {name: k for k, name in enumerate(FIELDS)}
+# grid of each field: "U" (nlat rows), "V" (nlat-1), "C" (nlat), "X" (nlat-1)
+FIELD_GRID = {"u_lon": "U", "v_lat": "V", "gz": "C", "u_f": "U", "v_f": "V",
+              "gz_f": "C", "gzs": "C", "du": "U", "dv": "V", "dgz": "C",
+              "pv": "X", "ke": "C", "div": "C", "m": "C"}
+
+
+def field_rows(name: str, nlat: int) -> int:
+    return nlat - 1 if FIELD_GRID[name] in ("V", "X") else nlat
+
+
+class GmcMesh(C.Structure):
+    _fields_ = [("nlon", C.c_int32), ("nlat", C.c_int32), ("dlon", C.c_double),
+                ("metrics", C.POINTER(C.c_double))]
+
+
+class GmcFilter(C.Structure):
+    _fields_ = [("max_taps", C.c_int32),
+                ("ngrid_lon", C.POINTER(C.c_int32)), ("wgt_lon", C.POINTER(C.c_double)),
+                ("ngrid_lat", C.POINTER(C.c_int32)), ("wgt_lat", C.POINTER(C.c_double))]
+
+
+class GmcConfig(C.Structure):
+    _fields_ = [("g", C.c_double), ("time_scheme", C.c_int32), ("pv_scheme", C.c_int32),
+                ("upwind_order_pv", C.c_int32), ("upwind_wgt_pv", C.c_double),
+                ("pv_pole_stokes", C.c_int32), ("coriolis_scheme", C.c_int32),
+                ("use_div_damp", C.c_int32), ("div_damp_order", C.c_int32),
+                ("div_damp_coef2", C.c_double),
+                ("big_filter", GmcFilter), ("small_filter1", GmcFilter)]
+
+
+def dptr(a):
+    """double* of a contiguous float64 numpy array."""
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+_hip = None
+
+
+def load_hip() -> C.CDLL:
+    """Open libgmcore_hip.so (built by __graft_entry__.build()).  Raises if absent."""
+    global _hip
+    if _hip is not None:
+        return _hip
+    if not os.path.exists(LIB_HIP):
+        raise RuntimeError(f"{LIB_HIP} is not built; run __graft_entry__.build() first")
+    lib = C.CDLL(LIB_HIP)
+    vp = C.c_void_p
+    P = C.POINTER
+    sig = {
+        "gmc_create": (C.c_int, [P(GmcMesh), P(GmcConfig), P(vp)]),
+        "gmc_destroy": (C.c_int, [vp]),
+        "gmc_last_error": (C.c_char_p, [vp]),
+        "gmc_upload": (C.c_int, [vp, C.c_int32, C.c_int32, P(C.c_double), C.c_int64]),
+        "gmc_download": (C.c_int, [vp, C.c_int32, C.c_int32, P(C.c_double), C.c_int64]),
+        "gmc_prepare": (C.c_int, [vp, C.c_int32]),
+        "gmc_space_operators": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                          C.c_int32, C.c_double, C.c_int32]),
+        "gmc_update_state": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32]),
+        "gmc_step": (C.c_int, [vp, C.c_double, C.c_int32]),
+        "gmc_time_integrate": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_double]),
+        "gmc_current_level": (C.c_int, [vp, P(C.c_int32)]),
+        "gmc_diagnose": (C.c_int, [vp, C.c_int32, P(C.c_double)]),
+        "gmc_synchronize": (C.c_int, [vp]),
+        "gmc_field_ptr": (C.c_int, [vp, C.c_int32, C.c_int32, P(P(C.c_double)), P(C.c_int64)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype, fn.argtypes = res, args
+    _hip = lib
+    return lib

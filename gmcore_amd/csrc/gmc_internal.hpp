@@ -1,0 +1,121 @@
+// gmc_internal.hpp -- device context and shared device formulas of the
+// gfx950 shallow-water dynamics library.
+//
+// Layout in HBM: every field is a dense row-major array [rows][pitch] of
+// doubles, longitude fastest, no halo.  Periodic longitude neighbours are
+// reached by modular column arithmetic; the cross-pole halo rows of the
+// reference (parallel_mod.F90:529-563) by mirrored row indices with a
+// half-circle column shift.  pitch is nlon rounded up to 32 doubles (256 B),
+// so every row starts on a 256-byte boundary.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+
+#include "../../include/gmcore_hip.h"
+
+namespace gmc {
+
+constexpr double kEps = 2.220446049250313e-16;  // epsilon(1.0_r8)
+constexpr int kWave = 64;
+
+// state_type fields kept on the device (state_mod.F90:22-59)
+enum StateField {
+  S_U_LON, S_V_LAT, S_GZ, S_GZ_F, S_U_F, S_V_F, S_M, S_M_LON, S_M_LAT, S_M_VTX,
+  S_MFX_LON, S_MFY_LAT, S_MFX_LAT, S_MFY_LON, S_U_LAT, S_V_LON, S_KE, S_DIV,
+  S_VOR, S_PV, S_PV_LON, S_PV_LAT, S_NFIELDS
+};
+// tend_type fields (tend_mod.F90:16-42)
+enum TendField {
+  T_DU, T_DV, T_DGZ, T_QHU, T_QHV, T_DKEDLON, T_DKEDLAT, T_DMFDLON, T_DMFDLAT,
+  T_PGF_LON, T_PGF_LAT, T_NFIELDS
+};
+
+// Read-only view of the packed metric table: M(f, r).
+struct Mesh {
+  const double *m;
+  int nlat;
+  __device__ __forceinline__ double operator()(int f, int r) const { return m[f * nlat + r]; }
+};
+
+struct FilterDev {
+  int max_taps;
+  const int32_t *ngrid_lon, *ngrid_lat;
+  const double *wgt_lon, *wgt_lat;
+};
+
+// Geometry shared by all kernels of a context.
+struct Grid {
+  int nlon, nlat, nhalf;
+  int64_t pitch;
+  int half;  // nlon / 2
+};
+
+// Column i+d, periodic.
+__device__ __forceinline__ int wrap(int i, int n) {
+  return i < 0 ? i + n : (i >= n ? i - n : i);
+}
+
+// Row index through the cross-pole mirror (halo_mod.F90:183-238 on one rank):
+// full-lat row -k <- row k, row nlat-1+k <- nlat-1-k; half-lat row -k <- k-1,
+// nhalf-1+k <- nhalf-k.  Mirrored rows take the column half a circle away.
+__device__ __forceinline__ void mirror_full(int &r, int &i, const Grid &g) {
+  if (r < 0) { r = -r; i = wrap(i + g.half, g.nlon); }
+  else if (r > g.nlat - 1) { r = 2 * (g.nlat - 1) - r; i = wrap(i + g.half, g.nlon); }
+}
+__device__ __forceinline__ void mirror_half(int &r, int &i, const Grid &g) {
+  if (r < 0) { r = -1 - r; i = wrap(i + g.half, g.nlon); }
+  else if (r > g.nhalf - 1) { r = 2 * g.nhalf - 1 - r; i = wrap(i + g.half, g.nlon); }
+}
+
+// ---- point formulas (one per reference expression) ----------------------
+// All use the source's left-to-right evaluation; the library is built with
+// -ffp-contract=off so each product and sum is rounded separately.
+
+// upwind1 / upwind3, upwind_mod.F90:14-39
+__device__ __forceinline__ double upwind1(double dir, double wgt, double f1, double f2) {
+  return 0.5 * (f2 + f1) + -0.5 * (f2 - f1) * wgt * dir;
+}
+__device__ __forceinline__ double upwind3(double dir, double wgt, double f1, double f2,
+                                          double f3, double f4) {
+  const double c31 = 7.0 / 12.0, c32 = -1.0 / 12.0, c33 = 1.0 / 12.0;
+  return c31 * (f3 + f2) + c32 * (f4 + f1) + c33 * (f4 - f1 - 3 * (f3 - f2)) * wgt * dir;
+}
+}  // namespace gmc
+
+// ---- device context ------------------------------------------------------
+struct gmc_ctx {
+  gmc::Grid grid;
+  gmc_config cfg;            // host copy; filter pointers point at device tables
+  double dlon;
+  double *d_mesh = nullptr;  // GMC_MESH_NFIELDS * nlat
+  gmc::FilterDev big{}, small1{};
+  void *d_filter_mem[8] = {};
+  double *state[4][gmc::S_NFIELDS] = {};
+  double *tend[4][gmc::T_NFIELDS] = {};
+  double *gzs = nullptr;
+  double *d_pole = nullptr;  // pole sums scratch (see gmc_ops.hip)
+  double *d_diag = nullptr;  // diagnose partial sums
+  double *slab = nullptr;    // one allocation holding every field
+  size_t slab_bytes = 0;
+  hipStream_t stream = nullptr;
+  int old_level = 1, new_level = 2;
+  int prepared = 0;
+  std::string err;
+};
+
+namespace gmc {
+int set_error(gmc_ctx *ctx, int code, const std::string &msg);
+int check_hip(gmc_ctx *ctx, hipError_t e, const char *what);
+// operator-level path (gmc_ops.hip)
+int ops_prepare(gmc_ctx *ctx, int level);
+int ops_space_operators(gmc_ctx *ctx, int old_l, int star_l, int new_l, int t1, int t2, double dt,
+                        int pass);
+int ops_update_state(gmc_ctx *ctx, int tend, int old_l, int new_l, double dt, int what);
+int ops_div_damp(gmc_ctx *ctx, int level);
+int ops_diagnose(gmc_ctx *ctx, int level, double out[4]);
+int ops_integrate(gmc_ctx *ctx, int old_l, int new_l, double dt);
+// fused stage path (gmc_fused.hip)
+int fused_integrate(gmc_ctx *ctx, int old_l, int new_l, double dt);
+}  // namespace gmc

@@ -1,0 +1,132 @@
+// gmc_point.hpp -- point formulas of the shallow-water operators.
+//
+// One inline function per reference expression, written in the source's
+// left-to-right order so that the operator-level kernels and the fused stage
+// kernels produce the same bits.  Row r is 0-based (reference j = r + 1).
+#pragma once
+
+#include "gmc_internal.hpp"
+
+namespace gmc {
+
+// calc_m, operators_mod.F90:530: m = (gz - gzs) / g
+__device__ __forceinline__ double m_of(double gz, double gzs, double g) { return (gz - gzs) / g; }
+
+// interp_cell_to_vtx, interp_mod.F90:455-459
+__device__ __forceinline__ double m_vtx_of(double m00, double m01, double m10, double m11,
+                                           const Mesh &M, int r) {
+  return ((m00 + m01) * M(GMC_AREA_SUBCELL2, r) + (m10 + m11) * M(GMC_AREA_SUBCELL1, r + 1)) /
+         M(GMC_AREA_VTX, r);
+}
+
+// w1 * (a + b) + w2 * (c + d): mfx_lat / mfy_lon, operators_mod.F90:585-600
+__device__ __forceinline__ double tangent_sum(double w1, double w2, double a, double b, double c,
+                                              double d) {
+  return w1 * (a + b) + w2 * (c + d);
+}
+
+// calc_ke, ke_scheme 1, operators_mod.F90:247-252.  uw/ue: u west/east of
+// the cell; vs/vn: v on the south/north edge.
+__device__ __forceinline__ double ke_of(double uw, double ue, double vs, double vn, const Mesh &M,
+                                        int r) {
+  return (M(GMC_AREA_LON_WEST, r) * (uw * uw) + M(GMC_AREA_LON_EAST, r) * (ue * ue) +
+          M(GMC_AREA_LAT_NORTH, r - 1) * (vs * vs) + M(GMC_AREA_LAT_SOUTH, r) * (vn * vn)) /
+         M(GMC_AREA_CELL, r);
+}
+
+// calc_div, operators_mod.F90:370-375
+__device__ __forceinline__ double div_of(double ue, double uw, double vn, double vs, const Mesh &M,
+                                         int r) {
+  return ((ue * M(GMC_LE_LON, r) - uw * M(GMC_LE_LON, r)) +
+          (vn * M(GMC_LE_LAT, r) - vs * M(GMC_LE_LAT, r - 1))) /
+         M(GMC_AREA_CELL, r);
+}
+
+// calc_vor, operators_mod.F90:629-634 (vertex row r)
+__device__ __forceinline__ double vor_of(double us, double un, double ve, double vw, const Mesh &M,
+                                         int r) {
+  return (us * M(GMC_DE_LON, r) - un * M(GMC_DE_LON, r + 1) + ve * M(GMC_DE_LAT, r) -
+          vw * M(GMC_DE_LAT, r)) /
+         M(GMC_AREA_VTX, r);
+}
+
+// calc_pv, operators_mod.F90:697
+__device__ __forceinline__ double pv_of(double vor, const Mesh &M, int r, double m_vtx) {
+  return (vor + M(GMC_HALF_F, r)) / m_vtx;
+}
+
+// interp_pv_upwind, operators_mod.F90:764-790.  t: wind along the stencil
+// (v_lon for pv_lon, u_lat for pv_lat), n: the other component; lat_edge
+// selects the operand order under the square root.  f1..f4: pv along the
+// stencil; the centred fallback averages f2 and f3.
+__device__ __forceinline__ double pv_upwind_edge(int order, double wgt, double t, double n,
+                                                 int lat_edge, double f1, double f2, double f3,
+                                                 double f4) {
+  const double b = fabs(t) / (sqrt(lat_edge ? t * t + n * n : n * n + t * t) + kEps);
+  const double dir = copysign(1.0, t);
+  const double up = order == 1 ? upwind1(dir, wgt, f2, f3) : upwind3(dir, wgt, f1, f2, f3, f4);
+  return b * up + (1 - b) * 0.5 * (f2 + f3);
+}
+
+// calc_coriolis scheme 1, operators_mod.F90:866-877 (V point, row r)
+__device__ __forceinline__ double qhu_of(const Mesh &M, int r, double pl, double mfx_w, double pvl_w,
+                                         double mfx_e, double pvl_e, double mfx_nw, double pvl_nw,
+                                         double mfx_ne, double pvl_ne) {
+  return (M(GMC_HALF_TANGENT_WGT1, r) * (mfx_w * (pl + pvl_w) + mfx_e * (pl + pvl_e)) +
+          M(GMC_HALF_TANGENT_WGT2, r) * (mfx_nw * (pl + pvl_nw) + mfx_ne * (pl + pvl_ne))) *
+         0.5;
+}
+
+// calc_coriolis scheme 1, operators_mod.F90:887-898 (U point, row r)
+__device__ __forceinline__ double qhv_of(const Mesh &M, int r, double pl, double mfy_sw, double pvt_sw,
+                                         double mfy_se, double pvt_se, double mfy_nw, double pvt_nw,
+                                         double mfy_ne, double pvt_ne) {
+  return (M(GMC_FULL_TANGENT_WGT1, r) * (mfy_sw * (pl + pvt_sw) + mfy_se * (pl + pvt_se)) +
+          M(GMC_FULL_TANGENT_WGT2, r) * (mfy_nw * (pl + pvt_nw) + mfy_ne * (pl + pvt_ne))) *
+         0.5;
+}
+
+// calc_grad_mf, operators_mod.F90:956-970
+__device__ __forceinline__ double dmfdlon_of(double mfx_e, double mfx_w, const Mesh &M, int r) {
+  return (mfx_e - mfx_w) * M(GMC_LE_LON, r) / M(GMC_AREA_CELL, r);
+}
+__device__ __forceinline__ double dmfdlat_of(double mfy_n, double mfy_s, const Mesh &M, int r) {
+  return (mfy_n * M(GMC_LE_LAT, r) - mfy_s * M(GMC_LE_LAT, r - 1)) / M(GMC_AREA_CELL, r);
+}
+
+// pgf_swm_run, pgf_swm_mod.F90:33, :40
+__device__ __forceinline__ double pgf_lon_of(double gz_e, double gz_w, const Mesh &M, int r) {
+  return (gz_e - gz_w) / M(GMC_DE_LON, r);
+}
+__device__ __forceinline__ double pgf_lat_of(double gz_n, double gz_s, const Mesh &M, int r) {
+  return (gz_n - gz_s) / M(GMC_DE_LAT, r);
+}
+
+// pole-row v smoothing, time_schemes_mod.F90:225-233 (wgt = 0.6)
+__device__ __forceinline__ double pole_smooth(double v, double v_next) {
+  const double wgt = 0.6;
+  return (1 - wgt) * v + wgt * v_next;
+}
+
+// one output of the zonal filter, filter_mod.F90:41: taps in ascending
+// order from x[0] (= column i - hn)
+__device__ __forceinline__ double filter_point(const double *__restrict__ w, int n, const double *x) {
+  double acc = 0.0;
+  for (int k = 0; k < n; k++) acc = acc + w[k] * x[k];
+  return acc;
+}
+
+// Sum of f(i), i = 0..n-1, over one wavefront in the library's fixed order:
+// lane l adds f(l), f(l+64), ... in ascending order, then an xor butterfly
+// combines the 64 partials (pairs (0,1), then (0,2), ...).  Every lane
+// returns the same value.  Stands in for zonal_sum (parallel_mod.F90:741-764)
+// whose order the reference leaves to the compiler and MPI.
+template <class F>
+__device__ __forceinline__ double wave_zonal_sum(int n, int lane, F f) {
+  double acc = 0.0;
+  for (int i = lane; i < n; i += kWave) acc = acc + f(i);
+  for (int w = 1; w < kWave; w <<= 1) acc = acc + __shfl_xor(acc, w, kWave);
+  return acc;
+}
+
+}  // namespace gmc

@@ -1,0 +1,108 @@
+! Thin ISO_C_BINDING layer over the C ABI of libgmcore_hip.so
+! (include/gmcore_hip.h).  Every function returns 0 on success; gmc_check
+! turns a non-zero code into the message of gmc_last_error and stops, which
+! keeps the reference's abort-on-error behaviour (log_error / process_stop,
+! src/utils/process_mod.F90:54-68).
+module gmc_hip_mod
+
+  use, intrinsic :: iso_c_binding
+  implicit none
+
+  integer(c_int32_t), parameter :: GMC_U_LON = 0, GMC_V_LAT = 1, GMC_GZ = 2, GMC_U_F = 3, &
+    GMC_V_F = 4, GMC_GZ_F = 5, GMC_GZS = 6, GMC_DU = 7, GMC_DV = 8, GMC_DGZ = 9, GMC_PV = 10, &
+    GMC_KE = 11, GMC_DIV = 12, GMC_M = 13
+  integer(c_int32_t), parameter :: GMC_TIME_PC2 = 0, GMC_TIME_WRFRK3 = 1, GMC_TIME_EULER = 2
+  integer(c_int32_t), parameter :: GMC_PV_MIDPOINT = 0, GMC_PV_UPWIND = 1
+
+  type, bind(C) :: gmc_mesh
+    integer(c_int32_t) :: nlon, nlat
+    real(c_double) :: dlon
+    type(c_ptr) :: metrics
+  end type gmc_mesh
+
+  type, bind(C) :: gmc_filter
+    integer(c_int32_t) :: max_taps
+    type(c_ptr) :: ngrid_lon, wgt_lon, ngrid_lat, wgt_lat
+  end type gmc_filter
+
+  type, bind(C) :: gmc_config
+    real(c_double) :: g
+    integer(c_int32_t) :: time_scheme, pv_scheme, upwind_order_pv
+    real(c_double) :: upwind_wgt_pv
+    integer(c_int32_t) :: pv_pole_stokes, coriolis_scheme, use_div_damp, div_damp_order
+    real(c_double) :: div_damp_coef2
+    type(gmc_filter) :: big_filter, small_filter1
+  end type gmc_config
+
+  interface
+    integer(c_int) function gmc_create(mesh, cfg, ctx) bind(C, name='gmc_create')
+      import; type(gmc_mesh), intent(in) :: mesh; type(gmc_config), intent(in) :: cfg
+      type(c_ptr), intent(out) :: ctx
+    end function
+    integer(c_int) function gmc_destroy(ctx) bind(C, name='gmc_destroy')
+      import; type(c_ptr), value :: ctx
+    end function
+    type(c_ptr) function gmc_last_error(ctx) bind(C, name='gmc_last_error')
+      import; type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function gmc_upload(ctx, level, field, host, ld) bind(C, name='gmc_upload')
+      import; type(c_ptr), value :: ctx; integer(c_int32_t), value :: level, field
+      real(c_double), intent(in) :: host(*); integer(c_int64_t), value :: ld
+    end function
+    integer(c_int) function gmc_download(ctx, level, field, host, ld) bind(C, name='gmc_download')
+      import; type(c_ptr), value :: ctx; integer(c_int32_t), value :: level, field
+      real(c_double), intent(out) :: host(*); integer(c_int64_t), value :: ld
+    end function
+    integer(c_int) function gmc_prepare(ctx, level) bind(C, name='gmc_prepare')
+      import; type(c_ptr), value :: ctx; integer(c_int32_t), value :: level
+    end function
+    integer(c_int) function gmc_space_operators(ctx, old, star, new, tend1, tend2, dt, pass) &
+        bind(C, name='gmc_space_operators')
+      import; type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: old, star, new, tend1, tend2, pass; real(c_double), value :: dt
+    end function
+    integer(c_int) function gmc_update_state(ctx, tend, old, new, dt, what) bind(C, name='gmc_update_state')
+      import; type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tend, old, new, what; real(c_double), value :: dt
+    end function
+    integer(c_int) function gmc_time_integrate(ctx, old, new, dt) bind(C, name='gmc_time_integrate')
+      import; type(c_ptr), value :: ctx; integer(c_int32_t), value :: old, new; real(c_double), value :: dt
+    end function
+    integer(c_int) function gmc_step(ctx, dt, nsteps) bind(C, name='gmc_step')
+      import; type(c_ptr), value :: ctx; real(c_double), value :: dt; integer(c_int32_t), value :: nsteps
+    end function
+    integer(c_int) function gmc_current_level(ctx, old) bind(C, name='gmc_current_level')
+      import; type(c_ptr), value :: ctx; integer(c_int32_t), intent(out) :: old
+    end function
+    integer(c_int) function gmc_diagnose(ctx, level, out) bind(C, name='gmc_diagnose')
+      import; type(c_ptr), value :: ctx; integer(c_int32_t), value :: level
+      real(c_double), intent(out) :: out(4)
+    end function
+    integer(c_int) function gmc_synchronize(ctx) bind(C, name='gmc_synchronize')
+      import; type(c_ptr), value :: ctx
+    end function
+  end interface
+
+contains
+
+  subroutine gmc_check(ctx, rc, what)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: what
+    character(kind=c_char), pointer :: msg(:)
+    integer n
+    if (rc == 0) return
+    write(*, '(a, a, a, i0)') 'gmc error in ', what, ': code ', rc
+    if (c_associated(ctx)) then
+      call c_f_pointer(gmc_last_error(ctx), msg, [512])
+      n = 0
+      do while (n < 512)
+        if (msg(n + 1) == c_null_char) exit
+        n = n + 1
+      end do
+      write(*, '(512a)') msg(1:n)
+    end if
+    error stop 1
+  end subroutine gmc_check
+
+end module gmc_hip_mod

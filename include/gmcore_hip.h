@@ -1,0 +1,186 @@
+/*
+ * gmcore_hip.h -- C ABI of the MI355X (gfx950) dynamics library.
+ *
+ * The library is the GPU target of GMCORE's dynamics seam: the procedure
+ * pointers `time_integrator`, `step` and `space_operators`
+ * (src/dynamics/time_schemes_mod.F90:23-59, src/dynamics/gmcore_mod.F90:41,73)
+ * and the routine `update_state` (time_schemes_mod.F90:120).  A Fortran host
+ * binds these functions with ISO_C_BINDING `bind(C)` interfaces.
+ *
+ * Conventions
+ *  - All floating point is IEEE binary64.
+ *  - Row index r is 0-based and counts from the south: full-latitude row r is
+ *    the reference's j = r + 1 (r = 0 is the south pole cell row, r = nlat-1
+ *    the north pole); half-latitude row r is the reference's half j = r + 1
+ *    (r = 0 .. nlat-2).
+ *  - Host field buffers hold the interior only, longitude fastest:
+ *    element (i, r) at host[r * ld + i], i = 0 .. nlon-1 being the
+ *    reference's i = 1 .. nlon.  `ld` >= nlon.  This is the interior of the
+ *    reference's column-major, halo-padded array (mesh_mod.F90:597-609) read
+ *    one latitude row at a time.
+ *  - Every function returns 0 on success and a positive GMC_E* code on
+ *    failure; gmc_last_error() gives the message.  The reference aborts
+ *    instead (log_error / process_stop, src/utils/process_mod.F90:54-68); the
+ *    Fortran shim turns a non-zero code back into its log_error call.
+ *  - A context belongs to one host thread and one HIP stream; it is not
+ *    re-entrant.
+ */
+#ifndef GMCORE_HIP_H
+#define GMCORE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------- */
+enum {
+  GMC_OK = 0,
+  GMC_EINVAL = 1,     /* bad argument (size, index, null pointer)            */
+  GMC_EDEVICE = 2,    /* HIP runtime error or no usable gfx950 device         */
+  GMC_ENOMEM = 3,     /* device allocation failed                            */
+  GMC_ENUMERIC = 4,   /* a device guard tripped (NaN, CFL beyond bounds ...)  */
+  GMC_ESTATE = 5      /* call out of order (e.g. step before upload)         */
+};
+
+/* ---- mesh metrics -------------------------------------------------------
+ * Packed 1-D metric table: metrics[f * nlat + r] for field f, row r.  Field
+ * names are those of mesh_type (src/dynamics/mesh_mod.F90:15-128).  Full-
+ * latitude fields use r = 0..nlat-1, half-latitude fields r = 0..nlat-2.
+ * The host computes them once (mesh_init_global, mesh_mod.F90:134-447). */
+enum gmc_mesh_field {
+  GMC_FULL_LAT = 0, GMC_HALF_LAT, GMC_FULL_LAT_DEG, GMC_HALF_LAT_DEG,
+  GMC_FULL_COS_LAT, GMC_HALF_COS_LAT, GMC_FULL_SIN_LAT, GMC_HALF_SIN_LAT,
+  GMC_AREA_CELL, GMC_AREA_LON, GMC_AREA_LON_WEST, GMC_AREA_LON_EAST,
+  GMC_AREA_LON_NORTH, GMC_AREA_LON_SOUTH,
+  GMC_AREA_LAT, GMC_AREA_LAT_WEST, GMC_AREA_LAT_EAST, GMC_AREA_LAT_NORTH,
+  GMC_AREA_LAT_SOUTH, GMC_AREA_VTX, GMC_AREA_SUBCELL1, GMC_AREA_SUBCELL2,
+  GMC_DE_LON, GMC_LE_LON, GMC_DE_LAT, GMC_LE_LAT,
+  GMC_FULL_F, GMC_HALF_F,
+  GMC_FULL_TANGENT_WGT1, GMC_FULL_TANGENT_WGT2,
+  GMC_HALF_TANGENT_WGT1, GMC_HALF_TANGENT_WGT2,
+  GMC_MESH_NFIELDS
+};
+
+typedef struct gmc_mesh {
+  int32_t nlon;            /* num_full_lon (even)                           */
+  int32_t nlat;            /* num_full_lat, poles included                  */
+  double dlon;             /* mesh%dlon                                     */
+  const double *metrics;   /* GMC_MESH_NFIELDS * nlat doubles              */
+} gmc_mesh;
+
+/* ---- zonal filter table -------------------------------------------------
+ * One filter_type (src/dynamics/filter_types_mod.F90:15-27): per full row the
+ * tap count ngrid_lon[r] and weights wgt_lon[r * max_taps + k]; per half row
+ * ngrid_lat / wgt_lat likewise.  A row with ngrid <= 1 is left unfiltered
+ * (filter_mod.F90:25-50). */
+typedef struct gmc_filter {
+  int32_t max_taps;
+  const int32_t *ngrid_lon;  /* nlat     */
+  const double *wgt_lon;     /* nlat     * max_taps */
+  const int32_t *ngrid_lat;  /* nlat - 1 */
+  const double *wgt_lat;     /* (nlat-1) * max_taps */
+} gmc_filter;
+
+/* scheme switches: namelist keys of &gmcore_control
+ * (src/utils/namelist_mod.F90:10-122) that select code on the SW path */
+enum { GMC_TIME_PC2 = 0, GMC_TIME_WRFRK3 = 1, GMC_TIME_EULER = 2 };
+enum { GMC_PV_MIDPOINT = 0, GMC_PV_UPWIND = 1 };
+
+typedef struct gmc_config {
+  double g;                  /* gravity, const_mod.F90:61                     */
+  int32_t time_scheme;       /* GMC_TIME_*   (time_schemes_mod.F90:67-76)     */
+  int32_t pv_scheme;         /* GMC_PV_*     (operators_mod.F90:56-71)        */
+  int32_t upwind_order_pv;   /* 1 or 3       (operators_mod.F90:738-795)      */
+  double upwind_wgt_pv;
+  int32_t pv_pole_stokes;    /* 0/1          (operators_mod.F90:640-674)      */
+  int32_t coriolis_scheme;   /* 1 or 2       (operators_mod.F90:846-906)      */
+  int32_t use_div_damp;      /* 0/1          (damp_mod.F90:56-58)             */
+  int32_t div_damp_order;    /* 2 only on this path (div_damp_mod.F90:89-139) */
+  double div_damp_coef2;
+  gmc_filter big_filter;     /* block%big_filter    (block_mod.F90:65)        */
+  gmc_filter small_filter1;  /* block%small_filter1 (block_mod.F90:66)        */
+} gmc_config;
+
+/* ---- fields of state_type / tend_type that cross the boundary ----------- */
+enum gmc_field {
+  GMC_U_LON = 0,  /* state%u_lon  U grid, nlat rows (pole rows unused) */
+  GMC_V_LAT,      /* state%v_lat  V grid, nlat-1 rows                  */
+  GMC_GZ,         /* state%gz     cells,  nlat rows                    */
+  GMC_U_F,        /* state%u_f    filtered u                           */
+  GMC_V_F,        /* state%v_f    filtered v                           */
+  GMC_GZ_F,       /* state%gz_f   filtered gz                          */
+  GMC_GZS,        /* static%gzs   surface geopotential (level ignored) */
+  GMC_DU,         /* tend%du      U grid                               */
+  GMC_DV,         /* tend%dv      V grid                               */
+  GMC_DGZ,        /* tend%dgz     cells                                */
+  GMC_PV,         /* state%pv     vertices, nlat-1 rows                */
+  GMC_KE,         /* state%ke     cells                                */
+  GMC_DIV,        /* state%div    cells                                */
+  GMC_M,          /* state%m      cells                                */
+  GMC_NFIELDS
+};
+
+typedef struct gmc_ctx gmc_ctx;
+
+/* Create a device context: uploads metrics and filter tables, allocates three
+ * state levels and three tendency levels (block_mod.F90:81-97).  Fields start
+ * at zero, as the reference's allocate_array does (allocator_mod.F90:29-54). */
+int gmc_create(const gmc_mesh *mesh, const gmc_config *cfg, gmc_ctx **out);
+int gmc_destroy(gmc_ctx *ctx);
+const char *gmc_last_error(const gmc_ctx *ctx);
+
+/* Copy one interior field between host and device state `level`
+ * (1..3, block%state(level); tendencies use block%tend(level)). */
+int gmc_upload(gmc_ctx *ctx, int32_t level, int32_t field, const double *host, int64_t ld);
+int gmc_download(gmc_ctx *ctx, int32_t level, int32_t field, double *host, int64_t ld);
+
+/* gmcore_run's set-up before the time loop (gmcore_mod.F90:112-122, :133):
+ * big filter u->u_f, v->v_f, gz->gz_f and operators_prepare on `level`. */
+int gmc_prepare(gmc_ctx *ctx, int32_t level);
+
+/* `space_operators` (gmcore_mod.F90:343-575, SW branches) with the
+ * reference's argument meaning: pass 0 = all, 1 = forward, 2 = backward. */
+int gmc_space_operators(gmc_ctx *ctx, int32_t old_level, int32_t star_level,
+                        int32_t new_level, int32_t tend1, int32_t tend2,
+                        double dt, int32_t pass);
+
+/* `update_state` (time_schemes_mod.F90:120-242, SW branch).  `what` is the
+ * tendency's update flag set: 1 = update_gz, 2 = update_u and update_v. */
+int gmc_update_state(gmc_ctx *ctx, int32_t tend, int32_t old_level,
+                     int32_t new_level, double dt, int32_t what);
+
+/* time_integrate (gmcore_mod.F90:577-591) for one step from level old_level
+ * into new_level: `time_integrator` (pc2 / wrfrk3 / euler with step =>
+ * step_forward_backward, time_schemes_mod.F90:102-118, 244-288) followed by
+ * damp_run (damp_mod.F90:49-63).  Level 3 is the intermediate state.  The
+ * caller swaps old and new afterwards, as time_advance does.  Uses the fused
+ * stage kernels. */
+int gmc_time_integrate(gmc_ctx *ctx, int32_t old_level, int32_t new_level, double dt);
+
+/* `time_integrator` (time_schemes_mod.F90:244-288) with step =>
+ * step_forward_backward, followed by damp_run (damp_mod.F90:49-63), then the
+ * old/new swap of time_advance; repeated nsteps times.  Uses the fused
+ * stage kernels; device resident, nothing crosses PCIe. */
+int gmc_step(gmc_ctx *ctx, double dt, int32_t nsteps);
+
+/* Current time level index of `old` (1..3) after gmc_step's swaps. */
+int gmc_current_level(const gmc_ctx *ctx, int32_t *old_level);
+
+/* diagnose (gmcore_mod.F90:213-341, SW): out[0] = tm, out[1] = te,
+ * out[2] = tpe, out[3] = tav, computed from operators_prepare on `level`. */
+int gmc_diagnose(gmc_ctx *ctx, int32_t level, double out[4]);
+
+/* Wait for all work queued on the context's stream. */
+int gmc_synchronize(gmc_ctx *ctx);
+
+/* Device pointer of a field (for profilers and HIP-graph callers).  The
+ * pointer stays valid until gmc_destroy.  Row pitch in doubles in *pitch. */
+int gmc_field_ptr(gmc_ctx *ctx, int32_t level, int32_t field, double **dptr, int64_t *pitch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GMCORE_HIP_H */

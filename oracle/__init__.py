@@ -1,0 +1,139 @@
+"""Oracle: CPU restatement of GMCORE's shallow-water dynamics path.
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py may import this package, and only to check or
+time the HIP library against it.  No product code path uses it.
+
+The C sources (mesh.c, filter.c, sw.c) restate the reference Fortran
+(each function cites its file:line); ``make -C oracle`` builds liboracle.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from gmcore_amd.capi import (GMC_MESH_NFIELDS, FIELD, GmcConfig, GmcMesh, dptr, field_rows,
+                             iptr)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} not built (make -C oracle)")
+        L = C.CDLL(LIB)
+        P, vp = C.POINTER, C.c_void_p
+        L.gmo_mesh_init.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, P(C.c_double)]
+        L.gmo_filter_init.argtypes = [C.c_int, C.c_int, P(C.c_double), C.c_double, C.c_double,
+                                      C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                      C.c_int, P(C.c_int32), P(C.c_int32), P(C.c_int32),
+                                      P(C.c_double), P(C.c_double)]
+        L.gmo_sw_create.restype = vp
+        L.gmo_sw_create.argtypes = [P(GmcMesh), P(GmcConfig)]
+        L.gmo_sw_destroy.argtypes = [vp]
+        L.gmo_sw_halo_width.argtypes = [vp]
+        for fn in ("gmo_sw_set", "gmo_sw_get"):
+            getattr(L, fn).argtypes = [vp, C.c_int, C.c_int, P(C.c_double), C.c_int64]
+        L.gmo_sw_prepare.argtypes = [vp, C.c_int]
+        L.gmo_sw_space_operators.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                             C.c_double, C.c_int]
+        L.gmo_sw_update_state.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int]
+        L.gmo_sw_step.argtypes = [vp, C.c_double, C.c_int]
+        L.gmo_sw_current_level.argtypes = [vp]
+        L.gmo_sw_diagnose.argtypes = [vp, C.c_int, P(C.c_double)]
+        L.gmo_zonal_sum.restype = C.c_double
+        L.gmo_zonal_sum.argtypes = [P(C.c_double), C.c_int, C.c_int]
+        _lib = L
+    return _lib
+
+
+def mesh_metrics(nlon: int, nlat: int, radius: float, omega: float) -> np.ndarray:
+    """Packed metric table [GMC_MESH_NFIELDS, nlat] (mesh_mod.F90:134-447)."""
+    m = np.zeros((GMC_MESH_NFIELDS, nlat))
+    rc = lib().gmo_mesh_init(nlon, nlat, radius, omega, dptr(m))
+    if rc:
+        raise ValueError(f"gmo_mesh_init failed ({rc})")
+    return m
+
+
+def filter_table(nlon, nlat, metrics, dt, nl, kind):
+    """(ngrid_lon[nlat], wgt_lon[nlat, max], ngrid_lat[nlat-1], wgt_lat[nlat-1, max])"""
+    ng_lon = np.zeros(nlat, np.int32)
+    ng_lat = np.zeros(nlat - 1, np.int32)
+    mx = np.zeros(1, np.int32)
+    args = (nlon, nlat, dptr(metrics), dt, nl.max_wave_speed, nl.max_cfl, nl.filter_coef_a,
+            nl.filter_coef_b, nl.filter_coef_c, nl.filter_coef_d, kind)
+    lib().gmo_filter_init(*args, iptr(ng_lon), iptr(ng_lat), iptr(mx), None, None)
+    w_lon = np.zeros((nlat, int(mx[0])))
+    w_lat = np.zeros((nlat - 1, int(mx[0])))
+    rc = lib().gmo_filter_init(*args, iptr(ng_lon), iptr(ng_lat), iptr(mx), dptr(w_lon), dptr(w_lat))
+    if rc:
+        raise ValueError(f"gmo_filter_init failed ({rc})")
+    return ng_lon, w_lon, ng_lat, w_lat
+
+
+def zonal_sum(x: np.ndarray) -> float:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return lib().gmo_zonal_sum(dptr(x), x.size, 1)
+
+
+class SWOracle:
+    """One block of the reference SW model on the CPU, driven like gmcore_run."""
+
+    def __init__(self, mesh_struct: GmcMesh, cfg: GmcConfig):
+        self._keep = (mesh_struct, cfg)
+        self.nlon, self.nlat = mesh_struct.nlon, mesh_struct.nlat
+        self.h = lib().gmo_sw_create(C.byref(mesh_struct), C.byref(cfg))
+        if not self.h:
+            raise ValueError("gmo_sw_create failed")
+
+    def close(self):
+        if self.h:
+            lib().gmo_sw_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    @property
+    def halo_width(self) -> int:
+        return lib().gmo_sw_halo_width(self.h)
+
+    def set(self, level: int, name: str, arr: np.ndarray):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape == (field_rows(name, self.nlat), self.nlon), (name, a.shape)
+        rc = lib().gmo_sw_set(self.h, level, FIELD[name], dptr(a), self.nlon)
+        assert rc == 0
+
+    def get(self, level: int, name: str) -> np.ndarray:
+        a = np.zeros((field_rows(name, self.nlat), self.nlon))
+        rc = lib().gmo_sw_get(self.h, level, FIELD[name], dptr(a), self.nlon)
+        assert rc == 0
+        return a
+
+    def prepare(self, level: int = 1):
+        lib().gmo_sw_prepare(self.h, level)
+
+    def space_operators(self, old, star, new, t1, t2, dt, pass_):
+        return lib().gmo_sw_space_operators(self.h, old, star, new, t1, t2, dt, pass_)
+
+    def update_state(self, tend, old, new, dt, what):
+        return lib().gmo_sw_update_state(self.h, tend, old, new, dt, what)
+
+    def step(self, dt: float, nsteps: int = 1):
+        lib().gmo_sw_step(self.h, dt, nsteps)
+
+    @property
+    def level(self) -> int:
+        return lib().gmo_sw_current_level(self.h)
+
+    def diagnose(self, level=None):
+        out = np.zeros(4)
+        lib().gmo_sw_diagnose(self.h, self.level if level is None else level, dptr(out))
+        return out

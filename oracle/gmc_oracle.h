@@ -1,0 +1,75 @@
+/*
+ * gmc_oracle.h -- CPU restatement of GMCORE's shallow-water dynamics path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker of the HIP library:
+ * tests, the smoke check and the CPU-baseline leg of the benchmark call it.
+ * No product code path may link or call it.
+ *
+ * It restates the reference algorithm with the reference's own data layout:
+ * halo-padded arrays, explicit fill_halo calls (periodic in longitude,
+ * mirrored across the poles), one routine per reference operator.  Each
+ * function cites the reference file:line it follows.  Arithmetic is IEEE
+ * binary64 in the source's left-to-right order; the build uses
+ * -ffp-contract=off so no multiply-add is fused.  The mesh is computed with
+ * __float128 where the reference uses real(16).
+ */
+#ifndef GMC_ORACLE_H
+#define GMC_ORACLE_H
+
+#include <stdint.h>
+#include "../include/gmcore_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* mesh_init_global (src/dynamics/mesh_mod.F90:134-447) into the packed
+ * metric table of gmcore_hip.h (GMC_MESH_NFIELDS * nlat doubles). */
+int gmo_mesh_init(int nlon, int nlat, double radius, double omega, double *metrics);
+
+/* filter_init (src/dynamics/filter_types_mod.F90:49-147).  kind 0 =
+ * big_filter, 1 = small_filter1.  First call with wgt == NULL returns the
+ * tap counts and *max_taps; then call again with wgt buffers sized
+ * nlat * max_taps and (nlat-1) * max_taps. */
+int gmo_filter_init(int nlon, int nlat, const double *metrics, double dt,
+                    double max_wave_speed, double max_cfl, double coef_a,
+                    double coef_b, double coef_c, double coef_d, int kind,
+                    int32_t *ngrid_lon, int32_t *ngrid_lat, int32_t *max_taps,
+                    double *wgt_lon, double *wgt_lat);
+
+/* Shallow-water model state of one block (block_mod.F90:27-45). */
+typedef struct gmo_sw gmo_sw;
+
+gmo_sw *gmo_sw_create(const gmc_mesh *mesh, const gmc_config *cfg);
+void gmo_sw_destroy(gmo_sw *s);
+int gmo_sw_halo_width(const gmo_sw *s);
+
+/* interior copy in/out, same layout as gmc_upload/gmc_download */
+int gmo_sw_set(gmo_sw *s, int level, int field, const double *host, int64_t ld);
+int gmo_sw_get(gmo_sw *s, int level, int field, double *host, int64_t ld);
+
+/* gmcore_run's initial filters + operators_prepare_1 (gmcore_mod.F90:112-133) */
+int gmo_sw_prepare(gmo_sw *s, int level);
+/* space_operators (gmcore_mod.F90:343-575), SW branches */
+int gmo_sw_space_operators(gmo_sw *s, int old_level, int star_level, int new_level,
+                           int tend1, int tend2, double dt, int pass);
+/* update_state (time_schemes_mod.F90:120-242); what: 1 = gz, 2 = u and v */
+int gmo_sw_update_state(gmo_sw *s, int tend, int old_level, int new_level,
+                        double dt, int what);
+/* time_integrator + damp_run + swap, nsteps times */
+int gmo_sw_step(gmo_sw *s, double dt, int nsteps);
+int gmo_sw_current_level(const gmo_sw *s);
+/* diagnose (gmcore_mod.F90:213-341) after operators_prepare on level */
+int gmo_sw_diagnose(gmo_sw *s, int level, double out[4]);
+
+/* Pole sum in the fixed order the library uses: 64 strided partial sums
+ * (partial p adds x[p], x[p+64], ... in ascending order) reduced by a
+ * pairwise tree over p.  The reference's zonal_sum (parallel_mod.F90:741-764)
+ * leaves the order to the compiler and MPI. */
+double gmo_zonal_sum(const double *x, int n, int stride);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
