@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Benchmark: FP64 shallow-water pc2 step at 0.25 degrees on one MI355X.
+
+Workload (the headline configuration): 1440 x 721 C-grid (poles as cell
+rows), FP64, polar filter on, pc2, dt = 60 s, seeded balanced initial state
+(random spherical harmonics l <= 60, 40 m/s peak wind, mean geopotential
+5.0e4 m2/s2, flat topography).  A step is one gmc_step: time_integrator
+(three forward-backward stages) through libgmcore_hip.so, replayed from a
+captured HIP graph, inputs resident in HBM.
+
+Prints one JSON line.  value = grid-point-steps per second (nlon * nlat per
+step).  roofline: algorithmic bytes of the step (35 FP64 field passes per
+grid point, 280 B/gp) over the measured step time against 8 TB/s.  The
+per-kernel breakdown comes from gmc_kernel_times (HIP events around every
+launch on the library's stream).  cpu_baseline: the oracle (a scalar C
+restatement of the reference, one core) timed on a bounded sample of the
+same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_GP_STEP = 280.0          # 35 FP64 field passes per pc2 step
+# compulsory bytes per grid point of one launch of each kernel class, mean
+# over the three stages (k_tend reads 4 fields in stage 1 and 5 in stages
+# 2-3 and writes 3; k_gzf reads 1, writes 2; k_uvf reads 5, writes 4)
+KERNEL_BYTES_PER_GP = {"k_tend": 8.0 * (7 + 8 + 8) / 3, "k_gzf": 24.0, "k_uvf": 72.0}
+
+CONFIGS = {
+    "sw025": dict(nlon=1440, nlat=721, dt=60.0, lmax=60, seed=1),
+    "sw2": dict(nlon=180, nlat=91, dt=600.0, lmax=20, seed=0),
+    "sw1": dict(nlon=360, nlat=181, dt=300.0, lmax=30, seed=4),
+    "sw05": dict(nlon=720, nlat=361, dt=150.0, lmax=40, seed=2),
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_setup(cfg):
+    from gmcore_amd.host import Setup
+    from gmcore_amd.namelist import Namelist
+    nl = Namelist(num_lon=cfg["nlon"], num_lat=cfg["nlat"], dt_dyn=cfg["dt"], time_scheme="pc2")
+    return Setup(nl)
+
+
+def make_ic(st, cfg):
+    from gmcore_amd import init_cond
+    nl = st.nl
+    return init_cond.seeded_state(nl.num_lon, nl.num_lat, st.mesh.metrics, cfg["lmax"], cfg["seed"],
+                                  omega=nl.omega, radius=nl.radius)
+
+
+def cpu_baseline(st, ic, cfg, budget_s=15.0):
+    """Oracle (scalar C, one core) on the same workload: steps until about
+    budget_s seconds of CPU work, after one untimed step."""
+    import oracle
+    o = oracle.SWOracle(st.mesh_struct, st.cfg)
+    for k, a in zip(("u_lon", "v_lat", "gz", "gzs"), ic):
+        o.set(1, k, a)
+    o.prepare(1)
+    o.step(cfg["dt"], 1)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        o.step(cfg["dt"], 1)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n >= 200:
+            break
+    o.close()
+    gp = cfg["nlon"] * cfg["nlat"]
+    return {"value": gp * n / el, "unit": "grid-point-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n} pc2 steps of the same {cfg['nlon']}x{cfg['nlat']} workload after 1 untimed "
+                      f"step, {el:.1f} s, oracle built gcc -O2 -ffp-contract=off, 1 thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="sw025", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of 1 GiB five times (for scaling the HBM "
+                         "byte counters of a profiler against a kernel of known traffic)")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = (1 << 30) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": 8 * n, "copy_calibration_bytes_written": 8 * n,
+                          "launches": 5}))
+        return
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU latitude-band decomposition is a design, not code)", file=sys.stderr)
+        sys.exit(2)
+    cfg = CONFIGS[args.config]
+    from gmcore_amd.host import ShallowWater
+    from gmcore_amd import capi
+    import ctypes as C
+
+    t = time.perf_counter()
+    st = make_setup(cfg)
+    ic = make_ic(st, cfg)
+    log(f"setup + initial state {time.perf_counter() - t:.2f} s")
+    m = ShallowWater(st)
+    m.set_state(*ic)
+    m.prepare(1)
+    d0 = m.diagnose(1)
+    m.step(args.warmup)
+    m.synchronize()
+
+    # timed region: HIP events on the library's stream + host wall clock
+    hip = C.CDLL("libamdhip64.so")
+    stream = C.c_void_p()
+    assert m.lib.gmc_stream(m.ctx, C.byref(stream)) == 0
+    ev0, ev1 = C.c_void_p(), C.c_void_p()
+    assert hip.hipEventCreate(C.byref(ev0)) == 0 and hip.hipEventCreate(C.byref(ev1)) == 0
+    m.synchronize()
+    w0 = time.perf_counter()
+    hip.hipEventRecord(ev0, stream)
+    m.step(args.steps)
+    hip.hipEventRecord(ev1, stream)
+    hip.hipEventSynchronize(ev1)
+    m.synchronize()
+    wall = time.perf_counter() - w0
+    ms = C.c_float()
+    hip.hipEventElapsedTime(C.byref(ms), ev0, ev1)
+    ms_step = ms.value / args.steps
+    d1 = m.diagnose()
+
+    # per-kernel breakdown (launches one by one with events)
+    kt = np.zeros(8)
+    nk = C.c_int32()
+    rc = m.lib.gmc_kernel_times(m.ctx, cfg["dt"], 3, capi.dptr(kt), 8, C.byref(nk))
+    if rc:
+        raise RuntimeError("gmc_kernel_times failed")
+    gp = cfg["nlon"] * cfg["nlat"]
+    kernels = []
+    for k in range(nk.value):
+        name = m.lib.gmc_kernel_name(k).decode()
+        b = KERNEL_BYTES_PER_GP.get(name, 0.0) * gp
+        kernels.append({"name": name, "avg_us": kt[k] * 1e3,
+                        "bytes_per_launch": b, "gbs": b / (kt[k] * 1e-3) / 1e9 if kt[k] > 0 else 0.0})
+    dom = max(kernels, key=lambda x: x["avg_us"])
+    achieved = BYTES_PER_GP_STEP * gp / (ms_step * 1e-3) / 1e9
+    result = {
+        "metric": "grid-point-steps/s (FP64 SW pc2 step, 1 GPU)",
+        "value": gp / (ms_step * 1e-3),
+        "unit": "grid-point-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "wall_ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded spherical-harmonic balanced initial state)",
+        "config": {"workload": f"shallow-water {args.config}: {cfg['nlon']}x{cfg['nlat']} C-grid, FP64, "
+                               f"pc2, dt={cfg['dt']:g} s, polar filter on, seeded l<={cfg['lmax']}",
+                   "nlon": cfg["nlon"], "nlat": cfg["nlat"], "dt": cfg["dt"], "time_scheme": "pc2"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "scope": "whole pc2 step (one graph launch): 280 B/grid point algorithmic"},
+        "dominant_kernel": dom,
+        "kernels": kernels,
+        "mass_rel_drift": float((d1[0] - d0[0]) / d0[0]),
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(st, ic, cfg, args.cpu_budget)
+    m.close()
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

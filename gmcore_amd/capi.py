@@ -104,6 +104,10 @@ def load_hip() -> C.CDLL:
         "gmc_current_level": (C.c_int, [vp, P(C.c_int32)]),
         "gmc_diagnose": (C.c_int, [vp, C.c_int32, P(C.c_double)]),
         "gmc_synchronize": (C.c_int, [vp]),
+        "gmc_kernel_times": (C.c_int, [vp, C.c_double, C.c_int32, P(C.c_double), C.c_int32,
+                                       P(C.c_int32)]),
+        "gmc_kernel_name": (C.c_char_p, [C.c_int32]),
+        "gmc_stream": (C.c_int, [vp, P(vp)]),
         "gmc_field_ptr": (C.c_int, [vp, C.c_int32, C.c_int32, P(P(C.c_double)), P(C.c_int64)]),
     }
     for name, (res, args) in sig.items():

@@ -1,6 +1,7 @@
 // gmc_capi.hip -- device context and the C ABI declared in
 // include/gmcore_hip.h.
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <utility>
 #include <vector>
@@ -154,6 +155,16 @@ int gmc_create(const gmc_mesh *mesh, const gmc_config *cfg, gmc_ctx **out) {
   if (!rc) rc = check_hip(c, hipMalloc((void **)&c->d_mesh, sizeof(double) * GMC_MESH_NFIELDS * g.nlat), "mesh alloc");
   if (!rc) rc = check_hip(c, hipMemcpy(c->d_mesh, mesh->metrics, sizeof(double) * GMC_MESH_NFIELDS * g.nlat,
                                        hipMemcpyHostToDevice), "mesh copy");
+  if (!rc) {
+    // row records of the metric table (zero records before the first and
+    // after the last row): one wavefront reads its rows with a few wide loads
+    std::vector<double> rows((size_t)(g.nlat + 2) * GMC_MESH_NFIELDS, 0.0);
+    for (int r = 0; r < g.nlat; r++)
+      for (int f = 0; f < GMC_MESH_NFIELDS; f++) rows[(size_t)(r + 1) * GMC_MESH_NFIELDS + f] = mesh->metrics[f * g.nlat + r];
+    rc = check_hip(c, hipMalloc((void **)&c->d_mrows, sizeof(double) * rows.size()), "mesh rows alloc");
+    if (!rc) rc = check_hip(c, hipMemcpy(c->d_mrows, rows.data(), sizeof(double) * rows.size(), hipMemcpyHostToDevice),
+                            "mesh rows copy");
+  }
   if (!rc) rc = upload_filter(c, cfg->big_filter, c->big, 0);
   if (!rc) rc = upload_filter(c, cfg->small_filter1, c->small1, 1);
   if (rc) {
@@ -168,8 +179,10 @@ int gmc_create(const gmc_mesh *mesh, const gmc_config *cfg, gmc_ctx **out) {
 int gmc_destroy(gmc_ctx *c) {
   if (!c) return GMC_EINVAL;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  fused_release(c);
   (void)hipFree(c->slab);
   (void)hipFree(c->d_mesh);
+  (void)hipFree(c->d_mrows);
   for (void *m : c->d_filter_mem) (void)hipFree(m);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -233,7 +246,7 @@ int gmc_step(gmc_ctx *c, double dt, int32_t nsteps) {
   if (!c->prepared) return set_error(c, GMC_ESTATE, "gmc_prepare must run before gmc_step");
   if (nsteps < 0) return set_error(c, GMC_EINVAL, "nsteps < 0");
   for (int n = 0; n < nsteps; n++) {
-    int rc = fused_integrate(c, c->old_level, c->new_level, dt);
+    int rc = fused_step_graph(c, c->old_level, c->new_level, dt);
     if (rc) return rc;
     std::swap(c->old_level, c->new_level);
   }
@@ -245,6 +258,51 @@ int gmc_time_integrate(gmc_ctx *c, int32_t old_level, int32_t new_level, double 
   if (old_level < 1 || old_level > 2 || new_level < 1 || new_level > 2 || old_level == new_level)
     return set_error(c, GMC_EINVAL, "old and new must be the distinct levels 1 and 2");
   return fused_integrate(c, old_level, new_level, dt);
+}
+
+int gmc_kernel_times(gmc_ctx *c, double dt, int32_t nsteps, double *out_ms, int32_t cap, int32_t *n) {
+  if (!c || !out_ms || !n || cap < 1) return GMC_EINVAL;
+  if (!c->prepared) return set_error(c, GMC_ESTATE, "gmc_prepare must run before gmc_kernel_times");
+  if (nsteps < 1) return set_error(c, GMC_EINVAL, "nsteps < 1");
+  int rc = fused_prepare(c);
+  if (rc) return rc;
+  const int stages = c->cfg.time_scheme == GMC_TIME_EULER ? 1 : 3;
+  const int per_stage = kKernelClasses + 1;   // events per stage
+  std::vector<hipEvent_t> ev((size_t)nsteps * stages * per_stage);
+  for (auto &e : ev) {
+    rc = check_hip(c, hipEventCreate(&e), "event create");
+    if (rc) return rc;
+  }
+  c->prof_ev = ev.data();
+  c->prof_cap = (int)ev.size();
+  c->prof_n = 0;
+  for (int s = 0; s < nsteps && !rc; s++) {
+    rc = fused_integrate(c, c->old_level, c->new_level, dt);
+    std::swap(c->old_level, c->new_level);
+  }
+  c->prof_ev = nullptr;
+  if (!rc) rc = check_hip(c, hipStreamSynchronize(c->stream), "timing sync");
+  double acc[kKernelClasses] = {};
+  int cnt = 0;
+  for (int b = 0; !rc && b + per_stage <= c->prof_n; b += per_stage, cnt++)
+    for (int k = 0; k < kKernelClasses; k++) {
+      float ms = 0.f;
+      rc = check_hip(c, hipEventElapsedTime(&ms, ev[b + k], ev[b + k + 1]), "event elapsed");
+      acc[k] += ms;
+    }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  if (rc) return rc;
+  *n = std::min(cap, kKernelClasses);
+  for (int k = 0; k < *n; k++) out_ms[k] = cnt ? acc[k] / cnt : 0.0;
+  return GMC_OK;
+}
+
+const char *gmc_kernel_name(int32_t k) { return k >= 0 && k < kKernelClasses ? kKernelNames[k] : ""; }
+
+int gmc_stream(gmc_ctx *c, void **stream) {
+  if (!c || !stream) return GMC_EINVAL;
+  *stream = (void *)c->stream;
+  return GMC_OK;
 }
 
 int gmc_current_level(const gmc_ctx *c, int32_t *old_level) {

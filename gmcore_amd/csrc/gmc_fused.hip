@@ -1,9 +1,530 @@
-// gmc_fused.hip -- fused stage kernels used by gmc_step.
+// gmc_fused.hip -- fused stage kernels used by gmc_step / gmc_time_integrate.
+//
+// One step_forward_backward stage (time_schemes_mod.F90:102-118) of the SW
+// path is three launches:
+//   k_tend   an LDS-tiled stencil over 58 x 11 cells: the pole sums (in the
+//            tiles next to the poles), m, mass fluxes, pv,
+//            ke, tangential winds, upwind pv on edges, then the forward
+//            tendencies du, dv and the updated, unfiltered gz
+//            (operators_prepare_2 forward + calc_grad_mf + calc_coriolis +
+//            calc_grad_ke + the gz part of update_state);
+//   k_gzf    the big and small zonal filters of the new gz;
+//   k_uvf    the backward pass (pgf from gz_f), the u/v update with the
+//            pole-row smoothing, and the big filters to u_f, v_f.
+// The filter kernels run over a host-built work list of (row, column chunk)
+// items so that polar rows with hundreds of taps are spread over many
+// workgroups.  Every expression is the same point formula as the
+// operator-level kernels (gmc_point.hpp), so both paths give the same bits.
+#include <algorithm>
+#include <vector>
+
 #include "gmc_internal.hpp"
 #include "gmc_point.hpp"
 
 namespace gmc {
 
-int fused_integrate(gmc_ctx *c, int o, int n, double dt) { return ops_integrate(c, o, n, dt); }
+namespace {
+
+// k_tend tile: one wavefront per tile row, lane q <-> column C0 - 3 + q
+constexpr int LC = kWave;               // 64 tile columns
+constexpr int HXL = 3;                  // column halo
+constexpr int TX = LC - 2 * HXL - 0;    // 58 output columns (q = 3 .. 60)
+constexpr int TY = 11;                  // output rows
+constexpr int LR = TY + 5;              // 16 tile rows: r = R0-2 .. R0+TY+2
+constexpr int NT = LR * kWave;          // 1024 threads, one wavefront per tile row
+constexpr int NF = 256;                 // threads per workgroup of the filter kernels
+constexpr int NE = LR * LC;
+constexpr int NXCD = 8;                 // XCDs of an MI355X
+constexpr int SH_TILES = 7 * NE;        // LDS field tiles of k_tend
+constexpr int SH_MESH = LR * GMC_MESH_NFIELDS;  // metric rows of the tile
+constexpr int kMaxPoleLon = SH_TILES / 3;  // the pole sums keep 3 row terms in the tile area
+
+struct StagePtrs {
+  const double *gz_star, *gzs, *u_f, *v_f, *gz_old;
+  double *gz_new, *du, *dv;
+};
+
+// pv at vertex (r, i) from global memory: rows reached through the
+// cross-pole mirror next to the poles
+__device__ double pv_global(const Grid &g, const Mesh &M, double grav, int stokes, const double *pole,
+                            const StagePtrs &P, int r, int i) {
+  const int64_t p = g.pitch;
+  const int ip = wrap(i + 1, g.nlon);
+  auto m = [&](int rr, int ii) { return m_of(P.gz_star[rr * p + ii], P.gzs[rr * p + ii], grav); };
+  double vor;
+  if (stokes && r == 0) vor = pole[4];
+  else if (stokes && r == g.nhalf - 1) vor = pole[5];
+  else vor = vor_of(P.u_f[r * p + i], P.u_f[(r + 1) * p + i], P.v_f[r * p + ip], P.v_f[r * p + i], M, r);
+  return pv_of(vor, M, r, m_vtx_of(m(r, i), m(r, ip), m(r + 1, i), m(r + 1, ip), M, r));
+}
+
+// Pole reductions of the star state for one pole (zonal_sum call sites in
+// calc_ke :320-349, calc_vor :640-674, calc_grad_mf :974-1003): the row
+// terms go to LDS scratch, one wavefront adds them in the fixed order of
+// wave_zonal_sum.  pole[north] = ke, pole[2+north] = dmfdlat at the pole
+// cell, pole[4+north] = vor of the vertex row next to the pole.
+__device__ void pole_sums(const Grid &g, const Mesh &M, double grav, const StagePtrs &P, int north,
+                          double *scratch, double *pole) {
+  const int tid = threadIdx.x, n = g.nlon;
+  const int64_t p = g.pitch;
+  const int vr = north ? g.nhalf - 1 : 0, cr = north ? g.nlat - 1 : 0, ur = north ? g.nlat - 2 : 1;
+  const double le = M(GMC_LE_LAT, vr), ac = M(GMC_AREA_CELL, cr);
+  const double w1 = M(GMC_HALF_TANGENT_WGT1, vr), w2 = M(GMC_HALF_TANGENT_WGT2, vr);
+  double *t0 = scratch, *t1 = scratch + n, *t2 = scratch + 2 * n;
+  auto m = [&](int r, int i) { return m_of(P.gz_star[r * p + i], P.gzs[r * p + i], grav); };
+  for (int i = tid; i < n; i += NT) {
+    const int im = wrap(i - 1, n), ip = wrap(i + 1, n);
+    const double v = P.v_f[vr * p + i];
+    const double m_lat = (m(vr, i) + m(vr + 1, i)) * 0.5;
+    const double mu = m(ur, i);
+    const double mfx_w = (m(ur, im) + mu) * 0.5 * P.u_f[ur * p + im];
+    const double mfx_e = (mu + m(ur, ip)) * 0.5 * P.u_f[ur * p + i];
+    // mfx_lat of the V row: the pole U row contributes zero flux
+    const double t = north ? tangent_sum(w1, w2, mfx_w, mfx_e, 0.0, 0.0) : tangent_sum(w1, w2, 0.0, 0.0, mfx_w, mfx_e);
+    const double ul = t / m_lat;
+    const double mfy = m_lat * v;
+    t0[i] = v * v;
+    t1[i] = north ? -mfy : mfy;
+    t2[i] = north ? ul * le : -ul * le;
+  }
+  __syncthreads();
+  if (tid < kWave) {
+    const double ske = wave_zonal_sum(n, tid, [&](int i) { return t0[i]; });
+    const double smf = wave_zonal_sum(n, tid, [&](int i) { return t1[i]; });
+    const double svo = wave_zonal_sum(n, tid, [&](int i) { return t2[i]; });
+    if (tid == 0) {
+      const double nl = (double)n;
+      pole[north] = ske / nl;
+      pole[2 + north] = smf * le / nl / ac;
+      pole[4 + north] = svo / nl / ac;
+    }
+  }
+  __syncthreads();
+}
+
+// Metric rows of one tile in LDS (row records, tile row k = r - r0):
+// M(f, r) is a broadcast LDS read for a wave-uniform r.
+struct MeshTile {
+  const double *s;
+  int r0;
+  __device__ __forceinline__ double operator()(int f, int r) const { return s[(r - r0) * GMC_MESH_NFIELDS + f]; }
+};
+
+// Forward tendencies and the unfiltered gz update on a 58 x 11 tile.
+// Wavefront w owns tile row k = w (global row R0 - 2 + k) in every phase and
+// lane q the column C0 - 3 + q, so metric terms are wave-uniform (broadcast
+// LDS reads of the tile's metric rows) and each phase is one pass over a row.
+// Blocks are dealt to the XCDs round-robin; each XCD gets a contiguous run of
+// tiles (row-major), so vertically adjacent tiles share that XCD's L2.  The
+// XCD holding the north pole walks its run backwards so that the polar tiles,
+// which also form the pole sums, start first.
+__global__ void __launch_bounds__(NT, 8) k_tend(Grid g, Mesh Mg, const double *__restrict__ mrows, double grav,
+                                                double dt, int stokes, int pv_scheme, int order, double wgt,
+                                                int coriolis, StagePtrs P) {
+  __shared__ double sh[SH_TILES + SH_MESH + 8];
+  double(*sm)[LC] = (double(*)[LC])(sh + 0 * NE);
+  double(*suf)[LC] = (double(*)[LC])(sh + 1 * NE);
+  double(*svf)[LC] = (double(*)[LC])(sh + 2 * NE);
+  double(*smfx)[LC] = (double(*)[LC])(sh + 3 * NE);
+  double(*smfy)[LC] = (double(*)[LC])(sh + 4 * NE);
+  double(*spv)[LC] = (double(*)[LC])(sh + 5 * NE);
+  double(*ske)[LC] = (double(*)[LC])(sh + 6 * NE);
+  double(*spl)[LC] = suf;  // pv_lon replaces u_f element by element (phase D)
+  double(*spt)[LC] = svf;  // pv_lat replaces v_f element by element (phase D)
+  double *sme = sh + SH_TILES;
+  double *pole = sh + SH_TILES + SH_MESH;
+
+  // tile of this block
+  const int nx = (g.nlon + TX - 1) / TX, ny = (g.nlat + TY - 1) / TY, T = nx * ny;
+  const int b = blockIdx.x, x = b % NXCD, s = b / NXCD;
+  int start = 0;
+  for (int y = 0; y < x; y++) start += (T - y + NXCD - 1) / NXCD;
+  const int cnt = (T - x + NXCD - 1) / NXCD;
+  const int t = start + (x == NXCD - 1 ? cnt - 1 - s : s);
+  const int C0 = (t % nx) * TX, R0 = (t / nx) * TY;
+
+  const int64_t p = g.pitch;
+  const int nlat = g.nlat, nhalf = g.nhalf;
+  const int k = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);   // tile row of this wavefront
+  const int q = threadIdx.x % kWave;                                   // tile column of this lane
+  const int r = R0 - 2 + k;                                            // global row (wave-uniform)
+  const int i = wrap(C0 - HXL + q, g.nlon);                            // global column
+
+  // 0: pole sums in the tiles that read them (uniform branch per block)
+  if (R0 <= 2) pole_sums(g, Mg, grav, P, 0, sh, pole);
+  if (R0 + TY + 3 >= nlat - 1) pole_sums(g, Mg, grav, P, 1, sh, pole);
+
+  // A: m, u_f, v_f of tile row k and the tile's metric rows
+  {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
+    if (r >= 0 && r < nlat) {
+      a0 = P.gz_star[r * p + i];
+      a1 = P.gzs[r * p + i];
+      a2 = P.u_f[r * p + i];
+    }
+    if (r >= 0 && r < nhalf) a3 = P.v_f[r * p + i];
+    if (threadIdx.x < SH_MESH) {
+      const int rr = R0 - 2 + (int)threadIdx.x / GMC_MESH_NFIELDS;
+      if (rr >= 0 && rr < nlat) a4 = mrows[(int64_t)(rr + 1) * GMC_MESH_NFIELDS + threadIdx.x % GMC_MESH_NFIELDS];
+    }
+    sm[k][q] = (r >= 0 && r < nlat) ? m_of(a0, a1, grav) : 0.0;
+    suf[k][q] = a2;
+    svf[k][q] = a3;
+    if (threadIdx.x < SH_MESH) sme[threadIdx.x] = a4;
+  }
+  const MeshTile M{sme, R0 - 2};
+  __syncthreads();
+
+  // B: mass fluxes (rows 1 .. TY+2), pv (rows 0 .. TY+3), ke (rows 2 .. TY+2)
+  if (k >= 1 && k < TY + 3 && q >= 2 && q < 63) {
+    smfx[k][q] = (r >= 1 && r <= nlat - 2) ? (sm[k][q] + sm[k][q + 1]) * 0.5 * suf[k][q] : 0.0;
+    smfy[k][q] = (r >= 0 && r < nhalf) ? (sm[k][q] + sm[k + 1][q]) * 0.5 * svf[k][q] : 0.0;
+  }
+  if (k < TY + 4 && q >= 1 && q < 63) {
+    double v = 0.0;
+    if (r >= 0 && r < nhalf) {
+      double vor;
+      if (stokes && r == 0) vor = pole[4];
+      else if (stokes && r == nhalf - 1) vor = pole[5];
+      else vor = vor_of(suf[k][q], suf[k + 1][q], svf[k][q + 1], svf[k][q], M, r);
+      v = pv_of(vor, M, r, m_vtx_of(sm[k][q], sm[k][q + 1], sm[k + 1][q], sm[k + 1][q + 1], M, r));
+    } else if (r >= -2 && r <= nhalf + 1) {
+      int rr = r, ii = i;
+      mirror_half(rr, ii, g);
+      v = pv_global(g, Mg, grav, stokes, pole, P, rr, ii);
+    }
+    spv[k][q] = v;
+  }
+  if (k >= 2 && k < TY + 3 && q >= 3 && q < 62) {
+    double v = 0.0;
+    if (r == 0) v = pole[0];
+    else if (r == nlat - 1) v = pole[1];
+    else if (r > 0 && r < nlat - 1) v = ke_of(suf[k][q - 1], suf[k][q], svf[k - 1][q], svf[k][q], M, r);
+    ske[k][q] = v;
+  }
+  __syncthreads();
+
+  // D: tangential winds and pv on the edges.  v_lon (U rows 2 .. TY+2) and
+  // u_lat (V rows 1 .. TY+1) are each read only by the lane that forms them,
+  // so they stay in registers; pv_lon replaces u_f and pv_lat replaces v_f
+  // element by element (each lane reads only its own u_f / v_f here).
+  double pl = 0.0, pt = 0.0;
+  if (k >= 2 && k < TY + 3 && q >= 2 && q < 62 && r >= 1 && r <= nlat - 2) {
+    const double tt = tangent_sum(M(GMC_FULL_TANGENT_WGT1, r), M(GMC_FULL_TANGENT_WGT2, r), smfy[k - 1][q],
+                                  smfy[k - 1][q + 1], smfy[k][q], smfy[k][q + 1]);
+    const double vl = tt / ((sm[k][q] + sm[k][q + 1]) * 0.5);
+    const double a1 = spv[k - 2][q], a2 = spv[k - 1][q], a3 = spv[k][q], a4 = spv[k + 1][q];
+    pl = pv_scheme == GMC_PV_MIDPOINT ? 0.5 * (a3 + a2) : pv_upwind_edge(order, wgt, vl, suf[k][q], 0, a1, a2, a3, a4);
+  }
+  if (k >= 1 && k < TY + 2 && q >= 3 && q < 62 && r >= 0 && r < nhalf) {
+    const double tt = tangent_sum(M(GMC_HALF_TANGENT_WGT1, r), M(GMC_HALF_TANGENT_WGT2, r), smfx[k][q - 1],
+                                  smfx[k][q], smfx[k + 1][q - 1], smfx[k + 1][q]);
+    const double ul = tt / ((sm[k][q] + sm[k + 1][q]) * 0.5);
+    const double a1 = spv[k][q - 2], a2 = spv[k][q - 1], a3 = spv[k][q], a4 = spv[k][q + 1];
+    pt = pv_scheme == GMC_PV_MIDPOINT ? 0.5 * (a2 + a3) : pv_upwind_edge(order, wgt, ul, svf[k][q], 1, a1, a2, a3, a4);
+  }
+  spl[k][q] = pl;
+  spt[k][q] = pt;
+  __syncthreads();
+
+  // E: du, dv (forward pass) and gz_new = gz_old + dt dgz on the output rows
+  const int ic = C0 + q - HXL;
+  if (k >= 2 && k < TY + 2 && q >= HXL && q < HXL + TX && ic < g.nlon && r < nlat) {
+    const int64_t o = r * p + ic;
+    const double gz_old = P.gz_old[o];
+    if (r >= 1 && r <= nlat - 2) {
+      const double qhv = coriolis == 1
+          ? qhv_of(M, r, spl[k][q], smfy[k - 1][q], spt[k - 1][q], smfy[k - 1][q + 1], spt[k - 1][q + 1],
+                   smfy[k][q], spt[k][q], smfy[k][q + 1], spt[k][q + 1])
+          : tangent_sum(M(GMC_FULL_TANGENT_WGT1, r), M(GMC_FULL_TANGENT_WGT2, r), smfy[k - 1][q],
+                        smfy[k - 1][q + 1], smfy[k][q], smfy[k][q + 1]) * spl[k][q];
+      const double dk = (ske[k][q + 1] - ske[k][q]) / M(GMC_DE_LON, r);
+      P.du[o] = qhv - dk;
+    }
+    if (r < nhalf) {
+      const double qhu = coriolis == 1
+          ? qhu_of(M, r, spt[k][q], smfx[k][q - 1], spl[k][q - 1], smfx[k][q], spl[k][q],
+                   smfx[k + 1][q - 1], spl[k + 1][q - 1], smfx[k + 1][q], spl[k + 1][q])
+          : tangent_sum(M(GMC_HALF_TANGENT_WGT1, r), M(GMC_HALF_TANGENT_WGT2, r), smfx[k][q - 1], smfx[k][q],
+                        smfx[k + 1][q - 1], smfx[k + 1][q]) * spt[k][q];
+      const double dk = (ske[k + 1][q] - ske[k][q]) / M(GMC_DE_LAT, r);
+      P.dv[o] = -qhu - dk;
+    }
+    double dmfdlon = 0.0, dmfdlat;
+    if (r == 0) dmfdlat = pole[2];
+    else if (r == nlat - 1) dmfdlat = pole[3];
+    else {
+      dmfdlon = dmfdlon_of(smfx[k][q], smfx[k][q - 1], M, r);
+      dmfdlat = dmfdlat_of(smfy[k][q], smfy[k - 1][q], M, r);
+    }
+    const double dgz = -(dmfdlon + dmfdlat) * grav;
+    P.gz_new[o] = gz_old + dt * dgz;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// zonal filters over (row, column chunk) work items
+struct Item {
+  int32_t row, c0, c1, kind;  // kind: 0 gz (cell row), 1 u (U row), 2 v (V row)
+};
+
+// column i - h + t of a row, t in [0, nlon + 2h), h <= nlon
+__device__ __forceinline__ int wrap2(int c, int n) { return c < 0 ? c + n : (c >= n ? c - n : c); }
+
+// big filter gz_new -> gz_f and small filter gz_new -> gz
+// (time_schemes_mod.F90:184-191, filter_mod.F90:25-50).  Weights and the
+// row window are staged in LDS: xs = [weights big | weights small | window].
+__global__ void __launch_bounds__(NF) k_gzf(Grid g, FilterDev big, FilterDev small, const Item *__restrict__ items,
+                                            const double *__restrict__ gz_new, double *gz_f, double *gz) {
+  extern __shared__ double xs[];
+  const Item it = items[blockIdx.x];
+  const int r = it.row;
+  const int64_t o = r * g.pitch;
+  const int nb = big.ngrid_lon[r], ns = small.ngrid_lon[r];
+  const int hb = nb > 1 ? (nb - 1) / 2 : 0, hs = ns > 1 ? (ns - 1) / 2 : 0;
+  double *wb = xs, *ws = xs + big.max_taps, *x0 = xs + big.max_taps + small.max_taps;
+  const int len = it.c1 - it.c0 + 2 * hb;
+  for (int k = threadIdx.x; k < nb; k += NF) wb[k] = big.wgt_lon[(int64_t)r * big.max_taps + k];
+  for (int k = threadIdx.x; k < ns; k += NF) ws[k] = small.wgt_lon[(int64_t)r * small.max_taps + k];
+  for (int t = threadIdx.x; t < len; t += 4 * NF) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (t + u * NF < len) v[u] = gz_new[o + wrap2(it.c0 - hb + t + u * NF, g.nlon)];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (t + u * NF < len) x0[t + u * NF] = v[u];
+  }
+  __syncthreads();
+  for (int c = it.c0 + threadIdx.x; c < it.c1; c += NF) {
+    const double *x = x0 + (c - it.c0);      // x[hb] is column c
+    gz_f[o + c] = nb > 1 ? filter_point(wb, nb, x) : x[hb];
+    gz[o + c] = ns > 1 ? filter_point(ws, ns, x + hb - hs) : x[hb];
+  }
+}
+
+struct UVPtrs {
+  const double *u_old, *v_old, *du, *dv, *gz_f;
+  double *u, *v, *u_f, *v_f;
+};
+
+// backward pass + u/v update + big filters
+// (gmcore_mod.F90:552-571, time_schemes_mod.F90:198-239).  xs = [weights |
+// window of the updated row].
+__global__ void __launch_bounds__(NF) k_uvf(Grid g, Mesh M, double dt, FilterDev big, const Item *__restrict__ items,
+                                            UVPtrs P) {
+  extern __shared__ double xs[];
+  const Item it = items[blockIdx.x];
+  const int r = it.row;
+  const int64_t p = g.pitch, o = r * p;
+  const int nl = g.nlon;
+  double *w = xs, *x0 = xs + big.max_taps;
+  if (it.kind == 1 && (r == 0 || r == g.nlat - 1)) {   // pole U rows: not updated, copied to u_f
+    for (int c = it.c0 + threadIdx.x; c < it.c1; c += NF) P.u_f[o + c] = P.u[o + c];
+    return;
+  }
+  const int n = it.kind == 1 ? big.ngrid_lon[r] : big.ngrid_lat[r];
+  const int h = n > 1 ? (n - 1) / 2 : 0;
+  const int len = it.c1 - it.c0 + 2 * h;
+  const double *wg = (it.kind == 1 ? big.wgt_lon : big.wgt_lat) + (int64_t)r * big.max_taps;
+  for (int k = threadIdx.x; k < n; k += NF) w[k] = wg[k];
+  if (it.kind == 1) {
+    const double de = M(GMC_DE_LON, r);
+    for (int t = threadIdx.x; t < len; t += 4 * NF) {
+      double uo[4], du[4], ge[4], gw[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (t + u * NF < len) {
+          const int i = wrap2(it.c0 - h + t + u * NF, nl);
+          uo[u] = P.u_old[o + i];
+          du[u] = P.du[o + i];
+          ge[u] = P.gz_f[o + wrap(i + 1, nl)];
+          gw[u] = P.gz_f[o + i];
+        }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (t + u * NF < len) x0[t + u * NF] = uo[u] + dt * (du[u] - (ge[u] - gw[u]) / de);
+    }
+  } else {
+    const int pole = r == 0 ? 1 : (r == g.nhalf - 1 ? -1 : 0);   // neighbour row of the smoothing
+    const double de0 = M(GMC_DE_LAT, r), de1 = pole ? M(GMC_DE_LAT, r + pole) : 1.0;
+    for (int t = threadIdx.x; t < len; t += 2 * NF) {
+      double a[2][8];
+#pragma unroll
+      for (int u = 0; u < 2; u++)
+        if (t + u * NF < len) {
+          const int64_t q = o + wrap2(it.c0 - h + t + u * NF, nl);
+          a[u][0] = P.v_old[q]; a[u][1] = P.dv[q]; a[u][2] = P.gz_f[q + p]; a[u][3] = P.gz_f[q];
+          if (pole) {
+            const int64_t q1 = q + pole * p;
+            a[u][4] = P.v_old[q1]; a[u][5] = P.dv[q1]; a[u][6] = P.gz_f[q1 + p]; a[u][7] = P.gz_f[q1];
+          }
+        }
+#pragma unroll
+      for (int u = 0; u < 2; u++)
+        if (t + u * NF < len) {
+          double v = a[u][0] + dt * (a[u][1] - (a[u][2] - a[u][3]) / de0);
+          if (pole) v = pole_smooth(v, a[u][4] + dt * (a[u][5] - (a[u][6] - a[u][7]) / de1));
+          x0[t + u * NF] = v;
+        }
+    }
+  }
+  __syncthreads();
+  double *out = it.kind == 1 ? P.u : P.v, *out_f = it.kind == 1 ? P.u_f : P.v_f;
+  for (int c = it.c0 + threadIdx.x; c < it.c1; c += NF) {
+    const double *x = x0 + (c - it.c0);
+    out[o + c] = x[h];
+    out_f[o + c] = n > 1 ? filter_point(w, n, x) : x[h];
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Work lists: a row whose filter has n taps is cut into chunks of about
+// kChunkWork / n columns so that each workgroup does similar work.
+static constexpr int kChunkWork = 16384;
+
+static void add_row(std::vector<Item> &v, int row, int kind, int n, int nlon, size_t &lds) {
+  int w = n <= 16 ? nlon : std::min(nlon, std::max(NF, (kChunkWork / n + NF - 1) / NF * NF));
+  const int parts = (nlon + w - 1) / w;
+  const int h = n > 1 ? (n - 1) / 2 : 0;
+  for (int k = 0; k < parts; k++) {
+    const int c0 = (int)((int64_t)nlon * k / parts), c1 = (int)((int64_t)nlon * (k + 1) / parts);
+    v.push_back(Item{row, c0, c1, kind});
+    lds = std::max(lds, sizeof(double) * (size_t)(c1 - c0 + 2 * h));
+  }
+}
+
+struct FusedPlan {
+  Item *d_items = nullptr;
+  int n_gz = 0, n_uv = 0;
+  size_t lds_gz = 0, lds_uv = 0;
+};
+
+static FusedPlan *plan_of(gmc_ctx *c) { return (FusedPlan *)c->fused_plan; }
+
+static int build_plan(gmc_ctx *c) {
+  const Grid &g = c->grid;
+  std::vector<int32_t> nb_lon(g.nlat), nb_lat(g.nhalf);
+  int rc = check_hip(c, hipMemcpy(nb_lon.data(), c->big.ngrid_lon, sizeof(int32_t) * g.nlat, hipMemcpyDeviceToHost),
+                     "plan");
+  if (!rc) rc = check_hip(c, hipMemcpy(nb_lat.data(), c->big.ngrid_lat, sizeof(int32_t) * g.nhalf,
+                                       hipMemcpyDeviceToHost), "plan");
+  if (rc) return rc;
+  for (int r = 0; r < g.nlat; r++)
+    if (nb_lon[r] > g.nlon) return set_error(c, GMC_EINVAL, "filter wider than a latitude circle");
+  for (int r = 0; r < g.nhalf; r++)
+    if (nb_lat[r] > g.nlon) return set_error(c, GMC_EINVAL, "filter wider than a latitude circle");
+  if (g.nlon > kMaxPoleLon) return set_error(c, GMC_EINVAL, "nlon too large for the fused stage kernels");
+  std::vector<Item> items;
+  FusedPlan *P = new FusedPlan();
+
+  for (int r = 0; r < g.nlat; r++) add_row(items, r, 0, nb_lon[r] + std::max(1, nb_lon[r] / 2), g.nlon, P->lds_gz);
+  P->n_gz = (int)items.size();
+  for (int r = 0; r < g.nlat; r++) add_row(items, r, 1, nb_lon[r], g.nlon, P->lds_uv);
+  for (int r = 0; r < g.nhalf; r++) add_row(items, r, 2, nb_lat[r], g.nlon, P->lds_uv);
+  P->n_uv = (int)items.size() - P->n_gz;
+  // the gz items filter with the big half-width; size their LDS for it
+  P->lds_gz = 0;
+  for (int k = 0; k < P->n_gz; k++) {
+    const int n = nb_lon[items[k].row], h = n > 1 ? (n - 1) / 2 : 0;
+    P->lds_gz = std::max(P->lds_gz, sizeof(double) * (size_t)(items[k].c1 - items[k].c0 + 2 * h));
+  }
+  P->lds_gz += sizeof(double) * (c->big.max_taps + c->small1.max_taps);
+  P->lds_uv += sizeof(double) * c->big.max_taps;
+  rc = check_hip(c, hipMalloc((void **)&P->d_items, sizeof(Item) * items.size()), "plan alloc");
+  if (!rc) rc = check_hip(c, hipMemcpy(P->d_items, items.data(), sizeof(Item) * items.size(), hipMemcpyHostToDevice),
+                          "plan copy");
+  if (rc) { delete P; return rc; }
+  c->fused_plan = P;
+  return GMC_OK;
+}
+
+void fused_release(gmc_ctx *c) {
+  for (auto &sg : c->graphs) (void)hipGraphExecDestroy(sg.exec);
+  c->graphs.clear();
+  if (FusedPlan *P = plan_of(c)) {
+    (void)hipFree(P->d_items);
+    delete P;
+    c->fused_plan = nullptr;
+  }
+}
+
+const char *const kKernelNames[kKernelClasses] = {"k_tend", "k_gzf", "k_uvf"};
+
+static void prof_mark(gmc_ctx *c) {
+  if (c->prof_ev && c->prof_n < c->prof_cap) (void)hipEventRecord(c->prof_ev[c->prof_n++], c->stream);
+}
+
+// one step_forward_backward stage: old -> new with the star state `star`
+static void fused_stage(gmc_ctx *c, int o, int s, int n, double dt) {
+  const Grid &g = c->grid;
+  const Mesh M{c->d_mesh, g.nlat};
+  FusedPlan *PL = plan_of(c);
+  double **O = c->state[o], **S = c->state[s], **N = c->state[n];
+  double *du = c->tend[o][T_DU], *dv = c->tend[o][T_DV];
+  double *gz_new = c->tend[o][T_DGZ];   // the unfiltered gz of the new state
+  prof_mark(c);
+  StagePtrs sp{S[S_GZ], c->gzs, S[S_U_F], S[S_V_F], O[S_GZ], gz_new, du, dv};
+  const int tiles = ((g.nlon + TX - 1) / TX) * ((g.nlat + TY - 1) / TY);
+  hipLaunchKernelGGL(k_tend, dim3(tiles), dim3(NT), 0, c->stream, g, M, c->d_mrows, c->cfg.g, dt, c->cfg.pv_pole_stokes,
+                     c->cfg.pv_scheme, c->cfg.upwind_order_pv, c->cfg.upwind_wgt_pv, c->cfg.coriolis_scheme, sp);
+  prof_mark(c);
+  hipLaunchKernelGGL(k_gzf, dim3(PL->n_gz), dim3(NF), PL->lds_gz, c->stream, g, c->big, c->small1, PL->d_items,
+                     gz_new, N[S_GZ_F], N[S_GZ]);
+  prof_mark(c);
+  UVPtrs up{O[S_U_LON], O[S_V_LAT], du, dv, N[S_GZ_F], N[S_U_LON], N[S_V_LAT], N[S_U_F], N[S_V_F]};
+  hipLaunchKernelGGL(k_uvf, dim3(PL->n_uv), dim3(NF), PL->lds_uv, c->stream, g, M, dt, c->big,
+                     PL->d_items + PL->n_gz, up);
+  prof_mark(c);
+}
+
+int fused_prepare(gmc_ctx *c) { return c->fused_plan ? GMC_OK : build_plan(c); }
+
+int fused_integrate(gmc_ctx *c, int o, int n, double dt) {
+  int rc = fused_prepare(c);
+  if (rc) return rc;
+  switch (c->cfg.time_scheme) {
+  case GMC_TIME_EULER:
+    fused_stage(c, o, o, n, dt);
+    break;
+  case GMC_TIME_WRFRK3:
+    fused_stage(c, o, o, n, dt / 3.0);
+    fused_stage(c, o, n, 3, dt / 2.0);
+    fused_stage(c, o, 3, n, dt);
+    break;
+  default:
+    fused_stage(c, o, o, n, dt / 2.0);
+    fused_stage(c, o, n, 3, dt / 2.0);
+    fused_stage(c, o, 3, n, dt);
+  }
+  rc = check_hip(c, hipGetLastError(), "fused stage");
+  if (!rc && c->cfg.use_div_damp) rc = ops_div_damp(c, n);
+  return rc;
+}
+
+// gmc_step: replay a captured graph of one fused step
+int fused_step_graph(gmc_ctx *c, int o, int n, double dt) {
+  int rc = fused_prepare(c);
+  if (rc) return rc;
+  hipGraphExec_t exec = nullptr;
+  for (auto &sg : c->graphs)
+    if (sg.o == o && sg.n == n && sg.dt == dt) exec = sg.exec;
+  if (!exec) {
+    hipGraph_t graph = nullptr;
+    rc = check_hip(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (rc) return rc;
+    int rc2 = fused_integrate(c, o, n, dt);
+    rc = check_hip(c, hipStreamEndCapture(c->stream, &graph), "end capture");
+    if (rc2) return rc2;
+    if (rc) return rc;
+    rc = check_hip(c, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
+    (void)hipGraphDestroy(graph);
+    if (rc) return rc;
+    c->graphs.push_back({o, n, dt, exec});
+  }
+  return check_hip(c, hipGraphLaunch(exec, c->stream), "graph launch");
+}
 
 }  // namespace gmc

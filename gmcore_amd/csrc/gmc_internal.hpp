@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/gmcore_hip.h"
 
@@ -90,6 +91,7 @@ struct gmc_ctx {
   gmc_config cfg;            // host copy; filter pointers point at device tables
   double dlon;
   double *d_mesh = nullptr;  // GMC_MESH_NFIELDS * nlat
+  double *d_mrows = nullptr; // row records: (nlat + 2) x GMC_MESH_NFIELDS, record r + 1 = row r
   gmc::FilterDev big{}, small1{};
   void *d_filter_mem[8] = {};
   double *state[4][gmc::S_NFIELDS] = {};
@@ -102,6 +104,13 @@ struct gmc_ctx {
   hipStream_t stream = nullptr;
   int old_level = 1, new_level = 2;
   int prepared = 0;
+  void *fused_plan = nullptr;  // work lists of the fused stage (gmc_fused.hip)
+  // graphs of one fused step, keyed by (old, new, dt)
+  struct StepGraph { int o, n; double dt; hipGraphExec_t exec; };
+  std::vector<StepGraph> graphs;
+  // kernel timing: when non-null, fused_stage records an event after every launch
+  hipEvent_t *prof_ev = nullptr;
+  int prof_n = 0, prof_cap = 0;
   std::string err;
 };
 
@@ -118,4 +127,9 @@ int ops_diagnose(gmc_ctx *ctx, int level, double out[4]);
 int ops_integrate(gmc_ctx *ctx, int old_l, int new_l, double dt);
 // fused stage path (gmc_fused.hip)
 int fused_integrate(gmc_ctx *ctx, int old_l, int new_l, double dt);
+void fused_release(gmc_ctx *ctx);
+int fused_prepare(gmc_ctx *ctx);
+int fused_step_graph(gmc_ctx *ctx, int old_l, int new_l, double dt);
+constexpr int kKernelClasses = 3;  // k_tend, k_gzf, k_uvf
+extern const char *const kKernelNames[kKernelClasses];
 }  // namespace gmc

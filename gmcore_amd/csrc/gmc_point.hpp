@@ -13,8 +13,9 @@ namespace gmc {
 __device__ __forceinline__ double m_of(double gz, double gzs, double g) { return (gz - gzs) / g; }
 
 // interp_cell_to_vtx, interp_mod.F90:455-459
+template <class MT>
 __device__ __forceinline__ double m_vtx_of(double m00, double m01, double m10, double m11,
-                                           const Mesh &M, int r) {
+                                           const MT &M, int r) {
   return ((m00 + m01) * M(GMC_AREA_SUBCELL2, r) + (m10 + m11) * M(GMC_AREA_SUBCELL1, r + 1)) /
          M(GMC_AREA_VTX, r);
 }
@@ -27,7 +28,8 @@ __device__ __forceinline__ double tangent_sum(double w1, double w2, double a, do
 
 // calc_ke, ke_scheme 1, operators_mod.F90:247-252.  uw/ue: u west/east of
 // the cell; vs/vn: v on the south/north edge.
-__device__ __forceinline__ double ke_of(double uw, double ue, double vs, double vn, const Mesh &M,
+template <class MT>
+__device__ __forceinline__ double ke_of(double uw, double ue, double vs, double vn, const MT &M,
                                         int r) {
   return (M(GMC_AREA_LON_WEST, r) * (uw * uw) + M(GMC_AREA_LON_EAST, r) * (ue * ue) +
           M(GMC_AREA_LAT_NORTH, r - 1) * (vs * vs) + M(GMC_AREA_LAT_SOUTH, r) * (vn * vn)) /
@@ -35,7 +37,8 @@ __device__ __forceinline__ double ke_of(double uw, double ue, double vs, double 
 }
 
 // calc_div, operators_mod.F90:370-375
-__device__ __forceinline__ double div_of(double ue, double uw, double vn, double vs, const Mesh &M,
+template <class MT>
+__device__ __forceinline__ double div_of(double ue, double uw, double vn, double vs, const MT &M,
                                          int r) {
   return ((ue * M(GMC_LE_LON, r) - uw * M(GMC_LE_LON, r)) +
           (vn * M(GMC_LE_LAT, r) - vs * M(GMC_LE_LAT, r - 1))) /
@@ -43,7 +46,8 @@ __device__ __forceinline__ double div_of(double ue, double uw, double vn, double
 }
 
 // calc_vor, operators_mod.F90:629-634 (vertex row r)
-__device__ __forceinline__ double vor_of(double us, double un, double ve, double vw, const Mesh &M,
+template <class MT>
+__device__ __forceinline__ double vor_of(double us, double un, double ve, double vw, const MT &M,
                                          int r) {
   return (us * M(GMC_DE_LON, r) - un * M(GMC_DE_LON, r + 1) + ve * M(GMC_DE_LAT, r) -
           vw * M(GMC_DE_LAT, r)) /
@@ -51,7 +55,8 @@ __device__ __forceinline__ double vor_of(double us, double un, double ve, double
 }
 
 // calc_pv, operators_mod.F90:697
-__device__ __forceinline__ double pv_of(double vor, const Mesh &M, int r, double m_vtx) {
+template <class MT>
+__device__ __forceinline__ double pv_of(double vor, const MT &M, int r, double m_vtx) {
   return (vor + M(GMC_HALF_F, r)) / m_vtx;
 }
 
@@ -69,7 +74,8 @@ __device__ __forceinline__ double pv_upwind_edge(int order, double wgt, double t
 }
 
 // calc_coriolis scheme 1, operators_mod.F90:866-877 (V point, row r)
-__device__ __forceinline__ double qhu_of(const Mesh &M, int r, double pl, double mfx_w, double pvl_w,
+template <class MT>
+__device__ __forceinline__ double qhu_of(const MT &M, int r, double pl, double mfx_w, double pvl_w,
                                          double mfx_e, double pvl_e, double mfx_nw, double pvl_nw,
                                          double mfx_ne, double pvl_ne) {
   return (M(GMC_HALF_TANGENT_WGT1, r) * (mfx_w * (pl + pvl_w) + mfx_e * (pl + pvl_e)) +
@@ -78,7 +84,8 @@ __device__ __forceinline__ double qhu_of(const Mesh &M, int r, double pl, double
 }
 
 // calc_coriolis scheme 1, operators_mod.F90:887-898 (U point, row r)
-__device__ __forceinline__ double qhv_of(const Mesh &M, int r, double pl, double mfy_sw, double pvt_sw,
+template <class MT>
+__device__ __forceinline__ double qhv_of(const MT &M, int r, double pl, double mfy_sw, double pvt_sw,
                                          double mfy_se, double pvt_se, double mfy_nw, double pvt_nw,
                                          double mfy_ne, double pvt_ne) {
   return (M(GMC_FULL_TANGENT_WGT1, r) * (mfy_sw * (pl + pvt_sw) + mfy_se * (pl + pvt_se)) +
@@ -87,18 +94,22 @@ __device__ __forceinline__ double qhv_of(const Mesh &M, int r, double pl, double
 }
 
 // calc_grad_mf, operators_mod.F90:956-970
-__device__ __forceinline__ double dmfdlon_of(double mfx_e, double mfx_w, const Mesh &M, int r) {
+template <class MT>
+__device__ __forceinline__ double dmfdlon_of(double mfx_e, double mfx_w, const MT &M, int r) {
   return (mfx_e - mfx_w) * M(GMC_LE_LON, r) / M(GMC_AREA_CELL, r);
 }
-__device__ __forceinline__ double dmfdlat_of(double mfy_n, double mfy_s, const Mesh &M, int r) {
+template <class MT>
+__device__ __forceinline__ double dmfdlat_of(double mfy_n, double mfy_s, const MT &M, int r) {
   return (mfy_n * M(GMC_LE_LAT, r) - mfy_s * M(GMC_LE_LAT, r - 1)) / M(GMC_AREA_CELL, r);
 }
 
 // pgf_swm_run, pgf_swm_mod.F90:33, :40
-__device__ __forceinline__ double pgf_lon_of(double gz_e, double gz_w, const Mesh &M, int r) {
+template <class MT>
+__device__ __forceinline__ double pgf_lon_of(double gz_e, double gz_w, const MT &M, int r) {
   return (gz_e - gz_w) / M(GMC_DE_LON, r);
 }
-__device__ __forceinline__ double pgf_lat_of(double gz_n, double gz_s, const Mesh &M, int r) {
+template <class MT>
+__device__ __forceinline__ double pgf_lat_of(double gz_n, double gz_s, const MT &M, int r) {
   return (gz_n - gz_s) / M(GMC_DE_LAT, r);
 }
 

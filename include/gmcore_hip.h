@@ -162,7 +162,8 @@ int gmc_time_integrate(gmc_ctx *ctx, int32_t old_level, int32_t new_level, doubl
 /* `time_integrator` (time_schemes_mod.F90:244-288) with step =>
  * step_forward_backward, followed by damp_run (damp_mod.F90:49-63), then the
  * old/new swap of time_advance; repeated nsteps times.  Uses the fused
- * stage kernels; device resident, nothing crosses PCIe. */
+ * stage kernels, replayed from a HIP graph captured on first use for each
+ * (old, new, dt); device resident, nothing crosses PCIe. */
 int gmc_step(gmc_ctx *ctx, double dt, int32_t nsteps);
 
 /* Current time level index of `old` (1..3) after gmc_step's swaps. */
@@ -174,6 +175,17 @@ int gmc_diagnose(gmc_ctx *ctx, int32_t level, double out[4]);
 
 /* Wait for all work queued on the context's stream. */
 int gmc_synchronize(gmc_ctx *ctx);
+
+/* Per-kernel timing of the fused step: runs nsteps steps exactly as gmc_step
+ * does (the state advances), launched one by one with HIP events around every
+ * launch on the context's stream.  out_ms[k] receives the mean duration in
+ * milliseconds of kernel class k (at most cap classes, *n set to the count);
+ * gmc_kernel_name(k) names them. */
+int gmc_kernel_times(gmc_ctx *ctx, double dt, int32_t nsteps, double *out_ms, int32_t cap, int32_t *n);
+const char *gmc_kernel_name(int32_t k);
+
+/* The HIP stream (hipStream_t) all work of the context is queued on. */
+int gmc_stream(gmc_ctx *ctx, void **stream);
 
 /* Device pointer of a field (for profilers and HIP-graph callers).  The
  * pointer stays valid until gmc_destroy.  Row pitch in doubles in *pitch. */

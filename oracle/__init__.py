@@ -41,6 +41,7 @@ def lib() -> C.CDLL:
         L.gmo_sw_halo_width.argtypes = [vp]
         for fn in ("gmo_sw_set", "gmo_sw_get"):
             getattr(L, fn).argtypes = [vp, C.c_int, C.c_int, P(C.c_double), C.c_int64]
+        L.gmo_sw_get_internal.argtypes = [vp, C.c_int, C.c_int, C.c_int, P(C.c_double), C.c_int64]
         L.gmo_sw_prepare.argtypes = [vp, C.c_int]
         L.gmo_sw_space_operators.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                              C.c_double, C.c_int]
@@ -114,6 +115,22 @@ class SWOracle:
     def get(self, level: int, name: str) -> np.ndarray:
         a = np.zeros((field_rows(name, self.nlat), self.nlon))
         rc = lib().gmo_sw_get(self.h, level, FIELD[name], dptr(a), self.nlon)
+        assert rc == 0
+        return a
+
+    STATE_FIELDS = ["u_lon", "v_lat", "gz", "gz_f", "u_f", "v_f", "m", "m_lon", "m_lat", "m_vtx",
+                    "mfx_lon", "mfy_lat", "mfx_lat", "mfy_lon", "u_lat", "v_lon", "ke", "div", "vor",
+                    "pv", "pv_lon", "pv_lat"]
+    STATE_HALF = {"v_lat", "v_f", "m_lat", "m_vtx", "mfy_lat", "mfx_lat", "u_lat", "vor", "pv", "pv_lat"}
+    TEND_FIELDS = ["du", "dv", "dgz", "qhu", "qhv", "dkedlon", "dkedlat", "dmfdlon", "dmfdlat",
+                   "pgf_lon", "pgf_lat"]
+    TEND_HALF = {"dv", "qhu", "dkedlat", "pgf_lat"}
+
+    def internal(self, level: int, name: str, tend: bool = False) -> np.ndarray:
+        """Any state_type (tend=False) or tend_type (tend=True) field, interior."""
+        names, half = (self.TEND_FIELDS, self.TEND_HALF) if tend else (self.STATE_FIELDS, self.STATE_HALF)
+        a = np.zeros((self.nlat - 1 if name in half else self.nlat, self.nlon))
+        rc = lib().gmo_sw_get_internal(self.h, level, int(tend), names.index(name), dptr(a), self.nlon)
         assert rc == 0
         return a
 

@@ -48,6 +48,13 @@ int gmo_sw_halo_width(const gmo_sw *s);
 int gmo_sw_set(gmo_sw *s, int level, int field, const double *host, int64_t ld);
 int gmo_sw_get(gmo_sw *s, int level, int field, double *host, int64_t ld);
 
+/* interior of any state_type field (is_tend = 0; index in the order u_lon,
+ * v_lat, gz, gz_f, u_f, v_f, m, m_lon, m_lat, m_vtx, mfx_lon, mfy_lat,
+ * mfx_lat, mfy_lon, u_lat, v_lon, ke, div, vor, pv, pv_lon, pv_lat) or
+ * tend_type field (is_tend = 1; du, dv, dgz, qhu, qhv, dkedlon, dkedlat,
+ * dmfdlon, dmfdlat, pgf_lon, pgf_lat) */
+int gmo_sw_get_internal(gmo_sw *s, int level, int is_tend, int index, double *host, int64_t ld);
+
 /* gmcore_run's initial filters + operators_prepare_1 (gmcore_mod.F90:112-133) */
 int gmo_sw_prepare(gmo_sw *s, int level);
 /* space_operators (gmcore_mod.F90:343-575), SW branches */

@@ -680,6 +680,19 @@ int gmo_sw_set(gmo_sw *s, int level, int field, const double *host, int64_t ld) 
   return GMC_OK;
 }
 
+/* interior of any state (is_tend = 0, index S_*) or tendency (1, T_*) field */
+int gmo_sw_get_internal(gmo_sw *s, int level, int is_tend, int index, double *host, int64_t ld) {
+  static const int s_half[S_N] = {0, 1, 0, 0, 0, 1, 0, 0, 1, 1, 0, 1, 1, 0, 1, 0, 0, 0, 1, 1, 0, 1};
+  static const int t_half[T_N] = {0, 1, 0, 1, 0, 0, 1, 0, 0, 0, 1};
+  if (level < 1 || level > 3 || !host) return GMC_EINVAL;
+  if (index < 0 || index >= (is_tend ? T_N : S_N)) return GMC_EINVAL;
+  double *a = is_tend ? s->td[level][index] : s->st[level][index];
+  const int nr = (is_tend ? t_half[index] : s_half[index]) ? s->nhalf : s->nlat;
+  for (int j = 1; j <= nr; j++)
+    for (int i = 1; i <= s->nlon; i++) host[(size_t)(j - 1) * ld + i - 1] = A(s, a, i, j);
+  return GMC_OK;
+}
+
 int gmo_sw_get(gmo_sw *s, int level, int field, double *host, int64_t ld) {
   int kind;
   double *a = field_of(s, level, field, &kind);

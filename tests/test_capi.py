@@ -1,0 +1,43 @@
+"""The C-ABI libraries load and export every entry point their headers
+declare (CPU only: no device calls)."""
+import ctypes as C
+import os
+import re
+
+from gmcore_amd import capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gmc_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_hip_library_exports_header():
+    names = declared("gmcore_hip.h")
+    assert len(names) >= 15
+    lib = C.CDLL(capi.LIB_HIP)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_mirror_covers_header():
+    lib = capi.load_hip()
+    for n in declared("gmcore_hip.h"):
+        assert getattr(lib, n).argtypes is not None, f"{n} has no ctypes signature in capi.py"
+
+
+def test_host_library_exports():
+    lib = C.CDLL(capi.LIB_HOST)
+    for n in ("gmch_mesh_metrics", "gmch_filter"):
+        assert hasattr(lib, n)
+
+
+def test_struct_layouts_match_header():
+    # gmc_mesh: int32, int32, double, pointer; gmc_filter: int32 + 4 pointers
+    assert C.sizeof(capi.GmcMesh) == 24
+    assert C.sizeof(capi.GmcFilter) == 40
+    assert capi.GmcConfig.big_filter.offset % 8 == 0
+    assert capi.GMC_MESH_NFIELDS == 32
