@@ -9,10 +9,11 @@ rows), FP64, polar filter on, pc2, dt = 60 s, seeded balanced initial state
 captured HIP graph, inputs resident in HBM.
 
 Prints one JSON line.  value = grid-point-steps per second (nlon * nlat per
-step).  roofline: algorithmic bytes of the step (35 FP64 field passes per
-grid point, 280 B/gp) over the measured step time against 8 TB/s.  The
-per-kernel breakdown comes from gmc_kernel_times (HIP events around every
-launch on the library's stream).  cpu_baseline: the oracle (a scalar C
+step).  roofline: the dominant kernel's algorithmic bytes per launch over its
+mean launch time from gmc_kernel_times (HIP events around every launch on the
+library's stream) against 8 TB/s, with the HBM traffic per launch measured by
+rocprofv3 counters.  step_roofline: the algorithmic bytes of the whole step
+(35 FP64 field passes per grid point, 280 B/gp) over the step time.  cpu_baseline: the oracle (a scalar C
 restatement of the reference, one core) timed on a bounded sample of the
 same workload.
 """
@@ -35,6 +36,12 @@ BYTES_PER_GP_STEP = 280.0          # 35 FP64 field passes per pc2 step
 # over the three stages (k_tend reads 4 fields in stage 1 and 5 in stages
 # 2-3 and writes 3; k_gzf reads 1, writes 2; k_uvf reads 5, writes 4)
 KERNEL_BYTES_PER_GP = {"k_tend": 8.0 * (7 + 8 + 8) / 3, "k_gzf": 24.0, "k_uvf": 72.0}
+# HBM bytes per grid point of one launch measured with rocprofv3 --pmc
+# FETCH_SIZE (x 2.00) and WRITE_SIZE (x 1.00), the factors a 1 GiB device copy
+# reads; 1440 x 721 workload, profiles/sw025_step_kernels.md
+KERNEL_TRAFFIC_PER_GP = {"k_tend": (2 * 19959.5 + 24314.3) * 1024 / (1440 * 721),
+                         "k_gzf": (2 * 4410.8 + 16222.5) * 1024 / (1440 * 721),
+                         "k_uvf": (2 * 29576.1 + 32400.0) * 1024 / (1440 * 721)}
 
 CONFIGS = {
     "sw025": dict(nlon=1440, nlat=721, dt=60.0, lmax=60, seed=1),
@@ -161,7 +168,8 @@ def main():
         kernels.append({"name": name, "avg_us": kt[k] * 1e3,
                         "bytes_per_launch": b, "gbs": b / (kt[k] * 1e-3) / 1e9 if kt[k] > 0 else 0.0})
     dom = max(kernels, key=lambda x: x["avg_us"])
-    achieved = BYTES_PER_GP_STEP * gp / (ms_step * 1e-3) / 1e9
+    step_gbs = BYTES_PER_GP_STEP * gp / (ms_step * 1e-3) / 1e9
+    traffic = KERNEL_TRAFFIC_PER_GP.get(dom["name"]) if args.config == "sw025" else None
     result = {
         "metric": "grid-point-steps/s (FP64 SW pc2 step, 1 GPU)",
         "value": gp / (ms_step * 1e-3),
@@ -179,9 +187,16 @@ def main():
         "config": {"workload": f"shallow-water {args.config}: {cfg['nlon']}x{cfg['nlat']} C-grid, FP64, "
                                f"pc2, dt={cfg['dt']:g} s, polar filter on, seeded l<={cfg['lmax']}",
                    "nlon": cfg["nlon"], "nlat": cfg["nlat"], "dt": cfg["dt"], "time_scheme": "pc2"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "scope": "whole pc2 step (one graph launch): 280 B/grid point algorithmic"},
+        "roofline": {"bound": "hbm", "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom["gbs"] / HBM_PEAK_GBS,
+                     "traffic": traffic * gp if traffic else None,
+                     "kernel": dom["name"],
+                     "scope": f"dominant kernel {dom['name']}: algorithmic bytes per launch "
+                              f"{KERNEL_BYTES_PER_GP[dom['name']]:.2f} B/grid point over its mean "
+                              "launch time (HIP events on the library stream)"},
+        "step_roofline": {"bound": "hbm", "achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": step_gbs / HBM_PEAK_GBS,
+                          "scope": "whole pc2 step (one graph launch): 280 B/grid point algorithmic"},
         "dominant_kernel": dom,
         "kernels": kernels,
         "mass_rel_drift": float((d1[0] - d0[0]) / d0[0]),
