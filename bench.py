@@ -43,6 +43,15 @@ KERNEL_TRAFFIC_PER_GP = {"k_tend": (2 * 19959.5 + 24314.3) * 1024 / (1440 * 721)
                          "k_gzf": (2 * 4410.8 + 16222.5) * 1024 / (1440 * 721),
                          "k_uvf": (2 * 29576.1 + 32400.0) * 1024 / (1440 * 721)}
 
+# FFSL tracer advection, 0.5 degrees, 8 tracers, dt = 900 s, one day
+ADV_CONFIGS = {
+    "ffsl05": dict(nlon=720, nlat=361, dt=900.0, ntracers=8, lmax=10, wind_seed=2, tracer_seed=3,
+                   vmax=50.0, hw=104, steps=96),
+}
+# compulsory bytes per tracer-grid-point of an advection step: every tracer
+# read and written once, the 4 wind fields (u, v, mfx, mfy) shared
+ADV_BYTES_PER_TRACER_GP = lambda ntr: (2 * ntr + 4) * 8.0 / ntr
+
 CONFIGS = {
     "sw025": dict(nlon=1440, nlat=721, dt=60.0, lmax=60, seed=1),
     "sw2": dict(nlon=180, nlat=91, dt=600.0, lmax=20, seed=0),
@@ -92,12 +101,121 @@ def cpu_baseline(st, ic, cfg, budget_s=15.0):
                       f"step, {el:.1f} s, oracle built gcc -O2 -ffp-contract=off, 1 thread"}
 
 
+def cpu_baseline_adv(st, qs, ub, vb, cfg, budget_s=15.0):
+    """Oracle (scalar C, one core) on the same advection workload: steps
+    until about budget_s seconds, after the accumulate-only first call."""
+    import oracle
+    from gmcore_amd import init_cond
+    o = oracle.AdvOracle(st.mesh_struct, st.cfg)
+    for l, q in enumerate(qs):
+        o.set_tracer(1, l, q)
+    nlat, nlon = cfg["nlat"], cfg["nlon"]
+    ones = np.ones((nlat, nlon))
+    sc = init_cond.wind_scales(200, cfg["dt"])
+    n, el = 0, 0.0
+    for step in range(200):
+        u, v = ub * sc[step], vb * sc[step]
+        o.set_wind(u, v, u, v, ones)
+        t0 = time.perf_counter()
+        o.run(step)
+        if step > 0:
+            el += time.perf_counter() - t0
+            n += 1
+        if el >= budget_s:
+            break
+    o.close()
+    tgp = cfg["nlon"] * cfg["nlat"] * cfg["ntracers"]
+    return {"value": tgp * n / el, "unit": "tracer-grid-point-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n} advection steps of the same {nlon}x{nlat}x{cfg['ntracers']}-tracer workload "
+                      f"after the accumulate-only first call, {el:.1f} s of adv_run, oracle built "
+                      "gcc -O2 -ffp-contract=off, 1 thread"}
+
+
+def run_adv(args):
+    """FFSL advection benchmark: one step = adv_run on all tracers (set_uv of
+    the prescribed flow included), device resident."""
+    import ctypes as C
+    from gmcore_amd import init_cond
+    from gmcore_amd.adv import Advection, AdvSetup
+    from gmcore_amd.namelist import Namelist
+    cfg = ADV_CONFIGS[args.config]
+    nlon, nlat, dt, ntr = cfg["nlon"], cfg["nlat"], cfg["dt"], cfg["ntracers"]
+    nl = Namelist(num_lon=nlon, num_lat=nlat, dt_adv=dt, dt_dyn=dt, limiter_type="mono")
+    st = AdvSetup(nl, ntr, lon_halo_width=cfg["hw"])
+    m = st.mesh.metrics
+    ub, vb = init_cond.deform_wind_basis(nlon, nlat, m, cfg["lmax"], cfg["wind_seed"], cfg["vmax"])
+    qs = init_cond.tracer_set(nlon, nlat, m, cfg["tracer_seed"], ntr)
+    g = Advection(st)
+    for l, q in enumerate(qs):
+        g.set_tracer(1, l, q)
+    g.set_wind_basis(ub, vb)
+    ac = st.mesh["area_cell"][:, None]
+    import math
+    m0 = [math.fsum((q * ac).ravel()) for q in qs]
+    total = args.warmup + args.steps + 1
+    sc = init_cond.wind_scales(total + 8, dt)
+    g.drive(0, sc[:1 + args.warmup])
+    g.synchronize()
+    hip = C.CDLL("libamdhip64.so")
+    stream = C.c_void_p()
+    assert g.lib.gmc_adv_stream(g.h, C.byref(stream)) == 0
+    ev0, ev1 = C.c_void_p(), C.c_void_p()
+    assert hip.hipEventCreate(C.byref(ev0)) == 0 and hip.hipEventCreate(C.byref(ev1)) == 0
+    w0 = time.perf_counter()
+    hip.hipEventRecord(ev0, stream)
+    g.drive(1 + args.warmup, sc[1 + args.warmup:total])
+    hip.hipEventRecord(ev1, stream)
+    hip.hipEventSynchronize(ev1)
+    wall = time.perf_counter() - w0
+    ms = C.c_float()
+    hip.hipEventElapsedTime(C.byref(ms), ev0, ev1)
+    ms_step = ms.value / args.steps
+    qend = [g.tracer(g.old, l) for l in range(ntr)]
+    drift = max(abs(math.fsum((q * ac).ravel()) - m0[l]) / m0[l] for l, q in enumerate(qend[:-1]))
+    qmin = min(float(q.min()) for q in qend)
+    # per-kernel breakdown
+    kt = np.zeros(8)
+    nk = C.c_int32()
+    from gmcore_amd import capi
+    rc = g.lib.gmc_adv_kernel_times(g.h, total, capi.dptr(sc[total:total + 4].copy()), 4, capi.dptr(kt), 8,
+                                    C.byref(nk))
+    if rc:
+        raise RuntimeError("gmc_adv_kernel_times failed")
+    tgp = nlon * nlat * ntr
+    kernels = [{"name": g.lib.gmc_adv_kernel_name(k).decode(), "avg_us": kt[k] * 1e3} for k in range(nk.value)]
+    step_bytes = ADV_BYTES_PER_TRACER_GP(ntr) * tgp
+    achieved = step_bytes / (ms_step * 1e-3) / 1e9
+    result = {
+        "metric": "tracer-grid-point-steps/s (FP64 FFSL advection, 1 GPU)",
+        "value": tgp / (ms_step * 1e-3),
+        "unit": "tracer-grid-point-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_step, "wall_ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded non-divergent wind x cos(pi t / 12 d); cosine bells + one uniform-random tracer)",
+        "config": {"workload": f"FFSL {args.config}: {nlon}x{nlat}, {ntr} tracers, FP64, dt={dt:g} s, "
+                               f"PPM mono, lon_halo_width {cfg['hw']}",
+                   "nlon": nlon, "nlat": nlat, "ntracers": ntr, "dt": dt},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "scope": f"whole advection step: {ADV_BYTES_PER_TRACER_GP(ntr):.1f} B per tracer-grid-point"},
+        "kernels": kernels,
+        "mass_rel_drift_bell_tracers": drift,
+        "min_tracer": qmin,
+        "fixes": g.fixes,
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline_adv(st, qs, ub, vb, cfg, args.cpu_budget)
+    g.close()
+    print(json.dumps(result))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="sw025", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="sw025", choices=sorted(CONFIGS) + sorted(ADV_CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -119,6 +237,9 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU latitude-band decomposition is a design, not code)", file=sys.stderr)
         sys.exit(2)
+    if args.config in ADV_CONFIGS:
+        run_adv(args)
+        return
     cfg = CONFIGS[args.config]
     from gmcore_amd.host import ShallowWater
     from gmcore_amd import capi

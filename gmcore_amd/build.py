@@ -1,8 +1,9 @@
 """Build the native parts in-tree.
 
 - libgmcore_hip.so: the gfx950 HIP library (hipcc), C ABI of include/gmcore_hip.h
+  and include/gmcore_adv.h
 - libgmcore_host.so: the Fortran host setup (amdflang): mesh metrics, filter tables
-- gmcore_swm_gpu: the Fortran driver (amdflang) on top of both
+- gmcore_swm_gpu, gmcore_adv_gpu: the Fortran drivers (amdflang) on top of both
 
 Every HIP translation unit is compiled for gfx950 with -ffp-contract=off
 (each product and sum separately rounded, as the parity tests require) and
@@ -20,12 +21,14 @@ LIB = os.path.join(HERE, "lib")
 BUILD = os.path.join(HERE, "_build")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-HIP_SOURCES = ["gmc_capi.hip", "gmc_ops.hip", "gmc_fused.hip"]
+HIP_SOURCES = ["gmc_capi.hip", "gmc_ops.hip", "gmc_fused.hip", "gmc_adv.hip"]
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall"]
 F_FLAGS = ["-O2", "-fPIC", "-ffp-contract=off"]
-F_HOST = ["gmc_kinds_mod.F90", "gmc_mesh_mod.F90", "gmc_filter_mod.F90", "gmc_host_capi.F90"]
+F_HOST = ["gmc_kinds_mod.F90", "gmc_mesh_mod.F90", "gmc_filter_mod.F90", "gmc_lon_damp_mod.F90",
+          "gmc_host_capi.F90"]
 F_DRIVER = ["gmc_hip_mod.F90", "gmc_dynamics_mod.F90", "gmcore_swm_gpu.F90"]
+F_ADV = ["gmc_adv_mod.F90", "gmcore_adv_gpu.F90"]
 
 
 def _run(cmd, cwd=None):
@@ -54,7 +57,7 @@ def build_fortran() -> tuple[str, str]:
     os.makedirs(BUILD, exist_ok=True)
     fc = os.path.join(ROCM, "bin", "amdflang")
     objs = {}
-    for src in F_HOST + F_DRIVER:
+    for src in F_HOST + F_DRIVER + F_ADV:
         obj = os.path.join(BUILD, src.replace(".F90", ".o"))
         _run([fc, *F_FLAGS, "-module-dir", BUILD, "-c", os.path.join(HERE, "fortran", src), "-o", obj])
         objs[src] = obj
@@ -65,6 +68,10 @@ def build_fortran() -> tuple[str, str]:
     _run([fc, *F_FLAGS, *[objs[s] for s in F_HOST[:3] + F_DRIVER], "-o", drv + ".tmp",
           "-L" + LIB, "-lgmcore_hip", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{ROCM}/lib"])
     os.replace(drv + ".tmp", drv)
+    adv = os.path.join(LIB, "gmcore_adv_gpu")
+    _run([fc, *F_FLAGS, *[objs[s] for s in F_HOST[:4] + F_DRIVER[:1] + F_ADV], "-o", adv + ".tmp",
+          "-L" + LIB, "-lgmcore_hip", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{ROCM}/lib"])
+    os.replace(adv + ".tmp", adv)
     return host, drv
 
 

@@ -1,4 +1,5 @@
-"""ctypes mirror of include/gmcore_hip.h (structs, enums, signatures).
+"""ctypes mirror of include/gmcore_hip.h and include/gmcore_adv.h (structs,
+enums, signatures).
 
 The C ABI is the drop-in boundary of the dynamics path; this module only
 describes it.  ``load_hip()`` opens the in-tree ``libgmcore_hip.so`` and fails
@@ -67,6 +68,18 @@ class GmcConfig(C.Structure):
                 ("big_filter", GmcFilter), ("small_filter1", GmcFilter)]
 
 
+GMC_LIMITER_MONO, GMC_LIMITER_PD, GMC_LIMITER_NONE = 0, 1, 2
+LIMITERS = {"mono": GMC_LIMITER_MONO, "pd": GMC_LIMITER_PD, "none": GMC_LIMITER_NONE}
+
+
+class GmcAdvConfig(C.Structure):
+    """gmc_adv_config of include/gmcore_adv.h"""
+    _fields_ = [("radius", C.c_double), ("dt_adv", C.c_double), ("nstep", C.c_int32),
+                ("ntracers", C.c_int32), ("lon_halo_width", C.c_int32), ("limiter", C.c_int32),
+                ("lon_damp_passes", C.c_int32), ("reserved", C.c_int32),
+                ("lon_damp_wgt", C.POINTER(C.c_double))]
+
+
 def dptr(a):
     """double* of a contiguous float64 numpy array."""
     return a.ctypes.data_as(C.POINTER(C.c_double))
@@ -109,6 +122,25 @@ def load_hip() -> C.CDLL:
         "gmc_kernel_name": (C.c_char_p, [C.c_int32]),
         "gmc_stream": (C.c_int, [vp, P(vp)]),
         "gmc_field_ptr": (C.c_int, [vp, C.c_int32, C.c_int32, P(P(C.c_double)), P(C.c_int64)]),
+        # FFSL advection, include/gmcore_adv.h
+        "gmc_adv_create": (C.c_int, [P(GmcMesh), P(GmcAdvConfig), P(vp)]),
+        "gmc_adv_destroy": (C.c_int, [vp]),
+        "gmc_adv_last_error": (C.c_char_p, [vp]),
+        "gmc_adv_upload_tracer": (C.c_int, [vp, C.c_int32, C.c_int32, P(C.c_double), C.c_int64]),
+        "gmc_adv_download_tracer": (C.c_int, [vp, C.c_int32, C.c_int32, P(C.c_double), C.c_int64]),
+        "gmc_adv_set_wind": (C.c_int, [vp, P(C.c_double), P(C.c_double), P(C.c_double),
+                                       P(C.c_double), P(C.c_double), C.c_int64]),
+        "gmc_adv_run": (C.c_int, [vp, C.c_int64]),
+        "gmc_adv_set_wind_basis": (C.c_int, [vp, P(C.c_double), P(C.c_double), C.c_int64]),
+        "gmc_adv_drive": (C.c_int, [vp, C.c_int64, P(C.c_double), C.c_int32]),
+        "gmc_adv_old_level": (C.c_int, [vp, P(C.c_int32)]),
+        "gmc_adv_mass": (C.c_int, [vp, C.c_int32, P(C.c_double)]),
+        "gmc_adv_fix_count": (C.c_int, [vp, P(C.c_int64)]),
+        "gmc_adv_synchronize": (C.c_int, [vp]),
+        "gmc_adv_stream": (C.c_int, [vp, P(vp)]),
+        "gmc_adv_kernel_times": (C.c_int, [vp, C.c_int64, P(C.c_double), C.c_int32, P(C.c_double),
+                                           C.c_int32, P(C.c_int32)]),
+        "gmc_adv_kernel_name": (C.c_char_p, [C.c_int32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

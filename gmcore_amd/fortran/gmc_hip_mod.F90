@@ -1,5 +1,5 @@
 ! Thin ISO_C_BINDING layer over the C ABI of libgmcore_hip.so
-! (include/gmcore_hip.h).  Every function returns 0 on success; gmc_check
+! (include/gmcore_hip.h, include/gmcore_adv.h).  Every function returns 0 on success; gmc_check
 ! turns a non-zero code into the message of gmc_last_error and stops, which
 ! keeps the reference's abort-on-error behaviour (log_error / process_stop,
 ! src/utils/process_mod.F90:54-68).
@@ -33,6 +33,14 @@ module gmc_hip_mod
     real(c_double) :: div_damp_coef2
     type(gmc_filter) :: big_filter, small_filter1
   end type gmc_config
+
+  integer(c_int32_t), parameter :: GMC_LIMITER_MONO = 0, GMC_LIMITER_PD = 1, GMC_LIMITER_NONE = 2
+
+  type, bind(C) :: gmc_adv_config
+    real(c_double) :: radius, dt_adv
+    integer(c_int32_t) :: nstep, ntracers, lon_halo_width, limiter, lon_damp_passes, reserved
+    type(c_ptr) :: lon_damp_wgt
+  end type gmc_adv_config
 
   interface
     integer(c_int) function gmc_create(mesh, cfg, ctx) bind(C, name='gmc_create')
@@ -81,6 +89,51 @@ module gmc_hip_mod
     integer(c_int) function gmc_synchronize(ctx) bind(C, name='gmc_synchronize')
       import; type(c_ptr), value :: ctx
     end function
+    ! FFSL tracer advection
+    integer(c_int) function gmc_adv_create(mesh, cfg, adv) bind(C, name='gmc_adv_create')
+      import; type(gmc_mesh), intent(in) :: mesh; type(gmc_adv_config), intent(in) :: cfg
+      type(c_ptr), intent(out) :: adv
+    end function
+    integer(c_int) function gmc_adv_destroy(adv) bind(C, name='gmc_adv_destroy')
+      import; type(c_ptr), value :: adv
+    end function
+    type(c_ptr) function gmc_adv_last_error(adv) bind(C, name='gmc_adv_last_error')
+      import; type(c_ptr), value :: adv
+    end function
+    integer(c_int) function gmc_adv_upload_tracer(adv, level, tracer, host, ld) &
+        bind(C, name='gmc_adv_upload_tracer')
+      import; type(c_ptr), value :: adv; integer(c_int32_t), value :: level, tracer
+      real(c_double), intent(in) :: host(*); integer(c_int64_t), value :: ld
+    end function
+    integer(c_int) function gmc_adv_download_tracer(adv, level, tracer, host, ld) &
+        bind(C, name='gmc_adv_download_tracer')
+      import; type(c_ptr), value :: adv; integer(c_int32_t), value :: level, tracer
+      real(c_double), intent(out) :: host(*); integer(c_int64_t), value :: ld
+    end function
+    integer(c_int) function gmc_adv_set_wind(adv, u, v, mfx, mfy, m, ld) bind(C, name='gmc_adv_set_wind')
+      import; type(c_ptr), value :: adv, u, v, mfx, mfy, m; integer(c_int64_t), value :: ld
+    end function
+    integer(c_int) function gmc_adv_run(adv, time_step) bind(C, name='gmc_adv_run')
+      import; type(c_ptr), value :: adv; integer(c_int64_t), value :: time_step
+    end function
+    integer(c_int) function gmc_adv_set_wind_basis(adv, u, v, ld) bind(C, name='gmc_adv_set_wind_basis')
+      import; type(c_ptr), value :: adv; real(c_double), intent(in) :: u(*), v(*)
+      integer(c_int64_t), value :: ld
+    end function
+    integer(c_int) function gmc_adv_drive(adv, time_step0, scale, nsteps) bind(C, name='gmc_adv_drive')
+      import; type(c_ptr), value :: adv; integer(c_int64_t), value :: time_step0
+      real(c_double), intent(in) :: scale(*); integer(c_int32_t), value :: nsteps
+    end function
+    integer(c_int) function gmc_adv_old_level(adv, old) bind(C, name='gmc_adv_old_level')
+      import; type(c_ptr), value :: adv; integer(c_int32_t), intent(out) :: old
+    end function
+    integer(c_int) function gmc_adv_mass(adv, level, out) bind(C, name='gmc_adv_mass')
+      import; type(c_ptr), value :: adv; integer(c_int32_t), value :: level
+      real(c_double), intent(out) :: out(*)
+    end function
+    integer(c_int) function gmc_adv_fix_count(adv, count) bind(C, name='gmc_adv_fix_count')
+      import; type(c_ptr), value :: adv; integer(c_int64_t), intent(out) :: count
+    end function
   end interface
 
 contains
@@ -94,7 +147,11 @@ contains
     if (rc == 0) return
     write(*, '(a, a, a, i0)') 'gmc error in ', what, ': code ', rc
     if (c_associated(ctx)) then
-      call c_f_pointer(gmc_last_error(ctx), msg, [512])
+      if (what(1:min(7, len(what))) == 'gmc_adv') then
+        call c_f_pointer(gmc_adv_last_error(ctx), msg, [512])
+      else
+        call c_f_pointer(gmc_last_error(ctx), msg, [512])
+      end if
       n = 0
       do while (n < 512)
         if (msg(n + 1) == c_null_char) exit

@@ -1,4 +1,5 @@
-! C entry points of the Fortran host setup (mesh metrics and filter tables),
+! C entry points of the Fortran host setup (mesh metrics, filter tables and
+! the longitudinal damping weights of the tracer advection),
 ! so that C, C++ and Python hosts build the same tables as the Fortran host.
 module gmc_host_capi
 
@@ -6,6 +7,7 @@ module gmc_host_capi
   use gmc_kinds_mod
   use gmc_mesh_mod
   use gmc_filter_mod
+  use gmc_lon_damp_mod
   implicit none
 
 contains
@@ -55,5 +57,17 @@ contains
     call filter_weights(nlat, int(maxn), ngl, ngt, w_lon, w_lat, wl(1:maxn*nlat), wt(1:maxn*(nlat-1)))
     rc = 0
   end function gmch_filter
+
+  ! lon_damp_init weights wgt(nlat) of the tracer advection
+  integer(c_int) function gmch_lon_damp_weights(nlat, metrics, wgt) &
+      bind(C, name='gmch_lon_damp_weights') result(rc)
+    integer(c_int), value :: nlat
+    real(c_double), intent(in) :: metrics(nlat, mesh_field_count)
+    real(c_double), intent(out) :: wgt(nlat)
+    rc = 1
+    if (nlat < 5) return
+    call lon_damp_weights(nlat, metrics, wgt)
+    rc = 0
+  end function gmch_lon_damp_weights
 
 end module gmc_host_capi

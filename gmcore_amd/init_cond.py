@@ -195,3 +195,65 @@ def steady_geostrophic(nlon, nlat, metrics, omega, radius, alpha=0.0):
     gz = gz0 - (radius * omega * u0 + u0 ** 2 * 0.5) * (
         fs[:, None] * ca - np.cos(full)[None, :] * fc[:, None] * sa) ** 2
     return u, v, gz, np.zeros((nlat, nlon))
+
+
+# ---- FFSL advection test inputs --------------------------------------------
+
+ADV_PERIOD = 12 * 86400.0   # deform_test_mod.F90:22
+
+
+def deform_wind_basis(nlon, nlat, metrics, lmax, seed, vmax=50.0):
+    """(u_lon, v_lat) of the prescribed advection flow at full amplitude:
+    discretely non-divergent winds from a seeded vertex streamfunction
+    (l <= lmax, variance l**-3) scaled to a peak of vmax.  The flow at time t
+    is this basis times cos(pi t / ADV_PERIOD), the time dependence of
+    deform_case1_test_set_uv (deform_test_mod.F90:129-167).  Pole rows of u
+    are zero: set_uv writes the rows between the poles only."""
+    full, half, dlon = _lons(nlon)
+    hlat = metrics[MESH["half_lat"], :nlat - 1]
+    psi = SphericalField(lmax, seed).eval(half, hlat)
+    u, v = winds_from_psi(psi, metrics, nlat)
+    s = vmax / max(np.abs(u).max(), np.abs(v).max())
+    return u * s, v * s
+
+
+def wind_scales(nsteps, dt, time_step0=0, period=ADV_PERIOD):
+    """Factor of the wind basis for adv_run at time steps time_step0 ..
+    time_step0 + nsteps - 1: the driver sets the wind of step n at
+    t = (n + 1) dt before calling adv_run (gmcore_adv_driver.F90:84-92)."""
+    import math
+    return np.array([math.cos(math.pi * ((n + 1) * dt) / period)
+                     for n in range(time_step0, time_step0 + nsteps)])
+
+
+def great_circle(lon1, lat1, lon, lat):
+    """Central angle between (lon1, lat1) and the points (lon, lat)."""
+    c = np.sin(lat1) * np.sin(lat) + np.cos(lat1) * np.cos(lat) * np.cos(lon - lon1)
+    return np.arccos(np.clip(c, -1.0, 1.0))
+
+
+def cosine_bells(nlon, nlat, metrics, seed, nbells=4, radius=0.5, background=0.1, amp=0.9):
+    """Background plus nbells cosine bells amp * (1 + cos(pi r / radius)) / 2
+    (r the central angle, radius 0.5 = R/2 on the sphere) at seeded centres
+    uniform on the sphere; the bell shape of deform_test_set_ic
+    (deform_test_mod.F90:75-90)."""
+    rng = np.random.default_rng(seed)
+    full, _, _ = _lons(nlon)
+    lat = metrics[MESH["full_lat"]]
+    LON, LAT = np.meshgrid(full, lat)
+    q = np.full((nlat, nlon), background)
+    for _ in range(nbells):
+        lon0 = rng.uniform(0, 2 * np.pi)
+        lat0 = np.arcsin(rng.uniform(-1, 1))
+        r = great_circle(lon0, lat0, LON, LAT)
+        q += np.where(r < radius, amp * 0.5 * (1 + np.cos(np.pi * r / radius)), 0.0)
+    return q
+
+
+def tracer_set(nlon, nlat, metrics, seed, ntracers):
+    """ntracers - 1 seeded sums of 4 cosine bells and one uniform-random
+    field in [0, 1], all positive."""
+    qs = [cosine_bells(nlon, nlat, metrics, seed * 1000 + l) for l in range(ntracers - 1)]
+    rng = np.random.default_rng(seed * 1000 + 999)
+    qs.append(rng.uniform(0.0, 1.0, size=(nlat, nlon)))
+    return qs

@@ -1,0 +1,114 @@
+/*
+ * gmcore_adv.h -- C ABI of the FFSL tracer-advection path of the MI355X
+ * (gfx950) library libgmcore_hip.so.
+ *
+ * The library is the GPU target of GMCORE's advection seam: the routine
+ * `adv_run(block, itime)` (src/dynamics/adv/adv_mod.F90:152-312) with the
+ * procedure pointer `adv_calc_tracer_hflx_cell => ffsl_calc_tracer_hflx_cell`
+ * (adv_mod.F90:106-112, 122-130; src/dynamics/adv/ffsl_mod.F90:318-413) and
+ * the wind accumulation of one `adv_batch_type` (adv_batch_mod.F90:19-69,
+ * 218-403).  One gmc_adv object is one advection batch of one block: its
+ * tracers q(:,:,1,l,1:2), the accumulated winds u, v, mfx, mfy, the CFL
+ * numbers cflx, cfly, the split divergences divx, divy and old_m.
+ *
+ * Conventions are those of gmcore_hip.h: binary64, 0-based rows counted
+ * from the south pole (full row r = reference j-1, half row r = half j-1),
+ * interior-only host buffers with leading dimension `ld`, return 0 or a
+ * GMC_E* code, one host thread and one HIP stream per object.
+ */
+#ifndef GMCORE_ADV_H
+#define GMCORE_ADV_H
+
+#include <stdint.h>
+#include "gmcore_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* slope limiter of the PPM edge values (namelist limiter_type,
+ * ffsl_mod.F90:131-140) */
+enum gmc_limiter {
+  GMC_LIMITER_MONO = 0,  /* 'mono', the default (slope_mono, ffsl_mod.F90:1183-1194) */
+  GMC_LIMITER_PD = 1,    /* 'pd'   (slope_pd,   ffsl_mod.F90:1196-1205) */
+  GMC_LIMITER_NONE = 2   /* 'none' (slope_simple, ffsl_mod.F90:1175-1181) */
+};
+
+typedef struct gmc_adv_config {
+  double radius;            /* sphere radius, const_mod radius                     */
+  double dt_adv;            /* batch%dt: advection time step in seconds            */
+  int32_t nstep;            /* batch%nstep = dt_adv / dt_dyn (adv_batch_mod.F90:89) */
+  int32_t ntracers;         /* tracers in the batch                                */
+  int32_t lon_halo_width;   /* bound on |cflx| (adv_batch_mod.F90:284-287); below nlon/2 */
+  int32_t limiter;          /* enum gmc_limiter                                    */
+  int32_t lon_damp_passes;  /* lon_damp_on_cell(order 2) calls per step (3, adv_mod.F90:299-301) */
+  int32_t reserved;
+  const double *lon_damp_wgt; /* nlat weights wgt_full_lat of lon_damp_init
+                                 (src/dynamics/damp/lon_damp_mod.F90:43-68); pole rows unused */
+} gmc_adv_config;
+
+typedef struct gmc_adv gmc_adv;
+
+/* adv_batch_init + adv_allocate_tracers (adv_batch_mod.F90:73-145,
+ * adv_mod.F90:334-406) for one cell-centred, non-dynamic batch.  All arrays
+ * start at zero; time level old = 1, new = 2. */
+int gmc_adv_create(const gmc_mesh *mesh, const gmc_adv_config *cfg, gmc_adv **out);
+int gmc_adv_destroy(gmc_adv *adv);
+const char *gmc_adv_last_error(const gmc_adv *adv);
+
+/* q(:,:,1,tracer+1,level) <-> host, level 1 or 2, tracer 0-based.  Upload
+ * also refreshes the periodic seam the reference keeps in the longitude
+ * halo (fill_halo after set_ic, deform_test_mod.F90:78-124). */
+int gmc_adv_upload_tracer(gmc_adv *adv, int32_t level, int32_t tracer, const double *host, int64_t ld);
+int gmc_adv_download_tracer(gmc_adv *adv, int32_t level, int32_t tracer, double *host, int64_t ld);
+
+/* Fields of block%state(itime) that adv_run reads (adv_mod.F90:420-455,
+ * 228-297, 309): u_f (nlat rows), v_f (nlat-1 rows), mfx_lon (nlat rows),
+ * mfy_lat (nlat-1 rows), m (nlat rows).  Any pointer may be NULL to keep the
+ * field's current value. */
+int gmc_adv_set_wind(gmc_adv *adv, const double *u_f, const double *v_f, const double *mfx_lon,
+                     const double *mfy_lat, const double *m, int64_t ld);
+
+/* adv_run(block, itime) (adv_mod.F90:152-312) at model step `time_step`:
+ * accumulate the winds (adv_accum_wind, :420-455); when time_step > 0 and the
+ * batch's alert rings (time_step a multiple of nstep, time_mod.F90:270-291)
+ * advance every tracer from level old to new and swap; finally copy_old_m. */
+int gmc_adv_run(gmc_adv *adv, int64_t time_step);
+
+/* Prescribed-flow driver (gmcore_adv_driver.F90:84-92): upload once a wind
+ * basis u_basis (nlat rows), v_basis (nlat-1 rows); gmc_adv_drive then does,
+ * for n = 0 .. nsteps-1, set_uv with u_f = mfx_lon = u_basis * scale[n],
+ * v_f = mfy_lat = v_basis * scale[n], m = 1 (the structure of
+ * deform_case1_test_set_uv, deform_test_mod.F90:129-167), followed by
+ * adv_run(time_step0 + n).  Device resident; one synchronisation at the end,
+ * where the device guards (|cflx| > lon_halo_width, |cfly| > 1, a negative
+ * tracer the fixer cannot fill) are checked. */
+int gmc_adv_set_wind_basis(gmc_adv *adv, const double *u_basis, const double *v_basis, int64_t ld);
+int gmc_adv_drive(gmc_adv *adv, int64_t time_step0, const double *scale, int32_t nsteps);
+
+/* Index (1 or 2) of the old time level after the swaps. */
+int gmc_adv_old_level(const gmc_adv *adv, int32_t *old_level);
+
+/* Tracer mass sum(q * area_cell) of every tracer on `level` (the driver's
+ * diagnose, gmcore_adv_driver.F90:103-126, per tracer); out[ntracers]. */
+int gmc_adv_mass(gmc_adv *adv, int32_t level, double *out);
+
+/* Number of cells the horizontal positivity fixer (adv_mod.F90:236-258) has
+ * filled since creation. */
+int gmc_adv_fix_count(gmc_adv *adv, int64_t *count);
+
+/* Wait for the object's stream; its HIP stream handle. */
+int gmc_adv_synchronize(gmc_adv *adv);
+int gmc_adv_stream(gmc_adv *adv, void **stream);
+
+/* Per-kernel timing: runs gmc_adv_drive with HIP events around every launch;
+ * out_ms[k] = mean duration (ms) of kernel class k, gmc_adv_kernel_name(k). */
+int gmc_adv_kernel_times(gmc_adv *adv, int64_t time_step0, const double *scale, int32_t nsteps,
+                         double *out_ms, int32_t cap, int32_t *n);
+const char *gmc_adv_kernel_name(int32_t k);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GMCORE_ADV_H */

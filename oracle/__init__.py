@@ -4,7 +4,7 @@ TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py may import this package, and only to check or
 time the HIP library against it.  No product code path uses it.
 
-The C sources (mesh.c, filter.c, sw.c) restate the reference Fortran
+The C sources (mesh.c, filter.c, sw.c, adv.c) restate the reference Fortran
 (each function cites its file:line); ``make -C oracle`` builds liboracle.so.
 """
 from __future__ import annotations
@@ -14,8 +14,8 @@ import os
 
 import numpy as np
 
-from gmcore_amd.capi import (GMC_MESH_NFIELDS, FIELD, GmcConfig, GmcMesh, dptr, field_rows,
-                             iptr)
+from gmcore_amd.capi import (GMC_MESH_NFIELDS, FIELD, GmcAdvConfig, GmcConfig, GmcMesh, dptr,
+                             field_rows, iptr)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
@@ -51,6 +51,21 @@ def lib() -> C.CDLL:
         L.gmo_sw_diagnose.argtypes = [vp, C.c_int, P(C.c_double)]
         L.gmo_zonal_sum.restype = C.c_double
         L.gmo_zonal_sum.argtypes = [P(C.c_double), C.c_int, C.c_int]
+        L.gmo_lon_damp_weights.argtypes = [C.c_int, P(C.c_double), P(C.c_double)]
+        L.gmo_adv_create.restype = vp
+        L.gmo_adv_create.argtypes = [P(GmcMesh), P(GmcAdvConfig)]
+        L.gmo_adv_destroy.argtypes = [vp]
+        for fn in ("gmo_adv_set_tracer", "gmo_adv_get_tracer"):
+            getattr(L, fn).argtypes = [vp, C.c_int, C.c_int, P(C.c_double), C.c_int64]
+        L.gmo_adv_set_wind.argtypes = [vp] + [P(C.c_double)] * 5 + [C.c_int64]
+        L.gmo_adv_run.argtypes = [vp, C.c_int64]
+        L.gmo_adv_old_level.argtypes = [vp]
+        L.gmo_adv_fix_count.restype = C.c_int64
+        L.gmo_adv_fix_count.argtypes = [vp]
+        L.gmo_adv_mass.restype = C.c_double
+        L.gmo_adv_mass.argtypes = [vp, C.c_int, C.c_int]
+        L.gmo_adv_get_internal.argtypes = [vp, C.c_int, P(C.c_double), C.c_int64]
+        L.gmo_ppm.argtypes = [C.c_int, P(C.c_double), P(C.c_double)]
         _lib = L
     return _lib
 
@@ -154,3 +169,87 @@ class SWOracle:
         out = np.zeros(4)
         lib().gmo_sw_diagnose(self.h, self.level if level is None else level, dptr(out))
         return out
+
+
+def lon_damp_weights(nlat: int, metrics: np.ndarray) -> np.ndarray:
+    """wgt_full_lat of lon_damp_init (lon_damp_mod.F90:43-68)."""
+    w = np.zeros(nlat)
+    lib().gmo_lon_damp_weights(nlat, dptr(np.ascontiguousarray(metrics)), dptr(w))
+    return w
+
+
+def ppm(f, limiter: int = 0):
+    """ppm() of ffsl_mod.F90:1147-1173 on five cell values: (fl, df, f6)."""
+    x = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.zeros(3)
+    lib().gmo_ppm(limiter, dptr(x), dptr(out))
+    return tuple(out)
+
+
+class AdvOracle:
+    """One cell-centred FFSL advection batch on the CPU, driven like adv_run."""
+
+    INTERNAL = ["u", "v", "mfx", "mfy", "cflx", "cfly", "divx", "divy", "qx", "qy", "qmf_lon",
+                "qmf_lat", "old_m"]
+    HALF = {"v", "mfy", "cfly", "qmf_lat"}
+
+    def __init__(self, mesh_struct: GmcMesh, cfg: GmcAdvConfig):
+        self._keep = (mesh_struct, cfg)
+        self.nlon, self.nlat = mesh_struct.nlon, mesh_struct.nlat
+        self.ntracers = cfg.ntracers
+        self.h = lib().gmo_adv_create(C.byref(mesh_struct), C.byref(cfg))
+        if not self.h:
+            raise ValueError("gmo_adv_create failed")
+
+    def close(self):
+        if self.h:
+            lib().gmo_adv_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def set_tracer(self, level: int, tracer: int, q: np.ndarray):
+        a = np.ascontiguousarray(q, dtype=np.float64)
+        assert a.shape == (self.nlat, self.nlon)
+        assert lib().gmo_adv_set_tracer(self.h, level, tracer, dptr(a), self.nlon) == 0
+
+    def tracer(self, level: int, tracer: int) -> np.ndarray:
+        a = np.zeros((self.nlat, self.nlon))
+        assert lib().gmo_adv_get_tracer(self.h, level, tracer, dptr(a), self.nlon) == 0
+        return a
+
+    def set_wind(self, u=None, v=None, mfx=None, mfy=None, m=None):
+        keep = []
+
+        def p(a, rows):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            assert a.shape == (rows, self.nlon)
+            keep.append(a)
+            return dptr(a)
+
+        n, h = self.nlat, self.nlat - 1
+        rc = lib().gmo_adv_set_wind(self.h, p(u, n), p(v, h), p(mfx, n), p(mfy, h), p(m, n),
+                                    self.nlon)
+        assert rc == 0
+
+    def run(self, time_step: int) -> int:
+        return lib().gmo_adv_run(self.h, time_step)
+
+    @property
+    def old(self) -> int:
+        return lib().gmo_adv_old_level(self.h)
+
+    @property
+    def fixes(self) -> int:
+        return lib().gmo_adv_fix_count(self.h)
+
+    def mass(self, level=None):
+        lv = self.old if level is None else level
+        return np.array([lib().gmo_adv_mass(self.h, lv, l) for l in range(self.ntracers)])
+
+    def internal(self, name: str) -> np.ndarray:
+        a = np.zeros((self.nlat - 1 if name in self.HALF else self.nlat, self.nlon))
+        assert lib().gmo_adv_get_internal(self.h, self.INTERNAL.index(name), dptr(a), self.nlon) == 0
+        return a

@@ -18,6 +18,7 @@
 
 #include <stdint.h>
 #include "../include/gmcore_hip.h"
+#include "../include/gmcore_adv.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -74,6 +75,30 @@ int gmo_sw_diagnose(gmo_sw *s, int level, double out[4]);
  * pairwise tree over p.  The reference's zonal_sum (parallel_mod.F90:741-764)
  * leaves the order to the compiler and MPI. */
 double gmo_zonal_sum(const double *x, int n, int stride);
+
+/* ---- FFSL tracer advection (adv.c) ------------------------------------- */
+
+/* lon_damp_init (src/dynamics/damp/lon_damp_mod.F90:43-68): wgt[nlat] */
+int gmo_lon_damp_weights(int nlat, const double *metrics, double *wgt);
+
+/* One cell-centred advection batch (adv_batch_mod.F90:19-69) */
+typedef struct gmo_adv gmo_adv;
+gmo_adv *gmo_adv_create(const gmc_mesh *mesh, const gmc_adv_config *cfg);
+void gmo_adv_destroy(gmo_adv *s);
+int gmo_adv_set_tracer(gmo_adv *s, int level, int tracer, const double *host, int64_t ld);
+int gmo_adv_get_tracer(gmo_adv *s, int level, int tracer, double *host, int64_t ld);
+int gmo_adv_set_wind(gmo_adv *s, const double *u, const double *v, const double *mfx,
+                     const double *mfy, const double *m, int64_t ld);
+/* adv_run (adv_mod.F90:152-312) at model step time_step */
+int gmo_adv_run(gmo_adv *s, int64_t time_step);
+int gmo_adv_old_level(const gmo_adv *s);
+int64_t gmo_adv_fix_count(const gmo_adv *s);
+double gmo_adv_mass(gmo_adv *s, int level, int tracer);
+/* interior of a batch array: 0 u, 1 v, 2 mfx, 3 mfy, 4 cflx, 5 cfly, 6 divx,
+ * 7 divy, 8 qx, 9 qy, 10 qmf_lon, 11 qmf_lat, 12 old_m */
+int gmo_adv_get_internal(gmo_adv *s, int index, double *host, int64_t ld);
+/* ppm (ffsl_mod.F90:1147-1173) of f[0..4]: out = fl, df, f6 */
+void gmo_ppm(int limiter, const double f[5], double out[3]);
 
 #ifdef __cplusplus
 }

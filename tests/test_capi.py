@@ -4,6 +4,8 @@ import ctypes as C
 import os
 import re
 
+import pytest
+
 from gmcore_amd import capi
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -15,23 +17,33 @@ def declared(header):
     return sorted(set(re.findall(r"\b(gmc_[a-z_0-9]+)\s*\(", text)))
 
 
-def test_hip_library_exports_header():
-    names = declared("gmcore_hip.h")
+HEADERS = ["gmcore_hip.h", "gmcore_adv.h"]
+
+
+def test_every_header_is_checked():
+    assert sorted(HEADERS) == sorted(f for f in os.listdir(os.path.join(ROOT, "include"))
+                                     if f.endswith(".h"))
+
+
+@pytest.mark.parametrize("header", HEADERS)
+def test_hip_library_exports_header(header):
+    names = declared(header)
     assert len(names) >= 15
     lib = C.CDLL(capi.LIB_HIP)
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
 
 
-def test_python_mirror_covers_header():
+@pytest.mark.parametrize("header", HEADERS)
+def test_python_mirror_covers_header(header):
     lib = capi.load_hip()
-    for n in declared("gmcore_hip.h"):
+    for n in declared(header):
         assert getattr(lib, n).argtypes is not None, f"{n} has no ctypes signature in capi.py"
 
 
 def test_host_library_exports():
     lib = C.CDLL(capi.LIB_HOST)
-    for n in ("gmch_mesh_metrics", "gmch_filter"):
+    for n in ("gmch_mesh_metrics", "gmch_filter", "gmch_lon_damp_weights"):
         assert hasattr(lib, n)
 
 
@@ -41,3 +53,6 @@ def test_struct_layouts_match_header():
     assert C.sizeof(capi.GmcFilter) == 40
     assert capi.GmcConfig.big_filter.offset % 8 == 0
     assert capi.GMC_MESH_NFIELDS == 32
+    # gmc_adv_config: 2 doubles, 6 int32, 1 pointer
+    assert C.sizeof(capi.GmcAdvConfig) == 48
+    assert capi.GmcAdvConfig.lon_damp_wgt.offset == 40
