@@ -44,13 +44,25 @@ KERNEL_TRAFFIC_PER_GP = {"k_tend": (2 * 19959.5 + 24314.3) * 1024 / (1440 * 721)
                          "k_uvf": (2 * 29576.1 + 32400.0) * 1024 / (1440 * 721)}
 
 # FFSL tracer advection, 0.5 degrees, 8 tracers, dt = 900 s, one day
-ADV_CONFIGS = {
+ADV_# batched ensemble: 64 members of uneven resolution, each with its stable dt
+ENS_CONFIGS = {
+    "ens64": dict(members=[(180, 91, 600.0, 32), (360, 181, 300.0, 24), (720, 361, 150.0, 8)],
+                  lmax=20, perturb=0.01, steps=100),
+}
+
+CONFIGS = {
     "ffsl05": dict(nlon=720, nlat=361, dt=900.0, ntracers=8, lmax=10, wind_seed=2, tracer_seed=3,
                    vmax=50.0, hw=104, steps=96),
 }
 # compulsory bytes per tracer-grid-point of an advection step: every tracer
 # read and written once, the 4 wind fields (u, v, mfx, mfy) shared
 ADV_BYTES_PER_TRACER_GP = lambda ntr: (2 * ntr + 4) * 8.0 / ntr
+
+# batched ensemble: 64 members of uneven resolution, each with its stable dt
+ENS_CONFIGS = {
+    "ens64": dict(members=[(180, 91, 600.0, 32), (360, 181, 300.0, 24), (720, 361, 150.0, 8)],
+                  lmax=20, perturb=0.01, steps=100),
+}
 
 CONFIGS = {
     "sw025": dict(nlon=1440, nlat=721, dt=60.0, lmax=60, seed=1),
@@ -210,12 +222,86 @@ def run_adv(args):
     print(json.dumps(result))
 
 
+def run_ens(args):
+    """Batched ensemble: every member advances `steps` steps; value =
+    member-steps/s of the ensemble (one graph per step for all members);
+    baseline = the same members run one at a time through gmc_step.  The
+    two runs start from the same states and must end bit-identical."""
+    from gmcore_amd import init_cond
+    from gmcore_amd.host import Ensemble, Setup, ShallowWater
+    from gmcore_amd.namelist import Namelist
+    cfg = ENS_CONFIGS[args.config]
+    steps = args.steps if args.steps_given else cfg["steps"]
+    t = time.perf_counter()
+    ens_m, plain = [], []
+    k = 0
+    for nlon, nlat, dt, count in cfg["members"]:
+        st = Setup(Namelist(num_lon=nlon, num_lat=nlat, dt_dyn=dt, time_scheme="pc2"))
+        for _ in range(count):
+            ic = init_cond.seeded_state(nlon, nlat, st.mesh.metrics, cfg["lmax"], 0,
+                                        perturb=(cfg["perturb"], 100 + k), omega=st.nl.omega,
+                                        radius=st.nl.radius)
+            for lst in (ens_m, plain):
+                m = ShallowWater(st)
+                m.set_state(*ic)
+                m.prepare(1)
+                lst.append(m)
+            k += 1
+    log(f"ensemble set-up ({k} members x 2) {time.perf_counter() - t:.1f} s")
+    ens = Ensemble(ens_m)
+    choice, times = ens.autotune(trial_steps=3)
+    log(f"autotune: shape {choice} ({Ensemble.shape_name(choice)}); ms/step "
+        + ", ".join(f"{x:.3f}" for x in times))
+    # warm-up: one step of every member on both paths (graphs, tables)
+    ens.step(1)
+    for m in plain:
+        m.step(1)
+        m.synchronize()
+    w0 = time.perf_counter()
+    ens.step(steps)
+    t_ens = time.perf_counter() - w0
+    w0 = time.perf_counter()
+    for m in plain:
+        m.step(steps)
+        m.synchronize()
+    t_plain = time.perf_counter() - w0
+    same = True
+    for a, b in zip(ens_m, plain):
+        sa, sb = a.state(), b.state()
+        same = same and all(np.array_equal(sa[f], sb[f]) for f in ("u_lon", "v_lat", "gz"))
+    n = len(ens_m)
+    gp = sum(cnt * nlon * nlat for nlon, nlat, _, cnt in cfg["members"])
+    v_ens, v_plain = n * steps / t_ens, n * steps / t_plain
+    result = {
+        "metric": "member-steps/s (batched FP64 SW ensemble, 1 GPU)",
+        "value": v_ens, "unit": "member-steps/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+        "ms_per_step": t_ens * 1e3 / steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded balanced states, member k perturbed 1 % rms with seed 100 + k)",
+        "config": {"workload": f"ensemble {args.config}: " + ", ".join(
+            f"{cnt} x {nlon}x{nlat} (dt {dt:g} s)" for nlon, nlat, dt, cnt in cfg["members"]),
+            "members": n, "grid_points": gp, "time_scheme": "pc2"},
+        "plain_member_steps_per_s": v_plain,
+        "speedup_vs_plain": v_ens / v_plain,
+        "identical_to_plain": bool(same),
+        "shape": Ensemble.shape_name(choice),
+        "grid_point_steps_per_s": gp * steps / t_ens,
+        "roofline": {"bound": "hbm", "achieved": BYTES_PER_GP_STEP * gp * steps / t_ens / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": BYTES_PER_GP_STEP * gp * steps / t_ens / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "scope": "whole ensemble step, 280 B per member grid point"},
+    }
+    ens.close()
+    print(json.dumps(result))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="sw025", choices=sorted(CONFIGS) + sorted(ADV_CONFIGS))
+    ap.add_argument("--config", default="sw025",
+                    choices=sorted(CONFIGS) + sorted(ADV_CONFIGS) + sorted(ENS_CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -237,8 +323,12 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU latitude-band decomposition is a design, not code)", file=sys.stderr)
         sys.exit(2)
+    args.steps_given = "--steps" in sys.argv
     if args.config in ADV_CONFIGS:
         run_adv(args)
+        return
+    if args.config in ENS_CONFIGS:
+        run_ens(args)
         return
     cfg = CONFIGS[args.config]
     from gmcore_amd.host import ShallowWater

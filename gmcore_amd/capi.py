@@ -1,5 +1,5 @@
-"""ctypes mirror of include/gmcore_hip.h and include/gmcore_adv.h (structs,
-enums, signatures).
+"""ctypes mirror of include/gmcore_hip.h, include/gmcore_adv.h,
+include/gmcore_ens.h and include/gmcore_tlad.h (structs, enums, signatures).
 
 The C ABI is the drop-in boundary of the dynamics path; this module only
 describes it.  ``load_hip()`` opens the in-tree ``libgmcore_hip.so`` and fails
@@ -141,6 +141,29 @@ def load_hip() -> C.CDLL:
         "gmc_adv_kernel_times": (C.c_int, [vp, C.c_int64, P(C.c_double), C.c_int32, P(C.c_double),
                                            C.c_int32, P(C.c_int32)]),
         "gmc_adv_kernel_name": (C.c_char_p, [C.c_int32]),
+        # batched ensembles, include/gmcore_ens.h
+        "gmc_ens_create": (C.c_int, [P(vp), C.c_int32, P(vp)]),
+        "gmc_ens_destroy": (C.c_int, [vp]),
+        "gmc_ens_last_error": (C.c_char_p, [vp]),
+        "gmc_ens_step": (C.c_int, [vp, P(C.c_double), C.c_int32]),
+        "gmc_ens_autotune": (C.c_int, [vp, P(C.c_double), C.c_int32, P(C.c_int32), P(C.c_double),
+                                       C.c_int32, P(C.c_int32)]),
+        "gmc_ens_shape": (C.c_int, [vp, C.c_int32]),
+        "gmc_ens_shape_name": (C.c_char_p, [C.c_int32]),
+        "gmc_ens_stream": (C.c_int, [vp, P(vp)]),
+        # tangent-linear and adjoint, include/gmcore_tlad.h
+        "gmc_tlad_create": (C.c_int, [vp, P(vp)]),
+        "gmc_tlad_destroy": (C.c_int, [vp]),
+        "gmc_tlad_last_error": (C.c_char_p, [vp]),
+        "gmc_tlad_upload": (C.c_int, [vp, C.c_int32, C.c_int32, P(C.c_double), C.c_int64]),
+        "gmc_tlad_download": (C.c_int, [vp, C.c_int32, C.c_int32, P(C.c_double), C.c_int64]),
+        "gmc_tlad_zero": (C.c_int, [vp]),
+        "gmc_tl_step": (C.c_int, [vp, C.c_double, C.c_int32]),
+        "gmc_ad_step": (C.c_int, [vp, C.c_double, C.c_int32]),
+        "gmc_gradient": (C.c_int, [vp, C.c_double, C.c_int32, C.c_int32, C.c_int32, P(C.c_int64),
+                                   P(C.c_double), C.c_double, P(C.c_double), P(C.c_double),
+                                   P(C.c_double), C.c_double, C.c_int64, P(C.c_double), P(C.c_double),
+                                   P(C.c_double), P(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -476,6 +476,7 @@ struct gmc_adv {
   hipStream_t stream = nullptr;
   int uv_step = 0, mf_step = 0, old = 1, new_ = 2;
   bool have_basis = false;
+  bool old_m_current = false;  // old_m already equals m (copy_old_m can be skipped)
   size_t lds_bytes = 0;
   hipEvent_t *prof_ev = nullptr;
   int prof_n = 0, prof_cap = 0;
@@ -543,9 +544,10 @@ int run_once(gmc_adv *a, int64_t time_step, bool basis, double scale) {
     std::swap(a->old, a->new_);
   }
   int rc = ahip(a, hipGetLastError(), "advection launch");
-  if (!rc)
+  if (!rc && !a->old_m_current)
     rc = ahip(a, hipMemcpyAsync(a->old_m, a->sm, (size_t)g.nlat * g.pitch * sizeof(double),
                                 hipMemcpyDeviceToDevice, a->stream), "copy_old_m");
+  if (!rc) a->old_m_current = true;
   return rc;
 }
 
@@ -673,7 +675,10 @@ int gmc_adv_set_wind(gmc_adv *a, const double *u, const double *v, const double 
   if (!rc && v) rc = copy_rows(a, a->sv, v, a->g.nhalf, ld);
   if (!rc && mfx) rc = copy_rows(a, a->smfx, mfx, a->g.nlat, ld);
   if (!rc && mfy) rc = copy_rows(a, a->smfy, mfy, a->g.nhalf, ld);
-  if (!rc && m) rc = copy_rows(a, a->sm, m, a->g.nlat, ld);
+  if (!rc && m) {
+    rc = copy_rows(a, a->sm, m, a->g.nlat, ld);
+    a->old_m_current = false;
+  }
   if (!rc) rc = ahip(a, hipStreamSynchronize(a->stream), "sync");
   return rc;
 }
@@ -697,6 +702,7 @@ int gmc_adv_set_wind_basis(gmc_adv *a, const double *ub, const double *vb, int64
   if (!rc) {
     std::vector<double> one((size_t)g.nlat * g.pitch, 1.0);
     rc = ahip(a, hipMemcpyAsync(a->sm, one.data(), one.size() * sizeof(double), hipMemcpyHostToDevice, a->stream), "m");
+    a->old_m_current = false;
     if (!rc) rc = ahip(a, hipStreamSynchronize(a->stream), "sync");
   }
   if (!rc) a->have_basis = true;

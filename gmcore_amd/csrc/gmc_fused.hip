@@ -20,6 +20,7 @@
 
 #include "gmc_internal.hpp"
 #include "gmc_point.hpp"
+#include "../../include/gmcore_ens.h"
 
 namespace gmc {
 
@@ -29,15 +30,21 @@ namespace {
 constexpr int LC = kWave;               // 64 tile columns
 constexpr int HXL = 3;                  // column halo
 constexpr int TX = LC - 2 * HXL - 0;    // 58 output columns (q = 3 .. 60)
-constexpr int TY = 11;                  // output rows
-constexpr int LR = TY + 5;              // 16 tile rows: r = R0-2 .. R0+TY+2
-constexpr int NT = LR * kWave;          // 1024 threads, one wavefront per tile row
+constexpr int TY_MAIN = 11;             // output rows of the single-member k_tend tile
 constexpr int NF = 256;                 // threads per workgroup of the filter kernels
-constexpr int NE = LR * LC;
 constexpr int NXCD = 8;                 // XCDs of an MI355X
-constexpr int SH_TILES = 7 * NE;        // LDS field tiles of k_tend
-constexpr int SH_MESH = LR * GMC_MESH_NFIELDS;  // metric rows of the tile
-constexpr int kMaxPoleLon = SH_TILES / 3;  // the pole sums keep 3 row terms in the tile area
+// A k_tend tile with TY output rows: LR = TY + 5 tile rows (r = R0-2 ..
+// R0+TY+2), one wavefront per tile row.
+template <int TY_> struct Tile {
+  static constexpr int TY = TY_;
+  static constexpr int LR = TY + 5;
+  static constexpr int NT = LR * kWave;
+  static constexpr int NE = LR * LC;
+  static constexpr int SH_TILES = 7 * NE;                 // LDS field tiles
+  static constexpr int SH_MESH = LR * GMC_MESH_NFIELDS;   // metric rows of the tile
+  static constexpr int SH_ALL = SH_TILES + SH_MESH + 8;
+  static constexpr int kMaxPoleLon = SH_TILES / 3;        // the pole sums keep 3 row terms in the tile area
+};
 
 struct StagePtrs {
   const double *gz_star, *gzs, *u_f, *v_f, *gz_old;
@@ -63,6 +70,7 @@ __device__ double pv_global(const Grid &g, const Mesh &M, double grav, int stoke
 // terms go to LDS scratch, one wavefront adds them in the fixed order of
 // wave_zonal_sum.  pole[north] = ke, pole[2+north] = dmfdlat at the pole
 // cell, pole[4+north] = vor of the vertex row next to the pole.
+template <int NT>
 __device__ void pole_sums(const Grid &g, const Mesh &M, double grav, const StagePtrs &P, int north,
                           double *scratch, double *pole) {
   const int tid = threadIdx.x, n = g.nlon;
@@ -110,38 +118,23 @@ struct MeshTile {
   __device__ __forceinline__ double operator()(int f, int r) const { return s[(r - r0) * GMC_MESH_NFIELDS + f]; }
 };
 
-// Forward tendencies and the unfiltered gz update on a 58 x 11 tile.
-// Wavefront w owns tile row k = w (global row R0 - 2 + k) in every phase and
-// lane q the column C0 - 3 + q, so metric terms are wave-uniform (broadcast
-// LDS reads of the tile's metric rows) and each phase is one pass over a row.
-// Blocks are dealt to the XCDs round-robin; each XCD gets a contiguous run of
-// tiles (row-major), so vertically adjacent tiles share that XCD's L2.  The
-// XCD holding the north pole walks its run backwards so that the polar tiles,
-// which also form the pole sums, start first.
-__global__ void __launch_bounds__(NT, 8) k_tend(Grid g, Mesh Mg, const double *__restrict__ mrows, double grav,
-                                                double dt, int stokes, int pv_scheme, int order, double wgt,
-                                                int coriolis, StagePtrs P) {
-  __shared__ double sh[SH_TILES + SH_MESH + 8];
-  double(*sm)[LC] = (double(*)[LC])(sh + 0 * NE);
-  double(*suf)[LC] = (double(*)[LC])(sh + 1 * NE);
-  double(*svf)[LC] = (double(*)[LC])(sh + 2 * NE);
-  double(*smfx)[LC] = (double(*)[LC])(sh + 3 * NE);
-  double(*smfy)[LC] = (double(*)[LC])(sh + 4 * NE);
-  double(*spv)[LC] = (double(*)[LC])(sh + 5 * NE);
-  double(*ske)[LC] = (double(*)[LC])(sh + 6 * NE);
+// One tile of the forward tendencies and the unfiltered gz update: output
+// columns C0 .. C0+TX-1, rows R0 .. R0+TY-1, LDS scratch sh[S::SH_ALL].
+template <class S>
+__device__ __forceinline__ void tend_tile(const Grid &g, const Mesh &Mg, const double *__restrict__ mrows, double grav,
+                                          double dt, int stokes, int pv_scheme, int order, double wgt, int coriolis,
+                                          const StagePtrs &P, int C0, int R0, double *sh) {
+  double(*sm)[LC] = (double(*)[LC])(sh + 0 * S::NE);
+  double(*suf)[LC] = (double(*)[LC])(sh + 1 * S::NE);
+  double(*svf)[LC] = (double(*)[LC])(sh + 2 * S::NE);
+  double(*smfx)[LC] = (double(*)[LC])(sh + 3 * S::NE);
+  double(*smfy)[LC] = (double(*)[LC])(sh + 4 * S::NE);
+  double(*spv)[LC] = (double(*)[LC])(sh + 5 * S::NE);
+  double(*ske)[LC] = (double(*)[LC])(sh + 6 * S::NE);
   double(*spl)[LC] = suf;  // pv_lon replaces u_f element by element (phase D)
   double(*spt)[LC] = svf;  // pv_lat replaces v_f element by element (phase D)
-  double *sme = sh + SH_TILES;
-  double *pole = sh + SH_TILES + SH_MESH;
-
-  // tile of this block
-  const int nx = (g.nlon + TX - 1) / TX, ny = (g.nlat + TY - 1) / TY, T = nx * ny;
-  const int b = blockIdx.x, x = b % NXCD, s = b / NXCD;
-  int start = 0;
-  for (int y = 0; y < x; y++) start += (T - y + NXCD - 1) / NXCD;
-  const int cnt = (T - x + NXCD - 1) / NXCD;
-  const int t = start + (x == NXCD - 1 ? cnt - 1 - s : s);
-  const int C0 = (t % nx) * TX, R0 = (t / nx) * TY;
+  double *sme = sh + S::SH_TILES;
+  double *pole = sh + S::SH_TILES + S::SH_MESH;
 
   const int64_t p = g.pitch;
   const int nlat = g.nlat, nhalf = g.nhalf;
@@ -151,8 +144,8 @@ __global__ void __launch_bounds__(NT, 8) k_tend(Grid g, Mesh Mg, const double *_
   const int i = wrap(C0 - HXL + q, g.nlon);                            // global column
 
   // 0: pole sums in the tiles that read them (uniform branch per block)
-  if (R0 <= 2) pole_sums(g, Mg, grav, P, 0, sh, pole);
-  if (R0 + TY + 3 >= nlat - 1) pole_sums(g, Mg, grav, P, 1, sh, pole);
+  if (R0 <= 2) pole_sums<S::NT>(g, Mg, grav, P, 0, sh, pole);
+  if (R0 + S::TY + 3 >= nlat - 1) pole_sums<S::NT>(g, Mg, grav, P, 1, sh, pole);
 
   // A: m, u_f, v_f of tile row k and the tile's metric rows
   {
@@ -163,24 +156,24 @@ __global__ void __launch_bounds__(NT, 8) k_tend(Grid g, Mesh Mg, const double *_
       a2 = P.u_f[r * p + i];
     }
     if (r >= 0 && r < nhalf) a3 = P.v_f[r * p + i];
-    if (threadIdx.x < SH_MESH) {
+    if (threadIdx.x < S::SH_MESH) {
       const int rr = R0 - 2 + (int)threadIdx.x / GMC_MESH_NFIELDS;
       if (rr >= 0 && rr < nlat) a4 = mrows[(int64_t)(rr + 1) * GMC_MESH_NFIELDS + threadIdx.x % GMC_MESH_NFIELDS];
     }
     sm[k][q] = (r >= 0 && r < nlat) ? m_of(a0, a1, grav) : 0.0;
     suf[k][q] = a2;
     svf[k][q] = a3;
-    if (threadIdx.x < SH_MESH) sme[threadIdx.x] = a4;
+    if (threadIdx.x < S::SH_MESH) sme[threadIdx.x] = a4;
   }
   const MeshTile M{sme, R0 - 2};
   __syncthreads();
 
   // B: mass fluxes (rows 1 .. TY+2), pv (rows 0 .. TY+3), ke (rows 2 .. TY+2)
-  if (k >= 1 && k < TY + 3 && q >= 2 && q < 63) {
+  if (k >= 1 && k < S::TY + 3 && q >= 2 && q < 63) {
     smfx[k][q] = (r >= 1 && r <= nlat - 2) ? (sm[k][q] + sm[k][q + 1]) * 0.5 * suf[k][q] : 0.0;
     smfy[k][q] = (r >= 0 && r < nhalf) ? (sm[k][q] + sm[k + 1][q]) * 0.5 * svf[k][q] : 0.0;
   }
-  if (k < TY + 4 && q >= 1 && q < 63) {
+  if (k < S::TY + 4 && q >= 1 && q < 63) {
     double v = 0.0;
     if (r >= 0 && r < nhalf) {
       double vor;
@@ -195,7 +188,7 @@ __global__ void __launch_bounds__(NT, 8) k_tend(Grid g, Mesh Mg, const double *_
     }
     spv[k][q] = v;
   }
-  if (k >= 2 && k < TY + 3 && q >= 3 && q < 62) {
+  if (k >= 2 && k < S::TY + 3 && q >= 3 && q < 62) {
     double v = 0.0;
     if (r == 0) v = pole[0];
     else if (r == nlat - 1) v = pole[1];
@@ -209,14 +202,14 @@ __global__ void __launch_bounds__(NT, 8) k_tend(Grid g, Mesh Mg, const double *_
   // so they stay in registers; pv_lon replaces u_f and pv_lat replaces v_f
   // element by element (each lane reads only its own u_f / v_f here).
   double pl = 0.0, pt = 0.0;
-  if (k >= 2 && k < TY + 3 && q >= 2 && q < 62 && r >= 1 && r <= nlat - 2) {
+  if (k >= 2 && k < S::TY + 3 && q >= 2 && q < 62 && r >= 1 && r <= nlat - 2) {
     const double tt = tangent_sum(M(GMC_FULL_TANGENT_WGT1, r), M(GMC_FULL_TANGENT_WGT2, r), smfy[k - 1][q],
                                   smfy[k - 1][q + 1], smfy[k][q], smfy[k][q + 1]);
     const double vl = tt / ((sm[k][q] + sm[k][q + 1]) * 0.5);
     const double a1 = spv[k - 2][q], a2 = spv[k - 1][q], a3 = spv[k][q], a4 = spv[k + 1][q];
     pl = pv_scheme == GMC_PV_MIDPOINT ? 0.5 * (a3 + a2) : pv_upwind_edge(order, wgt, vl, suf[k][q], 0, a1, a2, a3, a4);
   }
-  if (k >= 1 && k < TY + 2 && q >= 3 && q < 62 && r >= 0 && r < nhalf) {
+  if (k >= 1 && k < S::TY + 2 && q >= 3 && q < 62 && r >= 0 && r < nhalf) {
     const double tt = tangent_sum(M(GMC_HALF_TANGENT_WGT1, r), M(GMC_HALF_TANGENT_WGT2, r), smfx[k][q - 1],
                                   smfx[k][q], smfx[k + 1][q - 1], smfx[k + 1][q]);
     const double ul = tt / ((sm[k][q] + sm[k + 1][q]) * 0.5);
@@ -229,7 +222,7 @@ __global__ void __launch_bounds__(NT, 8) k_tend(Grid g, Mesh Mg, const double *_
 
   // E: du, dv (forward pass) and gz_new = gz_old + dt dgz on the output rows
   const int ic = C0 + q - HXL;
-  if (k >= 2 && k < TY + 2 && q >= HXL && q < HXL + TX && ic < g.nlon && r < nlat) {
+  if (k >= 2 && k < S::TY + 2 && q >= HXL && q < HXL + TX && ic < g.nlon && r < nlat) {
     const int64_t o = r * p + ic;
     const double gz_old = P.gz_old[o];
     if (r >= 1 && r <= nlat - 2) {
@@ -262,10 +255,36 @@ __global__ void __launch_bounds__(NT, 8) k_tend(Grid g, Mesh Mg, const double *_
   }
 }
 
+// Forward tendencies and the unfiltered gz update on a 58 x 11 tile.
+// Wavefront w owns tile row k = w (global row R0 - 2 + k) in every phase and
+// lane q the column C0 - 3 + q, so metric terms are wave-uniform (broadcast
+// LDS reads of the tile's metric rows) and each phase is one pass over a row.
+// Blocks are dealt to the XCDs round-robin; each XCD gets a contiguous run of
+// tiles (row-major), so vertically adjacent tiles share that XCD's L2.  The
+// XCD holding the north pole walks its run backwards so that the polar tiles,
+// which also form the pole sums, start first.
+__global__ void __launch_bounds__(Tile<TY_MAIN>::NT, 8) k_tend(Grid g, Mesh Mg, const double *__restrict__ mrows,
+                                                             double grav, double dt, int stokes, int pv_scheme,
+                                                             int order, double wgt, int coriolis, StagePtrs P) {
+  using S = Tile<TY_MAIN>;
+  __shared__ double sh[S::SH_ALL];
+  // tile of this block
+  const int nx = (g.nlon + TX - 1) / TX, ny = (g.nlat + S::TY - 1) / S::TY, T = nx * ny;
+  const int b = blockIdx.x, x = b % NXCD, s = b / NXCD;
+  int start = 0;
+  for (int y = 0; y < x; y++) start += (T - y + NXCD - 1) / NXCD;
+  const int cnt = (T - x + NXCD - 1) / NXCD;
+  const int t = start + (x == NXCD - 1 ? cnt - 1 - s : s);
+  const int C0 = (t % nx) * TX, R0 = (t / nx) * S::TY;
+
+  tend_tile<S>(g, Mg, mrows, grav, dt, stokes, pv_scheme, order, wgt, coriolis, P, C0, R0, sh);
+}
+
 // ---------------------------------------------------------------------------
 // zonal filters over (row, column chunk) work items
 struct Item {
   int32_t row, c0, c1, kind;  // kind: 0 gz (cell row), 1 u (U row), 2 v (V row)
+  int32_t member;             // ensemble member (0 for a single context)
 };
 
 // column i - h + t of a row, t in [0, nlon + 2h), h <= nlon
@@ -274,10 +293,8 @@ __device__ __forceinline__ int wrap2(int c, int n) { return c < 0 ? c + n : (c >
 // big filter gz_new -> gz_f and small filter gz_new -> gz
 // (time_schemes_mod.F90:184-191, filter_mod.F90:25-50).  Weights and the
 // row window are staged in LDS: xs = [weights big | weights small | window].
-__global__ void __launch_bounds__(NF) k_gzf(Grid g, FilterDev big, FilterDev small, const Item *__restrict__ items,
-                                            const double *__restrict__ gz_new, double *gz_f, double *gz) {
-  extern __shared__ double xs[];
-  const Item it = items[blockIdx.x];
+__device__ __forceinline__ void gzf_item(const Grid &g, const FilterDev &big, const FilterDev &small, const Item &it,
+                                         const double *__restrict__ gz_new, double *gz_f, double *gz, double *xs) {
   const int r = it.row;
   const int64_t o = r * g.pitch;
   const int nb = big.ngrid_lon[r], ns = small.ngrid_lon[r];
@@ -303,6 +320,12 @@ __global__ void __launch_bounds__(NF) k_gzf(Grid g, FilterDev big, FilterDev sma
   }
 }
 
+__global__ void __launch_bounds__(NF) k_gzf(Grid g, FilterDev big, FilterDev small, const Item *__restrict__ items,
+                                            const double *__restrict__ gz_new, double *gz_f, double *gz) {
+  extern __shared__ double xs[];
+  gzf_item(g, big, small, items[blockIdx.x], gz_new, gz_f, gz, xs);
+}
+
 struct UVPtrs {
   const double *u_old, *v_old, *du, *dv, *gz_f;
   double *u, *v, *u_f, *v_f;
@@ -311,10 +334,8 @@ struct UVPtrs {
 // backward pass + u/v update + big filters
 // (gmcore_mod.F90:552-571, time_schemes_mod.F90:198-239).  xs = [weights |
 // window of the updated row].
-__global__ void __launch_bounds__(NF) k_uvf(Grid g, Mesh M, double dt, FilterDev big, const Item *__restrict__ items,
-                                            UVPtrs P) {
-  extern __shared__ double xs[];
-  const Item it = items[blockIdx.x];
+__device__ __forceinline__ void uvf_item(const Grid &g, const Mesh &M, double dt, const FilterDev &big,
+                                         const Item &it, const UVPtrs &P, double *xs) {
   const int r = it.row;
   const int64_t p = g.pitch, o = r * p;
   const int nl = g.nlon;
@@ -378,6 +399,74 @@ __global__ void __launch_bounds__(NF) k_uvf(Grid g, Mesh M, double dt, FilterDev
   }
 }
 
+__global__ void __launch_bounds__(NF) k_uvf(Grid g, Mesh M, double dt, FilterDev big, const Item *__restrict__ items,
+                                            UVPtrs P) {
+  extern __shared__ double xs[];
+  uvf_item(g, M, dt, big, items[blockIdx.x], P, xs);
+}
+
+
+// ---------------------------------------------------------------------------
+// Batched ensembles: the same stage kernels over many contexts in one launch.
+// A member's descriptor for one stage holds its geometry, tables, options,
+// stage dt and field pointers; tiles and filter items of all members are
+// numbered consecutively and each workgroup finds its member by the tile
+// offset (k_tend) or the item's member index (filters).
+struct EnsMember {
+  Grid g;
+  Mesh M;
+  const double *mrows;
+  FilterDev big, small1;
+  double grav, dt, wgt;
+  int stokes, pv_scheme, order, coriolis;
+  StagePtrs sp;
+  UVPtrs up;
+  double *gz_f_out;    // gz_f of the new state (big filter output)
+  double *gz_out;      // gz of the new state (small filter output)
+  int tile0, nx;       // first tile of the member, tiles per tile row
+};
+
+// grid-stride over the tiles of all members: gridDim.x may be smaller than
+// the tile count (a persistent launch sized by the tuner)
+template <class S>
+__global__ void __launch_bounds__(S::NT, 8) k_tend_ens(const EnsMember *__restrict__ mem, int nmem, int total) {
+  __shared__ double sh[S::SH_ALL];
+  for (int t = blockIdx.x; t < total; t += gridDim.x) {
+    int lo = 0, hi = nmem - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (mem[mid].tile0 <= t) lo = mid; else hi = mid - 1;
+    }
+    const EnsMember &d = mem[lo];
+    const int lt = t - d.tile0;
+    tend_tile<S>(d.g, d.M, d.mrows, d.grav, d.dt, d.stokes, d.pv_scheme, d.order, d.wgt, d.coriolis, d.sp,
+                 (lt % d.nx) * TX, (lt / d.nx) * S::TY, sh);
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(NF) k_gzf_ens(const EnsMember *__restrict__ mem, const Item *__restrict__ items,
+                                                int nitems) {
+  extern __shared__ double xs[];
+  for (int b = blockIdx.x; b < nitems; b += gridDim.x) {
+    const Item it = items[b];
+    const EnsMember &d = mem[it.member];
+    gzf_item(d.g, d.big, d.small1, it, d.sp.gz_new, d.gz_f_out, d.gz_out, xs);
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(NF) k_uvf_ens(const EnsMember *__restrict__ mem, const Item *__restrict__ items,
+                                                int nitems) {
+  extern __shared__ double xs[];
+  for (int b = blockIdx.x; b < nitems; b += gridDim.x) {
+    const Item it = items[b];
+    const EnsMember &d = mem[it.member];
+    uvf_item(d.g, d.M, d.dt, d.big, it, d.up, xs);
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -391,7 +480,7 @@ static void add_row(std::vector<Item> &v, int row, int kind, int n, int nlon, si
   const int h = n > 1 ? (n - 1) / 2 : 0;
   for (int k = 0; k < parts; k++) {
     const int c0 = (int)((int64_t)nlon * k / parts), c1 = (int)((int64_t)nlon * (k + 1) / parts);
-    v.push_back(Item{row, c0, c1, kind});
+    v.push_back(Item{row, c0, c1, kind, 0});
     lds = std::max(lds, sizeof(double) * (size_t)(c1 - c0 + 2 * h));
   }
 }
@@ -416,7 +505,7 @@ static int build_plan(gmc_ctx *c) {
     if (nb_lon[r] > g.nlon) return set_error(c, GMC_EINVAL, "filter wider than a latitude circle");
   for (int r = 0; r < g.nhalf; r++)
     if (nb_lat[r] > g.nlon) return set_error(c, GMC_EINVAL, "filter wider than a latitude circle");
-  if (g.nlon > kMaxPoleLon) return set_error(c, GMC_EINVAL, "nlon too large for the fused stage kernels");
+  if (g.nlon > Tile<TY_MAIN>::kMaxPoleLon) return set_error(c, GMC_EINVAL, "nlon too large for the fused stage kernels");
   std::vector<Item> items;
   FusedPlan *P = new FusedPlan();
 
@@ -467,8 +556,8 @@ static void fused_stage(gmc_ctx *c, int o, int s, int n, double dt) {
   double *gz_new = c->tend[o][T_DGZ];   // the unfiltered gz of the new state
   prof_mark(c);
   StagePtrs sp{S[S_GZ], c->gzs, S[S_U_F], S[S_V_F], O[S_GZ], gz_new, du, dv};
-  const int tiles = ((g.nlon + TX - 1) / TX) * ((g.nlat + TY - 1) / TY);
-  hipLaunchKernelGGL(k_tend, dim3(tiles), dim3(NT), 0, c->stream, g, M, c->d_mrows, c->cfg.g, dt, c->cfg.pv_pole_stokes,
+  const int tiles = ((g.nlon + TX - 1) / TX) * ((g.nlat + TY_MAIN - 1) / TY_MAIN);
+  hipLaunchKernelGGL(k_tend, dim3(tiles), dim3(Tile<TY_MAIN>::NT), 0, c->stream, g, M, c->d_mrows, c->cfg.g, dt, c->cfg.pv_pole_stokes,
                      c->cfg.pv_scheme, c->cfg.upwind_order_pv, c->cfg.upwind_wgt_pv, c->cfg.coriolis_scheme, sp);
   prof_mark(c);
   hipLaunchKernelGGL(k_gzf, dim3(PL->n_gz), dim3(NF), PL->lds_gz, c->stream, g, c->big, c->small1, PL->d_items,
@@ -528,3 +617,356 @@ int fused_step_graph(gmc_ctx *c, int o, int n, double dt) {
 }
 
 }  // namespace gmc
+
+// ===========================================================================
+// Batched ensembles (include/gmcore_ens.h)
+namespace gmc {
+namespace {
+
+// launch shapes the tuner chooses from: rows of the k_tend tile and the
+// workgroups of the persistent tendency launch (0: one per tile, else
+// mult x 256)
+struct EnsShape {
+  int ty, mult;
+  const char *name;
+};
+constexpr EnsShape kEnsShapes[] = {
+    {11, 0, "tile 58x11, one workgroup per tile"}, {7, 0, "tile 58x7, one workgroup per tile"},
+    {5, 0, "tile 58x5, one workgroup per tile"},   {11, 4, "tile 58x11, 1024 persistent workgroups"},
+    {7, 4, "tile 58x7, 1024 persistent workgroups"}, {5, 4, "tile 58x5, 1024 persistent workgroups"},
+    {11, 8, "tile 58x11, 2048 persistent workgroups"}, {7, 8, "tile 58x7, 2048 persistent workgroups"},
+    {5, 8, "tile 58x5, 2048 persistent workgroups"}};
+constexpr int kNumEnsShapes = sizeof(kEnsShapes) / sizeof(kEnsShapes[0]);
+
+int stage_count(int scheme) { return scheme == GMC_TIME_EULER ? 1 : 3; }
+
+// (old, star, new) level of each stage and its dt, as fused_integrate
+void stage_levels(int scheme, int o, int n, double dt, int st, int &so, int &ss, int &sn, double &sdt) {
+  so = o;
+  if (scheme == GMC_TIME_EULER) { ss = o; sn = n; sdt = dt; return; }
+  if (st == 0) { ss = o; sn = n; sdt = scheme == GMC_TIME_WRFRK3 ? dt / 3.0 : dt / 2.0; }
+  else if (st == 1) { ss = n; sn = 3; sdt = dt / 2.0; }
+  else { ss = 3; sn = n; sdt = dt; }
+}
+
+}  // namespace
+}  // namespace gmc
+
+struct gmc_ens {
+  std::vector<gmc_ctx *> m;
+  int scheme = 0, nstage = 3;
+  int shape = 0;
+  gmc::EnsMember *d_mem = nullptr;  // [2 parities][nstage][members]
+  gmc::Item *d_items = nullptr;
+  int n_gz = 0, n_uv = 0;
+  size_t lds_gz = 0, lds_uv = 0;
+  int total_tiles = 0;
+  std::vector<double> dt;           // dt the tables were built for
+  int built_shape = -1;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  hipStream_t stream = nullptr;
+  std::string err;
+};
+
+namespace gmc {
+namespace {
+
+int eerr(gmc_ens *e, int code, const std::string &msg) {
+  if (e) e->err = msg;
+  return code;
+}
+int ehip(gmc_ens *e, hipError_t x, const char *what) {
+  if (x == hipSuccess) return GMC_OK;
+  return eerr(e, x == hipErrorOutOfMemory ? GMC_ENOMEM : GMC_EDEVICE, std::string(what) + ": " + hipGetErrorString(x));
+}
+
+void drop_graphs(gmc_ens *e) {
+  for (auto &g : e->graph)
+    if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+}
+
+// descriptor tables for the current shape and dts
+int build_tables(gmc_ens *e) {
+  const int nm = (int)e->m.size(), ty = kEnsShapes[e->shape].ty;
+  std::vector<EnsMember> tab((size_t)2 * e->nstage * nm);
+  int tiles = 0;
+  std::vector<int> tile0(nm), nx(nm);
+  for (int k = 0; k < nm; k++) {
+    const Grid &g = e->m[k]->grid;
+    nx[k] = (g.nlon + TX - 1) / TX;
+    tile0[k] = tiles;
+    tiles += nx[k] * ((g.nlat + ty - 1) / ty);
+  }
+  e->total_tiles = tiles;
+  for (int par = 0; par < 2; par++)
+    for (int st = 0; st < e->nstage; st++)
+      for (int k = 0; k < nm; k++) {
+        gmc_ctx *c = e->m[k];
+        const int o = par == 0 ? 1 : 2, n = par == 0 ? 2 : 1;
+        int so, ss, sn;
+        double sdt;
+        stage_levels(e->scheme, o, n, e->dt[k], st, so, ss, sn, sdt);
+        double **O = c->state[so], **S = c->state[ss], **N = c->state[sn];
+        double *du = c->tend[so][T_DU], *dv = c->tend[so][T_DV], *gz_new = c->tend[so][T_DGZ];
+        EnsMember &d = tab[((size_t)par * e->nstage + st) * nm + k];
+        d.g = c->grid;
+        d.M = Mesh{c->d_mesh, c->grid.nlat};
+        d.mrows = c->d_mrows;
+        d.big = c->big;
+        d.small1 = c->small1;
+        d.grav = c->cfg.g;
+        d.dt = sdt;
+        d.wgt = c->cfg.upwind_wgt_pv;
+        d.stokes = c->cfg.pv_pole_stokes;
+        d.pv_scheme = c->cfg.pv_scheme;
+        d.order = c->cfg.upwind_order_pv;
+        d.coriolis = c->cfg.coriolis_scheme;
+        d.sp = StagePtrs{S[S_GZ], c->gzs, S[S_U_F], S[S_V_F], O[S_GZ], gz_new, du, dv};
+        d.up = UVPtrs{O[S_U_LON], O[S_V_LAT], du, dv, N[S_GZ_F], N[S_U_LON], N[S_V_LAT], N[S_U_F], N[S_V_F]};
+        d.gz_f_out = N[S_GZ_F];
+        d.gz_out = N[S_GZ];
+        d.tile0 = tile0[k];
+        d.nx = nx[k];
+      }
+  int rc = ehip(e, hipMemcpy(e->d_mem, tab.data(), sizeof(EnsMember) * tab.size(), hipMemcpyHostToDevice), "tables");
+  if (rc) return rc;
+  drop_graphs(e);
+  e->built_shape = e->shape;
+  return GMC_OK;
+}
+
+template <int TY>
+void launch_tend(gmc_ens *e, const EnsMember *mem, int grid) {
+  using S = Tile<TY>;
+  hipLaunchKernelGGL(k_tend_ens<S>, dim3(grid), dim3(S::NT), 0, e->stream, mem, (int)e->m.size(), e->total_tiles);
+}
+
+// the launches of one ensemble step for parity par (0: old 1 -> new 2)
+void ens_launch_step(gmc_ens *e, int par) {
+  const int nm = (int)e->m.size();
+  const EnsShape &sh = kEnsShapes[e->shape];
+  const int grid = sh.mult ? std::min(e->total_tiles, sh.mult * 256) : e->total_tiles;
+  for (int st = 0; st < e->nstage; st++) {
+    const EnsMember *mem = e->d_mem + ((size_t)par * e->nstage + st) * nm;
+    switch (sh.ty) {
+    case 5: launch_tend<5>(e, mem, grid); break;
+    case 7: launch_tend<7>(e, mem, grid); break;
+    default: launch_tend<11>(e, mem, grid);
+    }
+    hipLaunchKernelGGL(k_gzf_ens, dim3(e->n_gz), dim3(NF), e->lds_gz, e->stream, mem, e->d_items, e->n_gz);
+    hipLaunchKernelGGL(k_uvf_ens, dim3(e->n_uv), dim3(NF), e->lds_uv, e->stream, mem, e->d_items + e->n_gz, e->n_uv);
+  }
+}
+
+int ens_graph(gmc_ens *e, int par, hipGraphExec_t *out) {
+  if (e->graph[par]) { *out = e->graph[par]; return GMC_OK; }
+  hipGraph_t graph = nullptr;
+  int rc = ehip(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+  if (rc) return rc;
+  ens_launch_step(e, par);
+  const hipError_t le = hipGetLastError();
+  rc = ehip(e, hipStreamEndCapture(e->stream, &graph), "end capture");
+  if (!rc) rc = ehip(e, le, "ensemble launch");
+  if (!rc) rc = ehip(e, hipGraphInstantiate(&e->graph[par], graph, nullptr, nullptr, 0), "graph instantiate");
+  if (graph) (void)hipGraphDestroy(graph);
+  if (!rc) *out = e->graph[par];
+  return rc;
+}
+
+int ens_run(gmc_ens *e, const double *dt, int nsteps) {
+  const int nm = (int)e->m.size();
+  bool same = e->built_shape == e->shape;
+  for (int k = 0; k < nm && same; k++) same = e->dt[k] == dt[k];
+  if (!same) {
+    for (int k = 0; k < nm; k++) e->dt[k] = dt[k];
+    int rc = build_tables(e);
+    if (rc) return rc;
+  }
+  for (int s = 0; s < nsteps; s++) {
+    const int par = e->m[0]->old_level == 1 ? 0 : 1;
+    hipGraphExec_t g = nullptr;
+    int rc = ens_graph(e, par, &g);
+    if (!rc) rc = ehip(e, hipGraphLaunch(g, e->stream), "graph launch");
+    if (rc) return rc;
+    for (gmc_ctx *c : e->m) std::swap(c->old_level, c->new_level);
+  }
+  return GMC_OK;
+}
+
+}  // namespace
+}  // namespace gmc
+
+using namespace gmc;
+
+extern "C" {
+
+int gmc_ens_create(gmc_ctx *const *members, int32_t n, gmc_ens **out) {
+  if (!out) return GMC_EINVAL;
+  *out = nullptr;
+  if (!members || n < 1) return GMC_EINVAL;
+  gmc_ens *e = new gmc_ens();
+  auto fail = [&](int code, const char *msg) {
+    gmc_ens_destroy(e);
+    (void)msg;
+    return code;
+  };
+  for (int k = 0; k < n; k++) {
+    gmc_ctx *c = members[k];
+    if (!c || !c->prepared || c->cfg.use_div_damp || c->cfg.time_scheme != members[0]->cfg.time_scheme ||
+        c->old_level != members[0]->old_level || c->grid.nlon > Tile<5>::kMaxPoleLon)
+      return fail(GMC_EINVAL, "member not batchable");
+    for (int j = 0; j < k; j++)
+      if (members[j] == c) return fail(GMC_EINVAL, "member listed twice");
+    if (fused_prepare(c)) return fail(GMC_EINVAL, "member plan");
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(GMC_EDEVICE, "member sync");
+    e->m.push_back(c);
+  }
+  e->scheme = members[0]->cfg.time_scheme;
+  e->nstage = stage_count(e->scheme);
+  e->dt.assign(n, 0.0);
+  // filter work items of every member, tagged with the member index
+  std::vector<Item> gz, uv;
+  size_t lgz = 0, luv = 0;
+  int maxb = 1, maxs = 1;
+  for (int k = 0; k < n; k++) {
+    gmc_ctx *c = e->m[k];
+    const Grid &g = c->grid;
+    std::vector<int32_t> nb_lon(g.nlat), nb_lat(g.nhalf);
+    if (hipMemcpy(nb_lon.data(), c->big.ngrid_lon, sizeof(int32_t) * g.nlat, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(nb_lat.data(), c->big.ngrid_lat, sizeof(int32_t) * g.nhalf, hipMemcpyDeviceToHost) != hipSuccess)
+      return fail(GMC_EDEVICE, "filter tables");
+    std::vector<Item> a, b;
+    size_t dummy = 0;
+    for (int r = 0; r < g.nlat; r++) add_row(a, r, 0, nb_lon[r] + std::max(1, nb_lon[r] / 2), g.nlon, dummy);
+    for (int r = 0; r < g.nlat; r++) add_row(b, r, 1, nb_lon[r], g.nlon, luv);
+    for (int r = 0; r < g.nhalf; r++) add_row(b, r, 2, nb_lat[r], g.nlon, luv);
+    for (auto &it : a) {
+      const int nbk = nb_lon[it.row], h = nbk > 1 ? (nbk - 1) / 2 : 0;
+      lgz = std::max(lgz, sizeof(double) * (size_t)(it.c1 - it.c0 + 2 * h));
+      it.member = k;
+      gz.push_back(it);
+    }
+    for (auto &it : b) {
+      it.member = k;
+      uv.push_back(it);
+    }
+    maxb = std::max(maxb, c->big.max_taps);
+    maxs = std::max(maxs, c->small1.max_taps);
+  }
+  e->n_gz = (int)gz.size();
+  e->n_uv = (int)uv.size();
+  e->lds_gz = lgz + sizeof(double) * (maxb + maxs);
+  e->lds_uv = luv + sizeof(double) * maxb;
+  gz.insert(gz.end(), uv.begin(), uv.end());
+  int rc = ehip(e, hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "stream");
+  if (!rc) rc = ehip(e, hipMalloc((void **)&e->d_items, sizeof(Item) * gz.size()), "items");
+  if (!rc) rc = ehip(e, hipMemcpy(e->d_items, gz.data(), sizeof(Item) * gz.size(), hipMemcpyHostToDevice), "items");
+  if (!rc) rc = ehip(e, hipMalloc((void **)&e->d_mem, sizeof(EnsMember) * 2 * e->nstage * n), "tables");
+  if (!rc && e->lds_gz > 65536)
+    rc = ehip(e, hipFuncSetAttribute((const void *)k_gzf_ens, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->lds_gz), "lds");
+  if (!rc && e->lds_uv > 65536)
+    rc = ehip(e, hipFuncSetAttribute((const void *)k_uvf_ens, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->lds_uv), "lds");
+  if (!rc && std::max(e->lds_gz, e->lds_uv) > 160 * 1024) rc = eerr(e, GMC_EINVAL, "filter window exceeds LDS");
+  if (rc) {
+    gmc_ens_destroy(e);
+    return rc;
+  }
+  *out = e;
+  return GMC_OK;
+}
+
+int gmc_ens_destroy(gmc_ens *e) {
+  if (!e) return GMC_EINVAL;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  drop_graphs(e);
+  if (e->d_mem) (void)hipFree(e->d_mem);
+  if (e->d_items) (void)hipFree(e->d_items);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+  return GMC_OK;
+}
+
+const char *gmc_ens_last_error(const gmc_ens *e) { return e ? e->err.c_str() : "null ensemble"; }
+
+int gmc_ens_step(gmc_ens *e, const double *dt, int32_t nsteps) {
+  if (!e) return GMC_EINVAL;
+  if (!dt || nsteps < 0) return eerr(e, GMC_EINVAL, "bad step arguments");
+  for (size_t k = 0; k < e->m.size(); k++)
+    if (!(dt[k] > 0)) return eerr(e, GMC_EINVAL, "dt must be positive");
+  int rc = ens_run(e, dt, nsteps);
+  if (!rc) rc = ehip(e, hipStreamSynchronize(e->stream), "ensemble sync");
+  return rc;
+}
+
+int gmc_ens_shape(gmc_ens *e, int32_t shape) {
+  if (!e) return GMC_EINVAL;
+  if (shape < 0 || shape >= kNumEnsShapes) return eerr(e, GMC_EINVAL, "no such shape");
+  e->shape = shape;
+  return GMC_OK;
+}
+
+const char *gmc_ens_shape_name(int32_t shape) {
+  return shape >= 0 && shape < kNumEnsShapes ? kEnsShapes[shape].name : "";
+}
+
+int gmc_ens_autotune(gmc_ens *e, const double *dt, int32_t trial_steps, int32_t *choice, double *times_ms,
+                     int32_t cap, int32_t *ncand) {
+  if (!e) return GMC_EINVAL;
+  if (!dt || trial_steps < 1 || !choice) return eerr(e, GMC_EINVAL, "bad autotune arguments");
+  const int nm = (int)e->m.size();
+  // snapshot every member's fields and levels; the trials run on them and
+  // the snapshot is restored afterwards
+  std::vector<void *> save(nm, nullptr);
+  std::vector<int> levels(nm);
+  int rc = GMC_OK;
+  for (int k = 0; k < nm && !rc; k++) {
+    rc = ehip(e, hipMalloc(&save[k], e->m[k]->slab_bytes), "snapshot alloc");
+    if (!rc) rc = ehip(e, hipMemcpyAsync(save[k], e->m[k]->slab, e->m[k]->slab_bytes, hipMemcpyDeviceToDevice, e->stream), "snapshot");
+    levels[k] = e->m[k]->old_level;
+  }
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  if (!rc) rc = ehip(e, hipEventCreate(&ev0), "event");
+  if (!rc) rc = ehip(e, hipEventCreate(&ev1), "event");
+  const int keep = e->shape;
+  int best = keep;
+  float best_ms = 0.f;
+  for (int s = 0; s < kNumEnsShapes && !rc; s++) {
+    e->shape = s;
+    rc = ens_run(e, dt, 1);  // warm-up, builds tables and graphs
+    if (!rc) rc = ehip(e, hipEventRecord(ev0, e->stream), "event");
+    if (!rc) rc = ens_run(e, dt, trial_steps);
+    if (!rc) rc = ehip(e, hipEventRecord(ev1, e->stream), "event");
+    if (!rc) rc = ehip(e, hipEventSynchronize(ev1), "event sync");
+    float ms = 0.f;
+    if (!rc) rc = ehip(e, hipEventElapsedTime(&ms, ev0, ev1), "elapsed");
+    ms /= trial_steps;
+    if (!rc && times_ms && s < cap) times_ms[s] = ms;
+    if (!rc && (s == 0 || ms < best_ms)) { best_ms = ms; best = s; }
+  }
+  for (int k = 0; k < nm; k++) {
+    if (save[k]) {
+      if (!rc) rc = ehip(e, hipMemcpyAsync(e->m[k]->slab, save[k], e->m[k]->slab_bytes, hipMemcpyDeviceToDevice, e->stream), "restore");
+      e->m[k]->old_level = levels[k];
+      e->m[k]->new_level = levels[k] == 1 ? 2 : 1;
+    }
+  }
+  if (!rc) rc = ehip(e, hipStreamSynchronize(e->stream), "restore sync");
+  for (auto p : save)
+    if (p) (void)hipFree(p);
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  e->shape = rc ? keep : best;
+  if (!rc) {
+    *choice = best;
+    if (ncand) *ncand = kNumEnsShapes;
+  }
+  return rc;
+}
+
+int gmc_ens_stream(gmc_ens *e, void **stream) {
+  if (!e || !stream) return GMC_EINVAL;
+  *stream = (void *)e->stream;
+  return GMC_OK;
+}
+
+}  // extern "C"

@@ -125,6 +125,8 @@ int ops_update_state(gmc_ctx *ctx, int tend, int old_l, int new_l, double dt, in
 int ops_div_damp(gmc_ctx *ctx, int level);
 int ops_diagnose(gmc_ctx *ctx, int level, double out[4]);
 int ops_integrate(gmc_ctx *ctx, int old_l, int new_l, double dt);
+void ops_filter(gmc_ctx *ctx, int small, int half, const double *x, double *y);
+int ops_stage_tape(gmc_ctx *ctx, int old_l, int star_l, int new_l, double dt);
 // fused stage path (gmc_fused.hip)
 int fused_integrate(gmc_ctx *ctx, int old_l, int new_l, double dt);
 void fused_release(gmc_ctx *ctx);

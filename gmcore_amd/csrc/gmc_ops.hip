@@ -428,6 +428,26 @@ int ops_update_state(gmc_ctx *c, int tend, int old_l, int new_l, double dt, int 
   return check_hip(c, hipGetLastError(), "update_state");
 }
 
+// Zonal filter of one field through the context's stream (x == y allowed).
+void ops_filter(gmc_ctx *c, int small, int half, const double *x, double *y) {
+  filter_field(c, small ? c->small1 : c->big, half, x, y);
+}
+
+// One step_forward_backward stage old -> new with star `s` on the operator
+// kernels, leaving every intermediate in place: m and the operators_prepare
+// fields of the star state, the forward tendencies in tend level 1, the
+// backward ones in tend level 2, the new state in `n`.  m of the star state
+// is recomputed first (the same values gmc_prepare / the previous backward
+// pass left there).
+int ops_stage_tape(gmc_ctx *c, int o, int s, int n, double dt) {
+  calc_m(c, c->state[s]);
+  int rc = ops_space_operators(c, o, s, n, 1, 2, dt, 1);
+  if (!rc) rc = ops_update_state(c, 1, o, n, dt, 1);
+  if (!rc) rc = ops_space_operators(c, o, s, n, 2, 1, dt, 2);
+  if (!rc) rc = ops_update_state(c, 2, o, n, dt, 2);
+  return rc;
+}
+
 // div_damp_run order 2 (div_damp_mod.F90:89-139)
 namespace {
 __global__ void k_div_damp(Grid g, Mesh M, double coef2, const double *__restrict__ div, double *u, double *v) {

@@ -191,3 +191,122 @@ class ShallowWater:
 
     def synchronize(self):
         self._check(self.lib.gmc_synchronize(self.ctx), "synchronize")
+
+
+class Ensemble:
+    """Batched members (include/gmcore_ens.h): one graph of three launches per
+    stage steps every member, member k with its own dt."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        self.lib = capi.load_hip()
+        arr = (C.c_void_p * len(self.members))(*[m.ctx.value for m in self.members])
+        self.h = C.c_void_p()
+        rc = self.lib.gmc_ens_create(arr, len(self.members), C.byref(self.h))
+        if rc:
+            raise GmcError(f"gmc_ens_create failed with code {rc}")
+        self.dt = np.array([m.setup.nl.dt_dyn for m in self.members])
+
+    def _check(self, rc, what):
+        if rc:
+            msg = self.lib.gmc_ens_last_error(self.h)
+            raise GmcError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self.h:
+            self.lib.gmc_ens_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, nsteps=1):
+        self._check(self.lib.gmc_ens_step(self.h, dptr(self.dt), nsteps), "ens_step")
+
+    def set_shape(self, shape: int):
+        self._check(self.lib.gmc_ens_shape(self.h, shape), "ens_shape")
+
+    def autotune(self, trial_steps=3):
+        """(chosen shape, mean step ms of every shape)"""
+        n = 32
+        t = np.zeros(n)
+        choice, ncand = C.c_int32(), C.c_int32()
+        self._check(self.lib.gmc_ens_autotune(self.h, dptr(self.dt), trial_steps, C.byref(choice), dptr(t), n,
+                                              C.byref(ncand)), "ens_autotune")
+        return choice.value, t[:ncand.value]
+
+    @staticmethod
+    def shape_name(k: int) -> str:
+        return capi.load_hip().gmc_ens_shape_name(k).decode()
+
+
+TLAD_VARS = {"u": 0, "v": 1, "gz": 2, "u_f": 3, "v_f": 4}
+
+
+class TLAD:
+    """Tangent-linear and adjoint variables of one ShallowWater context
+    (include/gmcore_tlad.h)."""
+
+    def __init__(self, model: ShallowWater):
+        self.model = model
+        self.lib = model.lib
+        self.h = C.c_void_p()
+        rc = self.lib.gmc_tlad_create(model.ctx, C.byref(self.h))
+        if rc:
+            raise GmcError(f"gmc_tlad_create failed with code {rc}")
+
+    def _check(self, rc, what):
+        if rc:
+            msg = self.lib.gmc_tlad_last_error(self.h)
+            raise GmcError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self.h:
+            self.lib.gmc_tlad_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _rows(self, var):
+        nlat = self.model.nlat
+        return nlat - 1 if var in ("v", "v_f") else nlat
+
+    def set(self, level, var, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape == (self._rows(var), self.model.nlon)
+        self._check(self.lib.gmc_tlad_upload(self.h, level, TLAD_VARS[var], dptr(a), self.model.nlon), "upload")
+
+    def get(self, level, var):
+        a = np.zeros((self._rows(var), self.model.nlon))
+        self._check(self.lib.gmc_tlad_download(self.h, level, TLAD_VARS[var], dptr(a), self.model.nlon), "download")
+        return a
+
+    def zero(self):
+        self._check(self.lib.gmc_tlad_zero(self.h), "zero")
+
+    def tl_step(self, nsteps=1, dt=None):
+        self._check(self.lib.gmc_tl_step(self.h, self.model.setup.nl.dt_dyn if dt is None else dt, nsteps), "tl_step")
+
+    def ad_step(self, nsteps=1, dt=None):
+        self._check(self.lib.gmc_ad_step(self.h, self.model.setup.nl.dt_dyn if dt is None else dt, nsteps), "ad_step")
+
+    def gradient(self, nsteps, ckpt_every, obs_cell, obs_value, sigma_o, xb, sigma_b, dt=None):
+        """(J, (dJ/du0, dJ/dv0, dJ/dgz0)); xb = (u_b, v_b, gz_b)"""
+        m = self.model
+        cell = np.ascontiguousarray(obs_cell, dtype=np.int64)
+        y = np.ascontiguousarray(obs_value, dtype=np.float64)
+        ub, vb, gb = (np.ascontiguousarray(a, dtype=np.float64) for a in xb)
+        gu, gv, gg = np.zeros_like(ub), np.zeros_like(vb), np.zeros_like(gb)
+        J = C.c_double()
+        self._check(self.lib.gmc_gradient(self.h, m.setup.nl.dt_dyn if dt is None else dt, nsteps, ckpt_every,
+                                          len(cell), cell.ctypes.data_as(C.POINTER(C.c_int64)), dptr(y), sigma_o,
+                                          dptr(ub), dptr(vb), dptr(gb), sigma_b, m.nlon, C.byref(J), dptr(gu),
+                                          dptr(gv), dptr(gg)), "gradient")
+        return J.value, (gu, gv, gg)
