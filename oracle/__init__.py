@@ -6,6 +6,8 @@ time the HIP library against it.  No product code path uses it.
 
 The C sources (mesh.c, filter.c, sw.c, adv.c) restate the reference Fortran
 (each function cites its file:line); ``make -C oracle`` builds liboracle.so.
+adjoint.py is a numpy FP64 adjoint of the step on the oracle's trajectory,
+the CPU checker of the GPU adjoint.
 """
 from __future__ import annotations
 
