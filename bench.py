@@ -44,13 +44,7 @@ KERNEL_TRAFFIC_PER_GP = {"k_tend": (2 * 19959.5 + 24314.3) * 1024 / (1440 * 721)
                          "k_uvf": (2 * 29576.1 + 32400.0) * 1024 / (1440 * 721)}
 
 # FFSL tracer advection, 0.5 degrees, 8 tracers, dt = 900 s, one day
-ADV_# batched ensemble: 64 members of uneven resolution, each with its stable dt
-ENS_CONFIGS = {
-    "ens64": dict(members=[(180, 91, 600.0, 32), (360, 181, 300.0, 24), (720, 361, 150.0, 8)],
-                  lmax=20, perturb=0.01, steps=100),
-}
-
-CONFIGS = {
+ADV_CONFIGS = {
     "ffsl05": dict(nlon=720, nlat=361, dt=900.0, ntracers=8, lmax=10, wind_seed=2, tracer_seed=3,
                    vmax=50.0, hw=104, steps=96),
 }
@@ -62,6 +56,12 @@ ADV_BYTES_PER_TRACER_GP = lambda ntr: (2 * ntr + 4) * 8.0 / ntr
 ENS_CONFIGS = {
     "ens64": dict(members=[(180, 91, 600.0, 32), (360, 181, 300.0, 24), (720, 361, 150.0, 8)],
                   lmax=20, perturb=0.01, steps=100),
+}
+
+# adjoint: 4D-Var-style gradient over a window at 1 degree
+ADJ_CONFIGS = {
+    "adj1": dict(nlon=360, nlat=181, dt=300.0, lmax=30, truth_seed=4, bg_perturb=(0.01, 6), nobs=5000,
+                 obs_seed=5, sigma_o=50.0, sigma_b=1.0, window=48, ckpt=8),
 }
 
 CONFIGS = {
@@ -295,13 +295,109 @@ def run_ens(args):
     print(json.dumps(result))
 
 
+def run_adj(args):
+    """Gradient of J = 1/2 |gz_N(obs) - y|^2 / sigma_o^2 + 1/2 |x0 - xb|^2 /
+    sigma_b^2 over a window of N steps, by the adjoint with checkpointing;
+    reports its time against N forward steps of the plain path (value =
+    gradient-window time / forward-window time), and the dot-product test of
+    the TL and AD over one step at the same size."""
+    from gmcore_amd import init_cond
+    from gmcore_amd.host import TLAD, Setup, ShallowWater
+    from gmcore_amd.namelist import Namelist
+    cfg = ADJ_CONFIGS[args.config]
+    nlon, nlat, dt, N = cfg["nlon"], cfg["nlat"], cfg["dt"], cfg["window"]
+    st = Setup(Namelist(num_lon=nlon, num_lat=nlat, dt_dyn=dt, time_scheme="pc2"))
+    m_ = st.mesh.metrics
+    truth_ic = init_cond.seeded_state(nlon, nlat, m_, cfg["lmax"], cfg["truth_seed"], omega=st.nl.omega,
+                                      radius=st.nl.radius)
+    bg = init_cond.seeded_state(nlon, nlat, m_, cfg["lmax"], cfg["truth_seed"], perturb=cfg["bg_perturb"],
+                                omega=st.nl.omega, radius=st.nl.radius)
+    truth = ShallowWater(st)
+    truth.set_state(*truth_ic)
+    truth.prepare(1)
+    truth.step(N)
+    rng = np.random.default_rng(cfg["obs_seed"])
+    cells = rng.choice(nlat * nlon, size=cfg["nobs"], replace=False)
+    y = truth.state()["gz"].ravel()[cells] + rng.normal(0.0, cfg["sigma_o"], cfg["nobs"])
+    truth.close()
+    m = ShallowWater(st)
+    m.set_state(*bg)
+    m.prepare(1)
+    t = TLAD(m)
+    xb = (bg[0], bg[1], bg[2])
+    # warm-up (graphs, trajectory buffers)
+    t.gradient(2, 1, cells, y, cfg["sigma_o"], xb, cfg["sigma_b"])
+    m.step(1)
+    m.synchronize()
+    m.set_state(*bg)
+    m.prepare(1)
+    w0 = time.perf_counter()
+    J, grad = t.gradient(N, cfg["ckpt"], cells, y, cfg["sigma_o"], xb, cfg["sigma_b"])
+    t_grad = time.perf_counter() - w0
+    # N forward steps of the plain path from the same state
+    m2 = ShallowWater(st)
+    m2.set_state(*bg)
+    m2.prepare(1)
+    m2.step(1)
+    m2.synchronize()
+    m2.set_state(*bg)
+    m2.prepare(1)
+    m2.synchronize()
+    w0 = time.perf_counter()
+    m2.step(N)
+    m2.synchronize()
+    t_fwd = time.perf_counter() - w0
+    # dot-product test over one step
+    rs = np.random.default_rng(7)
+    dx = (rs.standard_normal((nlat, nlon)) * 0.5, rs.standard_normal((nlat - 1, nlon)) * 0.5,
+          rs.standard_normal((nlat, nlon)) * 100.0)
+    dx[0][[0, -1]] = 0.0
+    lam = (rs.standard_normal((nlat, nlon)), rs.standard_normal((nlat - 1, nlon)),
+           rs.standard_normal((nlat, nlon)) * 1e-2)
+    lam[0][[0, -1]] = 0.0
+    names = ("u", "v", "gz")
+    t.zero()
+    lv = m.level
+    for k, a in zip(names, dx):
+        t.set(lv, k, a)
+    t.tl_step(1)
+    mdx = [t.get(m.level, k) for k in names]
+    m.set_state(*bg, level=m.level)
+    m.prepare(m.level)
+    t.zero()
+    lv = m.level
+    for k, a in zip(names, lam):
+        t.set(lv, k, a)
+    t.ad_step(1)
+    mtl = [t.get(lv, k) for k in names]
+    dot = lambda a, b: float(np.sum(a[0][1:-1] * b[0][1:-1]) + np.sum(a[1] * b[1]) + np.sum(a[2] * b[2]))
+    lhs, rhs = dot(mdx, lam), dot(dx, mtl)
+    t.close()
+    ratio = t_grad / t_fwd
+    result = {
+        "metric": "adjoint gradient cost / forward cost (FP64 SW, 1 GPU)",
+        "value": ratio, "unit": "x forward window", "n_gpus": 1, "steps": N, "warmup": 1,
+        "ms_per_step": t_grad * 1e3 / N, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded truth l<=30; background = truth perturbed 1 % rms; "
+                                "5000 observed gz cells with N(0, 50^2) noise)",
+        "config": {"workload": f"adjoint {args.config}: {nlon}x{nlat}, pc2, dt={dt:g} s, window {N} steps, "
+                               f"checkpoint every {cfg['ckpt']} steps", "nlon": nlon, "nlat": nlat},
+        "gradient_ms": t_grad * 1e3, "forward_ms": t_fwd * 1e3, "cost": J,
+        "grad_norm": float(np.sqrt(sum(float(np.sum(g * g)) for g in grad))),
+        "dot_product_rel_err": abs(lhs - rhs) / max(abs(lhs), abs(rhs)),
+    }
+    m.close()
+    m2.close()
+    print(json.dumps(result))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="sw025",
-                    choices=sorted(CONFIGS) + sorted(ADV_CONFIGS) + sorted(ENS_CONFIGS))
+                    choices=sorted(CONFIGS) + sorted(ADV_CONFIGS) + sorted(ENS_CONFIGS) + sorted(ADJ_CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -329,6 +425,9 @@ def main():
         return
     if args.config in ENS_CONFIGS:
         run_ens(args)
+        return
+    if args.config in ADJ_CONFIGS:
+        run_adj(args)
         return
     cfg = CONFIGS[args.config]
     from gmcore_amd.host import ShallowWater
