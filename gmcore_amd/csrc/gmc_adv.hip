@@ -5,16 +5,20 @@
 //   k_adv_accum   wind accumulation (adv_batch_mod.F90:218-252, 379-403)
 //   k_adv_cfl     RK4 CFL numbers and split divergences (:254-373), only on
 //                 the calls whose accumulation window closes
-//   k_adv_inner   per (row, tracer): inner PPM fluxes of q along x (row in
-//                 LDS) and y (the two V rows either side), then the advective
-//                 half-steps qx, qy (ffsl_mod.F90:350-407)
-//   k_adv_outer   per (row, tracer): outer PPM fluxes of qy along x and qx
-//                 along y (ffsl_mod.F90:410), the flux-form update, /m, the
-//                 sequential positivity fixer and the longitude damping of
-//                 the row (adv_mod.F90:183-301)
-// One workgroup owns one latitude row of one tracer, so the row-sequential
-// parts (the fixer, the Jacobi damping) stay inside LDS and the only
-// cross-row data are the qx, qy fields between the two tracer kernels.
+//   k_adv_inner   per cell: inner PPM fluxes of q along x (the lane's U
+//                 point; the west face comes from the lane below) and y (the
+//                 two V rows either side), then the advective half-steps qx,
+//                 qy (ffsl_mod.F90:350-407)
+//   k_adv_flux    per cell: outer PPM fluxes of qy along x and qx along y
+//                 (ffsl_mod.F90:410), the flux-form update and /m
+//                 (adv_mod.F90:183-235) into qp
+//   k_adv_fix     per (row, tracer): the positivity fixer and the longitude
+//                 damping of the row (adv_mod.F90:236-301), qp -> new level
+// The flux kernels read every stencil from global memory (L2): one lane per
+// cell, no LDS and no barrier, so many wavefronts hide the latency of the
+// data-dependent loads (the upwind cells).  The pole rows, whose y-flux is a
+// zonal sum, are one workgroup each.  Only the fixer and the damping, which
+// walk a row, keep the row in LDS.
 //
 // Layout in HBM: dense row-major [rows][pitch] doubles per field, no halo;
 // tracer l of level k at q + ((k-1) * ntracers + l) * nlat * pitch.
@@ -36,8 +40,8 @@ namespace {
 
 constexpr int NA = 256;  // threads per workgroup of the row kernels
 enum { F_CFLX = 1, F_CFLY = 2, F_NEG = 4 };
-enum { KA_ACCUM, KA_CFL, KA_INNER, KA_OUTER, KA_NCLASS };
-const char *const kAdvNames[KA_NCLASS] = {"k_adv_accum", "k_adv_cfl", "k_adv_inner", "k_adv_outer"};
+enum { KA_ACCUM, KA_CFL, KA_INNER, KA_FLUX, KA_FIX, KA_NCLASS };
+const char *const kAdvNames[KA_NCLASS] = {"k_adv_accum", "k_adv_cfl", "k_adv_inner", "k_adv_flux", "k_adv_fix"};
 
 struct AdvGrid {
   int nlon, nlat, nhalf, half;
@@ -71,8 +75,9 @@ __device__ __forceinline__ Ppm ppm(int lim, double fm2, double fm1, double f, do
   const double dfl = slope(lim, fm2, fm1, f);
   const double df = slope(lim, fm1, f, fp1);
   const double dfr = slope(lim, f, fp1, fp2);
-  double fl = 0.5 * (fm1 + f) + (dfl - df) / 6.0;
-  double fr = 0.5 * (fp1 + f) + (df - dfr) / 6.0;
+  // x / 6 through div_rn: the IEEE quotient in three operations
+  double fl = 0.5 * (fm1 + f) + div_rn(dfl - df, 6.0, 1.0 / 6.0);
+  double fr = 0.5 * (fp1 + f) + div_rn(df - dfr, 6.0, 1.0 / 6.0);
   fl = f - fsign(dmin(fabs(df), fabs(fl - f)), df);
   fr = f + fsign(dmin(fabs(df), fabs(fr - f)), df);
   return Ppm{fl, fr - fl, 6 * f - 3 * (fl + fr)};
@@ -83,78 +88,54 @@ __device__ __forceinline__ Ppm ppm(int lim, double fm2, double fm1, double f, do
 __device__ __forceinline__ double ppm_part_x(double s1, double s2, double ml, double dm, double m6,
                                              double sum) {
   const double ds1 = s2 - s1, ds2 = s2 * s2 - s1 * s1, ds3 = s2 * s2 * s2 - s1 * s1 * s1;
-  return sum + ml * ds1 + 0.5 * dm * ds2 + m6 * (ds2 / 2.0 - ds3 / 3.0);
+  return sum + ml * ds1 + 0.5 * dm * ds2 + m6 * (ds2 / 2.0 - div_rn(ds3, 3.0, 1.0 / 3.0));
 }
 __device__ __forceinline__ double ppm_part_y(double s1, double s2, double ml, double dm, double m6) {
   const double ds1 = s2 - s1, ds2 = s2 * s2 - s1 * s1, ds3 = s2 * s2 * s2 - s1 * s1 * s1;
-  return ml * ds1 + 0.5 * dm * ds2 + m6 * (ds2 / 2.0 - ds3 / 3.0);
+  return ml * ds1 + 0.5 * dm * ds2 + m6 * (ds2 / 2.0 - div_rn(ds3, 3.0, 1.0 / 3.0));
 }
 
-// x-flux through U point i of a row whose PPM coefficients are in LDS
-// (hflx_ppm_cell :831-853).  |int(cflx)| is clamped to the halo bound the
-// accumulation step has already checked, so every index stays within one
-// wrap of the row.
-__device__ __forceinline__ double xflux(const AdvGrid &g, double c, double u, int i, const double *mx,
-                                        const double *ml, const double *dm, const double *m6) {
-  int ci = (int)c;
-  const double cf = c - ci;
-  ci = ci > g.hw ? g.hw : (ci < -g.hw ? -g.hw : ci);
-  if (c > 0) {
-    const int iu = wrap(i - ci, g.nlon);
-    double sum = 0;
-    for (int k = i + 1 - ci; k <= i; k++) sum = sum + mx[wrap(k, g.nlon)];
-    return u * ppm_part_x(1 - cf, 1, ml[iu], dm[iu], m6[iu], sum) / c;
-  } else if (c < 0) {
-    const int iu = wrap(i - ci + 1, g.nlon);
-    double sum = 0;
-    for (int k = i + 1; k <= i - ci; k++) sum = sum + mx[wrap(k, g.nlon)];
-    return -u * ppm_part_x(0, -cf, ml[iu], dm[iu], m6[iu], sum) / c;
+// mx[k0] + mx[k0 + 1] + ... + mx[k0 + cnt - 1] (columns modulo n), added
+// from 0 in ascending order; the LDS reads are issued eight at a time ahead
+// of the dependent additions
+__device__ __forceinline__ double run_sum(const double *mx, int k0, int cnt, int n) {
+  double sum = 0;
+  int j = 0;
+  for (; j + 8 <= cnt; j += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = mx[wrap(k0 + j + u, n)];
+#pragma unroll
+    for (int u = 0; u < 8; u++) sum = sum + v[u];
   }
-  return 0;
+  for (; j < cnt; j++) sum = sum + mx[wrap(k0 + j, n)];
+  return sum;
 }
 
-// y-flux through V point (i, h) of the field my (hflx_ppm_cell :855-877),
-// with the PPM coefficients of the upwind full row computed here from the
-// five cells of column i around it (rows through the poles mirrored).
+// y-flux through V point (h, i) of the field my (hflx_ppm_cell :855-877),
+// with the PPM coefficients of the upwind full row (h if c > 0, else h + 1)
+// from the five cells of column i around it.  h is uniform over the
+// wavefront: the six rows h-2 .. h+3 are read through row pointers (rows
+// beyond a pole mirrored, column half a circle away) and the upwind five
+// selected per lane.
 __device__ __forceinline__ double yflux(const AdvGrid &g, int lim, double c, double v, int h, int i,
                                         const double *my) {
-  if (c == 0) return 0;
-  const int ju = c > 0 ? h : h + 1;
-  double f[5];
+  const int ih = wrap(i + g.half, g.nlon);
+  double f[6];
 #pragma unroll
-  for (int k = 0; k < 5; k++) {
-    int r = ju - 2 + k, ii = i;
-    if (r < 0) { r = -r; ii = wrap(i + g.half, g.nlon); }
-    else if (r > g.nlat - 1) { r = 2 * (g.nlat - 1) - r; ii = wrap(i + g.half, g.nlon); }
-    f[k] = my[r * g.pitch + ii];
+  for (int k = 0; k < 6; k++) {
+    const int r = h - 2 + k;
+    const bool mir = r < 0 || r > g.nlat - 1;
+    const int rr = r < 0 ? -r : (r > g.nlat - 1 ? 2 * (g.nlat - 1) - r : r);
+    f[k] = my[rr * g.pitch + (unsigned)(mir ? ih : i)];
   }
-  const Ppm p = ppm(lim, f[0], f[1], f[2], f[3], f[4]);
-  if (c > 0) return v * ppm_part_y(1 - c, 1, p.fl, p.df, p.f6) / c;
-  return -v * ppm_part_y(0, -c, p.fl, p.df, p.f6) / c;
-}
-
-// PPM coefficients of a whole row held in LDS (ppm along x, :822-827)
-__device__ __forceinline__ void ppm_row(const AdvGrid &g, int lim, const double *row, double *ml,
-                                        double *dm, double *m6) {
-  const int n = g.nlon;
-  for (int i = threadIdx.x; i < n; i += NA) {
-    const Ppm p = ppm(lim, row[wrap(i - 2, n)], row[wrap(i - 1, n)], row[i], row[wrap(i + 1, n)],
-                      row[wrap(i + 2, n)]);
-    ml[i] = p.fl;
-    dm[i] = p.df;
-    m6[i] = p.f6;
-  }
-}
-
-// zonal sum of an LDS row in the library's fixed order, broadcast
-__device__ __forceinline__ double block_zsum(const AdvGrid &g, const double *x, double *bcast) {
-  __syncthreads();
-  if (threadIdx.x < kWave) {
-    const double s = wave_zonal_sum(g.nlon, threadIdx.x, [&](int i) { return x[i]; });
-    if (threadIdx.x == 0) *bcast = s;
-  }
-  __syncthreads();
-  return *bcast;
+  // both wind directions on one path (no divergence within a wavefront):
+  // c > 0: v * part(1 - c, 1) / c from row h; c < 0: -v * part(0, -c) / c
+  // from row h + 1
+  const bool up = c > 0;
+  const Ppm p = ppm(lim, up ? f[0] : f[1], up ? f[1] : f[2], up ? f[2] : f[3], up ? f[3] : f[4], up ? f[4] : f[5]);
+  const double fy = (up ? v : -v) * ppm_part_y(up ? 1 - c : 0, up ? 1 : -c, p.fl, p.df, p.f6) / c;
+  return c == 0 ? 0 : fy;
 }
 
 // ---------------------------------------------------------------------------
@@ -168,9 +149,10 @@ struct AccumPtrs {
   const double *ub, *vb;
 };
 
+// grid (column chunks of NA, rows): one column per thread
 __global__ void __launch_bounds__(NA) k_adv_accum(AdvGrid g, AccumPtrs p, int mode, int nstep, double scale) {
-  const int r = blockIdx.x;
-  for (int i = threadIdx.x; i < g.nlon; i += NA) {
+  const int r = blockIdx.y, i = blockIdx.x * NA + threadIdx.x;
+  if (i < g.nlon) {
     const int64_t k = r * g.pitch + i;
     for (int f = 0; f < 4; f++) {
       if (r >= (f & 1 ? g.nhalf : g.nlat)) continue;
@@ -189,9 +171,10 @@ __global__ void __launch_bounds__(NA) k_adv_accum(AdvGrid g, AccumPtrs p, int mo
   }
 }
 
-// CFL numbers and split divergences (adv_batch_mod.F90:254-373); one
-// workgroup per row index r: cflx and divx, divy of full row r, cfly of half
-// row r; the pole rows' divy from the zonal sum of the adjacent V row.
+// CFL numbers and split divergences (adv_batch_mod.F90:254-373); grid
+// (column chunks of NA, row index r): cflx and divx, divy of full row r,
+// cfly of half row r, one column per thread; the pole rows' divy from the
+// zonal sum of the adjacent V row, by the first chunk's workgroup.
 struct CflPtrs {
   const double *u, *v;
   double *cflx, *cfly, *divx, *divy;
@@ -199,14 +182,13 @@ struct CflPtrs {
 };
 
 __global__ void __launch_bounds__(NA) k_adv_cfl(AdvGrid g, Mesh M, CflPtrs p) {
-  __shared__ double sv[4096 + 1];
-  const int r = blockIdx.x, n = g.nlon;
+  const int r = blockIdx.y, n = g.nlon, i = blockIdx.x * NA + threadIdx.x;
   const double R = g.radius, dt = g.dt;
   int bad = 0;
-  if (r > 0 && r < g.nlat - 1) {
+  if (r > 0 && r < g.nlat - 1 && i < n) {
     const double dx = M(GMC_DE_LON, r), cl = M(GMC_FULL_COS_LAT, r);
     const double *u = p.u + r * g.pitch;
-    for (int i = threadIdx.x; i < n; i += NA) {
+    {
       // 1-based indices as in the reference: i1 = i + 1, half_lon(l) =
       // (l - 1) dlon + dlon / 2 (mesh_mod.F90:181-183)
       const int i1 = i + 1;
@@ -230,9 +212,9 @@ __global__ void __launch_bounds__(NA) k_adv_cfl(AdvGrid g, Mesh M, CflPtrs p) {
                                  p.v[(r - 1) * g.pitch + i] * M(GMC_LE_LAT, r - 1)) / M(GMC_AREA_CELL, r);
     }
   }
-  if (r < g.nhalf) {
+  if (r < g.nhalf && i < n) {
     const double dy = M(GMC_DE_LAT, r), y0 = R * M(GMC_HALF_LAT, r);
-    for (int i = threadIdx.x; i < n; i += NA) {
+    {
       const int j1 = r + 1;
       double vk[4], yk;
       vk[0] = p.v[r * g.pitch + i];
@@ -253,13 +235,13 @@ __global__ void __launch_bounds__(NA) k_adv_cfl(AdvGrid g, Mesh M, CflPtrs p) {
       if (fabs(c) > 1) bad |= F_CFLY;
     }
   }
-  if (r == 0 || r == g.nlat - 1) {
+  if ((r == 0 || r == g.nlat - 1) && blockIdx.x == 0 && threadIdx.x < kWave) {
     const int h = r == 0 ? 0 : g.nhalf - 1;
     const double sgn = r == 0 ? 1.0 : -1.0;
-    for (int i = threadIdx.x; i < n; i += NA) sv[i] = sgn * p.v[h * g.pitch + i];
-    double pole = block_zsum(g, sv, &sv[4096]);
+    const double *vh = p.v + h * g.pitch;
+    double pole = wave_zonal_sum(n, threadIdx.x, [&](int k) { return sgn * vh[k]; });
     pole = pole * M(GMC_LE_LAT, h) / n / M(GMC_AREA_CELL, r);
-    for (int i = threadIdx.x; i < n; i += NA) p.divy[r * g.pitch + i] = pole;
+    for (int k = threadIdx.x; k < n; k += kWave) p.divy[r * g.pitch + k] = pole;
   }
   if (bad) atomicOr(p.flags, bad);
 }
@@ -272,149 +254,264 @@ struct InnerPtrs {
   int64_t tstride;       // doubles between tracers
 };
 
-// ffsl_calc_tracer_hflx_cell, inner part (ffsl_mod.F90:350-407) for full
-// row r of tracer blockIdx.y
-__global__ void __launch_bounds__(NA) k_adv_inner(AdvGrid g, Mesh M, int lim, InnerPtrs p) {
-  extern __shared__ double lds[];
-  const int n = g.nlon, r = blockIdx.x;
-  const int64_t toff = blockIdx.y * p.tstride;
+// x-flux through U point i of row `row` (hflx_ppm_cell, ffsl_mod.F90:831-853),
+// read from global memory: the whole cells crossed and the PPM coefficients
+// (ppm along x, :822-827) of the upwind cell.  |int(cflx)| is clamped to
+// the halo bound the accumulation step has already checked, so every index
+// stays within one wrap of the row.
+__device__ __forceinline__ double xflux_g(const AdvGrid &g, int lim, double c, double u, int i,
+                                          const double *row) {
+  const int n = g.nlon;
+  int ci = (int)c;
+  const double cf = c - ci;
+  ci = ci > g.hw ? g.hw : (ci < -g.hw ? -g.hw : ci);
+  // both wind directions on one path (no divergence within a wavefront):
+  // c > 0: upwind cell i - ci, whole cells i+1-ci .. i, u * part(1 - cf, 1) / c;
+  // c < 0: upwind cell i - ci + 1, whole cells i+1 .. i-ci, -u * part(0, -cf) / c
+  const bool up = c > 0;
+  const int iu = wrap(up ? i - ci : i - ci + 1, n);
+  const double sum = run_sum(row, up ? i + 1 - ci : i + 1, up ? ci : -ci, n);
+  const Ppm p = ppm(lim, row[wrap(iu - 2, n)], row[wrap(iu - 1, n)], row[iu], row[wrap(iu + 1, n)],
+                    row[wrap(iu + 2, n)]);
+  const double fx = (up ? u : -u) * ppm_part_x(up ? 1 - cf : 0, up ? 1 : -cf, p.fl, p.df, p.f6, sum) / c;
+  return c == 0 ? 0 : fx;
+}
+
+// Work of the two flux kernels, in blocks of NF = 8 wavefronts.  Items
+// [0, npole) are the pole rows (2 per tracer, padded to a multiple of the
+// XCD count so that the interior items keep their XCD placement).  An
+// interior item covers the cell rows r0 .. r0 + 6 (r0 = 1 + 7 rb) over one
+// column segment: wavefront w forms the y-fluxes of V row r0 - 1 + w (into
+// LDS, each shared by the cells either side) and, for w < 7, the x-fluxes
+// and the update of cell row r0 + w.  Lane l sits on column 63 s - 1 + l;
+// lanes 1..63 own the cells 63 s .. 63 s + 62, the x-flux on a cell's west
+// face coming from the lane below.  Interior items are numbered (row block,
+// segment, tracer), the tracer fastest, and dealt to the XCDs in contiguous
+// runs, so the wind rows and the neighbouring tracer rows an item reads are
+// already in its XCD's L2.
+constexpr int NF = 512;
+constexpr int kSeg = kWave - 1;
+constexpr int kRowsPerItem = NF / kWave - 1;
+struct FluxItem {
+  int pole;    // 1: pole row item
+  int r, t;    // pole: cell row; interior: first cell row r0; tracer
+  int c0;      // first cell of the segment
+};
+__device__ __forceinline__ FluxItem flux_item(const AdvGrid &g, int ntr) {
+  const int npole = (2 * ntr + kNumXcd - 1) / kNumXcd * kNumXcd;
+  const int b = blockIdx.x;
+  FluxItem it{0, -1, 0, 0};
+  if (b < npole) {
+    if (b < 2 * ntr) { it.pole = 1; it.r = b % 2 ? g.nlat - 1 : 0; it.t = b / 2; }
+    return it;
+  }
+  const int nseg = (g.nlon + kSeg - 1) / kSeg, nrb = (g.nlat - 2 + kRowsPerItem - 1) / kRowsPerItem;
+  const int L = xcd_item(b - npole, nrb * nseg * ntr);
+  it.t = L % ntr;
+  const int rs = L / ntr, s = rs % nseg, rb = rs / nseg;
+  it.r = 1 + kRowsPerItem * rb;
+  it.c0 = s * kSeg;
+  return it;
+}
+int flux_blocks(const AdvGrid &g, int ntr) {
+  const int npole = (2 * ntr + kNumXcd - 1) / kNumXcd * kNumXcd;
+  return npole + ((g.nlat - 2 + kRowsPerItem - 1) / kRowsPerItem) * ((g.nlon + kSeg - 1) / kSeg) * ntr;
+}
+
+// Sum over the columns of f(i) in wave_zonal_sum's order (lane l adds f(l),
+// f(l + 64), ... then the xor butterfly), by the first wavefront of the
+// block; the values are formed four at a time ahead of the additions.
+template <class F>
+__device__ __forceinline__ double wave_row_sum(int n, F f) {
+  const int lane = threadIdx.x;
+  double acc = 0.0;
+  for (int base = lane; base < n; base += 4 * kWave) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) v[u] = base + u * kWave < n ? f(base + u * kWave) : 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (base + u * kWave < n) acc = acc + v[u];
+  }
+  for (int w = 1; w < kWave; w <<= 1) acc = acc + __shfl_xor(acc, w, kWave);
+  return acc;
+}
+
+// ffsl_calc_tracer_hflx_cell, inner part (ffsl_mod.F90:350-407): the
+// advective half-steps qx, qy of every cell
+__global__ void __launch_bounds__(NF) k_adv_inner(AdvGrid g, Mesh M, int lim, int ntr, InnerPtrs p) {
+  __shared__ double sfy[NF / kWave][kWave];
+  __shared__ double bcast;
+  const FluxItem it = flux_item(g, ntr);
+  const int n = g.nlon;
+  const int64_t toff = it.t * p.tstride, P = g.pitch;
   const double *q = p.q + toff;
-  double *srow = lds, *sml = lds + n, *sdm = lds + 2 * n, *s6 = lds + 3 * n, *sfx = lds + 4 * n;
-  double *sb = lds + 5 * n;
   const double dt = g.dt;
-  for (int i = threadIdx.x; i < n; i += NA) srow[i] = q[r * g.pitch + i];
+  if (it.pole) {
+    // pole rows: qx = q, qy from the zonal sum of the adjacent V row's flux
+    const int r = it.r, h = r == 0 ? 0 : g.nhalf - 1;
+    if (threadIdx.x < kWave) {
+      const double s = wave_row_sum(n, [&](int i) { return yflux(g, lim, p.cfly[h * P + i], p.v[h * P + i], h, i, q); });
+      if (threadIdx.x == 0) bcast = s;
+    }
+    __syncthreads();
+    const double pole = bcast * M(GMC_LE_LAT, h) / n / M(GMC_AREA_CELL, r);
+    for (int i = threadIdx.x; i < n; i += NF) {
+      const int64_t k = r * P + i;
+      const double qc = q[k];
+      p.qx[toff + k] = qc;
+      p.qy[toff + k] = r == 0 ? qc - 0.5 * (pole - p.divy[k] * qc) * dt : qc + 0.5 * (pole - p.divy[k] * qc) * dt;
+    }
+    return;
+  }
+  if (it.r < 0) return;   // padding item (uniform over the block)
+  const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+  const int iw = wrap(it.c0 - 1 + l, n);   // column of the lane
+  const int h = it.r - 1 + w, r = it.r + w;
+  if (h <= g.nhalf - 1) sfy[w][l] = yflux(g, lim, p.cfly[h * P + iw], p.v[h * P + iw], h, iw, q);
+  const bool row = w < kRowsPerItem && r <= g.nlat - 2;   // uniform over the wavefront
+  const double fx = row ? xflux_g(g, lim, p.cflx[r * P + iw], p.u[r * P + iw], iw, q + r * P) : 0.0;
+  const double fxw = __shfl_up(fx, 1, kWave);
   __syncthreads();
-  if (r > 0 && r < g.nlat - 1) {
-    ppm_row(g, lim, srow, sml, sdm, s6);
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += NA)
-      sfx[i] = xflux(g, p.cflx[r * g.pitch + i], p.u[r * g.pitch + i], i, srow, sml, sdm, s6);
-    __syncthreads();
+  const int i = it.c0 - 1 + l;   // cell of the lane
+  if (row && l >= 1 && i < n) {
     const double lel = M(GMC_LE_LON, r), ac = M(GMC_AREA_CELL, r);
     const double ls = M(GMC_LE_LAT, r - 1), ln = M(GMC_LE_LAT, r);
-    for (int i = threadIdx.x; i < n; i += NA) {
-      const double fs = yflux(g, lim, p.cfly[(r - 1) * g.pitch + i], p.v[(r - 1) * g.pitch + i], r - 1, i, q);
-      const double fn = yflux(g, lim, p.cfly[r * g.pitch + i], p.v[r * g.pitch + i], r, i, q);
-      const double qc = srow[i];
-      const int64_t k = r * g.pitch + i;
-      p.qx[toff + k] = qc - 0.5 * ((sfx[i] - sfx[wrap(i - 1, n)]) * lel / ac - p.divx[k] * qc) * dt;
-      p.qy[toff + k] = qc - 0.5 * ((fn * ln - fs * ls) / ac - p.divy[k] * qc) * dt;
-    }
-  } else {
-    // pole rows: qx = q, qy from the zonal sum of the adjacent V row's flux
-    const int h = r == 0 ? 0 : g.nhalf - 1;
-    for (int i = threadIdx.x; i < n; i += NA)
-      sml[i] = yflux(g, lim, p.cfly[h * g.pitch + i], p.v[h * g.pitch + i], h, i, q);
-    double pole = block_zsum(g, sml, sb);
-    pole = pole * M(GMC_LE_LAT, h) / n / M(GMC_AREA_CELL, r);
-    for (int i = threadIdx.x; i < n; i += NA) {
-      const double qc = srow[i];
-      const int64_t k = r * g.pitch + i;
-      p.qx[toff + k] = qc;
-      p.qy[toff + k] = r == 0 ? qc - 0.5 * (pole - p.divy[k] * qc) * dt
-                              : qc + 0.5 * (pole - p.divy[k] * qc) * dt;
-    }
+    const double fs = sfy[w][l], fn = sfy[w + 1][l];
+    const int64_t k = r * P + i;
+    const double qc = q[k];
+    p.qx[toff + k] = qc - 0.5 * ((fx - fxw) * lel / ac - p.divx[k] * qc) * dt;
+    p.qy[toff + k] = qc - 0.5 * ((fn * ln - fs * ls) / ac - p.divy[k] * qc) * dt;
   }
 }
 
 struct OuterPtrs {
   const double *qx, *qy;   // tracer 0
   const double *q;         // level old, tracer 0
+  double *qp;              // tracer 0: the updated tracer before the fixer
+  const double *mfx, *mfy, *cflx, *cfly, *old_m, *m;
+  int64_t tstride;
+};
+
+// outer fluxes (ffsl_mod.F90:410) and adv_run's flux-form update and /m
+// (adv_mod.F90:183-235) of every cell, into qp
+__global__ void __launch_bounds__(NF) k_adv_flux(AdvGrid g, Mesh M, int lim, int ntr, OuterPtrs p) {
+  __shared__ double sfy[NF / kWave][kWave];
+  __shared__ double bcast;
+  const FluxItem it = flux_item(g, ntr);
+  const int n = g.nlon;
+  const int64_t toff = it.t * p.tstride, P = g.pitch;
+  const double *qx = p.qx + toff, *qy = p.qy + toff, *q = p.q + toff;
+  double *qp = p.qp + toff;
+  const double dt = g.dt;
+  if (it.pole) {
+    const int r = it.r, h = r == 0 ? 0 : g.nhalf - 1;
+    if (threadIdx.x < kWave) {
+      const double s = wave_row_sum(n, [&](int i) { return yflux(g, lim, p.cfly[h * P + i], p.mfy[h * P + i], h, i, qx); });
+      if (threadIdx.x == 0) bcast = s;
+    }
+    __syncthreads();
+    const double pole = bcast * M(GMC_LE_LAT, h) / n / M(GMC_AREA_CELL, r) * dt;
+    for (int i = threadIdx.x; i < n; i += NF) {
+      const int64_t k = r * P + i;
+      const double qv = r == 0 ? p.old_m[k] * q[k] - pole : p.old_m[k] * q[k] + pole;
+      qp[k] = qv / p.m[k];
+    }
+    return;
+  }
+  if (it.r < 0) return;
+  const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+  const int iw = wrap(it.c0 - 1 + l, n);
+  const int h = it.r - 1 + w, r = it.r + w;
+  if (h <= g.nhalf - 1) sfy[w][l] = yflux(g, lim, p.cfly[h * P + iw], p.mfy[h * P + iw], h, iw, qx);
+  const bool row = w < kRowsPerItem && r <= g.nlat - 2;
+  const double fx = row ? xflux_g(g, lim, p.cflx[r * P + iw], p.mfx[r * P + iw], iw, qy + r * P) : 0.0;
+  const double fxw = __shfl_up(fx, 1, kWave);
+  __syncthreads();
+  const int i = it.c0 - 1 + l;
+  if (row && l >= 1 && i < n) {
+    const double lel = M(GMC_LE_LON, r), ac = M(GMC_AREA_CELL, r);
+    const double ls = M(GMC_LE_LAT, r - 1), ln = M(GMC_LE_LAT, r);
+    const double fs = sfy[w][l], fn = sfy[w + 1][l];
+    const int64_t k = r * P + i;
+    const double qv = p.old_m[k] * q[k] - ((fx - fxw) * lel + (fn * ln - fs * ls)) / ac * dt;
+    qp[k] = qv / p.m[k];
+  }
+}
+
+struct FixPtrs {
+  const double *qp;        // tracer 0
   double *qn;              // level new, tracer 0
-  const double *mfx, *mfy, *cflx, *cfly, *old_m, *m, *wgt;
+  const double *m, *wgt;
   int64_t tstride;
   int *flags;
   unsigned long long *fixes;
   int damp_passes;
 };
 
-// outer fluxes + adv_run's update of one row (ffsl_mod.F90:410,
-// adv_mod.F90:183-301) for full row r of tracer blockIdx.y
-__global__ void __launch_bounds__(NA) k_adv_outer(AdvGrid g, Mesh M, int lim, OuterPtrs p) {
+// positivity fixer and longitude damping of one row of one tracer
+// (adv_mod.F90:236-301), block (r, t) = (blockIdx.x, blockIdx.y)
+__global__ void __launch_bounds__(NA) k_adv_fix(AdvGrid g, FixPtrs p) {
   extern __shared__ double lds[];
   const int n = g.nlon, r = blockIdx.x;
   const int64_t toff = blockIdx.y * p.tstride;
-  const double *qx = p.qx + toff, *qy = p.qy + toff, *q = p.q + toff;
   double *qn = p.qn + toff;
-  double *srow = lds, *sml = lds + n, *sdm = lds + 2 * n, *s6 = lds + 3 * n, *sfx = lds + 4 * n;
-  double *sb = lds + 5 * n;  // [0] pole sum, [1] seam west, [2] seam east, [3] flag
-  const double dt = g.dt;
+  double *sq = lds, *neg = lds + n, *sd = lds + 2 * n, *sb = lds + 3 * n;
   // the longitude halo of the new level still holds the values of its last
   // fill_halo: the row's own last and first values, read before overwriting
   if (threadIdx.x == 0) {
     sb[1] = qn[r * g.pitch + n - 1];
     sb[2] = qn[r * g.pitch];
   }
-  if (r > 0 && r < g.nlat - 1) {
-    for (int i = threadIdx.x; i < n; i += NA) srow[i] = qy[r * g.pitch + i];
-    __syncthreads();
-    ppm_row(g, lim, srow, sml, sdm, s6);
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += NA)
-      sfx[i] = xflux(g, p.cflx[r * g.pitch + i], p.mfx[r * g.pitch + i], i, srow, sml, sdm, s6);
-    __syncthreads();
-    const double lel = M(GMC_LE_LON, r), ac = M(GMC_AREA_CELL, r);
-    const double ls = M(GMC_LE_LAT, r - 1), ln = M(GMC_LE_LAT, r);
-    for (int i = threadIdx.x; i < n; i += NA) {
-      const double fs = yflux(g, lim, p.cfly[(r - 1) * g.pitch + i], p.mfy[(r - 1) * g.pitch + i], r - 1, i, qx);
-      const double fn = yflux(g, lim, p.cfly[r * g.pitch + i], p.mfy[r * g.pitch + i], r, i, qx);
-      const int64_t k = r * g.pitch + i;
-      const double qv = p.old_m[k] * q[k] -
-                        ((sfx[i] - sfx[wrap(i - 1, n)]) * lel + (fn * ln - fs * ls)) / ac * dt;
-      sml[i] = qv / p.m[k];  // the x-flux coefficients are no longer needed
-    }
-  } else {
-    const int h = r == 0 ? 0 : g.nhalf - 1;
-    for (int i = threadIdx.x; i < n; i += NA)
-      sml[i] = yflux(g, lim, p.cfly[h * g.pitch + i], p.mfy[h * g.pitch + i], h, i, qx);
-    double pole = block_zsum(g, sml, sb);
-    pole = pole * M(GMC_LE_LAT, h) / n / M(GMC_AREA_CELL, r) * dt;
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += NA) {
-      const int64_t k = r * g.pitch + i;
-      const double qv = r == 0 ? p.old_m[k] * q[k] - pole : p.old_m[k] * q[k] + pole;
-      sml[i] = qv / p.m[k];
-    }
-  }
-  // positivity fixer (adv_mod.F90:236-258): sequential along the row, run
-  // by one lane only when the row holds a negative value
-  double *sq = sml;
+  for (int i = threadIdx.x; i < n; i += NA) sq[i] = p.qp[toff + r * g.pitch + i];
+  __syncthreads();
+  // positivity fixer (adv_mod.F90:236-258).  The reference walks the row
+  // once from west to east and fixes each value that is still negative when
+  // reached, from its two neighbours.  A fix never makes a neighbour
+  // negative and touches only columns i-1 .. i+1, so negative columns more
+  // than two apart never see each other's fixes: the originally negative
+  // columns split into clusters with gaps of at most one column, and one
+  // lane walks each cluster west to east, in parallel with the others.  The
+  // first and last columns use the seam values (the new level's longitude
+  // halo) in place of each other.
   const double *mrow = p.m + r * g.pitch;
+  for (int i = threadIdx.x; i < n; i += NA) neg[i] = sq[i] < 0 ? 1.0 : 0.0;
   __syncthreads();
-  if (threadIdx.x == 0) sb[3] = 0.0;
-  __syncthreads();
-  int neg = 0;
-  for (int i = threadIdx.x; i < n; i += NA) neg |= sq[i] < 0;
-  if (neg) sb[3] = 1.0;
-  __syncthreads();
-  if (sb[3] != 0.0 && threadIdx.x == 0) {
-    double west = sb[1], east = sb[2];
+  {
     unsigned long long nfix = 0;
     int bad = 0;
-    for (int i = 0; i < n; i++) {
-      if (!(sq[i] < 0)) continue;
-      const int im = i == 0 ? n - 1 : i - 1, ip = i == n - 1 ? 0 : i + 1;
-      const double qm0 = sq[i] * mrow[i];
-      const double qm1 = (i == 0 ? west : sq[im]) * mrow[im];
-      const double qm2 = (i == n - 1 ? east : sq[ip]) * mrow[ip];
-      const double qm0_half = 0.5 * qm0;
-      if (qm1 >= qm0_half && qm2 >= qm0_half) {
-        const double a = (qm1 - qm0_half) / mrow[im], b = (qm2 - qm0_half) / mrow[ip];
-        if (i == 0) west = a; else sq[im] = a;
-        if (i == n - 1) east = b; else sq[ip] = b;
-      } else if (qm1 > qm0) {
-        const double a = (qm1 - qm0) / mrow[im];
-        if (i == 0) west = a; else sq[im] = a;
-      } else if (qm2 > qm0) {
-        const double b = (qm2 - qm0) / mrow[ip];
-        if (i == n - 1) east = b; else sq[ip] = b;
-      } else {
-        bad = 1;
+    double west = sb[1], east = sb[2];
+    for (int i0 = threadIdx.x; i0 < n; i0 += NA) {
+      if (neg[i0] == 0.0 || (i0 >= 1 && neg[i0 - 1] != 0.0) || (i0 >= 2 && neg[i0 - 2] != 0.0)) continue;
+      for (int i = i0;;) {
+        if (sq[i] < 0) {
+          const int im = i == 0 ? n - 1 : i - 1, ip = i == n - 1 ? 0 : i + 1;
+          const double qm0 = sq[i] * mrow[i];
+          const double qm1 = (i == 0 ? west : sq[im]) * mrow[im];
+          const double qm2 = (i == n - 1 ? east : sq[ip]) * mrow[ip];
+          const double qm0_half = 0.5 * qm0;
+          if (qm1 >= qm0_half && qm2 >= qm0_half) {
+            const double a = (qm1 - qm0_half) / mrow[im], b = (qm2 - qm0_half) / mrow[ip];
+            if (i == 0) west = a; else sq[im] = a;
+            if (i == n - 1) east = b; else sq[ip] = b;
+          } else if (qm1 > qm0) {
+            const double a = (qm1 - qm0) / mrow[im];
+            if (i == 0) west = a; else sq[im] = a;
+          } else if (qm2 > qm0) {
+            const double b = (qm2 - qm0) / mrow[ip];
+            if (i == n - 1) east = b; else sq[ip] = b;
+          } else {
+            bad = 1;
+          }
+          sq[i] = 0;
+          nfix++;
+        }
+        if (i + 1 < n && neg[i + 1] != 0.0) i = i + 1;
+        else if (i + 2 < n && neg[i + 2] != 0.0) i = i + 2;
+        else break;
       }
-      sq[i] = 0;
-      nfix++;
     }
-    atomicAdd(p.fixes, nfix);
+    if (nfix) atomicAdd(p.fixes, nfix);
     if (bad) atomicOr(p.flags, F_NEG);
   }
   __syncthreads();
@@ -422,7 +519,7 @@ __global__ void __launch_bounds__(NA) k_adv_outer(AdvGrid g, Mesh M, int lim, Ou
   // the row with periodic neighbours
   const double w = (r > 0 && r < g.nlat - 1) ? p.wgt[r] : 0.0;
   if (w != 0.0) {
-    double *a = sq, *b = sdm;
+    double *a = sq, *b = sd;
     for (int pass = 0; pass < p.damp_passes; pass++) {
       for (int i = threadIdx.x; i < n; i += NA)
         b[i] = w * (a[i] + 0.25 * (a[wrap(i - 1, n)] - 2.0 * a[i] + a[wrap(i + 1, n)])) + (1 - w) * a[i];
@@ -468,7 +565,7 @@ struct gmc_adv {
   double *ub = nullptr, *vb = nullptr;
   double *u = nullptr, *v = nullptr, *mfx = nullptr, *mfy = nullptr, *old_m = nullptr;
   double *cflx = nullptr, *cfly = nullptr, *divx = nullptr, *divy = nullptr;
-  double *q = nullptr, *qx = nullptr, *qy = nullptr;  // q: [2][ntracers][nlat][pitch]
+  double *q = nullptr, *qx = nullptr, *qy = nullptr, *qp = nullptr;  // q: [2][ntracers][nlat][pitch]
   double *d_rows = nullptr;
   int *d_flags = nullptr;
   unsigned long long *d_fixes = nullptr;
@@ -521,26 +618,29 @@ int run_once(gmc_adv *a, int64_t time_step, bool basis, double scale) {
   const int mode = a->uv_step == 0 ? 0 : (a->uv_step == ns ? 1 : 2);
   AccumPtrs ap{a->u, a->v, a->mfx, a->mfy, a->su, a->sv, a->smfx, a->smfy,
                basis ? a->ub : nullptr, basis ? a->vb : nullptr};
-  hipLaunchKernelGGL(k_adv_accum, dim3(g.nlat), dim3(NA), 0, a->stream, g, ap, mode, ns, scale);
+  const dim3 rows((g.nlon + NA - 1) / NA, g.nlat);
+  hipLaunchKernelGGL(k_adv_accum, rows, dim3(NA), 0, a->stream, g, ap, mode, ns, scale);
   mark(a, KA_ACCUM);
   a->uv_step = mode == 0 ? 1 : (mode == 1 ? 0 : a->uv_step + 1);
   a->mf_step = a->uv_step;
   if (a->uv_step == 0) {
     CflPtrs cp{a->u, a->v, a->cflx, a->cfly, a->divx, a->divy, a->d_flags};
-    hipLaunchKernelGGL(k_adv_cfl, dim3(g.nlat), dim3(NA), 0, a->stream, g, M, cp);
+    hipLaunchKernelGGL(k_adv_cfl, rows, dim3(NA), 0, a->stream, g, M, cp);
     mark(a, KA_CFL);
   }
   if (time_step > 0 && time_step % ns == 0) {
     const int64_t ts = (int64_t)g.nlat * g.pitch;
-    const dim3 grid(g.nlat, a->cfg.ntracers);
+    const int ntr = a->cfg.ntracers;
+    const dim3 grid(flux_blocks(g, ntr));
     InnerPtrs ip{tracer_ptr(a, a->old, 0), a->qx, a->qy, a->u, a->v, a->cflx, a->cfly, a->divx, a->divy, ts};
-    hipLaunchKernelGGL(k_adv_inner, grid, dim3(NA), a->lds_bytes, a->stream, g, M, a->cfg.limiter, ip);
+    hipLaunchKernelGGL(k_adv_inner, grid, dim3(NF), 0, a->stream, g, M, a->cfg.limiter, ntr, ip);
     mark(a, KA_INNER);
-    OuterPtrs op{a->qx, a->qy, tracer_ptr(a, a->old, 0), tracer_ptr(a, a->new_, 0), a->mfx, a->mfy,
-                 a->cflx, a->cfly, a->old_m, a->sm, a->d_wgt, ts, a->d_flags, a->d_fixes,
-                 a->cfg.lon_damp_passes};
-    hipLaunchKernelGGL(k_adv_outer, grid, dim3(NA), a->lds_bytes, a->stream, g, M, a->cfg.limiter, op);
-    mark(a, KA_OUTER);
+    OuterPtrs op{a->qx, a->qy, tracer_ptr(a, a->old, 0), a->qp, a->mfx, a->mfy, a->cflx, a->cfly, a->old_m, a->sm, ts};
+    hipLaunchKernelGGL(k_adv_flux, grid, dim3(NF), 0, a->stream, g, M, a->cfg.limiter, ntr, op);
+    mark(a, KA_FLUX);
+    FixPtrs fp{a->qp, tracer_ptr(a, a->new_, 0), a->sm, a->d_wgt, ts, a->d_flags, a->d_fixes, a->cfg.lon_damp_passes};
+    hipLaunchKernelGGL(k_adv_fix, dim3(g.nlat, ntr), dim3(NA), a->lds_bytes, a->stream, g, fp);
+    mark(a, KA_FIX);
     std::swap(a->old, a->new_);
   }
   int rc = ahip(a, hipGetLastError(), "advection launch");
@@ -593,11 +693,11 @@ int gmc_adv_create(const gmc_mesh *mesh, const gmc_adv_config *cfg, gmc_adv **ou
   g.dlon = mesh->dlon;
   g.radius = cfg->radius;
   g.dt = cfg->dt_adv;
-  a->lds_bytes = sizeof(double) * (5 * (size_t)nlon + 8);
+  a->lds_bytes = sizeof(double) * (3 * (size_t)nlon + 8);
   const size_t fld = (size_t)nlat * g.pitch;
   const int ntr = cfg->ntracers;
-  // 16 single fields + q (2 levels) + qx, qy per tracer
-  const size_t nd = fld * (16 + 4 * (size_t)ntr) + GMC_MESH_NFIELDS * (size_t)nlat + nlat +
+  // 16 single fields + q (2 levels) + qx, qy, qp per tracer
+  const size_t nd = fld * (16 + 5 * (size_t)ntr) + GMC_MESH_NFIELDS * (size_t)nlat + nlat +
                     (size_t)ntr * nlat;
   a->slab_bytes = nd * sizeof(double) + 256;
   int rc = ahip(a, hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking), "stream");
@@ -614,6 +714,7 @@ int gmc_adv_create(const gmc_mesh *mesh, const gmc_adv_config *cfg, gmc_adv **ou
   for (auto pp : f) { *pp = p; p += fld; }
   a->q = p; p += 2 * fld * ntr;
   a->qx = p; p += fld * ntr;
+  a->qp = p; p += fld * ntr;
   a->qy = p; p += fld * ntr;
   a->d_mesh = p; p += GMC_MESH_NFIELDS * (size_t)nlat;
   a->d_wgt = p; p += nlat;
@@ -622,8 +723,7 @@ int gmc_adv_create(const gmc_mesh *mesh, const gmc_adv_config *cfg, gmc_adv **ou
   a->d_fixes = (unsigned long long *)(p + 2);
   rc = ahip(a, hipMemcpy(a->d_mesh, mesh->metrics, sizeof(double) * GMC_MESH_NFIELDS * nlat, hipMemcpyHostToDevice), "mesh");
   if (!rc) rc = ahip(a, hipMemcpy(a->d_wgt, cfg->lon_damp_wgt, sizeof(double) * nlat, hipMemcpyHostToDevice), "weights");
-  if (!rc) rc = ahip(a, hipFuncSetAttribute((const void *)k_adv_inner, hipFuncAttributeMaxDynamicSharedMemorySize, (int)a->lds_bytes), "lds");
-  if (!rc) rc = ahip(a, hipFuncSetAttribute((const void *)k_adv_outer, hipFuncAttributeMaxDynamicSharedMemorySize, (int)a->lds_bytes), "lds");
+  if (!rc) rc = ahip(a, hipFuncSetAttribute((const void *)k_adv_fix, hipFuncAttributeMaxDynamicSharedMemorySize, (int)a->lds_bytes), "lds");
   if (rc) {
     gmc_adv_destroy(a);
     return rc;

@@ -157,10 +157,18 @@ int gmc_create(const gmc_mesh *mesh, const gmc_config *cfg, gmc_ctx **out) {
                                        hipMemcpyHostToDevice), "mesh copy");
   if (!rc) {
     // row records of the metric table (zero records before the first and
-    // after the last row): one wavefront reads its rows with a few wide loads
-    std::vector<double> rows((size_t)(g.nlat + 2) * GMC_MESH_NFIELDS, 0.0);
-    for (int r = 0; r < g.nlat; r++)
-      for (int f = 0; f < GMC_MESH_NFIELDS; f++) rows[(size_t)(r + 1) * GMC_MESH_NFIELDS + f] = mesh->metrics[f * g.nlat + r];
+    // after the last row): one wavefront reads its rows with a few wide loads.
+    // Each record ends with the correctly rounded reciprocals of the divisor
+    // fields (kRecipFields; 0 for a zero entry, which no formula divides by).
+    std::vector<double> rows((size_t)(g.nlat + 2) * kRowRec, 0.0);
+    for (int r = 0; r < g.nlat; r++) {
+      double *rec = &rows[(size_t)(r + 1) * kRowRec];
+      for (int f = 0; f < GMC_MESH_NFIELDS; f++) rec[f] = mesh->metrics[f * g.nlat + r];
+      for (int k = 0; k < kNumRecip; k++) {
+        const double d = rec[kRecipFields[k]];
+        rec[GMC_MESH_NFIELDS + k] = d != 0.0 ? 1.0 / d : 0.0;
+      }
+    }
     rc = check_hip(c, hipMalloc((void **)&c->d_mrows, sizeof(double) * rows.size()), "mesh rows alloc");
     if (!rc) rc = check_hip(c, hipMemcpy(c->d_mrows, rows.data(), sizeof(double) * rows.size(), hipMemcpyHostToDevice),
                             "mesh rows copy");

@@ -41,7 +41,7 @@ template <int TY_> struct Tile {
   static constexpr int NT = LR * kWave;
   static constexpr int NE = LR * LC;
   static constexpr int SH_TILES = 7 * NE;                 // LDS field tiles
-  static constexpr int SH_MESH = LR * GMC_MESH_NFIELDS;   // metric rows of the tile
+  static constexpr int SH_MESH = LR * kRowRec;            // metric row records of the tile
   static constexpr int SH_ALL = SH_TILES + SH_MESH + 8;
   static constexpr int kMaxPoleLon = SH_TILES / 3;        // the pole sums keep 3 row terms in the tile area
 };
@@ -53,11 +53,12 @@ struct StagePtrs {
 
 // pv at vertex (r, i) from global memory: rows reached through the
 // cross-pole mirror next to the poles
-__device__ double pv_global(const Grid &g, const Mesh &M, double grav, int stokes, const double *pole,
-                            const StagePtrs &P, int r, int i) {
+template <class MT, class PT>
+__device__ double pv_global(const Grid &g, const MT &M, double grav, double rgrav, int stokes, const double *pole,
+                            const PT &P, int r, int i) {
   const int64_t p = g.pitch;
   const int ip = wrap(i + 1, g.nlon);
-  auto m = [&](int rr, int ii) { return m_of(P.gz_star[rr * p + ii], P.gzs[rr * p + ii], grav); };
+  auto m = [&](int rr, int ii) { return div_rn(P.gz_star[rr * p + ii] - P.gzs[rr * p + ii], grav, rgrav); };
   double vor;
   if (stokes && r == 0) vor = pole[4];
   else if (stokes && r == g.nhalf - 1) vor = pole[5];
@@ -70,8 +71,8 @@ __device__ double pv_global(const Grid &g, const Mesh &M, double grav, int stoke
 // terms go to LDS scratch, one wavefront adds them in the fixed order of
 // wave_zonal_sum.  pole[north] = ke, pole[2+north] = dmfdlat at the pole
 // cell, pole[4+north] = vor of the vertex row next to the pole.
-template <int NT>
-__device__ void pole_sums(const Grid &g, const Mesh &M, double grav, const StagePtrs &P, int north,
+template <int NT, class MT, class PT>
+__device__ void pole_sums(const Grid &g, const MT &M, double grav, double rgrav, const PT &P, int north,
                           double *scratch, double *pole) {
   const int tid = threadIdx.x, n = g.nlon;
   const int64_t p = g.pitch;
@@ -79,7 +80,7 @@ __device__ void pole_sums(const Grid &g, const Mesh &M, double grav, const Stage
   const double le = M(GMC_LE_LAT, vr), ac = M(GMC_AREA_CELL, cr);
   const double w1 = M(GMC_HALF_TANGENT_WGT1, vr), w2 = M(GMC_HALF_TANGENT_WGT2, vr);
   double *t0 = scratch, *t1 = scratch + n, *t2 = scratch + 2 * n;
-  auto m = [&](int r, int i) { return m_of(P.gz_star[r * p + i], P.gzs[r * p + i], grav); };
+  auto m = [&](int r, int i) { return div_rn(P.gz_star[r * p + i] - P.gzs[r * p + i], grav, rgrav); };
   for (int i = tid; i < n; i += NT) {
     const int im = wrap(i - 1, n), ip = wrap(i + 1, n);
     const double v = P.v_f[vr * p + i];
@@ -111,19 +112,72 @@ __device__ void pole_sums(const Grid &g, const Mesh &M, double grav, const Stage
 }
 
 // Metric rows of one tile in LDS (row records, tile row k = r - r0):
-// M(f, r) is a broadcast LDS read for a wave-uniform r.
+// M(f, r) is a broadcast LDS read for a wave-uniform r.  The records carry
+// the reciprocals of the divisor fields, so M.div divides through div_rn.
 struct MeshTile {
   const double *s;
   int r0;
-  __device__ __forceinline__ double operator()(int f, int r) const { return s[(r - r0) * GMC_MESH_NFIELDS + f]; }
+  __device__ __forceinline__ double operator()(int f, int r) const { return s[(r - r0) * kRowRec + f]; }
+  __device__ __forceinline__ double div(double x, int f, int r) const {
+    const double *rec = s + (r - r0) * kRowRec;
+    return div_rn(x, rec[f], rec[GMC_MESH_NFIELDS + recip_slot(f)]);
+  }
+};
+
+// The global loads of one tile, per lane: gz, gzs, u_f, v_f of its tile
+// point, one metric record word, and gz_old of its output point.
+struct TileIn {
+  double gz, gzs, uf, vf, rec, gz_old;
+};
+
+template <class S, class PT, class RT>
+__device__ __forceinline__ TileIn tile_fetch(const Grid &g, RT mrows, const PT &P, int C0, int R0) {
+  const int64_t p = g.pitch;
+  const int k = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int q = threadIdx.x % kWave;
+  const int r = R0 - 2 + k;
+  const int i = wrap(C0 - HXL + q, g.nlon);
+  const int ic = C0 + q - HXL;
+  TileIn in{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (k >= 2 && k < S::TY + 2 && q >= HXL && q < HXL + TX && ic < g.nlon && r < g.nlat) in.gz_old = P.gz_old[r * p + ic];
+  if (r >= 0 && r < g.nlat) {
+    in.gz = P.gz_star[r * p + i];
+    in.gzs = P.gzs[r * p + i];
+    in.uf = P.u_f[r * p + i];
+  }
+  if (r >= 0 && r < g.nhalf) in.vf = P.v_f[r * p + i];
+  if (threadIdx.x < S::SH_MESH) {
+    const int rr = R0 - 2 + (int)threadIdx.x / kRowRec;
+    if (rr >= 0 && rr < g.nlat) in.rec = mrows[(int64_t)(rr + 1) * kRowRec + threadIdx.x % kRowRec];
+  }
+  return in;
+}
+
+// Pole sums of the star state into the tile's pole slots, in the tiles that
+// read them (a uniform branch per workgroup).  Runs before the tile's loads
+// are fetched: it uses the tile's LDS as scratch.
+template <class S, class MT, class PT>
+__device__ __forceinline__ void tile_poles(const Grid &g, const MT &Mg, double grav, double rgrav, const PT &P, int R0,
+                                           double *sh) {
+  double *pole = sh + S::SH_TILES + S::SH_MESH;
+  if (R0 <= 2) pole_sums<S::NT>(g, Mg, grav, rgrav, P, 0, sh, pole);
+  if (R0 + S::TY + 3 >= g.nlat - 1) pole_sums<S::NT>(g, Mg, grav, rgrav, P, 1, sh, pole);
+}
+
+// no next tile to fetch
+struct NoNext {
+  __device__ __forceinline__ bool operator()(TileIn &) const { return false; }
 };
 
 // One tile of the forward tendencies and the unfiltered gz update: output
-// columns C0 .. C0+TX-1, rows R0 .. R0+TY-1, LDS scratch sh[S::SH_ALL].
-template <class S>
-__device__ __forceinline__ void tend_tile(const Grid &g, const Mesh &Mg, const double *__restrict__ mrows, double grav,
+// columns C0 .. C0+TX-1, rows R0 .. R0+TY-1, LDS scratch sh[S::SH_ALL].  The
+// tile's pole slots hold its pole sums (tile_poles) and io its loads
+// (tile_fetch) on entry; once those are in LDS, next(io) may start the loads
+// of the workgroup's next tile into io, which then overlap phases B to E.
+template <class S, class MT, class PT, class Next>
+__device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double grav, double rgrav,
                                           double dt, int stokes, int pv_scheme, int order, double wgt, int coriolis,
-                                          const StagePtrs &P, int C0, int R0, double *sh) {
+                                          const PT &P, int C0, int R0, double *sh, TileIn &io, const Next &next) {
   double(*sm)[LC] = (double(*)[LC])(sh + 0 * S::NE);
   double(*suf)[LC] = (double(*)[LC])(sh + 1 * S::NE);
   double(*svf)[LC] = (double(*)[LC])(sh + 2 * S::NE);
@@ -134,7 +188,7 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const Mesh &Mg, const d
   double(*spl)[LC] = suf;  // pv_lon replaces u_f element by element (phase D)
   double(*spt)[LC] = svf;  // pv_lat replaces v_f element by element (phase D)
   double *sme = sh + S::SH_TILES;
-  double *pole = sh + S::SH_TILES + S::SH_MESH;
+  const double *pole = sh + S::SH_TILES + S::SH_MESH;
 
   const int64_t p = g.pitch;
   const int nlat = g.nlat, nhalf = g.nhalf;
@@ -143,28 +197,17 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const Mesh &Mg, const d
   const int r = R0 - 2 + k;                                            // global row (wave-uniform)
   const int i = wrap(C0 - HXL + q, g.nlon);                            // global column
 
-  // 0: pole sums in the tiles that read them (uniform branch per block)
-  if (R0 <= 2) pole_sums<S::NT>(g, Mg, grav, P, 0, sh, pole);
-  if (R0 + S::TY + 3 >= nlat - 1) pole_sums<S::NT>(g, Mg, grav, P, 1, sh, pole);
+  // output point of this lane (phase E)
+  const int ic = C0 + q - HXL;
+  const bool out_pt = k >= 2 && k < S::TY + 2 && q >= HXL && q < HXL + TX && ic < g.nlon && r < nlat;
+  const double gz_old = io.gz_old;
 
   // A: m, u_f, v_f of tile row k and the tile's metric rows
-  {
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
-    if (r >= 0 && r < nlat) {
-      a0 = P.gz_star[r * p + i];
-      a1 = P.gzs[r * p + i];
-      a2 = P.u_f[r * p + i];
-    }
-    if (r >= 0 && r < nhalf) a3 = P.v_f[r * p + i];
-    if (threadIdx.x < S::SH_MESH) {
-      const int rr = R0 - 2 + (int)threadIdx.x / GMC_MESH_NFIELDS;
-      if (rr >= 0 && rr < nlat) a4 = mrows[(int64_t)(rr + 1) * GMC_MESH_NFIELDS + threadIdx.x % GMC_MESH_NFIELDS];
-    }
-    sm[k][q] = (r >= 0 && r < nlat) ? m_of(a0, a1, grav) : 0.0;
-    suf[k][q] = a2;
-    svf[k][q] = a3;
-    if (threadIdx.x < S::SH_MESH) sme[threadIdx.x] = a4;
-  }
+  sm[k][q] = (r >= 0 && r < nlat) ? div_rn(io.gz - io.gzs, grav, rgrav) : 0.0;   // m_of
+  suf[k][q] = io.uf;
+  svf[k][q] = io.vf;
+  if (threadIdx.x < S::SH_MESH) sme[threadIdx.x] = io.rec;
+  next(io);
   const MeshTile M{sme, R0 - 2};
   __syncthreads();
 
@@ -184,7 +227,7 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const Mesh &Mg, const d
     } else if (r >= -2 && r <= nhalf + 1) {
       int rr = r, ii = i;
       mirror_half(rr, ii, g);
-      v = pv_global(g, Mg, grav, stokes, pole, P, rr, ii);
+      v = pv_global(g, Mg, grav, rgrav, stokes, pole, P, rr, ii);
     }
     spv[k][q] = v;
   }
@@ -221,17 +264,15 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const Mesh &Mg, const d
   __syncthreads();
 
   // E: du, dv (forward pass) and gz_new = gz_old + dt dgz on the output rows
-  const int ic = C0 + q - HXL;
-  if (k >= 2 && k < S::TY + 2 && q >= HXL && q < HXL + TX && ic < g.nlon && r < nlat) {
+  if (out_pt) {
     const int64_t o = r * p + ic;
-    const double gz_old = P.gz_old[o];
     if (r >= 1 && r <= nlat - 2) {
       const double qhv = coriolis == 1
           ? qhv_of(M, r, spl[k][q], smfy[k - 1][q], spt[k - 1][q], smfy[k - 1][q + 1], spt[k - 1][q + 1],
                    smfy[k][q], spt[k][q], smfy[k][q + 1], spt[k][q + 1])
           : tangent_sum(M(GMC_FULL_TANGENT_WGT1, r), M(GMC_FULL_TANGENT_WGT2, r), smfy[k - 1][q],
                         smfy[k - 1][q + 1], smfy[k][q], smfy[k][q + 1]) * spl[k][q];
-      const double dk = (ske[k][q + 1] - ske[k][q]) / M(GMC_DE_LON, r);
+      const double dk = M.div(ske[k][q + 1] - ske[k][q], GMC_DE_LON, r);
       P.du[o] = qhv - dk;
     }
     if (r < nhalf) {
@@ -240,7 +281,7 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const Mesh &Mg, const d
                    smfx[k + 1][q - 1], spl[k + 1][q - 1], smfx[k + 1][q], spl[k + 1][q])
           : tangent_sum(M(GMC_HALF_TANGENT_WGT1, r), M(GMC_HALF_TANGENT_WGT2, r), smfx[k][q - 1], smfx[k][q],
                         smfx[k + 1][q - 1], smfx[k + 1][q]) * spt[k][q];
-      const double dk = (ske[k + 1][q] - ske[k][q]) / M(GMC_DE_LAT, r);
+      const double dk = M.div(ske[k + 1][q] - ske[k][q], GMC_DE_LAT, r);
       P.dv[o] = -qhu - dk;
     }
     double dmfdlon = 0.0, dmfdlat;
@@ -264,7 +305,8 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const Mesh &Mg, const d
 // XCD holding the north pole walks its run backwards so that the polar tiles,
 // which also form the pole sums, start first.
 __global__ void __launch_bounds__(Tile<TY_MAIN>::NT, 8) k_tend(Grid g, Mesh Mg, const double *__restrict__ mrows,
-                                                             double grav, double dt, int stokes, int pv_scheme,
+                                                             double grav, double rgrav, double dt, int stokes,
+                                                             int pv_scheme,
                                                              int order, double wgt, int coriolis, StagePtrs P) {
   using S = Tile<TY_MAIN>;
   __shared__ double sh[S::SH_ALL];
@@ -277,7 +319,54 @@ __global__ void __launch_bounds__(Tile<TY_MAIN>::NT, 8) k_tend(Grid g, Mesh Mg, 
   const int t = start + (x == NXCD - 1 ? cnt - 1 - s : s);
   const int C0 = (t % nx) * TX, R0 = (t / nx) * S::TY;
 
-  tend_tile<S>(g, Mg, mrows, grav, dt, stokes, pv_scheme, order, wgt, coriolis, P, C0, R0, sh);
+  tile_poles<S>(g, Mg, grav, rgrav, P, R0, sh);
+  TileIn io = tile_fetch<S>(g, mrows, P, C0, R0);
+  tend_tile<S>(g, Mg, grav, rgrav, dt, stokes, pv_scheme, order, wgt, coriolis, P, C0, R0, sh, io, NoNext{});
+}
+
+// k_tend as a persistent launch of G workgroups (G a multiple of NXCD, at
+// most the tile count): the workgroups of XCD x stride over that XCD's run
+// of tiles, and each starts the global loads of its next tile as soon as
+// the current tile's inputs are in LDS, so they overlap its computation.
+__global__ void __launch_bounds__(Tile<TY_MAIN>::NT, 8) k_tend_pf(Grid g, Mesh Mg, const double *__restrict__ mrows,
+                                                                double grav, double rgrav, double dt, int stokes,
+                                                                int pv_scheme, int order, double wgt, int coriolis,
+                                                                StagePtrs P) {
+  using S = Tile<TY_MAIN>;
+  __shared__ double sh[S::SH_ALL];
+  const int nx = (g.nlon + TX - 1) / TX, ny = (g.nlat + S::TY - 1) / S::TY, T = nx * ny;
+  const int x = blockIdx.x % NXCD, per = (gridDim.x - x + NXCD - 1) / NXCD;
+  int start = 0;
+  for (int y = 0; y < x; y++) start += (T - y + NXCD - 1) / NXCD;
+  const int cnt = (T - x + NXCD - 1) / NXCD;
+  int s = blockIdx.x / NXCD;
+  if (s >= cnt) return;
+  auto tile = [&](int si) { return start + (x == NXCD - 1 ? cnt - 1 - si : si); };
+  // polar tiles (which form the pole sums first) are fetched when they start
+  auto polar = [&](int tt) { const int R0 = (tt / nx) * S::TY; return R0 <= 2 || R0 + S::TY + 3 >= g.nlat - 1; };
+  int t = tile(s);
+  TileIn io{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  bool fetched = false;
+  for (;;) {
+    const int C0 = (t % nx) * TX, R0 = (t / nx) * S::TY;
+    if (!fetched) {
+      tile_poles<S>(g, Mg, grav, rgrav, P, R0, sh);
+      io = tile_fetch<S>(g, mrows, P, C0, R0);
+    }
+    const int sn = s + per;
+    const bool more = sn < cnt;
+    const int tn = more ? tile(sn) : t;
+    fetched = more && !polar(tn);
+    auto next = [&](TileIn &in) {
+      if (fetched) in = tile_fetch<S>(g, mrows, P, (tn % nx) * TX, (tn / nx) * S::TY);
+      return fetched;
+    };
+    tend_tile<S>(g, Mg, grav, rgrav, dt, stokes, pv_scheme, order, wgt, coriolis, P, C0, R0, sh, io, next);
+    if (!more) break;
+    __syncthreads();
+    s = sn;
+    t = tn;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -293,8 +382,9 @@ __device__ __forceinline__ int wrap2(int c, int n) { return c < 0 ? c + n : (c >
 // big filter gz_new -> gz_f and small filter gz_new -> gz
 // (time_schemes_mod.F90:184-191, filter_mod.F90:25-50).  Weights and the
 // row window are staged in LDS: xs = [weights big | weights small | window].
-__device__ __forceinline__ void gzf_item(const Grid &g, const FilterDev &big, const FilterDev &small, const Item &it,
-                                         const double *__restrict__ gz_new, double *gz_f, double *gz, double *xs) {
+template <class FT, class CP, class WP>
+__device__ __forceinline__ void gzf_item(const Grid &g, const FT &big, const FT &small, const Item &it, CP gz_new,
+                                         WP gz_f, WP gz, double *xs) {
   const int r = it.row;
   const int64_t o = r * g.pitch;
   const int nb = big.ngrid_lon[r], ns = small.ngrid_lon[r];
@@ -334,8 +424,9 @@ struct UVPtrs {
 // backward pass + u/v update + big filters
 // (gmcore_mod.F90:552-571, time_schemes_mod.F90:198-239).  xs = [weights |
 // window of the updated row].
-__device__ __forceinline__ void uvf_item(const Grid &g, const Mesh &M, double dt, const FilterDev &big,
-                                         const Item &it, const UVPtrs &P, double *xs) {
+template <class MT, class FT, class PT>
+__device__ __forceinline__ void uvf_item(const Grid &g, const MT &M, double dt, const FT &big,
+                                         const Item &it, const PT &P, double *xs) {
   const int r = it.row;
   const int64_t p = g.pitch, o = r * p;
   const int nl = g.nlon;
@@ -347,10 +438,10 @@ __device__ __forceinline__ void uvf_item(const Grid &g, const Mesh &M, double dt
   const int n = it.kind == 1 ? big.ngrid_lon[r] : big.ngrid_lat[r];
   const int h = n > 1 ? (n - 1) / 2 : 0;
   const int len = it.c1 - it.c0 + 2 * h;
-  const double *wg = (it.kind == 1 ? big.wgt_lon : big.wgt_lat) + (int64_t)r * big.max_taps;
+  const auto wg = (it.kind == 1 ? big.wgt_lon : big.wgt_lat) + (int64_t)r * big.max_taps;
   for (int k = threadIdx.x; k < n; k += NF) w[k] = wg[k];
   if (it.kind == 1) {
-    const double de = M(GMC_DE_LON, r);
+    const double de = M(GMC_DE_LON, r), rde = 1.0 / de;
     for (int t = threadIdx.x; t < len; t += 4 * NF) {
       double uo[4], du[4], ge[4], gw[4];
 #pragma unroll
@@ -364,11 +455,12 @@ __device__ __forceinline__ void uvf_item(const Grid &g, const Mesh &M, double dt
         }
 #pragma unroll
       for (int u = 0; u < 4; u++)
-        if (t + u * NF < len) x0[t + u * NF] = uo[u] + dt * (du[u] - (ge[u] - gw[u]) / de);
+        if (t + u * NF < len) x0[t + u * NF] = uo[u] + dt * (du[u] - div_rn(ge[u] - gw[u], de, rde));
     }
   } else {
     const int pole = r == 0 ? 1 : (r == g.nhalf - 1 ? -1 : 0);   // neighbour row of the smoothing
     const double de0 = M(GMC_DE_LAT, r), de1 = pole ? M(GMC_DE_LAT, r + pole) : 1.0;
+    const double rde0 = 1.0 / de0, rde1 = 1.0 / de1;
     for (int t = threadIdx.x; t < len; t += 2 * NF) {
       double a[2][8];
 #pragma unroll
@@ -384,14 +476,15 @@ __device__ __forceinline__ void uvf_item(const Grid &g, const Mesh &M, double dt
 #pragma unroll
       for (int u = 0; u < 2; u++)
         if (t + u * NF < len) {
-          double v = a[u][0] + dt * (a[u][1] - (a[u][2] - a[u][3]) / de0);
-          if (pole) v = pole_smooth(v, a[u][4] + dt * (a[u][5] - (a[u][6] - a[u][7]) / de1));
+          double v = a[u][0] + dt * (a[u][1] - div_rn(a[u][2] - a[u][3], de0, rde0));
+          if (pole) v = pole_smooth(v, a[u][4] + dt * (a[u][5] - div_rn(a[u][6] - a[u][7], de1, rde1)));
           x0[t + u * NF] = v;
         }
     }
   }
   __syncthreads();
-  double *out = it.kind == 1 ? P.u : P.v, *out_f = it.kind == 1 ? P.u_f : P.v_f;
+  const auto out = it.kind == 1 ? P.u : P.v;
+  const auto out_f = it.kind == 1 ? P.u_f : P.v_f;
   for (int c = it.c0 + threadIdx.x; c < it.c1; c += NF) {
     const double *x = x0 + (c - it.c0);
     out[o + c] = x[h];
@@ -417,7 +510,7 @@ struct EnsMember {
   Mesh M;
   const double *mrows;
   FilterDev big, small1;
-  double grav, dt, wgt;
+  double grav, rgrav, dt, wgt;   // rgrav = 1 / grav correctly rounded
   int stokes, pv_scheme, order, coriolis;
   StagePtrs sp;
   UVPtrs up;
@@ -426,21 +519,71 @@ struct EnsMember {
   int tile0, nx;       // first tile of the member, tiles per tile row
 };
 
-// grid-stride over the tiles of all members: gridDim.x may be smaller than
-// the tile count (a persistent launch sized by the tuner)
+// Field pointers read from a descriptor in memory are generic (FLAT) to the
+// compiler, and FLAT accesses also count against the LDS wait counter.  The
+// ensemble kernels therefore hand the shared tile and item code pointers
+// typed in the global address space, so that it emits global loads and
+// stores as it does for pointers passed as kernel arguments.
+template <class T> using gptr = __attribute__((address_space(1))) T *;
+template <class T> __device__ __forceinline__ gptr<T> glb(T *p) { return (gptr<T>)p; }
+struct MeshG {
+  gptr<const double> m;
+  int nlat;
+  __device__ __forceinline__ double operator()(int f, int r) const { return m[f * nlat + r]; }
+  __device__ __forceinline__ double div(double x, int f, int r) const { return x / (*this)(f, r); }
+};
+__device__ __forceinline__ MeshG glb(const Mesh &M) { return MeshG{glb(M.m), M.nlat}; }
+struct FilterDevG {
+  int max_taps;
+  gptr<const int32_t> ngrid_lon, ngrid_lat;
+  gptr<const double> wgt_lon, wgt_lat;
+};
+struct StagePtrsG {
+  gptr<const double> gz_star, gzs, u_f, v_f, gz_old;
+  gptr<double> gz_new, du, dv;
+};
+struct UVPtrsG {
+  gptr<const double> u_old, v_old, du, dv, gz_f;
+  gptr<double> u, v, u_f, v_f;
+};
+__device__ __forceinline__ FilterDevG glb(const FilterDev &f) {
+  return FilterDevG{f.max_taps, glb(f.ngrid_lon), glb(f.ngrid_lat), glb(f.wgt_lon), glb(f.wgt_lat)};
+}
+__device__ __forceinline__ StagePtrsG glb(const StagePtrs &p) {
+  return StagePtrsG{glb(p.gz_star), glb(p.gzs), glb(p.u_f), glb(p.v_f), glb(p.gz_old), glb(p.gz_new), glb(p.du), glb(p.dv)};
+}
+__device__ __forceinline__ UVPtrsG glb(const UVPtrs &p) {
+  return UVPtrsG{glb(p.u_old), glb(p.v_old), glb(p.du), glb(p.dv), glb(p.gz_f), glb(p.u), glb(p.v), glb(p.u_f), glb(p.v_f)};
+}
+
+// The tiles of all members, numbered member by member and row-major within a
+// member, are cut into NXCD contiguous runs, one per XCD (workgroup b runs on
+// XCD b % NXCD), so vertically adjacent tiles share that XCD's L2.  The
+// workgroups of one XCD stride over its run: gridDim.x may be smaller than
+// the tile count (a persistent launch sized by the tuner).
 template <class S>
 __global__ void __launch_bounds__(S::NT, 8) k_tend_ens(const EnsMember *__restrict__ mem, int nmem, int total) {
   __shared__ double sh[S::SH_ALL];
-  for (int t = blockIdx.x; t < total; t += gridDim.x) {
+  const int G = gridDim.x, x = blockIdx.x % NXCD;
+  int start = 0;
+  for (int y = 0; y < x; y++) start += (total - y + NXCD - 1) / NXCD;
+  const int cnt = (total - x + NXCD - 1) / NXCD;    // tiles of this XCD's run
+  const int per = (G - x + NXCD - 1) / NXCD;        // workgroups on this XCD
+  for (int s = blockIdx.x / NXCD; s < cnt; s += per) {
+    const int t = start + s;
     int lo = 0, hi = nmem - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
       if (mem[mid].tile0 <= t) lo = mid; else hi = mid - 1;
     }
-    const EnsMember &d = mem[lo];
+    const EnsMember d = mem[lo];   // a private copy: no aliasing with the stores of the tile
     const int lt = t - d.tile0;
-    tend_tile<S>(d.g, d.M, d.mrows, d.grav, d.dt, d.stokes, d.pv_scheme, d.order, d.wgt, d.coriolis, d.sp,
-                 (lt % d.nx) * TX, (lt / d.nx) * S::TY, sh);
+    const int C0 = (lt % d.nx) * TX, R0 = (lt / d.nx) * S::TY;
+    const StagePtrsG sp = glb(d.sp);
+    tile_poles<S>(d.g, glb(d.M), d.grav, d.rgrav, sp, R0, sh);
+    TileIn io = tile_fetch<S>(d.g, glb(d.mrows), sp, C0, R0);
+    tend_tile<S>(d.g, glb(d.M), d.grav, d.rgrav, d.dt, d.stokes, d.pv_scheme, d.order, d.wgt, d.coriolis, sp, C0,
+                 R0, sh, io, NoNext{});
     __syncthreads();
   }
 }
@@ -450,8 +593,8 @@ __global__ void __launch_bounds__(NF) k_gzf_ens(const EnsMember *__restrict__ me
   extern __shared__ double xs[];
   for (int b = blockIdx.x; b < nitems; b += gridDim.x) {
     const Item it = items[b];
-    const EnsMember &d = mem[it.member];
-    gzf_item(d.g, d.big, d.small1, it, d.sp.gz_new, d.gz_f_out, d.gz_out, xs);
+    const EnsMember d = mem[it.member];
+    gzf_item(d.g, glb(d.big), glb(d.small1), it, glb(d.sp.gz_new), glb(d.gz_f_out), glb(d.gz_out), xs);
     __syncthreads();
   }
 }
@@ -461,8 +604,12 @@ __global__ void __launch_bounds__(NF) k_uvf_ens(const EnsMember *__restrict__ me
   extern __shared__ double xs[];
   for (int b = blockIdx.x; b < nitems; b += gridDim.x) {
     const Item it = items[b];
-    const EnsMember &d = mem[it.member];
-    uvf_item(d.g, d.M, d.dt, d.big, it, d.up, xs);
+    const EnsMember *d = mem + it.member;
+    const Grid g = d->g;
+    const MeshG M = glb(d->M);
+    const FilterDevG big = glb(d->big);
+    const UVPtrsG up = glb(d->up);
+    uvf_item(g, M, d->dt, big, it, up, xs);
     __syncthreads();
   }
 }
@@ -557,7 +704,8 @@ static void fused_stage(gmc_ctx *c, int o, int s, int n, double dt) {
   prof_mark(c);
   StagePtrs sp{S[S_GZ], c->gzs, S[S_U_F], S[S_V_F], O[S_GZ], gz_new, du, dv};
   const int tiles = ((g.nlon + TX - 1) / TX) * ((g.nlat + TY_MAIN - 1) / TY_MAIN);
-  hipLaunchKernelGGL(k_tend, dim3(tiles), dim3(Tile<TY_MAIN>::NT), 0, c->stream, g, M, c->d_mrows, c->cfg.g, dt, c->cfg.pv_pole_stokes,
+  hipLaunchKernelGGL(k_tend, dim3(tiles), dim3(Tile<TY_MAIN>::NT), 0, c->stream, g, M, c->d_mrows, c->cfg.g,
+                     1.0 / c->cfg.g, dt, c->cfg.pv_pole_stokes,
                      c->cfg.pv_scheme, c->cfg.upwind_order_pv, c->cfg.upwind_wgt_pv, c->cfg.coriolis_scheme, sp);
   prof_mark(c);
   hipLaunchKernelGGL(k_gzf, dim3(PL->n_gz), dim3(NF), PL->lds_gz, c->stream, g, c->big, c->small1, PL->d_items,
@@ -715,6 +863,7 @@ int build_tables(gmc_ens *e) {
         d.big = c->big;
         d.small1 = c->small1;
         d.grav = c->cfg.g;
+        d.rgrav = 1.0 / c->cfg.g;
         d.dt = sdt;
         d.wgt = c->cfg.upwind_wgt_pv;
         d.stokes = c->cfg.pv_pole_stokes;
@@ -745,6 +894,7 @@ void launch_tend(gmc_ens *e, const EnsMember *mem, int grid) {
 void ens_launch_step(gmc_ens *e, int par) {
   const int nm = (int)e->m.size();
   const EnsShape &sh = kEnsShapes[e->shape];
+  // k_tend_ens covers every tile when grid == total or grid >= NXCD
   const int grid = sh.mult ? std::min(e->total_tiles, sh.mult * 256) : e->total_tiles;
   for (int st = 0; st < e->nstage; st++) {
     const EnsMember *mem = e->d_mem + ((size_t)par * e->nstage + st) * nm;

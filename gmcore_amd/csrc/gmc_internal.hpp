@@ -33,11 +33,34 @@ enum TendField {
   T_PGF_LON, T_PGF_LAT, T_NFIELDS
 };
 
-// Read-only view of the packed metric table: M(f, r).
+// a / d correctly rounded, from y = 1 / d correctly rounded: q0 = a y, the
+// exact remainder e = a - d q0 (one fma), q = q0 + e y rounded once
+// (Markstein's correction step, P. W. Markstein, IBM J. Res. Dev. 34 (1990)
+// 111-119).  Gives the bits of the IEEE quotient for normal a, d and a / d;
+// e == 0 returns q0, which keeps the sign of a zero numerator.  Three FP64
+// operations instead of the scale / reciprocal / Newton / fixup sequence of a
+// division, for denominators shared by a wavefront (metric terms of one row).
+__device__ __forceinline__ double div_rn(double a, double d, double y) {
+  const double q0 = a * y;
+  const double e = __builtin_fma(-q0, d, a);
+  return e == 0.0 ? q0 : __builtin_fma(e, y, q0);
+}
+
+// Metric fields whose reciprocals the row records also hold (slot order).
+constexpr int kRecipFields[] = {GMC_AREA_VTX, GMC_AREA_CELL, GMC_DE_LON, GMC_DE_LAT};
+constexpr int kNumRecip = 4;
+constexpr int kRowRec = GMC_MESH_NFIELDS + kNumRecip;   // doubles per row record
+__host__ __device__ constexpr int recip_slot(int f) {
+  return f == GMC_AREA_VTX ? 0 : f == GMC_AREA_CELL ? 1 : f == GMC_DE_LON ? 2 : f == GMC_DE_LAT ? 3 : -1;
+}
+
+// Read-only view of the packed metric table: M(f, r).  M.div(x, f, r) is
+// x / M(f, r) (the point formulas' divisions by a metric term).
 struct Mesh {
   const double *m;
   int nlat;
   __device__ __forceinline__ double operator()(int f, int r) const { return m[f * nlat + r]; }
+  __device__ __forceinline__ double div(double x, int f, int r) const { return x / (*this)(f, r); }
 };
 
 struct FilterDev {
@@ -52,6 +75,18 @@ struct Grid {
   int64_t pitch;
   int half;  // nlon / 2
 };
+
+// Workgroup b of a launch runs on XCD b % kNumXcd, and each XCD has its own
+// L2.  xcd_item gives the item workgroup b should take so that every XCD
+// works through one contiguous run of the items [0, total) (a launch of
+// exactly total workgroups): XCD x takes [x q + min(x, m), ...), q = total /
+// kNumXcd, m = total % kNumXcd, in order.  Items that share rows then share
+// an L2.
+constexpr int kNumXcd = 8;
+__device__ __forceinline__ int xcd_item(int b, int total) {
+  const int x = b % kNumXcd, s = b / kNumXcd, q = total / kNumXcd, m = total % kNumXcd;
+  return x * q + (x < m ? x : m) + s;
+}
 
 // Column i+d, periodic.
 __device__ __forceinline__ int wrap(int i, int n) {
@@ -91,7 +126,7 @@ struct gmc_ctx {
   gmc_config cfg;            // host copy; filter pointers point at device tables
   double dlon;
   double *d_mesh = nullptr;  // GMC_MESH_NFIELDS * nlat
-  double *d_mrows = nullptr; // row records: (nlat + 2) x GMC_MESH_NFIELDS, record r + 1 = row r
+  double *d_mrows = nullptr; // row records: (nlat + 2) x kRowRec, record r + 1 = row r
   gmc::FilterDev big{}, small1{};
   void *d_filter_mem[8] = {};
   double *state[4][gmc::S_NFIELDS] = {};

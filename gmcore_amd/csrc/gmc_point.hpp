@@ -16,8 +16,8 @@ __device__ __forceinline__ double m_of(double gz, double gzs, double g) { return
 template <class MT>
 __device__ __forceinline__ double m_vtx_of(double m00, double m01, double m10, double m11,
                                            const MT &M, int r) {
-  return ((m00 + m01) * M(GMC_AREA_SUBCELL2, r) + (m10 + m11) * M(GMC_AREA_SUBCELL1, r + 1)) /
-         M(GMC_AREA_VTX, r);
+  return M.div((m00 + m01) * M(GMC_AREA_SUBCELL2, r) + (m10 + m11) * M(GMC_AREA_SUBCELL1, r + 1),
+               GMC_AREA_VTX, r);
 }
 
 // w1 * (a + b) + w2 * (c + d): mfx_lat / mfy_lon, operators_mod.F90:585-600
@@ -31,27 +31,27 @@ __device__ __forceinline__ double tangent_sum(double w1, double w2, double a, do
 template <class MT>
 __device__ __forceinline__ double ke_of(double uw, double ue, double vs, double vn, const MT &M,
                                         int r) {
-  return (M(GMC_AREA_LON_WEST, r) * (uw * uw) + M(GMC_AREA_LON_EAST, r) * (ue * ue) +
-          M(GMC_AREA_LAT_NORTH, r - 1) * (vs * vs) + M(GMC_AREA_LAT_SOUTH, r) * (vn * vn)) /
-         M(GMC_AREA_CELL, r);
+  return M.div(M(GMC_AREA_LON_WEST, r) * (uw * uw) + M(GMC_AREA_LON_EAST, r) * (ue * ue) +
+                   M(GMC_AREA_LAT_NORTH, r - 1) * (vs * vs) + M(GMC_AREA_LAT_SOUTH, r) * (vn * vn),
+               GMC_AREA_CELL, r);
 }
 
 // calc_div, operators_mod.F90:370-375
 template <class MT>
 __device__ __forceinline__ double div_of(double ue, double uw, double vn, double vs, const MT &M,
                                          int r) {
-  return ((ue * M(GMC_LE_LON, r) - uw * M(GMC_LE_LON, r)) +
-          (vn * M(GMC_LE_LAT, r) - vs * M(GMC_LE_LAT, r - 1))) /
-         M(GMC_AREA_CELL, r);
+  return M.div((ue * M(GMC_LE_LON, r) - uw * M(GMC_LE_LON, r)) +
+                   (vn * M(GMC_LE_LAT, r) - vs * M(GMC_LE_LAT, r - 1)),
+               GMC_AREA_CELL, r);
 }
 
 // calc_vor, operators_mod.F90:629-634 (vertex row r)
 template <class MT>
 __device__ __forceinline__ double vor_of(double us, double un, double ve, double vw, const MT &M,
                                          int r) {
-  return (us * M(GMC_DE_LON, r) - un * M(GMC_DE_LON, r + 1) + ve * M(GMC_DE_LAT, r) -
-          vw * M(GMC_DE_LAT, r)) /
-         M(GMC_AREA_VTX, r);
+  return M.div(us * M(GMC_DE_LON, r) - un * M(GMC_DE_LON, r + 1) + ve * M(GMC_DE_LAT, r) -
+                   vw * M(GMC_DE_LAT, r),
+               GMC_AREA_VTX, r);
 }
 
 // calc_pv, operators_mod.F90:697
@@ -96,21 +96,21 @@ __device__ __forceinline__ double qhv_of(const MT &M, int r, double pl, double m
 // calc_grad_mf, operators_mod.F90:956-970
 template <class MT>
 __device__ __forceinline__ double dmfdlon_of(double mfx_e, double mfx_w, const MT &M, int r) {
-  return (mfx_e - mfx_w) * M(GMC_LE_LON, r) / M(GMC_AREA_CELL, r);
+  return M.div((mfx_e - mfx_w) * M(GMC_LE_LON, r), GMC_AREA_CELL, r);
 }
 template <class MT>
 __device__ __forceinline__ double dmfdlat_of(double mfy_n, double mfy_s, const MT &M, int r) {
-  return (mfy_n * M(GMC_LE_LAT, r) - mfy_s * M(GMC_LE_LAT, r - 1)) / M(GMC_AREA_CELL, r);
+  return M.div(mfy_n * M(GMC_LE_LAT, r) - mfy_s * M(GMC_LE_LAT, r - 1), GMC_AREA_CELL, r);
 }
 
 // pgf_swm_run, pgf_swm_mod.F90:33, :40
 template <class MT>
 __device__ __forceinline__ double pgf_lon_of(double gz_e, double gz_w, const MT &M, int r) {
-  return (gz_e - gz_w) / M(GMC_DE_LON, r);
+  return M.div(gz_e - gz_w, GMC_DE_LON, r);
 }
 template <class MT>
 __device__ __forceinline__ double pgf_lat_of(double gz_n, double gz_s, const MT &M, int r) {
-  return (gz_n - gz_s) / M(GMC_DE_LAT, r);
+  return M.div(gz_n - gz_s, GMC_DE_LAT, r);
 }
 
 // pole-row v smoothing, time_schemes_mod.F90:225-233 (wgt = 0.6)

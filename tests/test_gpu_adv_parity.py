@@ -76,6 +76,31 @@ def test_prescribed_flow_bitwise(case):
         assert o.fixes > 0
 
 
+def test_dense_negatives_bitwise():
+    """Cell-random 0/1 tracers under unlimited PPM: about 40 % of the cells
+    go negative in each step, many next to each other, so the fixer's
+    clusters (walked in parallel by the kernel) are long and many."""
+    nlon, nlat, dt, steps = 72, 37, 3600.0, 4
+    st = _setup(nlon, nlat, dt, 2, 20, "none")
+    m = st.mesh.metrics
+    ub, vb = init_cond.deform_wind_basis(nlon, nlat, m, 6, 7, 40.0)
+    rng = np.random.default_rng(4)
+    q0 = (rng.random((nlat, nlon)) < 0.5).astype(float)
+    q1 = (rng.random((nlat, nlon)) < 0.3).astype(float)
+    o, g = _pair(st, [q0, q1])
+    g.set_wind_basis(ub, vb)
+    sc = init_cond.wind_scales(steps, dt)
+    ones = np.ones((nlat, nlon))
+    for n in range(steps):
+        u, v = ub * sc[n], vb * sc[n]
+        o.set_wind(u, v, u, v, ones)
+        assert o.run(n) == 0
+        g.drive(n, sc[n:n + 1])
+        _compare(o, g, 2, f"step {n}")
+    assert g.fixes == o.fixes
+    assert o.fixes > nlon * nlat // 2
+
+
 def test_multi_step_drive_equals_stepwise():
     st = _setup(72, 37, 3600.0, 2, 20)
     m = st.mesh.metrics
