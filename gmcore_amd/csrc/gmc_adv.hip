@@ -55,8 +55,13 @@ __device__ __forceinline__ int wrap_any(int i, int n) {
   return r < 0 ? r + n : r;
 }
 
-__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
-__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+// min / max of the limiters: one v_min_f64 / v_max_f64 each.  They may
+// break a tie between +0 and -0 differently from the oracle's a < b ? a : b,
+// but every result of the slope and PPM limiters passes through fabs (fsign)
+// or a difference with a value that is itself the tie, so the limiters'
+// outputs carry the same bits either way (finite inputs).
+__device__ __forceinline__ double dmin(double a, double b) { return fmin(a, b); }
+__device__ __forceinline__ double dmax(double a, double b) { return fmax(a, b); }
 __device__ __forceinline__ double fsign(double a, double b) { return copysign(fabs(a), b); }
 
 // slope_simple / slope_mono / slope_pd (ffsl_mod.F90:1175-1205)
