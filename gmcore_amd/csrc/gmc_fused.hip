@@ -164,20 +164,14 @@ __device__ __forceinline__ void tile_poles(const Grid &g, const MT &Mg, double g
   if (R0 + S::TY + 3 >= g.nlat - 1) pole_sums<S::NT>(g, Mg, grav, rgrav, P, 1, sh, pole);
 }
 
-// no next tile to fetch
-struct NoNext {
-  __device__ __forceinline__ bool operator()(TileIn &) const { return false; }
-};
-
 // One tile of the forward tendencies and the unfiltered gz update: output
 // columns C0 .. C0+TX-1, rows R0 .. R0+TY-1, LDS scratch sh[S::SH_ALL].  The
 // tile's pole slots hold its pole sums (tile_poles) and io its loads
-// (tile_fetch) on entry; once those are in LDS, next(io) may start the loads
-// of the workgroup's next tile into io, which then overlap phases B to E.
-template <class S, class MT, class PT, class Next>
+// (tile_fetch) on entry.
+template <class S, class MT, class PT>
 __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double grav, double rgrav,
                                           double dt, int stokes, int pv_scheme, int order, double wgt, int coriolis,
-                                          const PT &P, int C0, int R0, double *sh, TileIn &io, const Next &next) {
+                                          const PT &P, int C0, int R0, double *sh, const TileIn &io) {
   double(*sm)[LC] = (double(*)[LC])(sh + 0 * S::NE);
   double(*suf)[LC] = (double(*)[LC])(sh + 1 * S::NE);
   double(*svf)[LC] = (double(*)[LC])(sh + 2 * S::NE);
@@ -207,7 +201,6 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double gr
   suf[k][q] = io.uf;
   svf[k][q] = io.vf;
   if (threadIdx.x < S::SH_MESH) sme[threadIdx.x] = io.rec;
-  next(io);
   const MeshTile M{sme, R0 - 2};
   __syncthreads();
 
@@ -321,52 +314,7 @@ __global__ void __launch_bounds__(Tile<TY_MAIN>::NT, 8) k_tend(Grid g, Mesh Mg, 
 
   tile_poles<S>(g, Mg, grav, rgrav, P, R0, sh);
   TileIn io = tile_fetch<S>(g, mrows, P, C0, R0);
-  tend_tile<S>(g, Mg, grav, rgrav, dt, stokes, pv_scheme, order, wgt, coriolis, P, C0, R0, sh, io, NoNext{});
-}
-
-// k_tend as a persistent launch of G workgroups (G a multiple of NXCD, at
-// most the tile count): the workgroups of XCD x stride over that XCD's run
-// of tiles, and each starts the global loads of its next tile as soon as
-// the current tile's inputs are in LDS, so they overlap its computation.
-__global__ void __launch_bounds__(Tile<TY_MAIN>::NT, 8) k_tend_pf(Grid g, Mesh Mg, const double *__restrict__ mrows,
-                                                                double grav, double rgrav, double dt, int stokes,
-                                                                int pv_scheme, int order, double wgt, int coriolis,
-                                                                StagePtrs P) {
-  using S = Tile<TY_MAIN>;
-  __shared__ double sh[S::SH_ALL];
-  const int nx = (g.nlon + TX - 1) / TX, ny = (g.nlat + S::TY - 1) / S::TY, T = nx * ny;
-  const int x = blockIdx.x % NXCD, per = (gridDim.x - x + NXCD - 1) / NXCD;
-  int start = 0;
-  for (int y = 0; y < x; y++) start += (T - y + NXCD - 1) / NXCD;
-  const int cnt = (T - x + NXCD - 1) / NXCD;
-  int s = blockIdx.x / NXCD;
-  if (s >= cnt) return;
-  auto tile = [&](int si) { return start + (x == NXCD - 1 ? cnt - 1 - si : si); };
-  // polar tiles (which form the pole sums first) are fetched when they start
-  auto polar = [&](int tt) { const int R0 = (tt / nx) * S::TY; return R0 <= 2 || R0 + S::TY + 3 >= g.nlat - 1; };
-  int t = tile(s);
-  TileIn io{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  bool fetched = false;
-  for (;;) {
-    const int C0 = (t % nx) * TX, R0 = (t / nx) * S::TY;
-    if (!fetched) {
-      tile_poles<S>(g, Mg, grav, rgrav, P, R0, sh);
-      io = tile_fetch<S>(g, mrows, P, C0, R0);
-    }
-    const int sn = s + per;
-    const bool more = sn < cnt;
-    const int tn = more ? tile(sn) : t;
-    fetched = more && !polar(tn);
-    auto next = [&](TileIn &in) {
-      if (fetched) in = tile_fetch<S>(g, mrows, P, (tn % nx) * TX, (tn / nx) * S::TY);
-      return fetched;
-    };
-    tend_tile<S>(g, Mg, grav, rgrav, dt, stokes, pv_scheme, order, wgt, coriolis, P, C0, R0, sh, io, next);
-    if (!more) break;
-    __syncthreads();
-    s = sn;
-    t = tn;
-  }
+  tend_tile<S>(g, Mg, grav, rgrav, dt, stokes, pv_scheme, order, wgt, coriolis, P, C0, R0, sh, io);
 }
 
 // ---------------------------------------------------------------------------
@@ -583,7 +531,7 @@ __global__ void __launch_bounds__(S::NT, 8) k_tend_ens(const EnsMember *__restri
     tile_poles<S>(d.g, glb(d.M), d.grav, d.rgrav, sp, R0, sh);
     TileIn io = tile_fetch<S>(d.g, glb(d.mrows), sp, C0, R0);
     tend_tile<S>(d.g, glb(d.M), d.grav, d.rgrav, d.dt, d.stokes, d.pv_scheme, d.order, d.wgt, d.coriolis, sp, C0,
-                 R0, sh, io, NoNext{});
+                 R0, sh, io);
     __syncthreads();
   }
 }

@@ -120,10 +120,28 @@ __device__ __forceinline__ double pole_smooth(double v, double v_next) {
 }
 
 // one output of the zonal filter, filter_mod.F90:41: taps in ascending
-// order from x[0] (= column i - hn)
+// order from x[0] (= column i - hn).  The products of eight taps are formed
+// ahead of their additions, which stay in tap order, so the LDS reads of a
+// wide polar filter overlap instead of each waiting for the last.
 __device__ __forceinline__ double filter_point(const double *__restrict__ w, int n, const double *x) {
   double acc = 0.0;
-  for (int k = 0; k < n; k++) acc = acc + w[k] * x[k];
+  int k = 0;
+  for (; k + 8 <= n; k += 8) {
+    double pr[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) pr[u] = w[k + u] * x[k + u];
+#pragma unroll
+    for (int u = 0; u < 8; u++) acc = acc + pr[u];
+  }
+  if (k + 4 <= n) {
+    double pr[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) pr[u] = w[k + u] * x[k + u];
+#pragma unroll
+    for (int u = 0; u < 4; u++) acc = acc + pr[u];
+    k += 4;
+  }
+  for (; k < n; k++) acc = acc + w[k] * x[k];
   return acc;
 }
 
