@@ -614,6 +614,22 @@ __global__ void a_add(Grid g, int rows, const double *a, const double *b, double
   out[k] = a[k] + b[k];
 }
 
+// zero / copy up to six whole fields (fld doubles each), field blockIdx.y:
+// fill and copy kernels rather than memset / memcpy nodes, so that the
+// adjoint step can be captured in a graph
+struct FieldSet {
+  double *p[6];
+};
+__global__ void __launch_bounds__(256) k_fill_fields(FieldSet f, int64_t fld) {
+  double *p = f.p[blockIdx.y];
+  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < fld; k += (int64_t)gridDim.x * 256) p[k] = 0.0;
+}
+__global__ void __launch_bounds__(256) k_copy_fields(FieldSet d, FieldSet src, int64_t fld) {
+  double *q = d.p[blockIdx.y];
+  const double *p = src.p[blockIdx.y];
+  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < fld; k += (int64_t)gridDim.x * 256) q[k] = p[k];
+}
+
 // cost term and its gradient on the observed cells: J_o = 1/2 sum (gz - y)^2
 // / sigma^2; lam gz = (gz - y) / sigma^2; the per-observation terms go to jo
 __global__ void k_obs(Grid g, int nobs, const int64_t *__restrict__ cell, const double *__restrict__ y,
@@ -639,6 +655,10 @@ struct gmc_tlad {
   double *X[4][X_N] = {};            // perturbation / adjoint of state levels 1..3
   double *W[W_N] = {};
   double *pole = nullptr;
+  double *tape1[5] = {};   // m_lon, m_lat, m_vtx, u_f, v_f of stage 1's star state
+  // graphs of one adjoint step, keyed by (old, new, dt)
+  struct StepGraph { int o, n; double dt; hipGraphExec_t exec; };
+  std::vector<StepGraph> graphs;
   std::string err;
 };
 
@@ -700,14 +720,13 @@ void tl_stage(gmc_tlad *a, int o, int s, int n, double h) {
   ops_filter(c, 0, 1, a->X[n][X_V], a->X[n][X_VF]);
 }
 
-// AD of one stage around the tape just computed by ops_stage_tape: consumes
-// the adjoint of the new state (level n) and adds into levels o and s
-void ad_stage(gmc_tlad *a, int o, int s, int n, double h) {
+// AD of one stage around its tape T (ops_stage_tape): consumes the adjoint
+// of the new state (level n) and adds into levels o and s
+void ad_stage(gmc_tlad *a, int o, int s, int n, double h, const Tape &T) {
   gmc_ctx *c = a->c;
   const Grid &g = c->grid;
   const Mesh M{c->d_mesh, g.nlat};
   const Opt op = opt_of(c);
-  const Tape T = tape_of(c, s);
   const dim3 b(BX, BY);
   ADPtrs A{a->X[n][X_U], a->X[n][X_V], a->X[n][X_GZ], a->X[n][X_UF], a->X[n][X_VF], a->X[o][X_U], a->X[o][X_V],
            a->X[o][X_GZ], a->X[s][X_GZ], a->X[s][X_UF], a->X[s][X_VF], {}, a->pole};
@@ -733,26 +752,29 @@ void ad_stage(gmc_tlad *a, int o, int s, int n, double h) {
   hipLaunchKernelGGL(a_m, grid2d(g, g.nlat), b, 0, c->stream, g, M, op, A);
 }
 
-int zero_level(gmc_tlad *a, int lv) {
-  for (int f = 0; f < X_N; f++) {
-    int rc = thip(a, hipMemsetAsync(a->X[lv][f], 0, a->fld * sizeof(double), a->c->stream), "zero");
-    if (rc) return rc;
-  }
-  return GMC_OK;
+dim3 fields_grid(gmc_tlad *a, int nf) {
+  return dim3((unsigned)std::min<int64_t>(((int64_t)a->fld + 255) / 256, 512), (unsigned)nf);
 }
+int fill_fields(gmc_tlad *a, double *const *p, int nf) {
+  FieldSet f{};
+  for (int k = 0; k < nf; k++) f.p[k] = p[k];
+  hipLaunchKernelGGL(k_fill_fields, fields_grid(a, nf), dim3(256), 0, a->c->stream, f, (int64_t)a->fld);
+  return thip(a, hipGetLastError(), "zero");
+}
+int copy_fields(gmc_tlad *a, double *const *dst, double *const *src, int nf) {
+  FieldSet d{}, q{};
+  for (int k = 0; k < nf; k++) { d.p[k] = dst[k]; q.p[k] = src[k]; }
+  hipLaunchKernelGGL(k_copy_fields, fields_grid(a, nf), dim3(256), 0, a->c->stream, d, q, (int64_t)a->fld);
+  return thip(a, hipGetLastError(), "copy");
+}
+
+int zero_level(gmc_tlad *a, int lv) { return fill_fields(a, a->X[lv], X_N); }
 
 // the state fields one step reads and writes (u, v, gz, u_f, v_f, gz_f)
 constexpr int kStateFields[] = {S_U_LON, S_V_LAT, S_GZ, S_U_F, S_V_F, S_GZ_F};
 constexpr int kNState = 6;
 
-int copy_state(gmc_tlad *a, double *const *dst, double *const *src) {
-  for (int f = 0; f < kNState; f++) {
-    int rc = thip(a, hipMemcpyAsync(dst[f], src[f], a->fld * sizeof(double), hipMemcpyDeviceToDevice, a->c->stream),
-                  "state copy");
-    if (rc) return rc;
-  }
-  return GMC_OK;
-}
+int copy_state(gmc_tlad *a, double *const *dst, double *const *src) { return copy_fields(a, dst, src, kNState); }
 
 // snapshot of level lv into buf[6] (or back)
 int save_level(gmc_tlad *a, int lv, double *const *buf) {
@@ -781,28 +803,43 @@ int tl_step(gmc_tlad *a, int o, int n, double dt) {
   return thip(a, hipGetLastError(), "tl step");
 }
 
-// AD of one step whose old state is in level o (restored from `x` before each
-// stage tape): X[n] (adjoint of the step's result) -> added into X[o]
-int ad_step(gmc_tlad *a, int o, int n, double dt, double *const *x) {
+// AD of one step whose old state is in level o: X[n] (adjoint of the step's
+// result) -> added into X[o].  The stages run forward
+// once, each leaving its tape in its star level: stage 0 in o, stage 1 in n,
+// stage 2 in 3.  Stage 2 writes its new state into level n (u_f, v_f) and
+// the masses of that new state (m_lon, m_lat, m_vtx for its backward pass)
+// over stage 1's tape, so those five fields are kept aside after stage 1.
+int ad_step_body(gmc_tlad *a, int o, int n, double dt) {
   gmc_ctx *c = a->c;
   const int ns = nstages(c);
-  for (int st = ns - 1; st >= 0; st--) {
-    // recompute the stages before st, then st itself as the tape
-    int rc = load_level(a, o, x);
-    for (int q = 0; q <= st && !rc; q++) {
-      int so, ss, sn;
-      double h;
-      stage_of(c->cfg.time_scheme, o, n, dt, q, so, ss, sn, h);
-      rc = ops_stage_tape(c, so, ss, sn, h);
-    }
-    if (rc) return rc;
+  int rc = GMC_OK;
+  for (int st = 0; st < ns && !rc; st++) {
     int so, ss, sn;
     double h;
     stage_of(c->cfg.time_scheme, o, n, dt, st, so, ss, sn, h);
-    ad_stage(a, so, ss, sn, h);
-    rc = zero_level(a, sn);   // the new state's adjoint is consumed
-    if (rc) return rc;
+    rc = ops_stage_tape(c, so, ss, sn, h);
+    if (!rc && st == 1 && ns > 2) {
+      double *src[5] = {c->state[ss][S_M_LON], c->state[ss][S_M_LAT], c->state[ss][S_M_VTX], c->state[ss][S_U_F],
+                        c->state[ss][S_V_F]};
+      rc = copy_fields(a, a->tape1, src, 5);
+    }
   }
+  for (int st = ns - 1; st >= 0 && !rc; st--) {
+    int so, ss, sn;
+    double h;
+    stage_of(c->cfg.time_scheme, o, n, dt, st, so, ss, sn, h);
+    Tape T = tape_of(c, ss);
+    if (st == 1 && ns > 2) {
+      T.m_lon = a->tape1[0];
+      T.m_lat = a->tape1[1];
+      T.m_vtx = a->tape1[2];
+      T.u_f = a->tape1[3];
+      T.v_f = a->tape1[4];
+    }
+    ad_stage(a, so, ss, sn, h, T);
+    rc = zero_level(a, sn);   // the new state's adjoint is consumed
+  }
+  if (rc) return rc;
   return thip(a, hipGetLastError(), "ad step");
 }
 
@@ -819,7 +856,7 @@ int gmc_tlad_create(gmc_ctx *c, gmc_tlad **out) {
   a->c = c;
   const Grid &g = c->grid;
   a->fld = (size_t)g.nlat * g.pitch;
-  const size_t n = a->fld * (3 * X_N + W_N) + 8;
+  const size_t n = a->fld * (3 * X_N + W_N + 5) + 8;
   int rc = thip(a, hipMalloc((void **)&a->slab, n * sizeof(double)), "tlad alloc");
   if (!rc) rc = thip(a, hipMemset(a->slab, 0, n * sizeof(double)), "tlad memset");
   if (rc) {
@@ -830,6 +867,7 @@ int gmc_tlad_create(gmc_ctx *c, gmc_tlad **out) {
   for (int lv = 1; lv <= 3; lv++)
     for (int f = 0; f < X_N; f++) { a->X[lv][f] = p; p += a->fld; }
   for (int k = 0; k < W_N; k++) { a->W[k] = p; p += a->fld; }
+  for (int k = 0; k < 5; k++) { a->tape1[k] = p; p += a->fld; }
   a->pole = p;
   *out = a;
   return GMC_OK;
@@ -838,6 +876,7 @@ int gmc_tlad_create(gmc_ctx *c, gmc_tlad **out) {
 int gmc_tlad_destroy(gmc_tlad *a) {
   if (!a) return GMC_EINVAL;
   if (a->c) (void)hipStreamSynchronize(a->c->stream);
+  for (auto &sg : a->graphs) (void)hipGraphExecDestroy(sg.exec);
   if (a->slab) (void)hipFree(a->slab);
   delete a;
   return GMC_OK;
@@ -925,30 +964,55 @@ int alloc_traj(gmc_tlad *a, Traj &t, int nstates) {
   return GMC_OK;
 }
 
-// the new state of a step (level n) becomes the current one (level o)
-int advance(gmc_tlad *a, int o, int n) {
-  double *src[kNState], *dst[kNState];
-  for (int f = 0; f < kNState; f++) {
-    src[f] = a->c->state[n][kStateFields[f]];
-    dst[f] = a->c->state[o][kStateFields[f]];
+
+// One adjoint step as a graph: the adjoint of the step's result moves from
+// X[o] to X[n], X[o] and X[3] are cleared, then ad_step_body.  The step's
+// initial state is loaded into level o before the launch.
+int ad_step_graph(gmc_tlad *a, int o, int n, double dt) {
+  gmc_ctx *c = a->c;
+  hipGraphExec_t exec = nullptr;
+  for (auto &sg : a->graphs)
+    if (sg.o == o && sg.n == n && sg.dt == dt) exec = sg.exec;
+  if (!exec) {
+    hipGraph_t graph = nullptr;
+    int rc = thip(a, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (rc) return rc;
+    int rc2 = copy_fields(a, a->X[n], a->X[o], X_N);
+    if (!rc2) rc2 = zero_level(a, o);
+    if (!rc2) rc2 = zero_level(a, 3);
+    if (!rc2) rc2 = ad_step_body(a, o, n, dt);
+    rc = thip(a, hipStreamEndCapture(c->stream, &graph), "end capture");
+    if (rc2) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc2;
+    }
+    if (rc) return rc;
+    rc = thip(a, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
+    (void)hipGraphDestroy(graph);
+    if (rc) return rc;
+    a->graphs.push_back({o, n, dt, exec});
   }
-  return copy_state(a, dst, src);
+  return thip(a, hipGraphLaunch(exec, c->stream), "graph launch");
 }
 
 // adjoint of `steps` steps whose initial states are traj[first ..]: the
 // adjoint of the last result is in X[o] on entry, of the first initial state
 // on return (u_f, v_f parts still separate)
 int ad_sweep(gmc_tlad *a, int o, int n, double dt, Traj &t, int first, int steps) {
-  gmc_ctx *c = a->c;
   int rc = GMC_OK;
   for (int s = first + steps - 1; s >= first && !rc; s--) {
-    for (int f = 0; f < X_N && !rc; f++)
-      rc = thip(a, hipMemcpyAsync(a->X[n][f], a->X[o][f], a->fld * sizeof(double), hipMemcpyDeviceToDevice,
-                                  c->stream), "adjoint move");
-    if (!rc) rc = zero_level(a, o);
-    if (!rc) rc = zero_level(a, 3);
-    if (!rc) rc = ad_step(a, o, n, dt, &t.b[(size_t)s * kNState]);
+    rc = load_level(a, o, &t.b[(size_t)s * kNState]);
+    if (!rc) rc = ad_step_graph(a, o, n, dt);
   }
+  return rc;
+}
+
+// one plain forward step (the fused step graph) from level cur into the
+// other of levels o, n; cur becomes that level
+int fwd_step(gmc_tlad *a, int o, int n, double dt, int &cur) {
+  const int nxt = cur == o ? n : o;
+  const int rc = fused_step_graph(a->c, cur, nxt, dt);
+  cur = nxt;
   return rc;
 }
 
@@ -962,9 +1026,8 @@ int fold_filters(gmc_tlad *a, int o) {
   const dim3 b(BX, BY);
   hipLaunchKernelGGL(a_add, grid2d(g, g.nlat), b, 0, c->stream, g, g.nlat, a->W[W_T1], a->X[o][X_U], a->X[o][X_U]);
   hipLaunchKernelGGL(a_add, grid2d(g, g.nhalf), b, 0, c->stream, g, g.nhalf, a->W[W_T2], a->X[o][X_V], a->X[o][X_V]);
-  int rc = thip(a, hipMemsetAsync(a->X[o][X_UF], 0, a->fld * sizeof(double), c->stream), "zero");
-  if (!rc) rc = thip(a, hipMemsetAsync(a->X[o][X_VF], 0, a->fld * sizeof(double), c->stream), "zero");
-  return rc;
+  double *uvf[2] = {a->X[o][X_UF], a->X[o][X_VF]};
+  return fill_fields(a, uvf, 2);
 }
 
 }  // namespace
@@ -976,17 +1039,15 @@ int gmc_ad_step(gmc_tlad *a, double dt, int32_t nsteps) {
   const int o = c->old_level, n = c->new_level;
   int rc = fused_prepare(c);
   // the adjoint enters on u, v, gz only
-  if (!rc) rc = thip(a, hipMemsetAsync(a->X[o][X_UF], 0, a->fld * sizeof(double), c->stream), "zero");
-  if (!rc) rc = thip(a, hipMemsetAsync(a->X[o][X_VF], 0, a->fld * sizeof(double), c->stream), "zero");
-  if (!rc) rc = zero_level(a, n);
-  if (!rc) rc = zero_level(a, 3);
+  double *uvf[2] = {a->X[o][X_UF], a->X[o][X_VF]};
+  if (!rc) rc = fill_fields(a, uvf, 2);
   Traj t;
   if (!rc) rc = alloc_traj(a, t, nsteps + 1);
-  // forward: keep the initial state of every step (the last slot: the result)
+  // forward: keep the initial state of every step
+  int cur = o;
   for (int s = 0; s < nsteps && !rc; s++) {
-    rc = save_level(a, o, &t.b[(size_t)s * kNState]);
-    if (!rc) rc = fused_integrate(c, o, n, dt);
-    if (!rc) rc = advance(a, o, n);
+    rc = save_level(a, cur, &t.b[(size_t)s * kNState]);
+    if (!rc) rc = fwd_step(a, o, n, dt, cur);
   }
   if (!rc) rc = ad_sweep(a, o, n, dt, t, 0, nsteps);
   if (!rc) rc = fold_filters(a, o);
@@ -1019,10 +1080,10 @@ int gmc_gradient(gmc_tlad *a, double dt, int32_t nsteps, int32_t ckpt_every, int
   if (!rc) rc = alloc_traj(a, ck, nck);
   if (!rc) rc = alloc_traj(a, seg, ckpt_every);
   // forward with checkpoints every ckpt_every steps
+  int cur = o;
   for (int s = 0; s < nsteps && !rc; s++) {
-    if (s % ckpt_every == 0) rc = save_level(a, o, &ck.b[(size_t)(s / ckpt_every) * kNState]);
-    if (!rc) rc = fused_integrate(c, o, n, dt);
-    if (!rc) rc = advance(a, o, n);
+    if (s % ckpt_every == 0) rc = save_level(a, cur, &ck.b[(size_t)(s / ckpt_every) * kNState]);
+    if (!rc) rc = fwd_step(a, o, n, dt, cur);
   }
   // observation term at the final state and its adjoint
   int64_t *d_cell = nullptr;
@@ -1039,7 +1100,7 @@ int gmc_gradient(gmc_tlad *a, double dt, int32_t nsteps, int32_t ckpt_every, int
     if (!rc) rc = thip(a, hipMemcpyAsync(d_y, obs_value, sizeof(double) * nobs, hipMemcpyHostToDevice, c->stream), "obs copy");
     if (!rc) {
       hipLaunchKernelGGL(k_obs, dim3((nobs + 255) / 256), dim3(256), 0, c->stream, g, nobs, d_cell, d_y,
-                         1.0 / (sigma_o * sigma_o), c->state[o][S_GZ], a->X[o][X_GZ], d_jo);
+                         1.0 / (sigma_o * sigma_o), c->state[cur][S_GZ], a->X[o][X_GZ], d_jo);
       std::vector<double> h(nobs);
       rc = thip(a, hipMemcpyAsync(h.data(), d_jo, sizeof(double) * nobs, hipMemcpyDeviceToHost, c->stream), "obs copy");
       if (!rc) rc = thip(a, hipStreamSynchronize(c->stream), "sync");
@@ -1052,10 +1113,10 @@ int gmc_gradient(gmc_tlad *a, double dt, int32_t nsteps, int32_t ckpt_every, int
     // the adjoint of the segment's end state is in X[o]; recompute from the
     // checkpoint without touching it
     rc = load_level(a, o, &ck.b[(size_t)q * kNState]);
+    cur = o;
     for (int s = 0; s < steps && !rc; s++) {
-      rc = save_level(a, o, &seg.b[(size_t)s * kNState]);
-      if (!rc) rc = fused_integrate(c, o, n, dt);
-      if (!rc) rc = advance(a, o, n);
+      rc = save_level(a, cur, &seg.b[(size_t)s * kNState]);
+      if (!rc) rc = fwd_step(a, o, n, dt, cur);
     }
     if (!rc) rc = ad_sweep(a, o, n, dt, seg, 0, steps);
   }
