@@ -164,14 +164,23 @@ __device__ __forceinline__ void tile_poles(const Grid &g, const MT &Mg, double g
   if (R0 + S::TY + 3 >= g.nlat - 1) pole_sums<S::NT>(g, Mg, grav, rgrav, P, 1, sh, pole);
 }
 
+// The stage's intermediates of the star state that the adjoint reads (the
+// fields operators_prepare leaves, on the rows it writes them): written by
+// the taping tile for its own output points.
+struct TapeOut {
+  double *m_lon, *m_lat, *m_vtx, *mfx, *mfy, *mfx_lat, *mfy_lon, *u_lat, *v_lon, *pv, *pv_lon, *pv_lat;
+};
+
 // One tile of the forward tendencies and the unfiltered gz update: output
 // columns C0 .. C0+TX-1, rows R0 .. R0+TY-1, LDS scratch sh[S::SH_ALL].  The
 // tile's pole slots hold its pole sums (tile_poles) and io its loads
-// (tile_fetch) on entry.
-template <class S, class MT, class PT>
+// (tile_fetch) on entry.  With TAPE the tile also writes the intermediates
+// of its output points to *tape.
+template <class S, bool TAPE = false, class MT, class PT>
 __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double grav, double rgrav,
                                           double dt, int stokes, int pv_scheme, int order, double wgt, int coriolis,
-                                          const PT &P, int C0, int R0, double *sh, const TileIn &io) {
+                                          const PT &P, int C0, int R0, double *sh, const TileIn &io,
+                                          const TapeOut *tape = nullptr) {
   double(*sm)[LC] = (double(*)[LC])(sh + 0 * S::NE);
   double(*suf)[LC] = (double(*)[LC])(sh + 1 * S::NE);
   double(*svf)[LC] = (double(*)[LC])(sh + 2 * S::NE);
@@ -216,7 +225,14 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double gr
       if (stokes && r == 0) vor = pole[4];
       else if (stokes && r == nhalf - 1) vor = pole[5];
       else vor = vor_of(suf[k][q], suf[k + 1][q], svf[k][q + 1], svf[k][q], M, r);
-      v = pv_of(vor, M, r, m_vtx_of(sm[k][q], sm[k][q + 1], sm[k + 1][q], sm[k + 1][q + 1], M, r));
+      const double mv = m_vtx_of(sm[k][q], sm[k][q + 1], sm[k + 1][q], sm[k + 1][q + 1], M, r);
+      v = pv_of(vor, M, r, mv);
+      if (TAPE && out_pt) {
+        tape->m_vtx[r * p + ic] = mv;
+        tape->pv[r * p + ic] = v;
+        tape->mfy[r * p + ic] = smfy[k][q];
+        tape->m_lat[r * p + ic] = (sm[k][q] + sm[k + 1][q]) * 0.5;
+      }
     } else if (r >= -2 && r <= nhalf + 1) {
       int rr = r, ii = i;
       mirror_half(rr, ii, g);
@@ -244,6 +260,13 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double gr
     const double vl = tt / ((sm[k][q] + sm[k][q + 1]) * 0.5);
     const double a1 = spv[k - 2][q], a2 = spv[k - 1][q], a3 = spv[k][q], a4 = spv[k + 1][q];
     pl = pv_scheme == GMC_PV_MIDPOINT ? 0.5 * (a3 + a2) : pv_upwind_edge(order, wgt, vl, suf[k][q], 0, a1, a2, a3, a4);
+    if (TAPE && out_pt) {
+      tape->m_lon[r * p + ic] = (sm[k][q] + sm[k][q + 1]) * 0.5;
+      tape->mfx[r * p + ic] = smfx[k][q];
+      tape->mfy_lon[r * p + ic] = tt;
+      tape->v_lon[r * p + ic] = vl;
+      tape->pv_lon[r * p + ic] = pl;
+    }
   }
   if (k >= 1 && k < S::TY + 2 && q >= 3 && q < 62 && r >= 0 && r < nhalf) {
     const double tt = tangent_sum(M(GMC_HALF_TANGENT_WGT1, r), M(GMC_HALF_TANGENT_WGT2, r), smfx[k][q - 1],
@@ -251,6 +274,11 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double gr
     const double ul = tt / ((sm[k][q] + sm[k + 1][q]) * 0.5);
     const double a1 = spv[k][q - 2], a2 = spv[k][q - 1], a3 = spv[k][q], a4 = spv[k][q + 1];
     pt = pv_scheme == GMC_PV_MIDPOINT ? 0.5 * (a2 + a3) : pv_upwind_edge(order, wgt, ul, svf[k][q], 1, a1, a2, a3, a4);
+    if (TAPE && out_pt) {
+      tape->mfx_lat[r * p + ic] = tt;
+      tape->u_lat[r * p + ic] = ul;
+      tape->pv_lat[r * p + ic] = pt;
+    }
   }
   spl[k][q] = pl;
   spt[k][q] = pt;
@@ -297,10 +325,11 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double gr
 // tiles (row-major), so vertically adjacent tiles share that XCD's L2.  The
 // XCD holding the north pole walks its run backwards so that the polar tiles,
 // which also form the pole sums, start first.
+template <bool TAPE>
 __global__ void __launch_bounds__(Tile<TY_MAIN>::NT, 8) k_tend(Grid g, Mesh Mg, const double *__restrict__ mrows,
                                                              double grav, double rgrav, double dt, int stokes,
-                                                             int pv_scheme,
-                                                             int order, double wgt, int coriolis, StagePtrs P) {
+                                                             int pv_scheme, int order, double wgt, int coriolis,
+                                                             StagePtrs P, TapeOut tape) {
   using S = Tile<TY_MAIN>;
   __shared__ double sh[S::SH_ALL];
   // tile of this block
@@ -314,7 +343,7 @@ __global__ void __launch_bounds__(Tile<TY_MAIN>::NT, 8) k_tend(Grid g, Mesh Mg, 
 
   tile_poles<S>(g, Mg, grav, rgrav, P, R0, sh);
   TileIn io = tile_fetch<S>(g, mrows, P, C0, R0);
-  tend_tile<S>(g, Mg, grav, rgrav, dt, stokes, pv_scheme, order, wgt, coriolis, P, C0, R0, sh, io);
+  tend_tile<S, TAPE>(g, Mg, grav, rgrav, dt, stokes, pv_scheme, order, wgt, coriolis, P, C0, R0, sh, io, &tape);
 }
 
 // ---------------------------------------------------------------------------
@@ -641,8 +670,9 @@ static void prof_mark(gmc_ctx *c) {
   if (c->prof_ev && c->prof_n < c->prof_cap) (void)hipEventRecord(c->prof_ev[c->prof_n++], c->stream);
 }
 
-// one step_forward_backward stage: old -> new with the star state `star`
-static void fused_stage(gmc_ctx *c, int o, int s, int n, double dt) {
+// one step_forward_backward stage: old -> new with the star state `star`;
+// with tape, the star state's intermediates go to its state fields
+static void fused_stage(gmc_ctx *c, int o, int s, int n, double dt, bool tape = false) {
   const Grid &g = c->grid;
   const Mesh M{c->d_mesh, g.nlat};
   FusedPlan *PL = plan_of(c);
@@ -652,9 +682,11 @@ static void fused_stage(gmc_ctx *c, int o, int s, int n, double dt) {
   prof_mark(c);
   StagePtrs sp{S[S_GZ], c->gzs, S[S_U_F], S[S_V_F], O[S_GZ], gz_new, du, dv};
   const int tiles = ((g.nlon + TX - 1) / TX) * ((g.nlat + TY_MAIN - 1) / TY_MAIN);
-  hipLaunchKernelGGL(k_tend, dim3(tiles), dim3(Tile<TY_MAIN>::NT), 0, c->stream, g, M, c->d_mrows, c->cfg.g,
-                     1.0 / c->cfg.g, dt, c->cfg.pv_pole_stokes,
-                     c->cfg.pv_scheme, c->cfg.upwind_order_pv, c->cfg.upwind_wgt_pv, c->cfg.coriolis_scheme, sp);
+  TapeOut T{S[S_M_LON], S[S_M_LAT], S[S_M_VTX], S[S_MFX_LON], S[S_MFY_LAT], S[S_MFX_LAT], S[S_MFY_LON],
+            S[S_U_LAT], S[S_V_LON], S[S_PV], S[S_PV_LON], S[S_PV_LAT]};
+  hipLaunchKernelGGL(tape ? k_tend<true> : k_tend<false>, dim3(tiles), dim3(Tile<TY_MAIN>::NT), 0, c->stream, g, M,
+                     c->d_mrows, c->cfg.g, 1.0 / c->cfg.g, dt, c->cfg.pv_pole_stokes, c->cfg.pv_scheme,
+                     c->cfg.upwind_order_pv, c->cfg.upwind_wgt_pv, c->cfg.coriolis_scheme, sp, T);
   prof_mark(c);
   hipLaunchKernelGGL(k_gzf, dim3(PL->n_gz), dim3(NF), PL->lds_gz, c->stream, g, c->big, c->small1, PL->d_items,
                      gz_new, N[S_GZ_F], N[S_GZ]);
@@ -666,6 +698,13 @@ static void fused_stage(gmc_ctx *c, int o, int s, int n, double dt) {
 }
 
 int fused_prepare(gmc_ctx *c) { return c->fused_plan ? GMC_OK : build_plan(c); }
+
+int fused_stage_tape(gmc_ctx *c, int o, int s, int n, double dt) {
+  int rc = fused_prepare(c);
+  if (rc) return rc;
+  fused_stage(c, o, s, n, dt, true);
+  return check_hip(c, hipGetLastError(), "fused stage");
+}
 
 int fused_integrate(gmc_ctx *c, int o, int n, double dt) {
   int rc = fused_prepare(c);

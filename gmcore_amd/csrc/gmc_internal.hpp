@@ -167,6 +167,9 @@ int fused_integrate(gmc_ctx *ctx, int old_l, int new_l, double dt);
 void fused_release(gmc_ctx *ctx);
 int fused_prepare(gmc_ctx *ctx);
 int fused_step_graph(gmc_ctx *ctx, int old_l, int new_l, double dt);
+// one fused stage that also leaves the star state's intermediates (the
+// fields ops_stage_tape leaves for the adjoint) in its state fields
+int fused_stage_tape(gmc_ctx *ctx, int old_l, int star_l, int new_l, double dt);
 constexpr int kKernelClasses = 3;  // k_tend, k_gzf, k_uvf
 extern const char *const kKernelNames[kKernelClasses];
 }  // namespace gmc

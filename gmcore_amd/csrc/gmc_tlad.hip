@@ -6,10 +6,12 @@
 // (old state o, star state s) to the new state n.  The state seen by a stage
 // is (u, v, gz) of o and (gz, u_f, v_f) of s, with u_f = BF(u), v_f = BF(v)
 // formed by the previous stage (BF the big zonal filter).  The stage's
-// trajectory ("tape") comes from the operator-level kernels, which leave
-// every intermediate of operators_prepare and the tendencies in HBM
-// (ops_stage_tape); the TL kernels differentiate each operator around it,
-// and the AD kernels apply the transposes in reverse order.
+// trajectory ("tape") is the set of intermediates of operators_prepare in
+// HBM: the TL step takes it from the operator-level kernels
+// (ops_stage_tape), the adjoint from the fused tendency kernel's taping
+// variant (fused_stage_tape, same bits); the TL kernels differentiate each
+// operator around it, and the AD kernels apply the transposes in reverse
+// order.
 //
 // Every adjoint kernel is a gather: one thread owns one adjoint output and
 // adds the contributions of every forward output that read it, so no atomics
@@ -804,11 +806,11 @@ int tl_step(gmc_tlad *a, int o, int n, double dt) {
 }
 
 // AD of one step whose old state is in level o: X[n] (adjoint of the step's
-// result) -> added into X[o].  The stages run forward
-// once, each leaving its tape in its star level: stage 0 in o, stage 1 in n,
-// stage 2 in 3.  Stage 2 writes its new state into level n (u_f, v_f) and
-// the masses of that new state (m_lon, m_lat, m_vtx for its backward pass)
-// over stage 1's tape, so those five fields are kept aside after stage 1.
+// result) -> added into X[o].  The stages run forward once through the fused
+// kernels (the same bits as the operator path), each leaving its tape in its
+// star level: stage 0 in o, stage 1 in n, stage 2 in 3.  Stage 2 writes its
+// new state into level n, over the u_f, v_f of stage 1's tape, so stage 1's
+// tape fields that live in level n are kept aside after stage 1.
 int ad_step_body(gmc_tlad *a, int o, int n, double dt) {
   gmc_ctx *c = a->c;
   const int ns = nstages(c);
@@ -817,7 +819,7 @@ int ad_step_body(gmc_tlad *a, int o, int n, double dt) {
     int so, ss, sn;
     double h;
     stage_of(c->cfg.time_scheme, o, n, dt, st, so, ss, sn, h);
-    rc = ops_stage_tape(c, so, ss, sn, h);
+    rc = fused_stage_tape(c, so, ss, sn, h);
     if (!rc && st == 1 && ns > 2) {
       double *src[5] = {c->state[ss][S_M_LON], c->state[ss][S_M_LAT], c->state[ss][S_M_VTX], c->state[ss][S_U_F],
                         c->state[ss][S_V_F]};
