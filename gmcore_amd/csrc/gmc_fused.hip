@@ -390,7 +390,7 @@ __device__ __forceinline__ void gzf_item(const Grid &g, const FT &big, const FT 
 __global__ void __launch_bounds__(NF) k_gzf(Grid g, FilterDev big, FilterDev small, const Item *__restrict__ items,
                                             const double *__restrict__ gz_new, double *gz_f, double *gz) {
   extern __shared__ double xs[];
-  gzf_item(g, big, small, items[blockIdx.x], gz_new, gz_f, gz, xs);
+  gzf_item(g, big, small, items[xcd_item(blockIdx.x, gridDim.x)], gz_new, gz_f, gz, xs);
 }
 
 struct UVPtrs {
@@ -472,7 +472,7 @@ __device__ __forceinline__ void uvf_item(const Grid &g, const MT &M, double dt, 
 __global__ void __launch_bounds__(NF) k_uvf(Grid g, Mesh M, double dt, FilterDev big, const Item *__restrict__ items,
                                             UVPtrs P) {
   extern __shared__ double xs[];
-  uvf_item(g, M, dt, big, items[blockIdx.x], P, xs);
+  uvf_item(g, M, dt, big, items[xcd_item(blockIdx.x, gridDim.x)], P, xs);
 }
 
 
@@ -617,6 +617,42 @@ struct FusedPlan {
 
 static FusedPlan *plan_of(gmc_ctx *c) { return (FusedPlan *)c->fused_plan; }
 
+// Order a work list for a launch of one workgroup per item whose workgroup b
+// takes item xcd_item(b): each XCD works through one contiguous run of the
+// list.  The wide items (more than twice the median work: the polar rows'
+// long filters) are dealt round-robin to the XCDs and put first in each run,
+// so they start first everywhere; the rest follow in their list order, so
+// each XCD sweeps a band of neighbouring rows that share its L2.
+template <class W>
+static std::vector<Item> xcd_deal(const std::vector<Item> &v, W work) {
+  const int T = (int)v.size();
+  std::vector<int64_t> wk(T);
+  for (int k = 0; k < T; k++) wk[k] = work(v[k]);
+  std::vector<int64_t> sorted = wk;
+  std::nth_element(sorted.begin(), sorted.begin() + T / 2, sorted.end());
+  const int64_t med = T ? sorted[T / 2] : 0;
+  std::vector<int> wide, narrow;
+  for (int k = 0; k < T; k++) (wk[k] > 2 * med ? wide : narrow).push_back(k);
+  std::stable_sort(wide.begin(), wide.end(), [&](int a, int b) { return wk[a] > wk[b]; });
+  std::vector<std::vector<int>> run(NXCD);
+  std::vector<int> size(NXCD);
+  for (int x = 0; x < NXCD; x++) size[x] = T / NXCD + (x < T % NXCD ? 1 : 0);   // run lengths of xcd_item
+  int x = 0;
+  for (int k : wide) {   // round-robin over the XCDs that still have room
+    while ((int)run[x].size() >= size[x]) x = (x + 1) % NXCD;
+    run[x].push_back(k);
+    x = (x + 1) % NXCD;
+  }
+  size_t next = 0;
+  std::vector<Item> out;
+  out.reserve(T);
+  for (int y = 0; y < NXCD; y++) {
+    while ((int)run[y].size() < size[y]) run[y].push_back(narrow[next++]);
+    for (int k : run[y]) out.push_back(v[k]);
+  }
+  return out;
+}
+
 static int build_plan(gmc_ctx *c) {
   const Grid &g = c->grid;
   std::vector<int32_t> nb_lon(g.nlat), nb_lat(g.nhalf);
@@ -633,11 +669,24 @@ static int build_plan(gmc_ctx *c) {
   std::vector<Item> items;
   FusedPlan *P = new FusedPlan();
 
-  for (int r = 0; r < g.nlat; r++) add_row(items, r, 0, nb_lon[r] + std::max(1, nb_lon[r] / 2), g.nlon, P->lds_gz);
-  P->n_gz = (int)items.size();
-  for (int r = 0; r < g.nlat; r++) add_row(items, r, 1, nb_lon[r], g.nlon, P->lds_uv);
-  for (int r = 0; r < g.nhalf; r++) add_row(items, r, 2, nb_lat[r], g.nlon, P->lds_uv);
-  P->n_uv = (int)items.size() - P->n_gz;
+  {
+    std::vector<Item> gz, uv;
+    for (int r = 0; r < g.nlat; r++) add_row(gz, r, 0, nb_lon[r] + std::max(1, nb_lon[r] / 2), g.nlon, P->lds_gz);
+    // u row r next to v rows r - 1 and r: the three read gz_f row r
+    for (int r = 0; r < g.nlat; r++) {
+      add_row(uv, r, 1, nb_lon[r], g.nlon, P->lds_uv);
+      if (r < g.nhalf) add_row(uv, r, 2, nb_lat[r], g.nlon, P->lds_uv);
+    }
+    auto work = [&](const Item &it) {
+      const int n = it.kind == 2 ? nb_lat[it.row] : nb_lon[it.row];
+      return (int64_t)(it.c1 - it.c0) * std::max(n, 1);
+    };
+    const std::vector<Item> gz_d = xcd_deal(gz, work), uv_d = xcd_deal(uv, work);
+    items.insert(items.end(), gz_d.begin(), gz_d.end());
+    P->n_gz = (int)items.size();
+    items.insert(items.end(), uv_d.begin(), uv_d.end());
+    P->n_uv = (int)items.size() - P->n_gz;
+  }
   // the gz items filter with the big half-width; size their LDS for it
   P->lds_gz = 0;
   for (int k = 0; k < P->n_gz; k++) {
