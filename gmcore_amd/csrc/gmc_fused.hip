@@ -595,11 +595,14 @@ __global__ void __launch_bounds__(NF) k_uvf_ens(const EnsMember *__restrict__ me
 
 // ---------------------------------------------------------------------------
 // Work lists: a row whose filter has n taps is cut into chunks of about
-// kChunkWork / n columns so that each workgroup does similar work.
+// kChunkWork / n columns so that each workgroup does similar work; rows of
+// short filters into chunks of at most kNarrowChunk columns, so that a
+// workgroup's window is one round of loads.
 static constexpr int kChunkWork = 16384;
+static constexpr int kNarrowChunk = 2 * NF;
 
 static void add_row(std::vector<Item> &v, int row, int kind, int n, int nlon, size_t &lds) {
-  int w = n <= 16 ? nlon : std::min(nlon, std::max(NF, (kChunkWork / n + NF - 1) / NF * NF));
+  int w = n <= 16 ? std::min(nlon, kNarrowChunk) : std::min(nlon, std::max(NF, (kChunkWork / n + NF - 1) / NF * NF));
   const int parts = (nlon + w - 1) / w;
   const int h = n > 1 ? (n - 1) / 2 : 0;
   for (int k = 0; k < parts; k++) {
