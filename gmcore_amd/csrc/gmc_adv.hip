@@ -576,6 +576,7 @@ struct gmc_adv {
   unsigned long long *d_fixes = nullptr;
   int *h_flags = nullptr;  // pinned
   hipStream_t stream = nullptr;
+  gmc::HostStage stage;
   int uv_step = 0, mf_step = 0, old = 1, new_ = 2;
   bool have_basis = false;
   bool old_m_current = false;  // old_m already equals m (copy_old_m can be skipped)
@@ -609,9 +610,8 @@ void mark(gmc_adv *a, int kind) {
 }
 
 int copy_rows(gmc_adv *a, double *dst, const double *host, int rows, int64_t ld) {
-  return ahip(a, hipMemcpy2DAsync(dst, a->g.pitch * sizeof(double), host, ld * sizeof(double),
-                                  a->g.nlon * sizeof(double), rows, hipMemcpyHostToDevice, a->stream),
-              "upload");
+  return ahip(a, stage_copy(a->stage, a->stream, true, dst, a->g.pitch, const_cast<double *>(host), ld, a->g.nlon,
+                            rows), "upload");
 }
 
 // adv_run: accumulate, maybe advance, copy_old_m (adv_mod.F90:152-312)
@@ -745,6 +745,7 @@ int gmc_adv_destroy(gmc_adv *a) {
   if (a->slab) (void)hipFree(a->slab);
   if (a->h_flags) (void)hipHostFree(a->h_flags);
   if (a->stream) (void)hipStreamDestroy(a->stream);
+  stage_free(a->stage);
   delete a;
   return GMC_OK;
 }
@@ -764,11 +765,8 @@ int gmc_adv_download_tracer(gmc_adv *a, int32_t level, int32_t tracer, double *h
   if (!a) return GMC_EINVAL;
   if (level < 1 || level > 2 || tracer < 0 || tracer >= a->cfg.ntracers || !host || ld < a->g.nlon)
     return aerr(a, GMC_EINVAL, "bad tracer download arguments");
-  int rc = ahip(a, hipMemcpy2DAsync(host, ld * sizeof(double), tracer_ptr(a, level, tracer),
-                                    a->g.pitch * sizeof(double), a->g.nlon * sizeof(double), a->g.nlat,
-                                    hipMemcpyDeviceToHost, a->stream), "download");
-  if (!rc) rc = ahip(a, hipStreamSynchronize(a->stream), "sync");
-  return rc;
+  return ahip(a, stage_copy(a->stage, a->stream, false, tracer_ptr(a, level, tracer), a->g.pitch, host, ld,
+                            a->g.nlon, a->g.nlat), "download");
 }
 
 int gmc_adv_set_wind(gmc_adv *a, const double *u, const double *v, const double *mfx, const double *mfy,

@@ -111,6 +111,36 @@ static int upload_filter(gmc_ctx *c, const gmc_filter &f, FilterDev &d, int slot
   return rc;
 }
 
+hipError_t stage_copy(HostStage &s, hipStream_t stream, bool to_device, double *dev, int64_t pitch, double *host,
+                      int64_t ld, int nlon, int rows) {
+  const size_t n = (size_t)nlon * rows, row = sizeof(double) * nlon;
+  if (s.cap < n) {
+    stage_free(s);
+    hipError_t e = hipHostMalloc((void **)&s.p, n * sizeof(double), hipHostMallocDefault);
+    if (e != hipSuccess) {
+      s.p = nullptr;
+      return e;
+    }
+    s.cap = n;
+  }
+  if (to_device) {
+    for (int r = 0; r < rows; r++) std::memcpy(s.p + (size_t)r * nlon, host + r * ld, row);
+    hipError_t e = hipMemcpy2DAsync(dev, sizeof(double) * pitch, s.p, row, row, rows, hipMemcpyHostToDevice, stream);
+    return e == hipSuccess ? hipStreamSynchronize(stream) : e;
+  }
+  hipError_t e = hipMemcpy2DAsync(s.p, row, dev, sizeof(double) * pitch, row, rows, hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e == hipSuccess)
+    for (int r = 0; r < rows; r++) std::memcpy(host + r * ld, s.p + (size_t)r * nlon, row);
+  return e;
+}
+
+void stage_free(HostStage &s) {
+  if (s.p) (void)hipHostFree(s.p);
+  s.p = nullptr;
+  s.cap = 0;
+}
+
 }  // namespace gmc
 
 using namespace gmc;
@@ -193,6 +223,7 @@ int gmc_destroy(gmc_ctx *c) {
   (void)hipFree(c->d_mrows);
   for (void *m : c->d_filter_mem) (void)hipFree(m);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  stage_free(c->stage);
   delete c;
   return GMC_OK;
 }
@@ -203,22 +234,16 @@ int gmc_upload(gmc_ctx *c, int32_t level, int32_t field, const double *host, int
   if (!c || !host) return GMC_EINVAL;
   double *d = field_ptr(c, field == GMC_GZS ? 1 : level, field);
   if (!d || ld < c->grid.nlon) return set_error(c, GMC_EINVAL, "bad level/field/ld");
-  int rc = check_hip(c, hipMemcpy2DAsync(d, sizeof(double) * c->grid.pitch, host, sizeof(double) * ld,
-                                         sizeof(double) * c->grid.nlon, field_rows(c, field),
-                                         hipMemcpyHostToDevice, c->stream), "upload");
-  if (!rc) rc = check_hip(c, hipStreamSynchronize(c->stream), "upload sync");
-  return rc;
+  return check_hip(c, stage_copy(c->stage, c->stream, true, d, c->grid.pitch, const_cast<double *>(host), ld,
+                                 c->grid.nlon, field_rows(c, field)), "upload");
 }
 
 int gmc_download(gmc_ctx *c, int32_t level, int32_t field, double *host, int64_t ld) {
   if (!c || !host) return GMC_EINVAL;
   double *d = field_ptr(c, field == GMC_GZS ? 1 : level, field);
   if (!d || ld < c->grid.nlon) return set_error(c, GMC_EINVAL, "bad level/field/ld");
-  int rc = check_hip(c, hipMemcpy2DAsync(host, sizeof(double) * ld, d, sizeof(double) * c->grid.pitch,
-                                         sizeof(double) * c->grid.nlon, field_rows(c, field),
-                                         hipMemcpyDeviceToHost, c->stream), "download");
-  if (!rc) rc = check_hip(c, hipStreamSynchronize(c->stream), "download sync");
-  return rc;
+  return check_hip(c, stage_copy(c->stage, c->stream, false, d, c->grid.pitch, host, ld, c->grid.nlon,
+                                 field_rows(c, field)), "download");
 }
 
 int gmc_prepare(gmc_ctx *c, int32_t level) {

@@ -167,15 +167,19 @@ __device__ __forceinline__ void tile_poles(const Grid &g, const MT &Mg, double g
 // The stage's intermediates of the star state that the adjoint reads (the
 // fields operators_prepare leaves, on the rows it writes them): written by
 // the taping tile for its own output points.
-struct TapeOut {
-  double *m_lon, *m_lat, *m_vtx, *mfx, *mfy, *mfx_lat, *mfy_lon, *u_lat, *v_lon, *pv, *pv_lon, *pv_lat;
-};
-
 // One tile of the forward tendencies and the unfiltered gz update: output
 // columns C0 .. C0+TX-1, rows R0 .. R0+TY-1, LDS scratch sh[S::SH_ALL].  The
 // tile's pole slots hold its pole sums (tile_poles) and io its loads
 // (tile_fetch) on entry.  With TAPE the tile also writes the intermediates
 // of its output points to *tape.
+// the six upwind derivatives of one edge point into a tape
+__device__ __forceinline__ void tape_upwind(double *const *dst, int64_t o, const UpwD &d) {
+  dst[0][o] = d.dt;
+  dst[1][o] = d.dn;
+#pragma unroll
+  for (int j = 0; j < 4; j++) dst[2 + j][o] = d.df[j];
+}
+
 template <class S, bool TAPE = false, class MT, class PT>
 __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double grav, double rgrav,
                                           double dt, int stokes, int pv_scheme, int order, double wgt, int coriolis,
@@ -209,6 +213,10 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double gr
   sm[k][q] = (r >= 0 && r < nlat) ? div_rn(io.gz - io.gzs, grav, rgrav) : 0.0;   // m_of
   suf[k][q] = io.uf;
   svf[k][q] = io.vf;
+  if (TAPE && out_pt && tape->u_f) {
+    tape->u_f[r * p + ic] = io.uf;
+    if (r < nhalf) tape->v_f[r * p + ic] = io.vf;
+  }
   if (threadIdx.x < S::SH_MESH) sme[threadIdx.x] = io.rec;
   const MeshTile M{sme, R0 - 2};
   __syncthreads();
@@ -266,6 +274,8 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double gr
       tape->mfy_lon[r * p + ic] = tt;
       tape->v_lon[r * p + ic] = vl;
       tape->pv_lon[r * p + ic] = pl;
+      if (tape->dlon[0]) tape_upwind(tape->dlon, r * p + ic,
+                                     upwind_deriv(pv_scheme, order, wgt, vl, suf[k][q], 0, a1, a2, a3, a4));
     }
   }
   if (k >= 1 && k < S::TY + 2 && q >= 3 && q < 62 && r >= 0 && r < nhalf) {
@@ -278,6 +288,8 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double gr
       tape->mfx_lat[r * p + ic] = tt;
       tape->u_lat[r * p + ic] = ul;
       tape->pv_lat[r * p + ic] = pt;
+      if (tape->dlat[0]) tape_upwind(tape->dlat, r * p + ic,
+                                     upwind_deriv(pv_scheme, order, wgt, ul, svf[k][q], 1, a1, a2, a3, a4));
     }
   }
   spl[k][q] = pl;
@@ -724,7 +736,8 @@ static void prof_mark(gmc_ctx *c) {
 
 // one step_forward_backward stage: old -> new with the star state `star`;
 // with tape, the star state's intermediates go to its state fields
-static void fused_stage(gmc_ctx *c, int o, int s, int n, double dt, bool tape = false) {
+static void fused_stage(gmc_ctx *c, int o, int s, int n, double dt, bool tape = false,
+                        const TapeOut *dst = nullptr) {
   const Grid &g = c->grid;
   const Mesh M{c->d_mesh, g.nlat};
   FusedPlan *PL = plan_of(c);
@@ -735,7 +748,8 @@ static void fused_stage(gmc_ctx *c, int o, int s, int n, double dt, bool tape = 
   StagePtrs sp{S[S_GZ], c->gzs, S[S_U_F], S[S_V_F], O[S_GZ], gz_new, du, dv};
   const int tiles = ((g.nlon + TX - 1) / TX) * ((g.nlat + TY_MAIN - 1) / TY_MAIN);
   TapeOut T{S[S_M_LON], S[S_M_LAT], S[S_M_VTX], S[S_MFX_LON], S[S_MFY_LAT], S[S_MFX_LAT], S[S_MFY_LON],
-            S[S_U_LAT], S[S_V_LON], S[S_PV], S[S_PV_LON], S[S_PV_LAT]};
+            S[S_U_LAT], S[S_V_LON], S[S_PV], S[S_PV_LON], S[S_PV_LAT], nullptr, nullptr};
+  if (dst) T = *dst;
   hipLaunchKernelGGL(tape ? k_tend<true> : k_tend<false>, dim3(tiles), dim3(Tile<TY_MAIN>::NT), 0, c->stream, g, M,
                      c->d_mrows, c->cfg.g, 1.0 / c->cfg.g, dt, c->cfg.pv_pole_stokes, c->cfg.pv_scheme,
                      c->cfg.upwind_order_pv, c->cfg.upwind_wgt_pv, c->cfg.coriolis_scheme, sp, T);
@@ -751,10 +765,10 @@ static void fused_stage(gmc_ctx *c, int o, int s, int n, double dt, bool tape = 
 
 int fused_prepare(gmc_ctx *c) { return c->fused_plan ? GMC_OK : build_plan(c); }
 
-int fused_stage_tape(gmc_ctx *c, int o, int s, int n, double dt) {
+int fused_stage_tape(gmc_ctx *c, int o, int s, int n, double dt, const TapeOut *dst) {
   int rc = fused_prepare(c);
   if (rc) return rc;
-  fused_stage(c, o, s, n, dt, true);
+  fused_stage(c, o, s, n, dt, true, dst);
   return check_hip(c, hipGetLastError(), "fused stage");
 }
 

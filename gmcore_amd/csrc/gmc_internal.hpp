@@ -69,6 +69,19 @@ struct FilterDev {
   const double *wgt_lon, *wgt_lat;
 };
 
+// Pinned host buffer for the host <-> device transfers of pitched fields:
+// 2D copies between pageable host memory and pitched device rows go row by
+// row, so a field passes through this buffer (packed rows) instead.
+struct HostStage {
+  double *p = nullptr;
+  size_t cap = 0;   // doubles
+};
+// rows x nlon doubles between dev (row pitch `pitch`) and host (leading
+// dimension ld), through s; returns once the copy is complete.
+hipError_t stage_copy(HostStage &s, hipStream_t stream, bool to_device, double *dev, int64_t pitch, double *host,
+                      int64_t ld, int nlon, int rows);
+void stage_free(HostStage &s);
+
 // Geometry shared by all kernels of a context.
 struct Grid {
   int nlon, nlat, nhalf;
@@ -137,6 +150,7 @@ struct gmc_ctx {
   double *slab = nullptr;    // one allocation holding every field
   size_t slab_bytes = 0;
   hipStream_t stream = nullptr;
+  gmc::HostStage stage;
   int old_level = 1, new_level = 2;
   int prepared = 0;
   void *fused_plan = nullptr;  // work lists of the fused stage (gmc_fused.hip)
@@ -167,9 +181,20 @@ int fused_integrate(gmc_ctx *ctx, int old_l, int new_l, double dt);
 void fused_release(gmc_ctx *ctx);
 int fused_prepare(gmc_ctx *ctx);
 int fused_step_graph(gmc_ctx *ctx, int old_l, int new_l, double dt);
+// Destinations of the intermediates a taping stage writes for the adjoint:
+// the fields operators_prepare forms from the star state; when u_f is not
+// null the star state's u_f and v_f; when dlon[0] is not null the
+// derivatives of the upwind pv on the pv_lon and pv_lat edges (dt, dn,
+// df[0..3] of upwind_deriv, six fields each).
+struct TapeOut {
+  double *m_lon, *m_lat, *m_vtx, *mfx, *mfy, *mfx_lat, *mfy_lon, *u_lat, *v_lon, *pv, *pv_lon, *pv_lat;
+  double *u_f, *v_f;
+  double *dlon[6], *dlat[6];
+};
 // one fused stage that also leaves the star state's intermediates (the
-// fields ops_stage_tape leaves for the adjoint) in its state fields
-int fused_stage_tape(gmc_ctx *ctx, int old_l, int star_l, int new_l, double dt);
+// fields ops_stage_tape leaves for the adjoint) in its state fields, or in
+// *dst when dst is given
+int fused_stage_tape(gmc_ctx *ctx, int old_l, int star_l, int new_l, double dt, const TapeOut *dst = nullptr);
 constexpr int kKernelClasses = 3;  // k_tend, k_gzf, k_uvf
 extern const char *const kKernelNames[kKernelClasses];
 }  // namespace gmc

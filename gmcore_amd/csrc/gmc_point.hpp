@@ -73,6 +73,54 @@ __device__ __forceinline__ double pv_upwind_edge(int order, double wgt, double t
   return b * up + (1 - b) * 0.5 * (f2 + f3);
 }
 
+// Derivatives of pv_upwind_edge with respect to t, n and the four pv values
+// on the branch the trajectory took (the tangent-linear and adjoint of
+// interp_pv_upwind; |t| and sign(t) differentiated piecewise): dt, dn, df[4].
+struct UpwD {
+  double dt, dn, df[4];
+};
+__device__ __forceinline__ UpwD upwind_deriv(int pv_scheme, int order, double wgt, double t, double n,
+                                             int lat_edge, double f1, double f2, double f3, double f4) {
+  UpwD d;
+  if (pv_scheme == GMC_PV_MIDPOINT) {
+    d.dt = d.dn = 0.0;
+    d.df[0] = d.df[3] = 0.0;
+    d.df[1] = d.df[2] = 0.5;
+    return d;
+  }
+  const double S = sqrt(lat_edge ? t * t + n * n : n * n + t * t);
+  const double den = S + kEps;
+  const double b = fabs(t) / den;
+  const double dir = copysign(1.0, t);
+  const double w = wgt;
+  double up, u1, u2, u3, u4;
+  if (order == 1) {
+    up = upwind1(dir, w, f2, f3);
+    u1 = 0.0;
+    u2 = 0.5 + 0.5 * w * dir;
+    u3 = 0.5 - 0.5 * w * dir;
+    u4 = 0.0;
+  } else {
+    up = upwind3(dir, w, f1, f2, f3, f4);
+    const double c31 = 7.0 / 12.0, c32 = -1.0 / 12.0, c33 = 1.0 / 12.0;
+    u1 = c32 - c33 * w * dir;
+    u2 = c31 + 3 * c33 * w * dir;
+    u3 = c31 - 3 * c33 * w * dir;
+    u4 = c32 + c33 * w * dir;
+  }
+  const double c = 0.5 * (f2 + f3);
+  // b = |t| / (S + eps): db/dt = sign(t) / den - |t| t / (S den^2), db/dn = -|t| n / (S den^2)
+  const double q = S > 0 ? fabs(t) / (S * den * den) : 0.0;
+  const double dbdt = copysign(1.0, t) / den - q * t, dbdn = -q * n;
+  d.dt = dbdt * (up - c);
+  d.dn = dbdn * (up - c);
+  d.df[0] = b * u1;
+  d.df[1] = b * u2 + (1 - b) * 0.5;
+  d.df[2] = b * u3 + (1 - b) * 0.5;
+  d.df[3] = b * u4;
+  return d;
+}
+
 // calc_coriolis scheme 1, operators_mod.F90:866-877 (V point, row r)
 template <class MT>
 __device__ __forceinline__ double qhu_of(const MT &M, int r, double pl, double mfx_w, double pvl_w,

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -43,10 +44,13 @@ enum {
   W_PVLAT, W_QHU, W_QHV, W_DU, W_DV, W_DGZ, W_GZNEW, W_GZF, W_T1, W_T2, W_N
 };
 
-// the forward trajectory of one stage (fields of the star state, its tend)
+// the forward trajectory of one stage (fields of the star state, its tend);
+// the adjoint's tapes also hold the upwind-pv derivatives (UpwD dt, dn,
+// df[0..3]) of the pv_lon edges (dlon) and the pv_lat edges (dlat)
 struct Tape {
   const double *m_lon, *m_lat, *m_vtx, *mfx, *mfy, *mfx_lat, *mfy_lon, *u_lat, *v_lon, *pv, *pv_lon, *pv_lat;
   const double *u_f, *v_f;
+  const double *dlon[6], *dlat[6];
 };
 
 struct Opt {
@@ -55,53 +59,10 @@ struct Opt {
   double wgt;
 };
 
-__device__ __forceinline__ double dsign(double t) { return copysign(1.0, t); }
-
-// derivatives of pv_upwind_edge (gmc_point.hpp) with respect to t, n and the
-// four pv values, on the branch the trajectory took
-struct UpwD {
-  double dt, dn, df[4];
-};
+// derivatives of pv_upwind_edge on the branch the trajectory took
 __device__ __forceinline__ UpwD upwind_d(const Opt &o, double t, double n, int lat_edge, double f1, double f2,
                                          double f3, double f4) {
-  UpwD d;
-  if (o.pv_scheme == GMC_PV_MIDPOINT) {
-    d.dt = d.dn = 0.0;
-    d.df[0] = d.df[3] = 0.0;
-    d.df[1] = d.df[2] = 0.5;
-    return d;
-  }
-  const double S = sqrt(lat_edge ? t * t + n * n : n * n + t * t);
-  const double den = S + kEps;
-  const double b = fabs(t) / den;
-  const double dir = copysign(1.0, t);
-  const double w = o.wgt;
-  double up, u1, u2, u3, u4;
-  if (o.order == 1) {
-    up = upwind1(dir, w, f2, f3);
-    u1 = 0.0;
-    u2 = 0.5 + 0.5 * w * dir;
-    u3 = 0.5 - 0.5 * w * dir;
-    u4 = 0.0;
-  } else {
-    up = upwind3(dir, w, f1, f2, f3, f4);
-    const double c31 = 7.0 / 12.0, c32 = -1.0 / 12.0, c33 = 1.0 / 12.0;
-    u1 = c32 - c33 * w * dir;
-    u2 = c31 + 3 * c33 * w * dir;
-    u3 = c31 - 3 * c33 * w * dir;
-    u4 = c32 + c33 * w * dir;
-  }
-  const double c = 0.5 * (f2 + f3);
-  // b = |t| / (S + eps): db/dt = sign(t) / den - |t| t / (S den^2), db/dn = -|t| n / (S den^2)
-  const double q = S > 0 ? fabs(t) / (S * den * den) : 0.0;
-  const double dbdt = dsign(t) / den - q * t, dbdn = -q * n;
-  d.dt = dbdt * (up - c);
-  d.dn = dbdn * (up - c);
-  d.df[0] = b * u1;
-  d.df[1] = b * u2 + (1 - b) * 0.5;
-  d.df[2] = b * u3 + (1 - b) * 0.5;
-  d.df[3] = b * u4;
-  return d;
+  return upwind_deriv(o.pv_scheme, o.order, o.wgt, t, n, lat_edge, f1, f2, f3, f4);
 }
 
 // ===========================================================================
@@ -312,87 +273,173 @@ struct ADPtrs {
   double *pole;  // [0..1] ke, [2..3] dmfdlat, [4..5] vor: sums of pole-row adjoints
 };
 
-// W_T1 = BF(lam u_f) + lam u, W_T2 = BF(lam v_f) + lam v on entry (the
-// update's filter transposed); this forms the smoothing transpose and the
-// pgf transpose: W_DU = h lam u*, W_DV = h lam v*, lam_o u/v, W_GZF = lam gz_f
-__global__ void a_uv(Grid g, double h, ADPtrs A) {
-  const int i = blockIdx.x * BX + threadIdx.x, r = blockIdx.y * BY + threadIdx.y;
-  if (i >= g.nlon || r >= g.nlat) return;
-  const int64_t p = g.pitch, k = r * p + i;
-  const double *lu = A.w[W_T1], *lv = A.w[W_T2];
-  if (r >= 1 && r <= g.nlat - 2) {
-    A.o_u[k] += lu[k];
-    A.w[W_DU][k] = h * lu[k];
-  }
-  if (r <= g.nhalf - 1) {
-    const int last = g.nhalf - 1;
-    const double wgt = 0.6;
-    double ls = (r == 0 || r == last) ? (1 - wgt) * lv[k] : lv[k];
-    if (r == 1) ls += wgt * lv[k - p];
-    if (r == last - 1) ls += wgt * lv[k + p];
-    A.o_v[k] += ls;
-    A.w[W_DV][k] = h * ls;
+// Zonal-filter staging: slot[t] = x[(t - hn) mod nlon] for t in [0, nlon +
+// 2 hn), so the taps of column i start at slot + i (as k_filter); the n
+// weights of a row go next to it in LDS.
+constexpr int NR = 1024;   // threads of the row kernels
+__device__ __forceinline__ void stage_row(double *slot, const double *x, int nlon, int hn) {
+  for (int t = threadIdx.x; t < nlon + 2 * hn; t += NR) {
+    int c = (t - hn) % nlon;
+    if (c < 0) c += nlon;
+    slot[t] = x[c];
   }
 }
-
-// lam gz_f from the pgf terms: pgf_lon = (gzf(i+1) - gzf(i)) / de_lon with
-// lam pgf_lon = -W_DU, pgf_lat likewise
-__global__ void a_pgf(Grid g, Mesh M, ADPtrs A) {
-  const int i = blockIdx.x * BX + threadIdx.x, r = blockIdx.y * BY + threadIdx.y;
-  if (i >= g.nlon || r >= g.nlat) return;
-  const int64_t p = g.pitch, k = r * p + i;
-  const int im = wrap(i - 1, g.nlon);
-  const double *du = A.w[W_DU], *dv = A.w[W_DV];
-  double s = 0.0;
-  if (r >= 1 && r <= g.nlat - 2) s += (du[k] - du[r * p + im]) / M(GMC_DE_LON, r);
-  if (r >= 1) s += -dv[k - p] / M(GMC_DE_LAT, r - 1);
-  if (r <= g.nhalf - 1) s += dv[k] / M(GMC_DE_LAT, r);
-  A.w[W_GZF][k] = s;
+__device__ __forceinline__ void stage_weights(double *dst, const double *src, int n) {
+  for (int k = threadIdx.x; k < n; k += NR) dst[k] = src[k];
 }
 
-// lam gz_new = BF(lam gz_f) + SF(lam gz) arrives in W_T1 + W_T2
-__global__ void a_gznew(Grid g, double h, double grav, ADPtrs A) {
-  const int i = blockIdx.x * BX + threadIdx.x, r = blockIdx.y * BY + threadIdx.y;
-  if (i >= g.nlon || r >= g.nlat) return;
-  const int64_t k = r * g.pitch + i;
-  const double lg = A.w[W_T1][k] + A.w[W_T2][k];
-  A.o_gz[k] += lg;
-  A.w[W_DGZ][k] = -(h * lg) * grav;   // lam dmfdlon = lam dmfdlat
+// v rows that share a workgroup of a_uvrow: the pole smoothing's transpose
+// couples v rows 0 and 1, and nhalf-2 and nhalf-1
+__host__ __device__ inline int v_groups(int nhalf) { return nhalf < 4 ? 1 : nhalf - 2; }
+__host__ __device__ inline int v_group_max(int nhalf) { return nhalf < 4 ? nhalf : 2; }
+__device__ __forceinline__ void v_group(int q, int nhalf, int &ra, int &rb) {
+  if (nhalf < 4) { ra = 0; rb = nhalf - 1; }
+  else if (q == 0) { ra = 0; rb = 1; }
+  else if (q == nhalf - 3) { ra = nhalf - 2; rb = nhalf - 1; }
+  else { ra = rb = q + 1; }
 }
 
-// lam ke from grad_ke (lam dkedlon = -lam du, lam dkedlat = -lam dv)
-__global__ void a_ke(Grid g, Mesh M, ADPtrs A) {
-  const int i = blockIdx.x * BX + threadIdx.x, r = blockIdx.y * BY + threadIdx.y;
-  if (i >= g.nlon || r >= g.nlat) return;
-  const int64_t p = g.pitch, k = r * p + i;
-  const int im = wrap(i - 1, g.nlon);
-  const double *du = A.w[W_DU], *dv = A.w[W_DV];
-  double s = 0.0;
-  if (r >= 1 && r <= g.nlat - 2) s += (du[k] - du[r * p + im]) / M(GMC_DE_LON, r);
-  if (r >= 1) s += -dv[k - p] / M(GMC_DE_LAT, r - 1);
-  if (r <= g.nhalf - 1) s += dv[k] / M(GMC_DE_LAT, r);
-  A.w[W_KE][k] = s;
-}
-
-// sums over the pole rows: ke (pole cells), dmfdlat (pole cells, from
-// W_DGZ), vor (vertex rows next to the poles); one wavefront per pole
-__global__ void a_pole(Grid g, int which, ADPtrs A) {
-  const int north = blockIdx.x, lane = threadIdx.x;
+// Adjoint of the u, v update with its filters, one workgroup per item: items
+// [0, nlat) are the u rows, then the groups of v rows (v_group).  With lam
+// u_f, lam v_f the adjoints of u_f = BF(u), v_f = BF(v) of the new state (BF
+// symmetric, so its transpose is itself): lam u* = BF(lam u_f) + lam u, lam
+// v* likewise, then the transpose of the pole-row smoothing (v(0) = 0.4 v(0)
+// + 0.6 v(1), v(nhalf-1) likewise); lam_o u, v += lam u*, lam v* and W_DU,
+// W_DV = h lam u*, h lam v* (the adjoints of the tendencies, which enter the
+// update as h du, h dv).  The new state's u, v, u_f, v_f adjoints are
+// consumed: each is cleared once read.  LDS: a u row takes its weights and
+// staged row; a v group its rows' weights, staged rows and lam v* rows.
+struct UVRow {
+  double *n_u, *n_v, *n_uf, *n_vf, *o_u, *o_v, *du, *dv;
+};
+__global__ void __launch_bounds__(NR) a_uvrow(Grid g, FilterDev f, double h, UVRow A) {
+  extern __shared__ double xs[];
   const int64_t p = g.pitch;
-  if (which == 0) {
-    const int cr = north ? g.nlat - 1 : 0;
-    const double *ke = A.w[W_KE] + cr * p, *dg = A.w[W_DGZ] + cr * p;
-    const double sk = wave_zonal_sum(g.nlon, lane, [&](int i) { return ke[i]; });
-    const double sd = wave_zonal_sum(g.nlon, lane, [&](int i) { return dg[i]; });
+  const int span = g.nlon + f.max_taps;
+  if ((int)blockIdx.x < g.nlat) {
+    const int r = blockIdx.x;
+    double *__restrict__ u = A.n_u + r * p;
+    double *__restrict__ uf = A.n_uf + r * p;
+    double *__restrict__ ou = A.o_u + r * p;
+    double *__restrict__ du = A.du + r * p;
+    if (r == 0 || r == g.nlat - 1) {   // u is not formed on the pole rows
+      for (int i = threadIdx.x; i < g.nlon; i += NR) u[i] = uf[i] = 0.0;
+      return;
+    }
+    const int n = f.ngrid_lon[r];
+    double *w = xs, *x = xs + f.max_taps;
+    // the first output's addends are loaded ahead of the barrier
+    const int i0 = threadIdx.x;
+    const double un0 = i0 < g.nlon ? u[i0] : 0.0, uo0 = i0 < g.nlon ? ou[i0] : 0.0;
+    stage_weights(w, f.wgt_lon + (int64_t)r * f.max_taps, n);
+    stage_row(x, uf, g.nlon, n > 1 ? (n - 1) / 2 : 0);
+    __syncthreads();
+    for (int i = threadIdx.x; i < g.nlon; i += NR) {
+      const double un = i == i0 ? un0 : u[i], uo = i == i0 ? uo0 : ou[i];
+      const double lu = (n > 1 ? filter_point(w, n, x + i) : x[i]) + un;
+      ou[i] = uo + lu;
+      du[i] = h * lu;
+      u[i] = 0.0;
+      uf[i] = 0.0;
+    }
+    return;
+  }
+  int ra, rb;
+  v_group(blockIdx.x - g.nlat, g.nhalf, ra, rb);
+  const int G = rb - ra + 1, nl = g.nlon;
+  double *wv = xs, *xv = wv + G * f.max_taps, *lvs = xv + G * span;
+  const int q0 = threadIdx.x;
+  const int64_t k0 = q0 < G * nl ? (ra + q0 / nl) * p + q0 % nl : 0;
+  const double vn0 = q0 < G * nl ? A.n_v[k0] : 0.0, vo0 = q0 < G * nl ? A.o_v[k0] : 0.0;
+  for (int j = 0; j < G; j++) {
+    const int n = f.ngrid_lat[ra + j];
+    stage_weights(wv + j * f.max_taps, f.wgt_lat + (int64_t)(ra + j) * f.max_taps, n);
+    stage_row(xv + j * span, A.n_vf + (ra + j) * p, nl, n > 1 ? (n - 1) / 2 : 0);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < G * nl; q += NR) {
+    const int j = q / nl, i = q - j * nl, r = ra + j, n = f.ngrid_lat[r];
+    const double vn = q == q0 ? vn0 : A.n_v[r * p + i];
+    const double *x = xv + j * span + i;
+    lvs[q] = (n > 1 ? filter_point(wv + j * f.max_taps, n, x) : x[0]) + vn;
+  }
+  __syncthreads();
+  const int last = g.nhalf - 1;
+  const double wgt = 0.6;
+  for (int q = threadIdx.x; q < G * nl; q += NR) {
+    const int j = q / nl, i = q - j * nl, r = ra + j;
+    const int64_t k = r * p + i;
+    double ls = (r == 0 || r == last) ? (1 - wgt) * lvs[q] : lvs[q];
+    if (r == 1) ls += wgt * lvs[q - nl];          // row 0 is in the group
+    if (r == last - 1) ls += wgt * lvs[q + nl];   // and row nhalf-1
+    A.o_v[k] = (q == q0 ? vo0 : A.o_v[k]) + ls;
+    A.dv[k] = h * ls;
+    A.n_v[k] = 0.0;
+    A.n_vf[k] = 0.0;
+  }
+}
+
+// Adjoint of gz_f = BF(gz_new), gz = SF(gz_new) of the new state, of the pgf
+// and of grad_ke, one workgroup per row r.  lam gz_f comes from the pgf
+// terms (pgf_lon = (gz_f(i+1) - gz_f(i)) / de_lon with lam pgf_lon = -lam du,
+// pgf_lat likewise); lam ke has the same form (lam dkedlon = -lam du) and is
+// kept in W_KE.  lam gz_new = BF(lam gz_f) + SF(lam gz); lam_o gz += lam
+// gz_new, W_DGZ = -(h lam gz_new) g (= lam dmfdlon = lam dmfdlat).  The pole
+// rows leave the sums of W_KE and W_DGZ over the row in pole[0..3].  The new
+// state's gz adjoint is consumed and cleared.
+struct GZRow {
+  const double *du, *dv;
+  double *n_gz, *o_gz, *ke, *dgz, *pole;
+};
+__global__ void __launch_bounds__(NR) a_gzrow(Grid g, Mesh M, FilterDev fb, FilterDev fs, double h, double grav,
+                                              GZRow A) {
+  extern __shared__ double xs[];
+  const int r = blockIdx.x;
+  const int64_t p = g.pitch, o = r * p;
+  // LDS: big weights | small weights | lam gz_f (big halo) | lam gz (small halo) | W_DGZ of a pole row
+  double *wb = xs, *ws = wb + fb.max_taps, *xb = ws + fs.max_taps, *xg = xb + g.nlon + fb.max_taps;
+  double *xd = xg + g.nlon + fs.max_taps;
+  const int nb = fb.ngrid_lon[r], hb = nb > 1 ? (nb - 1) / 2 : 0;
+  const int ns = fs.ngrid_lon[r], hs = ns > 1 ? (ns - 1) / 2 : 0;
+  const bool u_row = r >= 1 && r <= g.nlat - 2;
+  const int i0 = threadIdx.x;
+  const double og0 = i0 < g.nlon ? A.o_gz[o + i0] : 0.0;   // ahead of the barrier
+  stage_weights(wb, fb.wgt_lon + (int64_t)r * fb.max_taps, nb);
+  stage_weights(ws, fs.wgt_lon + (int64_t)r * fs.max_taps, ns);
+  for (int t = threadIdx.x; t < g.nlon + 2 * hb; t += NR) {
+    int c = (t - hb) % g.nlon;
+    if (c < 0) c += g.nlon;
+    const int cm = c == 0 ? g.nlon - 1 : c - 1;
+    double s = 0.0;
+    if (u_row) s += (A.du[o + c] - A.du[o + cm]) / M(GMC_DE_LON, r);
+    if (r >= 1) s += -A.dv[o - p + c] / M(GMC_DE_LAT, r - 1);
+    if (r <= g.nhalf - 1) s += A.dv[o + c] / M(GMC_DE_LAT, r);
+    xb[t] = s;
+    if (t >= hb && t < hb + g.nlon) A.ke[o + c] = s;
+  }
+  stage_row(xg, A.n_gz + o, g.nlon, hs);
+  __syncthreads();
+  const bool pole = r == 0 || r == g.nlat - 1;
+  for (int i = threadIdx.x; i < g.nlon; i += NR) {
+    const double og = i == i0 ? og0 : A.o_gz[o + i];
+    const double lg =
+        (nb > 1 ? filter_point(wb, nb, xb + i) : xb[i]) + (ns > 1 ? filter_point(ws, ns, xg + i) : xg[i]);
+    A.o_gz[o + i] = og + lg;
+    const double d = -(h * lg) * grav;
+    A.dgz[o + i] = d;
+    A.n_gz[o + i] = 0.0;
+    if (pole) xd[i] = d;
+  }
+  if (!pole) return;   // the same for the whole workgroup
+  __syncthreads();
+  if (threadIdx.x < kWave) {
+    const int lane = threadIdx.x, north = r != 0;
+    const double sk = wave_zonal_sum(g.nlon, lane, [&](int i) { return xb[hb + i]; });
+    const double sd = wave_zonal_sum(g.nlon, lane, [&](int i) { return xd[i]; });
     if (lane == 0) {
       A.pole[north] = sk;
       A.pole[2 + north] = sd;
     }
-  } else {
-    const int vr = north ? g.nhalf - 1 : 0;
-    const double *vo = A.w[W_VOR] + vr * p;
-    const double s = wave_zonal_sum(g.nlon, lane, [&](int i) { return vo[i]; });
-    if (lane == 0) A.pole[4 + north] = s;
   }
 }
 
@@ -470,21 +517,21 @@ __global__ void a_cori(Grid g, Mesh M, Opt o, Tape T, ADPtrs A) {
 }
 
 // interp_pv transposed onto pv (vertex rows), then pv = (vor + f) / m_vtx
-// transposed: W_VOR = lam vor, W_MVTX = lam m_vtx
+// transposed: W_VOR = lam vor, W_MVTX = lam m_vtx.  The upwind weights df of
+// every edge come from the tape.
 __global__ void a_pv(Grid g, Mesh M, Opt o, Tape T, ADPtrs A) {
   const int i = blockIdx.x * BX + threadIdx.x, R = blockIdx.y * BY + threadIdx.y;
   if (i >= g.nlon || R >= g.nhalf) return;
   const int64_t p = g.pitch, k = R * p + i;
-  auto PV = [&](int rr, int ii) { mirror_half(rr, ii, g); return T.pv[rr * p + ii]; };
   const double *lpl = A.w[W_PVLON], *lpt = A.w[W_PVLAT];
   double s = 0.0;
   // pv_lon(r, i) reads pv(r-2+kk, i)
+#pragma unroll
   for (int kk = 0; kk < 4; kk++) {
     const int r = R + 2 - kk;
     if (r < 1 || r > g.nlat - 2) continue;
     const int64_t q = r * p + i;
-    const UpwD d = upwind_d(o, T.v_lon[q], T.u_f[q], 0, PV(r - 2, i), PV(r - 1, i), PV(r, i), PV(r + 1, i));
-    s += lpl[q] * d.df[kk];
+    s += lpl[q] * T.dlon[2 + kk][q];
   }
   // mirrored reads across the poles: pv_lon(1, i') reads pv(0, i'+half) as f1,
   // pv_lon(nlat-2, i') reads pv(nhalf-1, i'+half) as f4
@@ -492,69 +539,81 @@ __global__ void a_pv(Grid g, Mesh M, Opt o, Tape T, ADPtrs A) {
     const int r = R == 0 ? 1 : g.nlat - 2, kk = R == 0 ? 0 : 3;
     const int ii = wrap(i + g.half, g.nlon);   // i' = i - half = i + half (mod nlon)
     const int64_t q = r * p + ii;
-    const UpwD d = upwind_d(o, T.v_lon[q], T.u_f[q], 0, PV(r - 2, ii), PV(r - 1, ii), PV(r, ii), PV(r + 1, ii));
-    s += lpl[q] * d.df[kk];
+    s += lpl[q] * T.dlon[2 + kk][q];
   }
   // pv_lat(R, i') reads pv(R, i'-2+kk)
+#pragma unroll
   for (int kk = 0; kk < 4; kk++) {
-    const int ii = wrap(i + 2 - kk, g.nlon);
-    const int64_t q = R * p + ii;
-    const UpwD d = upwind_d(o, T.u_lat[q], T.v_f[q], 1, T.pv[R * p + wrap(ii - 2, g.nlon)],
-                            T.pv[R * p + wrap(ii - 1, g.nlon)], T.pv[q], T.pv[R * p + wrap(ii + 1, g.nlon)]);
-    s += lpt[q] * d.df[kk];
+    const int64_t q = R * p + wrap(i + 2 - kk, g.nlon);
+    s += lpt[q] * T.dlat[2 + kk][q];
   }
   A.w[W_PV][k] = s;
   A.w[W_VOR][k] = s / T.m_vtx[k];
   A.w[W_MVTX][k] = -s * T.pv[k] / T.m_vtx[k];
 }
 
-// u_lat = mfx_lat / m_lat and v_lon = mfy_lon / m_lon transposed, with the
-// upwind weights' dependence on t: W_MFXLAT, W_MFYLON = lam of the tangential
-// mass fluxes, W_ULAT = lam u_lat, W_VLON = lam v_lon (kept for a_mf)
-__global__ void a_lat(Grid g, Mesh M, Opt o, Tape T, ADPtrs A) {
-  const int i = blockIdx.x * BX + threadIdx.x, r = blockIdx.y * BY + threadIdx.y;
-  if (i >= g.nlon || r >= g.nlat) return;
-  const int64_t p = g.pitch, k = r * p + i;
-  auto PV = [&](int rr, int ii) { mirror_half(rr, ii, g); return T.pv[rr * p + ii]; };
-  if (r <= g.nhalf - 1) {
-    const UpwD d = upwind_d(o, T.u_lat[k], T.v_f[k], 1, T.pv[r * p + wrap(i - 2, g.nlon)],
-                            T.pv[r * p + wrap(i - 1, g.nlon)], T.pv[k], T.pv[r * p + wrap(i + 1, g.nlon)]);
-    double lul = A.w[W_PVLAT][k] * d.dt;
-    if (o.stokes && r == 0) lul += -A.pole[4] * M(GMC_LE_LAT, 0) / (double)g.nlon / M(GMC_AREA_CELL, 0);
-    if (o.stokes && r == g.nhalf - 1)
-      lul += A.pole[5] * M(GMC_LE_LAT, r) / (double)g.nlon / M(GMC_AREA_CELL, g.nlat - 1);
-    double lmf = lul / T.m_lat[k];
-    if (o.coriolis != 1) lmf += -A.w[W_DV][k] * T.pv_lat[k];
-    A.w[W_ULAT][k] = lul;
-    A.w[W_MFXLAT][k] = lmf;
+// a_lat and a_mf of one BX x BY tile.  a_lat transposes u_lat = mfx_lat /
+// m_lat and v_lon = mfy_lon / m_lon, with the upwind weights' dependence on
+// their tangential wind: lam u_lat, lam mfx_lat (V rows) and lam v_lon, lam
+// mfy_lon (U rows), formed in LDS over the tile and a one-point ring.  a_mf
+// transposes the mass fluxes onto lam u_f, lam v_f of the star state (from
+// the mass fluxes, ke, vor and the upwind weights) and forms W_MLON, W_MLAT
+// = lam m_lon, lam m_lat.
+constexpr int LX = BX + 2, LY = BY + 2;
+__global__ void __launch_bounds__(BX * BY) a_latmf(Grid g, Mesh M, Opt o, Tape T, ADPtrs A) {
+  __shared__ double s_ul[LY][LX], s_mxl[LY][LX], s_vl[LY][LX], s_myl[LY][LX];
+  __shared__ double s_vor[2];
+  const int C0 = blockIdx.x * BX, R0 = blockIdx.y * BY;
+  const int64_t p = g.pitch;
+  // lam vor summed over the vertex rows next to the poles (the Stokes pole
+  // vorticity), where the ring reaches them; one wavefront per row
+  if (o.stokes && threadIdx.y < 2) {
+    const int vr = threadIdx.y == 0 ? 0 : g.nhalf - 1;
+    if (R0 - 1 <= vr && vr <= R0 + BY) {
+      const double *vo = A.w[W_VOR] + vr * p;
+      const double s = wave_zonal_sum(g.nlon, threadIdx.x, [&](int i) { return vo[i]; });
+      if (threadIdx.x == 0) s_vor[threadIdx.y] = s;
+    }
   }
-  if (r >= 1 && r <= g.nlat - 2) {
-    const UpwD d = upwind_d(o, T.v_lon[k], T.u_f[k], 0, PV(r - 2, i), PV(r - 1, i), PV(r, i), PV(r + 1, i));
-    const double lvl = A.w[W_PVLON][k] * d.dt;
-    double lmf = lvl / T.m_lon[k];
-    if (o.coriolis != 1) lmf += A.w[W_DU][k] * T.pv_lon[k];
-    A.w[W_VLON][k] = lvl;
-    A.w[W_MFYLON][k] = lmf;
+  __syncthreads();
+  for (int t = threadIdx.y * BX + threadIdx.x; t < LY * LX; t += BX * BY) {
+    const int ly = t / LX, lx = t - ly * LX;
+    const int r = R0 - 1 + ly;
+    if (r < 0 || r > g.nlat - 1) continue;
+    int i = (C0 - 1 + lx) % g.nlon;
+    if (i < 0) i += g.nlon;
+    const int64_t k = r * p + i;
+    if (r <= g.nhalf - 1) {
+      double lul = A.w[W_PVLAT][k] * T.dlat[0][k];
+      if (o.stokes && r == 0) lul += -s_vor[0] * M(GMC_LE_LAT, 0) / (double)g.nlon / M(GMC_AREA_CELL, 0);
+      if (o.stokes && r == g.nhalf - 1)
+        lul += s_vor[1] * M(GMC_LE_LAT, r) / (double)g.nlon / M(GMC_AREA_CELL, g.nlat - 1);
+      double lmf = lul / T.m_lat[k];
+      if (o.coriolis != 1) lmf += -A.w[W_DV][k] * T.pv_lat[k];
+      s_ul[ly][lx] = lul;
+      s_mxl[ly][lx] = lmf;
+    }
+    if (r >= 1 && r <= g.nlat - 2) {
+      const double lvl = A.w[W_PVLON][k] * T.dlon[0][k];
+      double lmf = lvl / T.m_lon[k];
+      if (o.coriolis != 1) lmf += A.w[W_DU][k] * T.pv_lon[k];
+      s_vl[ly][lx] = lvl;
+      s_myl[ly][lx] = lmf;
+    }
   }
-}
-
-// mass fluxes transposed: lam u_f, lam v_f of the star state (from the mass
-// fluxes, ke, vor and the upwind weights) and lam m_lon, lam m_lat
-__global__ void a_mf(Grid g, Mesh M, Opt o, Tape T, ADPtrs A) {
-  const int i = blockIdx.x * BX + threadIdx.x, r = blockIdx.y * BY + threadIdx.y;
+  __syncthreads();
+  const int i = C0 + threadIdx.x, r = R0 + threadIdx.y;
   if (i >= g.nlon || r >= g.nlat) return;
-  const int64_t p = g.pitch, k = r * p + i;
+  const int lx = threadIdx.x + 1, ly = threadIdx.y + 1;
+  const int64_t k = r * p + i;
   const int im = wrap(i - 1, g.nlon), ip = wrap(i + 1, g.nlon);
-  auto PV = [&](int rr, int ii) { mirror_half(rr, ii, g); return T.pv[rr * p + ii]; };
   const double *lke = A.w[W_KE], *lvo = A.w[W_VOR];
   auto stokes_row = [&](int rr) { return o.stokes && (rr == 0 || rr == g.nhalf - 1); };
   if (r >= 1 && r <= g.nlat - 2) {
     // lam mfx total: grad_mf + Coriolis (a_cori) + mfx_lat transposed
     double lmfx = A.w[W_MFX][k];
-    if (r <= g.nhalf - 1)
-      lmfx += M(GMC_HALF_TANGENT_WGT1, r) * (A.w[W_MFXLAT][r * p + ip] + A.w[W_MFXLAT][k]);
-    if (r - 1 >= 0)
-      lmfx += M(GMC_HALF_TANGENT_WGT2, r - 1) * (A.w[W_MFXLAT][(r - 1) * p + ip] + A.w[W_MFXLAT][(r - 1) * p + i]);
+    if (r <= g.nhalf - 1) lmfx += M(GMC_HALF_TANGENT_WGT1, r) * (s_mxl[ly][lx + 1] + s_mxl[ly][lx]);
+    lmfx += M(GMC_HALF_TANGENT_WGT2, r - 1) * (s_mxl[ly - 1][lx + 1] + s_mxl[ly - 1][lx]);
     const double u = T.u_f[k];
     double luf = lmfx * T.m_lon[k];
     // ke(r, i+1) reads u as west, ke(r, i) as east
@@ -562,18 +621,16 @@ __global__ void a_mf(Grid g, Mesh M, Opt o, Tape T, ADPtrs A) {
            lke[k] * M(GMC_AREA_LON_EAST, r) * 2 * u / M(GMC_AREA_CELL, r);
     // vor(r, i) reads u * de_lon(r), vor(r-1, i) reads -u * de_lon(r)
     if (r <= g.nhalf - 1 && !stokes_row(r)) luf += lvo[k] * M(GMC_DE_LON, r) / M(GMC_AREA_VTX, r);
-    if (r - 1 >= 0 && !stokes_row(r - 1)) luf += -lvo[k - p] * M(GMC_DE_LON, r) / M(GMC_AREA_VTX, r - 1);
+    if (!stokes_row(r - 1)) luf += -lvo[k - p] * M(GMC_DE_LON, r) / M(GMC_AREA_VTX, r - 1);
     // the upwind weight of pv_lon reads u_f as its normal component
-    const UpwD d = upwind_d(o, T.v_lon[k], u, 0, PV(r - 2, i), PV(r - 1, i), PV(r, i), PV(r + 1, i));
-    luf += A.w[W_PVLON][k] * d.dn;
+    luf += A.w[W_PVLON][k] * T.dlon[1][k];
     A.s_uf[k] += luf;
-    A.w[W_MLON][k] = lmfx * u - A.w[W_VLON][k] * T.v_lon[k] / T.m_lon[k];
+    A.w[W_MLON][k] = lmfx * u - s_vl[ly][lx] * T.v_lon[k] / T.m_lon[k];
   }
   if (r <= g.nhalf - 1) {
     double lmfy = A.w[W_MFY][k];
-    if (r + 1 <= g.nlat - 2)
-      lmfy += M(GMC_FULL_TANGENT_WGT1, r + 1) * (A.w[W_MFYLON][k + p] + A.w[W_MFYLON][(r + 1) * p + im]);
-    if (r >= 1) lmfy += M(GMC_FULL_TANGENT_WGT2, r) * (A.w[W_MFYLON][k] + A.w[W_MFYLON][r * p + im]);
+    if (r + 1 <= g.nlat - 2) lmfy += M(GMC_FULL_TANGENT_WGT1, r + 1) * (s_myl[ly + 1][lx] + s_myl[ly + 1][lx - 1]);
+    if (r >= 1) lmfy += M(GMC_FULL_TANGENT_WGT2, r) * (s_myl[ly][lx] + s_myl[ly][lx - 1]);
     const double v = T.v_f[k];
     double lvf = lmfy * T.m_lat[k];
     if (r + 1 <= g.nlat - 2) lvf += lke[k + p] * M(GMC_AREA_LAT_NORTH, r) * 2 * v / M(GMC_AREA_CELL, r + 1);
@@ -581,11 +638,9 @@ __global__ void a_mf(Grid g, Mesh M, Opt o, Tape T, ADPtrs A) {
     if (r == 0) lvf += A.pole[0] * 2 * v / (double)g.nlon;
     if (r == g.nhalf - 1) lvf += A.pole[1] * 2 * v / (double)g.nlon;
     if (!stokes_row(r)) lvf += (lvo[r * p + im] - lvo[k]) * M(GMC_DE_LAT, r) / M(GMC_AREA_VTX, r);
-    const UpwD d = upwind_d(o, T.u_lat[k], v, 1, T.pv[r * p + wrap(i - 2, g.nlon)], T.pv[r * p + im], T.pv[k],
-                            T.pv[r * p + ip]);
-    lvf += A.w[W_PVLAT][k] * d.dn;
+    lvf += A.w[W_PVLAT][k] * T.dlat[1][k];
     A.s_vf[k] += lvf;
-    A.w[W_MLAT][k] = lmfy * v - A.w[W_ULAT][k] * T.u_lat[k] / T.m_lat[k];
+    A.w[W_MLAT][k] = lmfy * v - s_ul[ly][lx] * T.u_lat[k] / T.m_lat[k];
   }
 }
 
@@ -657,10 +712,18 @@ struct gmc_tlad {
   double *X[4][X_N] = {};            // perturbation / adjoint of state levels 1..3
   double *W[W_N] = {};
   double *pole = nullptr;
-  double *tape1[5] = {};   // m_lon, m_lat, m_vtx, u_f, v_f of stage 1's star state
-  // graphs of one adjoint step, keyed by (old, new, dt)
-  struct StepGraph { int o, n; double dt; hipGraphExec_t exec; };
-  std::vector<StepGraph> graphs;
+  size_t lds_uv = 0, lds_gz = 0;   // dynamic LDS of a_uvrow, a_gzrow
+  // tapes of one checkpoint segment: tapes[3 s + st] of stage st of step s
+  double *tape_slab = nullptr;
+  int tape_steps = 0;
+  std::vector<TapeOut> tapes;
+  std::vector<double *> ck;          // saved states (checkpoints), kNState fields each
+  double *obs_buf = nullptr;         // observation cells, values and cost terms
+  int obs_cap = 0;
+  // graphs of a segment's taping forward (kind 0) and of its adjoint sweep
+  // (kind 1), keyed by (kind, levels, steps, dt)
+  struct SegGraph { int kind, o, n, steps; double dt; hipGraphExec_t exec; };
+  std::vector<SegGraph> graphs;
   std::string err;
 };
 
@@ -723,34 +786,26 @@ void tl_stage(gmc_tlad *a, int o, int s, int n, double h) {
 }
 
 // AD of one stage around its tape T (ops_stage_tape): consumes the adjoint
-// of the new state (level n) and adds into levels o and s
+// of the new state (level n, cleared as it is read) and adds into levels o
+// and s.  Six launches: the two row kernels of the update and its filters,
+// then the tendencies' transposes from the Coriolis term back to m.
 void ad_stage(gmc_tlad *a, int o, int s, int n, double h, const Tape &T) {
   gmc_ctx *c = a->c;
   const Grid &g = c->grid;
   const Mesh M{c->d_mesh, g.nlat};
   const Opt op = opt_of(c);
   const dim3 b(BX, BY);
-  ADPtrs A{a->X[n][X_U], a->X[n][X_V], a->X[n][X_GZ], a->X[n][X_UF], a->X[n][X_VF], a->X[o][X_U], a->X[o][X_V],
-           a->X[o][X_GZ], a->X[s][X_GZ], a->X[s][X_UF], a->X[s][X_VF], {}, a->pole};
-  for (int k = 0; k < W_N; k++) A.w[k] = a->W[k];
-  // u_f = BF(u), v_f = BF(v) of the new state: lam u += BF(lam u_f)
-  ops_filter(c, 0, 0, A.n_uf, a->W[W_T1]);
-  ops_filter(c, 0, 1, A.n_vf, a->W[W_T2]);
-  hipLaunchKernelGGL(a_add, grid2d(g, g.nlat), b, 0, c->stream, g, g.nlat, a->W[W_T1], A.n_u, a->W[W_T1]);
-  hipLaunchKernelGGL(a_add, grid2d(g, g.nhalf), b, 0, c->stream, g, g.nhalf, a->W[W_T2], A.n_v, a->W[W_T2]);
-  hipLaunchKernelGGL(a_uv, grid2d(g, g.nlat), b, 0, c->stream, g, h, A);
-  hipLaunchKernelGGL(a_pgf, grid2d(g, g.nlat), b, 0, c->stream, g, M, A);
-  // gz_f = BF(gz_new), gz = SF(gz_new)
-  ops_filter(c, 0, 0, a->W[W_GZF], a->W[W_T1]);
-  ops_filter(c, 1, 0, A.n_gz, a->W[W_T2]);
-  hipLaunchKernelGGL(a_gznew, grid2d(g, g.nlat), b, 0, c->stream, g, h, op.grav, A);
-  hipLaunchKernelGGL(a_ke, grid2d(g, g.nlat), b, 0, c->stream, g, M, A);
-  hipLaunchKernelGGL(a_pole, dim3(2), dim3(kWave), 0, c->stream, g, 0, A);
+  double **Xo = a->X[o], **Xs = a->X[s], **Xn = a->X[n], **W = a->W;
+  ADPtrs A{Xn[X_U], Xn[X_V], Xn[X_GZ], Xn[X_UF], Xn[X_VF], Xo[X_U], Xo[X_V], Xo[X_GZ], Xs[X_GZ], Xs[X_UF], Xs[X_VF],
+           {}, a->pole};
+  for (int k = 0; k < W_N; k++) A.w[k] = W[k];
+  UVRow U{Xn[X_U], Xn[X_V], Xn[X_UF], Xn[X_VF], Xo[X_U], Xo[X_V], W[W_DU], W[W_DV]};
+  hipLaunchKernelGGL(a_uvrow, dim3(g.nlat + v_groups(g.nhalf)), dim3(NR), a->lds_uv, c->stream, g, c->big, h, U);
+  GZRow Z{W[W_DU], W[W_DV], Xn[X_GZ], Xo[X_GZ], W[W_KE], W[W_DGZ], a->pole};
+  hipLaunchKernelGGL(a_gzrow, dim3(g.nlat), dim3(NR), a->lds_gz, c->stream, g, M, c->big, c->small1, h, op.grav, Z);
   hipLaunchKernelGGL(a_cori, grid2d(g, g.nlat), b, 0, c->stream, g, M, op, T, A);
   hipLaunchKernelGGL(a_pv, grid2d(g, g.nhalf), b, 0, c->stream, g, M, op, T, A);
-  hipLaunchKernelGGL(a_pole, dim3(2), dim3(kWave), 0, c->stream, g, 1, A);
-  hipLaunchKernelGGL(a_lat, grid2d(g, g.nlat), b, 0, c->stream, g, M, op, T, A);
-  hipLaunchKernelGGL(a_mf, grid2d(g, g.nlat), b, 0, c->stream, g, M, op, T, A);
+  hipLaunchKernelGGL(a_latmf, grid2d(g, g.nlat), b, 0, c->stream, g, M, op, T, A);
   hipLaunchKernelGGL(a_m, grid2d(g, g.nlat), b, 0, c->stream, g, M, op, A);
 }
 
@@ -805,46 +860,6 @@ int tl_step(gmc_tlad *a, int o, int n, double dt) {
   return thip(a, hipGetLastError(), "tl step");
 }
 
-// AD of one step whose old state is in level o: X[n] (adjoint of the step's
-// result) -> added into X[o].  The stages run forward once through the fused
-// kernels (the same bits as the operator path), each leaving its tape in its
-// star level: stage 0 in o, stage 1 in n, stage 2 in 3.  Stage 2 writes its
-// new state into level n, over the u_f, v_f of stage 1's tape, so stage 1's
-// tape fields that live in level n are kept aside after stage 1.
-int ad_step_body(gmc_tlad *a, int o, int n, double dt) {
-  gmc_ctx *c = a->c;
-  const int ns = nstages(c);
-  int rc = GMC_OK;
-  for (int st = 0; st < ns && !rc; st++) {
-    int so, ss, sn;
-    double h;
-    stage_of(c->cfg.time_scheme, o, n, dt, st, so, ss, sn, h);
-    rc = fused_stage_tape(c, so, ss, sn, h);
-    if (!rc && st == 1 && ns > 2) {
-      double *src[5] = {c->state[ss][S_M_LON], c->state[ss][S_M_LAT], c->state[ss][S_M_VTX], c->state[ss][S_U_F],
-                        c->state[ss][S_V_F]};
-      rc = copy_fields(a, a->tape1, src, 5);
-    }
-  }
-  for (int st = ns - 1; st >= 0 && !rc; st--) {
-    int so, ss, sn;
-    double h;
-    stage_of(c->cfg.time_scheme, o, n, dt, st, so, ss, sn, h);
-    Tape T = tape_of(c, ss);
-    if (st == 1 && ns > 2) {
-      T.m_lon = a->tape1[0];
-      T.m_lat = a->tape1[1];
-      T.m_vtx = a->tape1[2];
-      T.u_f = a->tape1[3];
-      T.v_f = a->tape1[4];
-    }
-    ad_stage(a, so, ss, sn, h, T);
-    rc = zero_level(a, sn);   // the new state's adjoint is consumed
-  }
-  if (rc) return rc;
-  return thip(a, hipGetLastError(), "ad step");
-}
-
 }  // namespace
 
 extern "C" {
@@ -858,7 +873,17 @@ int gmc_tlad_create(gmc_ctx *c, gmc_tlad **out) {
   a->c = c;
   const Grid &g = c->grid;
   a->fld = (size_t)g.nlat * g.pitch;
-  const size_t n = a->fld * (3 * X_N + W_N + 5) + 8;
+  // the row kernels stage whole rows with their filter halos in LDS
+  const size_t span = (size_t)g.nlon + c->big.max_taps;
+  a->lds_uv = sizeof(double) * std::max(c->big.max_taps + span,
+                                        v_group_max(g.nhalf) * (c->big.max_taps + span + g.nlon));
+  a->lds_gz = sizeof(double) * (3 * (size_t)g.nlon + 2 * ((size_t)c->big.max_taps + c->small1.max_taps));
+  if (std::max(a->lds_uv, a->lds_gz) > 65536) {
+    const int rc = terr(a, GMC_EINVAL, "rows too long for the adjoint's row kernels (64 KiB of LDS)");
+    gmc_tlad_destroy(a);
+    return rc;
+  }
+  const size_t n = a->fld * (3 * X_N + W_N) + 8;
   int rc = thip(a, hipMalloc((void **)&a->slab, n * sizeof(double)), "tlad alloc");
   if (!rc) rc = thip(a, hipMemset(a->slab, 0, n * sizeof(double)), "tlad memset");
   if (rc) {
@@ -869,7 +894,6 @@ int gmc_tlad_create(gmc_ctx *c, gmc_tlad **out) {
   for (int lv = 1; lv <= 3; lv++)
     for (int f = 0; f < X_N; f++) { a->X[lv][f] = p; p += a->fld; }
   for (int k = 0; k < W_N; k++) { a->W[k] = p; p += a->fld; }
-  for (int k = 0; k < 5; k++) { a->tape1[k] = p; p += a->fld; }
   a->pole = p;
   *out = a;
   return GMC_OK;
@@ -880,6 +904,9 @@ int gmc_tlad_destroy(gmc_tlad *a) {
   if (a->c) (void)hipStreamSynchronize(a->c->stream);
   for (auto &sg : a->graphs) (void)hipGraphExecDestroy(sg.exec);
   if (a->slab) (void)hipFree(a->slab);
+  if (a->tape_slab) (void)hipFree(a->tape_slab);
+  for (double *p : a->ck) (void)hipFree(p);
+  if (a->obs_buf) (void)hipFree(a->obs_buf);
   delete a;
   return GMC_OK;
 }
@@ -901,9 +928,8 @@ int gmc_tlad_upload(gmc_tlad *a, int32_t level, int32_t var, const double *host,
   if (rc) return rc;
   const Grid &g = a->c->grid;
   if (ld < g.nlon) return terr(a, GMC_EINVAL, "ld < nlon");
-  rc = thip(a, hipMemcpy2DAsync(p, g.pitch * sizeof(double), host, ld * sizeof(double), g.nlon * sizeof(double),
-                                rows, hipMemcpyHostToDevice, a->c->stream), "upload");
-  if (!rc) rc = thip(a, hipStreamSynchronize(a->c->stream), "sync");
+  rc = thip(a, stage_copy(a->c->stage, a->c->stream, true, p, g.pitch, const_cast<double *>(host), ld, g.nlon, rows),
+            "upload");
   return rc;
 }
 
@@ -915,9 +941,7 @@ int gmc_tlad_download(gmc_tlad *a, int32_t level, int32_t var, double *host, int
   if (rc) return rc;
   const Grid &g = a->c->grid;
   if (ld < g.nlon) return terr(a, GMC_EINVAL, "ld < nlon");
-  rc = thip(a, hipMemcpy2DAsync(host, ld * sizeof(double), p, g.pitch * sizeof(double), g.nlon * sizeof(double),
-                                rows, hipMemcpyDeviceToHost, a->c->stream), "download");
-  if (!rc) rc = thip(a, hipStreamSynchronize(a->c->stream), "sync");
+  rc = thip(a, stage_copy(a->c->stage, a->c->stream, false, p, g.pitch, host, ld, g.nlon, rows), "download");
   return rc;
 }
 
@@ -949,40 +973,92 @@ int gmc_tl_step(gmc_tlad *a, double dt, int32_t nsteps) {
 
 namespace {
 
-struct Traj {
-  std::vector<double *> b;   // steps x kNState fields
-  ~Traj() {
-    for (auto p : b)
-      if (p) (void)hipFree(p);
-  }
-};
-
-int alloc_traj(gmc_tlad *a, Traj &t, int nstates) {
-  t.b.assign((size_t)nstates * kNState, nullptr);
-  for (auto &p : t.b) {
+// room for `nstates` saved states (kNState fields each) in a->ck, kept
+// from call to call
+int alloc_traj(gmc_tlad *a, int nstates) {
+  const size_t want = (size_t)nstates * kNState;
+  while (a->ck.size() < want) {
+    double *p = nullptr;
     int rc = thip(a, hipMalloc((void **)&p, a->fld * sizeof(double)), "trajectory alloc");
     if (rc) return rc;
+    a->ck.push_back(p);
   }
   return GMC_OK;
 }
 
+// device copies of the observation cells and values, and their cost terms
+int alloc_obs(gmc_tlad *a, int nobs) {
+  if (nobs <= a->obs_cap) return GMC_OK;
+  int rc = thip(a, hipStreamSynchronize(a->c->stream), "sync");
+  if (a->obs_buf) (void)hipFree(a->obs_buf);
+  a->obs_buf = nullptr;
+  a->obs_cap = 0;
+  if (!rc) rc = thip(a, hipMalloc((void **)&a->obs_buf, sizeof(double) * 3 * (size_t)nobs), "obs alloc");
+  if (!rc) a->obs_cap = nobs;
+  return rc;
+}
 
-// One adjoint step as a graph: the adjoint of the step's result moves from
-// X[o] to X[n], X[o] and X[3] are cleared, then ad_step_body.  The step's
-// initial state is loaded into level o before the launch.
-int ad_step_graph(gmc_tlad *a, int o, int n, double dt) {
+
+// one plain forward step (the fused step graph) from level cur into the
+// other of levels o, n; cur becomes that level
+int fwd_step(gmc_tlad *a, int o, int n, double dt, int &cur) {
+  const int nxt = cur == o ? n : o;
+  const int rc = fused_step_graph(a->c, cur, nxt, dt);
+  cur = nxt;
+  return rc;
+}
+
+constexpr int kTapeFields = 26;   // the fields of a TapeOut
+constexpr int kAdSegment = 8;     // checkpoint interval of gmc_ad_step
+
+// room for the tapes of `steps` steps; graphs captured over an older store
+// are dropped with it
+int ensure_tapes(gmc_tlad *a, int steps) {
+  if (steps <= a->tape_steps) return GMC_OK;
+  int rc = thip(a, hipStreamSynchronize(a->c->stream), "sync");
+  if (rc) return rc;
+  for (auto &sg : a->graphs) (void)hipGraphExecDestroy(sg.exec);
+  a->graphs.clear();
+  if (a->tape_slab) (void)hipFree(a->tape_slab);
+  a->tape_slab = nullptr;
+  a->tape_steps = 0;
+  const size_t n = (size_t)steps * 3 * kTapeFields * a->fld;
+  rc = thip(a, hipMalloc((void **)&a->tape_slab, n * sizeof(double)), "tape alloc");
+  if (rc) return rc;
+  a->tape_steps = steps;
+  a->tapes.assign((size_t)steps * 3, TapeOut{});
+  double *p = a->tape_slab;
+  for (TapeOut &t : a->tapes) {
+    double **f[kTapeFields] = {&t.m_lon,   &t.m_lat,   &t.m_vtx,   &t.mfx,     &t.mfy,     &t.mfx_lat, &t.mfy_lon,
+                               &t.u_lat,   &t.v_lon,   &t.pv,      &t.pv_lon,  &t.pv_lat,  &t.u_f,     &t.v_f,
+                               &t.dlon[0], &t.dlon[1], &t.dlon[2], &t.dlon[3], &t.dlon[4], &t.dlon[5], &t.dlat[0],
+                               &t.dlat[1], &t.dlat[2], &t.dlat[3], &t.dlat[4], &t.dlat[5]};
+    for (int k = 0; k < kTapeFields; k++) { *f[k] = p; p += a->fld; }
+  }
+  return GMC_OK;
+}
+
+Tape tape_from(const TapeOut &t) {
+  Tape T{t.m_lon, t.m_lat, t.m_vtx, t.mfx, t.mfy, t.mfx_lat, t.mfy_lon, t.u_lat, t.v_lon, t.pv, t.pv_lon,
+         t.pv_lat, t.u_f, t.v_f, {}, {}};
+  for (int j = 0; j < 6; j++) {
+    T.dlon[j] = t.dlon[j];
+    T.dlat[j] = t.dlat[j];
+  }
+  return T;
+}
+
+// replay the graph of `body` for this key, capturing it on first use
+int graph_run(gmc_tlad *a, int kind, int o, int n, int steps, double dt, const std::function<int()> &body) {
   gmc_ctx *c = a->c;
   hipGraphExec_t exec = nullptr;
   for (auto &sg : a->graphs)
-    if (sg.o == o && sg.n == n && sg.dt == dt) exec = sg.exec;
+    if (sg.kind == kind && sg.o == o && sg.n == n && sg.steps == steps && sg.dt == dt) exec = sg.exec;
   if (!exec) {
     hipGraph_t graph = nullptr;
     int rc = thip(a, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture");
     if (rc) return rc;
-    int rc2 = copy_fields(a, a->X[n], a->X[o], X_N);
-    if (!rc2) rc2 = zero_level(a, o);
-    if (!rc2) rc2 = zero_level(a, 3);
-    if (!rc2) rc2 = ad_step_body(a, o, n, dt);
+    const int rc2 = body();
     rc = thip(a, hipStreamEndCapture(c->stream, &graph), "end capture");
     if (rc2) {
       if (graph) (void)hipGraphDestroy(graph);
@@ -992,29 +1068,82 @@ int ad_step_graph(gmc_tlad *a, int o, int n, double dt) {
     rc = thip(a, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
     (void)hipGraphDestroy(graph);
     if (rc) return rc;
-    a->graphs.push_back({o, n, dt, exec});
+    a->graphs.push_back({kind, o, n, steps, dt, exec});
   }
   return thip(a, hipGraphLaunch(exec, c->stream), "graph launch");
 }
 
-// adjoint of `steps` steps whose initial states are traj[first ..]: the
-// adjoint of the last result is in X[o] on entry, of the first initial state
-// on return (u_f, v_f parts still separate)
-int ad_sweep(gmc_tlad *a, int o, int n, double dt, Traj &t, int first, int steps) {
-  int rc = GMC_OK;
-  for (int s = first + steps - 1; s >= first && !rc; s--) {
-    rc = load_level(a, o, &t.b[(size_t)s * kNState]);
-    if (!rc) rc = ad_step_graph(a, o, n, dt);
+// The segment's steps forward from level o through the taping fused stages
+// (the same bits as gmc_step); stage st of step s leaves its tape in
+// a->tapes[3 s + st].
+int seg_forward_body(gmc_tlad *a, int o, int n, double dt, int steps) {
+  gmc_ctx *c = a->c;
+  int cur = o, rc = GMC_OK;
+  for (int s = 0; s < steps && !rc; s++) {
+    const int nxt = cur == o ? n : o;
+    for (int st = 0; st < nstages(c) && !rc; st++) {
+      int so, ss, sn;
+      double h;
+      stage_of(c->cfg.time_scheme, cur, nxt, dt, st, so, ss, sn, h);
+      rc = fused_stage_tape(c, so, ss, sn, h, &a->tapes[(size_t)s * 3 + st]);
+    }
+    cur = nxt;
   }
   return rc;
 }
 
-// one plain forward step (the fused step graph) from level cur into the
-// other of levels o, n; cur becomes that level
-int fwd_step(gmc_tlad *a, int o, int n, double dt, int &cur) {
-  const int nxt = cur == o ? n : o;
-  const int rc = fused_step_graph(a->c, cur, nxt, dt);
-  cur = nxt;
+// Adjoint of the segment's steps, last first, around their tapes.  The
+// adjoint of the segment's end state is in X[xin] and the other of levels
+// (o, n) and level 3 are zero; each step consumes its input level and leaves
+// the adjoint of its initial state in the other, so the levels alternate.
+int seg_adjoint_body(gmc_tlad *a, int o, int n, double dt, int xin, int steps) {
+  gmc_ctx *c = a->c;
+  int x_in = xin, x_out = xin == o ? n : o;
+  for (int s = steps - 1; s >= 0; s--) {
+    for (int st = nstages(c) - 1; st >= 0; st--) {
+      int so, ss, sn;
+      double h;
+      stage_of(c->cfg.time_scheme, x_out, x_in, dt, st, so, ss, sn, h);
+      ad_stage(a, so, ss, sn, h, tape_from(a->tapes[(size_t)s * 3 + st]));
+    }
+    std::swap(x_in, x_out);
+  }
+  return thip(a, hipGetLastError(), "adjoint segment");
+}
+
+// Forward over the window from level o, keeping the state every `every`
+// steps in ck; cur is the level of the end state.
+int window_forward(gmc_tlad *a, int o, int n, double dt, int nsteps, int every, double *const *ck, int &cur) {
+  int rc = GMC_OK;
+  cur = o;
+  for (int s = 0; s < nsteps && !rc; s++) {
+    if (s % every == 0) rc = save_level(a, cur, &ck[(size_t)(s / every) * kNState]);
+    if (!rc) rc = fwd_step(a, o, n, dt, cur);
+  }
+  return rc;
+}
+
+// Reverse sweep over the window's checkpoint segments, last first: each
+// segment runs forward from its checkpoint once, keeping its tapes, then
+// its adjoint replays.  On entry the adjoint of the window's end state is in
+// X[o] and X[n], X[3] are zero; on return the adjoint of its initial state
+// is in X[o] (u_f, v_f parts still separate), X[n] and X[3] are zero and the
+// state is back at the window's start in level o.
+int window_adjoint(gmc_tlad *a, int o, int n, double dt, int nsteps, int every, double *const *ck) {
+  int rc = ensure_tapes(a, std::min(every, nsteps));
+  int xin = o;
+  for (int q = (nsteps + every - 1) / every - 1; q >= 0 && !rc; q--) {
+    const int steps = std::min(every, nsteps - q * every), xout = xin == o ? n : o;
+    rc = load_level(a, o, &ck[(size_t)q * kNState]);
+    if (!rc) rc = graph_run(a, 0, o, n, steps, dt, [&] { return seg_forward_body(a, o, n, dt, steps); });
+    if (!rc) rc = graph_run(a, 1, xin, xout, steps, dt, [&] { return seg_adjoint_body(a, o, n, dt, xin, steps); });
+    if (steps % 2) xin = xout;
+  }
+  if (!rc && xin != o) {
+    rc = copy_fields(a, a->X[o], a->X[xin], X_N);
+    if (!rc) rc = zero_level(a, xin);
+  }
+  if (!rc) rc = load_level(a, o, &ck[0]);
   return rc;
 }
 
@@ -1043,17 +1172,16 @@ int gmc_ad_step(gmc_tlad *a, double dt, int32_t nsteps) {
   // the adjoint enters on u, v, gz only
   double *uvf[2] = {a->X[o][X_UF], a->X[o][X_VF]};
   if (!rc) rc = fill_fields(a, uvf, 2);
-  Traj t;
-  if (!rc) rc = alloc_traj(a, t, nsteps + 1);
-  // forward: keep the initial state of every step
-  int cur = o;
-  for (int s = 0; s < nsteps && !rc; s++) {
-    rc = save_level(a, cur, &t.b[(size_t)s * kNState]);
-    if (!rc) rc = fwd_step(a, o, n, dt, cur);
+  if (!rc) rc = zero_level(a, n);
+  if (!rc) rc = zero_level(a, 3);
+  if (!rc && nsteps > 0) {
+    const int every = std::min<int>(nsteps, kAdSegment);
+    int cur = o;
+    rc = alloc_traj(a, (nsteps + every - 1) / every);
+    if (!rc) rc = window_forward(a, o, n, dt, nsteps, every, a->ck.data(), cur);
+    if (!rc) rc = window_adjoint(a, o, n, dt, nsteps, every, a->ck.data());
   }
-  if (!rc) rc = ad_sweep(a, o, n, dt, t, 0, nsteps);
   if (!rc) rc = fold_filters(a, o);
-  if (!rc && nsteps > 0) rc = load_level(a, o, &t.b[0]);
   if (!rc) rc = thip(a, hipStreamSynchronize(c->stream), "sync");
   return rc;
 }
@@ -1078,27 +1206,21 @@ int gmc_gradient(gmc_tlad *a, double dt, int32_t nsteps, int32_t ckpt_every, int
   const int o = c->old_level, n = c->new_level;
   const int nck = (nsteps + ckpt_every - 1) / ckpt_every;
   int rc = fused_prepare(c);
-  Traj ck, seg;
-  if (!rc) rc = alloc_traj(a, ck, nck);
-  if (!rc) rc = alloc_traj(a, seg, ckpt_every);
+  if (!rc) rc = alloc_traj(a, nck);
+  if (!rc) rc = alloc_obs(a, nobs);
+  double *const *ck = a->ck.data();
   // forward with checkpoints every ckpt_every steps
   int cur = o;
-  for (int s = 0; s < nsteps && !rc; s++) {
-    if (s % ckpt_every == 0) rc = save_level(a, cur, &ck.b[(size_t)(s / ckpt_every) * kNState]);
-    if (!rc) rc = fwd_step(a, o, n, dt, cur);
-  }
+  if (!rc) rc = window_forward(a, o, n, dt, nsteps, ckpt_every, ck, cur);
   // observation term at the final state and its adjoint
-  int64_t *d_cell = nullptr;
-  double *d_y = nullptr, *d_jo = nullptr;
+  int64_t *d_cell = (int64_t *)a->obs_buf;
+  double *d_y = a->obs_buf + nobs, *d_jo = a->obs_buf + 2 * (size_t)nobs;
   double jo = 0.0;
   if (!rc) rc = zero_level(a, o);
   if (!rc) rc = zero_level(a, n);
   if (!rc) rc = zero_level(a, 3);
   if (!rc && nobs) {
-    rc = thip(a, hipMalloc((void **)&d_cell, sizeof(int64_t) * nobs), "obs alloc");
-    if (!rc) rc = thip(a, hipMalloc((void **)&d_y, sizeof(double) * nobs), "obs alloc");
-    if (!rc) rc = thip(a, hipMalloc((void **)&d_jo, sizeof(double) * nobs), "obs alloc");
-    if (!rc) rc = thip(a, hipMemcpyAsync(d_cell, obs_cell, sizeof(int64_t) * nobs, hipMemcpyHostToDevice, c->stream), "obs copy");
+    rc = thip(a, hipMemcpyAsync(d_cell, obs_cell, sizeof(int64_t) * nobs, hipMemcpyHostToDevice, c->stream), "obs copy");
     if (!rc) rc = thip(a, hipMemcpyAsync(d_y, obs_value, sizeof(double) * nobs, hipMemcpyHostToDevice, c->stream), "obs copy");
     if (!rc) {
       hipLaunchKernelGGL(k_obs, dim3((nobs + 255) / 256), dim3(256), 0, c->stream, g, nobs, d_cell, d_y,
@@ -1109,24 +1231,9 @@ int gmc_gradient(gmc_tlad *a, double dt, int32_t nsteps, int32_t ckpt_every, int
       for (int k = 0; k < nobs; k++) jo += h[k];
     }
   }
-  // reverse over the checkpoint segments, recomputing each segment forward
-  for (int q = nck - 1; q >= 0 && !rc; q--) {
-    const int first = q * ckpt_every, steps = std::min(ckpt_every, nsteps - first);
-    // the adjoint of the segment's end state is in X[o]; recompute from the
-    // checkpoint without touching it
-    rc = load_level(a, o, &ck.b[(size_t)q * kNState]);
-    cur = o;
-    for (int s = 0; s < steps && !rc; s++) {
-      rc = save_level(a, cur, &seg.b[(size_t)s * kNState]);
-      if (!rc) rc = fwd_step(a, o, n, dt, cur);
-    }
-    if (!rc) rc = ad_sweep(a, o, n, dt, seg, 0, steps);
-  }
+  // reverse over the checkpoint segments, each recomputed forward once
+  if (!rc) rc = window_adjoint(a, o, n, dt, nsteps, ckpt_every, ck);
   if (!rc) rc = fold_filters(a, o);
-  if (!rc) rc = load_level(a, o, &ck.b[0]);
-  if (d_cell) (void)hipFree(d_cell);
-  if (d_y) (void)hipFree(d_y);
-  if (d_jo) (void)hipFree(d_jo);
   if (rc) return rc;
   // background term on the host: J_b = 1/2 |x0 - xb|^2 / sigma_b^2 over u
   // (rows between the poles), v and gz; its gradient added to the adjoint

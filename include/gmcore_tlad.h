@@ -49,8 +49,9 @@ int gmc_tl_step(gmc_tlad *t, double dt, int32_t nsteps);
 
 /* Adjoint model: on entry the adjoint (u, v, gz) of the state nsteps steps
  * after the context's current old state sits on the current old level; on
- * return that level holds the adjoint of the current state (M^T applied).
- * The forward trajectory is recomputed and stored; the context's state is
+ * return that level holds the adjoint of the current state (M^T applied) and
+ * the other adjoint levels are zero.  The trajectory is checkpointed every 8
+ * steps and each segment's tapes recomputed once; the context's state is
  * left where it was. */
 int gmc_ad_step(gmc_tlad *t, double dt, int32_t nsteps);
 
@@ -59,8 +60,10 @@ int gmc_ad_step(gmc_tlad *t, double dt, int32_t nsteps);
  *   J = 1/2 sum_k (gz_N(cell_k) - y_k)^2 / sigma_o^2 + 1/2 |x0 - xb|^2 / sigma_b^2
  * (gz_N: gz after nsteps steps; cell_k = r * nlon + i, distinct; |.| over u on
  * the rows between the poles, v and gz).  Checkpoints of the trajectory every
- * ckpt_every steps; each segment is recomputed forward during the reverse
- * sweep.  Writes J and dJ/du0, dJ/dv0, dJ/dgz0 (host, leading dimension ld);
+ * ckpt_every steps; during the reverse sweep each segment runs forward once
+ * from its checkpoint, keeping every stage's tape (26 fields of the grid per
+ * stage and step: memory grows with ckpt_every), then its adjoint replays
+ * from the tapes.  Writes J and dJ/du0, dJ/dv0, dJ/dgz0 (host, leading dimension ld);
  * the context's state is left at x0 and every adjoint variable at zero. */
 int gmc_gradient(gmc_tlad *t, double dt, int32_t nsteps, int32_t ckpt_every, int32_t nobs,
                  const int64_t *obs_cell, const double *obs_value, double sigma_o, const double *u_b,
