@@ -713,16 +713,21 @@ struct gmc_tlad {
   double *W[W_N] = {};
   double *pole = nullptr;
   size_t lds_uv = 0, lds_gz = 0;   // dynamic LDS of a_uvrow, a_gzrow
-  // tapes of one checkpoint segment: tapes[3 s + st] of stage st of step s
+  // tapes of two checkpoint segments (one being written while the other is
+  // read): tapes[(b tape_steps + s) 3 + st] of stage st of step s in buffer b
   double *tape_slab = nullptr;
   int tape_steps = 0;
   std::vector<TapeOut> tapes;
+  // the adjoint sweep of a segment runs on astream while the next segment's
+  // taping forward runs on the context's stream
+  hipStream_t astream = nullptr;
+  hipEvent_t ev_fwd = nullptr, ev_adj[2] = {}, ev_join = nullptr;
   std::vector<double *> ck;          // saved states (checkpoints), kNState fields each
   double *obs_buf = nullptr;         // observation cells, values and cost terms
   int obs_cap = 0;
   // graphs of a segment's taping forward (kind 0) and of its adjoint sweep
-  // (kind 1), keyed by (kind, levels, steps, dt)
-  struct SegGraph { int kind, o, n, steps; double dt; hipGraphExec_t exec; };
+  // (kind 1), keyed by (kind, levels, steps, tape buffer, dt)
+  struct SegGraph { int kind, o, n, steps, buf; double dt; hipGraphExec_t exec; };
   std::vector<SegGraph> graphs;
   std::string err;
 };
@@ -886,6 +891,9 @@ int gmc_tlad_create(gmc_ctx *c, gmc_tlad **out) {
   const size_t n = a->fld * (3 * X_N + W_N) + 8;
   int rc = thip(a, hipMalloc((void **)&a->slab, n * sizeof(double)), "tlad alloc");
   if (!rc) rc = thip(a, hipMemset(a->slab, 0, n * sizeof(double)), "tlad memset");
+  if (!rc) rc = thip(a, hipStreamCreate(&a->astream), "stream");
+  for (hipEvent_t *e : {&a->ev_fwd, &a->ev_adj[0], &a->ev_adj[1], &a->ev_join})
+    if (!rc) rc = thip(a, hipEventCreateWithFlags(e, hipEventDisableTiming), "event");
   if (rc) {
     gmc_tlad_destroy(a);
     return rc;
@@ -902,7 +910,11 @@ int gmc_tlad_create(gmc_ctx *c, gmc_tlad **out) {
 int gmc_tlad_destroy(gmc_tlad *a) {
   if (!a) return GMC_EINVAL;
   if (a->c) (void)hipStreamSynchronize(a->c->stream);
+  if (a->astream) (void)hipStreamSynchronize(a->astream);
   for (auto &sg : a->graphs) (void)hipGraphExecDestroy(sg.exec);
+  for (hipEvent_t e : {a->ev_fwd, a->ev_adj[0], a->ev_adj[1], a->ev_join})
+    if (e) (void)hipEventDestroy(e);
+  if (a->astream) (void)hipStreamDestroy(a->astream);
   if (a->slab) (void)hipFree(a->slab);
   if (a->tape_slab) (void)hipFree(a->tape_slab);
   for (double *p : a->ck) (void)hipFree(p);
@@ -1016,17 +1028,18 @@ constexpr int kAdSegment = 8;     // checkpoint interval of gmc_ad_step
 int ensure_tapes(gmc_tlad *a, int steps) {
   if (steps <= a->tape_steps) return GMC_OK;
   int rc = thip(a, hipStreamSynchronize(a->c->stream), "sync");
+  if (!rc) rc = thip(a, hipStreamSynchronize(a->astream), "sync");
   if (rc) return rc;
   for (auto &sg : a->graphs) (void)hipGraphExecDestroy(sg.exec);
   a->graphs.clear();
   if (a->tape_slab) (void)hipFree(a->tape_slab);
   a->tape_slab = nullptr;
   a->tape_steps = 0;
-  const size_t n = (size_t)steps * 3 * kTapeFields * a->fld;
+  const size_t n = 2 * (size_t)steps * 3 * kTapeFields * a->fld;
   rc = thip(a, hipMalloc((void **)&a->tape_slab, n * sizeof(double)), "tape alloc");
   if (rc) return rc;
   a->tape_steps = steps;
-  a->tapes.assign((size_t)steps * 3, TapeOut{});
+  a->tapes.assign(2 * (size_t)steps * 3, TapeOut{});
   double *p = a->tape_slab;
   for (TapeOut &t : a->tapes) {
     double **f[kTapeFields] = {&t.m_lon,   &t.m_lat,   &t.m_vtx,   &t.mfx,     &t.mfy,     &t.mfx_lat, &t.mfy_lon,
@@ -1038,6 +1051,8 @@ int ensure_tapes(gmc_tlad *a, int steps) {
   return GMC_OK;
 }
 
+TapeOut *tapes_of(gmc_tlad *a, int buf) { return a->tapes.data() + (size_t)buf * a->tape_steps * 3; }
+
 Tape tape_from(const TapeOut &t) {
   Tape T{t.m_lon, t.m_lat, t.m_vtx, t.mfx, t.mfy, t.mfx_lat, t.mfy_lon, t.u_lat, t.v_lon, t.pv, t.pv_lon,
          t.pv_lat, t.u_f, t.v_f, {}, {}};
@@ -1048,12 +1063,15 @@ Tape tape_from(const TapeOut &t) {
   return T;
 }
 
-// replay the graph of `body` for this key, capturing it on first use
-int graph_run(gmc_tlad *a, int kind, int o, int n, int steps, double dt, const std::function<int()> &body) {
+// replay on `stream` the graph of `body` for this key, capturing it (on the
+// context's stream) on first use
+int graph_run(gmc_tlad *a, int kind, int o, int n, int steps, int buf, double dt, hipStream_t stream,
+              const std::function<int()> &body) {
   gmc_ctx *c = a->c;
   hipGraphExec_t exec = nullptr;
   for (auto &sg : a->graphs)
-    if (sg.kind == kind && sg.o == o && sg.n == n && sg.steps == steps && sg.dt == dt) exec = sg.exec;
+    if (sg.kind == kind && sg.o == o && sg.n == n && sg.steps == steps && sg.buf == buf && sg.dt == dt)
+      exec = sg.exec;
   if (!exec) {
     hipGraph_t graph = nullptr;
     int rc = thip(a, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture");
@@ -1068,24 +1086,25 @@ int graph_run(gmc_tlad *a, int kind, int o, int n, int steps, double dt, const s
     rc = thip(a, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
     (void)hipGraphDestroy(graph);
     if (rc) return rc;
-    a->graphs.push_back({kind, o, n, steps, dt, exec});
+    a->graphs.push_back({kind, o, n, steps, buf, dt, exec});
   }
-  return thip(a, hipGraphLaunch(exec, c->stream), "graph launch");
+  return thip(a, hipGraphLaunch(exec, stream), "graph launch");
 }
 
-// The segment's steps forward from level o through the taping fused stages
-// (the same bits as gmc_step); stage st of step s leaves its tape in
-// a->tapes[3 s + st].
-int seg_forward_body(gmc_tlad *a, int o, int n, double dt, int steps) {
+// The segment's steps forward from level `start` (alternating with level
+// `other`) through the taping fused stages (the same bits as gmc_step);
+// stage st of step s leaves its tape in tape buffer buf.
+int seg_forward_body(gmc_tlad *a, int start, int other, double dt, int steps, int buf) {
   gmc_ctx *c = a->c;
-  int cur = o, rc = GMC_OK;
+  TapeOut *tp = tapes_of(a, buf);
+  int cur = start, rc = GMC_OK;
   for (int s = 0; s < steps && !rc; s++) {
-    const int nxt = cur == o ? n : o;
+    const int nxt = cur == start ? other : start;
     for (int st = 0; st < nstages(c) && !rc; st++) {
       int so, ss, sn;
       double h;
       stage_of(c->cfg.time_scheme, cur, nxt, dt, st, so, ss, sn, h);
-      rc = fused_stage_tape(c, so, ss, sn, h, &a->tapes[(size_t)s * 3 + st]);
+      rc = fused_stage_tape(c, so, ss, sn, h, &tp[(size_t)s * 3 + st]);
     }
     cur = nxt;
   }
@@ -1096,15 +1115,16 @@ int seg_forward_body(gmc_tlad *a, int o, int n, double dt, int steps) {
 // adjoint of the segment's end state is in X[xin] and the other of levels
 // (o, n) and level 3 are zero; each step consumes its input level and leaves
 // the adjoint of its initial state in the other, so the levels alternate.
-int seg_adjoint_body(gmc_tlad *a, int o, int n, double dt, int xin, int steps) {
+int seg_adjoint_body(gmc_tlad *a, int o, int n, double dt, int xin, int steps, int buf) {
   gmc_ctx *c = a->c;
+  const TapeOut *tp = tapes_of(a, buf);
   int x_in = xin, x_out = xin == o ? n : o;
   for (int s = steps - 1; s >= 0; s--) {
     for (int st = nstages(c) - 1; st >= 0; st--) {
       int so, ss, sn;
       double h;
       stage_of(c->cfg.time_scheme, x_out, x_in, dt, st, so, ss, sn, h);
-      ad_stage(a, so, ss, sn, h, tape_from(a->tapes[(size_t)s * 3 + st]));
+      ad_stage(a, so, ss, sn, h, tape_from(tp[(size_t)s * 3 + st]));
     }
     std::swap(x_in, x_out);
   }
@@ -1112,33 +1132,56 @@ int seg_adjoint_body(gmc_tlad *a, int o, int n, double dt, int xin, int steps) {
 }
 
 // Forward over the window from level o, keeping the state every `every`
-// steps in ck; cur is the level of the end state.
+// steps in ck; the last segment runs through the taping stages, its tapes
+// left in buffer (segments - 1) % 2.  cur is the level of the end state.
 int window_forward(gmc_tlad *a, int o, int n, double dt, int nsteps, int every, double *const *ck, int &cur) {
-  int rc = GMC_OK;
+  int rc = ensure_tapes(a, std::min(every, nsteps));
+  const int nck = (nsteps + every - 1) / every, last = (nck - 1) * every;
   cur = o;
   for (int s = 0; s < nsteps && !rc; s++) {
     if (s % every == 0) rc = save_level(a, cur, &ck[(size_t)(s / every) * kNState]);
-    if (!rc) rc = fwd_step(a, o, n, dt, cur);
+    if (!rc && s < last) rc = fwd_step(a, o, n, dt, cur);
   }
+  if (rc) return rc;
+  const int start = cur, other = cur == o ? n : o, steps = nsteps - last, buf = (nck - 1) % 2;
+  rc = graph_run(a, 0, start, other, steps, buf, dt, a->c->stream,
+                 [&] { return seg_forward_body(a, start, other, dt, steps, buf); });
+  if (steps % 2) cur = other;
   return rc;
 }
 
-// Reverse sweep over the window's checkpoint segments, last first: each
-// segment runs forward from its checkpoint once, keeping its tapes, then
-// its adjoint replays.  On entry the adjoint of the window's end state is in
-// X[o] and X[n], X[3] are zero; on return the adjoint of its initial state
-// is in X[o] (u_f, v_f parts still separate), X[n] and X[3] are zero and the
-// state is back at the window's start in level o.
+// Reverse sweep over the window's checkpoint segments, last first, after
+// window_forward.  The adjoint of segment q replays from tape buffer q % 2
+// on the adjoint stream while segment q-1 runs forward from its checkpoint
+// on the context's stream into the other buffer (events order each buffer's
+// writer after its last reader).  On entry the adjoint of the window's end
+// state is in X[o] and X[n], X[3] are zero; on return the adjoint of its
+// initial state is in X[o] (u_f, v_f parts still separate), X[n] and X[3]
+// are zero and the state is back at the window's start in level o.
 int window_adjoint(gmc_tlad *a, int o, int n, double dt, int nsteps, int every, double *const *ck) {
-  int rc = ensure_tapes(a, std::min(every, nsteps));
+  hipStream_t F = a->c->stream, A = a->astream;
+  const int nck = (nsteps + every - 1) / every;
+  bool read[2] = {false, false};
   int xin = o;
-  for (int q = (nsteps + every - 1) / every - 1; q >= 0 && !rc; q--) {
-    const int steps = std::min(every, nsteps - q * every), xout = xin == o ? n : o;
-    rc = load_level(a, o, &ck[(size_t)q * kNState]);
-    if (!rc) rc = graph_run(a, 0, o, n, steps, dt, [&] { return seg_forward_body(a, o, n, dt, steps); });
-    if (!rc) rc = graph_run(a, 1, xin, xout, steps, dt, [&] { return seg_adjoint_body(a, o, n, dt, xin, steps); });
+  int rc = thip(a, hipEventRecord(a->ev_fwd, F), "event");
+  for (int q = nck - 1; q >= 0 && !rc; q--) {
+    const int steps = std::min(every, nsteps - q * every), buf = q % 2, xout = xin == o ? n : o, x0 = xin;
+    rc = thip(a, hipStreamWaitEvent(A, a->ev_fwd, 0), "event wait");
+    if (!rc) rc = graph_run(a, 1, x0, xout, steps, buf, dt, A,
+                            [&] { return seg_adjoint_body(a, o, n, dt, x0, steps, buf); });
+    if (!rc) rc = thip(a, hipEventRecord(a->ev_adj[buf], A), "event");
+    read[buf] = true;
     if (steps % 2) xin = xout;
+    if (rc || q == 0) break;
+    const int pbuf = (q - 1) % 2;
+    if (read[pbuf]) rc = thip(a, hipStreamWaitEvent(F, a->ev_adj[pbuf], 0), "event wait");
+    if (!rc) rc = load_level(a, o, &ck[(size_t)(q - 1) * kNState]);
+    if (!rc) rc = graph_run(a, 0, o, n, every, pbuf, dt, F,
+                            [&] { return seg_forward_body(a, o, n, dt, every, pbuf); });
+    if (!rc) rc = thip(a, hipEventRecord(a->ev_fwd, F), "event");
   }
+  if (!rc) rc = thip(a, hipEventRecord(a->ev_join, A), "event");
+  if (!rc) rc = thip(a, hipStreamWaitEvent(F, a->ev_join, 0), "event wait");
   if (!rc && xin != o) {
     rc = copy_fields(a, a->X[o], a->X[xin], X_N);
     if (!rc) rc = zero_level(a, xin);
