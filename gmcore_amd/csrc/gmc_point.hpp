@@ -206,4 +206,21 @@ __device__ __forceinline__ double wave_zonal_sum(int n, int lane, F f) {
   return acc;
 }
 
+// wave_zonal_sum of x[0 .. n-1] read from memory, with each lane's loads
+// issued eight at a time ahead of its additions; the additions, and so the
+// result, are those of wave_zonal_sum.
+__device__ __forceinline__ double wave_zonal_sum_batched(int n, int lane, const double *x) {
+  double acc = 0.0;
+  for (int i = lane; i < n; i += 8 * kWave) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = i + u * kWave < n ? x[i + u * kWave] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (i + u * kWave < n) acc = acc + v[u];
+  }
+  for (int w = 1; w < kWave; w <<= 1) acc = acc + __shfl_xor(acc, w, kWave);
+  return acc;
+}
+
 }  // namespace gmc

@@ -267,19 +267,27 @@ __global__ void t_uv(Grid g, Mesh M, double h, TLPtrs P) {
 // Adjoint kernels (gathers).  lam_n*: adjoint of the new state; the kernels
 // add into the old state's (o_*) and the star state's (s_*) adjoints.
 struct ADPtrs {
-  const double *n_u, *n_v, *n_gz, *n_uf, *n_vf;
+  const double *n_u, *n_v;
+  double *n_gz, *n_uf, *n_vf;   // cleared by a_cori once the row kernels have read them
   double *o_u, *o_v, *o_gz, *s_gz, *s_uf, *s_vf;
   double *w[W_N];
   double *pole;  // [0..1] ke, [2..3] dmfdlat, [4..5] vor: sums of pole-row adjoints
 };
 
-// Zonal-filter staging: slot[t] = x[(t - hn) mod nlon] for t in [0, nlon +
-// 2 hn), so the taps of column i start at slot + i (as k_filter); the n
-// weights of a row go next to it in LDS.
-constexpr int NR = 1024;   // threads of the row kernels
-__device__ __forceinline__ void stage_row(double *slot, const double *x, int nlon, int hn) {
-  for (int t = threadIdx.x; t < nlon + 2 * hn; t += NR) {
-    int c = (t - hn) % nlon;
+// Work items of the adjoint's row kernels: a u row, a group of v rows
+// (rows r0..r1, see v_group) or a gz row, over columns [c0, c1).
+enum { RI_U, RI_V, RI_GZ };
+struct RowItem {
+  int32_t kind, r0, r1, c0, c1, pad;
+};
+
+// Zonal-filter staging: slot[t] = x[(c0 - hn + t) mod nlon] for t in [0, w +
+// 2 hn), so the taps of column c0 + j start at slot + j; a row's n weights
+// are staged next to it.
+constexpr int NR = 512;   // threads of the row kernels
+__device__ __forceinline__ void stage_window(double *slot, const double *x, int nlon, int c0, int w, int hn) {
+  for (int t = threadIdx.x; t < w + 2 * hn; t += NR) {
+    int c = (c0 - hn + t) % nlon;
     if (c < 0) c += nlon;
     slot[t] = x[c];
   }
@@ -288,126 +296,129 @@ __device__ __forceinline__ void stage_weights(double *dst, const double *src, in
   for (int k = threadIdx.x; k < n; k += NR) dst[k] = src[k];
 }
 
-// v rows that share a workgroup of a_uvrow: the pole smoothing's transpose
+// v rows that share a work item of a_uvrow: the pole smoothing's transpose
 // couples v rows 0 and 1, and nhalf-2 and nhalf-1
+constexpr int kMaxVGroup = 3;
 __host__ __device__ inline int v_groups(int nhalf) { return nhalf < 4 ? 1 : nhalf - 2; }
-__host__ __device__ inline int v_group_max(int nhalf) { return nhalf < 4 ? nhalf : 2; }
-__device__ __forceinline__ void v_group(int q, int nhalf, int &ra, int &rb) {
+__host__ __device__ inline void v_group(int q, int nhalf, int &ra, int &rb) {
   if (nhalf < 4) { ra = 0; rb = nhalf - 1; }
   else if (q == 0) { ra = 0; rb = 1; }
   else if (q == nhalf - 3) { ra = nhalf - 2; rb = nhalf - 1; }
   else { ra = rb = q + 1; }
 }
 
-// Adjoint of the u, v update with its filters, one workgroup per item: items
-// [0, nlat) are the u rows, then the groups of v rows (v_group).  With lam
-// u_f, lam v_f the adjoints of u_f = BF(u), v_f = BF(v) of the new state (BF
+// Adjoint of the u, v update with its filters, one workgroup per work item
+// (a u row, or a group of v rows, over a column chunk).  With lam u_f, lam
+// v_f the adjoints of u_f = BF(u), v_f = BF(v) of the new state (BF
 // symmetric, so its transpose is itself): lam u* = BF(lam u_f) + lam u, lam
 // v* likewise, then the transpose of the pole-row smoothing (v(0) = 0.4 v(0)
 // + 0.6 v(1), v(nhalf-1) likewise); lam_o u, v += lam u*, lam v* and W_DU,
 // W_DV = h lam u*, h lam v* (the adjoints of the tendencies, which enter the
-// update as h du, h dv).  The new state's u, v, u_f, v_f adjoints are
-// consumed: each is cleared once read.  LDS: a u row takes its weights and
-// staged row; a v group its rows' weights, staged rows and lam v* rows.
+// update as h du, h dv).  The new state's u, v adjoints are cleared once
+// read; its u_f, v_f adjoints, which neighbouring chunks read as halo, are
+// cleared later in the stage (a_cori).  LDS: a u item takes its weights and
+// window; a v group its rows' weights, windows and lam v* rows.
 struct UVRow {
-  double *n_u, *n_v, *n_uf, *n_vf, *o_u, *o_v, *du, *dv;
+  double *n_u, *n_v;
+  const double *n_uf, *n_vf;
+  double *o_u, *o_v, *du, *dv;
 };
-__global__ void __launch_bounds__(NR) a_uvrow(Grid g, FilterDev f, double h, UVRow A) {
+__global__ void __launch_bounds__(NR) a_uvrow(Grid g, FilterDev f, double h, const RowItem *__restrict__ items,
+                                              UVRow A) {
   extern __shared__ double xs[];
+  const RowItem it = items[blockIdx.x];
   const int64_t p = g.pitch;
-  const int span = g.nlon + f.max_taps;
-  if ((int)blockIdx.x < g.nlat) {
-    const int r = blockIdx.x;
+  const int c0 = it.c0, w = it.c1 - it.c0;
+  if (it.kind == RI_U) {
+    const int r = it.r0;
     double *__restrict__ u = A.n_u + r * p;
-    double *__restrict__ uf = A.n_uf + r * p;
     double *__restrict__ ou = A.o_u + r * p;
     double *__restrict__ du = A.du + r * p;
     if (r == 0 || r == g.nlat - 1) {   // u is not formed on the pole rows
-      for (int i = threadIdx.x; i < g.nlon; i += NR) u[i] = uf[i] = 0.0;
+      for (int j = threadIdx.x; j < w; j += NR) u[c0 + j] = 0.0;
       return;
     }
-    const int n = f.ngrid_lon[r];
-    double *w = xs, *x = xs + f.max_taps;
+    const int n = f.ngrid_lon[r], hn = n > 1 ? (n - 1) / 2 : 0;
+    double *wt = xs, *x = xs + f.max_taps;
     // the first output's addends are loaded ahead of the barrier
-    const int i0 = threadIdx.x;
-    const double un0 = i0 < g.nlon ? u[i0] : 0.0, uo0 = i0 < g.nlon ? ou[i0] : 0.0;
-    stage_weights(w, f.wgt_lon + (int64_t)r * f.max_taps, n);
-    stage_row(x, uf, g.nlon, n > 1 ? (n - 1) / 2 : 0);
+    const int j0 = threadIdx.x;
+    const double un0 = j0 < w ? u[c0 + j0] : 0.0, uo0 = j0 < w ? ou[c0 + j0] : 0.0;
+    stage_weights(wt, f.wgt_lon + (int64_t)r * f.max_taps, n);
+    stage_window(x, A.n_uf + r * p, g.nlon, c0, w, hn);
     __syncthreads();
-    for (int i = threadIdx.x; i < g.nlon; i += NR) {
-      const double un = i == i0 ? un0 : u[i], uo = i == i0 ? uo0 : ou[i];
-      const double lu = (n > 1 ? filter_point(w, n, x + i) : x[i]) + un;
-      ou[i] = uo + lu;
-      du[i] = h * lu;
-      u[i] = 0.0;
-      uf[i] = 0.0;
+    for (int j = threadIdx.x; j < w; j += NR) {
+      const int c = c0 + j;
+      const double un = j == j0 ? un0 : u[c], uo = j == j0 ? uo0 : ou[c];
+      const double lu = (n > 1 ? filter_point(wt, n, x + j) : x[j]) + un;
+      ou[c] = uo + lu;
+      du[c] = h * lu;
+      u[c] = 0.0;
     }
     return;
   }
-  int ra, rb;
-  v_group(blockIdx.x - g.nlat, g.nhalf, ra, rb);
-  const int G = rb - ra + 1, nl = g.nlon;
+  const int ra = it.r0, rb = it.r1, G = rb - ra + 1, span = w + f.max_taps;
   double *wv = xs, *xv = wv + G * f.max_taps, *lvs = xv + G * span;
   const int q0 = threadIdx.x;
-  const int64_t k0 = q0 < G * nl ? (ra + q0 / nl) * p + q0 % nl : 0;
-  const double vn0 = q0 < G * nl ? A.n_v[k0] : 0.0, vo0 = q0 < G * nl ? A.o_v[k0] : 0.0;
+  const int64_t k0 = q0 < G * w ? (ra + q0 / w) * p + c0 + q0 % w : 0;
+  const double vn0 = q0 < G * w ? A.n_v[k0] : 0.0, vo0 = q0 < G * w ? A.o_v[k0] : 0.0;
   for (int j = 0; j < G; j++) {
     const int n = f.ngrid_lat[ra + j];
     stage_weights(wv + j * f.max_taps, f.wgt_lat + (int64_t)(ra + j) * f.max_taps, n);
-    stage_row(xv + j * span, A.n_vf + (ra + j) * p, nl, n > 1 ? (n - 1) / 2 : 0);
+    stage_window(xv + j * span, A.n_vf + (ra + j) * p, g.nlon, c0, w, n > 1 ? (n - 1) / 2 : 0);
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < G * nl; q += NR) {
-    const int j = q / nl, i = q - j * nl, r = ra + j, n = f.ngrid_lat[r];
-    const double vn = q == q0 ? vn0 : A.n_v[r * p + i];
-    const double *x = xv + j * span + i;
+  for (int q = threadIdx.x; q < G * w; q += NR) {
+    const int j = q / w, jc = q - j * w, r = ra + j, n = f.ngrid_lat[r];
+    const double vn = q == q0 ? vn0 : A.n_v[r * p + c0 + jc];
+    const double *x = xv + j * span + jc;
     lvs[q] = (n > 1 ? filter_point(wv + j * f.max_taps, n, x) : x[0]) + vn;
   }
   __syncthreads();
   const int last = g.nhalf - 1;
   const double wgt = 0.6;
-  for (int q = threadIdx.x; q < G * nl; q += NR) {
-    const int j = q / nl, i = q - j * nl, r = ra + j;
-    const int64_t k = r * p + i;
+  for (int q = threadIdx.x; q < G * w; q += NR) {
+    const int j = q / w, r = ra + j;
+    const int64_t k = r * p + c0 + (q - j * w);
     double ls = (r == 0 || r == last) ? (1 - wgt) * lvs[q] : lvs[q];
-    if (r == 1) ls += wgt * lvs[q - nl];          // row 0 is in the group
-    if (r == last - 1) ls += wgt * lvs[q + nl];   // and row nhalf-1
+    if (r == 1) ls += wgt * lvs[q - w];          // row 0 is in the group
+    if (r == last - 1) ls += wgt * lvs[q + w];   // and row nhalf-1
     A.o_v[k] = (q == q0 ? vo0 : A.o_v[k]) + ls;
     A.dv[k] = h * ls;
     A.n_v[k] = 0.0;
-    A.n_vf[k] = 0.0;
   }
 }
 
 // Adjoint of gz_f = BF(gz_new), gz = SF(gz_new) of the new state, of the pgf
-// and of grad_ke, one workgroup per row r.  lam gz_f comes from the pgf
-// terms (pgf_lon = (gz_f(i+1) - gz_f(i)) / de_lon with lam pgf_lon = -lam du,
-// pgf_lat likewise); lam ke has the same form (lam dkedlon = -lam du) and is
-// kept in W_KE.  lam gz_new = BF(lam gz_f) + SF(lam gz); lam_o gz += lam
-// gz_new, W_DGZ = -(h lam gz_new) g (= lam dmfdlon = lam dmfdlat).  The pole
-// rows leave the sums of W_KE and W_DGZ over the row in pole[0..3].  The new
-// state's gz adjoint is consumed and cleared.
+// and of grad_ke, one workgroup per work item (a row over a column chunk;
+// the pole rows whole).  lam gz_f comes from the pgf terms (pgf_lon =
+// (gz_f(i+1) - gz_f(i)) / de_lon with lam pgf_lon = -lam du, pgf_lat
+// likewise); lam ke has the same form (lam dkedlon = -lam du) and is kept in
+// W_KE.  lam gz_new = BF(lam gz_f) + SF(lam gz); lam_o gz += lam gz_new,
+// W_DGZ = -(h lam gz_new) g (= lam dmfdlon = lam dmfdlat).  The pole rows
+// leave the sums of W_KE and W_DGZ over the row in pole[0..3].  The new
+// state's gz adjoint is cleared later in the stage (a_cori).
 struct GZRow {
-  const double *du, *dv;
-  double *n_gz, *o_gz, *ke, *dgz, *pole;
+  const double *du, *dv, *n_gz;
+  double *o_gz, *ke, *dgz, *pole;
 };
 __global__ void __launch_bounds__(NR) a_gzrow(Grid g, Mesh M, FilterDev fb, FilterDev fs, double h, double grav,
-                                              GZRow A) {
+                                              const RowItem *__restrict__ items, GZRow A) {
   extern __shared__ double xs[];
-  const int r = blockIdx.x;
+  const RowItem it = items[blockIdx.x];
+  const int r = it.r0, c0 = it.c0, w = it.c1 - it.c0;
   const int64_t p = g.pitch, o = r * p;
-  // LDS: big weights | small weights | lam gz_f (big halo) | lam gz (small halo) | W_DGZ of a pole row
-  double *wb = xs, *ws = wb + fb.max_taps, *xb = ws + fs.max_taps, *xg = xb + g.nlon + fb.max_taps;
-  double *xd = xg + g.nlon + fs.max_taps;
+  // LDS: big weights | small weights | lam gz_f window | lam gz window | W_DGZ of a pole row
+  double *wb = xs, *ws = wb + fb.max_taps, *xb = ws + fs.max_taps, *xg = xb + w + fb.max_taps;
+  double *xd = xg + w + fs.max_taps;
   const int nb = fb.ngrid_lon[r], hb = nb > 1 ? (nb - 1) / 2 : 0;
   const int ns = fs.ngrid_lon[r], hs = ns > 1 ? (ns - 1) / 2 : 0;
   const bool u_row = r >= 1 && r <= g.nlat - 2;
-  const int i0 = threadIdx.x;
-  const double og0 = i0 < g.nlon ? A.o_gz[o + i0] : 0.0;   // ahead of the barrier
+  const int j0 = threadIdx.x;
+  const double og0 = j0 < w ? A.o_gz[o + c0 + j0] : 0.0;   // ahead of the barrier
   stage_weights(wb, fb.wgt_lon + (int64_t)r * fb.max_taps, nb);
   stage_weights(ws, fs.wgt_lon + (int64_t)r * fs.max_taps, ns);
-  for (int t = threadIdx.x; t < g.nlon + 2 * hb; t += NR) {
-    int c = (t - hb) % g.nlon;
+  for (int t = threadIdx.x; t < w + 2 * hb; t += NR) {
+    int c = (c0 - hb + t) % g.nlon;
     if (c < 0) c += g.nlon;
     const int cm = c == 0 ? g.nlon - 1 : c - 1;
     double s = 0.0;
@@ -415,20 +426,20 @@ __global__ void __launch_bounds__(NR) a_gzrow(Grid g, Mesh M, FilterDev fb, Filt
     if (r >= 1) s += -A.dv[o - p + c] / M(GMC_DE_LAT, r - 1);
     if (r <= g.nhalf - 1) s += A.dv[o + c] / M(GMC_DE_LAT, r);
     xb[t] = s;
-    if (t >= hb && t < hb + g.nlon) A.ke[o + c] = s;
+    if (t >= hb && t < hb + w) A.ke[o + c] = s;
   }
-  stage_row(xg, A.n_gz + o, g.nlon, hs);
+  stage_window(xg, A.n_gz + o, g.nlon, c0, w, hs);
   __syncthreads();
-  const bool pole = r == 0 || r == g.nlat - 1;
-  for (int i = threadIdx.x; i < g.nlon; i += NR) {
-    const double og = i == i0 ? og0 : A.o_gz[o + i];
+  const bool pole = r == 0 || r == g.nlat - 1;   // whole rows
+  for (int j = threadIdx.x; j < w; j += NR) {
+    const int c = c0 + j;
+    const double og = j == j0 ? og0 : A.o_gz[o + c];
     const double lg =
-        (nb > 1 ? filter_point(wb, nb, xb + i) : xb[i]) + (ns > 1 ? filter_point(ws, ns, xg + i) : xg[i]);
-    A.o_gz[o + i] = og + lg;
+        (nb > 1 ? filter_point(wb, nb, xb + j) : xb[j]) + (ns > 1 ? filter_point(ws, ns, xg + j) : xg[j]);
+    A.o_gz[o + c] = og + lg;
     const double d = -(h * lg) * grav;
-    A.dgz[o + i] = d;
-    A.n_gz[o + i] = 0.0;
-    if (pole) xd[i] = d;
+    A.dgz[o + c] = d;
+    if (pole) xd[j] = d;
   }
   if (!pole) return;   // the same for the whole workgroup
   __syncthreads();
@@ -451,6 +462,10 @@ __global__ void a_cori(Grid g, Mesh M, Opt o, Tape T, ADPtrs A) {
   const int i = blockIdx.x * BX + threadIdx.x, r = blockIdx.y * BY + threadIdx.y;
   if (i >= g.nlon || r >= g.nlat) return;
   const int64_t p = g.pitch, k = r * p + i;
+  // the new state's u_f, v_f, gz adjoints, read by the row kernels, are consumed
+  A.n_uf[k] = 0.0;
+  A.n_gz[k] = 0.0;
+  if (r < g.nhalf) A.n_vf[k] = 0.0;
   const int im = wrap(i - 1, g.nlon), ip = wrap(i + 1, g.nlon);
   const double *lqv = A.w[W_DU];   // lam qhv = lam du
   const double *ldv = A.w[W_DV];   // lam qhu = -lam dv
@@ -558,88 +573,94 @@ __global__ void a_pv(Grid g, Mesh M, Opt o, Tape T, ADPtrs A) {
 // mfy_lon (U rows), formed in LDS over the tile and a one-point ring.  a_mf
 // transposes the mass fluxes onto lam u_f, lam v_f of the star state (from
 // the mass fluxes, ke, vor and the upwind weights) and forms W_MLON, W_MLAT
-// = lam m_lon, lam m_lat.
+// = lam m_lon, lam m_lat; its terms that do not involve the ring are formed
+// ahead of the ring, so that their loads overlap it.
 constexpr int LX = BX + 2, LY = BY + 2;
 __global__ void __launch_bounds__(BX * BY) a_latmf(Grid g, Mesh M, Opt o, Tape T, ADPtrs A) {
   __shared__ double s_ul[LY][LX], s_mxl[LY][LX], s_vl[LY][LX], s_myl[LY][LX];
   __shared__ double s_vor[2];
   const int C0 = blockIdx.x * BX, R0 = blockIdx.y * BY;
   const int64_t p = g.pitch;
-  // lam vor summed over the vertex rows next to the poles (the Stokes pole
-  // vorticity), where the ring reaches them; one wavefront per row
-  if (o.stokes && threadIdx.y < 2) {
-    const int vr = threadIdx.y == 0 ? 0 : g.nhalf - 1;
-    if (R0 - 1 <= vr && vr <= R0 + BY) {
-      const double *vo = A.w[W_VOR] + vr * p;
-      const double s = wave_zonal_sum(g.nlon, threadIdx.x, [&](int i) { return vo[i]; });
-      if (threadIdx.x == 0) s_vor[threadIdx.y] = s;
-    }
-  }
-  __syncthreads();
-  for (int t = threadIdx.y * BX + threadIdx.x; t < LY * LX; t += BX * BY) {
-    const int ly = t / LX, lx = t - ly * LX;
-    const int r = R0 - 1 + ly;
-    if (r < 0 || r > g.nlat - 1) continue;
-    int i = (C0 - 1 + lx) % g.nlon;
-    if (i < 0) i += g.nlon;
-    const int64_t k = r * p + i;
-    if (r <= g.nhalf - 1) {
-      double lul = A.w[W_PVLAT][k] * T.dlat[0][k];
-      if (o.stokes && r == 0) lul += -s_vor[0] * M(GMC_LE_LAT, 0) / (double)g.nlon / M(GMC_AREA_CELL, 0);
-      if (o.stokes && r == g.nhalf - 1)
-        lul += s_vor[1] * M(GMC_LE_LAT, r) / (double)g.nlon / M(GMC_AREA_CELL, g.nlat - 1);
-      double lmf = lul / T.m_lat[k];
-      if (o.coriolis != 1) lmf += -A.w[W_DV][k] * T.pv_lat[k];
-      s_ul[ly][lx] = lul;
-      s_mxl[ly][lx] = lmf;
-    }
-    if (r >= 1 && r <= g.nlat - 2) {
-      const double lvl = A.w[W_PVLON][k] * T.dlon[0][k];
-      double lmf = lvl / T.m_lon[k];
-      if (o.coriolis != 1) lmf += A.w[W_DU][k] * T.pv_lon[k];
-      s_vl[ly][lx] = lvl;
-      s_myl[ly][lx] = lmf;
-    }
-  }
-  __syncthreads();
+  // the thread's a_mf point
   const int i = C0 + threadIdx.x, r = R0 + threadIdx.y;
-  if (i >= g.nlon || r >= g.nlat) return;
-  const int lx = threadIdx.x + 1, ly = threadIdx.y + 1;
+  const bool pt = i < g.nlon && r < g.nlat;
   const int64_t k = r * p + i;
   const int im = wrap(i - 1, g.nlon), ip = wrap(i + 1, g.nlon);
   const double *lke = A.w[W_KE], *lvo = A.w[W_VOR];
   auto stokes_row = [&](int rr) { return o.stokes && (rr == 0 || rr == g.nhalf - 1); };
-  if (r >= 1 && r <= g.nlat - 2) {
-    // lam mfx total: grad_mf + Coriolis (a_cori) + mfx_lat transposed
-    double lmfx = A.w[W_MFX][k];
-    if (r <= g.nhalf - 1) lmfx += M(GMC_HALF_TANGENT_WGT1, r) * (s_mxl[ly][lx + 1] + s_mxl[ly][lx]);
-    lmfx += M(GMC_HALF_TANGENT_WGT2, r - 1) * (s_mxl[ly - 1][lx + 1] + s_mxl[ly - 1][lx]);
-    const double u = T.u_f[k];
-    double luf = lmfx * T.m_lon[k];
+  const bool u_pt = pt && r >= 1 && r <= g.nlat - 2, v_pt = pt && r <= g.nhalf - 1;
+  double u = 0.0, luf = 0.0, v = 0.0, lvf = 0.0;
+  if (u_pt) {
+    u = T.u_f[k];
     // ke(r, i+1) reads u as west, ke(r, i) as east
-    luf += lke[r * p + ip] * M(GMC_AREA_LON_WEST, r) * 2 * u / M(GMC_AREA_CELL, r) +
-           lke[k] * M(GMC_AREA_LON_EAST, r) * 2 * u / M(GMC_AREA_CELL, r);
+    luf = lke[r * p + ip] * M(GMC_AREA_LON_WEST, r) * 2 * u / M(GMC_AREA_CELL, r) +
+          lke[k] * M(GMC_AREA_LON_EAST, r) * 2 * u / M(GMC_AREA_CELL, r);
     // vor(r, i) reads u * de_lon(r), vor(r-1, i) reads -u * de_lon(r)
     if (r <= g.nhalf - 1 && !stokes_row(r)) luf += lvo[k] * M(GMC_DE_LON, r) / M(GMC_AREA_VTX, r);
     if (!stokes_row(r - 1)) luf += -lvo[k - p] * M(GMC_DE_LON, r) / M(GMC_AREA_VTX, r - 1);
     // the upwind weight of pv_lon reads u_f as its normal component
     luf += A.w[W_PVLON][k] * T.dlon[1][k];
-    A.s_uf[k] += luf;
-    A.w[W_MLON][k] = lmfx * u - s_vl[ly][lx] * T.v_lon[k] / T.m_lon[k];
   }
-  if (r <= g.nhalf - 1) {
-    double lmfy = A.w[W_MFY][k];
-    if (r + 1 <= g.nlat - 2) lmfy += M(GMC_FULL_TANGENT_WGT1, r + 1) * (s_myl[ly + 1][lx] + s_myl[ly + 1][lx - 1]);
-    if (r >= 1) lmfy += M(GMC_FULL_TANGENT_WGT2, r) * (s_myl[ly][lx] + s_myl[ly][lx - 1]);
-    const double v = T.v_f[k];
-    double lvf = lmfy * T.m_lat[k];
+  if (v_pt) {
+    v = T.v_f[k];
     if (r + 1 <= g.nlat - 2) lvf += lke[k + p] * M(GMC_AREA_LAT_NORTH, r) * 2 * v / M(GMC_AREA_CELL, r + 1);
     if (r >= 1) lvf += lke[k] * M(GMC_AREA_LAT_SOUTH, r) * 2 * v / M(GMC_AREA_CELL, r);
     if (r == 0) lvf += A.pole[0] * 2 * v / (double)g.nlon;
     if (r == g.nhalf - 1) lvf += A.pole[1] * 2 * v / (double)g.nlon;
     if (!stokes_row(r)) lvf += (lvo[r * p + im] - lvo[k]) * M(GMC_DE_LAT, r) / M(GMC_AREA_VTX, r);
     lvf += A.w[W_PVLAT][k] * T.dlat[1][k];
-    A.s_vf[k] += lvf;
+  }
+  // lam vor summed over the vertex rows next to the poles (the Stokes pole
+  // vorticity), where the ring reaches them; one wavefront per row
+  if (o.stokes && threadIdx.y < 2) {
+    const int vr = threadIdx.y == 0 ? 0 : g.nhalf - 1;
+    if (R0 - 1 <= vr && vr <= R0 + BY) {
+      const double *vo = A.w[W_VOR] + vr * p;
+      const double s = wave_zonal_sum_batched(g.nlon, threadIdx.x, vo);
+      if (threadIdx.x == 0) s_vor[threadIdx.y] = s;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.y * BX + threadIdx.x; t < LY * LX; t += BX * BY) {
+    const int ly = t / LX, lx = t - ly * LX;
+    const int rr = R0 - 1 + ly;
+    if (rr < 0 || rr > g.nlat - 1) continue;
+    int ii = (C0 - 1 + lx) % g.nlon;
+    if (ii < 0) ii += g.nlon;
+    const int64_t kk = rr * p + ii;
+    if (rr <= g.nhalf - 1) {
+      double lul = A.w[W_PVLAT][kk] * T.dlat[0][kk];
+      if (o.stokes && rr == 0) lul += -s_vor[0] * M(GMC_LE_LAT, 0) / (double)g.nlon / M(GMC_AREA_CELL, 0);
+      if (o.stokes && rr == g.nhalf - 1)
+        lul += s_vor[1] * M(GMC_LE_LAT, rr) / (double)g.nlon / M(GMC_AREA_CELL, g.nlat - 1);
+      double lmf = lul / T.m_lat[kk];
+      if (o.coriolis != 1) lmf += -A.w[W_DV][kk] * T.pv_lat[kk];
+      s_ul[ly][lx] = lul;
+      s_mxl[ly][lx] = lmf;
+    }
+    if (rr >= 1 && rr <= g.nlat - 2) {
+      const double lvl = A.w[W_PVLON][kk] * T.dlon[0][kk];
+      double lmf = lvl / T.m_lon[kk];
+      if (o.coriolis != 1) lmf += A.w[W_DU][kk] * T.pv_lon[kk];
+      s_vl[ly][lx] = lvl;
+      s_myl[ly][lx] = lmf;
+    }
+  }
+  __syncthreads();
+  const int lx = threadIdx.x + 1, ly = threadIdx.y + 1;
+  if (u_pt) {
+    // lam mfx total: grad_mf + Coriolis (a_cori) + mfx_lat transposed
+    double lmfx = A.w[W_MFX][k];
+    if (r <= g.nhalf - 1) lmfx += M(GMC_HALF_TANGENT_WGT1, r) * (s_mxl[ly][lx + 1] + s_mxl[ly][lx]);
+    lmfx += M(GMC_HALF_TANGENT_WGT2, r - 1) * (s_mxl[ly - 1][lx + 1] + s_mxl[ly - 1][lx]);
+    A.s_uf[k] += lmfx * T.m_lon[k] + luf;
+    A.w[W_MLON][k] = lmfx * u - s_vl[ly][lx] * T.v_lon[k] / T.m_lon[k];
+  }
+  if (v_pt) {
+    double lmfy = A.w[W_MFY][k];
+    if (r + 1 <= g.nlat - 2) lmfy += M(GMC_FULL_TANGENT_WGT1, r + 1) * (s_myl[ly + 1][lx] + s_myl[ly + 1][lx - 1]);
+    if (r >= 1) lmfy += M(GMC_FULL_TANGENT_WGT2, r) * (s_myl[ly][lx] + s_myl[ly][lx - 1]);
+    A.s_vf[k] += lmfy * T.m_lat[k] + lvf;
     A.w[W_MLAT][k] = lmfy * v - s_ul[ly][lx] * T.u_lat[k] / T.m_lat[k];
   }
 }
@@ -713,6 +734,8 @@ struct gmc_tlad {
   double *W[W_N] = {};
   double *pole = nullptr;
   size_t lds_uv = 0, lds_gz = 0;   // dynamic LDS of a_uvrow, a_gzrow
+  RowItem *d_items = nullptr;      // work items of a_uvrow, then of a_gzrow
+  int n_uv_items = 0, n_gz_items = 0;
   // tapes of two checkpoint segments (one being written while the other is
   // read): tapes[(b tape_steps + s) 3 + st] of stage st of step s in buffer b
   double *tape_slab = nullptr;
@@ -805,9 +828,10 @@ void ad_stage(gmc_tlad *a, int o, int s, int n, double h, const Tape &T) {
            {}, a->pole};
   for (int k = 0; k < W_N; k++) A.w[k] = W[k];
   UVRow U{Xn[X_U], Xn[X_V], Xn[X_UF], Xn[X_VF], Xo[X_U], Xo[X_V], W[W_DU], W[W_DV]};
-  hipLaunchKernelGGL(a_uvrow, dim3(g.nlat + v_groups(g.nhalf)), dim3(NR), a->lds_uv, c->stream, g, c->big, h, U);
+  hipLaunchKernelGGL(a_uvrow, dim3(a->n_uv_items), dim3(NR), a->lds_uv, c->stream, g, c->big, h, a->d_items, U);
   GZRow Z{W[W_DU], W[W_DV], Xn[X_GZ], Xo[X_GZ], W[W_KE], W[W_DGZ], a->pole};
-  hipLaunchKernelGGL(a_gzrow, dim3(g.nlat), dim3(NR), a->lds_gz, c->stream, g, M, c->big, c->small1, h, op.grav, Z);
+  hipLaunchKernelGGL(a_gzrow, dim3(a->n_gz_items), dim3(NR), a->lds_gz, c->stream, g, M, c->big, c->small1, h,
+                     op.grav, a->d_items + a->n_uv_items, Z);
   hipLaunchKernelGGL(a_cori, grid2d(g, g.nlat), b, 0, c->stream, g, M, op, T, A);
   hipLaunchKernelGGL(a_pv, grid2d(g, g.nhalf), b, 0, c->stream, g, M, op, T, A);
   hipLaunchKernelGGL(a_latmf, grid2d(g, g.nlat), b, 0, c->stream, g, M, op, T, A);
@@ -865,6 +889,68 @@ int tl_step(gmc_tlad *a, int o, int n, double dt) {
   return thip(a, hipGetLastError(), "tl step");
 }
 
+// Work lists of the row kernels: each u row, v group and gz row is cut
+// into column chunks of about kAdChunkWork / taps columns (the LDS traffic of
+// a workgroup grows with taps x columns); the pole rows, whose gz items form
+// the pole sums, stay whole.  Heaviest items first.
+constexpr int kAdChunkWork = 16384;
+
+void add_chunks(std::vector<RowItem> &v, int kind, int r0, int r1, int taps, int nlon, bool whole) {
+  const int w = whole ? nlon : std::min(nlon, std::max(64, (kAdChunkWork / std::max(taps, 1) + 63) / 64 * 64));
+  const int parts = (nlon + w - 1) / w;
+  for (int k = 0; k < parts; k++)
+    v.push_back(RowItem{kind, r0, r1, (int)((int64_t)nlon * k / parts), (int)((int64_t)nlon * (k + 1) / parts), 0});
+}
+
+int build_row_items(gmc_tlad *a) {
+  gmc_ctx *c = a->c;
+  const Grid &g = c->grid;
+  std::vector<int32_t> bl(g.nlat), bt(g.nhalf), sl(g.nlat);
+  int rc = thip(a, hipMemcpy(bl.data(), c->big.ngrid_lon, sizeof(int32_t) * g.nlat, hipMemcpyDeviceToHost), "taps");
+  if (!rc) rc = thip(a, hipMemcpy(bt.data(), c->big.ngrid_lat, sizeof(int32_t) * g.nhalf, hipMemcpyDeviceToHost), "taps");
+  if (!rc) rc = thip(a, hipMemcpy(sl.data(), c->small1.ngrid_lon, sizeof(int32_t) * g.nlat, hipMemcpyDeviceToHost), "taps");
+  if (rc) return rc;
+  auto taps = [](int n) { return std::max(n, 1); };
+  std::vector<RowItem> uv, gz;
+  for (int r = 0; r < g.nlat; r++) add_chunks(uv, RI_U, r, r, taps(bl[r]), g.nlon, r == 0 || r == g.nlat - 1);
+  for (int q = 0; q < v_groups(g.nhalf); q++) {
+    int ra, rb, t = 0;
+    v_group(q, g.nhalf, ra, rb);
+    for (int r = ra; r <= rb; r++) t += taps(bt[r]);
+    add_chunks(uv, RI_V, ra, rb, t, g.nlon, false);
+  }
+  for (int r = 0; r < g.nlat; r++)
+    add_chunks(gz, RI_GZ, r, r, taps(bl[r]) + taps(sl[r]), g.nlon, r == 0 || r == g.nlat - 1);
+  auto work = [&](const RowItem &it) {
+    int t = 0;
+    if (it.kind == RI_U) t = taps(bl[it.r0]);
+    else if (it.kind == RI_GZ) t = taps(bl[it.r0]) + taps(sl[it.r0]);
+    else
+      for (int r = it.r0; r <= it.r1; r++) t += taps(bt[r]);
+    return (int64_t)t * (it.c1 - it.c0);
+  };
+  auto heavy_first = [&](const RowItem &x, const RowItem &y) { return work(x) > work(y); };
+  std::stable_sort(uv.begin(), uv.end(), heavy_first);
+  std::stable_sort(gz.begin(), gz.end(), heavy_first);
+  const size_t mb = c->big.max_taps, ms = c->small1.max_taps;
+  for (const RowItem &it : uv) {
+    const size_t w = it.c1 - it.c0, G = it.r1 - it.r0 + 1;
+    a->lds_uv = std::max(a->lds_uv, sizeof(double) * (it.kind == RI_U ? mb + w + mb : G * (mb + w + mb + w)));
+  }
+  for (const RowItem &it : gz) {
+    const size_t w = it.c1 - it.c0;
+    a->lds_gz = std::max(a->lds_gz, sizeof(double) * (mb + ms + (w + mb) + (w + ms) + w));
+  }
+  if (std::max(a->lds_uv, a->lds_gz) > 65536)
+    return terr(a, GMC_EINVAL, "rows too long for the adjoint's row kernels (64 KiB of LDS)");
+  a->n_uv_items = (int)uv.size();
+  a->n_gz_items = (int)gz.size();
+  uv.insert(uv.end(), gz.begin(), gz.end());
+  rc = thip(a, hipMalloc((void **)&a->d_items, sizeof(RowItem) * uv.size()), "items");
+  if (!rc) rc = thip(a, hipMemcpy(a->d_items, uv.data(), sizeof(RowItem) * uv.size(), hipMemcpyHostToDevice), "items");
+  return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -878,18 +964,9 @@ int gmc_tlad_create(gmc_ctx *c, gmc_tlad **out) {
   a->c = c;
   const Grid &g = c->grid;
   a->fld = (size_t)g.nlat * g.pitch;
-  // the row kernels stage whole rows with their filter halos in LDS
-  const size_t span = (size_t)g.nlon + c->big.max_taps;
-  a->lds_uv = sizeof(double) * std::max(c->big.max_taps + span,
-                                        v_group_max(g.nhalf) * (c->big.max_taps + span + g.nlon));
-  a->lds_gz = sizeof(double) * (3 * (size_t)g.nlon + 2 * ((size_t)c->big.max_taps + c->small1.max_taps));
-  if (std::max(a->lds_uv, a->lds_gz) > 65536) {
-    const int rc = terr(a, GMC_EINVAL, "rows too long for the adjoint's row kernels (64 KiB of LDS)");
-    gmc_tlad_destroy(a);
-    return rc;
-  }
   const size_t n = a->fld * (3 * X_N + W_N) + 8;
-  int rc = thip(a, hipMalloc((void **)&a->slab, n * sizeof(double)), "tlad alloc");
+  int rc = build_row_items(a);
+  if (!rc) rc = thip(a, hipMalloc((void **)&a->slab, n * sizeof(double)), "tlad alloc");
   if (!rc) rc = thip(a, hipMemset(a->slab, 0, n * sizeof(double)), "tlad memset");
   if (!rc) rc = thip(a, hipStreamCreate(&a->astream), "stream");
   for (hipEvent_t *e : {&a->ev_fwd, &a->ev_adj[0], &a->ev_adj[1], &a->ev_join})
@@ -916,6 +993,7 @@ int gmc_tlad_destroy(gmc_tlad *a) {
     if (e) (void)hipEventDestroy(e);
   if (a->astream) (void)hipStreamDestroy(a->astream);
   if (a->slab) (void)hipFree(a->slab);
+  if (a->d_items) (void)hipFree(a->d_items);
   if (a->tape_slab) (void)hipFree(a->tape_slab);
   for (double *p : a->ck) (void)hipFree(p);
   if (a->obs_buf) (void)hipFree(a->obs_buf);
