@@ -607,14 +607,16 @@ __global__ void __launch_bounds__(NF) k_uvf_ens(const EnsMember *__restrict__ me
 
 // ---------------------------------------------------------------------------
 // Work lists: a row whose filter has n taps is cut into chunks of about
-// kChunkWork / n columns so that each workgroup does similar work; rows of
-// short filters into chunks of at most kNarrowChunk columns, so that a
-// workgroup's window is one round of loads.
-static constexpr int kChunkWork = 16384;
+// kChunkWork / n columns (a multiple of 64, at least 64) so that each
+// workgroup does similar work: the taps of a chunk's outputs are LDS reads,
+// and a polar row of several hundred taps over 512 columns kept one CU busy
+// for the whole launch.  Rows of short filters go into chunks of at most
+// kNarrowChunk columns, so that a workgroup's window is one round of loads.
+static constexpr int kChunkWork = 32768;
 static constexpr int kNarrowChunk = 2 * NF;
 
 static void add_row(std::vector<Item> &v, int row, int kind, int n, int nlon, size_t &lds) {
-  int w = n <= 16 ? std::min(nlon, kNarrowChunk) : std::min(nlon, std::max(NF, (kChunkWork / n + NF - 1) / NF * NF));
+  int w = n <= 16 ? std::min(nlon, kNarrowChunk) : std::min(nlon, std::max(64, (kChunkWork / n + 63) / 64 * 64));
   const int parts = (nlon + w - 1) / w;
   const int h = n > 1 ? (n - 1) / 2 : 0;
   for (int k = 0; k < parts; k++) {
