@@ -281,6 +281,27 @@ struct RowItem {
   int32_t kind, r0, r1, c0, c1, pad;
 };
 
+// The transpose of m_lon, m_lat, m_vtx and m = (gz - gzs) / g at cell (R,
+// i): the gz adjoint that lam m_lon (ll), lam m_lat (lt), lam m_vtx (lv)
+// give the star state (a_m adds it; a_gzrow of the next stage adds it to
+// the gz adjoint it stages, when the stage before it left it pending).
+__device__ __forceinline__ double am_point(const Grid &g, const Mesh &M, double grav, const double *ll,
+                                           const double *lt, const double *lv, int R, int i) {
+  const int64_t p = g.pitch, k = R * p + i;
+  const int im = wrap(i - 1, g.nlon);
+  double s = 0.0;
+  if (R >= 1 && R <= g.nlat - 2) s += 0.5 * (ll[k] + ll[R * p + im]);
+  if (R <= g.nhalf - 1) {
+    s += 0.5 * lt[k];
+    s += (lv[k] + lv[R * p + im]) * M(GMC_AREA_SUBCELL2, R) / M(GMC_AREA_VTX, R);
+  }
+  if (R >= 1) {
+    s += 0.5 * lt[k - p];
+    s += (lv[k - p] + lv[(R - 1) * p + im]) * M(GMC_AREA_SUBCELL1, R) / M(GMC_AREA_VTX, R - 1);
+  }
+  return s / grav;
+}
+
 // Zonal-filter staging: slot[t] = x[(c0 - hn + t) mod nlon] for t in [0, w +
 // 2 hn), so the taps of column c0 + j start at slot + j; a row's n weights
 // are staged next to it.
@@ -396,10 +417,15 @@ __global__ void __launch_bounds__(NR) a_uvrow(Grid g, FilterDev f, double h, con
 // W_KE.  lam gz_new = BF(lam gz_f) + SF(lam gz); lam_o gz += lam gz_new,
 // W_DGZ = -(h lam gz_new) g (= lam dmfdlon = lam dmfdlat).  The pole rows
 // leave the sums of W_KE and W_DGZ over the row in pole[0..3].  The new
-// state's gz adjoint is cleared later in the stage (a_cori).
+// state's gz adjoint is cleared later in the stage (a_cori).  When the stage
+// before left its a_m pending, the gz adjoint staged here is that of the
+// new state plus am_point of the previous stage's lam m_lon, m_lat, m_vtx
+// (the same sum a_m would have stored).
 struct GZRow {
   const double *du, *dv, *n_gz;
   double *o_gz, *ke, *dgz, *pole;
+  const double *mlon, *mlat, *mvtx;   // with pending
+  int pending;
 };
 __global__ void __launch_bounds__(NR) a_gzrow(Grid g, Mesh M, FilterDev fb, FilterDev fs, double h, double grav,
                                               const RowItem *__restrict__ items, GZRow A) {
@@ -428,7 +454,15 @@ __global__ void __launch_bounds__(NR) a_gzrow(Grid g, Mesh M, FilterDev fb, Filt
     xb[t] = s;
     if (t >= hb && t < hb + w) A.ke[o + c] = s;
   }
-  stage_window(xg, A.n_gz + o, g.nlon, c0, w, hs);
+  if (A.pending) {
+    for (int t = threadIdx.x; t < w + 2 * hs; t += NR) {
+      int c = (c0 - hs + t) % g.nlon;
+      if (c < 0) c += g.nlon;
+      xg[t] = A.n_gz[o + c] + am_point(g, M, grav, A.mlon, A.mlat, A.mvtx, r, c);
+    }
+  } else {
+    stage_window(xg, A.n_gz + o, g.nlon, c0, w, hs);
+  }
   __syncthreads();
   const bool pole = r == 0 || r == g.nlat - 1;   // whole rows
   for (int j = threadIdx.x; j < w; j += NR) {
@@ -669,20 +703,7 @@ __global__ void __launch_bounds__(BX * BY) a_latmf(Grid g, Mesh M, Opt o, Tape T
 __global__ void a_m(Grid g, Mesh M, Opt o, ADPtrs A) {
   const int i = blockIdx.x * BX + threadIdx.x, R = blockIdx.y * BY + threadIdx.y;
   if (i >= g.nlon || R >= g.nlat) return;
-  const int64_t p = g.pitch, k = R * p + i;
-  const int im = wrap(i - 1, g.nlon);
-  const double *ll = A.w[W_MLON], *lt = A.w[W_MLAT], *lv = A.w[W_MVTX];
-  double s = 0.0;
-  if (R >= 1 && R <= g.nlat - 2) s += 0.5 * (ll[k] + ll[R * p + im]);
-  if (R <= g.nhalf - 1) {
-    s += 0.5 * lt[k];
-    s += (lv[k] + lv[R * p + im]) * M(GMC_AREA_SUBCELL2, R) / M(GMC_AREA_VTX, R);
-  }
-  if (R >= 1) {
-    s += 0.5 * lt[k - p];
-    s += (lv[k - p] + lv[(R - 1) * p + im]) * M(GMC_AREA_SUBCELL1, R) / M(GMC_AREA_VTX, R - 1);
-  }
-  A.s_gz[k] += s / o.grav;
+  A.s_gz[R * g.pitch + i] += am_point(g, M, o.grav, A.w[W_MLON], A.w[W_MLAT], A.w[W_MVTX], R, i);
 }
 
 __global__ void a_add(Grid g, int rows, const double *a, const double *b, double *out) {
@@ -815,9 +836,12 @@ void tl_stage(gmc_tlad *a, int o, int s, int n, double h) {
 
 // AD of one stage around its tape T (ops_stage_tape): consumes the adjoint
 // of the new state (level n, cleared as it is read) and adds into levels o
-// and s.  Six launches: the two row kernels of the update and its filters,
-// then the tendencies' transposes from the Coriolis term back to m.
-void ad_stage(gmc_tlad *a, int o, int s, int n, double h, const Tape &T) {
+// and s.  The two row kernels of the update and its filters, then the
+// tendencies' transposes from the Coriolis term back to m.  The last of
+// them, a_m, adds into the star gz adjoint, which the next stage reads as
+// its new state's: with emit_am false it is left pending for the next
+// stage's a_gzrow (pending true there).
+void ad_stage(gmc_tlad *a, int o, int s, int n, double h, const Tape &T, bool pending, bool emit_am) {
   gmc_ctx *c = a->c;
   const Grid &g = c->grid;
   const Mesh M{c->d_mesh, g.nlat};
@@ -829,13 +853,14 @@ void ad_stage(gmc_tlad *a, int o, int s, int n, double h, const Tape &T) {
   for (int k = 0; k < W_N; k++) A.w[k] = W[k];
   UVRow U{Xn[X_U], Xn[X_V], Xn[X_UF], Xn[X_VF], Xo[X_U], Xo[X_V], W[W_DU], W[W_DV]};
   hipLaunchKernelGGL(a_uvrow, dim3(a->n_uv_items), dim3(NR), a->lds_uv, c->stream, g, c->big, h, a->d_items, U);
-  GZRow Z{W[W_DU], W[W_DV], Xn[X_GZ], Xo[X_GZ], W[W_KE], W[W_DGZ], a->pole};
+  GZRow Z{W[W_DU], W[W_DV], Xn[X_GZ], Xo[X_GZ], W[W_KE], W[W_DGZ], a->pole, W[W_MLON], W[W_MLAT], W[W_MVTX],
+          pending ? 1 : 0};
   hipLaunchKernelGGL(a_gzrow, dim3(a->n_gz_items), dim3(NR), a->lds_gz, c->stream, g, M, c->big, c->small1, h,
                      op.grav, a->d_items + a->n_uv_items, Z);
   hipLaunchKernelGGL(a_cori, grid2d(g, g.nlat), b, 0, c->stream, g, M, op, T, A);
   hipLaunchKernelGGL(a_pv, grid2d(g, g.nhalf), b, 0, c->stream, g, M, op, T, A);
   hipLaunchKernelGGL(a_latmf, grid2d(g, g.nlat), b, 0, c->stream, g, M, op, T, A);
-  hipLaunchKernelGGL(a_m, grid2d(g, g.nlat), b, 0, c->stream, g, M, op, A);
+  if (emit_am) hipLaunchKernelGGL(a_m, grid2d(g, g.nlat), b, 0, c->stream, g, M, op, A);
 }
 
 dim3 fields_grid(gmc_tlad *a, int nf) {
@@ -1197,12 +1222,18 @@ int seg_adjoint_body(gmc_tlad *a, int o, int n, double dt, int xin, int steps, i
   gmc_ctx *c = a->c;
   const TapeOut *tp = tapes_of(a, buf);
   int x_in = xin, x_out = xin == o ? n : o;
+  // a stage's star level is the next stage's new level (in this reverse
+  // order, across steps too), so a_m is deferred to the next stage's a_gzrow
+  // except after the segment's last stage
+  bool pending = false;
   for (int s = steps - 1; s >= 0; s--) {
     for (int st = nstages(c) - 1; st >= 0; st--) {
       int so, ss, sn;
       double h;
       stage_of(c->cfg.time_scheme, x_out, x_in, dt, st, so, ss, sn, h);
-      ad_stage(a, so, ss, sn, h, tape_from(tp[(size_t)s * 3 + st]));
+      const bool last = s == 0 && st == 0;
+      ad_stage(a, so, ss, sn, h, tape_from(tp[(size_t)s * 3 + st]), pending, last);
+      pending = !last;
     }
     std::swap(x_in, x_out);
   }
