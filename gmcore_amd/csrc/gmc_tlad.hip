@@ -531,6 +531,12 @@ __global__ void a_cori(Grid g, Mesh M, Opt o, Tape T, ADPtrs A) {
     }
     A.w[W_MFX][k] = s;
     A.w[W_PVLON][k] = spl;
+    // v_lon = mfy_lon / m_lon transposed, with the upwind weight's dependence on v_lon
+    const double lvl = spl * T.dlon[0][k];
+    double lmf = lvl / T.m_lon[k];
+    if (o.coriolis != 1) lmf += lqv[k] * T.pv_lon[k];
+    A.w[W_VLON][k] = lvl;
+    A.w[W_MFYLON][k] = lmf;
   }
   if (r <= g.nhalf - 1) {
     // lam mfy: grad_mf (cell rows r and r + 1) and the pole sums
@@ -562,6 +568,13 @@ __global__ void a_cori(Grid g, Mesh M, Opt o, Tape T, ADPtrs A) {
     }
     A.w[W_MFY][k] = s;
     A.w[W_PVLAT][k] = spt;
+    // u_lat = mfx_lat / m_lat transposed likewise; on the vertex rows next to
+    // the poles with Stokes pole vorticity a_mf adds the pole term
+    const double lul = spt * T.dlat[0][k];
+    double lmf = lul / T.m_lat[k];
+    if (o.coriolis != 1) lmf += ldv[k] * -T.pv_lat[k];
+    A.w[W_ULAT][k] = lul;
+    A.w[W_MFXLAT][k] = lmf;
   }
 }
 
@@ -601,29 +614,26 @@ __global__ void a_pv(Grid g, Mesh M, Opt o, Tape T, ADPtrs A) {
   A.w[W_MVTX][k] = -s * T.pv[k] / T.m_vtx[k];
 }
 
-// a_lat and a_mf of one BX x BY tile.  a_lat transposes u_lat = mfx_lat /
-// m_lat and v_lon = mfy_lon / m_lon, with the upwind weights' dependence on
-// their tangential wind: lam u_lat, lam mfx_lat (V rows) and lam v_lon, lam
-// mfy_lon (U rows), formed in LDS over the tile and a one-point ring.  a_mf
-// transposes the mass fluxes onto lam u_f, lam v_f of the star state (from
-// the mass fluxes, ke, vor and the upwind weights) and forms W_MLON, W_MLAT
-// = lam m_lon, lam m_lat; its terms that do not involve the ring are formed
-// ahead of the ring, so that their loads overlap it.
-constexpr int LX = BX + 2, LY = BY + 2;
-__global__ void __launch_bounds__(BX * BY) a_latmf(Grid g, Mesh M, Opt o, Tape T, ADPtrs A) {
-  __shared__ double s_ul[LY][LX], s_mxl[LY][LX], s_vl[LY][LX], s_myl[LY][LX];
+// Mass fluxes transposed: lam u_f, lam v_f of the star state (from the
+// mass fluxes, ke, vor and the upwind weights) and W_MLON, W_MLAT = lam
+// m_lon, lam m_lat, from the adjoints a_cori leaves (W_MFX, W_MFY, W_MFXLAT,
+// W_MFYLON, W_ULAT, W_VLON, W_PVLON, W_PVLAT).  With Stokes pole vorticity
+// the vor of the vertex rows next to the poles is the pole circulation, so
+// lam u_lat there gains lam vor summed over that row (W_VOR of a_pv) times
+// le_lat / (nlon area_cell): added here, where those rows are read.  The
+// point's loads are issued ahead of the pole sums.
+__global__ void __launch_bounds__(BX * BY) a_mf(Grid g, Mesh M, Opt o, Tape T, ADPtrs A) {
   __shared__ double s_vor[2];
   const int C0 = blockIdx.x * BX, R0 = blockIdx.y * BY;
   const int64_t p = g.pitch;
-  // the thread's a_mf point
   const int i = C0 + threadIdx.x, r = R0 + threadIdx.y;
   const bool pt = i < g.nlon && r < g.nlat;
   const int64_t k = r * p + i;
   const int im = wrap(i - 1, g.nlon), ip = wrap(i + 1, g.nlon);
-  const double *lke = A.w[W_KE], *lvo = A.w[W_VOR];
+  const double *lke = A.w[W_KE], *lvo = A.w[W_VOR], *mxl = A.w[W_MFXLAT], *myl = A.w[W_MFYLON];
   auto stokes_row = [&](int rr) { return o.stokes && (rr == 0 || rr == g.nhalf - 1); };
   const bool u_pt = pt && r >= 1 && r <= g.nlat - 2, v_pt = pt && r <= g.nhalf - 1;
-  double u = 0.0, luf = 0.0, v = 0.0, lvf = 0.0;
+  double u = 0.0, luf = 0.0, v = 0.0, lvf = 0.0, lmfx = 0.0, lmfy = 0.0;
   if (u_pt) {
     u = T.u_f[k];
     // ke(r, i+1) reads u as west, ke(r, i) as east
@@ -634,6 +644,10 @@ __global__ void __launch_bounds__(BX * BY) a_latmf(Grid g, Mesh M, Opt o, Tape T
     if (!stokes_row(r - 1)) luf += -lvo[k - p] * M(GMC_DE_LON, r) / M(GMC_AREA_VTX, r - 1);
     // the upwind weight of pv_lon reads u_f as its normal component
     luf += A.w[W_PVLON][k] * T.dlon[1][k];
+    // lam mfx total: grad_mf + Coriolis (a_cori) + mfx_lat transposed
+    lmfx = A.w[W_MFX][k];
+    if (r <= g.nhalf - 1) lmfx += M(GMC_HALF_TANGENT_WGT1, r) * (mxl[r * p + ip] + mxl[k]);
+    lmfx += M(GMC_HALF_TANGENT_WGT2, r - 1) * (mxl[(r - 1) * p + ip] + mxl[(r - 1) * p + i]);
   }
   if (v_pt) {
     v = T.v_f[k];
@@ -643,59 +657,40 @@ __global__ void __launch_bounds__(BX * BY) a_latmf(Grid g, Mesh M, Opt o, Tape T
     if (r == g.nhalf - 1) lvf += A.pole[1] * 2 * v / (double)g.nlon;
     if (!stokes_row(r)) lvf += (lvo[r * p + im] - lvo[k]) * M(GMC_DE_LAT, r) / M(GMC_AREA_VTX, r);
     lvf += A.w[W_PVLAT][k] * T.dlat[1][k];
+    lmfy = A.w[W_MFY][k];
+    if (r + 1 <= g.nlat - 2) lmfy += M(GMC_FULL_TANGENT_WGT1, r + 1) * (myl[k + p] + myl[(r + 1) * p + im]);
+    if (r >= 1) lmfy += M(GMC_FULL_TANGENT_WGT2, r) * (myl[k] + myl[r * p + im]);
   }
-  // lam vor summed over the vertex rows next to the poles (the Stokes pole
-  // vorticity), where the ring reaches them; one wavefront per row
-  if (o.stokes && threadIdx.y < 2) {
-    const int vr = threadIdx.y == 0 ? 0 : g.nhalf - 1;
-    if (R0 - 1 <= vr && vr <= R0 + BY) {
-      const double *vo = A.w[W_VOR] + vr * p;
-      const double s = wave_zonal_sum_batched(g.nlon, threadIdx.x, vo);
-      if (threadIdx.x == 0) s_vor[threadIdx.y] = s;
+  // the pole terms of lam u_lat: V row 0 is read by rows 0 and 1, V row
+  // nhalf-1 by row nhalf-1 (one wavefront sums each row)
+  const bool north_rows = R0 <= g.nhalf - 1 && g.nhalf - 1 < R0 + BY;
+  if (o.stokes && (R0 == 0 || north_rows)) {
+    if (threadIdx.y == 0 && R0 == 0) {
+      const double sv = wave_zonal_sum_batched(g.nlon, threadIdx.x, lvo);
+      if (threadIdx.x == 0) s_vor[0] = -sv * M(GMC_LE_LAT, 0) / (double)g.nlon / M(GMC_AREA_CELL, 0);
     }
+    if (threadIdx.y == 1 && north_rows) {
+      const int vr = g.nhalf - 1;
+      const double sv = wave_zonal_sum_batched(g.nlon, threadIdx.x, lvo + vr * p);
+      if (threadIdx.x == 0) s_vor[1] = sv * M(GMC_LE_LAT, vr) / (double)g.nlon / M(GMC_AREA_CELL, g.nlat - 1);
+    }
+    __syncthreads();   // the same for the whole workgroup
+    // lam u_lat of V row vr gains d; lam mfx_lat gains d / m_lat
+    if (u_pt && r == 1)
+      lmfx += M(GMC_HALF_TANGENT_WGT2, 0) * (s_vor[0] / T.m_lat[ip] + s_vor[0] / T.m_lat[i]);
+    if (u_pt && r == g.nhalf - 1)
+      lmfx += M(GMC_HALF_TANGENT_WGT1, r) * (s_vor[1] / T.m_lat[r * p + ip] + s_vor[1] / T.m_lat[k]);
   }
-  __syncthreads();
-  for (int t = threadIdx.y * BX + threadIdx.x; t < LY * LX; t += BX * BY) {
-    const int ly = t / LX, lx = t - ly * LX;
-    const int rr = R0 - 1 + ly;
-    if (rr < 0 || rr > g.nlat - 1) continue;
-    int ii = (C0 - 1 + lx) % g.nlon;
-    if (ii < 0) ii += g.nlon;
-    const int64_t kk = rr * p + ii;
-    if (rr <= g.nhalf - 1) {
-      double lul = A.w[W_PVLAT][kk] * T.dlat[0][kk];
-      if (o.stokes && rr == 0) lul += -s_vor[0] * M(GMC_LE_LAT, 0) / (double)g.nlon / M(GMC_AREA_CELL, 0);
-      if (o.stokes && rr == g.nhalf - 1)
-        lul += s_vor[1] * M(GMC_LE_LAT, rr) / (double)g.nlon / M(GMC_AREA_CELL, g.nlat - 1);
-      double lmf = lul / T.m_lat[kk];
-      if (o.coriolis != 1) lmf += -A.w[W_DV][kk] * T.pv_lat[kk];
-      s_ul[ly][lx] = lul;
-      s_mxl[ly][lx] = lmf;
-    }
-    if (rr >= 1 && rr <= g.nlat - 2) {
-      const double lvl = A.w[W_PVLON][kk] * T.dlon[0][kk];
-      double lmf = lvl / T.m_lon[kk];
-      if (o.coriolis != 1) lmf += A.w[W_DU][kk] * T.pv_lon[kk];
-      s_vl[ly][lx] = lvl;
-      s_myl[ly][lx] = lmf;
-    }
-  }
-  __syncthreads();
-  const int lx = threadIdx.x + 1, ly = threadIdx.y + 1;
   if (u_pt) {
-    // lam mfx total: grad_mf + Coriolis (a_cori) + mfx_lat transposed
-    double lmfx = A.w[W_MFX][k];
-    if (r <= g.nhalf - 1) lmfx += M(GMC_HALF_TANGENT_WGT1, r) * (s_mxl[ly][lx + 1] + s_mxl[ly][lx]);
-    lmfx += M(GMC_HALF_TANGENT_WGT2, r - 1) * (s_mxl[ly - 1][lx + 1] + s_mxl[ly - 1][lx]);
     A.s_uf[k] += lmfx * T.m_lon[k] + luf;
-    A.w[W_MLON][k] = lmfx * u - s_vl[ly][lx] * T.v_lon[k] / T.m_lon[k];
+    A.w[W_MLON][k] = lmfx * u - A.w[W_VLON][k] * T.v_lon[k] / T.m_lon[k];
   }
   if (v_pt) {
-    double lmfy = A.w[W_MFY][k];
-    if (r + 1 <= g.nlat - 2) lmfy += M(GMC_FULL_TANGENT_WGT1, r + 1) * (s_myl[ly + 1][lx] + s_myl[ly + 1][lx - 1]);
-    if (r >= 1) lmfy += M(GMC_FULL_TANGENT_WGT2, r) * (s_myl[ly][lx] + s_myl[ly][lx - 1]);
+    double lul = A.w[W_ULAT][k];
+    if (o.stokes && r == 0) lul += s_vor[0];
+    if (o.stokes && r == g.nhalf - 1) lul += s_vor[1];
     A.s_vf[k] += lmfy * T.m_lat[k] + lvf;
-    A.w[W_MLAT][k] = lmfy * v - s_ul[ly][lx] * T.u_lat[k] / T.m_lat[k];
+    A.w[W_MLAT][k] = lmfy * v - lul * T.u_lat[k] / T.m_lat[k];
   }
 }
 
@@ -859,7 +854,7 @@ void ad_stage(gmc_tlad *a, int o, int s, int n, double h, const Tape &T, bool pe
                      op.grav, a->d_items + a->n_uv_items, Z);
   hipLaunchKernelGGL(a_cori, grid2d(g, g.nlat), b, 0, c->stream, g, M, op, T, A);
   hipLaunchKernelGGL(a_pv, grid2d(g, g.nhalf), b, 0, c->stream, g, M, op, T, A);
-  hipLaunchKernelGGL(a_latmf, grid2d(g, g.nlat), b, 0, c->stream, g, M, op, T, A);
+  hipLaunchKernelGGL(a_mf, grid2d(g, g.nlat), b, 0, c->stream, g, M, op, T, A);
   if (emit_am) hipLaunchKernelGGL(a_m, grid2d(g, g.nlat), b, 0, c->stream, g, M, op, A);
 }
 
