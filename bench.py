@@ -39,9 +39,9 @@ KERNEL_BYTES_PER_GP = {"k_tend": 8.0 * (7 + 8 + 8) / 3, "k_gzf": 24.0, "k_uvf": 
 # HBM bytes per grid point of one launch measured with rocprofv3 --pmc
 # FETCH_SIZE (x 2.00) and WRITE_SIZE (x 1.00), the factors a 1 GiB device copy
 # reads; 1440 x 721 workload, profiles/sw025_step_kernels.md
-KERNEL_TRAFFIC_PER_GP = {"k_tend": (2 * 19976.9 + 24315.8) * 1024 / (1440 * 721),
-                         "k_gzf": (2 * 4432.2 + 16222.5) * 1024 / (1440 * 721),
-                         "k_uvf": (2 * 29623.0 + 32400.0) * 1024 / (1440 * 721)}
+KERNEL_TRAFFIC_PER_GP = {"k_tend": (2 * 19976.8 + 24310.8) * 1024 / (1440 * 721),
+                         "k_gzf": (2 * 4395.4 + 16222.5) * 1024 / (1440 * 721),
+                         "k_uvf": (2 * 22127.6 + 32400.4) * 1024 / (1440 * 721)}
 
 # FFSL tracer advection, 0.5 degrees, 8 tracers, dt = 900 s, one day
 ADV_CONFIGS = {
