@@ -760,7 +760,11 @@ struct gmc_tlad {
   // the adjoint sweep of a segment runs on astream while the next segment's
   // taping forward runs on the context's stream
   hipStream_t astream = nullptr;
-  hipEvent_t ev_fwd = nullptr, ev_adj[2] = {}, ev_join = nullptr;
+  hipEvent_t ev_fwd = nullptr, ev_adj[2] = {}, ev_join = nullptr, ev_ck0 = nullptr;
+  // gmc_gradient's host side: pinned copies of three fields (x0, then the
+  // gradient) and the background term's gradient
+  HostStage hbuf;
+  std::vector<double> gb;
   std::vector<double *> ck;          // saved states (checkpoints), kNState fields each
   double *obs_buf = nullptr;         // observation cells, values and cost terms
   int obs_cap = 0;
@@ -989,7 +993,7 @@ int gmc_tlad_create(gmc_ctx *c, gmc_tlad **out) {
   if (!rc) rc = thip(a, hipMalloc((void **)&a->slab, n * sizeof(double)), "tlad alloc");
   if (!rc) rc = thip(a, hipMemset(a->slab, 0, n * sizeof(double)), "tlad memset");
   if (!rc) rc = thip(a, hipStreamCreate(&a->astream), "stream");
-  for (hipEvent_t *e : {&a->ev_fwd, &a->ev_adj[0], &a->ev_adj[1], &a->ev_join})
+  for (hipEvent_t *e : {&a->ev_fwd, &a->ev_adj[0], &a->ev_adj[1], &a->ev_join, &a->ev_ck0})
     if (!rc) rc = thip(a, hipEventCreateWithFlags(e, hipEventDisableTiming), "event");
   if (rc) {
     gmc_tlad_destroy(a);
@@ -1009,8 +1013,9 @@ int gmc_tlad_destroy(gmc_tlad *a) {
   if (a->c) (void)hipStreamSynchronize(a->c->stream);
   if (a->astream) (void)hipStreamSynchronize(a->astream);
   for (auto &sg : a->graphs) (void)hipGraphExecDestroy(sg.exec);
-  for (hipEvent_t e : {a->ev_fwd, a->ev_adj[0], a->ev_adj[1], a->ev_join})
+  for (hipEvent_t e : {a->ev_fwd, a->ev_adj[0], a->ev_adj[1], a->ev_join, a->ev_ck0})
     if (e) (void)hipEventDestroy(e);
+  stage_free(a->hbuf);
   if (a->astream) (void)hipStreamDestroy(a->astream);
   if (a->slab) (void)hipFree(a->slab);
   if (a->d_items) (void)hipFree(a->d_items);
@@ -1238,12 +1243,15 @@ int seg_adjoint_body(gmc_tlad *a, int o, int n, double dt, int xin, int steps, i
 // Forward over the window from level o, keeping the state every `every`
 // steps in ck; the last segment runs through the taping stages, its tapes
 // left in buffer (segments - 1) % 2.  cur is the level of the end state.
-int window_forward(gmc_tlad *a, int o, int n, double dt, int nsteps, int every, double *const *ck, int &cur) {
+// ck0_saved, when given, is recorded once the initial state is saved.
+int window_forward(gmc_tlad *a, int o, int n, double dt, int nsteps, int every, double *const *ck, int &cur,
+                   hipEvent_t ck0_saved = nullptr) {
   int rc = ensure_tapes(a, std::min(every, nsteps));
   const int nck = (nsteps + every - 1) / every, last = (nck - 1) * every;
   cur = o;
   for (int s = 0; s < nsteps && !rc; s++) {
     if (s % every == 0) rc = save_level(a, cur, &ck[(size_t)(s / every) * kNState]);
+    if (!rc && s == 0 && ck0_saved) rc = thip(a, hipEventRecord(ck0_saved, a->c->stream), "event");
     if (!rc && s < last) rc = fwd_step(a, o, n, dt, cur);
   }
   if (rc) return rc;
@@ -1358,7 +1366,38 @@ int gmc_gradient(gmc_tlad *a, double dt, int32_t nsteps, int32_t ckpt_every, int
   double *const *ck = a->ck.data();
   // forward with checkpoints every ckpt_every steps
   int cur = o;
-  if (!rc) rc = window_forward(a, o, n, dt, nsteps, ckpt_every, ck, cur);
+  if (!rc) rc = window_forward(a, o, n, dt, nsteps, ckpt_every, ck, cur, a->ev_ck0);
+  // background term J_b = 1/2 |x0 - xb|^2 / sigma_b^2 over u (rows between
+  // the poles), v and gz, and its gradient, on the host while the window
+  // runs: x0 comes from its checkpoint copy on the adjoint stream
+  const size_t fsz = (size_t)g.nlat * g.nlon;
+  const int rows[3] = {g.nlat, g.nhalf, g.nlat}, r0[3] = {1, 0, 0}, r1[3] = {g.nlat - 1, g.nhalf, g.nlat};
+  const double *xb[3] = {ub, vb, gzb};
+  double *gr[3] = {grad_u, grad_v, grad_gz};
+  const double ib = 1.0 / (sigma_b * sigma_b);
+  double jb = 0.0;
+  if (!rc && a->hbuf.cap < 3 * fsz) {
+    stage_free(a->hbuf);
+    rc = thip(a, hipHostMalloc((void **)&a->hbuf.p, 3 * fsz * sizeof(double), hipHostMallocDefault), "pinned alloc");
+    if (rc) a->hbuf.p = nullptr;
+    else a->hbuf.cap = 3 * fsz;
+  }
+  if (!rc) rc = thip(a, hipStreamWaitEvent(a->astream, a->ev_ck0, 0), "event wait");
+  for (int f = 0; f < 3 && !rc; f++)
+    rc = thip(a, hipMemcpy2DAsync(a->hbuf.p + f * fsz, sizeof(double) * g.nlon, ck[f], sizeof(double) * g.pitch,
+                                  sizeof(double) * g.nlon, rows[f], hipMemcpyDeviceToHost, a->astream), "x0 copy");
+  if (!rc) rc = thip(a, hipStreamSynchronize(a->astream), "sync");
+  if (!rc) {
+    a->gb.assign(3 * fsz, 0.0);
+    for (int f = 0; f < 3; f++)
+      for (int r = r0[f]; r < r1[f]; r++)
+        for (int i = 0; i < g.nlon; i++) {
+          const size_t k = (size_t)r * g.nlon + i;
+          const double d = a->hbuf.p[f * fsz + k] - xb[f][(size_t)r * ld + i];
+          jb += 0.5 * d * d * ib;
+          a->gb[f * fsz + k] = d * ib;
+        }
+  }
   // observation term at the final state and its adjoint
   int64_t *d_cell = (int64_t *)a->obs_buf;
   double *d_y = a->obs_buf + nobs, *d_jo = a->obs_buf + 2 * (size_t)nobs;
@@ -1381,31 +1420,20 @@ int gmc_gradient(gmc_tlad *a, double dt, int32_t nsteps, int32_t ckpt_every, int
   // reverse over the checkpoint segments, each recomputed forward once
   if (!rc) rc = window_adjoint(a, o, n, dt, nsteps, ckpt_every, ck);
   if (!rc) rc = fold_filters(a, o);
+  // the gradient: the adjoint of x0 plus the background term's
+  const int xv[3] = {X_U, X_V, X_GZ};
+  for (int f = 0; f < 3 && !rc; f++)
+    rc = thip(a, hipMemcpy2DAsync(a->hbuf.p + f * fsz, sizeof(double) * g.nlon, a->X[o][xv[f]],
+                                  sizeof(double) * g.pitch, sizeof(double) * g.nlon, rows[f], hipMemcpyDeviceToHost,
+                                  c->stream), "gradient copy");
+  if (!rc) rc = thip(a, hipStreamSynchronize(c->stream), "sync");
   if (rc) return rc;
-  // background term on the host: J_b = 1/2 |x0 - xb|^2 / sigma_b^2 over u
-  // (rows between the poles), v and gz; its gradient added to the adjoint
-  rc = gmc_tlad_download(a, o, GMC_TLAD_U, grad_u, ld);
-  if (!rc) rc = gmc_tlad_download(a, o, GMC_TLAD_V, grad_v, ld);
-  if (!rc) rc = gmc_tlad_download(a, o, GMC_TLAD_GZ, grad_gz, ld);
-  if (rc) return rc;
-  std::vector<double> x((size_t)g.nlat * ld);
-  const double ib = 1.0 / (sigma_b * sigma_b);
-  double jb = 0.0;
-  struct F { int field, rows, r0, r1; const double *xb; double *gr; };
-  const F fs[3] = {{GMC_U_LON, g.nlat, 1, g.nlat - 1, ub, grad_u}, {GMC_V_LAT, g.nhalf, 0, g.nhalf, vb, grad_v},
-                   {GMC_GZ, g.nlat, 0, g.nlat, gzb, grad_gz}};
-  for (const F &f : fs) {
-    rc = gmc_download(c, o, f.field, x.data(), ld);
-    if (rc) return terr(a, rc, "state download");
-    for (int r = 0; r < f.rows; r++)
+  for (int f = 0; f < 3; f++)
+    for (int r = 0; r < rows[f]; r++)
       for (int i = 0; i < g.nlon; i++) {
-        const size_t k = (size_t)r * ld + i;
-        if (r < f.r0 || r >= f.r1) { f.gr[k] = 0.0; continue; }
-        const double d = x[k] - f.xb[k];
-        jb += 0.5 * d * d * ib;
-        f.gr[k] += d * ib;
+        const size_t k = (size_t)r * g.nlon + i;
+        gr[f][(size_t)r * ld + i] = r < r0[f] || r >= r1[f] ? 0.0 : a->hbuf.p[f * fsz + k] + a->gb[f * fsz + k];
       }
-  }
   *cost = jo + jb;
   return zero_level(a, o);
 }
