@@ -281,11 +281,23 @@ struct RowItem {
   int32_t kind, r0, r1, c0, c1, pad;
 };
 
+// The metric table and its row records, which also hold the reciprocals of
+// the divisor fields (area_vtx, area_cell, de_lon, de_lat): the adjoint
+// kernels multiply by rcp(f, r) where a transpose divides by a metric term.
+struct MeshR {
+  const double *m, *rows;
+  int nlat;
+  __device__ __forceinline__ double operator()(int f, int r) const { return m[f * nlat + r]; }
+  __device__ __forceinline__ double rcp(int f, int r) const {
+    return rows[(int64_t)(r + 1) * kRowRec + GMC_MESH_NFIELDS + recip_slot(f)];
+  }
+};
+
 // The transpose of m_lon, m_lat, m_vtx and m = (gz - gzs) / g at cell (R,
 // i): the gz adjoint that lam m_lon (ll), lam m_lat (lt), lam m_vtx (lv)
 // give the star state (a_m adds it; a_gzrow of the next stage adds it to
 // the gz adjoint it stages, when the stage before it left it pending).
-__device__ __forceinline__ double am_point(const Grid &g, const Mesh &M, double grav, const double *ll,
+__device__ __forceinline__ double am_point(const Grid &g, const MeshR &M, double grav, const double *ll,
                                            const double *lt, const double *lv, int R, int i) {
   const int64_t p = g.pitch, k = R * p + i;
   const int im = wrap(i - 1, g.nlon);
@@ -293,11 +305,11 @@ __device__ __forceinline__ double am_point(const Grid &g, const Mesh &M, double 
   if (R >= 1 && R <= g.nlat - 2) s += 0.5 * (ll[k] + ll[R * p + im]);
   if (R <= g.nhalf - 1) {
     s += 0.5 * lt[k];
-    s += (lv[k] + lv[R * p + im]) * M(GMC_AREA_SUBCELL2, R) / M(GMC_AREA_VTX, R);
+    s += (lv[k] + lv[R * p + im]) * M(GMC_AREA_SUBCELL2, R) * M.rcp(GMC_AREA_VTX, R);
   }
   if (R >= 1) {
     s += 0.5 * lt[k - p];
-    s += (lv[k - p] + lv[(R - 1) * p + im]) * M(GMC_AREA_SUBCELL1, R) / M(GMC_AREA_VTX, R - 1);
+    s += (lv[k - p] + lv[(R - 1) * p + im]) * M(GMC_AREA_SUBCELL1, R) * M.rcp(GMC_AREA_VTX, R - 1);
   }
   return s / grav;
 }
@@ -427,7 +439,7 @@ struct GZRow {
   const double *mlon, *mlat, *mvtx;   // with pending
   int pending;
 };
-__global__ void __launch_bounds__(NR) a_gzrow(Grid g, Mesh M, FilterDev fb, FilterDev fs, double h, double grav,
+__global__ void __launch_bounds__(NR) a_gzrow(Grid g, MeshR M, FilterDev fb, FilterDev fs, double h, double grav,
                                               const RowItem *__restrict__ items, GZRow A) {
   extern __shared__ double xs[];
   const RowItem it = items[blockIdx.x];
@@ -448,9 +460,9 @@ __global__ void __launch_bounds__(NR) a_gzrow(Grid g, Mesh M, FilterDev fb, Filt
     if (c < 0) c += g.nlon;
     const int cm = c == 0 ? g.nlon - 1 : c - 1;
     double s = 0.0;
-    if (u_row) s += (A.du[o + c] - A.du[o + cm]) / M(GMC_DE_LON, r);
-    if (r >= 1) s += -A.dv[o - p + c] / M(GMC_DE_LAT, r - 1);
-    if (r <= g.nhalf - 1) s += A.dv[o + c] / M(GMC_DE_LAT, r);
+    if (u_row) s += (A.du[o + c] - A.du[o + cm]) * M.rcp(GMC_DE_LON, r);
+    if (r >= 1) s += -A.dv[o - p + c] * M.rcp(GMC_DE_LAT, r - 1);
+    if (r <= g.nhalf - 1) s += A.dv[o + c] * M.rcp(GMC_DE_LAT, r);
     xb[t] = s;
     if (t >= hb && t < hb + w) A.ke[o + c] = s;
   }
@@ -492,7 +504,7 @@ __global__ void __launch_bounds__(NR) a_gzrow(Grid g, Mesh M, FilterDev fb, Filt
 // lam pv_lat (V rows); lam qhv = W_DU / h ... carried as lam du (= h lam u*):
 // the forward tendencies enter the update as h du, so W_DU already holds
 // lam du.
-__global__ void a_cori(Grid g, Mesh M, Opt o, Tape T, ADPtrs A) {
+__global__ void a_cori(Grid g, MeshR M, Opt o, Tape T, ADPtrs A) {
   const int i = blockIdx.x * BX + threadIdx.x, r = blockIdx.y * BY + threadIdx.y;
   if (i >= g.nlon || r >= g.nlat) return;
   const int64_t p = g.pitch, k = r * p + i;
@@ -507,7 +519,7 @@ __global__ void a_cori(Grid g, Mesh M, Opt o, Tape T, ADPtrs A) {
   auto lqhu = [&](int rr, int ii) { return -ldv[rr * p + ii]; };
   if (r >= 1 && r <= g.nlat - 2) {
     // lam mfx: grad_mf
-    double s = (ldm[k] - ldm[r * p + ip]) * M(GMC_LE_LON, r) / M(GMC_AREA_CELL, r);
+    double s = (ldm[k] - ldm[r * p + ip]) * M(GMC_LE_LON, r) * M.rcp(GMC_AREA_CELL, r);
     double spl = 0.0;
     if (o.coriolis == 1) {
       const double pl = T.pv_lon[k], mf = T.mfx[k];
@@ -541,8 +553,8 @@ __global__ void a_cori(Grid g, Mesh M, Opt o, Tape T, ADPtrs A) {
   if (r <= g.nhalf - 1) {
     // lam mfy: grad_mf (cell rows r and r + 1) and the pole sums
     double s = 0.0;
-    if (r >= 1) s += ldm[k] * M(GMC_LE_LAT, r) / M(GMC_AREA_CELL, r);
-    if (r + 1 <= g.nlat - 2) s += -ldm[k + p] * M(GMC_LE_LAT, r) / M(GMC_AREA_CELL, r + 1);
+    if (r >= 1) s += ldm[k] * M(GMC_LE_LAT, r) * M.rcp(GMC_AREA_CELL, r);
+    if (r + 1 <= g.nlat - 2) s += -ldm[k + p] * M(GMC_LE_LAT, r) * M.rcp(GMC_AREA_CELL, r + 1);
     if (r == 0) s += A.pole[2] * M(GMC_LE_LAT, 0) / (double)g.nlon / M(GMC_AREA_CELL, 0);
     if (r == g.nhalf - 1) s += -A.pole[3] * M(GMC_LE_LAT, r) / (double)g.nlon / M(GMC_AREA_CELL, g.nlat - 1);
     double spt = 0.0;
@@ -581,7 +593,7 @@ __global__ void a_cori(Grid g, Mesh M, Opt o, Tape T, ADPtrs A) {
 // interp_pv transposed onto pv (vertex rows), then pv = (vor + f) / m_vtx
 // transposed: W_VOR = lam vor, W_MVTX = lam m_vtx.  The upwind weights df of
 // every edge come from the tape.
-__global__ void a_pv(Grid g, Mesh M, Opt o, Tape T, ADPtrs A) {
+__global__ void a_pv(Grid g, MeshR M, Opt o, Tape T, ADPtrs A) {
   const int i = blockIdx.x * BX + threadIdx.x, R = blockIdx.y * BY + threadIdx.y;
   if (i >= g.nlon || R >= g.nhalf) return;
   const int64_t p = g.pitch, k = R * p + i;
@@ -609,9 +621,10 @@ __global__ void a_pv(Grid g, Mesh M, Opt o, Tape T, ADPtrs A) {
     const int64_t q = R * p + wrap(i + 2 - kk, g.nlon);
     s += lpt[q] * T.dlat[2 + kk][q];
   }
+  const double rv = 1.0 / T.m_vtx[k];
   A.w[W_PV][k] = s;
-  A.w[W_VOR][k] = s / T.m_vtx[k];
-  A.w[W_MVTX][k] = -s * T.pv[k] / T.m_vtx[k];
+  A.w[W_VOR][k] = s * rv;
+  A.w[W_MVTX][k] = -s * T.pv[k] * rv;
 }
 
 // Mass fluxes transposed: lam u_f, lam v_f of the star state (from the
@@ -622,7 +635,7 @@ __global__ void a_pv(Grid g, Mesh M, Opt o, Tape T, ADPtrs A) {
 // lam u_lat there gains lam vor summed over that row (W_VOR of a_pv) times
 // le_lat / (nlon area_cell): added here, where those rows are read.  The
 // point's loads are issued ahead of the pole sums.
-__global__ void __launch_bounds__(BX * BY) a_mf(Grid g, Mesh M, Opt o, Tape T, ADPtrs A) {
+__global__ void __launch_bounds__(BX * BY) a_mf(Grid g, MeshR M, Opt o, Tape T, ADPtrs A) {
   __shared__ double s_vor[2];
   const int C0 = blockIdx.x * BX, R0 = blockIdx.y * BY;
   const int64_t p = g.pitch;
@@ -637,11 +650,11 @@ __global__ void __launch_bounds__(BX * BY) a_mf(Grid g, Mesh M, Opt o, Tape T, A
   if (u_pt) {
     u = T.u_f[k];
     // ke(r, i+1) reads u as west, ke(r, i) as east
-    luf = lke[r * p + ip] * M(GMC_AREA_LON_WEST, r) * 2 * u / M(GMC_AREA_CELL, r) +
-          lke[k] * M(GMC_AREA_LON_EAST, r) * 2 * u / M(GMC_AREA_CELL, r);
+    const double rac = M.rcp(GMC_AREA_CELL, r);
+    luf = lke[r * p + ip] * M(GMC_AREA_LON_WEST, r) * 2 * u * rac + lke[k] * M(GMC_AREA_LON_EAST, r) * 2 * u * rac;
     // vor(r, i) reads u * de_lon(r), vor(r-1, i) reads -u * de_lon(r)
-    if (r <= g.nhalf - 1 && !stokes_row(r)) luf += lvo[k] * M(GMC_DE_LON, r) / M(GMC_AREA_VTX, r);
-    if (!stokes_row(r - 1)) luf += -lvo[k - p] * M(GMC_DE_LON, r) / M(GMC_AREA_VTX, r - 1);
+    if (r <= g.nhalf - 1 && !stokes_row(r)) luf += lvo[k] * M(GMC_DE_LON, r) * M.rcp(GMC_AREA_VTX, r);
+    if (!stokes_row(r - 1)) luf += -lvo[k - p] * M(GMC_DE_LON, r) * M.rcp(GMC_AREA_VTX, r - 1);
     // the upwind weight of pv_lon reads u_f as its normal component
     luf += A.w[W_PVLON][k] * T.dlon[1][k];
     // lam mfx total: grad_mf + Coriolis (a_cori) + mfx_lat transposed
@@ -651,11 +664,11 @@ __global__ void __launch_bounds__(BX * BY) a_mf(Grid g, Mesh M, Opt o, Tape T, A
   }
   if (v_pt) {
     v = T.v_f[k];
-    if (r + 1 <= g.nlat - 2) lvf += lke[k + p] * M(GMC_AREA_LAT_NORTH, r) * 2 * v / M(GMC_AREA_CELL, r + 1);
-    if (r >= 1) lvf += lke[k] * M(GMC_AREA_LAT_SOUTH, r) * 2 * v / M(GMC_AREA_CELL, r);
+    if (r + 1 <= g.nlat - 2) lvf += lke[k + p] * M(GMC_AREA_LAT_NORTH, r) * 2 * v * M.rcp(GMC_AREA_CELL, r + 1);
+    if (r >= 1) lvf += lke[k] * M(GMC_AREA_LAT_SOUTH, r) * 2 * v * M.rcp(GMC_AREA_CELL, r);
     if (r == 0) lvf += A.pole[0] * 2 * v / (double)g.nlon;
     if (r == g.nhalf - 1) lvf += A.pole[1] * 2 * v / (double)g.nlon;
-    if (!stokes_row(r)) lvf += (lvo[r * p + im] - lvo[k]) * M(GMC_DE_LAT, r) / M(GMC_AREA_VTX, r);
+    if (!stokes_row(r)) lvf += (lvo[r * p + im] - lvo[k]) * M(GMC_DE_LAT, r) * M.rcp(GMC_AREA_VTX, r);
     lvf += A.w[W_PVLAT][k] * T.dlat[1][k];
     lmfy = A.w[W_MFY][k];
     if (r + 1 <= g.nlat - 2) lmfy += M(GMC_FULL_TANGENT_WGT1, r + 1) * (myl[k + p] + myl[(r + 1) * p + im]);
@@ -695,7 +708,7 @@ __global__ void __launch_bounds__(BX * BY) a_mf(Grid g, Mesh M, Opt o, Tape T, A
 }
 
 // m_lon, m_lat, m_vtx and m = (gz - gzs) / g transposed onto the star gz
-__global__ void a_m(Grid g, Mesh M, Opt o, ADPtrs A) {
+__global__ void a_m(Grid g, MeshR M, Opt o, ADPtrs A) {
   const int i = blockIdx.x * BX + threadIdx.x, R = blockIdx.y * BY + threadIdx.y;
   if (i >= g.nlon || R >= g.nlat) return;
   A.s_gz[R * g.pitch + i] += am_point(g, M, o.grav, A.w[W_MLON], A.w[W_MLAT], A.w[W_MVTX], R, i);
@@ -843,7 +856,7 @@ void tl_stage(gmc_tlad *a, int o, int s, int n, double h) {
 void ad_stage(gmc_tlad *a, int o, int s, int n, double h, const Tape &T, bool pending, bool emit_am) {
   gmc_ctx *c = a->c;
   const Grid &g = c->grid;
-  const Mesh M{c->d_mesh, g.nlat};
+  const MeshR M{c->d_mesh, c->d_mrows, g.nlat};
   const Opt op = opt_of(c);
   const dim3 b(BX, BY);
   double **Xo = a->X[o], **Xs = a->X[s], **Xn = a->X[n], **W = a->W;
