@@ -508,16 +508,20 @@ __global__ void a_cori(Grid g, MeshR M, Opt o, Tape T, ADPtrs A) {
   const int i = blockIdx.x * BX + threadIdx.x, r = blockIdx.y * BY + threadIdx.y;
   if (i >= g.nlon || r >= g.nlat) return;
   const int64_t p = g.pitch, k = r * p + i;
+  const bool u_part = blockIdx.z == 0;   // the U points' and the V points' outputs in workgroups of their own
   // the new state's u_f, v_f, gz adjoints, read by the row kernels, are consumed
-  A.n_uf[k] = 0.0;
-  A.n_gz[k] = 0.0;
-  if (r < g.nhalf) A.n_vf[k] = 0.0;
+  if (u_part) {
+    A.n_uf[k] = 0.0;
+    A.n_gz[k] = 0.0;
+  } else if (r < g.nhalf) {
+    A.n_vf[k] = 0.0;
+  }
   const int im = wrap(i - 1, g.nlon), ip = wrap(i + 1, g.nlon);
   const double *lqv = A.w[W_DU];   // lam qhv = lam du
   const double *ldv = A.w[W_DV];   // lam qhu = -lam dv
   const double *ldm = A.w[W_DGZ];  // lam dmfdlon = lam dmfdlat
   auto lqhu = [&](int rr, int ii) { return -ldv[rr * p + ii]; };
-  if (r >= 1 && r <= g.nlat - 2) {
+  if (u_part && r >= 1 && r <= g.nlat - 2) {
     // lam mfx: grad_mf
     double s = (ldm[k] - ldm[r * p + ip]) * M(GMC_LE_LON, r) * M.rcp(GMC_AREA_CELL, r);
     double spl = 0.0;
@@ -550,7 +554,7 @@ __global__ void a_cori(Grid g, MeshR M, Opt o, Tape T, ADPtrs A) {
     A.w[W_VLON][k] = lvl;
     A.w[W_MFYLON][k] = lmf;
   }
-  if (r <= g.nhalf - 1) {
+  if (!u_part && r <= g.nhalf - 1) {
     // lam mfy: grad_mf (cell rows r and r + 1) and the pole sums
     double s = 0.0;
     if (r >= 1) s += ldm[k] * M(GMC_LE_LAT, r) * M.rcp(GMC_AREA_CELL, r);
@@ -645,7 +649,9 @@ __global__ void __launch_bounds__(BX * BY) a_mf(Grid g, MeshR M, Opt o, Tape T, 
   const int im = wrap(i - 1, g.nlon), ip = wrap(i + 1, g.nlon);
   const double *lke = A.w[W_KE], *lvo = A.w[W_VOR], *mxl = A.w[W_MFXLAT], *myl = A.w[W_MFYLON];
   auto stokes_row = [&](int rr) { return o.stokes && (rr == 0 || rr == g.nhalf - 1); };
-  const bool u_pt = pt && r >= 1 && r <= g.nlat - 2, v_pt = pt && r <= g.nhalf - 1;
+  // the U points' and the V points' outputs in workgroups of their own
+  const bool u_pt = pt && blockIdx.z == 0 && r >= 1 && r <= g.nlat - 2;
+  const bool v_pt = pt && blockIdx.z == 1 && r <= g.nhalf - 1;
   double u = 0.0, luf = 0.0, v = 0.0, lvf = 0.0, lmfx = 0.0, lmfy = 0.0;
   if (u_pt) {
     u = T.u_f[k];
@@ -869,9 +875,10 @@ void ad_stage(gmc_tlad *a, int o, int s, int n, double h, const Tape &T, bool pe
           pending ? 1 : 0};
   hipLaunchKernelGGL(a_gzrow, dim3(a->n_gz_items), dim3(NR), a->lds_gz, c->stream, g, M, c->big, c->small1, h,
                      op.grav, a->d_items + a->n_uv_items, Z);
-  hipLaunchKernelGGL(a_cori, grid2d(g, g.nlat), b, 0, c->stream, g, M, op, T, A);
+  const dim3 g2 = grid2d(g, g.nlat);
+  hipLaunchKernelGGL(a_cori, dim3(g2.x, g2.y, 2), b, 0, c->stream, g, M, op, T, A);
   hipLaunchKernelGGL(a_pv, grid2d(g, g.nhalf), b, 0, c->stream, g, M, op, T, A);
-  hipLaunchKernelGGL(a_mf, grid2d(g, g.nlat), b, 0, c->stream, g, M, op, T, A);
+  hipLaunchKernelGGL(a_mf, dim3(g2.x, g2.y, 2), b, 0, c->stream, g, M, op, T, A);
   if (emit_am) hipLaunchKernelGGL(a_m, grid2d(g, g.nlat), b, 0, c->stream, g, M, op, A);
 }
 
