@@ -626,7 +626,6 @@ __global__ void a_pv(Grid g, MeshR M, Opt o, Tape T, ADPtrs A) {
     s += lpt[q] * T.dlat[2 + kk][q];
   }
   const double rv = 1.0 / T.m_vtx[k];
-  A.w[W_PV][k] = s;
   A.w[W_VOR][k] = s * rv;
   A.w[W_MVTX][k] = -s * T.pv[k] * rv;
 }
@@ -787,8 +786,9 @@ struct gmc_tlad {
   std::vector<double *> ck;          // saved states (checkpoints), kNState fields each
   double *obs_buf = nullptr;         // observation cells, values and cost terms
   int obs_cap = 0;
-  // graphs of a segment's taping forward (kind 0) and of its adjoint sweep
-  // (kind 1), keyed by (kind, levels, steps, tape buffer, dt)
+  // graphs of a segment's taping forward (kind 0), of its adjoint sweep
+  // (kind 1) and of a plain segment of the window forward (kind 2), keyed by
+  // (kind, levels, steps, tape buffer, dt)
   struct SegGraph { int kind, o, n, steps, buf; double dt; hipGraphExec_t exec; };
   std::vector<SegGraph> graphs;
   std::string err;
@@ -1134,15 +1134,6 @@ int alloc_obs(gmc_tlad *a, int nobs) {
 }
 
 
-// one plain forward step (the fused step graph) from level cur into the
-// other of levels o, n; cur becomes that level
-int fwd_step(gmc_tlad *a, int o, int n, double dt, int &cur) {
-  const int nxt = cur == o ? n : o;
-  const int rc = fused_step_graph(a->c, cur, nxt, dt);
-  cur = nxt;
-  return rc;
-}
-
 constexpr int kTapeFields = 26;   // the fields of a TapeOut
 constexpr int kAdSegment = 8;     // checkpoint interval of gmc_ad_step
 
@@ -1266,19 +1257,33 @@ int seg_adjoint_body(gmc_tlad *a, int o, int n, double dt, int xin, int steps, i
 // ck0_saved, when given, is recorded once the initial state is saved.
 int window_forward(gmc_tlad *a, int o, int n, double dt, int nsteps, int every, double *const *ck, int &cur,
                    hipEvent_t ck0_saved = nullptr) {
+  gmc_ctx *c = a->c;
   int rc = ensure_tapes(a, std::min(every, nsteps));
-  const int nck = (nsteps + every - 1) / every, last = (nck - 1) * every;
+  const int nck = (nsteps + every - 1) / every;
   cur = o;
-  for (int s = 0; s < nsteps && !rc; s++) {
-    if (s % every == 0) rc = save_level(a, cur, &ck[(size_t)(s / every) * kNState]);
-    if (!rc && s == 0 && ck0_saved) rc = thip(a, hipEventRecord(ck0_saved, a->c->stream), "event");
-    if (!rc && s < last) rc = fwd_step(a, o, n, dt, cur);
+  for (int q = 0; q < nck && !rc; q++) {
+    const int start = cur, other = cur == o ? n : o, steps = std::min(every, nsteps - q * every);
+    rc = save_level(a, cur, &ck[(size_t)q * kNState]);
+    if (!rc && q == 0 && ck0_saved) rc = thip(a, hipEventRecord(ck0_saved, c->stream), "event");
+    if (rc) break;
+    if (q < nck - 1) {
+      // a plain segment: the fused steps as one graph
+      rc = graph_run(a, 2, start, other, steps, -1, dt, c->stream, [&] {
+        int cc = start, r2 = GMC_OK;
+        for (int s = 0; s < steps && !r2; s++) {
+          const int nxt = cc == start ? other : start;
+          r2 = fused_integrate(c, cc, nxt, dt);
+          cc = nxt;
+        }
+        return r2;
+      });
+    } else {
+      const int buf = (nck - 1) % 2;
+      rc = graph_run(a, 0, start, other, steps, buf, dt, c->stream,
+                     [&] { return seg_forward_body(a, start, other, dt, steps, buf); });
+    }
+    if (steps % 2) cur = other;
   }
-  if (rc) return rc;
-  const int start = cur, other = cur == o ? n : o, steps = nsteps - last, buf = (nck - 1) % 2;
-  rc = graph_run(a, 0, start, other, steps, buf, dt, a->c->stream,
-                 [&] { return seg_forward_body(a, start, other, dt, steps, buf); });
-  if (steps % 2) cur = other;
   return rc;
 }
 
