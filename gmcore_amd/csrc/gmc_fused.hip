@@ -271,7 +271,7 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double gr
     if (TAPE && out_pt) {
       tape->m_lon[r * p + ic] = (sm[k][q] + sm[k][q + 1]) * 0.5;
       tape->mfx[r * p + ic] = smfx[k][q];
-      tape->mfy_lon[r * p + ic] = tt;
+      if (tape->mfy_lon) tape->mfy_lon[r * p + ic] = tt;
       tape->v_lon[r * p + ic] = vl;
       tape->pv_lon[r * p + ic] = pl;
       if (tape->dlon[0]) tape_upwind(tape->dlon, r * p + ic,
@@ -285,7 +285,7 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double gr
     const double a1 = spv[k][q - 2], a2 = spv[k][q - 1], a3 = spv[k][q], a4 = spv[k][q + 1];
     pt = pv_scheme == GMC_PV_MIDPOINT ? 0.5 * (a2 + a3) : pv_upwind_edge(order, wgt, ul, svf[k][q], 1, a1, a2, a3, a4);
     if (TAPE && out_pt) {
-      tape->mfx_lat[r * p + ic] = tt;
+      if (tape->mfx_lat) tape->mfx_lat[r * p + ic] = tt;
       tape->u_lat[r * p + ic] = ul;
       tape->pv_lat[r * p + ic] = pt;
       if (tape->dlat[0]) tape_upwind(tape->dlat, r * p + ic,

@@ -182,10 +182,11 @@ void fused_release(gmc_ctx *ctx);
 int fused_prepare(gmc_ctx *ctx);
 int fused_step_graph(gmc_ctx *ctx, int old_l, int new_l, double dt);
 // Destinations of the intermediates a taping stage writes for the adjoint:
-// the fields operators_prepare forms from the star state; when u_f is not
-// null the star state's u_f and v_f; when dlon[0] is not null the
-// derivatives of the upwind pv on the pv_lon and pv_lat edges (dt, dn,
-// df[0..3] of upwind_deriv, six fields each).
+// the fields operators_prepare forms from the star state (mfx_lat and
+// mfy_lon only where not null); when u_f is not null the star state's u_f
+// and v_f; when dlon[0] is not null the derivatives of the upwind pv on the
+// pv_lon and pv_lat edges (dt, dn, df[0..3] of upwind_deriv, six fields
+// each).
 struct TapeOut {
   double *m_lon, *m_lat, *m_vtx, *mfx, *mfy, *mfx_lat, *mfy_lon, *u_lat, *v_lon, *pv, *pv_lon, *pv_lat;
   double *u_f, *v_f;

@@ -1161,6 +1161,8 @@ int ensure_tapes(gmc_tlad *a, int steps) {
                                &t.dlon[0], &t.dlon[1], &t.dlon[2], &t.dlon[3], &t.dlon[4], &t.dlon[5], &t.dlat[0],
                                &t.dlat[1], &t.dlat[2], &t.dlat[3], &t.dlat[4], &t.dlat[5]};
     for (int k = 0; k < kTapeFields; k++) { *f[k] = p; p += a->fld; }
+    // the tangential mass fluxes are read by the adjoint of Coriolis scheme 2 only
+    if (a->c->cfg.coriolis_scheme == 1) t.mfx_lat = t.mfy_lon = nullptr;
   }
   return GMC_OK;
 }
@@ -1375,13 +1377,13 @@ int gmc_gradient(gmc_tlad *a, double dt, int32_t nsteps, int32_t ckpt_every, int
   if (nsteps < 1 || ckpt_every < 1 || nobs < 0 || (nobs && (!obs_cell || !obs_value)) || !(sigma_o > 0) ||
       !(sigma_b > 0) || !ub || !vb || !gzb || ld < g.nlon || !cost || !grad_u || !grad_v || !grad_gz || !(dt > 0))
     return terr(a, GMC_EINVAL, "bad gradient arguments");
-  for (int k = 0; k < nobs; k++)
-    if (obs_cell[k] < 0 || obs_cell[k] >= (int64_t)g.nlat * g.nlon) return terr(a, GMC_EINVAL, "observation cell out of range");
   {
-    std::vector<int64_t> sorted(obs_cell, obs_cell + nobs);
-    std::sort(sorted.begin(), sorted.end());
-    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
-      return terr(a, GMC_EINVAL, "observation cells must be distinct");
+    std::vector<uint8_t> seen((size_t)g.nlat * g.nlon, 0);
+    for (int k = 0; k < nobs; k++) {
+      if (obs_cell[k] < 0 || obs_cell[k] >= (int64_t)g.nlat * g.nlon)
+        return terr(a, GMC_EINVAL, "observation cell out of range");
+      if (seen[obs_cell[k]]++) return terr(a, GMC_EINVAL, "observation cells must be distinct");
+    }
   }
   const int o = c->old_level, n = c->new_level;
   const int nck = (nsteps + ckpt_every - 1) / ckpt_every;
