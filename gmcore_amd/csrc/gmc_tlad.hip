@@ -1151,6 +1151,10 @@ int ensure_tapes(gmc_tlad *a, int steps) {
   a->tape_steps = 0;
   const size_t n = 2 * (size_t)steps * 3 * kTapeFields * a->fld;
   rc = thip(a, hipMalloc((void **)&a->tape_slab, n * sizeof(double)), "tape alloc");
+  // the taping stages write each field on the rows where the forward forms
+  // it; the adjoint also reads mfx on the pole rows, where the forward's
+  // flux is zero, so the rows never written must hold zeros
+  if (!rc) rc = thip(a, hipMemset(a->tape_slab, 0, n * sizeof(double)), "tape clear");
   if (rc) return rc;
   a->tape_steps = steps;
   a->tapes.assign(2 * (size_t)steps * 3, TapeOut{});
