@@ -781,7 +781,7 @@ struct gmc_tlad {
   hipEvent_t ev_fwd = nullptr, ev_adj[2] = {}, ev_join = nullptr, ev_ck0 = nullptr;
   // gmc_gradient's host side: pinned copies of three fields (x0, then the
   // gradient) and the background term's gradient
-  HostStage hbuf;
+  HostStage hbuf, obuf;
   std::vector<double> gb;
   std::vector<double *> ck;          // saved states (checkpoints), kNState fields each
   double *obs_buf = nullptr;         // observation cells, values and cost terms
@@ -1036,6 +1036,7 @@ int gmc_tlad_destroy(gmc_tlad *a) {
   for (hipEvent_t e : {a->ev_fwd, a->ev_adj[0], a->ev_adj[1], a->ev_join, a->ev_ck0})
     if (e) (void)hipEventDestroy(e);
   stage_free(a->hbuf);
+  stage_free(a->obuf);
   if (a->astream) (void)hipStreamDestroy(a->astream);
   if (a->slab) (void)hipFree(a->slab);
   if (a->d_items) (void)hipFree(a->d_items);
@@ -1417,6 +1418,12 @@ int gmc_gradient(gmc_tlad *a, double dt, int32_t nsteps, int32_t ckpt_every, int
   for (int f = 0; f < 3 && !rc; f++)
     rc = thip(a, hipMemcpy2DAsync(a->hbuf.p + f * fsz, sizeof(double) * g.nlon, ck[f], sizeof(double) * g.pitch,
                                   sizeof(double) * g.nlon, rows[f], hipMemcpyDeviceToHost, a->astream), "x0 copy");
+  for (int lv = 1; lv <= 3 && !rc; lv++) {
+    FieldSet f{};
+    for (int k = 0; k < X_N; k++) f.p[k] = a->X[lv][k];
+    hipLaunchKernelGGL(k_fill_fields, fields_grid(a, X_N), dim3(256), 0, a->astream, f, (int64_t)a->fld);
+  }
+  if (!rc) rc = thip(a, hipEventRecord(a->ev_join, a->astream), "event");
   if (!rc) rc = thip(a, hipStreamSynchronize(a->astream), "sync");
   if (!rc) {
     a->gb.assign(3 * fsz, 0.0);
@@ -1433,19 +1440,26 @@ int gmc_gradient(gmc_tlad *a, double dt, int32_t nsteps, int32_t ckpt_every, int
   int64_t *d_cell = (int64_t *)a->obs_buf;
   double *d_y = a->obs_buf + nobs, *d_jo = a->obs_buf + 2 * (size_t)nobs;
   double jo = 0.0;
-  if (!rc) rc = zero_level(a, o);
-  if (!rc) rc = zero_level(a, n);
-  if (!rc) rc = zero_level(a, 3);
+  if (!rc) rc = thip(a, hipStreamWaitEvent(c->stream, a->ev_join, 0), "event wait");   // levels cleared
+  if (!rc && nobs && a->obuf.cap < 3 * (size_t)nobs) {
+    stage_free(a->obuf);
+    rc = thip(a, hipHostMalloc((void **)&a->obuf.p, 3 * (size_t)nobs * sizeof(double), hipHostMallocDefault),
+              "pinned alloc");
+    if (rc) a->obuf.p = nullptr;
+    else a->obuf.cap = 3 * (size_t)nobs;
+  }
   if (!rc && nobs) {
-    rc = thip(a, hipMemcpyAsync(d_cell, obs_cell, sizeof(int64_t) * nobs, hipMemcpyHostToDevice, c->stream), "obs copy");
-    if (!rc) rc = thip(a, hipMemcpyAsync(d_y, obs_value, sizeof(double) * nobs, hipMemcpyHostToDevice, c->stream), "obs copy");
+    std::memcpy(a->obuf.p, obs_cell, sizeof(int64_t) * nobs);
+    std::memcpy(a->obuf.p + nobs, obs_value, sizeof(double) * nobs);
+    rc = thip(a, hipMemcpyAsync(d_cell, a->obuf.p, sizeof(double) * 2 * nobs, hipMemcpyHostToDevice, c->stream),
+              "obs copy");
     if (!rc) {
       hipLaunchKernelGGL(k_obs, dim3((nobs + 255) / 256), dim3(256), 0, c->stream, g, nobs, d_cell, d_y,
                          1.0 / (sigma_o * sigma_o), c->state[cur][S_GZ], a->X[o][X_GZ], d_jo);
-      std::vector<double> h(nobs);
-      rc = thip(a, hipMemcpyAsync(h.data(), d_jo, sizeof(double) * nobs, hipMemcpyDeviceToHost, c->stream), "obs copy");
+      rc = thip(a, hipMemcpyAsync(a->obuf.p + 2 * (size_t)nobs, d_jo, sizeof(double) * nobs, hipMemcpyDeviceToHost,
+                                  c->stream), "obs copy");
       if (!rc) rc = thip(a, hipStreamSynchronize(c->stream), "sync");
-      for (int k = 0; k < nobs; k++) jo += h[k];
+      for (int k = 0; k < nobs && !rc; k++) jo += a->obuf.p[2 * (size_t)nobs + k];
     }
   }
   // reverse over the checkpoint segments, each recomputed forward once

@@ -303,7 +303,7 @@ class TLAD:
         cell = np.ascontiguousarray(obs_cell, dtype=np.int64)
         y = np.ascontiguousarray(obs_value, dtype=np.float64)
         ub, vb, gb = (np.ascontiguousarray(a, dtype=np.float64) for a in xb)
-        gu, gv, gg = np.zeros_like(ub), np.zeros_like(vb), np.zeros_like(gb)
+        gu, gv, gg = np.empty_like(ub), np.empty_like(vb), np.empty_like(gb)   # every element is written
         J = C.c_double()
         self._check(self.lib.gmc_gradient(self.h, m.setup.nl.dt_dyn if dt is None else dt, nsteps, ckpt_every,
                                           len(cell), cell.ctypes.data_as(C.POINTER(C.c_int64)), dptr(y), sigma_o,
