@@ -8,7 +8,9 @@
   term) against centred finite differences of that cost;
 - the trajectory: gmc_tl_step advances the state exactly as gmc_step, and
   gmc_ad_step / gmc_gradient leave it where it was;
-- checkpointing: every checkpoint interval gives the same bits.
+- checkpointing: every checkpoint interval gives the same bits;
+- the dot-product test at 360 x 181 (the adjoint's row kernels cut polar
+  rows into column chunks) and with Coriolis scheme 2.
 """
 import numpy as np
 import pytest
@@ -190,3 +192,34 @@ def test_gpu_adjoint_matches_cpu_adjoint(scheme, pv):
     for k, name in enumerate(("u", "v", "gz")):
         a, b = (g[k][1:-1], c[k][1:-1]) if k == 0 else (g[k], c[k])
         assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max(), name
+
+
+def _model_at(nlon, nlat, dt, coriolis):
+    st = Setup(Namelist(num_lon=nlon, num_lat=nlat, dt_dyn=dt, time_scheme="pc2", coriolis_scheme=coriolis))
+    ic = init_cond.seeded_state(nlon, nlat, st.mesh.metrics, 8, 11, omega=st.nl.omega, radius=st.nl.radius)
+    m = ShallowWater(st)
+    m.set_state(*ic)
+    m.prepare(1)
+    return m
+
+
+def _rand_at(nlon, nlat, seed, scale):
+    rng = np.random.default_rng(seed)
+    u = rng.standard_normal((nlat, nlon)) * scale[0]
+    u[[0, -1]] = 0.0
+    return u, rng.standard_normal((nlat - 1, nlon)) * scale[1], rng.standard_normal((nlat, nlon)) * scale[2]
+
+
+@pytest.mark.parametrize("nlon,nlat,dt,coriolis", [(360, 181, 300.0, 1), (72, 37, 1800.0, 2)])
+def test_dot_product_chunked_rows_and_coriolis2(nlon, nlat, dt, coriolis):
+    """At 360 x 181 the adjoint's row kernels cut the polar rows into
+    column chunks that read each other's columns as halo; with Coriolis
+    scheme 2 the taping stages also keep the tangential mass fluxes, which
+    the adjoint of that scheme reads."""
+    n = 2
+    dx = _rand_at(nlon, nlat, 1, (0.5, 0.5, 100.0))
+    lam = _rand_at(nlon, nlat, 2, (1.0, 1.0, 1e-2))
+    mdx = _tl(_model_at(nlon, nlat, dt, coriolis), dx, n)
+    mtl = _ad(_model_at(nlon, nlat, dt, coriolis), lam, n)
+    lhs, rhs = _dot(mdx, lam), _dot(dx, mtl)
+    assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), abs(rhs)), (lhs, rhs)
