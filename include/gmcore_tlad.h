@@ -61,9 +61,9 @@ int gmc_ad_step(gmc_tlad *t, double dt, int32_t nsteps);
  * (gz_N: gz after nsteps steps; cell_k = r * nlon + i, distinct; |.| over u on
  * the rows between the poles, v and gz).  Checkpoints of the trajectory every
  * ckpt_every steps; during the reverse sweep each segment runs forward once
- * from its checkpoint, keeping every stage's tape (26 fields of the grid per
- * stage and step: memory grows with ckpt_every), then its adjoint replays
- * from the tapes.  Writes J and dJ/du0, dJ/dv0, dJ/dgz0 (host, leading dimension ld);
+ * from its checkpoint, keeping every stage's tape (24 fields of the grid per
+ * stage and step, 26 with coriolis_scheme 2, in two buffers: memory grows
+ * with ckpt_every), then its adjoint replays from the tapes.  Writes J and dJ/du0, dJ/dv0, dJ/dgz0 (host, leading dimension ld);
  * the context's state is left at x0 and every adjoint variable at zero. */
 int gmc_gradient(gmc_tlad *t, double dt, int32_t nsteps, int32_t ckpt_every, int32_t nobs,
                  const int64_t *obs_cell, const double *obs_value, double sigma_o, const double *u_b,
