@@ -68,6 +68,10 @@ def lib() -> C.CDLL:
         L.gmo_adv_mass.argtypes = [vp, C.c_int, C.c_int]
         L.gmo_adv_get_internal.argtypes = [vp, C.c_int, P(C.c_double), C.c_int64]
         L.gmo_ppm.argtypes = [C.c_int, P(C.c_double), P(C.c_double)]
+        L.gmo_ppm_jac.argtypes = [C.c_int, P(C.c_double), P(C.c_double)]
+        L.gmo_adv_tlad_run.argtypes = [vp, C.c_int64, P(C.c_double), P(C.c_double), C.c_int64]
+        L.gmo_adv_clone.restype = vp
+        L.gmo_adv_clone.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -188,6 +192,14 @@ def ppm(f, limiter: int = 0):
     return tuple(out)
 
 
+def ppm_jac(limiter: int, f) -> np.ndarray:
+    """d (fl, df, f6) / d f[0..4] of ppm at the five values f (3 x 5)."""
+    x = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.zeros((3, 5))
+    lib().gmo_ppm_jac(limiter, dptr(x), dptr(out))
+    return out
+
+
 class AdvOracle:
     """One cell-centred FFSL advection batch on the CPU, driven like adv_run."""
 
@@ -238,6 +250,23 @@ class AdvOracle:
 
     def run(self, time_step: int) -> int:
         return lib().gmo_adv_run(self.h, time_step)
+
+    def tlad_run(self, time_step: int, dq=None, lam=None) -> int:
+        """One step that also maps the tangent dq (old -> new level) and the
+        adjoint lam (new -> old level) of every tracer, in place; arrays of
+        shape (ntracers, nlat, nlon)."""
+        shape = (self.ntracers, self.nlat, self.nlon)
+        for a in (dq, lam):
+            assert a is None or (a.shape == shape and a.dtype == np.float64 and a.flags.c_contiguous)
+        return lib().gmo_adv_tlad_run(self.h, time_step, None if dq is None else dptr(dq),
+                                      None if lam is None else dptr(lam), self.nlon)
+
+    def clone(self) -> "AdvOracle":
+        c = object.__new__(AdvOracle)
+        c._keep = self._keep
+        c.nlon, c.nlat, c.ntracers = self.nlon, self.nlat, self.ntracers
+        c.h = lib().gmo_adv_clone(self.h)
+        return c
 
     @property
     def old(self) -> int:

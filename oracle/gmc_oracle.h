@@ -97,6 +97,13 @@ double gmo_adv_mass(gmo_adv *s, int level, int tracer);
 /* interior of a batch array: 0 u, 1 v, 2 mfx, 3 mfy, 4 cflx, 5 cfly, 6 divx,
  * 7 divy, 8 qx, 9 qy, 10 qmf_lon, 11 qmf_lat, 12 old_m */
 int gmo_adv_get_internal(gmo_adv *s, int index, double *host, int64_t ld);
+/* tangent linear and adjoint of one adv_run step with respect to the
+ * tracers (winds held): dq [ntracers][nlat][ld] old -> new level, lam new ->
+ * old level; either may be NULL.  Advances the base like gmo_adv_run. */
+int gmo_adv_tlad_run(gmo_adv *s, int64_t time_step, double *dq, double *lam, int64_t ld);
+gmo_adv *gmo_adv_clone(const gmo_adv *s);
+/* d (fl, df, f6) / d f[0..4] of ppm at the five values f */
+void gmo_ppm_jac(int limiter, const double f[5], double J[3][5]);
 /* ppm (ffsl_mod.F90:1147-1173) of f[0..4]: out = fl, df, f6 */
 void gmo_ppm(int limiter, const double f[5], double out[3]);
 
