@@ -108,6 +108,30 @@ class Advection:
         s = np.ascontiguousarray(scales, dtype=np.float64)
         self._check(self.lib.gmc_adv_drive(self.h, time_step0, dptr(s), len(s)), "drive")
 
+    def _tlad_buf(self, a):
+        shape = (self.ntracers, self.nlat, self.nlon)
+        b = np.array(a, dtype=np.float64, order="C")
+        if b.shape != shape:
+            raise ValueError(f"expected shape {shape}, got {b.shape}")
+        return b
+
+    def tl_drive(self, time_step0: int, scales, dq) -> np.ndarray:
+        """drive() carrying the tangent dq (ntracers, nlat, nlon) of the old
+        level at entry; returns the tangent of the old level at exit."""
+        s = np.ascontiguousarray(scales, dtype=np.float64)
+        b = self._tlad_buf(dq)
+        self._check(self.lib.gmc_adv_tl_drive(self.h, time_step0, dptr(s), len(s), dptr(b), self.nlon), "tl_drive")
+        return b
+
+    def ad_drive(self, time_step0: int, scales, lam, ckpt_every: int = 8) -> np.ndarray:
+        """drive(), then the adjoint of the window: lam, the adjoint of the
+        old level at exit, mapped to the adjoint of the old level at entry."""
+        s = np.ascontiguousarray(scales, dtype=np.float64)
+        b = self._tlad_buf(lam)
+        self._check(self.lib.gmc_adv_ad_drive(self.h, time_step0, dptr(s), len(s), ckpt_every, dptr(b), self.nlon),
+                    "ad_drive")
+        return b
+
     @property
     def old(self) -> int:
         out = C.c_int32()

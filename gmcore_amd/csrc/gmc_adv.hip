@@ -584,6 +584,11 @@ struct gmc_adv {
   hipEvent_t *prof_ev = nullptr;
   int prof_n = 0, prof_cap = 0;
   std::vector<int> prof_kind;
+  // tangent-linear / adjoint storage: work fields (5 x ntracers fields and
+  // the pole sums), and the slab copies (checkpoints, then tape slots)
+  double *tl_f = nullptr;
+  char *tl_ws = nullptr;
+  size_t tl_ws_bytes = 0;
   std::string err;
 };
 
@@ -614,8 +619,13 @@ int copy_rows(gmc_adv *a, double *dst, const double *host, int rows, int64_t ld)
                             rows), "upload");
 }
 
-// adv_run: accumulate, maybe advance, copy_old_m (adv_mod.F90:152-312)
-int run_once(gmc_adv *a, int64_t time_step, bool basis, double scale) {
+// adv_run: accumulate, maybe advance, copy_old_m (adv_mod.F90:152-312).
+// With tape != null an advancing step copies the whole slab there after its
+// flux kernel (the base state its tangent and adjoint read) and *taped_old
+// is the old level; 0 when the step does not advance.
+int run_once(gmc_adv *a, int64_t time_step, bool basis, double scale, char *tape = nullptr,
+             int *taped_old = nullptr) {
+  if (taped_old) *taped_old = 0;
   const AdvGrid &g = a->g;
   const Mesh M{a->d_mesh, g.nlat};
   const int ns = a->cfg.nstep;
@@ -643,6 +653,11 @@ int run_once(gmc_adv *a, int64_t time_step, bool basis, double scale) {
     OuterPtrs op{a->qx, a->qy, tracer_ptr(a, a->old, 0), a->qp, a->mfx, a->mfy, a->cflx, a->cfly, a->old_m, a->sm, ts};
     hipLaunchKernelGGL(k_adv_flux, grid, dim3(NF), 0, a->stream, g, M, a->cfg.limiter, ntr, op);
     mark(a, KA_FLUX);
+    if (tape) {
+      const int rc = ahip(a, hipMemcpyAsync(tape, a->slab, a->slab_bytes, hipMemcpyDeviceToDevice, a->stream), "tape");
+      if (rc) return rc;
+      if (taped_old) *taped_old = a->old;
+    }
     FixPtrs fp{a->qp, tracer_ptr(a, a->new_, 0), a->sm, a->d_wgt, ts, a->d_flags, a->d_fixes, a->cfg.lon_damp_passes};
     hipLaunchKernelGGL(k_adv_fix, dim3(g.nlat, ntr), dim3(NA), a->lds_bytes, a->stream, g, fp);
     mark(a, KA_FIX);
@@ -743,6 +758,8 @@ int gmc_adv_destroy(gmc_adv *a) {
   if (!a) return GMC_EINVAL;
   if (a->stream) (void)hipStreamSynchronize(a->stream);
   if (a->slab) (void)hipFree(a->slab);
+  if (a->tl_f) (void)hipFree(a->tl_f);
+  if (a->tl_ws) (void)hipFree(a->tl_ws);
   if (a->h_flags) (void)hipHostFree(a->h_flags);
   if (a->stream) (void)hipStreamDestroy(a->stream);
   stage_free(a->stage);
@@ -902,5 +919,683 @@ int gmc_adv_kernel_times(gmc_adv *a, int64_t time_step0, const double *scale, in
 }
 
 const char *gmc_adv_kernel_name(int32_t k) { return k >= 0 && k < KA_NCLASS ? kAdvNames[k] : ""; }
+
+}  // extern "C"
+
+// ===========================================================================
+// Tangent linear and adjoint of the FFSL step with respect to the tracers,
+// the winds (which do not depend on q) held.  With the winds fixed the step is
+// piecewise linear in q: the slope limiters, the PPM edge clipping and the
+// positivity fixer choose among linear expressions by comparisons of q.  The
+// tangent is the linear map of the piece the base state selects (ties go to
+// the first candidate, as in slope_jac and ppm_jac), the adjoint its
+// transpose.  Each kernel reads the base state of its step from a tape slot,
+// a copy of the object's slab taken after the step's flux kernel: the winds,
+// CFL numbers and split divergences, q of both levels (the new level still
+// holding the seam values the fixer reads), qx, qy and the pre-fixer qp.
+//
+// Tangent step: t_adv_psum (pole sums of the inner y-flux tangents),
+// t_adv_inner (tangents of qx, qy), t_adv_psum (outer), t_adv_outer (tangent
+// of qp), t_adv_fix (the fixer on the base's branches, then the damping).
+// Adjoint step: a_adv_fix (damping, the fixer transposed, / m), k_adv_psum2
+// (pole sums), a_adv_outer (outer fluxes transposed, added into the adjoints
+// of qy and qx with atomics), k_adv_psum2, a_adv_inner (inner fluxes
+// transposed, added into the adjoint of q).  The adjoint's sums therefore run
+// in no fixed order: it is reproducible to rounding, not bit for bit.
+namespace gmc {
+namespace {
+
+// d slope / d (fm1, f, fp1) of the branch the base values select (slope_*,
+// ffsl_mod.F90:1175-1205); mono: the smallest of |df|, df_min, df_max, the
+// first on a tie, with the minimum and maximum of (fm1, f, fp1) taken at
+// their first occurrence.
+__device__ __forceinline__ void slope_jac(int lim, double fm1, double f, double fp1, double &g0, double &g1,
+                                          double &g2) {
+  const double df = (fp1 - fm1) * 0.5;
+  g0 = -0.5; g1 = 0.0; g2 = 0.5;
+  if (lim == GMC_LIMITER_NONE) return;
+  const double a = fabs(df), s = copysign(1.0, df);
+  if (lim == GMC_LIMITER_PD) {
+    if (2 * f < a) { g0 = 0.0; g1 = 2 * s * (2 * f >= 0 ? 1.0 : -1.0); g2 = 0.0; }
+    return;
+  }
+  int lo = 0, hi = 0;
+  double vlo = fm1, vhi = fm1;
+  if (f < vlo) { lo = 1; vlo = f; }
+  if (f > vhi) { hi = 1; vhi = f; }
+  if (fp1 < vlo) { lo = 2; vlo = fp1; }
+  if (fp1 > vhi) { hi = 2; vhi = fp1; }
+  const double dmn = 2 * (f - vlo), dmx = 2 * (vhi - f);
+  int pick = 0;
+  double cur = a;
+  if (dmn < cur) { pick = 1; cur = dmn; }
+  if (dmx < cur) pick = 2;
+  if (pick == 0) return;  // fsign(|df|, df) = df
+  const double t = 2 * s;
+  if (pick == 1) {        // s * 2 (f - min)
+    g0 = lo == 0 ? -t : 0.0; g1 = lo == 1 ? 0.0 : t; g2 = lo == 2 ? -t : 0.0;
+  } else {                // s * 2 (max - f)
+    g0 = hi == 0 ? t : 0.0; g1 = hi == 1 ? 0.0 : -t; g2 = hi == 2 ? t : 0.0;
+  }
+}
+
+// rows of d (fl, df, f6) / d f[0..4] of ppm (ffsl_mod.F90:1147-1173) at the
+// base values f; the edge clipping keeps f -+ df when |df| is not above
+// |fl - f| (|fr - f|)
+struct Jac { double l[5], d[5], s[5]; };
+__device__ __forceinline__ Jac ppm_jac(int lim, const double f[5]) {
+  double gl[3], gc[3], gr[3];
+  slope_jac(lim, f[0], f[1], f[2], gl[0], gl[1], gl[2]);
+  slope_jac(lim, f[1], f[2], f[3], gc[0], gc[1], gc[2]);
+  slope_jac(lim, f[2], f[3], f[4], gr[0], gr[1], gr[2]);
+  const double dfl = slope(lim, f[0], f[1], f[2]);
+  const double df = slope(lim, f[1], f[2], f[3]);
+  const double dfr = slope(lim, f[2], f[3], f[4]);
+  const double fl0 = 0.5 * (f[1] + f[2]) + div_rn(dfl - df, 6.0, 1.0 / 6.0);
+  const double fr0 = 0.5 * (f[3] + f[2]) + div_rn(df - dfr, 6.0, 1.0 / 6.0);
+  double Jl0[5] = {0, 0, 0, 0, 0}, Jr0[5] = {0, 0, 0, 0, 0}, Jc[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    Jl0[k] += gl[k] / 6.0;
+    Jl0[k + 1] -= gc[k] / 6.0;
+    Jr0[k + 1] += gc[k] / 6.0;
+    Jr0[k + 2] -= gr[k] / 6.0;
+    Jc[k + 1] = gc[k];
+  }
+  Jl0[1] += 0.5; Jl0[2] += 0.5;
+  Jr0[3] += 0.5; Jr0[2] += 0.5;
+  const double a = fabs(df), s = copysign(1.0, df);
+  const bool cl = fabs(fl0 - f[2]) < a, cr = fabs(fr0 - f[2]) < a;
+  const double tl = s * (fl0 - f[2] >= 0 ? 1.0 : -1.0), tr = s * (fr0 - f[2] >= 0 ? 1.0 : -1.0);
+  Jac J;
+#pragma unroll
+  for (int t = 0; t < 5; t++) {
+    const double e = t == 2 ? 1.0 : 0.0;
+    const double jl = cl ? e - tl * (Jl0[t] - e) : e - Jc[t];
+    const double jr = cr ? e + tr * (Jr0[t] - e) : e + Jc[t];
+    J.l[t] = jl;
+    J.d[t] = jr - jl;
+    J.s[t] = 6 * e - 3 * (jl + jr);
+  }
+  return J;
+}
+
+// quadrature weights of the x-flux through U point i (hflx_ppm_cell
+// :831-853): flux = wu (sum of cells k0 .. k0+n-1 + w1 fl + w2 df + w3 f6 of
+// the upwind cell iu), |int(c)| clamped to the halo bound as in the forward
+struct XW { int iu, k0, n; double wu, w1, w2, w3; };
+__device__ __forceinline__ XW x_weights(const AdvGrid &g, double c, double u, int i) {
+  int ci = (int)c;
+  const double cf = c - ci;
+  ci = ci > g.hw ? g.hw : (ci < -g.hw ? -g.hw : ci);
+  const bool up = c > 0;
+  const double s1 = up ? 1 - cf : 0, s2 = up ? 1 : -cf;
+  const double ds1 = s2 - s1, ds2 = s2 * s2 - s1 * s1, ds3 = s2 * s2 * s2 - s1 * s1 * s1;
+  XW w;
+  w.iu = wrap(up ? i - ci : i - ci + 1, g.nlon);
+  w.k0 = up ? i + 1 - ci : i + 1;
+  w.n = up ? ci : -ci;
+  w.wu = (up ? u : -u) / c;
+  w.w1 = ds1;
+  w.w2 = 0.5 * ds2;
+  w.w3 = ds2 / 2.0 - ds3 / 3.0;
+  return w;
+}
+// y-flux through V point (h, i): upwind full row ju (:855-877)
+struct YW { int ju; double wv, w1, w2, w3; };
+__device__ __forceinline__ YW y_weights(double c, double v, int h) {
+  const bool up = c > 0;
+  const double s1 = up ? 1 - c : 0, s2 = up ? 1 : -c;
+  const double ds1 = s2 - s1, ds2 = s2 * s2 - s1 * s1, ds3 = s2 * s2 * s2 - s1 * s1 * s1;
+  return YW{up ? h : h + 1, (up ? v : -v) / c, ds1, 0.5 * ds2, ds2 / 2.0 - ds3 / 3.0};
+}
+// offset of full row r (beyond a pole: mirrored, column half a circle away)
+__device__ __forceinline__ int64_t yidx(const AdvGrid &g, int r, int i) {
+  const bool mir = r < 0 || r > g.nlat - 1;
+  const int rr = r < 0 ? -r : (r > g.nlat - 1 ? 2 * (g.nlat - 1) - r : r);
+  return rr * g.pitch + (mir ? wrap(i + g.half, g.nlon) : i);
+}
+
+// tangent of the x-flux through U point i of a row: base row q, tangent d
+__device__ __forceinline__ double xflux_tl(const AdvGrid &g, int lim, double c, double u, int i, const double *q,
+                                           const double *d) {
+  if (c == 0) return 0.0;
+  const int n = g.nlon;
+  const XW w = x_weights(g, c, u, i);
+  double sum = 0.0;
+  for (int k = 0; k < w.n; k++) sum += d[wrap(w.k0 + k, n)];
+  double f[5], dd[5];
+#pragma unroll
+  for (int t = 0; t < 5; t++) {
+    const int k = wrap(w.iu - 2 + t, n);
+    f[t] = q[k];
+    dd[t] = d[k];
+  }
+  const Jac J = ppm_jac(lim, f);
+  double dl = 0, dm = 0, d6 = 0;
+#pragma unroll
+  for (int t = 0; t < 5; t++) { dl += J.l[t] * dd[t]; dm += J.d[t] * dd[t]; d6 += J.s[t] * dd[t]; }
+  return w.wu * (sum + w.w1 * dl + w.w2 * dm + w.w3 * d6);
+}
+// tangent of the y-flux through V point (h, i): base field q, tangent d
+__device__ __forceinline__ double yflux_tl(const AdvGrid &g, int lim, double c, double v, int h, int i,
+                                           const double *q, const double *d) {
+  if (c == 0) return 0.0;
+  const YW w = y_weights(c, v, h);
+  double f[5], dd[5];
+#pragma unroll
+  for (int t = 0; t < 5; t++) {
+    const int64_t k = yidx(g, w.ju - 2 + t, i);
+    f[t] = q[k];
+    dd[t] = d[k];
+  }
+  const Jac J = ppm_jac(lim, f);
+  double dl = 0, dm = 0, d6 = 0;
+#pragma unroll
+  for (int t = 0; t < 5; t++) { dl += J.l[t] * dd[t]; dm += J.d[t] * dd[t]; d6 += J.s[t] * dd[t]; }
+  return w.wv * (w.w1 * dl + w.w2 * dm + w.w3 * d6);
+}
+// adjoint of xflux_tl: lam * d flux / d d added into the row a
+__device__ __forceinline__ void xflux_ad(const AdvGrid &g, int lim, double c, double u, int i, const double *q,
+                                         double *a, double lam) {
+  if (c == 0 || lam == 0) return;
+  const int n = g.nlon;
+  const XW w = x_weights(g, c, u, i);
+  const double gw = w.wu * lam;
+  for (int k = 0; k < w.n; k++) unsafeAtomicAdd(&a[wrap(w.k0 + k, n)], gw);
+  double f[5];
+#pragma unroll
+  for (int t = 0; t < 5; t++) f[t] = q[wrap(w.iu - 2 + t, n)];
+  const Jac J = ppm_jac(lim, f);
+#pragma unroll
+  for (int t = 0; t < 5; t++)
+    unsafeAtomicAdd(&a[wrap(w.iu - 2 + t, n)], gw * (w.w1 * J.l[t] + w.w2 * J.d[t] + w.w3 * J.s[t]));
+}
+__device__ __forceinline__ void yflux_ad(const AdvGrid &g, int lim, double c, double v, int h, int i,
+                                         const double *q, double *a, double lam) {
+  if (c == 0 || lam == 0) return;
+  const YW w = y_weights(c, v, h);
+  const double gw = w.wv * lam;
+  double f[5];
+  int64_t k[5];
+#pragma unroll
+  for (int t = 0; t < 5; t++) {
+    k[t] = yidx(g, w.ju - 2 + t, i);
+    f[t] = q[k[t]];
+  }
+  const Jac J = ppm_jac(lim, f);
+#pragma unroll
+  for (int t = 0; t < 5; t++) unsafeAtomicAdd(&a[k[t]], gw * (w.w1 * J.l[t] + w.w2 * J.d[t] + w.w3 * J.s[t]));
+}
+
+// base state of one taped step (tracer 0 of every per-tracer field)
+struct TBase {
+  const double *q, *qst, *qx, *qy, *qp;
+  const double *u, *v, *mfx, *mfy, *cflx, *cfly, *divx, *divy, *old_m, *m;
+};
+// tangent / adjoint fields of one step (tracer 0), tracer stride ts
+struct TWork {
+  const double *in;   // tangent of the old level / adjoint of the new level
+  double *dqx, *dqy, *dqp, *out;
+  const double *ps;   // pole sums [ntracers][2]
+  int64_t ts;
+};
+
+__device__ __forceinline__ double block_sum(double v, double *red) {
+  for (int w = 1; w < kWave; w <<= 1) v += __shfl_xor(v, w, kWave);
+  if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int k = 0; k < (int)(blockDim.x / kWave); k++) s += red[k];
+  return s;
+}
+
+// ps[t][p] = sum over the columns of the y-flux tangent of the V row next to
+// pole p (0 south, 1 north); block (p, t)
+__global__ void __launch_bounds__(NA) t_adv_psum(AdvGrid g, int lim, const double *q, const double *d,
+                                                 const double *v, const double *cfly, int64_t ts, double *ps) {
+  __shared__ double red[NA / kWave];
+  const int p = blockIdx.x, t = blockIdx.y, h = p == 0 ? 0 : g.nhalf - 1;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < g.nlon; i += NA)
+    acc += yflux_tl(g, lim, cfly[h * g.pitch + i], v[h * g.pitch + i], h, i, q + t * ts, d + t * ts);
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) ps[2 * t + p] = s;
+}
+// ps[t][p] = sum over the columns of the pole row p of f
+__global__ void __launch_bounds__(NA) k_adv_psum2(AdvGrid g, const double *f, int64_t ts, double *ps) {
+  __shared__ double red[NA / kWave];
+  const int p = blockIdx.x, t = blockIdx.y, r = p == 0 ? 0 : g.nlat - 1;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < g.nlon; i += NA) acc += f[t * ts + r * g.pitch + i];
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) ps[2 * t + p] = s;
+}
+
+// tangents of qx, qy (ffsl_mod.F90:350-407); grid (column chunks, rows,
+// tracers)
+__global__ void __launch_bounds__(NA) t_adv_inner(AdvGrid g, Mesh M, int lim, TBase b, TWork w) {
+  const int i = blockIdx.x * NA + threadIdx.x, r = blockIdx.y, t = blockIdx.z;
+  if (i >= g.nlon) return;
+  const int64_t to = t * w.ts, P = g.pitch, k = r * P + i;
+  const double *q = b.q + to, *d = w.in + to;
+  const double dt = g.dt, dc = d[k];
+  if (r == 0 || r == g.nlat - 1) {
+    const int h = r == 0 ? 0 : g.nhalf - 1;
+    const double pole = w.ps[2 * t + (r == 0 ? 0 : 1)] * M(GMC_LE_LAT, h) / g.nlon / M(GMC_AREA_CELL, r);
+    w.dqx[to + k] = dc;
+    w.dqy[to + k] = r == 0 ? dc - 0.5 * (pole - b.divy[k] * dc) * dt : dc + 0.5 * (pole - b.divy[k] * dc) * dt;
+    return;
+  }
+  const int iw = wrap(i - 1, g.nlon);
+  const double fx = xflux_tl(g, lim, b.cflx[k], b.u[k], i, q + r * P, d + r * P);
+  const double fxw = xflux_tl(g, lim, b.cflx[r * P + iw], b.u[r * P + iw], iw, q + r * P, d + r * P);
+  const double fs = yflux_tl(g, lim, b.cfly[(r - 1) * P + i], b.v[(r - 1) * P + i], r - 1, i, q, d);
+  const double fn = yflux_tl(g, lim, b.cfly[r * P + i], b.v[r * P + i], r, i, q, d);
+  const double lel = M(GMC_LE_LON, r), ac = M(GMC_AREA_CELL, r);
+  const double ls = M(GMC_LE_LAT, r - 1), ln = M(GMC_LE_LAT, r);
+  w.dqx[to + k] = dc - 0.5 * ((fx - fxw) * lel / ac - b.divx[k] * dc) * dt;
+  w.dqy[to + k] = dc - 0.5 * ((fn * ln - fs * ls) / ac - b.divy[k] * dc) * dt;
+}
+
+// tangent of the outer fluxes and the update (adv_mod.F90:183-235) into dqp
+__global__ void __launch_bounds__(NA) t_adv_outer(AdvGrid g, Mesh M, int lim, TBase b, TWork w) {
+  const int i = blockIdx.x * NA + threadIdx.x, r = blockIdx.y, t = blockIdx.z;
+  if (i >= g.nlon) return;
+  const int64_t to = t * w.ts, P = g.pitch, k = r * P + i;
+  const double dt = g.dt, dc = w.in[to + k];
+  double num;
+  if (r == 0 || r == g.nlat - 1) {
+    const int h = r == 0 ? 0 : g.nhalf - 1;
+    const double pole = w.ps[2 * t + (r == 0 ? 0 : 1)] * M(GMC_LE_LAT, h) / g.nlon / M(GMC_AREA_CELL, r) * dt;
+    num = r == 0 ? b.old_m[k] * dc - pole : b.old_m[k] * dc + pole;
+  } else {
+    const double *qy = b.qy + to + r * P, *dqy = w.dqy + to + r * P;
+    const int iw = wrap(i - 1, g.nlon);
+    const double fx = xflux_tl(g, lim, b.cflx[k], b.mfx[k], i, qy, dqy);
+    const double fxw = xflux_tl(g, lim, b.cflx[r * P + iw], b.mfx[r * P + iw], iw, qy, dqy);
+    const double fs = yflux_tl(g, lim, b.cfly[(r - 1) * P + i], b.mfy[(r - 1) * P + i], r - 1, i, b.qx + to, w.dqx + to);
+    const double fn = yflux_tl(g, lim, b.cfly[r * P + i], b.mfy[r * P + i], r, i, b.qx + to, w.dqx + to);
+    const double lel = M(GMC_LE_LON, r), ac = M(GMC_AREA_CELL, r);
+    const double ls = M(GMC_LE_LAT, r - 1), ln = M(GMC_LE_LAT, r);
+    num = b.old_m[k] * dc - ((fx - fxw) * lel + (fn * ln - fs * ls)) / ac * dt;
+  }
+  w.dqp[to + k] = num / b.m[k];
+}
+
+// One row's positivity fixer (adv_mod.F90:236-258) on the base values sb,
+// carrying the tangent sd (when sd != null) or recording each cell's branch
+// in code (0 none, 1 both neighbours, 2 west, 3 east, 4 none possible) for
+// the adjoint.  Clusters of originally negative columns (gaps of at most one
+// column) are independent, one lane each, walked west to east as in
+// k_adv_fix; with code_end != null, code_end[i0] receives the last column
+// walked from cluster start i0.
+__device__ __forceinline__ void fixer_walk(int n, double *sb, double *sd, const double *neg, double *code,
+                                           double *code_end, const double *mrow, double west, double east) {
+  for (int i0 = threadIdx.x; i0 < n; i0 += NA) {
+    if (neg[i0] == 0.0 || (i0 >= 1 && neg[i0 - 1] != 0.0) || (i0 >= 2 && neg[i0 - 2] != 0.0)) continue;
+    int i = i0;
+    for (;;) {
+      double c = 0.0;
+      if (sb[i] < 0) {
+        const int im = i == 0 ? n - 1 : i - 1, ip = i == n - 1 ? 0 : i + 1;
+        const double qm0 = sb[i] * mrow[i];
+        const double qm1 = (i == 0 ? west : sb[im]) * mrow[im];
+        const double qm2 = (i == n - 1 ? east : sb[ip]) * mrow[ip];
+        const double qm0_half = 0.5 * qm0;
+        const double d0 = sd ? sd[i] * mrow[i] : 0.0;
+        if (qm1 >= qm0_half && qm2 >= qm0_half) {
+          c = 1.0;
+          const double a = (qm1 - qm0_half) / mrow[im], bb = (qm2 - qm0_half) / mrow[ip];
+          if (i == 0) west = a; else sb[im] = a;
+          if (i == n - 1) east = bb; else sb[ip] = bb;
+          if (sd && i > 0) sd[im] = (sd[im] * mrow[im] - 0.5 * d0) / mrow[im];
+          if (sd && i < n - 1) sd[ip] = (sd[ip] * mrow[ip] - 0.5 * d0) / mrow[ip];
+        } else if (qm1 > qm0) {
+          c = 2.0;
+          const double a = (qm1 - qm0) / mrow[im];
+          if (i == 0) west = a; else sb[im] = a;
+          if (sd && i > 0) sd[im] = (sd[im] * mrow[im] - d0) / mrow[im];
+        } else if (qm2 > qm0) {
+          c = 3.0;
+          const double bb = (qm2 - qm0) / mrow[ip];
+          if (i == n - 1) east = bb; else sb[ip] = bb;
+          if (sd && i < n - 1) sd[ip] = (sd[ip] * mrow[ip] - d0) / mrow[ip];
+        } else {
+          c = 4.0;
+        }
+        sb[i] = 0;
+        if (sd) sd[i] = 0;
+      }
+      if (code) code[i] = c;
+      if (i + 1 < n && neg[i + 1] != 0.0) i = i + 1;
+      else if (i + 2 < n && neg[i + 2] != 0.0) i = i + 2;
+      else break;
+    }
+    if (code_end) code_end[i0] = (double)i;
+  }
+}
+
+// Jacobi sweeps of lon_damp_on_cell, order 2 (lon_damp_mod.F90:158-241), on
+// the LDS row a (b scratch); returns the row holding the result.  The sweep
+// is a symmetric circulant, its own transpose.
+__device__ __forceinline__ double *damp_row(int n, double w, int passes, double *a, double *b) {
+  if (w == 0.0) return a;
+  for (int pass = 0; pass < passes; pass++) {
+    for (int i = threadIdx.x; i < n; i += NA)
+      b[i] = w * (a[i] + 0.25 * (a[wrap(i - 1, n)] - 2.0 * a[i] + a[wrap(i + 1, n)])) + (1 - w) * a[i];
+    __syncthreads();
+    double *t = a; a = b; b = t;
+  }
+  return a;
+}
+
+// tangent of the fixer and the damping of row r of tracer t: dqp -> out
+__global__ void __launch_bounds__(NA) t_adv_fix(AdvGrid g, TBase b, TWork w, const double *wgt, int passes) {
+  extern __shared__ double lds[];
+  const int n = g.nlon, r = blockIdx.x;
+  const int64_t to = blockIdx.y * w.ts, row = to + r * g.pitch;
+  double *sb = lds, *sd = lds + n, *st = lds + 2 * n, *neg = lds + 3 * n;
+  const double *mrow = b.m + r * g.pitch;
+  for (int i = threadIdx.x; i < n; i += NA) {
+    sb[i] = b.qp[row + i];
+    sd[i] = w.dqp[row + i];
+    neg[i] = sb[i] < 0 ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  fixer_walk(n, sb, sd, neg, nullptr, nullptr, mrow, b.qst[row + n - 1], b.qst[row]);
+  __syncthreads();
+  const double wr = (r > 0 && r < g.nlat - 1) ? wgt[r] : 0.0;
+  const double *res = damp_row(n, wr, passes, sd, st);
+  for (int i = threadIdx.x; i < n; i += NA) w.out[row + i] = res[i];
+}
+
+// adjoint of t_adv_fix and of the division by m: the adjoint of the new
+// level (in) -> dqp holds the adjoint of the update's numerator; clears the
+// rows of dqx, dqy the flux adjoints accumulate into
+__global__ void __launch_bounds__(NA) a_adv_fix(AdvGrid g, TBase b, TWork w, const double *wgt, int passes) {
+  extern __shared__ double lds[];
+  const int n = g.nlon, r = blockIdx.x;
+  const int64_t to = blockIdx.y * w.ts, row = to + r * g.pitch;
+  double *sl = lds, *st = lds + n, *sb = lds + 2 * n, *neg = lds + 3 * n, *code = lds + 4 * n;
+  double *cend = st;  // the damping scratch is free again once it is done
+  const double *mrow = b.m + r * g.pitch;
+  for (int i = threadIdx.x; i < n; i += NA) {
+    sl[i] = w.in[row + i];
+    sb[i] = b.qp[row + i];
+    neg[i] = sb[i] < 0 ? 1.0 : 0.0;
+    code[i] = 0.0;
+    w.dqx[row + i] = 0.0;
+    w.dqy[row + i] = 0.0;
+  }
+  __syncthreads();
+  const double wr = (r > 0 && r < g.nlat - 1) ? wgt[r] : 0.0;
+  double *lam = damp_row(n, wr, passes, sl, st);
+  if (lam != sl) {
+    for (int i = threadIdx.x; i < n; i += NA) sl[i] = lam[i];
+    __syncthreads();
+  }
+  fixer_walk(n, sb, nullptr, neg, code, cend, mrow, b.qst[row + n - 1], b.qst[row]);
+  __syncthreads();
+  // the fixer transposed, each cluster east to west: the cell's own
+  // adjoint is replaced by what its neighbours' adjoints take from it; the
+  // seam writes were discarded, so they contribute nothing
+  for (int i0 = threadIdx.x; i0 < n; i0 += NA) {
+    if (neg[i0] == 0.0 || (i0 >= 1 && neg[i0 - 1] != 0.0) || (i0 >= 2 && neg[i0 - 2] != 0.0)) continue;
+    for (int i = (int)cend[i0]; i >= i0; i--) {
+      const int c = (int)code[i];
+      if (c == 0) continue;
+      const double f = c == 1 ? 0.5 : 1.0;
+      double gsum = 0.0;
+      if ((c == 1 || c == 2) && i > 0) gsum -= f * mrow[i] / mrow[i - 1] * sl[i - 1];
+      if ((c == 1 || c == 3) && i < n - 1) gsum -= f * mrow[i] / mrow[i + 1] * sl[i + 1];
+      sl[i] = gsum;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += NA) w.dqp[row + i] = sl[i] / mrow[i];
+}
+
+// adjoint of the update and the outer fluxes: dqp (numerator adjoint) ->
+// out = old_m * dqp (the direct term), and the flux adjoints added into
+// dqy (x-fluxes of qy) and dqx (y-fluxes of qx)
+__global__ void __launch_bounds__(NA) a_adv_outer(AdvGrid g, Mesh M, int lim, TBase b, TWork w) {
+  const int i = blockIdx.x * NA + threadIdx.x, r = blockIdx.y, t = blockIdx.z;
+  if (i >= g.nlon) return;
+  const int64_t to = t * w.ts, P = g.pitch, k = r * P + i;
+  const double dt = g.dt;
+  const double *an = w.dqp + to;
+  w.out[to + k] = b.old_m[k] * an[k];
+  if (r >= 1 && r <= g.nlat - 2) {
+    const double lf = (an[r * P + wrap(i + 1, g.nlon)] - an[k]) * M(GMC_LE_LON, r) / M(GMC_AREA_CELL, r) * dt;
+    xflux_ad(g, lim, b.cflx[k], b.mfx[k], i, b.qy + to + r * P, w.dqy + to + r * P, lf);
+  }
+  if (r <= g.nlat - 2) {
+    const int h = r;
+    double lf = 0.0;
+    if (h >= 1) lf -= M(GMC_LE_LAT, h) / M(GMC_AREA_CELL, h) * dt * an[h * P + i];
+    if (h + 1 <= g.nlat - 2) lf += M(GMC_LE_LAT, h) / M(GMC_AREA_CELL, h + 1) * dt * an[(h + 1) * P + i];
+    if (h == 0) lf -= M(GMC_LE_LAT, 0) / g.nlon / M(GMC_AREA_CELL, 0) * dt * w.ps[2 * t];
+    if (h == g.nlat - 2) lf += M(GMC_LE_LAT, h) / g.nlon / M(GMC_AREA_CELL, g.nlat - 1) * dt * w.ps[2 * t + 1];
+    yflux_ad(g, lim, b.cfly[h * P + i], b.mfy[h * P + i], h, i, b.qx + to, w.dqx + to, lf);
+  }
+}
+
+// adjoint of qx, qy and the inner fluxes, added into out
+__global__ void __launch_bounds__(NA) a_adv_inner(AdvGrid g, Mesh M, int lim, TBase b, TWork w) {
+  const int i = blockIdx.x * NA + threadIdx.x, r = blockIdx.y, t = blockIdx.z;
+  if (i >= g.nlon) return;
+  const int64_t to = t * w.ts, P = g.pitch, k = r * P + i;
+  const double dt = g.dt;
+  const double *ax = w.dqx + to, *ay = w.dqy + to;
+  const double gx = ax[k], gy = ay[k];
+  double add;
+  if (r == 0) add = gx + (1 + 0.5 * b.divy[k] * dt) * gy;
+  else if (r == g.nlat - 1) add = gx + (1 - 0.5 * b.divy[k] * dt) * gy;
+  else add = (1 + 0.5 * b.divx[k] * dt) * gx + (1 + 0.5 * b.divy[k] * dt) * gy;
+  unsafeAtomicAdd(&w.out[to + k], add);
+  if (r >= 1 && r <= g.nlat - 2) {
+    const double lf = 0.5 * dt * M(GMC_LE_LON, r) / M(GMC_AREA_CELL, r) * (ax[r * P + wrap(i + 1, g.nlon)] - gx);
+    xflux_ad(g, lim, b.cflx[k], b.u[k], i, b.q + to + r * P, w.out + to + r * P, lf);
+  }
+  if (r <= g.nlat - 2) {
+    const int h = r;
+    double lf = 0.0;
+    if (h >= 1) lf -= 0.5 * dt * M(GMC_LE_LAT, h) / M(GMC_AREA_CELL, h) * ay[h * P + i];
+    if (h + 1 <= g.nlat - 2) lf += 0.5 * dt * M(GMC_LE_LAT, h) / M(GMC_AREA_CELL, h + 1) * ay[(h + 1) * P + i];
+    if (h == 0) lf -= 0.5 * dt * M(GMC_LE_LAT, 0) / g.nlon / M(GMC_AREA_CELL, 0) * w.ps[2 * t];
+    if (h == g.nlat - 2) lf += 0.5 * dt * M(GMC_LE_LAT, h) / g.nlon / M(GMC_AREA_CELL, g.nlat - 1) * w.ps[2 * t + 1];
+    yflux_ad(g, lim, b.cfly[h * P + i], b.v[h * P + i], h, i, b.q + to, w.out + to, lf);
+  }
+}
+
+}  // namespace
+}  // namespace gmc
+
+namespace {
+
+// host-side part of a checkpoint (the device part is a copy of the slab)
+struct HostCk { int uv_step, mf_step, old, new_; bool old_m_current; };
+HostCk host_ck(const gmc_adv *a) { return HostCk{a->uv_step, a->mf_step, a->old, a->new_, a->old_m_current}; }
+void host_restore(gmc_adv *a, const HostCk &c) {
+  a->uv_step = c.uv_step; a->mf_step = c.mf_step; a->old = c.old; a->new_ = c.new_;
+  a->old_m_current = c.old_m_current;
+}
+
+size_t slot_bytes(const gmc_adv *a) { return (a->slab_bytes + 255) / 256 * 256; }
+size_t tl_fields(const gmc_adv *a) { return (size_t)a->cfg.ntracers * a->g.nlat * a->g.pitch; }
+
+// work fields (5 per tracer: two tangent / adjoint levels, dqx, dqy, dqp,
+// then 4 pole sums per tracer) and nslots slab copies
+int tl_alloc(gmc_adv *a, int nslots) {
+  int rc = GMC_OK;
+  if (!a->tl_f) {
+    const size_t bytes = (5 * tl_fields(a) + 4 * (size_t)a->cfg.ntracers) * sizeof(double);
+    rc = ahip(a, hipMalloc((void **)&a->tl_f, bytes), "tangent-linear work fields");
+    if (!rc) rc = ahip(a, hipMemset(a->tl_f, 0, bytes), "memset");
+    const int lds = (int)(5 * (size_t)a->g.nlon * sizeof(double));
+    if (!rc) rc = ahip(a, hipFuncSetAttribute((const void *)t_adv_fix, hipFuncAttributeMaxDynamicSharedMemorySize, lds), "lds");
+    if (!rc) rc = ahip(a, hipFuncSetAttribute((const void *)a_adv_fix, hipFuncAttributeMaxDynamicSharedMemorySize, lds), "lds");
+    if (rc) return rc;
+  }
+  const size_t need = (size_t)nslots * slot_bytes(a);
+  if (need > a->tl_ws_bytes) {
+    if (a->tl_ws) {
+      (void)hipStreamSynchronize(a->stream);
+      (void)hipFree(a->tl_ws);
+      a->tl_ws = nullptr;
+      a->tl_ws_bytes = 0;
+    }
+    rc = ahip(a, hipMalloc((void **)&a->tl_ws, need), "tangent-linear checkpoints and tapes");
+    if (!rc) a->tl_ws_bytes = need;
+  }
+  return rc;
+}
+char *tl_slot(gmc_adv *a, int k) { return a->tl_ws + (size_t)k * slot_bytes(a); }
+int slab_to(gmc_adv *a, char *dst) {
+  return ahip(a, hipMemcpyAsync(dst, a->slab, a->slab_bytes, hipMemcpyDeviceToDevice, a->stream), "checkpoint");
+}
+int slab_from(gmc_adv *a, const char *src) {
+  return ahip(a, hipMemcpyAsync(a->slab, src, a->slab_bytes, hipMemcpyDeviceToDevice, a->stream), "restore");
+}
+
+TBase tape_base(gmc_adv *a, const char *slot, int old) {
+  auto S = [&](const double *p) { return (const double *)(slot + ((const char *)p - (const char *)a->slab)); };
+  return TBase{S(tracer_ptr(a, old, 0)), S(tracer_ptr(a, 3 - old, 0)), S(a->qx), S(a->qy), S(a->qp),
+               S(a->u), S(a->v), S(a->mfx), S(a->mfy), S(a->cflx), S(a->cfly), S(a->divx), S(a->divy),
+               S(a->old_m), S(a->sm)};
+}
+
+// tangent of one taped step: in (old level) -> out (new level)
+void tl_step(gmc_adv *a, const char *slot, int old, const double *in, double *out) {
+  const AdvGrid &g = a->g;
+  const Mesh M{a->d_mesh, g.nlat};
+  const int ntr = a->cfg.ntracers, lim = a->cfg.limiter;
+  const int64_t ts = (int64_t)g.nlat * g.pitch;
+  const size_t F = tl_fields(a);
+  double *dqx = a->tl_f + 2 * F, *dqy = a->tl_f + 3 * F, *dqp = a->tl_f + 4 * F, *ps = a->tl_f + 5 * F;
+  const TBase b = tape_base(a, slot, old);
+  const dim3 cells((g.nlon + NA - 1) / NA, g.nlat, ntr), poles(2, ntr);
+  hipLaunchKernelGGL(t_adv_psum, poles, dim3(NA), 0, a->stream, g, lim, b.q, in, b.v, b.cfly, ts, ps);
+  hipLaunchKernelGGL(t_adv_inner, cells, dim3(NA), 0, a->stream, g, M, lim, b, TWork{in, dqx, dqy, dqp, out, ps, ts});
+  double *ps2 = ps + 2 * ntr;
+  hipLaunchKernelGGL(t_adv_psum, poles, dim3(NA), 0, a->stream, g, lim, b.qx, dqx, b.mfy, b.cfly, ts, ps2);
+  hipLaunchKernelGGL(t_adv_outer, cells, dim3(NA), 0, a->stream, g, M, lim, b, TWork{in, dqx, dqy, dqp, out, ps2, ts});
+  hipLaunchKernelGGL(t_adv_fix, dim3(g.nlat, ntr), dim3(NA), 4 * (size_t)g.nlon * sizeof(double), a->stream, g, b,
+                     TWork{in, dqx, dqy, dqp, out, ps, ts}, a->d_wgt, a->cfg.lon_damp_passes);
+}
+
+// adjoint of one taped step: in (adjoint of the new level) -> out (of the
+// old level)
+void ad_step(gmc_adv *a, const char *slot, int old, const double *in, double *out) {
+  const AdvGrid &g = a->g;
+  const Mesh M{a->d_mesh, g.nlat};
+  const int ntr = a->cfg.ntracers, lim = a->cfg.limiter;
+  const int64_t ts = (int64_t)g.nlat * g.pitch;
+  const size_t F = tl_fields(a);
+  double *dqx = a->tl_f + 2 * F, *dqy = a->tl_f + 3 * F, *dqp = a->tl_f + 4 * F, *ps = a->tl_f + 5 * F;
+  double *ps2 = ps + 2 * ntr;
+  const TBase b = tape_base(a, slot, old);
+  const dim3 cells((g.nlon + NA - 1) / NA, g.nlat, ntr), poles(2, ntr);
+  hipLaunchKernelGGL(a_adv_fix, dim3(g.nlat, ntr), dim3(NA), 5 * (size_t)g.nlon * sizeof(double), a->stream, g, b,
+                     TWork{in, dqx, dqy, dqp, out, ps, ts}, a->d_wgt, a->cfg.lon_damp_passes);
+  hipLaunchKernelGGL(k_adv_psum2, poles, dim3(NA), 0, a->stream, g, dqp, ts, ps);
+  hipLaunchKernelGGL(a_adv_outer, cells, dim3(NA), 0, a->stream, g, M, lim, b, TWork{in, dqx, dqy, dqp, out, ps, ts});
+  hipLaunchKernelGGL(k_adv_psum2, poles, dim3(NA), 0, a->stream, g, dqy, ts, ps2);
+  hipLaunchKernelGGL(a_adv_inner, cells, dim3(NA), 0, a->stream, g, M, lim, b, TWork{in, dqx, dqy, dqp, out, ps2, ts});
+}
+
+int tl_check(gmc_adv *a, const double *scale, int32_t nsteps, int64_t time_step0, const double *buf, int64_t ld) {
+  if (!a->have_basis) return aerr(a, GMC_ESTATE, "gmc_adv_set_wind_basis must come first");
+  if (!scale || nsteps < 0 || time_step0 < 0 || !buf || ld < a->g.nlon)
+    return aerr(a, GMC_EINVAL, "bad tangent-linear / adjoint arguments");
+  return GMC_OK;
+}
+
+// [ntracers][nlat][ld] host <-> a device tangent / adjoint level
+int tl_copy(gmc_adv *a, bool up, double *dev, double *host, int64_t ld) {
+  const int rows = a->cfg.ntracers * a->g.nlat;
+  int rc = ahip(a, stage_copy(a->stage, a->stream, up, dev, a->g.pitch, host, ld, a->g.nlon, rows),
+                up ? "upload" : "download");
+  if (!rc) rc = ahip(a, hipGetLastError(), "tangent-linear / adjoint launch");
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gmc_adv_tl_drive(gmc_adv *a, int64_t time_step0, const double *scale, int32_t nsteps, double *dq, int64_t ld) {
+  if (!a) return GMC_EINVAL;
+  int rc = tl_check(a, scale, nsteps, time_step0, dq, ld);
+  if (!rc) rc = tl_alloc(a, 1);
+  if (rc) return rc;
+  const size_t F = tl_fields(a);
+  double *cur = a->tl_f, *nxt = a->tl_f + F;
+  rc = tl_copy(a, true, cur, dq, ld);
+  for (int s = 0; s < nsteps && !rc; s++) {
+    int old = 0;
+    rc = run_once(a, time_step0 + s, true, scale[s], tl_slot(a, 0), &old);
+    if (!rc && old) {
+      tl_step(a, tl_slot(a, 0), old, cur, nxt);
+      std::swap(cur, nxt);
+      rc = ahip(a, hipGetLastError(), "tangent-linear launch");
+    }
+  }
+  if (!rc) rc = check_flags(a);
+  if (!rc) rc = tl_copy(a, false, cur, dq, ld);
+  if (!rc) rc = ahip(a, hipStreamSynchronize(a->stream), "sync");
+  return rc;
+}
+
+int gmc_adv_ad_drive(gmc_adv *a, int64_t time_step0, const double *scale, int32_t nsteps, int32_t ckpt_every,
+                     double *lam, int64_t ld) {
+  if (!a) return GMC_EINVAL;
+  int rc = tl_check(a, scale, nsteps, time_step0, lam, ld);
+  if (!rc && ckpt_every < 1) rc = aerr(a, GMC_EINVAL, "ckpt_every must be at least 1");
+  if (rc) return rc;
+  const int K = std::min<int>(ckpt_every, std::max<int>(nsteps, 1));
+  const int nseg = (nsteps + K - 1) / K;
+  // slots: checkpoints 0 .. nseg (the last is the window's end), then K tapes
+  rc = tl_alloc(a, nseg + 1 + K);
+  if (rc) return rc;
+  std::vector<HostCk> hck(nseg + 1);
+  for (int s = 0; s < nsteps && !rc; s++) {
+    if (s % K == 0) {
+      hck[s / K] = host_ck(a);
+      rc = slab_to(a, tl_slot(a, s / K));
+    }
+    if (!rc) rc = run_once(a, time_step0 + s, true, scale[s]);
+  }
+  if (!rc) {
+    hck[nseg] = host_ck(a);
+    rc = slab_to(a, tl_slot(a, nseg));
+  }
+  if (!rc) rc = check_flags(a);
+  const size_t F = tl_fields(a);
+  double *cur = a->tl_f, *nxt = a->tl_f + F;
+  if (!rc) rc = tl_copy(a, true, cur, lam, ld);
+  std::vector<int> olds(K);
+  for (int seg = nseg - 1; seg >= 0 && !rc; seg--) {
+    host_restore(a, hck[seg]);
+    rc = slab_from(a, tl_slot(a, seg));
+    const int s0 = seg * K, s1 = std::min(nsteps, s0 + K);
+    for (int s = s0; s < s1 && !rc; s++)
+      rc = run_once(a, time_step0 + s, true, scale[s], tl_slot(a, nseg + 1 + (s - s0)), &olds[s - s0]);
+    for (int s = s1 - 1; s >= s0 && !rc; s--) {
+      if (!olds[s - s0]) continue;
+      ad_step(a, tl_slot(a, nseg + 1 + (s - s0)), olds[s - s0], cur, nxt);
+      std::swap(cur, nxt);
+      rc = ahip(a, hipGetLastError(), "adjoint launch");
+    }
+  }
+  if (!rc) {
+    host_restore(a, hck[nseg]);
+    rc = slab_from(a, tl_slot(a, nseg));
+  }
+  if (!rc) rc = tl_copy(a, false, cur, lam, ld);
+  if (!rc) rc = ahip(a, hipStreamSynchronize(a->stream), "sync");
+  return rc;
+}
 
 }  // extern "C"

@@ -86,6 +86,30 @@ int gmc_adv_run(gmc_adv *adv, int64_t time_step);
 int gmc_adv_set_wind_basis(gmc_adv *adv, const double *u_basis, const double *v_basis, int64_t ld);
 int gmc_adv_drive(gmc_adv *adv, int64_t time_step0, const double *scale, int32_t nsteps);
 
+/* Tangent linear and adjoint of the prescribed-flow window with respect to
+ * the tracers (no counterpart in the reference: the linearisation of adv_run,
+ * adv_mod.F90:152-312, with ffsl_calc_tracer_hflx_cell, ffsl_mod.F90:318-413,
+ * about a base trajectory, the winds held).  The step is piecewise linear in
+ * q; the tangent follows the branches the base takes in the slope limiter,
+ * the PPM edge clipping and the positivity fixer.
+ *
+ * gmc_adv_tl_drive runs gmc_adv_drive(time_step0, scale, nsteps) and maps dq
+ * [ntracers][nlat][ld], the tangent of the old level at entry, to the tangent
+ * of the old level at exit, in place.
+ *
+ * gmc_adv_ad_drive runs the same window and maps lam [ntracers][nlat][ld],
+ * the adjoint of the old level at exit (for a cost J(q_N): dJ/dq_N), to the
+ * adjoint of the old level at entry (dJ/dq_0), in place.  It keeps the slab
+ * every ckpt_every steps, then, last segment first, recomputes the segment
+ * from its checkpoint keeping every step's base state and runs the adjoint
+ * steps back over it.  Memory: (nsteps / ckpt_every + 1 + ckpt_every) copies
+ * of the object's device slab, kept between calls.  The object ends as after
+ * gmc_adv_drive.  The adjoint adds with atomics: reproducible to rounding, not
+ * bit for bit. */
+int gmc_adv_tl_drive(gmc_adv *adv, int64_t time_step0, const double *scale, int32_t nsteps, double *dq, int64_t ld);
+int gmc_adv_ad_drive(gmc_adv *adv, int64_t time_step0, const double *scale, int32_t nsteps, int32_t ckpt_every,
+                     double *lam, int64_t ld);
+
 /* Index (1 or 2) of the old time level after the swaps. */
 int gmc_adv_old_level(const gmc_adv *adv, int32_t *old_level);
 
