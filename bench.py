@@ -48,6 +48,11 @@ ADV_CONFIGS = {
     "ffsl05": dict(nlon=720, nlat=361, dt=900.0, ntracers=8, lmax=10, wind_seed=2, tracer_seed=3,
                    vmax=50.0, hw=104, steps=96),
 }
+# gradient of a final-state tracer misfit over a one-day FFSL window at the
+# ffsl05 workload, against the same window's forward
+ADV_ADJ_CONFIGS = {
+    "adjffsl05": dict(base="ffsl05", window=96, ckpt=8, bg_seed=4),
+}
 # compulsory bytes per tracer-grid-point of an advection step: every tracer
 # read and written once, the 4 wind fields (u, v, mfx, mfy) shared
 ADV_BYTES_PER_TRACER_GP = lambda ntr: (2 * ntr + 4) * 8.0 / ntr
@@ -393,13 +398,88 @@ def run_adj(args):
     print(json.dumps(result))
 
 
+def run_adv_adj(args):
+    """Gradient of J = 1/2 sum (q_N - y)^2 over the tracers of an FFSL window
+    (gmc_adv_gradient: window forward with checkpoints, segments recomputed
+    with their base states kept, adjoint steps), against the forward of the
+    same window (gmc_adv_drive); value = gradient time / forward time.  Both
+    are timed on a second window (steps N .. 2N-1) after a first one that
+    allocates the checkpoints and tapes; y is the truth run's state at 2N.
+    Also the dot-product test of the tangent and adjoint over 4 steps."""
+    from gmcore_amd import init_cond
+    from gmcore_amd.adv import Advection, AdvSetup
+    from gmcore_amd.namelist import Namelist
+    acfg = ADV_ADJ_CONFIGS[args.config]
+    cfg = ADV_CONFIGS[acfg["base"]]
+    nlon, nlat, dt, ntr, N = cfg["nlon"], cfg["nlat"], cfg["dt"], cfg["ntracers"], acfg["window"]
+    nl = Namelist(num_lon=nlon, num_lat=nlat, dt_adv=dt, dt_dyn=dt, limiter_type="mono")
+    st = AdvSetup(nl, ntr, lon_halo_width=cfg["hw"])
+    m = st.mesh.metrics
+    ub, vb = init_cond.deform_wind_basis(nlon, nlat, m, cfg["lmax"], cfg["wind_seed"], cfg["vmax"])
+    sc = init_cond.wind_scales(2 * N, dt)
+
+    def make(qs):
+        g = Advection(st)
+        for l, q in enumerate(qs):
+            g.set_tracer(1, l, q)
+            g.set_tracer(2, l, q)
+        g.set_wind_basis(ub, vb)
+        return g
+
+    truth = make(init_cond.tracer_set(nlon, nlat, m, cfg["tracer_seed"], ntr))
+    truth.drive(0, sc)
+    y = np.stack([truth.tracer(truth.old, l) for l in range(ntr)])
+    truth.close()
+    bg = init_cond.tracer_set(nlon, nlat, m, acfg["bg_seed"], ntr)
+    g = make(bg)
+    g.gradient(0, sc[:N], y, ckpt_every=acfg["ckpt"])
+    g.synchronize()
+    w0 = time.perf_counter()
+    J, grad = g.gradient(N, sc[N:], y, ckpt_every=acfg["ckpt"])
+    t_grad = time.perf_counter() - w0
+    g.close()
+    f = make(bg)
+    f.drive(0, sc[:N])
+    f.synchronize()
+    w0 = time.perf_counter()
+    f.drive(N, sc[N:])
+    f.synchronize()
+    t_fwd = time.perf_counter() - w0
+    f.close()
+    rs = np.random.default_rng(7)
+    d = rs.standard_normal((ntr, nlat, nlon))
+    lam = rs.standard_normal((ntr, nlat, nlon))
+    a = make(bg)
+    tl = a.tl_drive(0, sc[:4], d)
+    a.close()
+    a = make(bg)
+    ad = a.ad_drive(0, sc[:4], lam, ckpt_every=2)
+    a.close()
+    lhs, rhs = float(np.vdot(lam, tl)), float(np.vdot(ad, d))
+    result = {
+        "metric": "FFSL adjoint gradient cost / forward cost (FP64, 1 GPU)",
+        "value": t_grad / t_fwd, "unit": "x forward window", "n_gpus": 1, "steps": N, "warmup": N,
+        "ms_per_step": t_grad * 1e3 / N, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (cosine-bell and random tracers; y = truth run at the window's end, "
+                                "background from another seed)",
+        "config": {"workload": f"adjoint {args.config}: FFSL {nlon}x{nlat}, {ntr} tracers, dt={dt:g} s, "
+                               f"window {N} steps, checkpoint every {acfg['ckpt']} steps", "nlon": nlon,
+                   "nlat": nlat, "ntracers": ntr},
+        "gradient_ms": t_grad * 1e3, "forward_ms": t_fwd * 1e3, "cost": J,
+        "grad_norm": float(np.sqrt(np.sum(grad * grad))),
+        "dot_product_rel_err": abs(lhs - rhs) / max(abs(lhs), abs(rhs)),
+    }
+    print(json.dumps(result))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="sw025",
-                    choices=sorted(CONFIGS) + sorted(ADV_CONFIGS) + sorted(ENS_CONFIGS) + sorted(ADJ_CONFIGS))
+                    choices=sorted(CONFIGS) + sorted(ADV_CONFIGS) + sorted(ENS_CONFIGS) + sorted(ADJ_CONFIGS)
+                    + sorted(ADV_ADJ_CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -430,6 +510,9 @@ def main():
         return
     if args.config in ADJ_CONFIGS:
         run_adj(args)
+        return
+    if args.config in ADV_ADJ_CONFIGS:
+        run_adv_adj(args)
         return
     cfg = CONFIGS[args.config]
     from gmcore_amd.host import ShallowWater

@@ -132,6 +132,17 @@ class Advection:
                     "ad_drive")
         return b
 
+    def gradient(self, time_step0: int, scales, y, ckpt_every: int = 8):
+        """(J, dJ/dq_0) of J = 1/2 sum (q_N - y)^2 over the window drive()
+        would run; y has shape (ntracers, nlat, nlon)."""
+        s = np.ascontiguousarray(scales, dtype=np.float64)
+        yb = self._tlad_buf(y)
+        grad = np.empty_like(yb)
+        cost = C.c_double()
+        self._check(self.lib.gmc_adv_gradient(self.h, time_step0, dptr(s), len(s), ckpt_every, dptr(yb),
+                                              dptr(grad), C.byref(cost), self.nlon), "gradient")
+        return cost.value, grad
+
     @property
     def old(self) -> int:
         out = C.c_int32()

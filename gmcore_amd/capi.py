@@ -144,6 +144,8 @@ def load_hip() -> C.CDLL:
         "gmc_adv_tl_drive": (C.c_int, [vp, C.c_int64, P(C.c_double), C.c_int32, P(C.c_double), C.c_int64]),
         "gmc_adv_ad_drive": (C.c_int, [vp, C.c_int64, P(C.c_double), C.c_int32, C.c_int32, P(C.c_double),
                                        C.c_int64]),
+        "gmc_adv_gradient": (C.c_int, [vp, C.c_int64, P(C.c_double), C.c_int32, C.c_int32, P(C.c_double),
+                                       P(C.c_double), P(C.c_double), C.c_int64]),
         # batched ensembles, include/gmcore_ens.h
         "gmc_ens_create": (C.c_int, [P(vp), C.c_int32, P(vp)]),
         "gmc_ens_destroy": (C.c_int, [vp]),

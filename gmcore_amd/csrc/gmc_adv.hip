@@ -1409,6 +1409,23 @@ __global__ void __launch_bounds__(NA) a_adv_inner(AdvGrid g, Mesh M, int lim, TB
   }
 }
 
+// misfit of the window's end state: d = q - y in place of y (every tracer),
+// rows[t][r] = sum over the row of d^2; block (r, t)
+__global__ void __launch_bounds__(NA) k_adv_misfit(AdvGrid g, const double *q, double *d, int64_t ts,
+                                                   double *rows) {
+  __shared__ double red[NA / kWave];
+  const int r = blockIdx.x, t = blockIdx.y;
+  const int64_t row = t * ts + r * g.pitch;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < g.nlon; i += NA) {
+    const double e = q[row + i] - d[row + i];
+    d[row + i] = e;
+    acc += e * e;
+  }
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) rows[t * g.nlat + r] = s;
+}
+
 }  // namespace
 }  // namespace gmc
 
@@ -1548,10 +1565,18 @@ int gmc_adv_tl_drive(gmc_adv *a, int64_t time_step0, const double *scale, int32_
   return rc;
 }
 
-int gmc_adv_ad_drive(gmc_adv *a, int64_t time_step0, const double *scale, int32_t nsteps, int32_t ckpt_every,
-                     double *lam, int64_t ld) {
+}  // extern "C"
+
+namespace {
+
+// the adjoint of the window, forced at its end by lam (y == null) or by the
+// misfit q_N - y of J = 1/2 sum (q_N - y)^2 (then *cost = J); the result
+// goes to out
+int adv_adjoint(gmc_adv *a, int64_t time_step0, const double *scale, int32_t nsteps, int32_t ckpt_every,
+                const double *lam, const double *y, double *out, double *cost, int64_t ld) {
   if (!a) return GMC_EINVAL;
-  int rc = tl_check(a, scale, nsteps, time_step0, lam, ld);
+  int rc = tl_check(a, scale, nsteps, time_step0, y ? y : lam, ld);
+  if (!rc && !out) rc = aerr(a, GMC_EINVAL, "null output");
   if (!rc && ckpt_every < 1) rc = aerr(a, GMC_EINVAL, "ckpt_every must be at least 1");
   if (rc) return rc;
   const int K = std::min<int>(ckpt_every, std::max<int>(nsteps, 1));
@@ -1574,7 +1599,21 @@ int gmc_adv_ad_drive(gmc_adv *a, int64_t time_step0, const double *scale, int32_
   if (!rc) rc = check_flags(a);
   const size_t F = tl_fields(a);
   double *cur = a->tl_f, *nxt = a->tl_f + F;
-  if (!rc) rc = tl_copy(a, true, cur, lam, ld);
+  if (!rc) rc = tl_copy(a, true, cur, const_cast<double *>(y ? y : lam), ld);
+  if (!rc && y) {
+    const AdvGrid &g = a->g;
+    const int ntr = a->cfg.ntracers;
+    hipLaunchKernelGGL(k_adv_misfit, dim3(g.nlat, ntr), dim3(NA), 0, a->stream, g, tracer_ptr(a, a->old, 0), cur,
+                       (int64_t)g.nlat * g.pitch, a->d_rows);
+    std::vector<double> rows((size_t)ntr * g.nlat);
+    rc = ahip(a, hipGetLastError(), "misfit launch");
+    if (!rc) rc = ahip(a, hipMemcpyAsync(rows.data(), a->d_rows, rows.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                         a->stream), "misfit copy");
+    if (!rc) rc = ahip(a, hipStreamSynchronize(a->stream), "sync");
+    double j = 0.0;
+    for (double v : rows) j += v;
+    if (cost) *cost = 0.5 * j;
+  }
   std::vector<int> olds(K);
   for (int seg = nseg - 1; seg >= 0 && !rc; seg--) {
     host_restore(a, hck[seg]);
@@ -1593,9 +1632,23 @@ int gmc_adv_ad_drive(gmc_adv *a, int64_t time_step0, const double *scale, int32_
     host_restore(a, hck[nseg]);
     rc = slab_from(a, tl_slot(a, nseg));
   }
-  if (!rc) rc = tl_copy(a, false, cur, lam, ld);
+  if (!rc) rc = tl_copy(a, false, cur, out, ld);
   if (!rc) rc = ahip(a, hipStreamSynchronize(a->stream), "sync");
   return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gmc_adv_ad_drive(gmc_adv *a, int64_t time_step0, const double *scale, int32_t nsteps, int32_t ckpt_every,
+                     double *lam, int64_t ld) {
+  return adv_adjoint(a, time_step0, scale, nsteps, ckpt_every, lam, nullptr, lam, nullptr, ld);
+}
+
+int gmc_adv_gradient(gmc_adv *a, int64_t time_step0, const double *scale, int32_t nsteps, int32_t ckpt_every,
+                     const double *y, double *grad, double *cost, int64_t ld) {
+  return adv_adjoint(a, time_step0, scale, nsteps, ckpt_every, nullptr, y, grad, cost, ld);
 }
 
 }  // extern "C"

@@ -110,6 +110,14 @@ int gmc_adv_tl_drive(gmc_adv *adv, int64_t time_step0, const double *scale, int3
 int gmc_adv_ad_drive(gmc_adv *adv, int64_t time_step0, const double *scale, int32_t nsteps, int32_t ckpt_every,
                      double *lam, int64_t ld);
 
+/* Gradient of J = 1/2 sum over tracers and cells of (q_N - y)^2, q_N the
+ * old level at the window's end, with respect to the old level at entry:
+ * gmc_adv_ad_drive forced by q_N - y, formed on the device after the
+ * window's forward (no extra forward).  y, grad: [ntracers][nlat][ld];
+ * *cost = J (may be NULL). */
+int gmc_adv_gradient(gmc_adv *adv, int64_t time_step0, const double *scale, int32_t nsteps, int32_t ckpt_every,
+                     const double *y, double *grad, double *cost, int64_t ld);
+
 /* Index (1 or 2) of the old time level after the swaps. */
 int gmc_adv_old_level(const gmc_adv *adv, int32_t *old_level);
 

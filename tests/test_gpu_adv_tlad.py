@@ -151,3 +151,42 @@ def test_checkpoint_interval_independent():
     outs = [c.gpu().ad_drive(0, c.sc, c.lam, ckpt_every=k) for k in (1, 4, 100)]
     for o in outs[1:]:
         assert _rel(o, outs[0]) <= 1e-14
+
+
+@pytest.mark.parametrize("case", [(72, 37, 3600.0, 2, 20, "mono", "random", 4), BIG], ids=["72x37", "720x361-8tr"])
+def test_gradient_of_final_misfit(case):
+    """gmc_adv_gradient: J = 1/2 |q_N - y|^2 and dJ/dq_0.  J against the
+    forward's end state; the gradient against gmc_adv_ad_drive forced by
+    q_N - y, and (on the tie-free random base) against central differences
+    of J along a direction: J is quadratic on each linear piece of the step,
+    so the quotient is exact there up to rounding.  (The cosine bells' flat
+    background puts the limiters on ties, where the step has one-sided
+    derivatives only.)"""
+    c = Case(*case)
+    y = np.stack(init_cond.tracer_set(c.nlon, c.nlat, c.st.mesh.metrics, 9, c.ntr))
+    g = c.gpu()
+    J, grad = g.gradient(0, c.sc, y, ckpt_every=3)
+    g2 = c.gpu()
+    g2.drive(0, c.sc)
+    qN = c.final(g2)
+    assert np.array_equal(c.final(g), qN)
+    jref = 0.5 * float(np.sum((qN - y) ** 2))
+    assert abs(J - jref) <= 1e-13 * jref
+    ad = c.gpu().ad_drive(0, c.sc, qN - y, ckpt_every=2)
+    assert _rel(grad, ad) <= 1e-13, _rel(grad, ad)
+    if c.nlon > 72:
+        return
+    e = 1e-6
+
+    def cost(q0):
+        h = Advection(c.st)
+        for l in range(c.ntr):
+            h.set_tracer(1, l, q0[l])
+            h.set_tracer(2, l, c.q0[l])
+        h.set_wind_basis(c.ub, c.vb)
+        h.drive(0, c.sc)
+        return 0.5 * float(np.sum((c.final(h) - y) ** 2))
+
+    fd = (cost(c.q0 + e * c.d) - cost(c.q0 - e * c.d)) / (2 * e)
+    gd = float(np.vdot(grad, c.d))
+    assert abs(fd - gd) <= 1e-6 * abs(gd), (fd, gd)
