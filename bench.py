@@ -438,6 +438,16 @@ def run_adv_adj(args):
     J, grad = g.gradient(N, sc[N:], y, ckpt_every=acfg["ckpt"])
     t_grad = time.perf_counter() - w0
     g.close()
+    # the same gradient with the whole window taped during its forward
+    # (checkpoint interval = window: no recompute, N slab copies of memory)
+    g = make(bg)
+    g.gradient(0, sc[:N], y, ckpt_every=N)
+    g.synchronize()
+    w0 = time.perf_counter()
+    J2, grad2 = g.gradient(N, sc[N:], y, ckpt_every=N)
+    t_grad_full = time.perf_counter() - w0
+    g.close()
+    same = float(np.abs(grad2 - grad).max() / np.abs(grad).max())
     f = make(bg)
     f.drive(0, sc[:N])
     f.synchronize()
@@ -468,6 +478,8 @@ def run_adv_adj(args):
         "gradient_ms": t_grad * 1e3, "forward_ms": t_fwd * 1e3, "cost": J,
         "grad_norm": float(np.sqrt(np.sum(grad * grad))),
         "dot_product_rel_err": abs(lhs - rhs) / max(abs(lhs), abs(rhs)),
+        "gradient_ms_window_taped": t_grad_full * 1e3, "ratio_window_taped": t_grad_full / t_fwd,
+        "window_taped_grad_rel_diff": same,
     }
     print(json.dumps(result))
 

@@ -620,9 +620,10 @@ int copy_rows(gmc_adv *a, double *dst, const double *host, int rows, int64_t ld)
 }
 
 // adv_run: accumulate, maybe advance, copy_old_m (adv_mod.F90:152-312).
-// With tape != null an advancing step copies the whole slab there after its
-// flux kernel (the base state its tangent and adjoint read) and *taped_old
-// is the old level; 0 when the step does not advance.
+// With tape != null an advancing step leaves its base state (what its
+// tangent and adjoint read) in the tape slot, at the offsets the fields have
+// in the slab: qx, qy and qp are written there, the rest is copied after the
+// flux kernel; *taped_old is the old level, 0 when the step does not advance.
 int run_once(gmc_adv *a, int64_t time_step, bool basis, double scale, char *tape = nullptr,
              int *taped_old = nullptr) {
   if (taped_old) *taped_old = 0;
@@ -647,18 +648,25 @@ int run_once(gmc_adv *a, int64_t time_step, bool basis, double scale, char *tape
     const int64_t ts = (int64_t)g.nlat * g.pitch;
     const int ntr = a->cfg.ntracers;
     const dim3 grid(flux_blocks(g, ntr));
-    InnerPtrs ip{tracer_ptr(a, a->old, 0), a->qx, a->qy, a->u, a->v, a->cflx, a->cfly, a->divx, a->divy, ts};
+    // when taping, qx, qy and qp go straight to the tape slot's copies
+    auto T = [&](double *p) { return tape ? (double *)(tape + ((char *)p - (char *)a->slab)) : p; };
+    double *qx = T(a->qx), *qy = T(a->qy), *qp = T(a->qp);
+    InnerPtrs ip{tracer_ptr(a, a->old, 0), qx, qy, a->u, a->v, a->cflx, a->cfly, a->divx, a->divy, ts};
     hipLaunchKernelGGL(k_adv_inner, grid, dim3(NF), 0, a->stream, g, M, a->cfg.limiter, ntr, ip);
     mark(a, KA_INNER);
-    OuterPtrs op{a->qx, a->qy, tracer_ptr(a, a->old, 0), a->qp, a->mfx, a->mfy, a->cflx, a->cfly, a->old_m, a->sm, ts};
+    OuterPtrs op{qx, qy, tracer_ptr(a, a->old, 0), qp, a->mfx, a->mfy, a->cflx, a->cfly, a->old_m, a->sm, ts};
     hipLaunchKernelGGL(k_adv_flux, grid, dim3(NF), 0, a->stream, g, M, a->cfg.limiter, ntr, op);
     mark(a, KA_FLUX);
     if (tape) {
-      const int rc = ahip(a, hipMemcpyAsync(tape, a->slab, a->slab_bytes, hipMemcpyDeviceToDevice, a->stream), "tape");
+      // the rest of the step's base state: m, the winds, CFL numbers, split
+      // divergences and old_m, and q of both levels (the new level's seams)
+      const size_t off = (char *)a->sm - (char *)a->slab;
+      const size_t len = (char *)(a->q + 2 * ts * ntr) - (char *)a->sm;
+      const int rc = ahip(a, hipMemcpyAsync(tape + off, a->sm, len, hipMemcpyDeviceToDevice, a->stream), "tape");
       if (rc) return rc;
       if (taped_old) *taped_old = a->old;
     }
-    FixPtrs fp{a->qp, tracer_ptr(a, a->new_, 0), a->sm, a->d_wgt, ts, a->d_flags, a->d_fixes, a->cfg.lon_damp_passes};
+    FixPtrs fp{qp, tracer_ptr(a, a->new_, 0), a->sm, a->d_wgt, ts, a->d_flags, a->d_fixes, a->cfg.lon_damp_passes};
     hipLaunchKernelGGL(k_adv_fix, dim3(g.nlat, ntr), dim3(NA), a->lds_bytes, a->stream, g, fp);
     mark(a, KA_FIX);
     std::swap(a->old, a->new_);
@@ -1095,21 +1103,22 @@ __device__ __forceinline__ double yflux_tl(const AdvGrid &g, int lim, double c, 
   for (int t = 0; t < 5; t++) { dl += J.l[t] * dd[t]; dm += J.d[t] * dd[t]; d6 += J.s[t] * dd[t]; }
   return w.wv * (w.w1 * dl + w.w2 * dm + w.w3 * d6);
 }
-// adjoint of xflux_tl: lam * d flux / d d added into the row a
+// adjoint of xflux_tl: lam * d flux / d d added into the LDS row a (every
+// cell an x-flux reads lies in its own row)
 __device__ __forceinline__ void xflux_ad(const AdvGrid &g, int lim, double c, double u, int i, const double *q,
                                          double *a, double lam) {
   if (c == 0 || lam == 0) return;
   const int n = g.nlon;
   const XW w = x_weights(g, c, u, i);
   const double gw = w.wu * lam;
-  for (int k = 0; k < w.n; k++) unsafeAtomicAdd(&a[wrap(w.k0 + k, n)], gw);
+  for (int k = 0; k < w.n; k++) atomicAdd(&a[wrap(w.k0 + k, n)], gw);
   double f[5];
 #pragma unroll
   for (int t = 0; t < 5; t++) f[t] = q[wrap(w.iu - 2 + t, n)];
   const Jac J = ppm_jac(lim, f);
 #pragma unroll
   for (int t = 0; t < 5; t++)
-    unsafeAtomicAdd(&a[wrap(w.iu - 2 + t, n)], gw * (w.w1 * J.l[t] + w.w2 * J.d[t] + w.w3 * J.s[t]));
+    atomicAdd(&a[wrap(w.iu - 2 + t, n)], gw * (w.w1 * J.l[t] + w.w2 * J.d[t] + w.w3 * J.s[t]));
 }
 __device__ __forceinline__ void yflux_ad(const AdvGrid &g, int lim, double c, double v, int h, int i,
                                          const double *q, double *a, double lam) {
@@ -1312,7 +1321,7 @@ __global__ void __launch_bounds__(NA) t_adv_fix(AdvGrid g, TBase b, TWork w, con
 
 // adjoint of t_adv_fix and of the division by m: the adjoint of the new
 // level (in) -> dqp holds the adjoint of the update's numerator; clears the
-// rows of dqx, dqy the flux adjoints accumulate into
+// rows of dqx the y-flux adjoints accumulate into
 __global__ void __launch_bounds__(NA) a_adv_fix(AdvGrid g, TBase b, TWork w, const double *wgt, int passes) {
   extern __shared__ double lds[];
   const int n = g.nlon, r = blockIdx.x;
@@ -1326,7 +1335,6 @@ __global__ void __launch_bounds__(NA) a_adv_fix(AdvGrid g, TBase b, TWork w, con
     neg[i] = sb[i] < 0 ? 1.0 : 0.0;
     code[i] = 0.0;
     w.dqx[row + i] = 0.0;
-    w.dqy[row + i] = 0.0;
   }
   __syncthreads();
   const double wr = (r > 0 && r < g.nlat - 1) ? wgt[r] : 0.0;
@@ -1356,21 +1364,26 @@ __global__ void __launch_bounds__(NA) a_adv_fix(AdvGrid g, TBase b, TWork w, con
   for (int i = threadIdx.x; i < n; i += NA) w.dqp[row + i] = sl[i] / mrow[i];
 }
 
-// adjoint of the update and the outer fluxes: dqp (numerator adjoint) ->
-// out = old_m * dqp (the direct term), and the flux adjoints added into
-// dqy (x-fluxes of qy) and dqx (y-fluxes of qx)
+// adjoint of the update and the outer fluxes, block (row r, tracer t): dqp
+// (numerator adjoint) -> out = old_m * dqp (the direct term); the x-flux
+// adjoints gathered in LDS and stored as row r of dqy (nothing else adds
+// there), the y-flux adjoints added into dqx with atomics
 __global__ void __launch_bounds__(NA) a_adv_outer(AdvGrid g, Mesh M, int lim, TBase b, TWork w) {
-  const int i = blockIdx.x * NA + threadIdx.x, r = blockIdx.y, t = blockIdx.z;
-  if (i >= g.nlon) return;
-  const int64_t to = t * w.ts, P = g.pitch, k = r * P + i;
+  extern __shared__ double acc[];
+  const int n = g.nlon, r = blockIdx.x, t = blockIdx.y;
+  const int64_t to = t * w.ts, P = g.pitch;
   const double dt = g.dt;
   const double *an = w.dqp + to;
-  w.out[to + k] = b.old_m[k] * an[k];
-  if (r >= 1 && r <= g.nlat - 2) {
-    const double lf = (an[r * P + wrap(i + 1, g.nlon)] - an[k]) * M(GMC_LE_LON, r) / M(GMC_AREA_CELL, r) * dt;
-    xflux_ad(g, lim, b.cflx[k], b.mfx[k], i, b.qy + to + r * P, w.dqy + to + r * P, lf);
-  }
-  if (r <= g.nlat - 2) {
+  for (int i = threadIdx.x; i < n; i += NA) acc[i] = 0.0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += NA) {
+    const int64_t k = r * P + i;
+    w.out[to + k] = b.old_m[k] * an[k];
+    if (r >= 1 && r <= g.nlat - 2) {
+      const double lf = (an[r * P + wrap(i + 1, n)] - an[k]) * M(GMC_LE_LON, r) / M(GMC_AREA_CELL, r) * dt;
+      xflux_ad(g, lim, b.cflx[k], b.mfx[k], i, b.qy + to + r * P, acc, lf);
+    }
+    if (r > g.nlat - 2) continue;
     const int h = r;
     double lf = 0.0;
     if (h >= 1) lf -= M(GMC_LE_LAT, h) / M(GMC_AREA_CELL, h) * dt * an[h * P + i];
@@ -1379,26 +1392,34 @@ __global__ void __launch_bounds__(NA) a_adv_outer(AdvGrid g, Mesh M, int lim, TB
     if (h == g.nlat - 2) lf += M(GMC_LE_LAT, h) / g.nlon / M(GMC_AREA_CELL, g.nlat - 1) * dt * w.ps[2 * t + 1];
     yflux_ad(g, lim, b.cfly[h * P + i], b.mfy[h * P + i], h, i, b.qx + to, w.dqx + to, lf);
   }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += NA) w.dqy[to + r * P + i] = acc[i];
 }
 
-// adjoint of qx, qy and the inner fluxes, added into out
+// adjoint of qx, qy and the inner fluxes, block (row r, tracer t): the
+// direct terms and the x-flux adjoints gathered in LDS and added into row r
+// of out once per cell, the y-flux adjoints added into out with atomics
 __global__ void __launch_bounds__(NA) a_adv_inner(AdvGrid g, Mesh M, int lim, TBase b, TWork w) {
-  const int i = blockIdx.x * NA + threadIdx.x, r = blockIdx.y, t = blockIdx.z;
-  if (i >= g.nlon) return;
-  const int64_t to = t * w.ts, P = g.pitch, k = r * P + i;
+  extern __shared__ double acc[];
+  const int n = g.nlon, r = blockIdx.x, t = blockIdx.y;
+  const int64_t to = t * w.ts, P = g.pitch;
   const double dt = g.dt;
   const double *ax = w.dqx + to, *ay = w.dqy + to;
-  const double gx = ax[k], gy = ay[k];
-  double add;
-  if (r == 0) add = gx + (1 + 0.5 * b.divy[k] * dt) * gy;
-  else if (r == g.nlat - 1) add = gx + (1 - 0.5 * b.divy[k] * dt) * gy;
-  else add = (1 + 0.5 * b.divx[k] * dt) * gx + (1 + 0.5 * b.divy[k] * dt) * gy;
-  unsafeAtomicAdd(&w.out[to + k], add);
-  if (r >= 1 && r <= g.nlat - 2) {
-    const double lf = 0.5 * dt * M(GMC_LE_LON, r) / M(GMC_AREA_CELL, r) * (ax[r * P + wrap(i + 1, g.nlon)] - gx);
-    xflux_ad(g, lim, b.cflx[k], b.u[k], i, b.q + to + r * P, w.out + to + r * P, lf);
-  }
-  if (r <= g.nlat - 2) {
+  for (int i = threadIdx.x; i < n; i += NA) acc[i] = 0.0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += NA) {
+    const int64_t k = r * P + i;
+    const double gx = ax[k], gy = ay[k];
+    double add;
+    if (r == 0) add = gx + (1 + 0.5 * b.divy[k] * dt) * gy;
+    else if (r == g.nlat - 1) add = gx + (1 - 0.5 * b.divy[k] * dt) * gy;
+    else add = (1 + 0.5 * b.divx[k] * dt) * gx + (1 + 0.5 * b.divy[k] * dt) * gy;
+    atomicAdd(&acc[i], add);
+    if (r >= 1 && r <= g.nlat - 2) {
+      const double lf = 0.5 * dt * M(GMC_LE_LON, r) / M(GMC_AREA_CELL, r) * (ax[r * P + wrap(i + 1, n)] - gx);
+      xflux_ad(g, lim, b.cflx[k], b.u[k], i, b.q + to + r * P, acc, lf);
+    }
+    if (r > g.nlat - 2) continue;
     const int h = r;
     double lf = 0.0;
     if (h >= 1) lf -= 0.5 * dt * M(GMC_LE_LAT, h) / M(GMC_AREA_CELL, h) * ay[h * P + i];
@@ -1407,6 +1428,8 @@ __global__ void __launch_bounds__(NA) a_adv_inner(AdvGrid g, Mesh M, int lim, TB
     if (h == g.nlat - 2) lf += 0.5 * dt * M(GMC_LE_LAT, h) / g.nlon / M(GMC_AREA_CELL, g.nlat - 1) * w.ps[2 * t + 1];
     yflux_ad(g, lim, b.cfly[h * P + i], b.v[h * P + i], h, i, b.q + to, w.out + to, lf);
   }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += NA) unsafeAtomicAdd(&w.out[to + r * P + i], acc[i]);
 }
 
 // misfit of the window's end state: d = q - y in place of y (every tracer),
@@ -1513,13 +1536,14 @@ void ad_step(gmc_adv *a, const char *slot, int old, const double *in, double *ou
   double *dqx = a->tl_f + 2 * F, *dqy = a->tl_f + 3 * F, *dqp = a->tl_f + 4 * F, *ps = a->tl_f + 5 * F;
   double *ps2 = ps + 2 * ntr;
   const TBase b = tape_base(a, slot, old);
-  const dim3 cells((g.nlon + NA - 1) / NA, g.nlat, ntr), poles(2, ntr);
+  const dim3 rows(g.nlat, ntr), poles(2, ntr);
   hipLaunchKernelGGL(a_adv_fix, dim3(g.nlat, ntr), dim3(NA), 5 * (size_t)g.nlon * sizeof(double), a->stream, g, b,
                      TWork{in, dqx, dqy, dqp, out, ps, ts}, a->d_wgt, a->cfg.lon_damp_passes);
   hipLaunchKernelGGL(k_adv_psum2, poles, dim3(NA), 0, a->stream, g, dqp, ts, ps);
-  hipLaunchKernelGGL(a_adv_outer, cells, dim3(NA), 0, a->stream, g, M, lim, b, TWork{in, dqx, dqy, dqp, out, ps, ts});
+  const size_t lrow = (size_t)g.nlon * sizeof(double);
+  hipLaunchKernelGGL(a_adv_outer, rows, dim3(NA), lrow, a->stream, g, M, lim, b, TWork{in, dqx, dqy, dqp, out, ps, ts});
   hipLaunchKernelGGL(k_adv_psum2, poles, dim3(NA), 0, a->stream, g, dqy, ts, ps2);
-  hipLaunchKernelGGL(a_adv_inner, cells, dim3(NA), 0, a->stream, g, M, lim, b, TWork{in, dqx, dqy, dqp, out, ps2, ts});
+  hipLaunchKernelGGL(a_adv_inner, rows, dim3(NA), lrow, a->stream, g, M, lim, b, TWork{in, dqx, dqy, dqp, out, ps2, ts});
 }
 
 int tl_check(gmc_adv *a, const double *scale, int32_t nsteps, int64_t time_step0, const double *buf, int64_t ld) {
@@ -1585,12 +1609,17 @@ int adv_adjoint(gmc_adv *a, int64_t time_step0, const double *scale, int32_t nst
   rc = tl_alloc(a, nseg + 1 + K);
   if (rc) return rc;
   std::vector<HostCk> hck(nseg + 1);
+  std::vector<int> olds(K);
+  const int slast = (nseg - 1) * K;  // the last segment is taped on the way
   for (int s = 0; s < nsteps && !rc; s++) {
     if (s % K == 0) {
       hck[s / K] = host_ck(a);
       rc = slab_to(a, tl_slot(a, s / K));
     }
-    if (!rc) rc = run_once(a, time_step0 + s, true, scale[s]);
+    if (!rc && s >= slast)
+      rc = run_once(a, time_step0 + s, true, scale[s], tl_slot(a, nseg + 1 + (s - slast)), &olds[s - slast]);
+    else if (!rc)
+      rc = run_once(a, time_step0 + s, true, scale[s]);
   }
   if (!rc) {
     hck[nseg] = host_ck(a);
@@ -1614,13 +1643,14 @@ int adv_adjoint(gmc_adv *a, int64_t time_step0, const double *scale, int32_t nst
     for (double v : rows) j += v;
     if (cost) *cost = 0.5 * j;
   }
-  std::vector<int> olds(K);
   for (int seg = nseg - 1; seg >= 0 && !rc; seg--) {
-    host_restore(a, hck[seg]);
-    rc = slab_from(a, tl_slot(a, seg));
     const int s0 = seg * K, s1 = std::min(nsteps, s0 + K);
-    for (int s = s0; s < s1 && !rc; s++)
-      rc = run_once(a, time_step0 + s, true, scale[s], tl_slot(a, nseg + 1 + (s - s0)), &olds[s - s0]);
+    if (seg < nseg - 1) {
+      host_restore(a, hck[seg]);
+      rc = slab_from(a, tl_slot(a, seg));
+      for (int s = s0; s < s1 && !rc; s++)
+        rc = run_once(a, time_step0 + s, true, scale[s], tl_slot(a, nseg + 1 + (s - s0)), &olds[s - s0]);
+    }
     for (int s = s1 - 1; s >= s0 && !rc; s--) {
       if (!olds[s - s0]) continue;
       ad_step(a, tl_slot(a, nseg + 1 + (s - s0)), olds[s - s0], cur, nxt);
