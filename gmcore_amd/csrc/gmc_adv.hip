@@ -589,6 +589,10 @@ struct gmc_adv {
   double *tl_f = nullptr;
   char *tl_ws = nullptr;
   size_t tl_ws_bytes = 0;
+  // the adjoint's stream (default priority) and the events that order it
+  // against the segment recomputes on the object's stream
+  hipStream_t astream = nullptr;
+  hipEvent_t ev_fwd = nullptr, ev_join = nullptr, ev_ready[2] = {}, ev_done[2] = {};
   std::string err;
 };
 
@@ -768,6 +772,10 @@ int gmc_adv_destroy(gmc_adv *a) {
   if (a->slab) (void)hipFree(a->slab);
   if (a->tl_f) (void)hipFree(a->tl_f);
   if (a->tl_ws) (void)hipFree(a->tl_ws);
+  if (a->astream) (void)hipStreamSynchronize(a->astream);
+  for (hipEvent_t e : {a->ev_fwd, a->ev_join, a->ev_ready[0], a->ev_ready[1], a->ev_done[0], a->ev_done[1]})
+    if (e) (void)hipEventDestroy(e);
+  if (a->astream) (void)hipStreamDestroy(a->astream);
   if (a->h_flags) (void)hipHostFree(a->h_flags);
   if (a->stream) (void)hipStreamDestroy(a->stream);
   stage_free(a->stage);
@@ -1498,6 +1506,19 @@ int slab_to(gmc_adv *a, char *dst) {
 int slab_from(gmc_adv *a, const char *src) {
   return ahip(a, hipMemcpyAsync(a->slab, src, a->slab_bytes, hipMemcpyDeviceToDevice, a->stream), "restore");
 }
+// the fields only (not the mesh, weights and counters after them, which a
+// concurrent adjoint reads)
+int fields_from(gmc_adv *a, const char *src) {
+  const size_t len = (char *)a->d_mesh - (char *)a->slab;
+  return ahip(a, hipMemcpyAsync(a->slab, src, len, hipMemcpyDeviceToDevice, a->stream), "restore");
+}
+int adj_stream(gmc_adv *a) {
+  if (a->astream) return GMC_OK;
+  int rc = ahip(a, hipStreamCreateWithFlags(&a->astream, hipStreamNonBlocking), "adjoint stream");
+  for (hipEvent_t *e : {&a->ev_fwd, &a->ev_join, &a->ev_ready[0], &a->ev_ready[1], &a->ev_done[0], &a->ev_done[1]})
+    if (!rc) rc = ahip(a, hipEventCreateWithFlags(e, hipEventDisableTiming), "event");
+  return rc;
+}
 
 TBase tape_base(gmc_adv *a, const char *slot, int old) {
   auto S = [&](const double *p) { return (const double *)(slot + ((const char *)p - (const char *)a->slab)); };
@@ -1525,9 +1546,9 @@ void tl_step(gmc_adv *a, const char *slot, int old, const double *in, double *ou
                      TWork{in, dqx, dqy, dqp, out, ps, ts}, a->d_wgt, a->cfg.lon_damp_passes);
 }
 
-// adjoint of one taped step: in (adjoint of the new level) -> out (of the
-// old level)
-void ad_step(gmc_adv *a, const char *slot, int old, const double *in, double *out) {
+// adjoint of one taped step on stream st: in (adjoint of the new level) ->
+// out (of the old level)
+void ad_step(gmc_adv *a, const char *slot, int old, const double *in, double *out, hipStream_t st) {
   const AdvGrid &g = a->g;
   const Mesh M{a->d_mesh, g.nlat};
   const int ntr = a->cfg.ntracers, lim = a->cfg.limiter;
@@ -1537,13 +1558,13 @@ void ad_step(gmc_adv *a, const char *slot, int old, const double *in, double *ou
   double *ps2 = ps + 2 * ntr;
   const TBase b = tape_base(a, slot, old);
   const dim3 rows(g.nlat, ntr), poles(2, ntr);
-  hipLaunchKernelGGL(a_adv_fix, dim3(g.nlat, ntr), dim3(NA), 5 * (size_t)g.nlon * sizeof(double), a->stream, g, b,
+  hipLaunchKernelGGL(a_adv_fix, dim3(g.nlat, ntr), dim3(NA), 5 * (size_t)g.nlon * sizeof(double), st, g, b,
                      TWork{in, dqx, dqy, dqp, out, ps, ts}, a->d_wgt, a->cfg.lon_damp_passes);
-  hipLaunchKernelGGL(k_adv_psum2, poles, dim3(NA), 0, a->stream, g, dqp, ts, ps);
+  hipLaunchKernelGGL(k_adv_psum2, poles, dim3(NA), 0, st, g, dqp, ts, ps);
   const size_t lrow = (size_t)g.nlon * sizeof(double);
-  hipLaunchKernelGGL(a_adv_outer, rows, dim3(NA), lrow, a->stream, g, M, lim, b, TWork{in, dqx, dqy, dqp, out, ps, ts});
-  hipLaunchKernelGGL(k_adv_psum2, poles, dim3(NA), 0, a->stream, g, dqy, ts, ps2);
-  hipLaunchKernelGGL(a_adv_inner, rows, dim3(NA), lrow, a->stream, g, M, lim, b, TWork{in, dqx, dqy, dqp, out, ps2, ts});
+  hipLaunchKernelGGL(a_adv_outer, rows, dim3(NA), lrow, st, g, M, lim, b, TWork{in, dqx, dqy, dqp, out, ps, ts});
+  hipLaunchKernelGGL(k_adv_psum2, poles, dim3(NA), 0, st, g, dqy, ts, ps2);
+  hipLaunchKernelGGL(a_adv_inner, rows, dim3(NA), lrow, st, g, M, lim, b, TWork{in, dqx, dqy, dqp, out, ps2, ts});
 }
 
 int tl_check(gmc_adv *a, const double *scale, int32_t nsteps, int64_t time_step0, const double *buf, int64_t ld) {
@@ -1605,11 +1626,14 @@ int adv_adjoint(gmc_adv *a, int64_t time_step0, const double *scale, int32_t nst
   if (rc) return rc;
   const int K = std::min<int>(ckpt_every, std::max<int>(nsteps, 1));
   const int nseg = (nsteps + K - 1) / K;
-  // slots: checkpoints 0 .. nseg (the last is the window's end), then K tapes
-  rc = tl_alloc(a, nseg + 1 + K);
+  // slots: checkpoints 0 .. nseg (the last is the window's end), then two
+  // tape buffers of K slots; segment seg is taped into buffer seg % 2
+  rc = tl_alloc(a, nseg + 1 + 2 * K);
+  if (!rc) rc = adj_stream(a);
   if (rc) return rc;
+  auto tape = [&](int seg, int k) { return tl_slot(a, nseg + 1 + (seg % 2) * K + k); };
   std::vector<HostCk> hck(nseg + 1);
-  std::vector<int> olds(K);
+  std::vector<int> olds[2] = {std::vector<int>(K), std::vector<int>(K)};
   const int slast = (nseg - 1) * K;  // the last segment is taped on the way
   for (int s = 0; s < nsteps && !rc; s++) {
     if (s % K == 0) {
@@ -1617,7 +1641,7 @@ int adv_adjoint(gmc_adv *a, int64_t time_step0, const double *scale, int32_t nst
       rc = slab_to(a, tl_slot(a, s / K));
     }
     if (!rc && s >= slast)
-      rc = run_once(a, time_step0 + s, true, scale[s], tl_slot(a, nseg + 1 + (s - slast)), &olds[s - slast]);
+      rc = run_once(a, time_step0 + s, true, scale[s], tape(nseg - 1, s - slast), &olds[(nseg - 1) % 2][s - slast]);
     else if (!rc)
       rc = run_once(a, time_step0 + s, true, scale[s]);
   }
@@ -1643,27 +1667,42 @@ int adv_adjoint(gmc_adv *a, int64_t time_step0, const double *scale, int32_t nst
     for (double v : rows) j += v;
     if (cost) *cost = 0.5 * j;
   }
+  // Last segment first: the adjoint of segment seg runs on the adjoint
+  // stream from tape buffer seg % 2 while the object's stream recomputes
+  // segment seg - 1 from its checkpoint into the other buffer, once the
+  // adjoint of segment seg + 1 has finished reading it.
+  if (!rc) rc = ahip(a, hipEventRecord(a->ev_fwd, a->stream), "event");
+  if (!rc) rc = ahip(a, hipStreamWaitEvent(a->astream, a->ev_fwd, 0), "wait");
   for (int seg = nseg - 1; seg >= 0 && !rc; seg--) {
-    const int s0 = seg * K, s1 = std::min(nsteps, s0 + K);
-    if (seg < nseg - 1) {
-      host_restore(a, hck[seg]);
-      rc = slab_from(a, tl_slot(a, seg));
-      for (int s = s0; s < s1 && !rc; s++)
-        rc = run_once(a, time_step0 + s, true, scale[s], tl_slot(a, nseg + 1 + (s - s0)), &olds[s - s0]);
+    const int b = seg % 2;
+    if (seg > 0) {
+      const int p0 = (seg - 1) * K, p1 = seg * K;
+      if (seg + 1 <= nseg - 1) rc = ahip(a, hipStreamWaitEvent(a->stream, a->ev_done[1 - b], 0), "wait");
+      host_restore(a, hck[seg - 1]);
+      if (!rc) rc = fields_from(a, tl_slot(a, seg - 1));
+      for (int s = p0; s < p1 && !rc; s++)
+        rc = run_once(a, time_step0 + s, true, scale[s], tape(seg - 1, s - p0), &olds[1 - b][s - p0]);
+      if (!rc) rc = ahip(a, hipEventRecord(a->ev_ready[1 - b], a->stream), "event");
     }
+    if (!rc && seg < nseg - 1) rc = ahip(a, hipStreamWaitEvent(a->astream, a->ev_ready[b], 0), "wait");
+    const int s0 = seg * K, s1 = std::min(nsteps, s0 + K);
     for (int s = s1 - 1; s >= s0 && !rc; s--) {
-      if (!olds[s - s0]) continue;
-      ad_step(a, tl_slot(a, nseg + 1 + (s - s0)), olds[s - s0], cur, nxt);
+      if (!olds[b][s - s0]) continue;
+      ad_step(a, tape(seg, s - s0), olds[b][s - s0], cur, nxt, a->astream);
       std::swap(cur, nxt);
       rc = ahip(a, hipGetLastError(), "adjoint launch");
     }
+    if (!rc) rc = ahip(a, hipEventRecord(a->ev_done[b], a->astream), "event");
   }
+  if (!rc) rc = ahip(a, hipEventRecord(a->ev_join, a->astream), "event");
+  if (!rc) rc = ahip(a, hipStreamWaitEvent(a->stream, a->ev_join, 0), "wait");
   if (!rc) {
     host_restore(a, hck[nseg]);
     rc = slab_from(a, tl_slot(a, nseg));
   }
   if (!rc) rc = tl_copy(a, false, cur, out, ld);
   if (!rc) rc = ahip(a, hipStreamSynchronize(a->stream), "sync");
+  if (rc) (void)hipStreamSynchronize(a->astream);  // nothing left running on an error
   return rc;
 }
 
