@@ -955,8 +955,8 @@ const char *gmc_adv_kernel_name(int32_t k) { return k >= 0 && k < KA_NCLASS ? kA
 // of qp), t_adv_fix (the fixer on the base's branches, then the damping).
 // Adjoint step: a_adv_fix (damping, the fixer transposed, / m), k_adv_psum2
 // (pole sums), a_adv_outer (outer fluxes transposed, added into the adjoints
-// of qy and qx with atomics), k_adv_psum2, a_adv_inner (inner fluxes
-// transposed, added into the adjoint of q).  The adjoint's sums therefore run
+// of qy and qx with atomics), a_adv_inner (inner fluxes transposed, added
+// into the adjoint of q).  The adjoint's sums therefore run
 // in no fixed order: it is reproducible to rounding, not bit for bit.
 namespace gmc {
 namespace {
@@ -1144,6 +1144,9 @@ __device__ __forceinline__ void yflux_ad(const AdvGrid &g, int lim, double c, do
 #pragma unroll
   for (int t = 0; t < 5; t++) unsafeAtomicAdd(&a[k[t]], gw * (w.w1 * J.l[t] + w.w2 * J.d[t] + w.w3 * J.s[t]));
 }
+
+// threads of the row-block flux adjoints
+constexpr int NAD = 512;
 
 // base state of one taped step (tracer 0 of every per-tracer field)
 struct TBase {
@@ -1376,15 +1379,15 @@ __global__ void __launch_bounds__(NA) a_adv_fix(AdvGrid g, TBase b, TWork w, con
 // (numerator adjoint) -> out = old_m * dqp (the direct term); the x-flux
 // adjoints gathered in LDS and stored as row r of dqy (nothing else adds
 // there), the y-flux adjoints added into dqx with atomics
-__global__ void __launch_bounds__(NA) a_adv_outer(AdvGrid g, Mesh M, int lim, TBase b, TWork w) {
+__global__ void __launch_bounds__(NAD) a_adv_outer(AdvGrid g, Mesh M, int lim, TBase b, TWork w) {
   extern __shared__ double acc[];
   const int n = g.nlon, r = blockIdx.x, t = blockIdx.y;
   const int64_t to = t * w.ts, P = g.pitch;
   const double dt = g.dt;
   const double *an = w.dqp + to;
-  for (int i = threadIdx.x; i < n; i += NA) acc[i] = 0.0;
+  for (int i = threadIdx.x; i < n; i += NAD) acc[i] = 0.0;
   __syncthreads();
-  for (int i = threadIdx.x; i < n; i += NA) {
+  for (int i = threadIdx.x; i < n; i += NAD) {
     const int64_t k = r * P + i;
     w.out[to + k] = b.old_m[k] * an[k];
     if (r >= 1 && r <= g.nlat - 2) {
@@ -1401,21 +1404,22 @@ __global__ void __launch_bounds__(NA) a_adv_outer(AdvGrid g, Mesh M, int lim, TB
     yflux_ad(g, lim, b.cfly[h * P + i], b.mfy[h * P + i], h, i, b.qx + to, w.dqx + to, lf);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < n; i += NA) w.dqy[to + r * P + i] = acc[i];
+  for (int i = threadIdx.x; i < n; i += NAD) w.dqy[to + r * P + i] = acc[i];
 }
 
 // adjoint of qx, qy and the inner fluxes, block (row r, tracer t): the
 // direct terms and the x-flux adjoints gathered in LDS and added into row r
 // of out once per cell, the y-flux adjoints added into out with atomics
-__global__ void __launch_bounds__(NA) a_adv_inner(AdvGrid g, Mesh M, int lim, TBase b, TWork w) {
+// (the pole sums in w.ps are not read)
+__global__ void __launch_bounds__(NAD) a_adv_inner(AdvGrid g, Mesh M, int lim, TBase b, TWork w) {
   extern __shared__ double acc[];
   const int n = g.nlon, r = blockIdx.x, t = blockIdx.y;
   const int64_t to = t * w.ts, P = g.pitch;
   const double dt = g.dt;
   const double *ax = w.dqx + to, *ay = w.dqy + to;
-  for (int i = threadIdx.x; i < n; i += NA) acc[i] = 0.0;
+  for (int i = threadIdx.x; i < n; i += NAD) acc[i] = 0.0;
   __syncthreads();
-  for (int i = threadIdx.x; i < n; i += NA) {
+  for (int i = threadIdx.x; i < n; i += NAD) {
     const int64_t k = r * P + i;
     const double gx = ax[k], gy = ay[k];
     double add;
@@ -1432,12 +1436,13 @@ __global__ void __launch_bounds__(NA) a_adv_inner(AdvGrid g, Mesh M, int lim, TB
     double lf = 0.0;
     if (h >= 1) lf -= 0.5 * dt * M(GMC_LE_LAT, h) / M(GMC_AREA_CELL, h) * ay[h * P + i];
     if (h + 1 <= g.nlat - 2) lf += 0.5 * dt * M(GMC_LE_LAT, h) / M(GMC_AREA_CELL, h + 1) * ay[(h + 1) * P + i];
-    if (h == 0) lf -= 0.5 * dt * M(GMC_LE_LAT, 0) / g.nlon / M(GMC_AREA_CELL, 0) * w.ps[2 * t];
-    if (h == g.nlat - 2) lf += 0.5 * dt * M(GMC_LE_LAT, h) / g.nlon / M(GMC_AREA_CELL, g.nlat - 1) * w.ps[2 * t + 1];
+    // the pole rows' qy feed no outer flux (the outer x-fluxes skip the
+    // pole rows), so their adjoint, and the pole-sum terms it would bring
+    // into the V rows next to the poles, are zero
     yflux_ad(g, lim, b.cfly[h * P + i], b.v[h * P + i], h, i, b.q + to, w.out + to, lf);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < n; i += NA) unsafeAtomicAdd(&w.out[to + r * P + i], acc[i]);
+  for (int i = threadIdx.x; i < n; i += NAD) unsafeAtomicAdd(&w.out[to + r * P + i], acc[i]);
 }
 
 // misfit of the window's end state: d = q - y in place of y (every tracer),
@@ -1555,16 +1560,14 @@ void ad_step(gmc_adv *a, const char *slot, int old, const double *in, double *ou
   const int64_t ts = (int64_t)g.nlat * g.pitch;
   const size_t F = tl_fields(a);
   double *dqx = a->tl_f + 2 * F, *dqy = a->tl_f + 3 * F, *dqp = a->tl_f + 4 * F, *ps = a->tl_f + 5 * F;
-  double *ps2 = ps + 2 * ntr;
   const TBase b = tape_base(a, slot, old);
   const dim3 rows(g.nlat, ntr), poles(2, ntr);
   hipLaunchKernelGGL(a_adv_fix, dim3(g.nlat, ntr), dim3(NA), 5 * (size_t)g.nlon * sizeof(double), st, g, b,
                      TWork{in, dqx, dqy, dqp, out, ps, ts}, a->d_wgt, a->cfg.lon_damp_passes);
   hipLaunchKernelGGL(k_adv_psum2, poles, dim3(NA), 0, st, g, dqp, ts, ps);
   const size_t lrow = (size_t)g.nlon * sizeof(double);
-  hipLaunchKernelGGL(a_adv_outer, rows, dim3(NA), lrow, st, g, M, lim, b, TWork{in, dqx, dqy, dqp, out, ps, ts});
-  hipLaunchKernelGGL(k_adv_psum2, poles, dim3(NA), 0, st, g, dqy, ts, ps2);
-  hipLaunchKernelGGL(a_adv_inner, rows, dim3(NA), lrow, st, g, M, lim, b, TWork{in, dqx, dqy, dqp, out, ps2, ts});
+  hipLaunchKernelGGL(a_adv_outer, rows, dim3(NAD), lrow, st, g, M, lim, b, TWork{in, dqx, dqy, dqp, out, ps, ts});
+  hipLaunchKernelGGL(a_adv_inner, rows, dim3(NAD), lrow, st, g, M, lim, b, TWork{in, dqx, dqy, dqp, out, ps, ts});
 }
 
 int tl_check(gmc_adv *a, const double *scale, int32_t nsteps, int64_t time_step0, const double *buf, int64_t ld) {
