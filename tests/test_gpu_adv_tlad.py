@@ -19,6 +19,7 @@ import pytest
 import oracle
 from gmcore_amd import init_cond
 from gmcore_amd.adv import Advection, AdvSetup
+from gmcore_amd.host import GmcError
 from gmcore_amd.namelist import Namelist
 
 pytestmark = pytest.mark.gpu
@@ -190,3 +191,51 @@ def test_gradient_of_final_misfit(case):
     fd = (cost(c.q0 + e * c.d) - cost(c.q0 - e * c.d)) / (2 * e)
     gd = float(np.vdot(grad, c.d))
     assert abs(fd - gd) <= 1e-6 * abs(gd), (fd, gd)
+
+
+def test_window_without_advancing_step_and_bad_arguments():
+    """Time step 0 only accumulates the wind (adv_mod.F90:166): a one-step
+    window from step 0, and an empty window, leave the tangent and the
+    adjoint as they are.  ckpt_every < 1 and a wrongly shaped array are
+    rejected."""
+    c = Case(*SMALL["mono-smooth"])
+    assert np.array_equal(c.gpu().tl_drive(0, c.sc[:1], c.d), c.d)
+    assert np.array_equal(c.gpu().ad_drive(0, c.sc[:1], c.lam, ckpt_every=4), c.lam)
+    assert np.array_equal(c.gpu().ad_drive(0, np.zeros(0), c.lam, ckpt_every=4), c.lam)
+    with pytest.raises(GmcError):
+        c.gpu().ad_drive(0, c.sc, c.lam, ckpt_every=0)
+    with pytest.raises(ValueError):
+        c.gpu().tl_drive(0, c.sc, c.d[:, :-1])
+
+
+def test_second_window_matches_oracle():
+    """A window that starts after 3 plain steps, mid-way through the wind
+    accumulation's cadence, against the oracle's tangent and adjoint of the
+    same steps."""
+    c = Case(72, 37, 3600.0, 2, 20, "mono", "random", 8)
+    n0, n1 = 3, 8
+    o = c.oracle()
+    for n in range(n0):
+        c.owind(o, n)
+        assert o.run(n) == 0
+    ot, d = o.clone(), np.array(c.d)
+    for n in range(n0, n1):
+        c.owind(ot, n)
+        assert ot.tlad_run(n, dq=d) == 0
+    states, oa = [], o.clone()
+    for n in range(n0, n1):
+        states.append(oa.clone())
+        c.owind(oa, n)
+        assert oa.run(n) == 0
+    lam = np.array(c.lam)
+    for k in reversed(range(n1 - n0)):
+        c.owind(states[k], n0 + k)
+        assert states[k].tlad_run(n0 + k, lam=lam) == 0
+    g = c.gpu()
+    g.drive(0, c.sc[:n0])
+    tl = g.tl_drive(n0, c.sc[n0:n1], c.d)
+    assert _rel(tl, d) <= 1e-12, _rel(tl, d)
+    g = c.gpu()
+    g.drive(0, c.sc[:n0])
+    ad = g.ad_drive(n0, c.sc[n0:n1], c.lam, ckpt_every=2)
+    assert _rel(ad, lam) <= 1e-12, _rel(ad, lam)
