@@ -26,6 +26,7 @@
 // half-circle shift (the reference's fill_halo on one rank).  Every formula
 // keeps the reference's operation order; the build uses -ffp-contract=off.
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -628,8 +629,11 @@ int copy_rows(gmc_adv *a, double *dst, const double *host, int rows, int64_t ld)
 // tangent and adjoint read) in the tape slot, at the offsets the fields have
 // in the slab: qx, qy and qp are written there, the rest is copied after the
 // flux kernel; *taped_old is the old level, 0 when the step does not advance.
+// before_fix(old level), when given, is called after an advancing step's
+// flux kernel and before its fixer, while the slab holds the step's base
+// state (the new level's seams included).
 int run_once(gmc_adv *a, int64_t time_step, bool basis, double scale, char *tape = nullptr,
-             int *taped_old = nullptr) {
+             int *taped_old = nullptr, const std::function<void(int)> &before_fix = nullptr) {
   if (taped_old) *taped_old = 0;
   const AdvGrid &g = a->g;
   const Mesh M{a->d_mesh, g.nlat};
@@ -670,6 +674,7 @@ int run_once(gmc_adv *a, int64_t time_step, bool basis, double scale, char *tape
       if (rc) return rc;
       if (taped_old) *taped_old = a->old;
     }
+    if (before_fix) before_fix(a->old);
     FixPtrs fp{qp, tracer_ptr(a, a->new_, 0), a->sm, a->d_wgt, ts, a->d_flags, a->d_fixes, a->cfg.lon_damp_passes};
     hipLaunchKernelGGL(k_adv_fix, dim3(g.nlat, ntr), dim3(NA), a->lds_bytes, a->stream, g, fp);
     mark(a, KA_FIX);
@@ -950,9 +955,9 @@ const char *gmc_adv_kernel_name(int32_t k) { return k >= 0 && k < KA_NCLASS ? kA
 // CFL numbers and split divergences, q of both levels (the new level still
 // holding the seam values the fixer reads), qx, qy and the pre-fixer qp.
 //
-// Tangent step: t_adv_psum (pole sums of the inner y-flux tangents),
-// t_adv_inner (tangents of qx, qy), t_adv_psum (outer), t_adv_outer (tangent
-// of qp), t_adv_fix (the fixer on the base's branches, then the damping).
+// Tangent step: t_adv_inner (tangents of qx, qy), t_adv_outer (tangent of
+// qp), both on the forward's work items, t_adv_fix (the fixer on the base's
+// branches, then the damping).
 // Adjoint step: a_adv_fix (damping, the fixer transposed, / m), k_adv_psum2
 // (pole sums), a_adv_outer (outer fluxes transposed, added into the adjoints
 // of qy and qx with atomics), a_adv_inner (inner fluxes transposed, added
@@ -1170,18 +1175,6 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
   return s;
 }
 
-// ps[t][p] = sum over the columns of the y-flux tangent of the V row next to
-// pole p (0 south, 1 north); block (p, t)
-__global__ void __launch_bounds__(NA) t_adv_psum(AdvGrid g, int lim, const double *q, const double *d,
-                                                 const double *v, const double *cfly, int64_t ts, double *ps) {
-  __shared__ double red[NA / kWave];
-  const int p = blockIdx.x, t = blockIdx.y, h = p == 0 ? 0 : g.nhalf - 1;
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < g.nlon; i += NA)
-    acc += yflux_tl(g, lim, cfly[h * g.pitch + i], v[h * g.pitch + i], h, i, q + t * ts, d + t * ts);
-  const double s = block_sum(acc, red);
-  if (threadIdx.x == 0) ps[2 * t + p] = s;
-}
 // ps[t][p] = sum over the columns of the pole row p of f
 __global__ void __launch_bounds__(NA) k_adv_psum2(AdvGrid g, const double *f, int64_t ts, double *ps) {
   __shared__ double red[NA / kWave];
@@ -1192,55 +1185,99 @@ __global__ void __launch_bounds__(NA) k_adv_psum2(AdvGrid g, const double *f, in
   if (threadIdx.x == 0) ps[2 * t + p] = s;
 }
 
-// tangents of qx, qy (ffsl_mod.F90:350-407); grid (column chunks, rows,
-// tracers)
-__global__ void __launch_bounds__(NA) t_adv_inner(AdvGrid g, Mesh M, int lim, TBase b, TWork w) {
-  const int i = blockIdx.x * NA + threadIdx.x, r = blockIdx.y, t = blockIdx.z;
-  if (i >= g.nlon) return;
-  const int64_t to = t * w.ts, P = g.pitch, k = r * P + i;
+// tangents of qx, qy (ffsl_mod.F90:350-407), on the work items of
+// k_adv_inner: each wavefront forms the y-flux tangents of one V row (into
+// LDS, shared by the cells either side) and the x-flux tangents of one cell
+// row, the west face's from the lane below; a pole row's item sums the
+// tangents of its V row's fluxes itself
+__global__ void __launch_bounds__(NF) t_adv_inner(AdvGrid g, Mesh M, int lim, int ntr, TBase b, TWork w) {
+  __shared__ double sfy[NF / kWave][kWave];
+  __shared__ double bcast;
+  const FluxItem it = flux_item(g, ntr);
+  const int n = g.nlon;
+  const int64_t to = it.t * w.ts, P = g.pitch;
   const double *q = b.q + to, *d = w.in + to;
-  const double dt = g.dt, dc = d[k];
-  if (r == 0 || r == g.nlat - 1) {
-    const int h = r == 0 ? 0 : g.nhalf - 1;
-    const double pole = w.ps[2 * t + (r == 0 ? 0 : 1)] * M(GMC_LE_LAT, h) / g.nlon / M(GMC_AREA_CELL, r);
-    w.dqx[to + k] = dc;
-    w.dqy[to + k] = r == 0 ? dc - 0.5 * (pole - b.divy[k] * dc) * dt : dc + 0.5 * (pole - b.divy[k] * dc) * dt;
+  const double dt = g.dt;
+  if (it.pole) {
+    const int r = it.r, h = r == 0 ? 0 : g.nhalf - 1;
+    if (threadIdx.x < kWave) {
+      const double s = wave_row_sum(n, [&](int i) { return yflux_tl(g, lim, b.cfly[h * P + i], b.v[h * P + i], h, i, q, d); });
+      if (threadIdx.x == 0) bcast = s;
+    }
+    __syncthreads();
+    const double pole = bcast * M(GMC_LE_LAT, h) / n / M(GMC_AREA_CELL, r);
+    for (int i = threadIdx.x; i < n; i += NF) {
+      const int64_t k = r * P + i;
+      const double dc = d[k];
+      w.dqx[to + k] = dc;
+      w.dqy[to + k] = r == 0 ? dc - 0.5 * (pole - b.divy[k] * dc) * dt : dc + 0.5 * (pole - b.divy[k] * dc) * dt;
+    }
     return;
   }
-  const int iw = wrap(i - 1, g.nlon);
-  const double fx = xflux_tl(g, lim, b.cflx[k], b.u[k], i, q + r * P, d + r * P);
-  const double fxw = xflux_tl(g, lim, b.cflx[r * P + iw], b.u[r * P + iw], iw, q + r * P, d + r * P);
-  const double fs = yflux_tl(g, lim, b.cfly[(r - 1) * P + i], b.v[(r - 1) * P + i], r - 1, i, q, d);
-  const double fn = yflux_tl(g, lim, b.cfly[r * P + i], b.v[r * P + i], r, i, q, d);
-  const double lel = M(GMC_LE_LON, r), ac = M(GMC_AREA_CELL, r);
-  const double ls = M(GMC_LE_LAT, r - 1), ln = M(GMC_LE_LAT, r);
-  w.dqx[to + k] = dc - 0.5 * ((fx - fxw) * lel / ac - b.divx[k] * dc) * dt;
-  w.dqy[to + k] = dc - 0.5 * ((fn * ln - fs * ls) / ac - b.divy[k] * dc) * dt;
-}
-
-// tangent of the outer fluxes and the update (adv_mod.F90:183-235) into dqp
-__global__ void __launch_bounds__(NA) t_adv_outer(AdvGrid g, Mesh M, int lim, TBase b, TWork w) {
-  const int i = blockIdx.x * NA + threadIdx.x, r = blockIdx.y, t = blockIdx.z;
-  if (i >= g.nlon) return;
-  const int64_t to = t * w.ts, P = g.pitch, k = r * P + i;
-  const double dt = g.dt, dc = w.in[to + k];
-  double num;
-  if (r == 0 || r == g.nlat - 1) {
-    const int h = r == 0 ? 0 : g.nhalf - 1;
-    const double pole = w.ps[2 * t + (r == 0 ? 0 : 1)] * M(GMC_LE_LAT, h) / g.nlon / M(GMC_AREA_CELL, r) * dt;
-    num = r == 0 ? b.old_m[k] * dc - pole : b.old_m[k] * dc + pole;
-  } else {
-    const double *qy = b.qy + to + r * P, *dqy = w.dqy + to + r * P;
-    const int iw = wrap(i - 1, g.nlon);
-    const double fx = xflux_tl(g, lim, b.cflx[k], b.mfx[k], i, qy, dqy);
-    const double fxw = xflux_tl(g, lim, b.cflx[r * P + iw], b.mfx[r * P + iw], iw, qy, dqy);
-    const double fs = yflux_tl(g, lim, b.cfly[(r - 1) * P + i], b.mfy[(r - 1) * P + i], r - 1, i, b.qx + to, w.dqx + to);
-    const double fn = yflux_tl(g, lim, b.cfly[r * P + i], b.mfy[r * P + i], r, i, b.qx + to, w.dqx + to);
+  if (it.r < 0) return;   // padding item (uniform over the block)
+  const int wv = threadIdx.x / kWave, l = threadIdx.x % kWave;
+  const int iw = wrap(it.c0 - 1 + l, n);   // column of the lane
+  const int h = it.r - 1 + wv, r = it.r + wv;
+  if (h <= g.nhalf - 1) sfy[wv][l] = yflux_tl(g, lim, b.cfly[h * P + iw], b.v[h * P + iw], h, iw, q, d);
+  const bool row = wv < kRowsPerItem && r <= g.nlat - 2;   // uniform over the wavefront
+  const double fx = row ? xflux_tl(g, lim, b.cflx[r * P + iw], b.u[r * P + iw], iw, q + r * P, d + r * P) : 0.0;
+  const double fxw = __shfl_up(fx, 1, kWave);
+  __syncthreads();
+  const int i = it.c0 - 1 + l;   // cell of the lane
+  if (row && l >= 1 && i < n) {
     const double lel = M(GMC_LE_LON, r), ac = M(GMC_AREA_CELL, r);
     const double ls = M(GMC_LE_LAT, r - 1), ln = M(GMC_LE_LAT, r);
-    num = b.old_m[k] * dc - ((fx - fxw) * lel + (fn * ln - fs * ls)) / ac * dt;
+    const double fs = sfy[wv][l], fn = sfy[wv + 1][l];
+    const int64_t k = r * P + i;
+    const double dc = d[k];
+    w.dqx[to + k] = dc - 0.5 * ((fx - fxw) * lel / ac - b.divx[k] * dc) * dt;
+    w.dqy[to + k] = dc - 0.5 * ((fn * ln - fs * ls) / ac - b.divy[k] * dc) * dt;
   }
-  w.dqp[to + k] = num / b.m[k];
+}
+
+// tangent of the outer fluxes and the update (adv_mod.F90:183-235) into
+// dqp, on the work items of k_adv_flux
+__global__ void __launch_bounds__(NF) t_adv_outer(AdvGrid g, Mesh M, int lim, int ntr, TBase b, TWork w) {
+  __shared__ double sfy[NF / kWave][kWave];
+  __shared__ double bcast;
+  const FluxItem it = flux_item(g, ntr);
+  const int n = g.nlon;
+  const int64_t to = it.t * w.ts, P = g.pitch;
+  const double *qx = b.qx + to, *qy = b.qy + to, *dqx = w.dqx + to, *dqy = w.dqy + to, *d = w.in + to;
+  const double dt = g.dt;
+  if (it.pole) {
+    const int r = it.r, h = r == 0 ? 0 : g.nhalf - 1;
+    if (threadIdx.x < kWave) {
+      const double s = wave_row_sum(n, [&](int i) { return yflux_tl(g, lim, b.cfly[h * P + i], b.mfy[h * P + i], h, i, qx, dqx); });
+      if (threadIdx.x == 0) bcast = s;
+    }
+    __syncthreads();
+    const double pole = bcast * M(GMC_LE_LAT, h) / n / M(GMC_AREA_CELL, r) * dt;
+    for (int i = threadIdx.x; i < n; i += NF) {
+      const int64_t k = r * P + i;
+      const double num = r == 0 ? b.old_m[k] * d[k] - pole : b.old_m[k] * d[k] + pole;
+      w.dqp[to + k] = num / b.m[k];
+    }
+    return;
+  }
+  if (it.r < 0) return;
+  const int wv = threadIdx.x / kWave, l = threadIdx.x % kWave;
+  const int iw = wrap(it.c0 - 1 + l, n);
+  const int h = it.r - 1 + wv, r = it.r + wv;
+  if (h <= g.nhalf - 1) sfy[wv][l] = yflux_tl(g, lim, b.cfly[h * P + iw], b.mfy[h * P + iw], h, iw, qx, dqx);
+  const bool row = wv < kRowsPerItem && r <= g.nlat - 2;
+  const double fx = row ? xflux_tl(g, lim, b.cflx[r * P + iw], b.mfx[r * P + iw], iw, qy + r * P, dqy + r * P) : 0.0;
+  const double fxw = __shfl_up(fx, 1, kWave);
+  __syncthreads();
+  const int i = it.c0 - 1 + l;
+  if (row && l >= 1 && i < n) {
+    const double lel = M(GMC_LE_LON, r), ac = M(GMC_AREA_CELL, r);
+    const double ls = M(GMC_LE_LAT, r - 1), ln = M(GMC_LE_LAT, r);
+    const double fs = sfy[wv][l], fn = sfy[wv + 1][l];
+    const int64_t k = r * P + i;
+    const double num = b.old_m[k] * d[k] - ((fx - fxw) * lel + (fn * ln - fs * ls)) / ac * dt;
+    w.dqp[to + k] = num / b.m[k];
+  }
 }
 
 // One row's positivity fixer (adv_mod.F90:236-258) on the base values sb,
@@ -1541,12 +1578,9 @@ void tl_step(gmc_adv *a, const char *slot, int old, const double *in, double *ou
   const size_t F = tl_fields(a);
   double *dqx = a->tl_f + 2 * F, *dqy = a->tl_f + 3 * F, *dqp = a->tl_f + 4 * F, *ps = a->tl_f + 5 * F;
   const TBase b = tape_base(a, slot, old);
-  const dim3 cells((g.nlon + NA - 1) / NA, g.nlat, ntr), poles(2, ntr);
-  hipLaunchKernelGGL(t_adv_psum, poles, dim3(NA), 0, a->stream, g, lim, b.q, in, b.v, b.cfly, ts, ps);
-  hipLaunchKernelGGL(t_adv_inner, cells, dim3(NA), 0, a->stream, g, M, lim, b, TWork{in, dqx, dqy, dqp, out, ps, ts});
-  double *ps2 = ps + 2 * ntr;
-  hipLaunchKernelGGL(t_adv_psum, poles, dim3(NA), 0, a->stream, g, lim, b.qx, dqx, b.mfy, b.cfly, ts, ps2);
-  hipLaunchKernelGGL(t_adv_outer, cells, dim3(NA), 0, a->stream, g, M, lim, b, TWork{in, dqx, dqy, dqp, out, ps2, ts});
+  const dim3 items(flux_blocks(g, ntr));
+  hipLaunchKernelGGL(t_adv_inner, items, dim3(NF), 0, a->stream, g, M, lim, ntr, b, TWork{in, dqx, dqy, dqp, out, ps, ts});
+  hipLaunchKernelGGL(t_adv_outer, items, dim3(NF), 0, a->stream, g, M, lim, ntr, b, TWork{in, dqx, dqy, dqp, out, ps, ts});
   hipLaunchKernelGGL(t_adv_fix, dim3(g.nlat, ntr), dim3(NA), 4 * (size_t)g.nlon * sizeof(double), a->stream, g, b,
                      TWork{in, dqx, dqy, dqp, out, ps, ts}, a->d_wgt, a->cfg.lon_damp_passes);
 }
@@ -1593,19 +1627,20 @@ extern "C" {
 int gmc_adv_tl_drive(gmc_adv *a, int64_t time_step0, const double *scale, int32_t nsteps, double *dq, int64_t ld) {
   if (!a) return GMC_EINVAL;
   int rc = tl_check(a, scale, nsteps, time_step0, dq, ld);
-  if (!rc) rc = tl_alloc(a, 1);
+  if (!rc) rc = tl_alloc(a, 0);
   if (rc) return rc;
   const size_t F = tl_fields(a);
   double *cur = a->tl_f, *nxt = a->tl_f + F;
   rc = tl_copy(a, true, cur, dq, ld);
+  // each advancing step's tangent runs on the live slab, between the step's
+  // flux kernel and its fixer
+  const auto tangent = [&](int old) {
+    tl_step(a, (const char *)a->slab, old, cur, nxt);
+    std::swap(cur, nxt);
+  };
   for (int s = 0; s < nsteps && !rc; s++) {
-    int old = 0;
-    rc = run_once(a, time_step0 + s, true, scale[s], tl_slot(a, 0), &old);
-    if (!rc && old) {
-      tl_step(a, tl_slot(a, 0), old, cur, nxt);
-      std::swap(cur, nxt);
-      rc = ahip(a, hipGetLastError(), "tangent-linear launch");
-    }
+    rc = run_once(a, time_step0 + s, true, scale[s], nullptr, nullptr, tangent);
+    if (!rc) rc = ahip(a, hipGetLastError(), "tangent-linear launch");
   }
   if (!rc) rc = check_flags(a);
   if (!rc) rc = tl_copy(a, false, cur, dq, ld);

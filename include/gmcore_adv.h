@@ -103,7 +103,8 @@ int gmc_adv_drive(gmc_adv *adv, int64_t time_step0, const double *scale, int32_t
  * every ckpt_every steps, then, last segment first, recomputes the segment
  * from its checkpoint keeping every step's base state and runs the adjoint
  * steps back over it.  Memory: (nsteps / ckpt_every + 1 + ckpt_every) copies
- * of the object's device slab, kept between calls.  The object ends as after
+ * of the object's device slab, kept between calls (the tangent keeps none:
+ * it runs inside each forward step).  The object ends as after
  * gmc_adv_drive.  The adjoint adds with atomics: reproducible to rounding, not
  * bit for bit. */
 int gmc_adv_tl_drive(gmc_adv *adv, int64_t time_step0, const double *scale, int32_t nsteps, double *dq, int64_t ld);
