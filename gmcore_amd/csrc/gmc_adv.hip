@@ -1041,6 +1041,37 @@ __device__ __forceinline__ Jac ppm_jac(int lim, const double f[5]) {
   return J;
 }
 
+// J^T (a1, a2, a3) of ppm_jac: the five stencil cells' shares of the
+// adjoints a1, a2, a3 of (fl, df, f6), with the same branch choices; written
+// through the edge values, a1 fl + a2 (fr - fl) + a3 (6 f - 3 (fl + fr)) =
+// (a1 - a2 - 3 a3) fl + (a2 - 3 a3) fr + 6 a3 f
+__device__ __forceinline__ void ppm_adj(int lim, const double f[5], double a1, double a2, double a3, double out[5]) {
+  double gl[3], gc[3], gr[3];
+  slope_jac(lim, f[0], f[1], f[2], gl[0], gl[1], gl[2]);
+  slope_jac(lim, f[1], f[2], f[3], gc[0], gc[1], gc[2]);
+  slope_jac(lim, f[2], f[3], f[4], gr[0], gr[1], gr[2]);
+  const double dfl = slope(lim, f[0], f[1], f[2]);
+  const double df = slope(lim, f[1], f[2], f[3]);
+  const double dfr = slope(lim, f[2], f[3], f[4]);
+  const double fl0 = 0.5 * (f[1] + f[2]) + div_rn(dfl - df, 6.0, 1.0 / 6.0);
+  const double fr0 = 0.5 * (f[3] + f[2]) + div_rn(df - dfr, 6.0, 1.0 / 6.0);
+  const double a = fabs(df), s = copysign(1.0, df);
+  const bool cl = fabs(fl0 - f[2]) < a, cr = fabs(fr0 - f[2]) < a;
+  const double tl = s * (fl0 - f[2] >= 0 ? 1.0 : -1.0), tr = s * (fr0 - f[2] >= 0 ? 1.0 : -1.0);
+  const double A = a1 - a2 - 3 * a3, B = a2 - 3 * a3;
+  // fl = f - df, or f - tl (fl0 - f); fr = f + df, or f + tr (fr0 - f):
+  // the adjoints of fl0, fr0, of the three slopes, and f's direct share
+  const double al0 = cl ? -tl * A : 0.0, ar0 = cr ? tr * B : 0.0;
+  const double adf = (cl ? 0.0 : -A) + (cr ? 0.0 : B) - al0 / 6.0 + ar0 / 6.0;
+  const double adl = al0 / 6.0, adr = -ar0 / 6.0;
+  const double a_f = A + B + 6 * a3 - al0 - ar0;
+  out[0] = adl * gl[0];
+  out[1] = adl * gl[1] + adf * gc[0] + 0.5 * al0;
+  out[2] = a_f + 0.5 * (al0 + ar0) + adl * gl[2] + adf * gc[1] + adr * gr[0];
+  out[3] = adf * gc[2] + adr * gr[1] + 0.5 * ar0;
+  out[4] = adr * gr[2];
+}
+
 // quadrature weights of the x-flux through U point i (hflx_ppm_cell
 // :831-853): flux = wu (sum of cells k0 .. k0+n-1 + w1 fl + w2 df + w3 f6 of
 // the upwind cell iu), |int(c)| clamped to the halo bound as in the forward
@@ -1124,30 +1155,33 @@ __device__ __forceinline__ void xflux_ad(const AdvGrid &g, int lim, double c, do
   const int n = g.nlon;
   const XW w = x_weights(g, c, u, i);
   const double gw = w.wu * lam;
-  for (int k = 0; k < w.n; k++) atomicAdd(&a[wrap(w.k0 + k, n)], gw);
-  double f[5];
+  for (int k = 0, j = wrap(w.k0, n); k < w.n; k++, j = j + 1 == n ? 0 : j + 1) atomicAdd(&a[j], gw);
+  double f[5], o[5];
+  int col[5];
 #pragma unroll
-  for (int t = 0; t < 5; t++) f[t] = q[wrap(w.iu - 2 + t, n)];
-  const Jac J = ppm_jac(lim, f);
+  for (int t = 0; t < 5; t++) {
+    col[t] = wrap(w.iu - 2 + t, n);
+    f[t] = q[col[t]];
+  }
+  ppm_adj(lim, f, gw * w.w1, gw * w.w2, gw * w.w3, o);
 #pragma unroll
-  for (int t = 0; t < 5; t++)
-    atomicAdd(&a[wrap(w.iu - 2 + t, n)], gw * (w.w1 * J.l[t] + w.w2 * J.d[t] + w.w3 * J.s[t]));
+  for (int t = 0; t < 5; t++) atomicAdd(&a[col[t]], o[t]);
 }
 __device__ __forceinline__ void yflux_ad(const AdvGrid &g, int lim, double c, double v, int h, int i,
                                          const double *q, double *a, double lam) {
   if (c == 0 || lam == 0) return;
   const YW w = y_weights(c, v, h);
   const double gw = w.wv * lam;
-  double f[5];
-  int64_t k[5];
+  double f[5], o[5];
+  int k[5];  // a field has fewer than 2^31 elements
 #pragma unroll
   for (int t = 0; t < 5; t++) {
-    k[t] = yidx(g, w.ju - 2 + t, i);
+    k[t] = (int)yidx(g, w.ju - 2 + t, i);
     f[t] = q[k[t]];
   }
-  const Jac J = ppm_jac(lim, f);
+  ppm_adj(lim, f, gw * w.w1, gw * w.w2, gw * w.w3, o);
 #pragma unroll
-  for (int t = 0; t < 5; t++) unsafeAtomicAdd(&a[k[t]], gw * (w.w1 * J.l[t] + w.w2 * J.d[t] + w.w3 * J.s[t]));
+  for (int t = 0; t < 5; t++) unsafeAtomicAdd(&a[k[t]], o[t]);
 }
 
 // threads of the row-block flux adjoints
@@ -1423,21 +1457,26 @@ __global__ void __launch_bounds__(NAD) a_adv_outer(AdvGrid g, Mesh M, int lim, T
   const double dt = g.dt;
   const double *an = w.dqp + to;
   for (int i = threadIdx.x; i < n; i += NAD) acc[i] = 0.0;
+  // the row's factors: x-flux, and the V row h = r's shares of the cell
+  // rows either side (a pole row through its zonal sum)
+  const bool inner = r >= 1 && r <= g.nlat - 2;
+  const double kx = inner ? M(GMC_LE_LON, r) / M(GMC_AREA_CELL, r) * dt : 0.0;
+  const int h = r;
+  const bool vrow = h <= g.nlat - 2;
+  const double ks = vrow && h >= 1 ? M(GMC_LE_LAT, h) / M(GMC_AREA_CELL, h) * dt : 0.0;
+  const double kn = vrow && h + 1 <= g.nlat - 2 ? M(GMC_LE_LAT, h) / M(GMC_AREA_CELL, h + 1) * dt : 0.0;
+  double kp = 0.0;
+  if (vrow && h == 0) kp -= M(GMC_LE_LAT, 0) / g.nlon / M(GMC_AREA_CELL, 0) * dt * w.ps[2 * t];
+  if (vrow && h == g.nlat - 2) kp += M(GMC_LE_LAT, h) / g.nlon / M(GMC_AREA_CELL, g.nlat - 1) * dt * w.ps[2 * t + 1];
   __syncthreads();
   for (int i = threadIdx.x; i < n; i += NAD) {
     const int64_t k = r * P + i;
     w.out[to + k] = b.old_m[k] * an[k];
-    if (r >= 1 && r <= g.nlat - 2) {
-      const double lf = (an[r * P + wrap(i + 1, n)] - an[k]) * M(GMC_LE_LON, r) / M(GMC_AREA_CELL, r) * dt;
-      xflux_ad(g, lim, b.cflx[k], b.mfx[k], i, b.qy + to + r * P, acc, lf);
-    }
-    if (r > g.nlat - 2) continue;
-    const int h = r;
-    double lf = 0.0;
-    if (h >= 1) lf -= M(GMC_LE_LAT, h) / M(GMC_AREA_CELL, h) * dt * an[h * P + i];
-    if (h + 1 <= g.nlat - 2) lf += M(GMC_LE_LAT, h) / M(GMC_AREA_CELL, h + 1) * dt * an[(h + 1) * P + i];
-    if (h == 0) lf -= M(GMC_LE_LAT, 0) / g.nlon / M(GMC_AREA_CELL, 0) * dt * w.ps[2 * t];
-    if (h == g.nlat - 2) lf += M(GMC_LE_LAT, h) / g.nlon / M(GMC_AREA_CELL, g.nlat - 1) * dt * w.ps[2 * t + 1];
+    if (inner) xflux_ad(g, lim, b.cflx[k], b.mfx[k], i, b.qy + to + r * P, acc, (an[r * P + wrap(i + 1, n)] - an[k]) * kx);
+    if (!vrow) continue;
+    double lf = kp;
+    if (h >= 1) lf -= ks * an[h * P + i];
+    if (h + 1 <= g.nlat - 2) lf += kn * an[(h + 1) * P + i];
     yflux_ad(g, lim, b.cfly[h * P + i], b.mfy[h * P + i], h, i, b.qx + to, w.dqx + to, lf);
   }
   __syncthreads();
@@ -1455,6 +1494,10 @@ __global__ void __launch_bounds__(NAD) a_adv_inner(AdvGrid g, Mesh M, int lim, T
   const double dt = g.dt;
   const double *ax = w.dqx + to, *ay = w.dqy + to;
   for (int i = threadIdx.x; i < n; i += NAD) acc[i] = 0.0;
+  const bool inner = r >= 1 && r <= g.nlat - 2;
+  const double kx = inner ? 0.5 * dt * M(GMC_LE_LON, r) / M(GMC_AREA_CELL, r) : 0.0;
+  const double ks = r >= 1 && r <= g.nlat - 2 ? 0.5 * dt * M(GMC_LE_LAT, r) / M(GMC_AREA_CELL, r) : 0.0;
+  const double kn = r + 1 <= g.nlat - 2 ? 0.5 * dt * M(GMC_LE_LAT, r) / M(GMC_AREA_CELL, r + 1) : 0.0;
   __syncthreads();
   for (int i = threadIdx.x; i < n; i += NAD) {
     const int64_t k = r * P + i;
@@ -1464,15 +1507,12 @@ __global__ void __launch_bounds__(NAD) a_adv_inner(AdvGrid g, Mesh M, int lim, T
     else if (r == g.nlat - 1) add = gx + (1 - 0.5 * b.divy[k] * dt) * gy;
     else add = (1 + 0.5 * b.divx[k] * dt) * gx + (1 + 0.5 * b.divy[k] * dt) * gy;
     atomicAdd(&acc[i], add);
-    if (r >= 1 && r <= g.nlat - 2) {
-      const double lf = 0.5 * dt * M(GMC_LE_LON, r) / M(GMC_AREA_CELL, r) * (ax[r * P + wrap(i + 1, n)] - gx);
-      xflux_ad(g, lim, b.cflx[k], b.u[k], i, b.q + to + r * P, acc, lf);
-    }
+    if (inner) xflux_ad(g, lim, b.cflx[k], b.u[k], i, b.q + to + r * P, acc, kx * (ax[r * P + wrap(i + 1, n)] - gx));
     if (r > g.nlat - 2) continue;
     const int h = r;
     double lf = 0.0;
-    if (h >= 1) lf -= 0.5 * dt * M(GMC_LE_LAT, h) / M(GMC_AREA_CELL, h) * ay[h * P + i];
-    if (h + 1 <= g.nlat - 2) lf += 0.5 * dt * M(GMC_LE_LAT, h) / M(GMC_AREA_CELL, h + 1) * ay[(h + 1) * P + i];
+    if (h >= 1) lf -= ks * ay[h * P + i];
+    if (h + 1 <= g.nlat - 2) lf += kn * ay[(h + 1) * P + i];
     // the pole rows' qy feed no outer flux (the outer x-fluxes skip the
     // pole rows), so their adjoint, and the pole-sum terms it would bring
     // into the V rows next to the poles, are zero
