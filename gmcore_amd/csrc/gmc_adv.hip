@@ -774,10 +774,10 @@ int gmc_adv_create(const gmc_mesh *mesh, const gmc_adv_config *cfg, gmc_adv **ou
 int gmc_adv_destroy(gmc_adv *a) {
   if (!a) return GMC_EINVAL;
   if (a->stream) (void)hipStreamSynchronize(a->stream);
+  if (a->astream) (void)hipStreamSynchronize(a->astream);
   if (a->slab) (void)hipFree(a->slab);
   if (a->tl_f) (void)hipFree(a->tl_f);
   if (a->tl_ws) (void)hipFree(a->tl_ws);
-  if (a->astream) (void)hipStreamSynchronize(a->astream);
   for (hipEvent_t e : {a->ev_fwd, a->ev_join, a->ev_ready[0], a->ev_ready[1], a->ev_done[0], a->ev_done[1]})
     if (e) (void)hipEventDestroy(e);
   if (a->astream) (void)hipStreamDestroy(a->astream);
