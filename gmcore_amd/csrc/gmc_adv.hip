@@ -126,14 +126,14 @@ __device__ __forceinline__ double run_sum(const double *mx, int k0, int cnt, int
 // selected per lane.
 __device__ __forceinline__ double yflux(const AdvGrid &g, int lim, double c, double v, int h, int i,
                                         const double *my) {
-  const int ih = wrap(i + g.half, g.nlon);
+  const int ih = wrap(i + g.half, g.nlon), P = (int)g.pitch;  // a field has fewer than 2^31 elements
   double f[6];
 #pragma unroll
   for (int k = 0; k < 6; k++) {
     const int r = h - 2 + k;
     const bool mir = r < 0 || r > g.nlat - 1;
     const int rr = r < 0 ? -r : (r > g.nlat - 1 ? 2 * (g.nlat - 1) - r : r);
-    f[k] = my[rr * g.pitch + (unsigned)(mir ? ih : i)];
+    f[k] = my[rr * P + (mir ? ih : i)];
   }
   // both wind directions on one path (no divergence within a wavefront):
   // c > 0: v * part(1 - c, 1) / c from row h; c < 0: -v * part(0, -c) / c
