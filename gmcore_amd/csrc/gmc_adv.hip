@@ -949,11 +949,13 @@ const char *gmc_adv_kernel_name(int32_t k) { return k >= 0 && k < KA_NCLASS ? kA
 // piecewise linear in q: the slope limiters, the PPM edge clipping and the
 // positivity fixer choose among linear expressions by comparisons of q.  The
 // tangent is the linear map of the piece the base state selects (ties go to
-// the first candidate, as in slope_jac and ppm_jac), the adjoint its
-// transpose.  Each kernel reads the base state of its step from a tape slot,
-// a copy of the object's slab taken after the step's flux kernel: the winds,
-// CFL numbers and split divergences, q of both levels (the new level still
-// holding the seam values the fixer reads), qx, qy and the pre-fixer qp.
+// the first candidate, as in slope_jac and ppm_tan), the adjoint its
+// transpose.  Each kernel reads the base state of its step from a slab
+// image: the object's own slab for the tangent, which runs inside the
+// forward step between its flux kernel and its fixer, or a tape slot for
+// the adjoint (the winds, CFL numbers and split divergences, q of both
+// levels, the new level still holding the seam values the fixer reads, qx,
+// qy and the pre-fixer qp, at their slab offsets).
 //
 // Tangent step: t_adv_inner (tangents of qx, qy), t_adv_outer (tangent of
 // qp), both on the forward's work items, t_adv_fix (the fixer on the base's
@@ -1000,11 +1002,15 @@ __device__ __forceinline__ void slope_jac(int lim, double fm1, double f, double 
   }
 }
 
-// rows of d (fl, df, f6) / d f[0..4] of ppm (ffsl_mod.F90:1147-1173) at the
-// base values f; the edge clipping keeps f -+ df when |df| is not above
-// |fl - f| (|fr - f|)
-struct Jac { double l[5], d[5], s[5]; };
-__device__ __forceinline__ Jac ppm_jac(int lim, const double f[5]) {
+// The PPM Jacobian J = d (fl, df, f6) / d f[0..4] (ppm, ffsl_mod.F90:1147-1173)
+// at the base values f is applied, never formed: forward (ppm_tan, J d) in
+// the tangent, transposed (ppm_adj, J^T a) in the adjoint.  Both take the
+// branch the base values select; the edge clipping keeps f -+ df when |df|
+// is not above |fl - f| (|fr - f|).
+// J d: the tangents (dl, dd, d6) of (fl, df, f6) for the stencil tangent
+// d, by the chain rule through the slopes and the edge values
+__device__ __forceinline__ void ppm_tan(int lim, const double f[5], const double d[5], double &dl, double &dd,
+                                        double &d6) {
   double gl[3], gc[3], gr[3];
   slope_jac(lim, f[0], f[1], f[2], gl[0], gl[1], gl[2]);
   slope_jac(lim, f[1], f[2], f[3], gc[0], gc[1], gc[2]);
@@ -1014,37 +1020,24 @@ __device__ __forceinline__ Jac ppm_jac(int lim, const double f[5]) {
   const double dfr = slope(lim, f[2], f[3], f[4]);
   const double fl0 = 0.5 * (f[1] + f[2]) + div_rn(dfl - df, 6.0, 1.0 / 6.0);
   const double fr0 = 0.5 * (f[3] + f[2]) + div_rn(df - dfr, 6.0, 1.0 / 6.0);
-  double Jl0[5] = {0, 0, 0, 0, 0}, Jr0[5] = {0, 0, 0, 0, 0}, Jc[5] = {0, 0, 0, 0, 0};
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    Jl0[k] += gl[k] / 6.0;
-    Jl0[k + 1] -= gc[k] / 6.0;
-    Jr0[k + 1] += gc[k] / 6.0;
-    Jr0[k + 2] -= gr[k] / 6.0;
-    Jc[k + 1] = gc[k];
-  }
-  Jl0[1] += 0.5; Jl0[2] += 0.5;
-  Jr0[3] += 0.5; Jr0[2] += 0.5;
   const double a = fabs(df), s = copysign(1.0, df);
   const bool cl = fabs(fl0 - f[2]) < a, cr = fabs(fr0 - f[2]) < a;
   const double tl = s * (fl0 - f[2] >= 0 ? 1.0 : -1.0), tr = s * (fr0 - f[2] >= 0 ? 1.0 : -1.0);
-  Jac J;
-#pragma unroll
-  for (int t = 0; t < 5; t++) {
-    const double e = t == 2 ? 1.0 : 0.0;
-    const double jl = cl ? e - tl * (Jl0[t] - e) : e - Jc[t];
-    const double jr = cr ? e + tr * (Jr0[t] - e) : e + Jc[t];
-    J.l[t] = jl;
-    J.d[t] = jr - jl;
-    J.s[t] = 6 * e - 3 * (jl + jr);
-  }
-  return J;
+  const double tdl = gl[0] * d[0] + gl[1] * d[1] + gl[2] * d[2];
+  const double tdc = gc[0] * d[1] + gc[1] * d[2] + gc[2] * d[3];
+  const double tdr = gr[0] * d[2] + gr[1] * d[3] + gr[2] * d[4];
+  const double tl0 = 0.5 * (d[1] + d[2]) + (tdl - tdc) / 6.0;
+  const double tr0 = 0.5 * (d[3] + d[2]) + (tdc - tdr) / 6.0;
+  const double tfl = cl ? d[2] - tl * (tl0 - d[2]) : d[2] - tdc;
+  const double tfr = cr ? d[2] + tr * (tr0 - d[2]) : d[2] + tdc;
+  dl = tfl;
+  dd = tfr - tfl;
+  d6 = 6 * d[2] - 3 * (tfl + tfr);
 }
 
-// J^T (a1, a2, a3) of ppm_jac: the five stencil cells' shares of the
-// adjoints a1, a2, a3 of (fl, df, f6), with the same branch choices; written
-// through the edge values, a1 fl + a2 (fr - fl) + a3 (6 f - 3 (fl + fr)) =
-// (a1 - a2 - 3 a3) fl + (a2 - 3 a3) fr + 6 a3 f
+// J^T (a1, a2, a3): the five stencil cells' shares of the adjoints a1, a2,
+// a3 of (fl, df, f6); written through the edge values, a1 fl + a2 (fr - fl)
+// + a3 (6 f - 3 (fl + fr)) = (a1 - a2 - 3 a3) fl + (a2 - 3 a3) fr + 6 a3 f
 __device__ __forceinline__ void ppm_adj(int lim, const double f[5], double a1, double a2, double a3, double out[5]) {
   double gl[3], gc[3], gr[3];
   slope_jac(lim, f[0], f[1], f[2], gl[0], gl[1], gl[2]);
@@ -1123,10 +1116,8 @@ __device__ __forceinline__ double xflux_tl(const AdvGrid &g, int lim, double c, 
     f[t] = q[k];
     dd[t] = d[k];
   }
-  const Jac J = ppm_jac(lim, f);
-  double dl = 0, dm = 0, d6 = 0;
-#pragma unroll
-  for (int t = 0; t < 5; t++) { dl += J.l[t] * dd[t]; dm += J.d[t] * dd[t]; d6 += J.s[t] * dd[t]; }
+  double dl, dm, d6;
+  ppm_tan(lim, f, dd, dl, dm, d6);
   return w.wu * (sum + w.w1 * dl + w.w2 * dm + w.w3 * d6);
 }
 // tangent of the y-flux through V point (h, i): base field q, tangent d
@@ -1141,10 +1132,8 @@ __device__ __forceinline__ double yflux_tl(const AdvGrid &g, int lim, double c, 
     f[t] = q[k];
     dd[t] = d[k];
   }
-  const Jac J = ppm_jac(lim, f);
-  double dl = 0, dm = 0, d6 = 0;
-#pragma unroll
-  for (int t = 0; t < 5; t++) { dl += J.l[t] * dd[t]; dm += J.d[t] * dd[t]; d6 += J.s[t] * dd[t]; }
+  double dl, dm, d6;
+  ppm_tan(lim, f, dd, dl, dm, d6);
   return w.wv * (w.w1 * dl + w.w2 * dm + w.w3 * d6);
 }
 // adjoint of xflux_tl: lam * d flux / d d added into the LDS row a (every
