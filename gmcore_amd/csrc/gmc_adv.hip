@@ -971,16 +971,17 @@ namespace {
 // d slope / d (fm1, f, fp1) of the branch the base values select (slope_*,
 // ffsl_mod.F90:1175-1205); mono: the smallest of |df|, df_min, df_max, the
 // first on a tie, with the minimum and maximum of (fm1, f, fp1) taken at
-// their first occurrence.
-__device__ __forceinline__ void slope_jac(int lim, double fm1, double f, double fp1, double &g0, double &g1,
-                                          double &g2) {
+// their first occurrence.  Returns the slope itself, bit for bit slope()'s
+// value: a tie changes at most the sign of a zero, which fsign discards.
+__device__ __forceinline__ double slope_jac(int lim, double fm1, double f, double fp1, double &g0, double &g1,
+                                            double &g2) {
   const double df = (fp1 - fm1) * 0.5;
   g0 = -0.5; g1 = 0.0; g2 = 0.5;
-  if (lim == GMC_LIMITER_NONE) return;
+  if (lim == GMC_LIMITER_NONE) return df;
   const double a = fabs(df), s = copysign(1.0, df);
   if (lim == GMC_LIMITER_PD) {
-    if (2 * f < a) { g0 = 0.0; g1 = 2 * s * (2 * f >= 0 ? 1.0 : -1.0); g2 = 0.0; }
-    return;
+    if (2 * f < a) { g0 = 0.0; g1 = 2 * s * (2 * f >= 0 ? 1.0 : -1.0); g2 = 0.0; return fsign(2 * f, df); }
+    return df;
   }
   int lo = 0, hi = 0;
   double vlo = fm1, vhi = fm1;
@@ -992,14 +993,15 @@ __device__ __forceinline__ void slope_jac(int lim, double fm1, double f, double 
   int pick = 0;
   double cur = a;
   if (dmn < cur) { pick = 1; cur = dmn; }
-  if (dmx < cur) pick = 2;
-  if (pick == 0) return;  // fsign(|df|, df) = df
+  if (dmx < cur) { pick = 2; cur = dmx; }
+  if (pick == 0) return df;  // fsign(|df|, df) = df
   const double t = 2 * s;
   if (pick == 1) {        // s * 2 (f - min)
     g0 = lo == 0 ? -t : 0.0; g1 = lo == 1 ? 0.0 : t; g2 = lo == 2 ? -t : 0.0;
   } else {                // s * 2 (max - f)
     g0 = hi == 0 ? t : 0.0; g1 = hi == 1 ? 0.0 : -t; g2 = hi == 2 ? t : 0.0;
   }
+  return fsign(cur, df);
 }
 
 // The PPM Jacobian J = d (fl, df, f6) / d f[0..4] (ppm, ffsl_mod.F90:1147-1173)
@@ -1012,12 +1014,9 @@ __device__ __forceinline__ void slope_jac(int lim, double fm1, double f, double 
 __device__ __forceinline__ void ppm_tan(int lim, const double f[5], const double d[5], double &dl, double &dd,
                                         double &d6) {
   double gl[3], gc[3], gr[3];
-  slope_jac(lim, f[0], f[1], f[2], gl[0], gl[1], gl[2]);
-  slope_jac(lim, f[1], f[2], f[3], gc[0], gc[1], gc[2]);
-  slope_jac(lim, f[2], f[3], f[4], gr[0], gr[1], gr[2]);
-  const double dfl = slope(lim, f[0], f[1], f[2]);
-  const double df = slope(lim, f[1], f[2], f[3]);
-  const double dfr = slope(lim, f[2], f[3], f[4]);
+  const double dfl = slope_jac(lim, f[0], f[1], f[2], gl[0], gl[1], gl[2]);
+  const double df = slope_jac(lim, f[1], f[2], f[3], gc[0], gc[1], gc[2]);
+  const double dfr = slope_jac(lim, f[2], f[3], f[4], gr[0], gr[1], gr[2]);
   const double fl0 = 0.5 * (f[1] + f[2]) + div_rn(dfl - df, 6.0, 1.0 / 6.0);
   const double fr0 = 0.5 * (f[3] + f[2]) + div_rn(df - dfr, 6.0, 1.0 / 6.0);
   const double a = fabs(df), s = copysign(1.0, df);
@@ -1040,12 +1039,9 @@ __device__ __forceinline__ void ppm_tan(int lim, const double f[5], const double
 // + a3 (6 f - 3 (fl + fr)) = (a1 - a2 - 3 a3) fl + (a2 - 3 a3) fr + 6 a3 f
 __device__ __forceinline__ void ppm_adj(int lim, const double f[5], double a1, double a2, double a3, double out[5]) {
   double gl[3], gc[3], gr[3];
-  slope_jac(lim, f[0], f[1], f[2], gl[0], gl[1], gl[2]);
-  slope_jac(lim, f[1], f[2], f[3], gc[0], gc[1], gc[2]);
-  slope_jac(lim, f[2], f[3], f[4], gr[0], gr[1], gr[2]);
-  const double dfl = slope(lim, f[0], f[1], f[2]);
-  const double df = slope(lim, f[1], f[2], f[3]);
-  const double dfr = slope(lim, f[2], f[3], f[4]);
+  const double dfl = slope_jac(lim, f[0], f[1], f[2], gl[0], gl[1], gl[2]);
+  const double df = slope_jac(lim, f[1], f[2], f[3], gc[0], gc[1], gc[2]);
+  const double dfr = slope_jac(lim, f[2], f[3], f[4], gr[0], gr[1], gr[2]);
   const double fl0 = 0.5 * (f[1] + f[2]) + div_rn(dfl - df, 6.0, 1.0 / 6.0);
   const double fr0 = 0.5 * (f[3] + f[2]) + div_rn(df - dfr, 6.0, 1.0 / 6.0);
   const double a = fabs(df), s = copysign(1.0, df);
