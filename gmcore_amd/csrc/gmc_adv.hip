@@ -1690,8 +1690,9 @@ int adv_adjoint(gmc_adv *a, int64_t time_step0, const double *scale, int32_t nst
   const int K = std::min<int>(ckpt_every, std::max<int>(nsteps, 1));
   const int nseg = (nsteps + K - 1) / K;
   // slots: checkpoints 0 .. nseg (the last is the window's end), then two
-  // tape buffers of K slots; segment seg is taped into buffer seg % 2
-  rc = tl_alloc(a, nseg + 1 + 2 * K);
+  // tape buffers of K slots (one when there is a single segment); segment
+  // seg is taped into buffer seg % 2
+  rc = tl_alloc(a, nseg + 1 + (nseg > 1 ? 2 : 1) * K);
   if (!rc) rc = adj_stream(a);
   if (rc) return rc;
   auto tape = [&](int seg, int k) { return tl_slot(a, nseg + 1 + (seg % 2) * K + k); };

@@ -102,8 +102,9 @@ int gmc_adv_drive(gmc_adv *adv, int64_t time_step0, const double *scale, int32_t
  * adjoint of the old level at entry (dJ/dq_0), in place.  It keeps the slab
  * every ckpt_every steps, then, last segment first, recomputes the segment
  * from its checkpoint keeping every step's base state and runs the adjoint
- * steps back over it.  Memory: (nsteps / ckpt_every + 1 + ckpt_every) copies
- * of the object's device slab, kept between calls (the tangent keeps none:
+ * steps back over it.  Memory: nsteps / ckpt_every + 1 checkpoints and two
+ * tape buffers of ckpt_every steps (one when ckpt_every >= nsteps), each a
+ * copy of the object's device slab, kept between calls (the tangent keeps none:
  * it runs inside each forward step).  The object ends as after
  * gmc_adv_drive.  The adjoint adds with atomics: reproducible to rounding, not
  * bit for bit. */
