@@ -364,4 +364,9 @@ int gmc_field_ptr(gmc_ctx *c, int32_t level, int32_t field, double **dptr, int64
   return GMC_OK;
 }
 
+int gmc_filter_plan(gmc_ctx *c, int32_t out[4]) {
+  if (!c || !out) return GMC_EINVAL;
+  return fused_plan_info(c, out);
+}
+
 }  // extern "C"

@@ -615,7 +615,8 @@ __global__ void __launch_bounds__(NF) k_uvf_ens(const EnsMember *__restrict__ me
 static constexpr int kChunkWork = 32768;
 static constexpr int kNarrowChunk = 2 * NF;
 
-static void add_row(std::vector<Item> &v, int row, int kind, int n, int nlon, size_t &lds) {
+// returns the number of chunks the row was cut into
+static int add_row(std::vector<Item> &v, int row, int kind, int n, int nlon, size_t &lds) {
   int w = n <= 16 ? std::min(nlon, kNarrowChunk) : std::min(nlon, std::max(64, (kChunkWork / n + 63) / 64 * 64));
   const int parts = (nlon + w - 1) / w;
   const int h = n > 1 ? (n - 1) / 2 : 0;
@@ -624,12 +625,14 @@ static void add_row(std::vector<Item> &v, int row, int kind, int n, int nlon, si
     v.push_back(Item{row, c0, c1, kind, 0});
     lds = std::max(lds, sizeof(double) * (size_t)(c1 - c0 + 2 * h));
   }
+  return parts;
 }
 
 struct FusedPlan {
   Item *d_items = nullptr;
   int n_gz = 0, n_uv = 0;
   size_t lds_gz = 0, lds_uv = 0;
+  int rows_split = 0, max_parts = 1, widest = 0;   // reported by gmc_filter_plan
 };
 
 static FusedPlan *plan_of(gmc_ctx *c) { return (FusedPlan *)c->fused_plan; }
@@ -688,11 +691,17 @@ static int build_plan(gmc_ctx *c) {
 
   {
     std::vector<Item> gz, uv;
-    for (int r = 0; r < g.nlat; r++) add_row(gz, r, 0, nb_lon[r] + std::max(1, nb_lon[r] / 2), g.nlon, P->lds_gz);
+    auto count = [&](int parts, int n) {
+      P->rows_split += parts > 1;
+      P->max_parts = std::max(P->max_parts, parts);
+      P->widest = std::max(P->widest, n);
+    };
+    for (int r = 0; r < g.nlat; r++)
+      count(add_row(gz, r, 0, nb_lon[r] + std::max(1, nb_lon[r] / 2), g.nlon, P->lds_gz), nb_lon[r]);
     // u row r next to v rows r - 1 and r: the three read gz_f row r
     for (int r = 0; r < g.nlat; r++) {
-      add_row(uv, r, 1, nb_lon[r], g.nlon, P->lds_uv);
-      if (r < g.nhalf) add_row(uv, r, 2, nb_lat[r], g.nlon, P->lds_uv);
+      count(add_row(uv, r, 1, nb_lon[r], g.nlon, P->lds_uv), nb_lon[r]);
+      if (r < g.nhalf) count(add_row(uv, r, 2, nb_lat[r], g.nlon, P->lds_uv), nb_lat[r]);
     }
     auto work = [&](const Item &it) {
       const int n = it.kind == 2 ? nb_lat[it.row] : nb_lon[it.row];
@@ -766,6 +775,17 @@ static void fused_stage(gmc_ctx *c, int o, int s, int n, double dt, bool tape = 
 }
 
 int fused_prepare(gmc_ctx *c) { return c->fused_plan ? GMC_OK : build_plan(c); }
+
+int fused_plan_info(gmc_ctx *c, int32_t out[4]) {
+  const int rc = fused_prepare(c);
+  if (rc) return rc;
+  const FusedPlan *P = plan_of(c);
+  out[0] = P->n_gz + P->n_uv;
+  out[1] = P->rows_split;
+  out[2] = P->max_parts;
+  out[3] = P->widest;
+  return GMC_OK;
+}
 
 int fused_stage_tape(gmc_ctx *c, int o, int s, int n, double dt, const TapeOut *dst) {
   int rc = fused_prepare(c);

@@ -180,6 +180,7 @@ int ops_stage_tape(gmc_ctx *ctx, int old_l, int star_l, int new_l, double dt);
 int fused_integrate(gmc_ctx *ctx, int old_l, int new_l, double dt);
 void fused_release(gmc_ctx *ctx);
 int fused_prepare(gmc_ctx *ctx);
+int fused_plan_info(gmc_ctx *ctx, int32_t out[4]);
 int fused_step_graph(gmc_ctx *ctx, int old_l, int new_l, double dt);
 // Destinations of the intermediates a taping stage writes for the adjoint:
 // the fields operators_prepare forms from the star state (mfx_lat and

@@ -192,6 +192,12 @@ class ShallowWater:
     def synchronize(self):
         self._check(self.lib.gmc_synchronize(self.ctx), "synchronize")
 
+    def filter_plan(self) -> dict:
+        """Shape of the fused step's filter work list (gmc_filter_plan)."""
+        out = (C.c_int32 * 4)()
+        self._check(self.lib.gmc_filter_plan(self.ctx, out), "filter_plan")
+        return {"items": out[0], "rows_split": out[1], "max_chunks": out[2], "widest": out[3]}
+
 
 class Ensemble:
     """Batched members (include/gmcore_ens.h): one graph of three launches per

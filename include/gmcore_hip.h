@@ -191,6 +191,12 @@ int gmc_stream(gmc_ctx *ctx, void **stream);
  * pointer stays valid until gmc_destroy.  Row pitch in doubles in *pitch. */
 int gmc_field_ptr(gmc_ctx *ctx, int32_t level, int32_t field, double **dptr, int64_t *pitch);
 
+/* Shape of the fused step's zonal-filter work list, built on first use (no
+ * counterpart in the reference; for tests and profilers): out[0] = work
+ * items of one stage, out[1] = latitude rows cut into more than one column
+ * chunk, out[2] = most chunks of one row, out[3] = widest big filter (taps). */
+int gmc_filter_plan(gmc_ctx *ctx, int32_t out[4]);
+
 #ifdef __cplusplus
 }
 #endif
