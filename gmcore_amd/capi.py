@@ -33,7 +33,10 @@ MESH = {name: k for k, name in enumerate(MESH_FIELDS)}
 GMC_MESH_NFIELDS = len(MESH_FIELDS)
 
 GMC_TIME_PC2, GMC_TIME_WRFRK3, GMC_TIME_EULER = 0, 1, 2
-GMC_PV_MIDPOINT, GMC_PV_UPWIND = 0, 1
+GMC_PV_MIDPOINT, GMC_PV_UPWIND, GMC_PV_TVD = 0, 1, 2
+GMC_TVD_NONE, GMC_TVD_VAN_LEER, GMC_TVD_MC = 0, 1, 2
+PV_SCHEMES = {"midpoint": GMC_PV_MIDPOINT, "upwind": GMC_PV_UPWIND, "tvd": GMC_PV_TVD}
+TVD_LIMITERS = {"none": GMC_TVD_NONE, "van_leer": GMC_TVD_VAN_LEER, "mc": GMC_TVD_MC}
 
 FIELDS = ["u_lon", "v_lat", "gz", "u_f", "v_f", "gz_f", "gzs", "du", "dv",
           "dgz", "pv", "ke", "div", "m"]
@@ -65,7 +68,9 @@ class GmcConfig(C.Structure):
                 ("pv_pole_stokes", C.c_int32), ("coriolis_scheme", C.c_int32),
                 ("use_div_damp", C.c_int32), ("div_damp_order", C.c_int32),
                 ("div_damp_coef2", C.c_double),
-                ("big_filter", GmcFilter), ("small_filter1", GmcFilter)]
+                ("big_filter", GmcFilter), ("small_filter1", GmcFilter),
+                ("div_damp_coef4", C.c_double), ("dt_dyn", C.c_double),
+                ("tvd_limiter", C.c_int32), ("reserved", C.c_int32)]
 
 
 GMC_LIMITER_MONO, GMC_LIMITER_PD, GMC_LIMITER_NONE = 0, 1, 2
@@ -123,6 +128,7 @@ def load_hip() -> C.CDLL:
         "gmc_stream": (C.c_int, [vp, P(vp)]),
         "gmc_field_ptr": (C.c_int, [vp, C.c_int32, C.c_int32, P(P(C.c_double)), P(C.c_int64)]),
         "gmc_filter_plan": (C.c_int, [vp, P(C.c_int32)]),
+        "gmc_keep_derived": (C.c_int, [vp, C.c_int32]),
         # FFSL advection, include/gmcore_adv.h
         "gmc_adv_create": (C.c_int, [P(GmcMesh), P(GmcAdvConfig), P(vp)]),
         "gmc_adv_destroy": (C.c_int, [vp]),

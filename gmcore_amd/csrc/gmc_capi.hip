@@ -154,7 +154,11 @@ int gmc_create(const gmc_mesh *mesh, const gmc_config *cfg, gmc_ctx **out) {
   if (mesh->nlon < 8 || mesh->nlon % 2 || mesh->nlat < 5) return GMC_EINVAL;
   if (cfg->coriolis_scheme != 1 && cfg->coriolis_scheme != 2) return GMC_EINVAL;
   if (cfg->upwind_order_pv != 1 && cfg->upwind_order_pv != 3) return GMC_EINVAL;
-  if (cfg->use_div_damp && cfg->div_damp_order != 2) return GMC_EINVAL;
+  if (cfg->use_div_damp && cfg->div_damp_order != 2 && cfg->div_damp_order != 4) return GMC_EINVAL;
+  if (cfg->pv_scheme < GMC_PV_MIDPOINT || cfg->pv_scheme > GMC_PV_TVD) return GMC_EINVAL;
+  if (cfg->pv_scheme == GMC_PV_TVD && (cfg->tvd_limiter < GMC_TVD_NONE || cfg->tvd_limiter > GMC_TVD_MC))
+    return GMC_EINVAL;
+  if (cfg->pv_scheme == GMC_PV_TVD && !(cfg->dt_dyn > 0)) return GMC_EINVAL;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return GMC_EDEVICE;
   gmc_ctx *c = new gmc_ctx();
@@ -251,6 +255,7 @@ int gmc_prepare(gmc_ctx *c, int32_t level) {
   if (level < 1 || level > 3) return set_error(c, GMC_EINVAL, "level must be 1..3");
   int rc = ops_prepare(c, level);
   if (!rc) {
+    c->derived_kept[level] = 1;
     c->old_level = level;
     c->new_level = level == 1 ? 2 : 1;
     c->prepared = 1;
@@ -263,7 +268,9 @@ int gmc_space_operators(gmc_ctx *c, int32_t old_level, int32_t star_level, int32
   if (!c) return GMC_EINVAL;
   for (int l : {old_level, star_level, new_level, tend1, tend2})
     if (l < 1 || l > 3) return set_error(c, GMC_EINVAL, "level must be 1..3");
-  return ops_space_operators(c, old_level, star_level, new_level, tend1, tend2, dt, pass);
+  const int rc = ops_space_operators(c, old_level, star_level, new_level, tend1, tend2, dt, pass);
+  if (!rc && pass != 2) c->derived_kept[star_level] = 1;   // operators_prepare_2 ran on the star state
+  return rc;
 }
 
 int gmc_update_state(gmc_ctx *c, int32_t tend, int32_t old_level, int32_t new_level, double dt, int32_t what) {
@@ -281,6 +288,7 @@ int gmc_step(gmc_ctx *c, double dt, int32_t nsteps) {
   for (int n = 0; n < nsteps; n++) {
     int rc = fused_step_graph(c, c->old_level, c->new_level, dt);
     if (rc) return rc;
+    derived_after_step(c, c->old_level, c->new_level);
     std::swap(c->old_level, c->new_level);
   }
   return GMC_OK;
@@ -290,7 +298,9 @@ int gmc_time_integrate(gmc_ctx *c, int32_t old_level, int32_t new_level, double 
   if (!c) return GMC_EINVAL;
   if (old_level < 1 || old_level > 2 || new_level < 1 || new_level > 2 || old_level == new_level)
     return set_error(c, GMC_EINVAL, "old and new must be the distinct levels 1 and 2");
-  return fused_integrate(c, old_level, new_level, dt);
+  const int rc = fused_integrate(c, old_level, new_level, dt);
+  if (!rc) derived_after_step(c, old_level, new_level);
+  return rc;
 }
 
 int gmc_kernel_times(gmc_ctx *c, double dt, int32_t nsteps, double *out_ms, int32_t cap, int32_t *n) {
@@ -311,6 +321,7 @@ int gmc_kernel_times(gmc_ctx *c, double dt, int32_t nsteps, double *out_ms, int3
   c->prof_n = 0;
   for (int s = 0; s < nsteps && !rc; s++) {
     rc = fused_integrate(c, c->old_level, c->new_level, dt);
+    derived_after_step(c, c->old_level, c->new_level);
     std::swap(c->old_level, c->new_level);
   }
   c->prof_ev = nullptr;
@@ -361,6 +372,13 @@ int gmc_field_ptr(gmc_ctx *c, int32_t level, int32_t field, double **dptr, int64
   if (!d) return set_error(c, GMC_EINVAL, "bad level/field");
   *dptr = d;
   *pitch = c->grid.pitch;
+  return GMC_OK;
+}
+
+int gmc_keep_derived(gmc_ctx *c, int32_t on) {
+  if (!c) return GMC_EINVAL;
+  if (on != 0 && on != 1) return set_error(c, GMC_EINVAL, "on must be 0 or 1");
+  c->keep_derived = on;
   return GMC_OK;
 }
 

@@ -51,6 +51,11 @@ struct StagePtrs {
   double *gz_new, *du, *dv;
 };
 
+// the `order` argument of pv_edge: upwind_order_pv, or the limiter for tvd
+inline int pv_order(const gmc_config &cfg) {
+  return cfg.pv_scheme == GMC_PV_TVD ? cfg.tvd_limiter : cfg.upwind_order_pv;
+}
+
 // pv at vertex (r, i) from global memory: rows reached through the
 // cross-pole mirror next to the poles
 template <class MT, class PT>
@@ -267,7 +272,7 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double gr
                                   smfy[k - 1][q + 1], smfy[k][q], smfy[k][q + 1]);
     const double vl = tt / ((sm[k][q] + sm[k][q + 1]) * 0.5);
     const double a1 = spv[k - 2][q], a2 = spv[k - 1][q], a3 = spv[k][q], a4 = spv[k + 1][q];
-    pl = pv_scheme == GMC_PV_MIDPOINT ? 0.5 * (a3 + a2) : pv_upwind_edge(order, wgt, vl, suf[k][q], 0, a1, a2, a3, a4);
+    pl = pv_edge(pv_scheme, order, wgt, vl, suf[k][q], 0, dt, M(GMC_LE_LON, r), a1, a2, a3, a4);
     if (TAPE && out_pt) {
       tape->m_lon[r * p + ic] = (sm[k][q] + sm[k][q + 1]) * 0.5;
       tape->mfx[r * p + ic] = smfx[k][q];
@@ -283,7 +288,7 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double gr
                                   smfx[k][q], smfx[k + 1][q - 1], smfx[k + 1][q]);
     const double ul = tt / ((sm[k][q] + sm[k + 1][q]) * 0.5);
     const double a1 = spv[k][q - 2], a2 = spv[k][q - 1], a3 = spv[k][q], a4 = spv[k][q + 1];
-    pt = pv_scheme == GMC_PV_MIDPOINT ? 0.5 * (a2 + a3) : pv_upwind_edge(order, wgt, ul, svf[k][q], 1, a1, a2, a3, a4);
+    pt = pv_edge(pv_scheme, order, wgt, ul, svf[k][q], 1, dt, M(GMC_LE_LAT, r), a1, a2, a3, a4);
     if (TAPE && out_pt) {
       if (tape->mfx_lat) tape->mfx_lat[r * p + ic] = tt;
       tape->u_lat[r * p + ic] = ul;
@@ -763,7 +768,7 @@ static void fused_stage(gmc_ctx *c, int o, int s, int n, double dt, bool tape = 
   if (dst) T = *dst;
   hipLaunchKernelGGL(tape ? k_tend<true> : k_tend<false>, dim3(tiles), dim3(Tile<TY_MAIN>::NT), 0, c->stream, g, M,
                      c->d_mrows, c->cfg.g, 1.0 / c->cfg.g, dt, c->cfg.pv_pole_stokes, c->cfg.pv_scheme,
-                     c->cfg.upwind_order_pv, c->cfg.upwind_wgt_pv, c->cfg.coriolis_scheme, sp, T);
+                     pv_order(c->cfg), c->cfg.upwind_wgt_pv, c->cfg.coriolis_scheme, sp, T);
   prof_mark(c);
   hipLaunchKernelGGL(k_gzf, dim3(PL->n_gz), dim3(NF), PL->lds_gz, c->stream, g, c->big, c->small1, PL->d_items,
                      gz_new, N[S_GZ_F], N[S_GZ]);
@@ -797,23 +802,37 @@ int fused_stage_tape(gmc_ctx *c, int o, int s, int n, double dt, const TapeOut *
 int fused_integrate(gmc_ctx *c, int o, int n, double dt) {
   int rc = fused_prepare(c);
   if (rc) return rc;
+  const bool keep = c->keep_derived != 0;
   switch (c->cfg.time_scheme) {
   case GMC_TIME_EULER:
-    fused_stage(c, o, o, n, dt);
+    fused_stage(c, o, o, n, dt, keep);
     break;
   case GMC_TIME_WRFRK3:
-    fused_stage(c, o, o, n, dt / 3.0);
-    fused_stage(c, o, n, 3, dt / 2.0);
-    fused_stage(c, o, 3, n, dt);
+    fused_stage(c, o, o, n, dt / 3.0, keep);
+    fused_stage(c, o, n, 3, dt / 2.0, keep);
+    fused_stage(c, o, 3, n, dt, keep);
     break;
   default:
-    fused_stage(c, o, o, n, dt / 2.0);
-    fused_stage(c, o, n, 3, dt / 2.0);
-    fused_stage(c, o, 3, n, dt);
+    fused_stage(c, o, o, n, dt / 2.0, keep);
+    fused_stage(c, o, n, 3, dt / 2.0, keep);
+    fused_stage(c, o, 3, n, dt, keep);
   }
   rc = check_hip(c, hipGetLastError(), "fused stage");
   if (!rc && c->cfg.use_div_damp) rc = ops_div_damp(c, n);
   return rc;
+}
+
+void derived_after_step(gmc_ctx *c, int o, int n) {
+  const int v = c->keep_derived ? 1 : 0;
+  c->derived_kept[o] = v;
+  if (c->cfg.time_scheme != GMC_TIME_EULER) {
+    c->derived_kept[n] = v;
+    c->derived_kept[3] = v;
+  }
+}
+
+void derived_forget(gmc_ctx *c) {
+  for (int &k : c->derived_kept) k = 0;
 }
 
 // gmc_step: replay a captured graph of one fused step
@@ -822,7 +841,7 @@ int fused_step_graph(gmc_ctx *c, int o, int n, double dt) {
   if (rc) return rc;
   hipGraphExec_t exec = nullptr;
   for (auto &sg : c->graphs)
-    if (sg.o == o && sg.n == n && sg.dt == dt) exec = sg.exec;
+    if (sg.o == o && sg.n == n && sg.dt == dt && sg.keep == c->keep_derived) exec = sg.exec;
   if (!exec) {
     hipGraph_t graph = nullptr;
     rc = check_hip(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture");
@@ -834,7 +853,7 @@ int fused_step_graph(gmc_ctx *c, int o, int n, double dt) {
     rc = check_hip(c, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
     (void)hipGraphDestroy(graph);
     if (rc) return rc;
-    c->graphs.push_back({o, n, dt, exec});
+    c->graphs.push_back({o, n, dt, c->keep_derived, exec});
   }
   return check_hip(c, hipGraphLaunch(exec, c->stream), "graph launch");
 }
@@ -943,7 +962,7 @@ int build_tables(gmc_ens *e) {
         d.wgt = c->cfg.upwind_wgt_pv;
         d.stokes = c->cfg.pv_pole_stokes;
         d.pv_scheme = c->cfg.pv_scheme;
-        d.order = c->cfg.upwind_order_pv;
+        d.order = pv_order(c->cfg);
         d.coriolis = c->cfg.coriolis_scheme;
         d.sp = StagePtrs{S[S_GZ], c->gzs, S[S_U_F], S[S_V_F], O[S_GZ], gz_new, du, dv};
         d.up = UVPtrs{O[S_U_LON], O[S_V_LAT], du, dv, N[S_GZ_F], N[S_U_LON], N[S_V_LAT], N[S_U_F], N[S_V_F]};
@@ -1000,6 +1019,12 @@ int ens_graph(gmc_ens *e, int par, hipGraphExec_t *out) {
 
 int ens_run(gmc_ens *e, const double *dt, int nsteps) {
   const int nm = (int)e->m.size();
+  for (gmc_ctx *c : e->m) {
+    // the batch steps every member from member 0's time level
+    if (c->old_level != e->m[0]->old_level)
+      return eerr(e, GMC_ESTATE, "members are at different time levels (one was stepped on its own)");
+  }
+  for (gmc_ctx *c : e->m) derived_forget(c);
   bool same = e->built_shape == e->shape;
   for (int k = 0; k < nm && same; k++) same = e->dt[k] == dt[k];
   if (!same) {

@@ -25,7 +25,7 @@ constexpr int kWave = 64;
 enum StateField {
   S_U_LON, S_V_LAT, S_GZ, S_GZ_F, S_U_F, S_V_F, S_M, S_M_LON, S_M_LAT, S_M_VTX,
   S_MFX_LON, S_MFY_LAT, S_MFX_LAT, S_MFY_LON, S_U_LAT, S_V_LON, S_KE, S_DIV,
-  S_VOR, S_PV, S_PV_LON, S_PV_LAT, S_NFIELDS
+  S_VOR, S_PV, S_PV_LON, S_PV_LAT, S_DIV2, S_NFIELDS
 };
 // tend_type fields (tend_mod.F90:16-42)
 enum TendField {
@@ -153,9 +153,15 @@ struct gmc_ctx {
   gmc::HostStage stage;
   int old_level = 1, new_level = 2;
   int prepared = 0;
+  // derived fields: with keep_derived, every fused stage also stores the
+  // star state's forward-pass fields in the star level (as the reference's
+  // operators leave them in state(star)); derived_kept[l] says level l holds
+  // the fields its last use as a star state (or gmc_prepare) left there
+  int keep_derived = 0;
+  int derived_kept[4] = {0, 0, 0, 0};
   void *fused_plan = nullptr;  // work lists of the fused stage (gmc_fused.hip)
-  // graphs of one fused step, keyed by (old, new, dt)
-  struct StepGraph { int o, n; double dt; hipGraphExec_t exec; };
+  // graphs of one fused step, keyed by (old, new, dt, keep_derived)
+  struct StepGraph { int o, n; double dt; int keep; hipGraphExec_t exec; };
   std::vector<StepGraph> graphs;
   // kernel timing: when non-null, fused_stage records an event after every launch
   hipEvent_t *prof_ev = nullptr;
@@ -181,6 +187,11 @@ int fused_integrate(gmc_ctx *ctx, int old_l, int new_l, double dt);
 void fused_release(gmc_ctx *ctx);
 int fused_prepare(gmc_ctx *ctx);
 int fused_plan_info(gmc_ctx *ctx, int32_t out[4]);
+// host bookkeeping of derived_kept after one time step old -> new of the
+// configured scheme (the star levels are old, new and 3; old alone for euler)
+void derived_after_step(gmc_ctx *ctx, int old_l, int new_l);
+// a path that steps without keeping derived fields (ensemble, TL/AD)
+void derived_forget(gmc_ctx *ctx);
 int fused_step_graph(gmc_ctx *ctx, int old_l, int new_l, double dt);
 // Destinations of the intermediates a taping stage writes for the adjoint:
 // the fields operators_prepare forms from the star state (mfx_lat and

@@ -130,27 +130,26 @@ __global__ void k_calc_pv(Grid g, Mesh M, int pole_stokes, const double *__restr
   pv[r * p + i] = pv_of(w, M, r, m_vtx[r * p + i]);
 }
 
-// interp_pv_upwind / interp_pv_midpoint, operators_mod.F90:705-795
-__global__ void k_interp_pv(Grid g, int scheme, int order, double wgt, const double *__restrict__ pv,
-                            const double *__restrict__ u_f, const double *__restrict__ v_f,
-                            const double *__restrict__ u_lat, const double *__restrict__ v_lon,
-                            double *pv_lon, double *pv_lat) {
+// interp_pv_upwind / interp_pv_midpoint / interp_pv_tvd, operators_mod.F90:705-832
+// (order: upwind_order_pv, or the tvd limiter; dt: the stage dt)
+__global__ void k_interp_pv(Grid g, Mesh M, int scheme, int order, double wgt, double dt,
+                            const double *__restrict__ pv, const double *__restrict__ u_f,
+                            const double *__restrict__ v_f, const double *__restrict__ u_lat,
+                            const double *__restrict__ v_lon, double *pv_lon, double *pv_lat) {
   const int i = blockIdx.x * BX + threadIdx.x, r = blockIdx.y * BY + threadIdx.y;
   if (i >= g.nlon || r >= g.nlat) return;
   const int64_t p = g.pitch;
   auto PV = [&](int rr, int ii) { mirror_half(rr, ii, g); return pv[rr * p + ii]; };
   if (r >= 1 && r <= g.nlat - 2) {
     const double f1 = PV(r - 2, i), f2 = PV(r - 1, i), f3 = PV(r, i), f4 = PV(r + 1, i);
-    pv_lon[r * p + i] = scheme == GMC_PV_MIDPOINT
-        ? 0.5 * (f3 + f2)
-        : pv_upwind_edge(order, wgt, v_lon[r * p + i], u_f[r * p + i], 0, f1, f2, f3, f4);
+    pv_lon[r * p + i] = pv_edge(scheme, order, wgt, v_lon[r * p + i], u_f[r * p + i], 0, dt, M(GMC_LE_LON, r),
+                                f1, f2, f3, f4);
   }
   if (r <= g.nhalf - 1) {
     const double f1 = pv[r * p + wrap(i - 2, g.nlon)], f2 = pv[r * p + wrap(i - 1, g.nlon)];
     const double f3 = pv[r * p + i], f4 = pv[r * p + wrap(i + 1, g.nlon)];
-    pv_lat[r * p + i] = scheme == GMC_PV_MIDPOINT
-        ? 0.5 * (f2 + f3)
-        : pv_upwind_edge(order, wgt, u_lat[r * p + i], v_f[r * p + i], 1, f1, f2, f3, f4);
+    pv_lat[r * p + i] = pv_edge(scheme, order, wgt, u_lat[r * p + i], v_f[r * p + i], 1, dt, M(GMC_LE_LAT, r),
+                                f1, f2, f3, f4);
   }
 }
 
@@ -343,7 +342,8 @@ static TendPtrs tend_ptrs(double **T) {
                   T[T_DMFDLON], T[T_DMFDLAT], T[T_PGF_LON], T[T_PGF_LAT]};
 }
 
-static void forward_prepare(gmc_ctx *c, double **S) {
+// operators_prepare_2's forward pass on state S with the stage dt
+static void forward_prepare(gmc_ctx *c, double **S, double dt) {
   const Grid &g = c->grid;
   const Mesh M = mesh_of(c);
   const dim3 b(BX, BY);
@@ -358,9 +358,10 @@ static void forward_prepare(gmc_ctx *c, double **S) {
                      S[S_DIV]);
   hipLaunchKernelGGL(k_calc_pv, grid2d(g, g.nhalf), b, 0, c->stream, g, M, c->cfg.pv_pole_stokes,
                      S[S_U_F], S[S_V_F], S[S_M_VTX], c->d_pole, S[S_VOR], S[S_PV]);
-  hipLaunchKernelGGL(k_interp_pv, grid2d(g, g.nlat), b, 0, c->stream, g, c->cfg.pv_scheme,
-                     c->cfg.upwind_order_pv, c->cfg.upwind_wgt_pv, S[S_PV], S[S_U_F], S[S_V_F],
-                     S[S_U_LAT], S[S_V_LON], S[S_PV_LON], S[S_PV_LAT]);
+  const int order = c->cfg.pv_scheme == GMC_PV_TVD ? c->cfg.tvd_limiter : c->cfg.upwind_order_pv;
+  hipLaunchKernelGGL(k_interp_pv, grid2d(g, g.nlat), b, 0, c->stream, g, M, c->cfg.pv_scheme, order,
+                     c->cfg.upwind_wgt_pv, dt, S[S_PV], S[S_U_F], S[S_V_F], S[S_U_LAT], S[S_V_LON],
+                     S[S_PV_LON], S[S_PV_LAT]);
 }
 
 static void calc_m(gmc_ctx *c, double **S) {
@@ -375,19 +376,19 @@ int ops_prepare(gmc_ctx *c, int level) {
   filter_field(c, c->big, 1, S[S_V_LAT], S[S_V_F]);
   filter_field(c, c->big, 0, S[S_GZ], S[S_GZ_F]);
   calc_m(c, S);
-  forward_prepare(c, S);
+  forward_prepare(c, S, c->cfg.dt_dyn);   // operators_prepare_1 with dt_dyn (gmcore_mod.F90:133)
   return check_hip(c, hipGetLastError(), "prepare");
 }
 
 int ops_space_operators(gmc_ctx *c, int old_l, int star_l, int new_l, int t1, int t2, double dt, int pass) {
-  (void)old_l; (void)dt;
+  (void)old_l;
   const Grid &g = c->grid;
   const Mesh M = mesh_of(c);
   const dim3 b(BX, BY);
   double **S = c->state[star_l], **N = c->state[new_l], **T1 = c->tend[t1], **T2 = c->tend[t2];
   if (pass == 0 || pass == 1) {
     if (pass == 0) calc_m(c, S);
-    forward_prepare(c, S);
+    forward_prepare(c, S, dt);
     hipLaunchKernelGGL(k_grad_mf, grid2d(g, g.nlat), b, 0, c->stream, g, M, S[S_MFX_LON], S[S_MFY_LAT],
                        c->d_pole, T1[T_DMFDLON], T1[T_DMFDLAT]);
     hipLaunchKernelGGL(k_coriolis, grid2d(g, g.nlat), b, 0, c->stream, g, M, c->cfg.coriolis_scheme,
@@ -450,18 +451,40 @@ int ops_stage_tape(gmc_ctx *c, int o, int s, int n, double dt) {
 
 // div_damp_run order 2 (div_damp_mod.F90:89-139)
 namespace {
-__global__ void k_div_damp(Grid g, Mesh M, double coef2, const double *__restrict__ div, double *u, double *v) {
+// div2 of calc_div for div_damp_order 4 (operators_mod.F90:414-428): the
+// Laplacian of div on the non-pole rows; the pole rows of div2 stay zero.
+__global__ void k_calc_div2(Grid g, Mesh M, const double *__restrict__ div, double *div2) {
+  const int i = blockIdx.x * BX + threadIdx.x, r = blockIdx.y * BY + threadIdx.y;
+  if (i >= g.nlon || r < 1 || r > g.nlat - 2) return;
+  const int64_t p = g.pitch, o = r * p + i;
+  const int ip = wrap(i + 1, g.nlon), im = wrap(i - 1, g.nlon);
+  const double dl = M(GMC_DE_LON, r);
+  div2[o] = (div[r * p + ip] - 2 * div[o] + div[r * p + im]) / (dl * dl) +
+            ((div[o + p] - div[o]) * M(GMC_HALF_COS_LAT, r) / M(GMC_DE_LAT, r) -
+             (div[o] - div[o - p]) * M(GMC_HALF_COS_LAT, r - 1) / M(GMC_DE_LAT, r - 1)) /
+                M(GMC_LE_LON, r) / M(GMC_FULL_COS_LAT, r);
+}
+
+// div_damp_run (div_damp_mod.F90:89-139) with the c_lon / c_lat of
+// div_damp_init (:27-80): order 2 adds the gradient of div, order 4
+// subtracts the gradient of div2 (d is div or div2).
+__global__ void k_div_damp(Grid g, Mesh M, int order, double coef, const double *__restrict__ d, double *u,
+                           double *v) {
   const int i = blockIdx.x * BX + threadIdx.x, r = blockIdx.y * BY + threadIdx.y;
   if (i >= g.nlon || r >= g.nlat) return;
   const int64_t p = g.pitch, o = r * p + i;
   const int ip = wrap(i + 1, g.nlon);
   if (r >= 1 && r <= g.nlat - 2) {
-    const double c = coef2 * M(GMC_LE_LON, r) * M(GMC_DE_LON, r);
-    u[o] = u[o] + c * (div[r * p + ip] - div[o]) / M(GMC_DE_LON, r);
+    const double le = M(GMC_LE_LON, r), de = M(GMC_DE_LON, r);
+    const double c = order == 4 ? coef * (le * le) * (de * de) : coef * le * de;
+    const double t = c * (d[r * p + ip] - d[o]) / de;
+    u[o] = order == 4 ? u[o] - t : u[o] + t;
   }
   if (r <= g.nhalf - 1) {
-    const double c = coef2 * M(GMC_LE_LAT, r) * M(GMC_DE_LAT, r);
-    v[o] = v[o] + c * (div[o + p] - div[o]) / M(GMC_DE_LAT, r);
+    const double le = M(GMC_LE_LAT, r), de = M(GMC_DE_LAT, r);
+    const double c = order == 4 ? coef * (le * le) * (de * de) : coef * le * de;
+    const double t = c * (d[o + p] - d[o]) / de;
+    v[o] = order == 4 ? v[o] - t : v[o] + t;
   }
 }
 }  // namespace
@@ -476,7 +499,10 @@ int ops_div_damp(gmc_ctx *c, int level) {
                      S[S_U_LAT], c->d_pole);
   hipLaunchKernelGGL(k_calc_div, grid2d(g, g.nlat), b, 0, c->stream, g, M, S[S_U_F], S[S_V_F], c->d_pole,
                      S[S_DIV]);
-  hipLaunchKernelGGL(k_div_damp, grid2d(g, g.nlat), b, 0, c->stream, g, M, c->cfg.div_damp_coef2, S[S_DIV],
+  const int o4 = c->cfg.div_damp_order == 4;
+  if (o4) hipLaunchKernelGGL(k_calc_div2, grid2d(g, g.nlat), b, 0, c->stream, g, M, S[S_DIV], S[S_DIV2]);
+  hipLaunchKernelGGL(k_div_damp, grid2d(g, g.nlat), b, 0, c->stream, g, M, o4 ? 4 : 2,
+                     o4 ? c->cfg.div_damp_coef4 : c->cfg.div_damp_coef2, o4 ? S[S_DIV2] : S[S_DIV],
                      S[S_U_LON], S[S_V_LAT]);
   return check_hip(c, hipGetLastError(), "div_damp");
 }
@@ -484,8 +510,12 @@ int ops_div_damp(gmc_ctx *c, int level) {
 int ops_diagnose(gmc_ctx *c, int level, double out[4]) {
   const Grid &g = c->grid;
   double **S = c->state[level];
+  // m, m_lon, m_lat as the backward pass's calc_m left them on the current
+  // gz; the forward-pass fields as the level's last use as a star state left
+  // them (the reference reads them as stored, gmcore_mod.F90:230-313), or,
+  // when the level does not hold them, formed now from its own state
   calc_m(c, S);
-  forward_prepare(c, S);
+  if (!c->derived_kept[level]) forward_prepare(c, S, c->cfg.dt_dyn);
   hipLaunchKernelGGL(k_diag_rows, dim3(g.nlat), dim3(kWave), 0, c->stream, g, mesh_of(c), c->cfg.g, S[S_M],
                      c->gzs, S[S_MFX_LON], S[S_U_LON], S[S_MFY_LAT], S[S_V_LAT], S[S_PV], S[S_M_LON],
                      S[S_PV_LON], S[S_M_LAT], S[S_PV_LAT], c->d_diag);

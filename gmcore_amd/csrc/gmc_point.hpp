@@ -73,6 +73,31 @@ __device__ __forceinline__ double pv_upwind_edge(int order, double wgt, double t
   return b * up + (1 - b) * 0.5 * (f2 + f3);
 }
 
+// tvd (tvd_mod.F90:42-61) with the flux limiter `lim` (GMC_TVD_*: none,
+// van_leer, mc; tvd_mod.F90:63-100): the pv of interp_pv_tvd
+// (operators_mod.F90:797-832) at an edge with Courant number cfl, from pv
+// fm1, f, fp1, fp2 along the stencil.  eps is the default-real literal
+// 1.0e-6 widened to r8, as in the reference.
+__device__ __forceinline__ double tvd_edge(int lim, double cfl, double fm1, double f, double fp1, double fp2) {
+  const double eps = (double)1.0e-6f;
+  const double r = cfl > 0 ? (f - fm1 + eps) / (fp1 - f + eps) : (fp2 - fp1 + eps) / (fp1 - f + eps);
+  double C;
+  if (lim == GMC_TVD_NONE) C = 1;
+  else if (lim == GMC_TVD_VAN_LEER) C = (r + fabs(r)) / (1 + fabs(r));
+  else C = fmax(0.0, fmin(fmin(2.0 * r, 0.5 * (1 + r)), 2.0));
+  return 0.5 * (fp1 + f - ((1 - C) * copysign(1.0, cfl) + cfl * C) * (fp1 - f));
+}
+
+// pv on an edge for pv_scheme `scheme`.  `order` is upwind_order_pv for the
+// upwind scheme and the tvd limiter for tvd; dt_le = dt / le of the edge's
+// row (tvd's Courant number t * dt / le is formed as (t * dt) / le).
+__device__ __forceinline__ double pv_edge(int scheme, int order, double wgt, double t, double n, int lat_edge,
+                                          double dt, double le, double f1, double f2, double f3, double f4) {
+  if (scheme == GMC_PV_MIDPOINT) return 0.5 * (f3 + f2);
+  if (scheme == GMC_PV_TVD) return tvd_edge(order, t * dt / le, f1, f2, f3, f4);
+  return pv_upwind_edge(order, wgt, t, n, lat_edge, f1, f2, f3, f4);
+}
+
 // Derivatives of pv_upwind_edge with respect to t, n and the four pv values
 // on the branch the trajectory took (the tangent-linear and adjoint of
 // interp_pv_upwind; |t| and sign(t) differentiated piecewise): dt, dn, df[4].

@@ -1003,7 +1003,8 @@ int gmc_tlad_create(gmc_ctx *c, gmc_tlad **out) {
   if (!out) return GMC_EINVAL;
   *out = nullptr;
   if (!c || !c->prepared) return GMC_ESTATE;
-  if (c->cfg.use_div_damp) return GMC_EINVAL;
+  // no tangent linear of divergence damping or of the tvd limiter
+  if (c->cfg.use_div_damp || c->cfg.pv_scheme == GMC_PV_TVD) return GMC_EINVAL;
   gmc_tlad *a = new gmc_tlad();
   a->c = c;
   const Grid &g = c->grid;
@@ -1093,6 +1094,7 @@ int gmc_tl_step(gmc_tlad *a, double dt, int32_t nsteps) {
   if (!a) return GMC_EINVAL;
   if (nsteps < 0 || !(dt > 0)) return terr(a, GMC_EINVAL, "bad step arguments");
   gmc_ctx *c = a->c;
+  derived_forget(c);
   int rc = GMC_OK;
   for (int s = 0; s < nsteps && !rc; s++) {
     const int o = c->old_level, n = c->new_level;
@@ -1354,6 +1356,7 @@ int gmc_ad_step(gmc_tlad *a, double dt, int32_t nsteps) {
   if (!a) return GMC_EINVAL;
   if (nsteps < 0 || !(dt > 0)) return terr(a, GMC_EINVAL, "bad step arguments");
   gmc_ctx *c = a->c;
+  derived_forget(c);
   const int o = c->old_level, n = c->new_level;
   int rc = fused_prepare(c);
   // the adjoint enters on u, v, gz only
@@ -1378,6 +1381,7 @@ int gmc_gradient(gmc_tlad *a, double dt, int32_t nsteps, int32_t ckpt_every, int
                  double sigma_b, int64_t ld, double *cost, double *grad_u, double *grad_v, double *grad_gz) {
   if (!a) return GMC_EINVAL;
   gmc_ctx *c = a->c;
+  derived_forget(c);
   const Grid &g = c->grid;
   if (nsteps < 1 || ckpt_every < 1 || nobs < 0 || (nobs && (!obs_cell || !obs_value)) || !(sigma_o > 0) ||
       !(sigma_b > 0) || !ub || !vb || !gzb || ld < g.nlon || !cost || !grad_u || !grad_v || !grad_gz || !(dt > 0))

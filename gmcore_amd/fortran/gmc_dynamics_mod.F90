@@ -86,7 +86,17 @@ contains
     case default;    block%time_scheme = GMC_TIME_PC2
     end select
     cfg%time_scheme = block%time_scheme
-    cfg%pv_scheme = merge(GMC_PV_MIDPOINT, GMC_PV_UPWIND, pv_scheme == 'midpoint')
+    select case (pv_scheme)
+    case ('midpoint'); cfg%pv_scheme = GMC_PV_MIDPOINT
+    case ('tvd');      cfg%pv_scheme = GMC_PV_TVD
+    case default;      cfg%pv_scheme = GMC_PV_UPWIND
+    end select
+    select case (tvd_limiter_type)
+    case ('none'); cfg%tvd_limiter = GMC_TVD_NONE
+    case ('mc');   cfg%tvd_limiter = GMC_TVD_MC
+    case default;  cfg%tvd_limiter = GMC_TVD_VAN_LEER
+    end select
+    cfg%reserved = 0
     cfg%upwind_order_pv = upwind_order_pv
     cfg%upwind_wgt_pv = upwind_wgt_pv
     cfg%pv_pole_stokes = merge(1, 0, pv_pole_stokes)
@@ -94,6 +104,8 @@ contains
     cfg%use_div_damp = merge(1, 0, use_div_damp)
     cfg%div_damp_order = div_damp_order
     cfg%div_damp_coef2 = div_damp_coef2
+    cfg%div_damp_coef4 = div_damp_coef4
+    cfg%dt_dyn = dt_dyn
     cfg%big_filter = gmc_filter(maxn, c_loc(block%ng(1,1)), c_loc(block%wbig_lon), &
                                 c_loc(block%ng(1,2)), c_loc(block%wbig_lat))
     cfg%small_filter1 = gmc_filter(maxn, c_loc(block%ng(1,3)), c_loc(block%wsml_lon), &

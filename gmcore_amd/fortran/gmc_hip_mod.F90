@@ -12,7 +12,8 @@ module gmc_hip_mod
     GMC_V_F = 4, GMC_GZ_F = 5, GMC_GZS = 6, GMC_DU = 7, GMC_DV = 8, GMC_DGZ = 9, GMC_PV = 10, &
     GMC_KE = 11, GMC_DIV = 12, GMC_M = 13
   integer(c_int32_t), parameter :: GMC_TIME_PC2 = 0, GMC_TIME_WRFRK3 = 1, GMC_TIME_EULER = 2
-  integer(c_int32_t), parameter :: GMC_PV_MIDPOINT = 0, GMC_PV_UPWIND = 1
+  integer(c_int32_t), parameter :: GMC_PV_MIDPOINT = 0, GMC_PV_UPWIND = 1, GMC_PV_TVD = 2
+  integer(c_int32_t), parameter :: GMC_TVD_NONE = 0, GMC_TVD_VAN_LEER = 1, GMC_TVD_MC = 2
 
   type, bind(C) :: gmc_mesh
     integer(c_int32_t) :: nlon, nlat
@@ -32,6 +33,8 @@ module gmc_hip_mod
     integer(c_int32_t) :: pv_pole_stokes, coriolis_scheme, use_div_damp, div_damp_order
     real(c_double) :: div_damp_coef2
     type(gmc_filter) :: big_filter, small_filter1
+    real(c_double) :: div_damp_coef4, dt_dyn
+    integer(c_int32_t) :: tvd_limiter, reserved
   end type gmc_config
 
   integer(c_int32_t), parameter :: GMC_LIMITER_MONO = 0, GMC_LIMITER_PD = 1, GMC_LIMITER_NONE = 2

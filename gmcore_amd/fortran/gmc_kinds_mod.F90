@@ -43,6 +43,8 @@ module gmc_kinds_mod
   logical  :: use_div_damp        = .false.
   integer  :: div_damp_order      = 2
   real(r8) :: div_damp_coef2      = 1.0_r8 / 128.0_r8
+  real(r8) :: div_damp_coef4      = 0.01_r8
+  character(30) :: tvd_limiter_type = 'van_leer'
   character(256) :: initial_file  = 'N/A'
   ! tracer advection (namelist_mod.F90:38, 68-69)
   integer  :: lon_halo_width      = 2
@@ -52,8 +54,8 @@ module gmc_kinds_mod
   namelist /gmcore_control/ num_lon, num_lat, num_lev, dt_dyn, dt_adv, run_steps, &
     time_scheme, pv_scheme, pv_pole_stokes, upwind_order_pv, upwind_wgt_pv, coriolis_scheme, &
     max_wave_speed, max_cfl, filter_coef_a, filter_coef_b, filter_coef_c, filter_coef_d, &
-    use_div_damp, div_damp_order, div_damp_coef2, initial_file, lon_halo_width, limiter_type, &
-    ffsl_flux_type
+    use_div_damp, div_damp_order, div_damp_coef2, div_damp_coef4, tvd_limiter_type, initial_file, &
+    lon_halo_width, limiter_type, ffsl_flux_type
 
 contains
 

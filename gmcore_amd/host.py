@@ -84,13 +84,19 @@ class Filter:
 
 def make_config(nl: Namelist, big: Filter, small: Filter) -> GmcConfig:
     ts = {"pc2": capi.GMC_TIME_PC2, "wrfrk3": capi.GMC_TIME_WRFRK3, "euler": capi.GMC_TIME_EULER}
+    if nl.pv_scheme not in capi.PV_SCHEMES:
+        raise ValueError(f"pv_scheme {nl.pv_scheme!r} is not built (midpoint, upwind, tvd)")
+    if nl.tvd_limiter_type not in capi.TVD_LIMITERS:
+        raise ValueError(f"Invalid tvd_limiter_type {nl.tvd_limiter_type!r}")
     return GmcConfig(
         g=nl.g, time_scheme=ts.get(nl.time_scheme, capi.GMC_TIME_PC2),
-        pv_scheme=capi.GMC_PV_MIDPOINT if nl.pv_scheme == "midpoint" else capi.GMC_PV_UPWIND,
+        pv_scheme=capi.PV_SCHEMES[nl.pv_scheme],
         upwind_order_pv=nl.upwind_order_pv, upwind_wgt_pv=nl.upwind_wgt_pv,
         pv_pole_stokes=int(nl.pv_pole_stokes), coriolis_scheme=nl.coriolis_scheme,
         use_div_damp=int(nl.use_div_damp), div_damp_order=nl.div_damp_order,
-        div_damp_coef2=nl.div_damp_coef2, big_filter=big.struct(), small_filter1=small.struct())
+        div_damp_coef2=nl.div_damp_coef2, big_filter=big.struct(), small_filter1=small.struct(),
+        div_damp_coef4=nl.div_damp_coef4, dt_dyn=nl.dt_dyn,
+        tvd_limiter=capi.TVD_LIMITERS[nl.tvd_limiter_type], reserved=0)
 
 
 class Setup:
@@ -191,6 +197,10 @@ class ShallowWater:
 
     def synchronize(self):
         self._check(self.lib.gmc_synchronize(self.ctx), "synchronize")
+
+    def keep_derived(self, on: bool = True):
+        """Store each stage's forward-pass fields in the star level (gmc_keep_derived)."""
+        self._check(self.lib.gmc_keep_derived(self.ctx, int(on)), "keep_derived")
 
     def filter_plan(self) -> dict:
         """Shape of the fused step's filter work list (gmc_filter_plan)."""

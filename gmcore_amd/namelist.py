@@ -50,6 +50,8 @@ class Namelist:
     use_div_damp: bool = False           # :100
     div_damp_order: int = 2              # :101
     div_damp_coef2: float = 1.0 / 128.0  # :107
+    div_damp_coef4: float = 0.01         # :108
+    tvd_limiter_type: str = "van_leer"   # :70
     lon_halo_width: int = 2              # :38
     advection: bool = False
     limiter_type: str = "mono"           # :68

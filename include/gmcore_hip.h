@@ -86,7 +86,9 @@ typedef struct gmc_filter {
 /* scheme switches: namelist keys of &gmcore_control
  * (src/utils/namelist_mod.F90:10-122) that select code on the SW path */
 enum { GMC_TIME_PC2 = 0, GMC_TIME_WRFRK3 = 1, GMC_TIME_EULER = 2 };
-enum { GMC_PV_MIDPOINT = 0, GMC_PV_UPWIND = 1 };
+enum { GMC_PV_MIDPOINT = 0, GMC_PV_UPWIND = 1, GMC_PV_TVD = 2 };
+/* tvd_limiter_type (tvd_mod.F90:27-37) */
+enum { GMC_TVD_NONE = 0, GMC_TVD_VAN_LEER = 1, GMC_TVD_MC = 2 };
 
 typedef struct gmc_config {
   double g;                  /* gravity, const_mod.F90:61                     */
@@ -97,10 +99,15 @@ typedef struct gmc_config {
   int32_t pv_pole_stokes;    /* 0/1          (operators_mod.F90:640-674)      */
   int32_t coriolis_scheme;   /* 1 or 2       (operators_mod.F90:846-906)      */
   int32_t use_div_damp;      /* 0/1          (damp_mod.F90:56-58)             */
-  int32_t div_damp_order;    /* 2 only on this path (div_damp_mod.F90:89-139) */
+  int32_t div_damp_order;    /* 2 or 4       (div_damp_mod.F90:27-139)        */
   double div_damp_coef2;
   gmc_filter big_filter;     /* block%big_filter    (block_mod.F90:65)        */
   gmc_filter small_filter1;  /* block%small_filter1 (block_mod.F90:66)        */
+  double div_damp_coef4;     /* order-4 coefficient (namelist_mod.F90:108)    */
+  double dt_dyn;             /* the dt operators_prepare_1 runs with
+                                (gmcore_mod.F90:133); read by pv_scheme tvd  */
+  int32_t tvd_limiter;       /* GMC_TVD_*, for pv_scheme GMC_PV_TVD           */
+  int32_t reserved;          /* 0                                            */
 } gmc_config;
 
 /* ---- fields of state_type / tend_type that cross the boundary ----------- */
@@ -170,8 +177,26 @@ int gmc_step(gmc_ctx *ctx, double dt, int32_t nsteps);
 int gmc_current_level(const gmc_ctx *ctx, int32_t *old_level);
 
 /* diagnose (gmcore_mod.F90:213-341, SW): out[0] = tm, out[1] = te,
- * out[2] = tpe, out[3] = tav, computed from operators_prepare on `level`. */
+ * out[2] = tpe, out[3] = tav.  m, m_lon and m_lat are those of the level's
+ * current gz (what the backward pass's calc_m leaves).  The reference reads
+ * the forward-pass fields (mfx_lon, mfy_lat, pv, pv_lon, pv_lat) as they are
+ * stored in state(itime): after a pc2 or wrfrk3 step those of the second
+ * stage's star state, after a euler step those of the step before.  When the
+ * level holds them (gmc_prepare, gmc_space_operators, or steps taken with
+ * gmc_keep_derived on), they are used as stored and te, tpe and tav are the
+ * reference's values; otherwise they are formed from the level's own state
+ * first (te, tpe and tav of the current fields). */
 int gmc_diagnose(gmc_ctx *ctx, int32_t level, double out[4]);
+
+/* on = 1: every stage of gmc_step, gmc_time_integrate and gmc_kernel_times
+ * also stores the star state's forward-pass fields (m_lon, m_lat, m_vtx,
+ * mfx_lon, mfy_lat, mfx_lat, mfy_lon, u_lat, v_lon, pv, pv_lon, pv_lat) in
+ * the star level, as operators_prepare_2 leaves them in state(star)
+ * (operators_mod.F90:117-122), so that gmc_diagnose reads what the
+ * reference's diagnose reads.  Costs about twelve field writes per stage; a
+ * host loop turns it on for the step before a diagnostics print.  on = 0
+ * (the default) skips those writes. */
+int gmc_keep_derived(gmc_ctx *ctx, int32_t on);
 
 /* Wait for all work queued on the context's stream. */
 int gmc_synchronize(gmc_ctx *ctx);

@@ -51,6 +51,10 @@ def lib() -> C.CDLL:
         L.gmo_sw_step.argtypes = [vp, C.c_double, C.c_int]
         L.gmo_sw_current_level.argtypes = [vp]
         L.gmo_sw_diagnose.argtypes = [vp, C.c_int, P(C.c_double)]
+        L.gmo_sw_diagnose_fresh.argtypes = [vp, C.c_int, P(C.c_double)]
+        L.gmo_sw_div_damp.argtypes = [vp, C.c_int]
+        L.gmo_tvd.restype = C.c_double
+        L.gmo_tvd.argtypes = [C.c_int] + [C.c_double] * 5
         L.gmo_zonal_sum.restype = C.c_double
         L.gmo_zonal_sum.argtypes = [P(C.c_double), C.c_int, C.c_int]
         L.gmo_lon_damp_weights.argtypes = [C.c_int, P(C.c_double), P(C.c_double)]
@@ -141,7 +145,7 @@ class SWOracle:
 
     STATE_FIELDS = ["u_lon", "v_lat", "gz", "gz_f", "u_f", "v_f", "m", "m_lon", "m_lat", "m_vtx",
                     "mfx_lon", "mfy_lat", "mfx_lat", "mfy_lon", "u_lat", "v_lon", "ke", "div", "vor",
-                    "pv", "pv_lon", "pv_lat"]
+                    "pv", "pv_lon", "pv_lat", "div2"]
     STATE_HALF = {"v_lat", "v_f", "m_lat", "m_vtx", "mfy_lat", "mfx_lat", "u_lat", "vor", "pv", "pv_lat"}
     TEND_FIELDS = ["du", "dv", "dgz", "qhu", "qhv", "dkedlon", "dkedlat", "dmfdlon", "dmfdlat",
                    "pgf_lon", "pgf_lat"]
@@ -167,14 +171,28 @@ class SWOracle:
     def step(self, dt: float, nsteps: int = 1):
         lib().gmo_sw_step(self.h, dt, nsteps)
 
+    def div_damp(self, level: int):
+        """damp_run's divergence damping alone on the level."""
+        assert lib().gmo_sw_div_damp(self.h, level) == 0
+
     @property
     def level(self) -> int:
         return lib().gmo_sw_current_level(self.h)
 
-    def diagnose(self, level=None):
+    def diagnose(self, level=None, fresh=False):
+        """(tm, te, tpe, tav) over the fields stored in the level, as the
+        reference's diagnose reads them; fresh=True forms the forward-pass
+        fields of the level's own state first."""
         out = np.zeros(4)
-        lib().gmo_sw_diagnose(self.h, self.level if level is None else level, dptr(out))
+        fn = lib().gmo_sw_diagnose_fresh if fresh else lib().gmo_sw_diagnose
+        fn(self.h, self.level if level is None else level, dptr(out))
         return out
+
+
+def tvd(limiter: str, cfl: float, fm1: float, f: float, fp1: float, fp2: float) -> float:
+    """tvd (tvd_mod.F90:42-61) with tvd_limiter_type `limiter`."""
+    from gmcore_amd.capi import TVD_LIMITERS
+    return lib().gmo_tvd(TVD_LIMITERS[limiter], cfl, fm1, f, fp1, fp2)
 
 
 def lon_damp_weights(nlat: int, metrics: np.ndarray) -> np.ndarray:

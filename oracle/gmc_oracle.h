@@ -67,8 +67,14 @@ int gmo_sw_update_state(gmo_sw *s, int tend, int old_level, int new_level,
 /* time_integrator + damp_run + swap, nsteps times */
 int gmo_sw_step(gmo_sw *s, double dt, int nsteps);
 int gmo_sw_current_level(const gmo_sw *s);
-/* diagnose (gmcore_mod.F90:213-341) after operators_prepare on level */
+/* diagnose (gmcore_mod.F90:213-341) over the fields stored in state(level) */
 int gmo_sw_diagnose(gmo_sw *s, int level, double out[4]);
+/* the same sums after operators_prepare on state(level) */
+int gmo_sw_diagnose_fresh(gmo_sw *s, int level, double out[4]);
+/* damp_run's divergence damping alone on state(level) (order 2 or 4) */
+int gmo_sw_div_damp(gmo_sw *s, int level);
+/* tvd (tvd_mod.F90:42-61) with limiter GMC_TVD_* */
+double gmo_tvd(int lim, double cfl, double fm1, double f, double fp1, double fp2);
 
 /* Pole sum in the fixed order the library uses: 64 strided partial sums
  * (partial p adds x[p], x[p+64], ... in ascending order) reduced by a

@@ -23,7 +23,7 @@ enum { KF = 0, KH = 1 };  /* full-lat / half-lat row grid */
 enum {
   S_U_LON, S_V_LAT, S_GZ, S_GZ_F, S_U_F, S_V_F, S_M, S_M_LON, S_M_LAT, S_M_VTX,
   S_MFX_LON, S_MFY_LAT, S_MFX_LAT, S_MFY_LON, S_U_LAT, S_V_LON, S_KE, S_DIV,
-  S_VOR, S_PV, S_PV_LON, S_PV_LAT, S_N
+  S_VOR, S_PV, S_PV_LON, S_PV_LAT, S_DIV2, S_N
 };
 /* tend_type fields (tend_mod.F90:16-42) */
 enum {
@@ -221,7 +221,8 @@ static void calc_ke(gmo_sw *s, double **S) {
   for (int i = 1; i <= s->nlon; i++) A(s, ke, i, s->nlat) = pole;
 }
 
-/* calc_div, operators_mod.F90:354-412 (div_damp_order 2) */
+/* calc_div, operators_mod.F90:354-431; div2 (the Laplacian of div on the
+ * non-pole rows) when div_damp_order is 4 */
 static void calc_div(gmo_sw *s, double **S) {
   double *u = S[S_U_F], *v = S[S_V_F], *div = S[S_DIV];
   for (int j = 2; j <= s->nlat - 1; j++)
@@ -235,7 +236,22 @@ static void calc_div(gmo_sw *s, double **S) {
   pole = row_sum(s, v, s->nhalf, -1.0, NULL);
   pole = pole * MS(s, GMC_LE_LAT, s->nhalf) / s->nlon / MS(s, GMC_AREA_CELL, s->nlat);
   for (int i = 1; i <= s->nlon; i++) A(s, div, i, s->nlat) = pole;
-  fill_halo(s, div, KF, 0, 1, 0, 1);
+  if (s->cfg.div_damp_order != 4) {
+    fill_halo(s, div, KF, 0, 1, 0, 1);
+    return;
+  }
+  fill_halo(s, div, KF, 1, 1, 1, 1);
+  double *div2 = S[S_DIV2];
+  for (int j = 2; j <= s->nlat - 1; j++) {
+    const double dl = MS(s, GMC_DE_LON, j);
+    for (int i = 1; i <= s->nlon; i++)
+      A(s, div2, i, j) =
+          (A(s, div, i + 1, j) - 2 * A(s, div, i, j) + A(s, div, i - 1, j)) / (dl * dl) +
+          ((A(s, div, i, j + 1) - A(s, div, i, j)) * MS(s, GMC_HALF_COS_LAT, j) / MS(s, GMC_DE_LAT, j) -
+           (A(s, div, i, j) - A(s, div, i, j - 1)) * MS(s, GMC_HALF_COS_LAT, j - 1) / MS(s, GMC_DE_LAT, j - 1)) /
+              MS(s, GMC_LE_LON, j) / MS(s, GMC_FULL_COS_LAT, j);
+  }
+  fill_halo(s, div2, KF, 0, 1, 0, 1);
 }
 
 /* calc_vor + calc_pv, operators_mod.F90:612-703 */
@@ -272,10 +288,43 @@ static double upwind3(double dir, double wgt, double f1, double f2, double f3, d
   return c31 * (f3 + f2) + c32 * (f4 + f1) + c33 * (f4 - f1 - 3 * (f3 - f2)) * wgt * dir;
 }
 
-/* interp_pv_upwind / interp_pv_midpoint, operators_mod.F90:705-795 */
-static void interp_pv(gmo_sw *s, double **S) {
+/* tvd with the flux limiters of tvd_mod.F90:42-100.  eps is the default-real
+ * literal 1.0e-6 widened to r8 (tvd_mod.F90:51). */
+static double tvd_limiter(int lim, double r) {
+  if (lim == GMC_TVD_NONE) return 1;
+  if (lim == GMC_TVD_VAN_LEER) return (r + fabs(r)) / (1 + fabs(r));
+  return fmax(0.0, fmin(fmin(2.0 * r, 0.5 * (1 + r)), 2.0));
+}
+double gmo_tvd(int lim, double cfl, double fm1, double f, double fp1, double fp2) {
+  const double eps = (double)1.0e-6f;
+  const double r = cfl > 0 ? (f - fm1 + eps) / (fp1 - f + eps) : (fp2 - fp1 + eps) / (fp1 - f + eps);
+  const double C = tvd_limiter(lim, r);
+  return 0.5 * (fp1 + f - ((1 - C) * copysign(1.0, cfl) + cfl * C) * (fp1 - f));
+}
+
+/* interp_pv_upwind / interp_pv_midpoint / interp_pv_tvd,
+ * operators_mod.F90:705-832; dt is the stage dt of operators_prepare */
+static void interp_pv(gmo_sw *s, double **S, double dt) {
   double *pv = S[S_PV], *un = S[S_U_F], *vn = S[S_V_F], *ut = S[S_U_LAT], *vt = S[S_V_LON];
   const double wgt = s->cfg.upwind_wgt_pv;
+  if (s->cfg.pv_scheme == GMC_PV_TVD) {
+    const int lim = s->cfg.tvd_limiter;
+    for (int j = 2; j <= s->nlat - 1; j++)
+      for (int i = 1; i <= s->nlon; i++) {
+        const double cfl = A(s, vt, i, j) * dt / MS(s, GMC_LE_LON, j);
+        A(s, S[S_PV_LON], i, j) = gmo_tvd(lim, cfl, A(s, pv, i, j - 2), A(s, pv, i, j - 1), A(s, pv, i, j),
+                                          A(s, pv, i, j + 1));
+      }
+    for (int j = 1; j <= s->nhalf; j++)
+      for (int i = 1; i <= s->nlon; i++) {
+        const double cfl = A(s, ut, i, j) * dt / MS(s, GMC_LE_LAT, j);
+        A(s, S[S_PV_LAT], i, j) = gmo_tvd(lim, cfl, A(s, pv, i - 2, j), A(s, pv, i - 1, j), A(s, pv, i, j),
+                                          A(s, pv, i + 1, j));
+      }
+    fill_halo(s, S[S_PV_LON], KF, 1, 1, 0, 1);
+    fill_halo(s, S[S_PV_LAT], KH, 1, 1, 1, 0);
+    return;
+  }
   if (s->cfg.pv_scheme == GMC_PV_MIDPOINT) {
     for (int j = 1; j <= s->nhalf; j++)
       for (int i = 1; i <= s->nlon; i++)
@@ -383,18 +432,18 @@ static void pgf_swm(gmo_sw *s, double **S, double **T) {
 }
 
 /* operators_prepare_2 forward pass, operators_mod.F90:117-122 */
-static void prepare_forward(gmo_sw *s, double **S) {
+static void prepare_forward(gmo_sw *s, double **S, double dt) {
   calc_mf(s, S);
   calc_ke(s, S);
   calc_div(s, S);
   calc_pv(s, S);
-  interp_pv(s, S);
+  interp_pv(s, S, dt);
 }
 
 /* ---------------------------------------------------------------------- */
 int gmo_sw_space_operators(gmo_sw *s, int old_level, int star_level, int new_level,
                            int tend1, int tend2, double dt, int pass) {
-  (void)old_level; (void)dt;
+  (void)old_level;
   if (star_level < 1 || star_level > 3 || new_level < 1 || new_level > 3 || tend1 < 1 || tend1 > 3 ||
       tend2 < 1 || tend2 > 3)
     return GMC_EINVAL;
@@ -403,7 +452,7 @@ int gmo_sw_space_operators(gmo_sw *s, int old_level, int star_level, int new_lev
   if (pass == 0 || pass == 1) {
     /* operators_mod.F90:96-134; all_pass also runs calc_m first */
     if (pass == 0) calc_m(s, S);
-    prepare_forward(s, S);
+    prepare_forward(s, S, dt);
     calc_grad_mf(s, S, T1);
     calc_coriolis(s, S, T1);
     calc_grad_ke(s, S, T1);
@@ -475,26 +524,40 @@ int gmo_sw_update_state(gmo_sw *s, int tend, int old_level, int new_level, doubl
   return GMC_OK;
 }
 
-/* div_damp_run, order 2, div_damp_mod.F90:27-56 (c_lon, c_lat for one level)
- * and :89-139.  div comes from the filtered winds (calc_div reads u_f, v_f);
- * u_lon and v_lat are damped and u_f, v_f are not refiltered. */
+/* div_damp_init / div_damp_run, orders 2 and 4, div_damp_mod.F90:27-139 (one
+ * level, no baroclinic factor).  div comes from the filtered winds (calc_div
+ * reads u_f, v_f); u_lon and v_lat are damped and u_f, v_f are not
+ * refiltered. */
 static void div_damp(gmo_sw *s, double **N) {
   calc_div(s, N);
-  const double c2 = s->cfg.div_damp_coef2;
+  const int o4 = s->cfg.div_damp_order == 4;
+  const double c2 = s->cfg.div_damp_coef2, c4 = s->cfg.div_damp_coef4;
+  const double *d = o4 ? N[S_DIV2] : N[S_DIV];
   for (int j = 2; j <= s->nlat - 1; j++) {
-    double c = c2 * MS(s, GMC_LE_LON, j) * MS(s, GMC_DE_LON, j);
-    for (int i = 1; i <= s->nlon; i++)
-      A(s, N[S_U_LON], i, j) = A(s, N[S_U_LON], i, j) +
-          c * (A(s, N[S_DIV], i + 1, j) - A(s, N[S_DIV], i, j)) / MS(s, GMC_DE_LON, j);
+    const double le = MS(s, GMC_LE_LON, j), de = MS(s, GMC_DE_LON, j);
+    const double c = o4 ? c4 * (le * le) * (de * de) : c2 * le * de;
+    for (int i = 1; i <= s->nlon; i++) {
+      const double t = c * (A(s, d, i + 1, j) - A(s, d, i, j)) / de;
+      A(s, N[S_U_LON], i, j) = o4 ? A(s, N[S_U_LON], i, j) - t : A(s, N[S_U_LON], i, j) + t;
+    }
   }
   for (int j = 1; j <= s->nhalf; j++) {
-    double c = c2 * MS(s, GMC_LE_LAT, j) * MS(s, GMC_DE_LAT, j);
-    for (int i = 1; i <= s->nlon; i++)
-      A(s, N[S_V_LAT], i, j) = A(s, N[S_V_LAT], i, j) +
-          c * (A(s, N[S_DIV], i, j + 1) - A(s, N[S_DIV], i, j)) / MS(s, GMC_DE_LAT, j);
+    const double le = MS(s, GMC_LE_LAT, j), de = MS(s, GMC_DE_LAT, j);
+    const double c = o4 ? c4 * (le * le) * (de * de) : c2 * le * de;
+    for (int i = 1; i <= s->nlon; i++) {
+      const double t = c * (A(s, d, i, j + 1) - A(s, d, i, j)) / de;
+      A(s, N[S_V_LAT], i, j) = o4 ? A(s, N[S_V_LAT], i, j) - t : A(s, N[S_V_LAT], i, j) + t;
+    }
   }
   fill_all(s, N[S_U_LON], KF);
   fill_all(s, N[S_V_LAT], KH);
+}
+
+/* damp_run's divergence damping alone on state(level) (damp_mod.F90:56-58) */
+int gmo_sw_div_damp(gmo_sw *s, int level) {
+  if (level < 1 || level > 3) return GMC_EINVAL;
+  div_damp(s, s->st[level]);
+  return GMC_OK;
 }
 
 /* step_forward_backward, time_schemes_mod.F90:102-118 */
@@ -543,17 +606,18 @@ int gmo_sw_prepare(gmo_sw *s, int level) {
   calc_mf(s, S);
   calc_ke(s, S);
   calc_pv(s, S);
-  interp_pv(s, S);
+  interp_pv(s, S, s->cfg.dt_dyn);
   calc_div(s, S);
   s->old = level;
   s->new_ = level == 1 ? 2 : 1;
   return GMC_OK;
 }
 
-/* diagnose, gmcore_mod.F90:213-341 (SW), after operators_prepare_1 */
-int gmo_sw_diagnose(gmo_sw *s, int level, double out[4]) {
-  double **S = s->st[level];
-  calc_m(s, S); calc_mf(s, S); calc_ke(s, S); calc_pv(s, S); interp_pv(s, S); calc_div(s, S);
+/* diagnose, gmcore_mod.F90:213-341 (SW): the sums over the fields as they
+ * are stored in state(level).  After a step those are m, m_lon, m_lat of the
+ * backward pass's calc_m and mfx_lon, mfy_lat, pv, pv_lon, pv_lat of the
+ * level's last forward pass as a star state (operators_mod.F90:117-122). */
+static void diag_sums(gmo_sw *s, double **S, double out[4]) {
   double tm = 0, te_ke = 0, te_pe = 0, tav = 0, tpe = 0;
   const double g = s->cfg.g;
   for (int j = 1; j <= s->nlat; j++)
@@ -582,6 +646,21 @@ int gmo_sw_diagnose(gmo_sw *s, int level, double out[4]) {
       tpe = tpe + A(s, S[S_M_LAT], i, j) * (p * p) * 0.5 * MS(s, GMC_AREA_LAT, j);
     }
   out[0] = tm; out[1] = te_ke + 0.0 + te_pe; out[2] = tpe; out[3] = tav;
+}
+
+int gmo_sw_diagnose(gmo_sw *s, int level, double out[4]) {
+  if (level < 1 || level > 3) return GMC_EINVAL;
+  diag_sums(s, s->st[level], out);
+  return GMC_OK;
+}
+
+/* the same sums after operators_prepare on the level's own state (fields of
+ * one time; not what the reference reads after a step) */
+int gmo_sw_diagnose_fresh(gmo_sw *s, int level, double out[4]) {
+  if (level < 1 || level > 3) return GMC_EINVAL;
+  double **S = s->st[level];
+  calc_m(s, S); calc_mf(s, S); calc_ke(s, S); calc_pv(s, S); interp_pv(s, S, s->cfg.dt_dyn); calc_div(s, S);
+  diag_sums(s, S, out);
   return GMC_OK;
 }
 
@@ -682,7 +761,7 @@ int gmo_sw_set(gmo_sw *s, int level, int field, const double *host, int64_t ld) 
 
 /* interior of any state (is_tend = 0, index S_*) or tendency (1, T_*) field */
 int gmo_sw_get_internal(gmo_sw *s, int level, int is_tend, int index, double *host, int64_t ld) {
-  static const int s_half[S_N] = {0, 1, 0, 0, 0, 1, 0, 0, 1, 1, 0, 1, 1, 0, 1, 0, 0, 0, 1, 1, 0, 1};
+  static const int s_half[S_N] = {0, 1, 0, 0, 0, 1, 0, 0, 1, 1, 0, 1, 1, 0, 1, 0, 0, 0, 1, 1, 0, 1, 0};
   static const int t_half[T_N] = {0, 1, 0, 1, 0, 0, 1, 0, 0, 0, 1};
   if (level < 1 || level > 3 || !host) return GMC_EINVAL;
   if (index < 0 || index >= (is_tend ? T_N : S_N)) return GMC_EINVAL;
