@@ -129,6 +129,7 @@ def load_hip() -> C.CDLL:
         "gmc_field_ptr": (C.c_int, [vp, C.c_int32, C.c_int32, P(P(C.c_double)), P(C.c_int64)]),
         "gmc_filter_plan": (C.c_int, [vp, P(C.c_int32)]),
         "gmc_keep_derived": (C.c_int, [vp, C.c_int32]),
+        "gmc_diagnose_dev": (C.c_int, [vp, C.c_int32, vp]),
         # FFSL advection, include/gmcore_adv.h
         "gmc_adv_create": (C.c_int, [P(GmcMesh), P(GmcAdvConfig), P(vp)]),
         "gmc_adv_destroy": (C.c_int, [vp]),

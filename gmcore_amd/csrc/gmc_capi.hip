@@ -209,6 +209,7 @@ int gmc_create(const gmc_mesh *mesh, const gmc_config *cfg, gmc_ctx **out) {
   }
   if (!rc) rc = upload_filter(c, cfg->big_filter, c->big, 0);
   if (!rc) rc = upload_filter(c, cfg->small_filter1, c->small1, 1);
+  if (!rc) rc = ops_scratch_init(c);
   if (rc) {
     std::fprintf(stderr, "gmc_create: %s\n", c->err.c_str());
     gmc_destroy(c);
@@ -359,6 +360,12 @@ int gmc_diagnose(gmc_ctx *c, int32_t level, double out[4]) {
   if (!c || !out) return GMC_EINVAL;
   if (level < 1 || level > 3) return set_error(c, GMC_EINVAL, "level must be 1..3");
   return ops_diagnose(c, level, out);
+}
+
+int gmc_diagnose_dev(gmc_ctx *c, int32_t level, double *d_out) {
+  if (!c || !d_out) return GMC_EINVAL;
+  if (level < 1 || level > 3) return set_error(c, GMC_EINVAL, "level must be 1..3");
+  return ops_diagnose_dev(c, level, d_out);
 }
 
 int gmc_synchronize(gmc_ctx *c) {

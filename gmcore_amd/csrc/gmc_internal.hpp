@@ -147,6 +147,7 @@ struct gmc_ctx {
   double *gzs = nullptr;
   double *d_pole = nullptr;  // pole sums scratch (see gmc_ops.hip)
   double *d_diag = nullptr;  // diagnose partial sums
+  double *scratch = nullptr; // this device's static scratch of gmc_diagnose_dev
   double *slab = nullptr;    // one allocation holding every field
   size_t slab_bytes = 0;
   hipStream_t stream = nullptr;
@@ -179,6 +180,9 @@ int ops_space_operators(gmc_ctx *ctx, int old_l, int star_l, int new_l, int t1, 
 int ops_update_state(gmc_ctx *ctx, int tend, int old_l, int new_l, double dt, int what);
 int ops_div_damp(gmc_ctx *ctx, int level);
 int ops_diagnose(gmc_ctx *ctx, int level, double out[4]);
+// resolve (once per device) the address of gmc_diagnose_dev's static scratch
+int ops_scratch_init(gmc_ctx *ctx);
+int ops_diagnose_dev(gmc_ctx *ctx, int level, double *d_out);
 int ops_integrate(gmc_ctx *ctx, int old_l, int new_l, double dt);
 void ops_filter(gmc_ctx *ctx, int small, int half, const double *x, double *y);
 int ops_stage_tape(gmc_ctx *ctx, int old_l, int star_l, int new_l, double dt);

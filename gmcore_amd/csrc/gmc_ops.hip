@@ -6,6 +6,8 @@
 // back the per-pass entry points (gmc_space_operators, gmc_update_state,
 // gmc_prepare, gmc_diagnose).  gmc_step uses the fused stage kernels of
 // gmc_fused.hip, which call the same point formulas.
+#include <mutex>
+
 #include "gmc_internal.hpp"
 #include "gmc_point.hpp"
 
@@ -342,8 +344,9 @@ static TendPtrs tend_ptrs(double **T) {
                   T[T_DMFDLON], T[T_DMFDLAT], T[T_PGF_LON], T[T_PGF_LAT]};
 }
 
-// operators_prepare_2's forward pass on state S with the stage dt
-static void forward_prepare(gmc_ctx *c, double **S, double dt) {
+// operators_prepare_2's forward pass on state S with the stage dt; pole:
+// 8 doubles for the pole sums (k_pole_sums)
+static void forward_prepare(gmc_ctx *c, double **S, double dt, double *pole) {
   const Grid &g = c->grid;
   const Mesh M = mesh_of(c);
   const dim3 b(BX, BY);
@@ -351,13 +354,13 @@ static void forward_prepare(gmc_ctx *c, double **S, double dt) {
                      S[S_V_F], M, S[S_MFX_LON], S[S_MFY_LAT], S[S_MFX_LAT], S[S_MFY_LON], S[S_U_LAT],
                      S[S_V_LON]);
   hipLaunchKernelGGL(k_pole_sums, dim3(2), dim3(kWave), 0, c->stream, g, M, S[S_V_F], S[S_MFY_LAT],
-                     S[S_U_LAT], c->d_pole);
-  hipLaunchKernelGGL(k_calc_ke, grid2d(g, g.nlat), b, 0, c->stream, g, M, S[S_U_F], S[S_V_F], c->d_pole,
+                     S[S_U_LAT], pole);
+  hipLaunchKernelGGL(k_calc_ke, grid2d(g, g.nlat), b, 0, c->stream, g, M, S[S_U_F], S[S_V_F], pole,
                      S[S_KE]);
-  hipLaunchKernelGGL(k_calc_div, grid2d(g, g.nlat), b, 0, c->stream, g, M, S[S_U_F], S[S_V_F], c->d_pole,
+  hipLaunchKernelGGL(k_calc_div, grid2d(g, g.nlat), b, 0, c->stream, g, M, S[S_U_F], S[S_V_F], pole,
                      S[S_DIV]);
   hipLaunchKernelGGL(k_calc_pv, grid2d(g, g.nhalf), b, 0, c->stream, g, M, c->cfg.pv_pole_stokes,
-                     S[S_U_F], S[S_V_F], S[S_M_VTX], c->d_pole, S[S_VOR], S[S_PV]);
+                     S[S_U_F], S[S_V_F], S[S_M_VTX], pole, S[S_VOR], S[S_PV]);
   const int order = c->cfg.pv_scheme == GMC_PV_TVD ? c->cfg.tvd_limiter : c->cfg.upwind_order_pv;
   hipLaunchKernelGGL(k_interp_pv, grid2d(g, g.nlat), b, 0, c->stream, g, M, c->cfg.pv_scheme, order,
                      c->cfg.upwind_wgt_pv, dt, S[S_PV], S[S_U_F], S[S_V_F], S[S_U_LAT], S[S_V_LON],
@@ -376,7 +379,7 @@ int ops_prepare(gmc_ctx *c, int level) {
   filter_field(c, c->big, 1, S[S_V_LAT], S[S_V_F]);
   filter_field(c, c->big, 0, S[S_GZ], S[S_GZ_F]);
   calc_m(c, S);
-  forward_prepare(c, S, c->cfg.dt_dyn);   // operators_prepare_1 with dt_dyn (gmcore_mod.F90:133)
+  forward_prepare(c, S, c->cfg.dt_dyn, c->d_pole);   // operators_prepare_1 with dt_dyn (gmcore_mod.F90:133)
   return check_hip(c, hipGetLastError(), "prepare");
 }
 
@@ -388,7 +391,7 @@ int ops_space_operators(gmc_ctx *c, int old_l, int star_l, int new_l, int t1, in
   double **S = c->state[star_l], **N = c->state[new_l], **T1 = c->tend[t1], **T2 = c->tend[t2];
   if (pass == 0 || pass == 1) {
     if (pass == 0) calc_m(c, S);
-    forward_prepare(c, S, dt);
+    forward_prepare(c, S, dt, c->d_pole);
     hipLaunchKernelGGL(k_grad_mf, grid2d(g, g.nlat), b, 0, c->stream, g, M, S[S_MFX_LON], S[S_MFY_LAT],
                        c->d_pole, T1[T_DMFDLON], T1[T_DMFDLAT]);
     hipLaunchKernelGGL(k_coriolis, grid2d(g, g.nlat), b, 0, c->stream, g, M, c->cfg.coriolis_scheme,
@@ -507,7 +510,10 @@ int ops_div_damp(gmc_ctx *c, int level) {
   return check_hip(c, hipGetLastError(), "div_damp");
 }
 
-int ops_diagnose(gmc_ctx *c, int level, double out[4]) {
+// The launches diagnose shares with gmc_diagnose_dev: m and, where the
+// level does not hold them, the forward-pass fields (pole sums into pole[0..7]),
+// then one row of partial sums per latitude row into part[r * 7 + k].
+static int diag_rows(gmc_ctx *c, int level, double *pole, double *part) {
   const Grid &g = c->grid;
   double **S = c->state[level];
   // m, m_lon, m_lat as the backward pass's calc_m left them on the current
@@ -515,11 +521,16 @@ int ops_diagnose(gmc_ctx *c, int level, double out[4]) {
   // them (the reference reads them as stored, gmcore_mod.F90:230-313), or,
   // when the level does not hold them, formed now from its own state
   calc_m(c, S);
-  if (!c->derived_kept[level]) forward_prepare(c, S, c->cfg.dt_dyn);
+  if (!c->derived_kept[level]) forward_prepare(c, S, c->cfg.dt_dyn, pole);
   hipLaunchKernelGGL(k_diag_rows, dim3(g.nlat), dim3(kWave), 0, c->stream, g, mesh_of(c), c->cfg.g, S[S_M],
                      c->gzs, S[S_MFX_LON], S[S_U_LON], S[S_MFY_LAT], S[S_V_LAT], S[S_PV], S[S_M_LON],
-                     S[S_PV_LON], S[S_M_LAT], S[S_PV_LAT], c->d_diag);
-  int rc = check_hip(c, hipGetLastError(), "diagnose");
+                     S[S_PV_LON], S[S_M_LAT], S[S_PV_LAT], part);
+  return check_hip(c, hipGetLastError(), "diagnose");
+}
+
+int ops_diagnose(gmc_ctx *c, int level, double out[4]) {
+  const Grid &g = c->grid;
+  int rc = diag_rows(c, level, c->d_pole, c->d_diag);
   if (rc) return rc;
   double *h = new double[(size_t)g.nlat * 7];
   rc = check_hip(c, hipMemcpyAsync(h, c->d_diag, sizeof(double) * g.nlat * 7, hipMemcpyDeviceToHost, c->stream),
@@ -536,6 +547,65 @@ int ops_diagnose(gmc_ctx *c, int level, double out[4]) {
   }
   delete[] h;
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// gmc_diagnose_dev: diagnose without allocating.  Its pole sums and row
+// partial sums live in `scratch`, a static device array of this file (one
+// copy per GPU, kept until the process ends); the row sums are added on the
+// device in the host loop's order, so out[] holds gmc_diagnose's bits.
+// Layout: [0, 64) pole sums (k_pole_sums writes 8 of them), [64, 64 + 7 nlat)
+// row partials.  Every element a call reads is written earlier in the same
+// call, so nothing needs clearing between calls.
+constexpr int kScratchMaxRows = 4097;   // latitude rows, poles included
+constexpr size_t kScratchDoubles = 64 + 7 * (size_t)kScratchMaxRows;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static double *g_scratch[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+namespace {
+// the totals of diagnose from the row partials, in the order of ops_diagnose's
+// host loop (row r ascending for each sum, then the same combinations)
+__global__ void k_diag_total(int nlat, const double *__restrict__ part, double *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int r = 0; r < nlat; r++)
+    for (int k = 0; k < 7; k++) s[k] += part[r * 7 + k];
+  out[0] = s[0];
+  out[1] = s[1] + s[2] + s[3];
+  out[2] = s[5] + s[6];
+  out[3] = s[4];
+}
+}  // namespace
+
+int ops_scratch_init(gmc_ctx *c) {
+  int dev = 0;
+  int rc = check_hip(c, hipGetDevice(&dev), "get device");
+  if (rc) return rc;
+  if (dev < 0 || dev >= kMaxDevices) return set_error(c, GMC_EDEVICE, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *ptr = nullptr;
+    rc = check_hip(c, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "scratch symbol address");
+    if (rc) return rc;
+    g_scratch[dev] = (double *)ptr;
+  }
+  c->scratch = g_scratch[dev];
+  return GMC_OK;
+}
+
+int ops_diagnose_dev(gmc_ctx *c, int level, double *d_out) {
+  const Grid &g = c->grid;
+  if (g.nlat > kScratchMaxRows)
+    return set_error(c, GMC_EINVAL, "gmc_diagnose_dev: more latitude rows than its static scratch holds (4097)");
+  if (!c->scratch) return set_error(c, GMC_ESTATE, "gmc_diagnose_dev: scratch address not resolved");
+  double *pole = c->scratch, *part = c->scratch + 64;
+  int rc = diag_rows(c, level, pole, part);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_diag_total, dim3(1), dim3(kWave), 0, c->stream, g.nlat, part, d_out);
+  return check_hip(c, hipGetLastError(), "diagnose total");
 }
 
 }  // namespace gmc

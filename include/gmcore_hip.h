@@ -188,6 +188,18 @@ int gmc_current_level(const gmc_ctx *ctx, int32_t *old_level);
  * first (te, tpe and tav of the current fields). */
 int gmc_diagnose(gmc_ctx *ctx, int32_t level, double out[4]);
 
+/* gmc_diagnose without any allocation (no counterpart in the reference; for
+ * callers that cannot allocate: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The same sums as gmc_diagnose,
+ * bit for bit, written to d_out[0..3] in DEVICE memory, queued on the
+ * context's stream (the call returns without waiting; gmc_synchronize or an
+ * event on gmc_stream orders a read).  Its pole sums and row partial sums use
+ * a static device array of the library, one per GPU, sized for at most 4097
+ * latitude rows (about 230 KB); larger grids get GMC_EINVAL.  Every call on a
+ * GPU shares that array: calls that overlap on one GPU (any contexts, any
+ * streams) race and must be serialised by the caller. */
+int gmc_diagnose_dev(gmc_ctx *ctx, int32_t level, double *d_out);
+
 /* on = 1: every stage of gmc_step, gmc_time_integrate and gmc_kernel_times
  * also stores the star state's forward-pass fields (m_lon, m_lat, m_vtx,
  * mfx_lon, mfy_lat, mfx_lat, mfy_lon, u_lat, v_lon, pv, pv_lon, pv_lat) in
