@@ -21,16 +21,18 @@ from gmcore_amd.capi import (GMC_MESH_NFIELDS, FIELD, GmcAdvConfig, GmcConfig, G
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
+LIB_OMP = os.path.join(HERE, "liboracle_omp.so")   # row loops on OpenMP threads, same bits
 
-_lib = None
+_libs = {}
 
 
-def lib() -> C.CDLL:
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
-            raise RuntimeError(f"{LIB} not built (make -C oracle)")
-        L = C.CDLL(LIB)
+def lib(omp: bool = False) -> C.CDLL:
+    """liboracle.so (one thread), or with omp=True liboracle_omp.so."""
+    if omp not in _libs:
+        path = LIB_OMP if omp else LIB
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} not built (make -C oracle)")
+        L = C.CDLL(path)
         P, vp = C.POINTER, C.c_void_p
         L.gmo_mesh_init.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, P(C.c_double)]
         L.gmo_filter_init.argtypes = [C.c_int, C.c_int, P(C.c_double), C.c_double, C.c_double,
@@ -76,8 +78,8 @@ def lib() -> C.CDLL:
         L.gmo_adv_tlad_run.argtypes = [vp, C.c_int64, P(C.c_double), P(C.c_double), C.c_int64]
         L.gmo_adv_clone.restype = vp
         L.gmo_adv_clone.argtypes = [vp]
-        _lib = L
-    return _lib
+        _libs[omp] = L
+    return _libs[omp]
 
 
 def mesh_metrics(nlon: int, nlat: int, radius: float, omega: float) -> np.ndarray:
@@ -113,33 +115,35 @@ def zonal_sum(x: np.ndarray) -> float:
 class SWOracle:
     """One block of the reference SW model on the CPU, driven like gmcore_run."""
 
-    def __init__(self, mesh_struct: GmcMesh, cfg: GmcConfig):
+    def __init__(self, mesh_struct: GmcMesh, cfg: GmcConfig, omp: bool = False):
+        """omp=True runs the row loops on OpenMP threads (OMP_NUM_THREADS)."""
+        self.L = lib(omp)
         self._keep = (mesh_struct, cfg)
         self.nlon, self.nlat = mesh_struct.nlon, mesh_struct.nlat
-        self.h = lib().gmo_sw_create(C.byref(mesh_struct), C.byref(cfg))
+        self.h = self.L.gmo_sw_create(C.byref(mesh_struct), C.byref(cfg))
         if not self.h:
             raise ValueError("gmo_sw_create failed")
 
     def close(self):
         if self.h:
-            lib().gmo_sw_destroy(self.h)
+            self.L.gmo_sw_destroy(self.h)
             self.h = None
 
     __del__ = close
 
     @property
     def halo_width(self) -> int:
-        return lib().gmo_sw_halo_width(self.h)
+        return self.L.gmo_sw_halo_width(self.h)
 
     def set(self, level: int, name: str, arr: np.ndarray):
         a = np.ascontiguousarray(arr, dtype=np.float64)
         assert a.shape == (field_rows(name, self.nlat), self.nlon), (name, a.shape)
-        rc = lib().gmo_sw_set(self.h, level, FIELD[name], dptr(a), self.nlon)
+        rc = self.L.gmo_sw_set(self.h, level, FIELD[name], dptr(a), self.nlon)
         assert rc == 0
 
     def get(self, level: int, name: str) -> np.ndarray:
         a = np.zeros((field_rows(name, self.nlat), self.nlon))
-        rc = lib().gmo_sw_get(self.h, level, FIELD[name], dptr(a), self.nlon)
+        rc = self.L.gmo_sw_get(self.h, level, FIELD[name], dptr(a), self.nlon)
         assert rc == 0
         return a
 
@@ -155,36 +159,36 @@ class SWOracle:
         """Any state_type (tend=False) or tend_type (tend=True) field, interior."""
         names, half = (self.TEND_FIELDS, self.TEND_HALF) if tend else (self.STATE_FIELDS, self.STATE_HALF)
         a = np.zeros((self.nlat - 1 if name in half else self.nlat, self.nlon))
-        rc = lib().gmo_sw_get_internal(self.h, level, int(tend), names.index(name), dptr(a), self.nlon)
+        rc = self.L.gmo_sw_get_internal(self.h, level, int(tend), names.index(name), dptr(a), self.nlon)
         assert rc == 0
         return a
 
     def prepare(self, level: int = 1):
-        lib().gmo_sw_prepare(self.h, level)
+        self.L.gmo_sw_prepare(self.h, level)
 
     def space_operators(self, old, star, new, t1, t2, dt, pass_):
-        return lib().gmo_sw_space_operators(self.h, old, star, new, t1, t2, dt, pass_)
+        return self.L.gmo_sw_space_operators(self.h, old, star, new, t1, t2, dt, pass_)
 
     def update_state(self, tend, old, new, dt, what):
-        return lib().gmo_sw_update_state(self.h, tend, old, new, dt, what)
+        return self.L.gmo_sw_update_state(self.h, tend, old, new, dt, what)
 
     def step(self, dt: float, nsteps: int = 1):
-        lib().gmo_sw_step(self.h, dt, nsteps)
+        self.L.gmo_sw_step(self.h, dt, nsteps)
 
     def div_damp(self, level: int):
         """damp_run's divergence damping alone on the level."""
-        assert lib().gmo_sw_div_damp(self.h, level) == 0
+        assert self.L.gmo_sw_div_damp(self.h, level) == 0
 
     @property
     def level(self) -> int:
-        return lib().gmo_sw_current_level(self.h)
+        return self.L.gmo_sw_current_level(self.h)
 
     def diagnose(self, level=None, fresh=False):
         """(tm, te, tpe, tav) over the fields stored in the level, as the
         reference's diagnose reads them; fresh=True forms the forward-pass
         fields of the level's own state first."""
         out = np.zeros(4)
-        fn = lib().gmo_sw_diagnose_fresh if fresh else lib().gmo_sw_diagnose
+        fn = self.L.gmo_sw_diagnose_fresh if fresh else self.L.gmo_sw_diagnose
         fn(self.h, self.level if level is None else level, dptr(out))
         return out
 

@@ -15,6 +15,15 @@
 #include "gmc_oracle.h"
 
 #define HY 2
+/* Row loops whose rows are independent run on all threads when the library
+ * is built with -fopenmp (liboracle_omp.so); every element is still formed
+ * by one thread in the same operation order, so both builds give the same
+ * bits.  Reductions (zonal sums, diagnose) stay sequential. */
+#ifdef _OPENMP
+#define ROWS _Pragma("omp parallel for schedule(static)")
+#else
+#define ROWS
+#endif
 static const double EPS = 2.220446049250313e-16;  /* epsilon(1.0_r8), const_mod.F90:17 */
 
 enum { KF = 0, KH = 1 };  /* full-lat / half-lat row grid */
@@ -64,9 +73,11 @@ static void fill_halo(gmo_sw *s, double *a, int kind, int west, int east, int so
   const int nlon = s->nlon, hx = s->hx, nr = kind == KF ? s->nlat : s->nhalf;
   const int mx = nlon / 2;
   if (west)
+    ROWS
     for (int j = 1; j <= s->nlat; j++)
       for (int i = 1 - hx; i <= 0; i++) A(s, a, i, j) = A(s, a, i + nlon, j);
   if (east)
+    ROWS
     for (int j = 1; j <= s->nlat; j++)
       for (int i = nlon + 1; i <= nlon + hx; i++) A(s, a, i, j) = A(s, a, i - nlon, j);
   double *tmp = malloc(sizeof(double) * s->ncol * HY);
@@ -130,10 +141,11 @@ static double row_sum(const gmo_sw *s, const double *a, int j, double sign, cons
  * place; otherwise rows with n <= 1 are copied to y. */
 static void filter_apply(gmo_sw *s, const ofilter *f, double *x, double *y, int kind) {
   const int nr = kind == KF ? s->nlat : s->nhalf;
-  double tmp[s->ncol];
+  ROWS
   for (int j = 1; j <= nr; j++) {
     const int n = kind == KF ? f->ngrid_lon[j] : f->ngrid_lat[j];
     const double *w = (kind == KF ? f->wgt_lon : f->wgt_lat) + (size_t)j * f->max_taps;
+    double tmp[s->ncol];
     if (n > 1) {
       const int hn = (n - 1) / 2;
       for (int i = 1; i <= s->nlon; i++) {
@@ -156,17 +168,21 @@ static void filter_apply(gmo_sw *s, const ofilter *f, double *x, double *y, int 
  * interp_mod.F90:233-275, 438-463 */
 static void calc_m(gmo_sw *s, double **S) {
   const double g = s->cfg.g;
+  ROWS
   for (int j = 1; j <= s->nlat; j++)
     for (int i = 1; i <= s->nlon; i++) A(s, S[S_M], i, j) = (A(s, S[S_GZ], i, j) - A(s, s->gzs, i, j)) / g;
   fill_all(s, S[S_M], KF);
+  ROWS
   for (int j = 2; j <= s->nlat - 1; j++)
     for (int i = 1; i <= s->nlon; i++)
       A(s, S[S_M_LON], i, j) = (A(s, S[S_M], i, j) + A(s, S[S_M], i + 1, j)) * 0.5;
   fill_all(s, S[S_M_LON], KF);
+  ROWS
   for (int j = 1; j <= s->nhalf; j++)
     for (int i = 1; i <= s->nlon; i++)
       A(s, S[S_M_LAT], i, j) = (A(s, S[S_M], i, j) + A(s, S[S_M], i, j + 1)) * 0.5;
   fill_all(s, S[S_M_LAT], KH);
+  ROWS
   for (int j = 1; j <= s->nhalf; j++)
     for (int i = 1; i <= s->nlon; i++)
       A(s, S[S_M_VTX], i, j) = ((A(s, S[S_M], i, j) + A(s, S[S_M], i + 1, j)) * MS(s, GMC_AREA_SUBCELL2, j) +
@@ -177,12 +193,15 @@ static void calc_m(gmo_sw *s, double **S) {
 /* calc_mf, operators_mod.F90:545-610 (accum_uv_cell of the mass batch, whose
  * output the SW path never reads, is omitted) */
 static void calc_mf(gmo_sw *s, double **S) {
+  ROWS
   for (int j = 2; j <= s->nlat - 1; j++)
     for (int i = 0; i <= s->nlon; i++)
       A(s, S[S_MFX_LON], i, j) = A(s, S[S_M_LON], i, j) * A(s, S[S_U_F], i, j);
+  ROWS
   for (int j = 1; j <= s->nhalf; j++)
     for (int i = 1; i <= s->nlon + 1; i++)
       A(s, S[S_MFY_LAT], i, j) = A(s, S[S_M_LAT], i, j) * A(s, S[S_V_F], i, j);
+  ROWS
   for (int j = 1; j <= s->nhalf; j++)
     for (int i = 1; i <= s->nlon; i++) {
       double v = MS(s, GMC_HALF_TANGENT_WGT1, j) * (A(s, S[S_MFX_LON], i - 1, j) + A(s, S[S_MFX_LON], i, j)) +
@@ -191,6 +210,7 @@ static void calc_mf(gmo_sw *s, double **S) {
       A(s, S[S_U_LAT], i, j) = v / A(s, S[S_M_LAT], i, j);
     }
   fill_all(s, S[S_U_LAT], KH);
+  ROWS
   for (int j = 2; j <= s->nlat - 1; j++)
     for (int i = 1; i <= s->nlon; i++) {
       double v = MS(s, GMC_FULL_TANGENT_WGT1, j) * (A(s, S[S_MFY_LAT], i, j - 1) + A(s, S[S_MFY_LAT], i + 1, j - 1)) +
@@ -205,6 +225,7 @@ static void calc_mf(gmo_sw *s, double **S) {
  * operators_mod.F90:234-258, 320-349 */
 static void calc_ke(gmo_sw *s, double **S) {
   double *u = S[S_U_F], *v = S[S_V_F], *ke = S[S_KE];
+  ROWS
   for (int j = 2; j <= s->nlat - 1; j++)
     for (int i = 1; i <= s->nlon + 1; i++)
       A(s, ke, i, j) = (MS(s, GMC_AREA_LON_WEST, j) * (A(s, u, i - 1, j) * A(s, u, i - 1, j)) +
@@ -225,6 +246,7 @@ static void calc_ke(gmo_sw *s, double **S) {
  * non-pole rows) when div_damp_order is 4 */
 static void calc_div(gmo_sw *s, double **S) {
   double *u = S[S_U_F], *v = S[S_V_F], *div = S[S_DIV];
+  ROWS
   for (int j = 2; j <= s->nlat - 1; j++)
     for (int i = 1; i <= s->nlon; i++)
       A(s, div, i, j) = ((A(s, u, i, j) * MS(s, GMC_LE_LON, j) - A(s, u, i - 1, j) * MS(s, GMC_LE_LON, j)) +
@@ -242,6 +264,7 @@ static void calc_div(gmo_sw *s, double **S) {
   }
   fill_halo(s, div, KF, 1, 1, 1, 1);
   double *div2 = S[S_DIV2];
+  ROWS
   for (int j = 2; j <= s->nlat - 1; j++) {
     const double dl = MS(s, GMC_DE_LON, j);
     for (int i = 1; i <= s->nlon; i++)
@@ -257,6 +280,7 @@ static void calc_div(gmo_sw *s, double **S) {
 /* calc_vor + calc_pv, operators_mod.F90:612-703 */
 static void calc_pv(gmo_sw *s, double **S) {
   double *u = S[S_U_F], *v = S[S_V_F], *vor = S[S_VOR];
+  ROWS
   for (int j = 1; j <= s->nhalf; j++)
     for (int i = 0; i <= s->nlon; i++)
       A(s, vor, i, j) = (A(s, u, i, j) * MS(s, GMC_DE_LON, j) - A(s, u, i, j + 1) * MS(s, GMC_DE_LON, j + 1) +
@@ -273,6 +297,7 @@ static void calc_pv(gmo_sw *s, double **S) {
     pole = gmo_zonal_sum(w, s->nlon, 1) / s->nlon / MS(s, GMC_AREA_CELL, j + 1);
     for (int i = 1; i <= s->nlon; i++) A(s, vor, i, j) = pole;
   }
+  ROWS
   for (int j = 1; j <= s->nhalf; j++)
     for (int i = 1; i <= s->nlon; i++)
       A(s, S[S_PV], i, j) = (A(s, vor, i, j) + MS(s, GMC_HALF_F, j)) / A(s, S[S_M_VTX], i, j);
@@ -309,12 +334,14 @@ static void interp_pv(gmo_sw *s, double **S, double dt) {
   const double wgt = s->cfg.upwind_wgt_pv;
   if (s->cfg.pv_scheme == GMC_PV_TVD) {
     const int lim = s->cfg.tvd_limiter;
+    ROWS
     for (int j = 2; j <= s->nlat - 1; j++)
       for (int i = 1; i <= s->nlon; i++) {
         const double cfl = A(s, vt, i, j) * dt / MS(s, GMC_LE_LON, j);
         A(s, S[S_PV_LON], i, j) = gmo_tvd(lim, cfl, A(s, pv, i, j - 2), A(s, pv, i, j - 1), A(s, pv, i, j),
                                           A(s, pv, i, j + 1));
       }
+    ROWS
     for (int j = 1; j <= s->nhalf; j++)
       for (int i = 1; i <= s->nlon; i++) {
         const double cfl = A(s, ut, i, j) * dt / MS(s, GMC_LE_LAT, j);
@@ -326,16 +353,19 @@ static void interp_pv(gmo_sw *s, double **S, double dt) {
     return;
   }
   if (s->cfg.pv_scheme == GMC_PV_MIDPOINT) {
+    ROWS
     for (int j = 1; j <= s->nhalf; j++)
       for (int i = 1; i <= s->nlon; i++)
         A(s, S[S_PV_LAT], i, j) = 0.5 * (A(s, pv, i - 1, j) + A(s, pv, i, j));
     fill_halo(s, S[S_PV_LAT], KH, 1, 1, 1, 0);
+    ROWS
     for (int j = 2; j <= s->nlat - 1; j++)
       for (int i = 1; i <= s->nlon; i++)
         A(s, S[S_PV_LON], i, j) = 0.5 * (A(s, pv, i, j) + A(s, pv, i, j - 1));
     fill_halo(s, S[S_PV_LON], KF, 1, 1, 0, 1);
     return;
   }
+  ROWS
   for (int j = 2; j <= s->nlat - 1; j++)
     for (int i = 1; i <= s->nlon; i++) {
       double t = A(s, vt, i, j), n = A(s, un, i, j);
@@ -345,6 +375,7 @@ static void interp_pv(gmo_sw *s, double **S, double dt) {
           : upwind3(copysign(1.0, t), wgt, A(s, pv, i, j - 2), A(s, pv, i, j - 1), A(s, pv, i, j), A(s, pv, i, j + 1));
       A(s, S[S_PV_LON], i, j) = b * up + (1 - b) * 0.5 * (A(s, pv, i, j - 1) + A(s, pv, i, j));
     }
+  ROWS
   for (int j = 1; j <= s->nhalf; j++)
     for (int i = 1; i <= s->nlon; i++) {
       double t = A(s, ut, i, j), n = A(s, vn, i, j);
@@ -361,9 +392,11 @@ static void interp_pv(gmo_sw *s, double **S, double dt) {
 /* calc_grad_mf, operators_mod.F90:939-1006 */
 static void calc_grad_mf(gmo_sw *s, double **S, double **T) {
   double *mfx = S[S_MFX_LON], *mfy = S[S_MFY_LAT];
+  ROWS
   for (int j = 2; j <= s->nlat - 1; j++)
     for (int i = 1; i <= s->nlon; i++)
       A(s, T[T_DMFDLON], i, j) = (A(s, mfx, i, j) - A(s, mfx, i - 1, j)) * MS(s, GMC_LE_LON, j) / MS(s, GMC_AREA_CELL, j);
+  ROWS
   for (int j = 2; j <= s->nlat - 1; j++)
     for (int i = 1; i <= s->nlon; i++)
       A(s, T[T_DMFDLAT], i, j) = (A(s, mfy, i, j) * MS(s, GMC_LE_LAT, j) - A(s, mfy, i, j - 1) * MS(s, GMC_LE_LAT, j - 1)) /
@@ -379,6 +412,7 @@ static void calc_grad_mf(gmo_sw *s, double **S, double **T) {
 /* calc_coriolis, operators_mod.F90:846-906 */
 static void calc_coriolis(gmo_sw *s, double **S, double **T) {
   double *mfx = S[S_MFX_LON], *mfy = S[S_MFY_LAT], *pl = S[S_PV_LON], *pt = S[S_PV_LAT];
+  ROWS
   for (int j = 1; j <= s->nhalf; j++)
     for (int i = 1; i <= s->nlon; i++) {
       double q;
@@ -393,6 +427,7 @@ static void calc_coriolis(gmo_sw *s, double **S, double **T) {
       }
       A(s, T[T_QHU], i, j) = q;
     }
+  ROWS
   for (int j = 2; j <= s->nlat - 1; j++)
     for (int i = 1; i <= s->nlon; i++) {
       double q;
@@ -412,9 +447,11 @@ static void calc_coriolis(gmo_sw *s, double **S, double **T) {
 /* calc_grad_ke, operators_mod.F90:908-937 */
 static void calc_grad_ke(gmo_sw *s, double **S, double **T) {
   double *ke = S[S_KE];
+  ROWS
   for (int j = 2; j <= s->nlat - 1; j++)
     for (int i = 1; i <= s->nlon; i++)
       A(s, T[T_DKEDLON], i, j) = (A(s, ke, i + 1, j) - A(s, ke, i, j)) / MS(s, GMC_DE_LON, j);
+  ROWS
   for (int j = 1; j <= s->nhalf; j++)
     for (int i = 1; i <= s->nlon; i++)
       A(s, T[T_DKEDLAT], i, j) = (A(s, ke, i, j + 1) - A(s, ke, i, j)) / MS(s, GMC_DE_LAT, j);
@@ -423,9 +460,11 @@ static void calc_grad_ke(gmo_sw *s, double **S, double **T) {
 /* pgf_swm_run, pgf_swm_mod.F90:18-46 */
 static void pgf_swm(gmo_sw *s, double **S, double **T) {
   double *gz = S[S_GZ_F];
+  ROWS
   for (int j = 2; j <= s->nlat - 1; j++)
     for (int i = 1; i <= s->nlon; i++)
       A(s, T[T_PGF_LON], i, j) = (A(s, gz, i + 1, j) - A(s, gz, i, j)) / MS(s, GMC_DE_LON, j);
+  ROWS
   for (int j = 1; j <= s->nhalf; j++)
     for (int i = 1; i <= s->nlon; i++)
       A(s, T[T_PGF_LAT], i, j) = (A(s, gz, i, j + 1) - A(s, gz, i, j)) / MS(s, GMC_DE_LAT, j);
@@ -457,16 +496,19 @@ int gmo_sw_space_operators(gmo_sw *s, int old_level, int star_level, int new_lev
     calc_coriolis(s, S, T1);
     calc_grad_ke(s, S, T1);
     if (pass == 0) pgf_swm(s, S, T1);
+    ROWS
     for (int j = 2; j <= s->nlat - 1; j++)
       for (int i = 1; i <= s->nlon; i++)
         A(s, T1[T_DU], i, j) = pass == 0
             ? A(s, T1[T_QHV], i, j) - A(s, T1[T_PGF_LON], i, j) - A(s, T1[T_DKEDLON], i, j)
             : A(s, T1[T_QHV], i, j) - A(s, T1[T_DKEDLON], i, j);
+    ROWS
     for (int j = 1; j <= s->nhalf; j++)
       for (int i = 1; i <= s->nlon; i++)
         A(s, T1[T_DV], i, j) = pass == 0
             ? -A(s, T1[T_QHU], i, j) - A(s, T1[T_PGF_LAT], i, j) - A(s, T1[T_DKEDLAT], i, j)
             : -A(s, T1[T_QHU], i, j) - A(s, T1[T_DKEDLAT], i, j);
+    ROWS
     for (int j = 1; j <= s->nlat; j++)
       for (int i = 1; i <= s->nlon; i++)
         A(s, T1[T_DGZ], i, j) = -(A(s, T1[T_DMFDLON], i, j) + A(s, T1[T_DMFDLAT], i, j)) * g;
@@ -476,9 +518,11 @@ int gmo_sw_space_operators(gmo_sw *s, int old_level, int star_level, int new_lev
     /* gmcore_mod.F90:552-571 */
     calc_m(s, N);
     pgf_swm(s, N, T1);
+    ROWS
     for (int j = 2; j <= s->nlat - 1; j++)
       for (int i = 1; i <= s->nlon; i++)
         A(s, T1[T_DU], i, j) = A(s, T2[T_DU], i, j) - A(s, T1[T_PGF_LON], i, j);
+    ROWS
     for (int j = 1; j <= s->nhalf; j++)
       for (int i = 1; i <= s->nlon; i++)
         A(s, T1[T_DV], i, j) = A(s, T2[T_DV], i, j) - A(s, T1[T_PGF_LAT], i, j);
@@ -491,6 +535,7 @@ int gmo_sw_space_operators(gmo_sw *s, int old_level, int star_level, int new_lev
 int gmo_sw_update_state(gmo_sw *s, int tend, int old_level, int new_level, double dt, int what) {
   double **O = s->st[old_level], **N = s->st[new_level], **T = s->td[tend];
   if (what & 1) {
+    ROWS
     for (int j = 1; j <= s->nlat; j++)
       for (int i = 1; i <= s->nlon; i++)
         A(s, N[S_GZ], i, j) = A(s, O[S_GZ], i, j) + dt * A(s, T[T_DGZ], i, j);
@@ -501,9 +546,11 @@ int gmo_sw_update_state(gmo_sw *s, int tend, int old_level, int new_level, doubl
     fill_all(s, N[S_GZ], KF);
   }
   if (what & 2) {
+    ROWS
     for (int j = 2; j <= s->nlat - 1; j++)
       for (int i = 1; i <= s->nlon; i++)
         A(s, N[S_U_LON], i, j) = A(s, O[S_U_LON], i, j) + dt * A(s, T[T_DU], i, j);
+    ROWS
     for (int j = 1; j <= s->nhalf; j++)
       for (int i = 1; i <= s->nlon; i++)
         A(s, N[S_V_LAT], i, j) = A(s, O[S_V_LAT], i, j) + dt * A(s, T[T_DV], i, j);
@@ -533,6 +580,7 @@ static void div_damp(gmo_sw *s, double **N) {
   const int o4 = s->cfg.div_damp_order == 4;
   const double c2 = s->cfg.div_damp_coef2, c4 = s->cfg.div_damp_coef4;
   const double *d = o4 ? N[S_DIV2] : N[S_DIV];
+  ROWS
   for (int j = 2; j <= s->nlat - 1; j++) {
     const double le = MS(s, GMC_LE_LON, j), de = MS(s, GMC_DE_LON, j);
     const double c = o4 ? c4 * (le * le) * (de * de) : c2 * le * de;
@@ -541,6 +589,7 @@ static void div_damp(gmo_sw *s, double **N) {
       A(s, N[S_U_LON], i, j) = o4 ? A(s, N[S_U_LON], i, j) - t : A(s, N[S_U_LON], i, j) + t;
     }
   }
+  ROWS
   for (int j = 1; j <= s->nhalf; j++) {
     const double le = MS(s, GMC_LE_LAT, j), de = MS(s, GMC_DE_LAT, j);
     const double c = o4 ? c4 * (le * le) * (de * de) : c2 * le * de;
