@@ -161,3 +161,48 @@ def test_tvd_limiters(cfl):
         for fm1_, C in ((-9.0, 2.0), (-0.5, 1.25)):
             expect = 0.5 * (2.0 + 1.0 - ((1 - C) * 1.0 + cfl * C) * 1.0)
             assert oracle.tvd("mc", cfl, fm1_, 1.0, 2.0, 2.5) == pytest.approx(expect, rel=1e-5)
+
+
+def _response(w, n, k, dl):
+    """frequency response of a centred symmetric filter of n taps at zonal wavenumber k"""
+    h = (n - 1) // 2
+    return float(sum(w[m] * math.cos(k * (m - h) * dl) for m in range(n)))
+
+
+def test_update_state_filter_sequence():
+    """update_state (time_schemes_mod.F90:178-239) on a state with zero
+    tendencies: gz_f is the big filter of the unfiltered new gz and gz the
+    small filter of it (not one filter after the other), u_f and v_f the big
+    filters of u and v with their own (U-row, V-row) tables.  On a single
+    zonal wavenumber each filter multiplies the mode by its frequency
+    response, formed here from the filter tables (pinned separately); a
+    swapped order or table gives the product or the other table's
+    response instead."""
+    st = _setup()
+    dl = st.mesh.dlon
+    big, small = st.big, st.small
+    r = 3                                     # a polar full row with both filters on
+    assert big.ngrid_lon[r] > 1 and small.ngrid_lon[r] > 1
+    hr = 2                                    # a polar half row with the big filter on
+    assert big.ngrid_lat[hr] > 1
+    k, a, phi0 = NLON // 6, 100.0, 5.0e4
+    lam = np.arange(NLON) * dl
+    gz = np.full((NLAT, NLON), phi0)
+    gz[r] += a * np.cos(k * lam)
+    u = np.zeros((NLAT, NLON))
+    u[r] = np.cos(k * (lam + dl / 2))
+    v = np.zeros((NLAT - 1, NLON))
+    v[hr] = np.sin(k * lam)
+    o = _with_winds(st, u, v, gz)
+    o.update_state(1, 1, 2, 1800.0, 3)        # tend level 1 is all zero
+    B = _response(big.wgt_lon[r], big.ngrid_lon[r], k, dl)
+    S = _response(small.wgt_lon[r], small.ngrid_lon[r], k, dl)
+    Bu = B
+    Bv = _response(big.wgt_lat[hr], big.ngrid_lat[hr], k, dl)
+    assert min(abs(B - S), abs(B - B * S), abs(S - B * S), abs(B - Bv)) > 1e-3, (B, S, Bv)
+    gz_f, gz_new = o.get(2, "gz_f"), o.get(2, "gz")
+    assert np.abs(gz_f[r] - phi0 - B * a * np.cos(k * lam)).max() <= 1e-9
+    assert np.abs(gz_new[r] - phi0 - S * a * np.cos(k * lam)).max() <= 1e-9
+    assert np.abs(o.get(2, "u_f")[r] - Bu * u[r]).max() <= 1e-12
+    assert np.abs(o.get(2, "v_f")[hr] - Bv * v[hr]).max() <= 1e-12
+    assert np.array_equal(o.get(2, "u_lon"), u) and np.array_equal(o.get(2, "v_lat")[1:-1], v[1:-1])
