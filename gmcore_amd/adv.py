@@ -42,7 +42,8 @@ class AdvSetup:
         self.cfg = GmcAdvConfig(radius=nl.radius, dt_adv=nl.dt_adv, nstep=int(nl.dt_adv / dt_dyn),
                                 ntracers=ntracers, lon_halo_width=hw,
                                 limiter=capi.LIMITERS[nl.limiter_type],
-                                lon_damp_passes=lon_damp_passes, reserved=0,
+                                lon_damp_passes=lon_damp_passes,
+                                flux_type=capi.FLUX_TYPES[nl.ffsl_flux_type],
                                 lon_damp_wgt=dptr(self.wgt))
         self.mesh_struct = self.mesh.struct()
 

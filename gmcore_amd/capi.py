@@ -75,13 +75,15 @@ class GmcConfig(C.Structure):
 
 GMC_LIMITER_MONO, GMC_LIMITER_PD, GMC_LIMITER_NONE = 0, 1, 2
 LIMITERS = {"mono": GMC_LIMITER_MONO, "pd": GMC_LIMITER_PD, "none": GMC_LIMITER_NONE}
+GMC_FLUX_PPM, GMC_FLUX_VAN_LEER = 0, 1
+FLUX_TYPES = {"ppm": GMC_FLUX_PPM, "van_leer": GMC_FLUX_VAN_LEER}
 
 
 class GmcAdvConfig(C.Structure):
     """gmc_adv_config of include/gmcore_adv.h"""
     _fields_ = [("radius", C.c_double), ("dt_adv", C.c_double), ("nstep", C.c_int32),
                 ("ntracers", C.c_int32), ("lon_halo_width", C.c_int32), ("limiter", C.c_int32),
-                ("lon_damp_passes", C.c_int32), ("reserved", C.c_int32),
+                ("lon_damp_passes", C.c_int32), ("flux_type", C.c_int32),
                 ("lon_damp_wgt", C.POINTER(C.c_double))]
 
 

@@ -283,6 +283,51 @@ __device__ __forceinline__ double xflux_g(const AdvGrid &g, int lim, double c, d
   return c == 0 ? 0 : fx;
 }
 
+// van Leer fluxes (hflx_van_leer_cell, ffsl_mod.F90:577-611) of the same
+// stencils: along x the upwind cell's value plus half its limited slope
+// times 1 - cf (c > 0) or minus it times 1 + cf (c < 0), scaled by cf, plus
+// or minus the whole cells crossed, times u / c; along y the upwind cell's
+// value plus half its limited slope times sign(c) - c, times v.
+__device__ __forceinline__ double xflux_vl(const AdvGrid &g, int lim, double c, double u, int i,
+                                           const double *row) {
+  const int n = g.nlon;
+  int ci = (int)c;
+  const double cf = c - ci;
+  ci = ci > g.hw ? g.hw : (ci < -g.hw ? -g.hw : ci);
+  const bool up = c > 0;
+  const int iu = wrap(up ? i - ci : i - ci + 1, n);
+  const double sum = run_sum(row, up ? i + 1 - ci : i + 1, up ? ci : -ci, n);
+  const double dm = slope(lim, row[wrap(iu - 1, n)], row[iu], row[wrap(iu + 1, n)]);
+  const double a = up ? row[iu] + dm * 0.5 * (1 - cf) : row[iu] - dm * 0.5 * (1 + cf);
+  const double fx = u * (up ? cf * a + sum : cf * a - sum) / c;
+  return c == 0 ? 0 : fx;
+}
+__device__ __forceinline__ double yflux_vl(const AdvGrid &g, int lim, double c, double v, int h, int i,
+                                           const double *my) {
+  const int ih = wrap(i + g.half, g.nlon), P = (int)g.pitch;
+  double f[4];   // full rows h-1 .. h+2, beyond a pole mirrored
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int r = h - 1 + k;
+    const bool mir = r < 0 || r > g.nlat - 1;
+    const int rr = r < 0 ? -r : (r > g.nlat - 1 ? 2 * (g.nlat - 1) - r : r);
+    f[k] = my[rr * P + (mir ? ih : i)];
+  }
+  const bool up = c > 0;   // upwind row h, else h + 1
+  const double fu = up ? f[1] : f[2];
+  const double dm = slope(lim, up ? f[0] : f[1], fu, up ? f[2] : f[3]);
+  return v * (fu + dm * 0.5 * (copysign(1.0, c) - c));
+}
+template <bool VL>
+__device__ __forceinline__ double xflux(const AdvGrid &g, int lim, double c, double u, int i, const double *row) {
+  return VL ? xflux_vl(g, lim, c, u, i, row) : xflux_g(g, lim, c, u, i, row);
+}
+template <bool VL>
+__device__ __forceinline__ double yflux_any(const AdvGrid &g, int lim, double c, double v, int h, int i,
+                                            const double *my) {
+  return VL ? yflux_vl(g, lim, c, v, h, i, my) : yflux(g, lim, c, v, h, i, my);
+}
+
 // Work of the two flux kernels, in blocks of NF = 8 wavefronts.  Items
 // [0, npole) are the pole rows (2 per tracer, padded to a multiple of the
 // XCD count so that the interior items keep their XCD placement).  An
@@ -345,6 +390,7 @@ __device__ __forceinline__ double wave_row_sum(int n, F f) {
 
 // ffsl_calc_tracer_hflx_cell, inner part (ffsl_mod.F90:350-407): the
 // advective half-steps qx, qy of every cell
+template <bool VL>
 __global__ void __launch_bounds__(NF) k_adv_inner(AdvGrid g, Mesh M, int lim, int ntr, InnerPtrs p) {
   __shared__ double sfy[NF / kWave][kWave];
   __shared__ double bcast;
@@ -357,7 +403,7 @@ __global__ void __launch_bounds__(NF) k_adv_inner(AdvGrid g, Mesh M, int lim, in
     // pole rows: qx = q, qy from the zonal sum of the adjacent V row's flux
     const int r = it.r, h = r == 0 ? 0 : g.nhalf - 1;
     if (threadIdx.x < kWave) {
-      const double s = wave_row_sum(n, [&](int i) { return yflux(g, lim, p.cfly[h * P + i], p.v[h * P + i], h, i, q); });
+      const double s = wave_row_sum(n, [&](int i) { return yflux_any<VL>(g, lim, p.cfly[h * P + i], p.v[h * P + i], h, i, q); });
       if (threadIdx.x == 0) bcast = s;
     }
     __syncthreads();
@@ -374,9 +420,9 @@ __global__ void __launch_bounds__(NF) k_adv_inner(AdvGrid g, Mesh M, int lim, in
   const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
   const int iw = wrap(it.c0 - 1 + l, n);   // column of the lane
   const int h = it.r - 1 + w, r = it.r + w;
-  if (h <= g.nhalf - 1) sfy[w][l] = yflux(g, lim, p.cfly[h * P + iw], p.v[h * P + iw], h, iw, q);
+  if (h <= g.nhalf - 1) sfy[w][l] = yflux_any<VL>(g, lim, p.cfly[h * P + iw], p.v[h * P + iw], h, iw, q);
   const bool row = w < kRowsPerItem && r <= g.nlat - 2;   // uniform over the wavefront
-  const double fx = row ? xflux_g(g, lim, p.cflx[r * P + iw], p.u[r * P + iw], iw, q + r * P) : 0.0;
+  const double fx = row ? xflux<VL>(g, lim, p.cflx[r * P + iw], p.u[r * P + iw], iw, q + r * P) : 0.0;
   const double fxw = __shfl_up(fx, 1, kWave);
   __syncthreads();
   const int i = it.c0 - 1 + l;   // cell of the lane
@@ -401,6 +447,7 @@ struct OuterPtrs {
 
 // outer fluxes (ffsl_mod.F90:410) and adv_run's flux-form update and /m
 // (adv_mod.F90:183-235) of every cell, into qp
+template <bool VL>
 __global__ void __launch_bounds__(NF) k_adv_flux(AdvGrid g, Mesh M, int lim, int ntr, OuterPtrs p) {
   __shared__ double sfy[NF / kWave][kWave];
   __shared__ double bcast;
@@ -413,7 +460,7 @@ __global__ void __launch_bounds__(NF) k_adv_flux(AdvGrid g, Mesh M, int lim, int
   if (it.pole) {
     const int r = it.r, h = r == 0 ? 0 : g.nhalf - 1;
     if (threadIdx.x < kWave) {
-      const double s = wave_row_sum(n, [&](int i) { return yflux(g, lim, p.cfly[h * P + i], p.mfy[h * P + i], h, i, qx); });
+      const double s = wave_row_sum(n, [&](int i) { return yflux_any<VL>(g, lim, p.cfly[h * P + i], p.mfy[h * P + i], h, i, qx); });
       if (threadIdx.x == 0) bcast = s;
     }
     __syncthreads();
@@ -429,9 +476,9 @@ __global__ void __launch_bounds__(NF) k_adv_flux(AdvGrid g, Mesh M, int lim, int
   const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
   const int iw = wrap(it.c0 - 1 + l, n);
   const int h = it.r - 1 + w, r = it.r + w;
-  if (h <= g.nhalf - 1) sfy[w][l] = yflux(g, lim, p.cfly[h * P + iw], p.mfy[h * P + iw], h, iw, qx);
+  if (h <= g.nhalf - 1) sfy[w][l] = yflux_any<VL>(g, lim, p.cfly[h * P + iw], p.mfy[h * P + iw], h, iw, qx);
   const bool row = w < kRowsPerItem && r <= g.nlat - 2;
-  const double fx = row ? xflux_g(g, lim, p.cflx[r * P + iw], p.mfx[r * P + iw], iw, qy + r * P) : 0.0;
+  const double fx = row ? xflux<VL>(g, lim, p.cflx[r * P + iw], p.mfx[r * P + iw], iw, qy + r * P) : 0.0;
   const double fxw = __shfl_up(fx, 1, kWave);
   __syncthreads();
   const int i = it.c0 - 1 + l;
@@ -638,6 +685,7 @@ int run_once(gmc_adv *a, int64_t time_step, bool basis, double scale, char *tape
   const AdvGrid &g = a->g;
   const Mesh M{a->d_mesh, g.nlat};
   const int ns = a->cfg.nstep;
+  const bool vl = a->cfg.flux_type == GMC_FLUX_VAN_LEER;
   // the two counters advance in step (adv_batch_mod.F90:238-251, 389-401)
   const int mode = a->uv_step == 0 ? 0 : (a->uv_step == ns ? 1 : 2);
   AccumPtrs ap{a->u, a->v, a->mfx, a->mfy, a->su, a->sv, a->smfx, a->smfy,
@@ -660,10 +708,12 @@ int run_once(gmc_adv *a, int64_t time_step, bool basis, double scale, char *tape
     auto T = [&](double *p) { return tape ? (double *)(tape + ((char *)p - (char *)a->slab)) : p; };
     double *qx = T(a->qx), *qy = T(a->qy), *qp = T(a->qp);
     InnerPtrs ip{tracer_ptr(a, a->old, 0), qx, qy, a->u, a->v, a->cflx, a->cfly, a->divx, a->divy, ts};
-    hipLaunchKernelGGL(k_adv_inner, grid, dim3(NF), 0, a->stream, g, M, a->cfg.limiter, ntr, ip);
+    hipLaunchKernelGGL(vl ? k_adv_inner<true> : k_adv_inner<false>, grid, dim3(NF), 0, a->stream, g, M,
+                       a->cfg.limiter, ntr, ip);
     mark(a, KA_INNER);
     OuterPtrs op{qx, qy, tracer_ptr(a, a->old, 0), qp, a->mfx, a->mfy, a->cflx, a->cfly, a->old_m, a->sm, ts};
-    hipLaunchKernelGGL(k_adv_flux, grid, dim3(NF), 0, a->stream, g, M, a->cfg.limiter, ntr, op);
+    hipLaunchKernelGGL(vl ? k_adv_flux<true> : k_adv_flux<false>, grid, dim3(NF), 0, a->stream, g, M,
+                       a->cfg.limiter, ntr, op);
     mark(a, KA_FLUX);
     if (tape) {
       // the rest of the step's base state: m, the winds, CFL numbers, split
@@ -710,7 +760,8 @@ int gmc_adv_create(const gmc_mesh *mesh, const gmc_adv_config *cfg, gmc_adv **ou
   const int nlon = mesh->nlon, nlat = mesh->nlat;
   if (nlon < 8 || nlat < 5 || (nlon & 1) || nlon > 4096 || cfg->ntracers < 1 || cfg->nstep < 1 ||
       cfg->lon_halo_width < 1 || cfg->lon_halo_width > nlon / 2 - 4 || cfg->limiter < 0 ||
-      cfg->limiter > 2 || cfg->lon_damp_passes < 0 || !(cfg->dt_adv > 0) || !(cfg->radius > 0))
+      cfg->limiter > 2 || cfg->lon_damp_passes < 0 || !(cfg->dt_adv > 0) || !(cfg->radius > 0) ||
+      (cfg->flux_type != GMC_FLUX_PPM && cfg->flux_type != GMC_FLUX_VAN_LEER))
     return GMC_EINVAL;
   int dev = 0;
   hipDeviceProp_t prop;
@@ -1631,6 +1682,8 @@ void ad_step(gmc_adv *a, const char *slot, int old, const double *in, double *ou
 
 int tl_check(gmc_adv *a, const double *scale, int32_t nsteps, int64_t time_step0, const double *buf, int64_t ld) {
   if (!a->have_basis) return aerr(a, GMC_ESTATE, "gmc_adv_set_wind_basis must come first");
+  if (a->cfg.flux_type != GMC_FLUX_PPM)
+    return aerr(a, GMC_EINVAL, "the tangent linear and adjoint cover the PPM flux only");
   if (!scale || nsteps < 0 || time_step0 < 0 || !buf || ld < a->g.nlon)
     return aerr(a, GMC_EINVAL, "bad tangent-linear / adjoint arguments");
   return GMC_OK;

@@ -50,7 +50,7 @@ contains
     case default;  cfg%limiter = GMC_LIMITER_MONO
     end select
     cfg%lon_damp_passes = 3
-    cfg%reserved = 0
+    cfg%flux_type = merge(GMC_FLUX_VAN_LEER, GMC_FLUX_PPM, ffsl_flux_type == 'van_leer')
     cfg%lon_damp_wgt = c_loc(batch%wgt)
     call gmc_check(batch%adv, gmc_adv_create(mesh, cfg, batch%adv), 'gmc_adv_create')
   end subroutine adv_init

@@ -38,10 +38,11 @@ module gmc_hip_mod
   end type gmc_config
 
   integer(c_int32_t), parameter :: GMC_LIMITER_MONO = 0, GMC_LIMITER_PD = 1, GMC_LIMITER_NONE = 2
+  integer(c_int32_t), parameter :: GMC_FLUX_PPM = 0, GMC_FLUX_VAN_LEER = 1
 
   type, bind(C) :: gmc_adv_config
     real(c_double) :: radius, dt_adv
-    integer(c_int32_t) :: nstep, ntracers, lon_halo_width, limiter, lon_damp_passes, reserved
+    integer(c_int32_t) :: nstep, ntracers, lon_halo_width, limiter, lon_damp_passes, flux_type
     type(c_ptr) :: lon_damp_wgt
   end type gmc_adv_config
 

@@ -34,6 +34,13 @@ enum gmc_limiter {
   GMC_LIMITER_NONE = 2   /* 'none' (slope_simple, ffsl_mod.F90:1175-1181) */
 };
 
+/* horizontal flux of the FFSL tracer step (namelist ffsl_flux_type,
+ * ffsl_mod.F90:120-129) */
+enum gmc_flux_type {
+  GMC_FLUX_PPM = 0,      /* 'ppm', the default (hflx_ppm_cell, ffsl_mod.F90:778-881)         */
+  GMC_FLUX_VAN_LEER = 1  /* 'van_leer' (hflx_van_leer_cell, ffsl_mod.F90:556-616); no TL/AD */
+};
+
 typedef struct gmc_adv_config {
   double radius;            /* sphere radius, const_mod radius                     */
   double dt_adv;            /* batch%dt: advection time step in seconds            */
@@ -42,7 +49,7 @@ typedef struct gmc_adv_config {
   int32_t lon_halo_width;   /* bound on |cflx| (adv_batch_mod.F90:284-287); below nlon/2 */
   int32_t limiter;          /* enum gmc_limiter                                    */
   int32_t lon_damp_passes;  /* lon_damp_on_cell(order 2) calls per step (3, adv_mod.F90:299-301) */
-  int32_t reserved;
+  int32_t flux_type;        /* enum gmc_flux_type                                  */
   const double *lon_damp_wgt; /* nlat weights wgt_full_lat of lon_damp_init
                                  (src/dynamics/damp/lon_damp_mod.F90:43-68); pole rows unused */
 } gmc_adv_config;

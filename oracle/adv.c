@@ -328,6 +328,50 @@ static void hflx_ppm_cell(gmo_adv *s, const double *u, const double *v, const do
     }
 }
 
+/* hflx_van_leer_cell (ffsl_mod.F90:556-616): van Leer fluxes of mx along x
+ * (the upwind cell's limited slope plus the whole cells crossed, over
+ * cflx) and of my along y (the upwind cell's value plus half its limited
+ * slope times sign(cfly) - cfly), for winds u, v. */
+static void hflx_van_leer_cell(gmo_adv *s, const double *u, const double *v, const double *mx,
+                               const double *my, double *mfx, double *mfy) {
+  const int nlon = s->nlon, nlat = s->nlat, nhalf = s->nhalf, lim = s->cfg.limiter;
+  for (int j = 2; j <= nlat - 1; j++)
+    for (int i = 1; i <= nlon; i++) {
+      const double c = A(s, s->cflx, i, j);
+      const int ci = (int)c;  /* int(): truncation toward zero */
+      const double cf = c - ci;
+      if (c > 0) {
+        const int iu = i - ci;
+        const double dm = slope(lim, A(s, mx, iu - 1, j), A(s, mx, iu, j), A(s, mx, iu + 1, j));
+        double sum = 0;
+        for (int k = i + 1 - ci; k <= i; k++) sum = sum + A(s, mx, k, j);
+        A(s, mfx, i, j) = A(s, u, i, j) * (cf * (A(s, mx, iu, j) + dm * 0.5 * (1 - cf)) + sum) / c;
+      } else if (c < 0) {
+        const int iu = i - ci + 1;
+        const double dm = slope(lim, A(s, mx, iu - 1, j), A(s, mx, iu, j), A(s, mx, iu + 1, j));
+        double sum = 0;
+        for (int k = i + 1; k <= i - ci; k++) sum = sum + A(s, mx, k, j);
+        A(s, mfx, i, j) = A(s, u, i, j) * (cf * (A(s, mx, iu, j) - dm * 0.5 * (1 + cf)) - sum) / c;
+      } else {
+        A(s, mfx, i, j) = 0;
+      }
+    }
+  for (int j = 1; j <= nhalf; j++)
+    for (int i = 1; i <= nlon; i++) {
+      const double cf = A(s, s->cfly, i, j);
+      const int ju = cf > 0 ? j : j + 1;
+      const double dm = slope(lim, A(s, my, i, ju - 1), A(s, my, i, ju), A(s, my, i, ju + 1));
+      A(s, mfy, i, j) = A(s, v, i, j) * (A(s, my, i, ju) + dm * 0.5 * (copysign(1.0, cf) - cf));
+    }
+}
+
+/* the hflx procedure pointer of ffsl_init (ffsl_mod.F90:120-129) */
+static void hflx(gmo_adv *s, const double *u, const double *v, const double *mx, const double *my,
+                 double *mfx, double *mfy) {
+  if (s->cfg.flux_type == GMC_FLUX_VAN_LEER) hflx_van_leer_cell(s, u, v, mx, my, mfx, mfy);
+  else hflx_ppm_cell(s, u, v, mx, my, mfx, mfy);
+}
+
 /* ffsl_calc_tracer_hflx_cell (ffsl_mod.F90:318-413): inner advective
  * half-steps qx, qy from the PPM fluxes of q, then the outer fluxes of qy
  * along x and qx along y. */
@@ -335,7 +379,7 @@ static void tracer_hflx(gmo_adv *s, double *q) {
   const int nlon = s->nlon, nlat = s->nlat, nhalf = s->nhalf;
   const double dt = s->cfg.dt_adv;
   double *qmfx = s->qmf_lon, *qmfy = s->qmf_lat;
-  hflx_ppm_cell(s, s->u, s->v, q, q, qmfx, qmfy);
+  hflx(s, s->u, s->v, q, q, qmfx, qmfy);
   fill_halo(s, qmfx, KF, 1, 1, 0, 0);
   fill_halo(s, qmfy, KH, 0, 0, 1, 1);
   for (int j = 2; j <= nlat - 1; j++)
@@ -365,7 +409,7 @@ static void tracer_hflx(gmo_adv *s, double *q) {
   }
   fill_halo(s, s->qx, KF, 0, 0, 1, 1);
   fill_halo(s, s->qy, KF, 1, 1, 0, 0);
-  hflx_ppm_cell(s, s->mfx, s->mfy, s->qy, s->qx, qmfx, qmfy);
+  hflx(s, s->mfx, s->mfy, s->qy, s->qx, qmfx, qmfy);
 }
 
 /* lon_damp_1d, order 2 (lon_damp_mod.F90:200-241): one Jacobi sweep of
@@ -1109,7 +1153,7 @@ static void ad_tracer(gmo_adv *s, struct gmo_tlad *w, int l) {
  * interior arrays.  A step that does not advance leaves them unchanged. */
 int gmo_adv_tlad_run(gmo_adv *s, int64_t time_step, double *dq, double *lam, int64_t ld) {
   if (s->err) return s->err;
-  if (ld < s->nlon) return GMC_EINVAL;
+  if (ld < s->nlon || s->cfg.flux_type != GMC_FLUX_PPM) return GMC_EINVAL;   /* TL/AD of the PPM flux only */
   if (!(time_step > 0 && time_step % s->cfg.nstep == 0)) return gmo_adv_run(s, time_step);
   accum_uv_cell(s);
   accum_mf_cell(s);

@@ -19,8 +19,8 @@ from gmcore_amd.namelist import Namelist
 pytestmark = pytest.mark.gpu
 
 
-def _setup(nlon, nlat, dt, ntracers, hw, limiter="mono", passes=3):
-    nl = Namelist(num_lon=nlon, num_lat=nlat, dt_adv=dt, dt_dyn=dt, limiter_type=limiter)
+def _setup(nlon, nlat, dt, ntracers, hw, limiter="mono", passes=3, flux="ppm"):
+    nl = Namelist(num_lon=nlon, num_lat=nlat, dt_adv=dt, dt_dyn=dt, limiter_type=limiter, ffsl_flux_type=flux)
     return AdvSetup(nl, ntracers, lon_halo_width=hw, lon_damp_passes=passes)
 
 
@@ -44,20 +44,25 @@ def _compare(o, g, ntr, what):
 
 
 CASES = [
-    # nlon, nlat, dt, ntracers, hw, limiter, lmax, vmax, steps
+    # nlon, nlat, dt, ntracers, hw, limiter, lmax, vmax, steps[, flux type]
     (72, 37, 3600.0, 3, 20, "mono", 6, 40.0, 9),
     (72, 37, 3600.0, 2, 20, "none", 6, 40.0, 6),
     (72, 37, 3600.0, 2, 20, "pd", 6, 40.0, 6),
     (180, 91, 1800.0, 8, 60, "mono", 10, 50.0, 7),
+    (72, 37, 3600.0, 3, 20, "mono", 6, 40.0, 7, "van_leer"),
+    (72, 37, 3600.0, 2, 20, "none", 6, 40.0, 6, "van_leer"),
+    (180, 91, 1800.0, 4, 60, "pd", 10, 50.0, 5, "van_leer"),
 ]
 
 
-@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[0]}x{c[1]}-{c[5]}-{c[3]}tr")
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[0]}x{c[1]}-{c[5]}-{c[3]}tr"
+                         + (f"-{c[9]}" if len(c) > 9 else ""))
 def test_prescribed_flow_bitwise(case):
     """gmc_adv_drive (device wind basis x cos(pi t / T)) against the oracle
-    driven step by step with the same winds."""
-    nlon, nlat, dt, ntr, hw, lim, lmax, vmax, steps = case
-    st = _setup(nlon, nlat, dt, ntr, hw, lim)
+    driven step by step with the same winds; PPM and van Leer fluxes."""
+    nlon, nlat, dt, ntr, hw, lim, lmax, vmax, steps = case[:9]
+    flux = case[9] if len(case) > 9 else "ppm"
+    st = _setup(nlon, nlat, dt, ntr, hw, lim, flux=flux)
     m = st.mesh.metrics
     ub, vb = init_cond.deform_wind_basis(nlon, nlat, m, lmax, 7, vmax)
     qs = init_cond.tracer_set(nlon, nlat, m, 3, ntr)
