@@ -101,6 +101,18 @@ class Advection:
     def run(self, time_step: int):
         self._check(self.lib.gmc_adv_run(self.h, time_step), "adv_run")
 
+    def init_dyn(self, model, level=None):
+        """gmcore_run's accumulation before the time loop from a prepared
+        ShallowWater context's state (gmc_adv_init_dyn)."""
+        lv = model.level if level is None else level
+        self._check(self.lib.gmc_adv_init_dyn(self.h, model.ctx, lv), "init_dyn")
+
+    def run_dyn(self, model, time_step: int, level=None):
+        """adv_run(block, new) on a ShallowWater context's state after its
+        step (gmc_adv_run_dyn); the model's current level is that state."""
+        lv = model.level if level is None else level
+        self._check(self.lib.gmc_adv_run_dyn(self.h, model.ctx, lv, time_step), "run_dyn")
+
     def set_wind_basis(self, u, v):
         ub, vb = self._arr(u, self.nlat), self._arr(v, self.nlat - 1)
         self._check(self.lib.gmc_adv_set_wind_basis(self.h, dptr(ub), dptr(vb), self.nlon), "set_wind_basis")

@@ -142,6 +142,8 @@ def load_hip() -> C.CDLL:
                                        P(C.c_double), P(C.c_double), C.c_int64]),
         "gmc_adv_run": (C.c_int, [vp, C.c_int64]),
         "gmc_adv_set_wind_basis": (C.c_int, [vp, P(C.c_double), P(C.c_double), C.c_int64]),
+        "gmc_adv_run_dyn": (C.c_int, [vp, vp, C.c_int32, C.c_int64]),
+        "gmc_adv_init_dyn": (C.c_int, [vp, vp, C.c_int32]),
         "gmc_adv_drive": (C.c_int, [vp, C.c_int64, P(C.c_double), C.c_int32]),
         "gmc_adv_old_level": (C.c_int, [vp, P(C.c_int32)]),
         "gmc_adv_mass": (C.c_int, [vp, C.c_int32, P(C.c_double)]),

@@ -641,6 +641,9 @@ struct gmc_adv {
   // against the segment recomputes on the object's stream
   hipStream_t astream = nullptr;
   hipEvent_t ev_fwd = nullptr, ev_join = nullptr, ev_ready[2] = {}, ev_done[2] = {};
+  // coupling to a dynamics context: its stream's work before the wind copy,
+  // and the copy's end for the dynamics stream to wait on
+  hipEvent_t ev_dyn_in = nullptr, ev_dyn_out = nullptr;
   std::string err;
 };
 
@@ -751,6 +754,66 @@ int check_flags(gmc_adv *a) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Coupling to the shallow-water dynamics (time_integrate, gmcore_mod.F90:588):
+// block%state(itime) of the advection is a state level of a dynamics context.
+
+// m = (gz - gzs) / g of a dynamics state (calc_m, operators_mod.F90:527-533):
+// the mass the backward pass's calc_m leaves in state(new)
+__global__ void __launch_bounds__(NA) k_adv_dyn_m(AdvGrid g, const double *__restrict__ gz,
+                                                 const double *__restrict__ gzs, double grav, double rgrav,
+                                                 double *m) {
+  const int r = blockIdx.y, i = blockIdx.x * NA + threadIdx.x;
+  if (i >= g.nlon) return;
+  const int64_t k = r * g.pitch + i;
+  m[k] = div_rn(gz[k] - gzs[k], grav, rgrav);
+}
+
+// f = value on rows 0 .. gridDim.y - 1 of a field
+__global__ void __launch_bounds__(NA) k_adv_fill(AdvGrid g, double *f, double value) {
+  const int r = blockIdx.y, i = blockIdx.x * NA + threadIdx.x;
+  if (i < g.nlon) f[r * g.pitch + i] = value;
+}
+
+// block%state(level) of `dyn` as the advection's state: u_f, v_f, mfx_lon,
+// mfy_lat (zero when zero_mf: the mass fluxes before any operator has run)
+// and m, copied on the batch's stream after the dynamics stream's work so
+// far; the dynamics stream then waits for the copies.
+int dyn_state(gmc_adv *a, gmc_ctx *c, int level, bool zero_mf) {
+  const AdvGrid &g = a->g;
+  if (c->grid.nlon != g.nlon || c->grid.nlat != g.nlat || c->grid.pitch != g.pitch)
+    return aerr(a, GMC_EINVAL, "the dynamics context is on another grid");
+  if (level < 1 || level > 3) return aerr(a, GMC_EINVAL, "level must be 1..3");
+  int rc = GMC_OK;
+  for (hipEvent_t *e : {&a->ev_dyn_in, &a->ev_dyn_out})
+    if (!rc && !*e) rc = ahip(a, hipEventCreateWithFlags(e, hipEventDisableTiming), "event");
+  if (!rc) rc = ahip(a, hipEventRecord(a->ev_dyn_in, c->stream), "event record");
+  if (!rc) rc = ahip(a, hipStreamWaitEvent(a->stream, a->ev_dyn_in, 0), "stream wait");
+  if (rc) return rc;
+  double **S = c->state[level];
+  const size_t full = (size_t)g.nlat * g.pitch * sizeof(double), half = (size_t)g.nhalf * g.pitch * sizeof(double);
+  rc = ahip(a, hipMemcpyAsync(a->su, S[S_U_F], full, hipMemcpyDeviceToDevice, a->stream), "copy u_f");
+  if (!rc) rc = ahip(a, hipMemcpyAsync(a->sv, S[S_V_F], half, hipMemcpyDeviceToDevice, a->stream), "copy v_f");
+  if (!rc && zero_mf) {
+    // zeros through a fill kernel rather than a memset node, so that the
+    // call may be captured in a graph
+    hipLaunchKernelGGL(k_adv_fill, dim3((g.nlon + NA - 1) / NA, g.nlat), dim3(NA), 0, a->stream, g, a->smfx, 0.0);
+    hipLaunchKernelGGL(k_adv_fill, dim3((g.nlon + NA - 1) / NA, g.nhalf), dim3(NA), 0, a->stream, g, a->smfy, 0.0);
+  } else if (!rc) {
+    rc = ahip(a, hipMemcpyAsync(a->smfx, S[S_MFX_LON], full, hipMemcpyDeviceToDevice, a->stream), "copy mfx_lon");
+    if (!rc) rc = ahip(a, hipMemcpyAsync(a->smfy, S[S_MFY_LAT], half, hipMemcpyDeviceToDevice, a->stream),
+                       "copy mfy_lat");
+  }
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_adv_dyn_m, dim3((g.nlon + NA - 1) / NA, g.nlat), dim3(NA), 0, a->stream, g, S[S_GZ], c->gzs,
+                     c->cfg.g, 1.0 / c->cfg.g, a->sm);
+  a->old_m_current = false;
+  rc = ahip(a, hipGetLastError(), "dynamics state");
+  if (!rc) rc = ahip(a, hipEventRecord(a->ev_dyn_out, a->stream), "event record");
+  if (!rc) rc = ahip(a, hipStreamWaitEvent(c->stream, a->ev_dyn_out, 0), "stream wait");
+  return rc;
+}
+
 extern "C" {
 
 int gmc_adv_create(const gmc_mesh *mesh, const gmc_adv_config *cfg, gmc_adv **out) {
@@ -829,7 +892,8 @@ int gmc_adv_destroy(gmc_adv *a) {
   if (a->slab) (void)hipFree(a->slab);
   if (a->tl_f) (void)hipFree(a->tl_f);
   if (a->tl_ws) (void)hipFree(a->tl_ws);
-  for (hipEvent_t e : {a->ev_fwd, a->ev_join, a->ev_ready[0], a->ev_ready[1], a->ev_done[0], a->ev_done[1]})
+  for (hipEvent_t e : {a->ev_fwd, a->ev_join, a->ev_ready[0], a->ev_ready[1], a->ev_done[0], a->ev_done[1],
+                       a->ev_dyn_in, a->ev_dyn_out})
     if (e) (void)hipEventDestroy(e);
   if (a->astream) (void)hipStreamDestroy(a->astream);
   if (a->h_flags) (void)hipHostFree(a->h_flags);
@@ -879,6 +943,28 @@ int gmc_adv_run(gmc_adv *a, int64_t time_step) {
   if (!a) return GMC_EINVAL;
   if (time_step < 0) return aerr(a, GMC_EINVAL, "time_step < 0");
   int rc = run_once(a, time_step, false, 1.0);
+  if (!rc) rc = check_flags(a);
+  return rc;
+}
+
+int gmc_adv_init_dyn(gmc_adv *a, gmc_ctx *dyn, int32_t level) {
+  if (!a) return GMC_EINVAL;
+  if (!dyn || !dyn->prepared) return aerr(a, GMC_ESTATE, "the dynamics context must be prepared first");
+  int rc = dyn_state(a, dyn, level, true);
+  if (!rc) rc = run_once(a, 0, false, 1.0);
+  if (!rc) rc = ahip(a, hipStreamSynchronize(a->stream), "sync");
+  return rc;
+}
+
+int gmc_adv_run_dyn(gmc_adv *a, gmc_ctx *dyn, int32_t level, int64_t time_step) {
+  if (!a) return GMC_EINVAL;
+  if (!dyn) return aerr(a, GMC_EINVAL, "null dynamics context");
+  if (time_step < 0) return aerr(a, GMC_EINVAL, "time_step < 0");
+  if (level < 1 || level > 3 || !dyn->derived_kept[level])
+    return aerr(a, GMC_ESTATE, "the level does not hold the mass fluxes of its last forward pass "
+                               "(step the dynamics with gmc_keep_derived on)");
+  int rc = dyn_state(a, dyn, level, false);
+  if (!rc) rc = run_once(a, time_step, false, 1.0);
   if (!rc) rc = check_flags(a);
   return rc;
 }

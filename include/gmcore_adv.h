@@ -82,6 +82,25 @@ int gmc_adv_set_wind(gmc_adv *adv, const double *u_f, const double *v_f, const d
  * advance every tracer from level old to new and swap; finally copy_old_m. */
 int gmc_adv_run(gmc_adv *adv, int64_t time_step);
 
+/* adv_run(block, itime) with block%state(itime) = state `level` of the
+ * dynamics context `dyn` on the same grid (time_integrate, gmcore_mod.F90:588;
+ * adv_mod.F90:152-312, adv_accum_wind :420-455): the batch accumulates that
+ * state's u_f, v_f, mfx_lon and mfy_lat, advances its tracers when its
+ * cadence rings with m of that state, and keeps m as old_m.  As in the
+ * reference, mfx_lon and mfy_lat are those the level's last forward pass as a
+ * star state left (after a pc2 step: the second stage's), so `dyn` must have
+ * been stepped with gmc_keep_derived on (GMC_ESTATE otherwise).  The work is
+ * ordered after the dynamics stream's work and the dynamics stream waits for
+ * the copy of the fields; no device-host transfer. */
+int gmc_adv_run_dyn(gmc_adv *adv, gmc_ctx *dyn, int32_t level, int64_t time_step);
+
+/* What gmcore_run does for the batch before its time loop
+ * (gmcore_mod.F90:117-131): copy_old_m and adv_accum_wind(block, old) on the
+ * state the initial filters leave, whose mass fluxes no operator has formed
+ * yet (zero).  `dyn` must be prepared; call it once, before the first
+ * gmc_adv_run_dyn. */
+int gmc_adv_init_dyn(gmc_adv *adv, gmc_ctx *dyn, int32_t level);
+
 /* Prescribed-flow driver (gmcore_adv_driver.F90:84-92): upload once a wind
  * basis u_basis (nlat rows), v_basis (nlat-1 rows); gmc_adv_drive then does,
  * for n = 0 .. nsteps-1, set_uv with u_f = mfx_lon = u_basis * scale[n],

@@ -306,3 +306,38 @@ class AdvOracle:
         a = np.zeros((self.nlat - 1 if name in self.HALF else self.nlat, self.nlon))
         assert lib().gmo_adv_get_internal(self.h, self.INTERNAL.index(name), dptr(a), self.nlon) == 0
         return a
+
+
+class CoupledOracle:
+    """The oracle's time_integrate with tracers (gmcore_mod.F90:577-591): each
+    shallow-water step is followed by adv_run(block, new) (adv_mod.F90:
+    152-312) on the advection batch, whose block%state(new) fields are those
+    the SW oracle's state(new) holds: u_f, v_f, mfx_lon and mfy_lat as its
+    last forward pass left them, m of its backward pass."""
+
+    def __init__(self, sw: "SWOracle", adv: "AdvOracle", dt: float):
+        self.sw, self.adv, self.dt = sw, adv, dt
+        self.time_step = 0
+
+    def _state(self, lv, zero_mf=False):
+        sw = self.sw
+        mfx = sw.internal(lv, "mfx_lon")
+        mfy = sw.internal(lv, "mfy_lat")
+        if zero_mf:
+            mfx, mfy = np.zeros_like(mfx), np.zeros_like(mfy)
+        self.adv.set_wind(sw.get(lv, "u_f"), sw.get(lv, "v_f"), mfx, mfy, sw.internal(lv, "m"))
+
+    def init(self):
+        """gmcore_run:117-131: copy_old_m and adv_accum_wind(block, old) on the
+        filtered initial state, before any operator forms its mass fluxes."""
+        self._state(self.sw.level, zero_mf=True)
+        assert self.adv.run(0) == 0
+
+    def step(self):
+        """time_integrate + adv_run(block, new); then time_advance."""
+        self.sw.step(self.dt, 1)
+        self._state(self.sw.level)
+        rc = self.adv.run(self.time_step)
+        self.time_step += 1
+        return rc
+
