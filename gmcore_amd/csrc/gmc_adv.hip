@@ -644,6 +644,9 @@ struct gmc_adv {
   // coupling to a dynamics context: its stream's work before the wind copy,
   // and the copy's end for the dynamics stream to wait on
   hipEvent_t ev_dyn_in = nullptr, ev_dyn_out = nullptr;
+  // set when a failed adjoint window could not put the batch back at the
+  // window's end: every later call refuses with GMC_ESTATE
+  bool lost = false;
   std::string err;
 };
 
@@ -941,6 +944,7 @@ int gmc_adv_set_wind(gmc_adv *a, const double *u, const double *v, const double 
 
 int gmc_adv_run(gmc_adv *a, int64_t time_step) {
   if (!a) return GMC_EINVAL;
+  if (a->lost) return aerr(a, GMC_ESTATE, "a failed adjoint window left the batch mid-window; create it anew");
   if (time_step < 0) return aerr(a, GMC_EINVAL, "time_step < 0");
   int rc = run_once(a, time_step, false, 1.0);
   if (!rc) rc = check_flags(a);
@@ -949,6 +953,7 @@ int gmc_adv_run(gmc_adv *a, int64_t time_step) {
 
 int gmc_adv_init_dyn(gmc_adv *a, gmc_ctx *dyn, int32_t level) {
   if (!a) return GMC_EINVAL;
+  if (a->lost) return aerr(a, GMC_ESTATE, "a failed adjoint window left the batch mid-window; create it anew");
   if (!dyn || !dyn->prepared) return aerr(a, GMC_ESTATE, "the dynamics context must be prepared first");
   int rc = dyn_state(a, dyn, level, true);
   if (!rc) rc = run_once(a, 0, false, 1.0);
@@ -958,6 +963,7 @@ int gmc_adv_init_dyn(gmc_adv *a, gmc_ctx *dyn, int32_t level) {
 
 int gmc_adv_run_dyn(gmc_adv *a, gmc_ctx *dyn, int32_t level, int64_t time_step) {
   if (!a) return GMC_EINVAL;
+  if (a->lost) return aerr(a, GMC_ESTATE, "a failed adjoint window left the batch mid-window; create it anew");
   if (!dyn) return aerr(a, GMC_EINVAL, "null dynamics context");
   if (time_step < 0) return aerr(a, GMC_EINVAL, "time_step < 0");
   if (level < 1 || level > 3 || !dyn->derived_kept[level])
@@ -989,6 +995,7 @@ int gmc_adv_set_wind_basis(gmc_adv *a, const double *ub, const double *vb, int64
 
 int gmc_adv_drive(gmc_adv *a, int64_t time_step0, const double *scale, int32_t nsteps) {
   if (!a) return GMC_EINVAL;
+  if (a->lost) return aerr(a, GMC_ESTATE, "a failed adjoint window left the batch mid-window; create it anew");
   if (!a->have_basis) return aerr(a, GMC_ESTATE, "gmc_adv_set_wind_basis must come first");
   if (!scale || nsteps < 0 || time_step0 < 0) return aerr(a, GMC_EINVAL, "bad drive arguments");
   int rc = GMC_OK;
@@ -1767,6 +1774,7 @@ void ad_step(gmc_adv *a, const char *slot, int old, const double *in, double *ou
 }
 
 int tl_check(gmc_adv *a, const double *scale, int32_t nsteps, int64_t time_step0, const double *buf, int64_t ld) {
+  if (a->lost) return aerr(a, GMC_ESTATE, "a failed adjoint window left the batch mid-window; create it anew");
   if (!a->have_basis) return aerr(a, GMC_ESTATE, "gmc_adv_set_wind_basis must come first");
   if (a->cfg.flux_type != GMC_FLUX_PPM)
     return aerr(a, GMC_EINVAL, "the tangent linear and adjoint cover the PPM flux only");
@@ -1848,9 +1856,11 @@ int adv_adjoint(gmc_adv *a, int64_t time_step0, const double *scale, int32_t nst
     else if (!rc)
       rc = run_once(a, time_step0 + s, true, scale[s]);
   }
+  bool window_done = false;   // slot nseg holds the window's end
   if (!rc) {
     hck[nseg] = host_ck(a);
     rc = slab_to(a, tl_slot(a, nseg));
+    window_done = !rc;
   }
   if (!rc) rc = check_flags(a);
   const size_t F = tl_fields(a);
@@ -1905,7 +1915,19 @@ int adv_adjoint(gmc_adv *a, int64_t time_step0, const double *scale, int32_t nst
   }
   if (!rc) rc = tl_copy(a, false, cur, out, ld);
   if (!rc) rc = ahip(a, hipStreamSynchronize(a->stream), "sync");
-  if (rc) (void)hipStreamSynchronize(a->astream);  // nothing left running on an error
+  if (rc) {
+    // nothing left running; then put the batch back at the window's end (the
+    // recomputes moved its counters and fields), or refuse every later call
+    const std::string msg = a->err;
+    const bool synced = hipStreamSynchronize(a->astream) == hipSuccess &&
+                        hipStreamSynchronize(a->stream) == hipSuccess;
+    if (window_done) {
+      host_restore(a, hck[nseg]);
+      if (!synced || slab_from(a, tl_slot(a, nseg)) != GMC_OK || hipStreamSynchronize(a->stream) != hipSuccess)
+        a->lost = true;
+    }
+    a->err = msg;
+  }
   return rc;
 }
 

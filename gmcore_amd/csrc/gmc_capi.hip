@@ -221,13 +221,13 @@ int gmc_create(const gmc_mesh *mesh, const gmc_config *cfg, gmc_ctx **out) {
 
 int gmc_destroy(gmc_ctx *c) {
   if (!c) return GMC_EINVAL;
-  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->stream) hip_note(hipStreamSynchronize(c->stream), "context sync");
   fused_release(c);
-  (void)hipFree(c->slab);
-  (void)hipFree(c->d_mesh);
-  (void)hipFree(c->d_mrows);
-  for (void *m : c->d_filter_mem) (void)hipFree(m);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  hip_note(hipFree(c->slab), "field slab free");
+  hip_note(hipFree(c->d_mesh), "mesh free");
+  hip_note(hipFree(c->d_mrows), "mesh rows free");
+  for (void *m : c->d_filter_mem) hip_note(hipFree(m), "filter table free");
+  if (c->stream) hip_note(hipStreamDestroy(c->stream), "stream destroy");
   stage_free(c->stage);
   delete c;
   return GMC_OK;

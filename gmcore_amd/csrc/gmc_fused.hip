@@ -735,10 +735,10 @@ static int build_plan(gmc_ctx *c) {
 }
 
 void fused_release(gmc_ctx *c) {
-  for (auto &sg : c->graphs) (void)hipGraphExecDestroy(sg.exec);
+  for (auto &sg : c->graphs) hip_note(hipGraphExecDestroy(sg.exec), "step graph destroy");
   c->graphs.clear();
   if (FusedPlan *P = plan_of(c)) {
-    (void)hipFree(P->d_items);
+    hip_note(hipFree(P->d_items), "plan free");
     delete P;
     c->fused_plan = nullptr;
   }
@@ -924,7 +924,7 @@ int ehip(gmc_ens *e, hipError_t x, const char *what) {
 
 void drop_graphs(gmc_ens *e) {
   for (auto &g : e->graph)
-    if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    if (g) { hip_note(hipGraphExecDestroy(g), "ensemble graph destroy"); g = nullptr; }
 }
 
 // descriptor tables for the current shape and dts
@@ -1127,11 +1127,11 @@ int gmc_ens_create(gmc_ctx *const *members, int32_t n, gmc_ens **out) {
 
 int gmc_ens_destroy(gmc_ens *e) {
   if (!e) return GMC_EINVAL;
-  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  if (e->stream) hip_note(hipStreamSynchronize(e->stream), "ensemble sync");
   drop_graphs(e);
-  if (e->d_mem) (void)hipFree(e->d_mem);
-  if (e->d_items) (void)hipFree(e->d_items);
-  if (e->stream) (void)hipStreamDestroy(e->stream);
+  if (e->d_mem) hip_note(hipFree(e->d_mem), "ensemble tables free");
+  if (e->d_items) hip_note(hipFree(e->d_items), "ensemble items free");
+  if (e->stream) hip_note(hipStreamDestroy(e->stream), "ensemble stream destroy");
   delete e;
   return GMC_OK;
 }
@@ -1164,16 +1164,19 @@ int gmc_ens_autotune(gmc_ens *e, const double *dt, int32_t trial_steps, int32_t 
   if (!e) return GMC_EINVAL;
   if (!dt || trial_steps < 1 || !choice) return eerr(e, GMC_EINVAL, "bad autotune arguments");
   const int nm = (int)e->m.size();
-  // snapshot every member's fields and levels; the trials run on them and
-  // the snapshot is restored afterwards
-  std::vector<void *> save(nm, nullptr);
+  // snapshot every member's fields and levels into one buffer; the trials
+  // run on the members and the snapshot is restored afterwards
+  std::vector<size_t> off(nm + 1, 0);
+  for (int k = 0; k < nm; k++) off[k + 1] = off[k] + (e->m[k]->slab_bytes + 255) / 256 * 256;
+  char *save = nullptr;
   std::vector<int> levels(nm);
+  for (int k = 0; k < nm; k++) levels[k] = e->m[k]->old_level;
   int rc = GMC_OK;
-  for (int k = 0; k < nm && !rc; k++) {
-    rc = ehip(e, hipMalloc(&save[k], e->m[k]->slab_bytes), "snapshot alloc");
-    if (!rc) rc = ehip(e, hipMemcpyAsync(save[k], e->m[k]->slab, e->m[k]->slab_bytes, hipMemcpyDeviceToDevice, e->stream), "snapshot");
-    levels[k] = e->m[k]->old_level;
-  }
+  for (int k = 0; k < nm && !rc; k++) rc = ehip(e, hipStreamSynchronize(e->m[k]->stream), "member sync");
+  if (!rc) rc = ehip(e, hipMalloc((void **)&save, off[nm]), "snapshot alloc");
+  for (int k = 0; k < nm && !rc; k++)
+    rc = ehip(e, hipMemcpyAsync(save + off[k], e->m[k]->slab, e->m[k]->slab_bytes, hipMemcpyDeviceToDevice, e->stream),
+              "snapshot");
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   if (!rc) rc = ehip(e, hipEventCreate(&ev0), "event");
   if (!rc) rc = ehip(e, hipEventCreate(&ev1), "event");
@@ -1193,18 +1196,20 @@ int gmc_ens_autotune(gmc_ens *e, const double *dt, int32_t trial_steps, int32_t 
     if (!rc && times_ms && s < cap) times_ms[s] = ms;
     if (!rc && (s == 0 || ms < best_ms)) { best_ms = ms; best = s; }
   }
-  for (int k = 0; k < nm; k++) {
-    if (save[k]) {
-      if (!rc) rc = ehip(e, hipMemcpyAsync(e->m[k]->slab, save[k], e->m[k]->slab_bytes, hipMemcpyDeviceToDevice, e->stream), "restore");
+  if (save) {
+    for (int k = 0; k < nm; k++) {
+      if (!rc)
+        rc = ehip(e, hipMemcpyAsync(e->m[k]->slab, save + off[k], e->m[k]->slab_bytes, hipMemcpyDeviceToDevice,
+                                    e->stream), "restore");
       e->m[k]->old_level = levels[k];
       e->m[k]->new_level = levels[k] == 1 ? 2 : 1;
     }
+    if (!rc) rc = ehip(e, hipStreamSynchronize(e->stream), "restore sync");
+    else hip_note(hipStreamSynchronize(e->stream), "autotune sync");
+    hip_note(hipFree(save), "snapshot free");
   }
-  if (!rc) rc = ehip(e, hipStreamSynchronize(e->stream), "restore sync");
-  for (auto p : save)
-    if (p) (void)hipFree(p);
-  if (ev0) (void)hipEventDestroy(ev0);
-  if (ev1) (void)hipEventDestroy(ev1);
+  if (ev0) hip_note(hipEventDestroy(ev0), "event destroy");
+  if (ev1) hip_note(hipEventDestroy(ev1), "event destroy");
   e->shape = rc ? keep : best;
   if (!rc) {
     *choice = best;

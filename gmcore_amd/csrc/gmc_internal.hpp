@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <cstdio>
 #include <string>
 #include <vector>
 
@@ -171,6 +172,11 @@ struct gmc_ctx {
 };
 
 namespace gmc {
+// A HIP call on a release path, whose failure the caller cannot return
+// (destroy, graph and buffer release): reported on stderr.
+inline void hip_note(hipError_t e, const char *what) {
+  if (e != hipSuccess) std::fprintf(stderr, "gmcore_hip: %s: %s\n", what, hipGetErrorString(e));
+}
 int set_error(gmc_ctx *ctx, int code, const std::string &msg);
 int check_hip(gmc_ctx *ctx, hipError_t e, const char *what);
 // operator-level path (gmc_ops.hip)

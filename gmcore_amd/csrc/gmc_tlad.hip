@@ -1013,7 +1013,7 @@ int gmc_tlad_create(gmc_ctx *c, gmc_tlad **out) {
   int rc = build_row_items(a);
   if (!rc) rc = thip(a, hipMalloc((void **)&a->slab, n * sizeof(double)), "tlad alloc");
   if (!rc) rc = thip(a, hipMemset(a->slab, 0, n * sizeof(double)), "tlad memset");
-  if (!rc) rc = thip(a, hipStreamCreate(&a->astream), "stream");
+  if (!rc) rc = thip(a, hipStreamCreateWithFlags(&a->astream, hipStreamNonBlocking), "stream");
   for (hipEvent_t *e : {&a->ev_fwd, &a->ev_adj[0], &a->ev_adj[1], &a->ev_join, &a->ev_ck0})
     if (!rc) rc = thip(a, hipEventCreateWithFlags(e, hipEventDisableTiming), "event");
   if (rc) {
