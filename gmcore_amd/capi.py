@@ -79,6 +79,17 @@ GMC_FLUX_PPM, GMC_FLUX_VAN_LEER = 0, 1
 FLUX_TYPES = {"ppm": GMC_FLUX_PPM, "van_leer": GMC_FLUX_VAN_LEER}
 
 
+class GmcH0Record(C.Structure):
+    """gmc_h0_record of include/gmcore_io.h"""
+    _fields_ = [("nlon", C.c_int32), ("nlat", C.c_int32),
+                ("lon_deg", C.POINTER(C.c_double)), ("lat_deg", C.POINTER(C.c_double)),
+                ("ilon_deg", C.POINTER(C.c_double)), ("ilat_deg", C.POINTER(C.c_double)),
+                ("time_value", C.c_double), ("time_units", C.c_char_p), ("dt", C.c_double),
+                ("tm", C.c_double), ("te", C.c_double), ("tpe", C.c_double),
+                ("u", C.POINTER(C.c_double)), ("v", C.POINTER(C.c_double)), ("z", C.POINTER(C.c_double)),
+                ("pv", C.POINTER(C.c_double)), ("div", C.POINTER(C.c_double)), ("zs", C.POINTER(C.c_double))]
+
+
 class GmcAdvConfig(C.Structure):
     """gmc_adv_config of include/gmcore_adv.h"""
     _fields_ = [("radius", C.c_double), ("dt_adv", C.c_double), ("nstep", C.c_int32),
@@ -158,6 +169,15 @@ def load_hip() -> C.CDLL:
                                        C.c_int64]),
         "gmc_adv_gradient": (C.c_int, [vp, C.c_int64, P(C.c_double), C.c_int32, C.c_int32, P(C.c_double),
                                        P(C.c_double), P(C.c_double), C.c_int64]),
+        # history and restart files, include/gmcore_io.h
+        "gmc_h0_write": (C.c_int, [C.c_char_p, P(GmcH0Record), C.c_int32]),
+        "gmc_r0_write": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32] + [P(C.c_double)] * 4
+                         + [C.c_double, C.c_char_p, C.c_double, C.c_char_p] + [P(C.c_double)] * 4),
+        "gmc_r0_read": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32] + [P(C.c_double)] * 5),
+        "gmc_history_h0": (C.c_int, [vp, C.c_char_p, C.c_int32, C.c_double, C.c_char_p, C.c_double, C.c_int32]),
+        "gmc_restart_write": (C.c_int, [vp, C.c_char_p, C.c_int32, C.c_double, C.c_char_p, C.c_double,
+                                        C.c_char_p]),
+        "gmc_restart_read": (C.c_int, [vp, C.c_char_p, C.c_int32, P(C.c_double)]),
         # batched ensembles, include/gmcore_ens.h
         "gmc_ens_create": (C.c_int, [P(vp), C.c_int32, P(vp)]),
         "gmc_ens_destroy": (C.c_int, [vp]),

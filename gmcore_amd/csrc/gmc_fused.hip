@@ -30,8 +30,22 @@ namespace {
 constexpr int LC = kWave;               // 64 tile columns
 constexpr int HXL = 3;                  // column halo
 constexpr int TX = LC - 2 * HXL - 0;    // 58 output columns (q = 3 .. 60)
-constexpr int TY_MAIN = 11;             // output rows of the single-member k_tend tile
-constexpr int NF = 256;                 // threads per workgroup of the filter kernels
+// launch-shape parameters (the defaults are the measured best on MI355X;
+// the macros let a sweep build other shapes)
+#ifndef GMC_TY_MAIN
+#define GMC_TY_MAIN 11
+#endif
+#ifndef GMC_FILTER_THREADS
+#define GMC_FILTER_THREADS 256
+#endif
+#ifndef GMC_CHUNK_WORK
+#define GMC_CHUNK_WORK 32768
+#endif
+#ifndef GMC_NARROW_CHUNK
+#define GMC_NARROW_CHUNK (2 * GMC_FILTER_THREADS)
+#endif
+constexpr int TY_MAIN = GMC_TY_MAIN;    // output rows of the single-member k_tend tile
+constexpr int NF = GMC_FILTER_THREADS;  // threads per workgroup of the filter kernels
 constexpr int NXCD = 8;                 // XCDs of an MI355X
 // A k_tend tile with TY output rows: LR = TY + 5 tile rows (r = R0-2 ..
 // R0+TY+2), one wavefront per tile row.
@@ -617,8 +631,8 @@ __global__ void __launch_bounds__(NF) k_uvf_ens(const EnsMember *__restrict__ me
 // and a polar row of several hundred taps over 512 columns kept one CU busy
 // for the whole launch.  Rows of short filters go into chunks of at most
 // kNarrowChunk columns, so that a workgroup's window is one round of loads.
-static constexpr int kChunkWork = 32768;
-static constexpr int kNarrowChunk = 2 * NF;
+static constexpr int kChunkWork = GMC_CHUNK_WORK;
+static constexpr int kNarrowChunk = GMC_NARROW_CHUNK;
 
 // returns the number of chunks the row was cut into
 static int add_row(std::vector<Item> &v, int row, int kind, int n, int nlon, size_t &lds) {

@@ -185,6 +185,8 @@ int ops_space_operators(gmc_ctx *ctx, int old_l, int star_l, int new_l, int t1, 
                         int pass);
 int ops_update_state(gmc_ctx *ctx, int tend, int old_l, int new_l, double dt, int what);
 int ops_div_damp(gmc_ctx *ctx, int level);
+// calc_div of state `level` (its filtered winds) into its div field
+int ops_calc_div(gmc_ctx *ctx, int level);
 int ops_diagnose(gmc_ctx *ctx, int level, double out[4]);
 // resolve (once per device) the address of gmc_diagnose_dev's static scratch
 int ops_scratch_init(gmc_ctx *ctx);

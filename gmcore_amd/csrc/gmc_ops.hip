@@ -492,6 +492,17 @@ __global__ void k_div_damp(Grid g, Mesh M, int order, double coef, const double 
 }
 }  // namespace
 
+int ops_calc_div(gmc_ctx *c, int level) {
+  const Grid &g = c->grid;
+  const Mesh M = mesh_of(c);
+  double **S = c->state[level];
+  hipLaunchKernelGGL(k_pole_sums, dim3(2), dim3(kWave), 0, c->stream, g, M, S[S_V_F], S[S_MFY_LAT],
+                     S[S_U_LAT], c->d_pole);
+  hipLaunchKernelGGL(k_calc_div, grid2d(g, g.nlat), dim3(BX, BY), 0, c->stream, g, M, S[S_U_F], S[S_V_F],
+                     c->d_pole, S[S_DIV]);
+  return check_hip(c, hipGetLastError(), "calc_div");
+}
+
 int ops_div_damp(gmc_ctx *c, int level) {
   const Grid &g = c->grid;
   const Mesh M = mesh_of(c);

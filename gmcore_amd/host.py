@@ -198,6 +198,23 @@ class ShallowWater:
     def synchronize(self):
         self._check(self.lib.gmc_synchronize(self.ctx), "synchronize")
 
+    def history_h0(self, path: str, time_value: float, time_units: str, new_file: bool, level=None):
+        """Write (or append) one h0 record of the level (gmc_history_h0)."""
+        lv = self.level if level is None else level
+        self._check(self.lib.gmc_history_h0(self.ctx, path.encode(), lv, time_value, time_units.encode(),
+                                            self.setup.nl.dt_dyn, int(new_file)), "history_h0")
+
+    def restart_write(self, path: str, time_value: float, time_units: str, restart_interval: str, level=None):
+        lv = self.level if level is None else level
+        self._check(self.lib.gmc_restart_write(self.ctx, path.encode(), lv, time_value, time_units.encode(),
+                                               self.setup.nl.dt_dyn, restart_interval.encode()), "restart_write")
+
+    def restart_read(self, path: str, level: int = 1) -> float:
+        """u, v, gz, gzs of an r0 file into the level; gmc_prepare must follow."""
+        t = C.c_double()
+        self._check(self.lib.gmc_restart_read(self.ctx, path.encode(), level, C.byref(t)), "restart_read")
+        return t.value
+
     def keep_derived(self, on: bool = True):
         """Store each stage's forward-pass fields in the star level (gmc_keep_derived)."""
         self._check(self.lib.gmc_keep_derived(self.ctx, int(on)), "keep_derived")
