@@ -93,6 +93,24 @@ module gmc_hip_mod
     integer(c_int) function gmc_synchronize(ctx) bind(C, name='gmc_synchronize')
       import; type(c_ptr), value :: ctx
     end function
+    integer(c_int) function gmc_keep_derived(ctx, on) bind(C, name='gmc_keep_derived')
+      import; type(c_ptr), value :: ctx; integer(c_int32_t), value :: on
+    end function
+    ! h0 / r0 files (include/gmcore_io.h); paths end with c_null_char
+    integer(c_int) function gmc_history_h0(ctx, path, level, time_value, time_units, dt, new_file) &
+        bind(C, name='gmc_history_h0')
+      import; type(c_ptr), value :: ctx; character(kind=c_char), intent(in) :: path(*), time_units(*)
+      integer(c_int32_t), value :: level, new_file; real(c_double), value :: time_value, dt
+    end function
+    integer(c_int) function gmc_restart_write(ctx, path, level, time_value, time_units, dt_dyn, interval) &
+        bind(C, name='gmc_restart_write')
+      import; type(c_ptr), value :: ctx; character(kind=c_char), intent(in) :: path(*), time_units(*), interval(*)
+      integer(c_int32_t), value :: level; real(c_double), value :: time_value, dt_dyn
+    end function
+    integer(c_int) function gmc_restart_read(ctx, path, level, time_value) bind(C, name='gmc_restart_read')
+      import; type(c_ptr), value :: ctx; character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: level; real(c_double), intent(out) :: time_value
+    end function
     ! FFSL tracer advection
     integer(c_int) function gmc_adv_create(mesh, cfg, adv) bind(C, name='gmc_adv_create')
       import; type(gmc_mesh), intent(in) :: mesh; type(gmc_adv_config), intent(in) :: cfg
@@ -119,6 +137,13 @@ module gmc_hip_mod
     end function
     integer(c_int) function gmc_adv_run(adv, time_step) bind(C, name='gmc_adv_run')
       import; type(c_ptr), value :: adv; integer(c_int64_t), value :: time_step
+    end function
+    integer(c_int) function gmc_adv_run_dyn(adv, ctx, level, time_step) bind(C, name='gmc_adv_run_dyn')
+      import; type(c_ptr), value :: adv, ctx; integer(c_int32_t), value :: level
+      integer(c_int64_t), value :: time_step
+    end function
+    integer(c_int) function gmc_adv_init_dyn(adv, ctx, level) bind(C, name='gmc_adv_init_dyn')
+      import; type(c_ptr), value :: adv, ctx; integer(c_int32_t), value :: level
     end function
     integer(c_int) function gmc_adv_set_wind_basis(adv, u, v, ld) bind(C, name='gmc_adv_set_wind_basis')
       import; type(c_ptr), value :: adv; real(c_double), intent(in) :: u(*), v(*)
