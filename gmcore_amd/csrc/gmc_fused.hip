@@ -240,7 +240,22 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double gr
   const MeshTile M{sme, R0 - 2};
   __syncthreads();
 
-  // B: mass fluxes (rows 1 .. TY+2), pv (rows 0 .. TY+3), ke (rows 2 .. TY+2)
+  // B: mass fluxes (rows 1 .. TY+2), pv (rows 0 .. TY+3), ke (rows 2 .. TY+2).
+  // Tile rows that need all three away from the poles form them in one block,
+  // so that the compiler interleaves the three dependency chains; the lanes
+  // that block adds (mass fluxes on lane 1, ke on lanes 1, 2 and 62) write
+  // tile elements that no later phase reads.
+  if (!TAPE && k >= 2 && k < S::TY + 3 && r >= 1 && r <= nlat - 3) {
+    if (q >= 1 && q < 63) {
+      const double m0 = sm[k][q], m1 = sm[k][q + 1], m2 = sm[k + 1][q], m3 = sm[k + 1][q + 1];
+      const double u0 = suf[k][q], u1 = suf[k + 1][q], uw = suf[k][q - 1];
+      const double v0 = svf[k][q], v1 = svf[k][q + 1], vs = svf[k - 1][q];
+      smfx[k][q] = (m0 + m1) * 0.5 * u0;
+      smfy[k][q] = (m0 + m2) * 0.5 * v0;
+      spv[k][q] = pv_of(vor_of(u0, u1, v1, v0, M, r), M, r, m_vtx_of(m0, m1, m2, m3, M, r));
+      ske[k][q] = ke_of(uw, u0, vs, v0, M, r);
+    }
+  } else {
   if (k >= 1 && k < S::TY + 3 && q >= 2 && q < 63) {
     smfx[k][q] = (r >= 1 && r <= nlat - 2) ? (sm[k][q] + sm[k][q + 1]) * 0.5 * suf[k][q] : 0.0;
     smfy[k][q] = (r >= 0 && r < nhalf) ? (sm[k][q] + sm[k + 1][q]) * 0.5 * svf[k][q] : 0.0;
@@ -274,6 +289,7 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double gr
     else if (r > 0 && r < nlat - 1) v = ke_of(suf[k][q - 1], suf[k][q], svf[k - 1][q], svf[k][q], M, r);
     ske[k][q] = v;
   }
+  }
   __syncthreads();
 
   // D: tangential winds and pv on the edges.  v_lon (U rows 2 .. TY+2) and
@@ -281,6 +297,24 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double gr
   // so they stay in registers; pv_lon replaces u_f and pv_lat replaces v_f
   // element by element (each lane reads only its own u_f / v_f here).
   double pl = 0.0, pt = 0.0;
+  if (!TAPE && k >= 2 && k < S::TY + 2 && r >= 1 && r <= nlat - 2) {
+    // both edges in one block (interleaved chains); lane 2 forms a pv_lat
+    // that is replaced by the 0 the edge loop below leaves there
+    if (q >= 2 && q < 62) {
+      const double tl = tangent_sum(M(GMC_FULL_TANGENT_WGT1, r), M(GMC_FULL_TANGENT_WGT2, r), smfy[k - 1][q],
+                                    smfy[k - 1][q + 1], smfy[k][q], smfy[k][q + 1]);
+      const double tu = tangent_sum(M(GMC_HALF_TANGENT_WGT1, r), M(GMC_HALF_TANGENT_WGT2, r), smfx[k][q - 1],
+                                    smfx[k][q], smfx[k + 1][q - 1], smfx[k + 1][q]);
+      const double vl = tl / ((sm[k][q] + sm[k][q + 1]) * 0.5);
+      const double ul = tu / ((sm[k][q] + sm[k + 1][q]) * 0.5);
+      const double el = pv_edge(pv_scheme, order, wgt, vl, suf[k][q], 0, dt, M(GMC_LE_LON, r), spv[k - 2][q],
+                                spv[k - 1][q], spv[k][q], spv[k + 1][q]);
+      const double et = pv_edge(pv_scheme, order, wgt, ul, svf[k][q], 1, dt, M(GMC_LE_LAT, r), spv[k][q - 2],
+                                spv[k][q - 1], spv[k][q], spv[k][q + 1]);
+      pl = el;
+      pt = q >= 3 ? et : 0.0;
+    }
+  } else {
   if (k >= 2 && k < S::TY + 3 && q >= 2 && q < 62 && r >= 1 && r <= nlat - 2) {
     const double tt = tangent_sum(M(GMC_FULL_TANGENT_WGT1, r), M(GMC_FULL_TANGENT_WGT2, r), smfy[k - 1][q],
                                   smfy[k - 1][q + 1], smfy[k][q], smfy[k][q + 1]);
@@ -311,22 +345,39 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double gr
                                      upwind_deriv(pv_scheme, order, wgt, ul, svf[k][q], 1, a1, a2, a3, a4));
     }
   }
+  }
   spl[k][q] = pl;
   spt[k][q] = pt;
   __syncthreads();
 
-  // E: du, dv (forward pass) and gz_new = gz_old + dt dgz on the output rows
-  if (out_pt) {
-    const int64_t o = r * p + ic;
-    if (r >= 1 && r <= nlat - 2) {
+  // E: du, dv (forward pass) and gz_new = gz_old + dt dgz on the output rows;
+  // rows away from the poles in one block
+  if (r >= 1 && r <= nlat - 2) {
+    if (out_pt) {
+      const int64_t o = r * p + ic;
       const double qhv = coriolis == 1
           ? qhv_of(M, r, spl[k][q], smfy[k - 1][q], spt[k - 1][q], smfy[k - 1][q + 1], spt[k - 1][q + 1],
                    smfy[k][q], spt[k][q], smfy[k][q + 1], spt[k][q + 1])
           : tangent_sum(M(GMC_FULL_TANGENT_WGT1, r), M(GMC_FULL_TANGENT_WGT2, r), smfy[k - 1][q],
                         smfy[k - 1][q + 1], smfy[k][q], smfy[k][q + 1]) * spl[k][q];
-      const double dk = M.div(ske[k][q + 1] - ske[k][q], GMC_DE_LON, r);
-      P.du[o] = qhv - dk;
+      const double dku = M.div(ske[k][q + 1] - ske[k][q], GMC_DE_LON, r);
+      const double qhu = coriolis == 1
+          ? qhu_of(M, r, spt[k][q], smfx[k][q - 1], spl[k][q - 1], smfx[k][q], spl[k][q],
+                   smfx[k + 1][q - 1], spl[k + 1][q - 1], smfx[k + 1][q], spl[k + 1][q])
+          : tangent_sum(M(GMC_HALF_TANGENT_WGT1, r), M(GMC_HALF_TANGENT_WGT2, r), smfx[k][q - 1], smfx[k][q],
+                        smfx[k + 1][q - 1], smfx[k + 1][q]) * spt[k][q];
+      const double dkv = M.div(ske[k + 1][q] - ske[k][q], GMC_DE_LAT, r);
+      const double dgz = -(dmfdlon_of(smfx[k][q], smfx[k][q - 1], M, r) +
+                           dmfdlat_of(smfy[k][q], smfy[k - 1][q], M, r)) * grav;
+      P.du[o] = qhv - dku;
+      P.dv[o] = -qhu - dkv;
+      P.gz_new[o] = gz_old + dt * dgz;
     }
+    return;
+  }
+  // the pole rows: v on the southern pole edge row, gz from the pole sums
+  if (out_pt) {
+    const int64_t o = r * p + ic;
     if (r < nhalf) {
       const double qhu = coriolis == 1
           ? qhu_of(M, r, spt[k][q], smfx[k][q - 1], spl[k][q - 1], smfx[k][q], spl[k][q],
@@ -336,13 +387,7 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double gr
       const double dk = M.div(ske[k + 1][q] - ske[k][q], GMC_DE_LAT, r);
       P.dv[o] = -qhu - dk;
     }
-    double dmfdlon = 0.0, dmfdlat;
-    if (r == 0) dmfdlat = pole[2];
-    else if (r == nlat - 1) dmfdlat = pole[3];
-    else {
-      dmfdlon = dmfdlon_of(smfx[k][q], smfx[k][q - 1], M, r);
-      dmfdlat = dmfdlat_of(smfy[k][q], smfy[k - 1][q], M, r);
-    }
+    const double dmfdlon = 0.0, dmfdlat = r == 0 ? pole[2] : pole[3];
     const double dgz = -(dmfdlon + dmfdlat) * grav;
     P.gz_new[o] = gz_old + dt * dgz;
   }
@@ -382,6 +427,9 @@ __global__ void __launch_bounds__(Tile<TY_MAIN>::NT, 8) k_tend(Grid g, Mesh Mg, 
 struct Item {
   int32_t row, c0, c1, kind;  // kind: 0 gz (cell row), 1 u (U row), 2 v (V row)
   int32_t member;             // ensemble member (0 for a single context)
+  int32_t n, n2;              // taps of the row's filters (copies of the filter tables): the big filter
+                              // of the item's row kind, and small_filter1 for gz items
+  int32_t pad;
 };
 
 // column i - h + t of a row, t in [0, nlon + 2h), h <= nlon
@@ -395,26 +443,26 @@ __device__ __forceinline__ void gzf_item(const Grid &g, const FT &big, const FT 
                                          WP gz_f, WP gz, double *xs) {
   const int r = it.row;
   const int64_t o = r * g.pitch;
-  const int nb = big.ngrid_lon[r], ns = small.ngrid_lon[r];
-  const int hb = nb > 1 ? (nb - 1) / 2 : 0, hs = ns > 1 ? (ns - 1) / 2 : 0;
+  const int nb = it.n, ns = it.n2;
+  const int hb = nb > 1 ? (nb - 1) / 2 : 0, hs = ns > 1 ? (ns - 1) / 2 : 0, hw = hb > hs ? hb : hs;
   double *wb = xs, *ws = xs + big.max_taps, *x0 = xs + big.max_taps + small.max_taps;
-  const int len = it.c1 - it.c0 + 2 * hb;
+  const int len = it.c1 - it.c0 + 2 * hw;
   for (int k = threadIdx.x; k < nb; k += NF) wb[k] = big.wgt_lon[(int64_t)r * big.max_taps + k];
   for (int k = threadIdx.x; k < ns; k += NF) ws[k] = small.wgt_lon[(int64_t)r * small.max_taps + k];
   for (int t = threadIdx.x; t < len; t += 4 * NF) {
     double v[4];
 #pragma unroll
     for (int u = 0; u < 4; u++)
-      if (t + u * NF < len) v[u] = gz_new[o + wrap2(it.c0 - hb + t + u * NF, g.nlon)];
+      if (t + u * NF < len) v[u] = gz_new[o + wrap2(it.c0 - hw + t + u * NF, g.nlon)];
 #pragma unroll
     for (int u = 0; u < 4; u++)
       if (t + u * NF < len) x0[t + u * NF] = v[u];
   }
   __syncthreads();
   for (int c = it.c0 + threadIdx.x; c < it.c1; c += NF) {
-    const double *x = x0 + (c - it.c0);      // x[hb] is column c
-    gz_f[o + c] = nb > 1 ? filter_point(wb, nb, x) : x[hb];
-    gz[o + c] = ns > 1 ? filter_point(ws, ns, x + hb - hs) : x[hb];
+    const double *x = x0 + (c - it.c0);      // x[hw] is column c
+    gz_f[o + c] = nb > 1 ? filter_point(wb, nb, x + hw - hb) : x[hw];
+    gz[o + c] = ns > 1 ? filter_point(ws, ns, x + hw - hs) : x[hw];
   }
 }
 
@@ -443,7 +491,7 @@ __device__ __forceinline__ void uvf_item(const Grid &g, const MT &M, double dt, 
     for (int c = it.c0 + threadIdx.x; c < it.c1; c += NF) P.u_f[o + c] = P.u[o + c];
     return;
   }
-  const int n = it.kind == 1 ? big.ngrid_lon[r] : big.ngrid_lat[r];
+  const int n = it.n;
   const int h = n > 1 ? (n - 1) / 2 : 0;
   const int len = it.c1 - it.c0 + 2 * h;
   const auto wg = (it.kind == 1 ? big.wgt_lon : big.wgt_lat) + (int64_t)r * big.max_taps;
@@ -634,14 +682,18 @@ __global__ void __launch_bounds__(NF) k_uvf_ens(const EnsMember *__restrict__ me
 static constexpr int kChunkWork = GMC_CHUNK_WORK;
 static constexpr int kNarrowChunk = GMC_NARROW_CHUNK;
 
-// returns the number of chunks the row was cut into
-static int add_row(std::vector<Item> &v, int row, int kind, int n, int nlon, size_t &lds) {
-  int w = n <= 16 ? std::min(nlon, kNarrowChunk) : std::min(nlon, std::max(64, (kChunkWork / n + 63) / 64 * 64));
+// One row of a filter work list: `work` taps per output sets the chunk
+// width; n (and n2) are the taps of the row's filters, the window of a chunk
+// is its columns and (n - 1) / 2 on either side (lds: the largest window).
+// Returns the number of chunks the row was cut into.
+static int add_row(std::vector<Item> &v, int row, int kind, int work, int nlon, size_t &lds, int n, int n2 = 0) {
+  int w = work <= 16 ? std::min(nlon, kNarrowChunk)
+                     : std::min(nlon, std::max(64, (kChunkWork / work + 63) / 64 * 64));
   const int parts = (nlon + w - 1) / w;
   const int h = n > 1 ? (n - 1) / 2 : 0;
   for (int k = 0; k < parts; k++) {
     const int c0 = (int)((int64_t)nlon * k / parts), c1 = (int)((int64_t)nlon * (k + 1) / parts);
-    v.push_back(Item{row, c0, c1, kind, 0});
+    v.push_back(Item{row, c0, c1, kind, 0, n, n2, 0});
     lds = std::max(lds, sizeof(double) * (size_t)(c1 - c0 + 2 * h));
   }
   return parts;
@@ -694,12 +746,16 @@ static std::vector<Item> xcd_deal(const std::vector<Item> &v, W work) {
 
 static int build_plan(gmc_ctx *c) {
   const Grid &g = c->grid;
-  std::vector<int32_t> nb_lon(g.nlat), nb_lat(g.nhalf);
+  std::vector<int32_t> nb_lon(g.nlat), nb_lat(g.nhalf), ns_lon(g.nlat);
   int rc = check_hip(c, hipMemcpy(nb_lon.data(), c->big.ngrid_lon, sizeof(int32_t) * g.nlat, hipMemcpyDeviceToHost),
                      "plan");
   if (!rc) rc = check_hip(c, hipMemcpy(nb_lat.data(), c->big.ngrid_lat, sizeof(int32_t) * g.nhalf,
                                        hipMemcpyDeviceToHost), "plan");
+  if (!rc) rc = check_hip(c, hipMemcpy(ns_lon.data(), c->small1.ngrid_lon, sizeof(int32_t) * g.nlat,
+                                       hipMemcpyDeviceToHost), "plan");
   if (rc) return rc;
+  for (int r = 0; r < g.nlat; r++)
+    if (ns_lon[r] > g.nlon) return set_error(c, GMC_EINVAL, "filter wider than a latitude circle");
   for (int r = 0; r < g.nlat; r++)
     if (nb_lon[r] > g.nlon) return set_error(c, GMC_EINVAL, "filter wider than a latitude circle");
   for (int r = 0; r < g.nhalf; r++)
@@ -715,12 +771,15 @@ static int build_plan(gmc_ctx *c) {
       P->max_parts = std::max(P->max_parts, parts);
       P->widest = std::max(P->widest, n);
     };
+    // gz items: the window is that of the wider of the two filters
     for (int r = 0; r < g.nlat; r++)
-      count(add_row(gz, r, 0, nb_lon[r] + std::max(1, nb_lon[r] / 2), g.nlon, P->lds_gz), nb_lon[r]);
+      count(add_row(gz, r, 0, nb_lon[r] + std::max(1, nb_lon[r] / 2), g.nlon, P->lds_gz,
+                    std::max(nb_lon[r], ns_lon[r])), nb_lon[r]);
+    for (auto &it : gz) { it.n = nb_lon[it.row]; it.n2 = ns_lon[it.row]; }
     // u row r next to v rows r - 1 and r: the three read gz_f row r
     for (int r = 0; r < g.nlat; r++) {
-      count(add_row(uv, r, 1, nb_lon[r], g.nlon, P->lds_uv), nb_lon[r]);
-      if (r < g.nhalf) count(add_row(uv, r, 2, nb_lat[r], g.nlon, P->lds_uv), nb_lat[r]);
+      count(add_row(uv, r, 1, nb_lon[r], g.nlon, P->lds_uv, nb_lon[r]), nb_lon[r]);
+      if (r < g.nhalf) count(add_row(uv, r, 2, nb_lat[r], g.nlon, P->lds_uv, nb_lat[r]), nb_lat[r]);
     }
     auto work = [&](const Item &it) {
       const int n = it.kind == 2 ? nb_lat[it.row] : nb_lon[it.row];
@@ -731,12 +790,6 @@ static int build_plan(gmc_ctx *c) {
     P->n_gz = (int)items.size();
     items.insert(items.end(), uv_d.begin(), uv_d.end());
     P->n_uv = (int)items.size() - P->n_gz;
-  }
-  // the gz items filter with the big half-width; size their LDS for it
-  P->lds_gz = 0;
-  for (int k = 0; k < P->n_gz; k++) {
-    const int n = nb_lon[items[k].row], h = n > 1 ? (n - 1) / 2 : 0;
-    P->lds_gz = std::max(P->lds_gz, sizeof(double) * (size_t)(items[k].c1 - items[k].c0 + 2 * h));
   }
   P->lds_gz += sizeof(double) * (c->big.max_taps + c->small1.max_taps);
   P->lds_uv += sizeof(double) * c->big.max_taps;
@@ -1095,18 +1148,19 @@ int gmc_ens_create(gmc_ctx *const *members, int32_t n, gmc_ens **out) {
   for (int k = 0; k < n; k++) {
     gmc_ctx *c = e->m[k];
     const Grid &g = c->grid;
-    std::vector<int32_t> nb_lon(g.nlat), nb_lat(g.nhalf);
+    std::vector<int32_t> nb_lon(g.nlat), nb_lat(g.nhalf), ns_lon(g.nlat);
     if (hipMemcpy(nb_lon.data(), c->big.ngrid_lon, sizeof(int32_t) * g.nlat, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(nb_lat.data(), c->big.ngrid_lat, sizeof(int32_t) * g.nhalf, hipMemcpyDeviceToHost) != hipSuccess)
+        hipMemcpy(nb_lat.data(), c->big.ngrid_lat, sizeof(int32_t) * g.nhalf, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(ns_lon.data(), c->small1.ngrid_lon, sizeof(int32_t) * g.nlat, hipMemcpyDeviceToHost) != hipSuccess)
       return fail(GMC_EDEVICE, "filter tables");
     std::vector<Item> a, b;
-    size_t dummy = 0;
-    for (int r = 0; r < g.nlat; r++) add_row(a, r, 0, nb_lon[r] + std::max(1, nb_lon[r] / 2), g.nlon, dummy);
-    for (int r = 0; r < g.nlat; r++) add_row(b, r, 1, nb_lon[r], g.nlon, luv);
-    for (int r = 0; r < g.nhalf; r++) add_row(b, r, 2, nb_lat[r], g.nlon, luv);
+    for (int r = 0; r < g.nlat; r++)
+      add_row(a, r, 0, nb_lon[r] + std::max(1, nb_lon[r] / 2), g.nlon, lgz, std::max(nb_lon[r], ns_lon[r]));
+    for (int r = 0; r < g.nlat; r++) add_row(b, r, 1, nb_lon[r], g.nlon, luv, nb_lon[r]);
+    for (int r = 0; r < g.nhalf; r++) add_row(b, r, 2, nb_lat[r], g.nlon, luv, nb_lat[r]);
     for (auto &it : a) {
-      const int nbk = nb_lon[it.row], h = nbk > 1 ? (nbk - 1) / 2 : 0;
-      lgz = std::max(lgz, sizeof(double) * (size_t)(it.c1 - it.c0 + 2 * h));
+      it.n = nb_lon[it.row];
+      it.n2 = ns_lon[it.row];
       it.member = k;
       gz.push_back(it);
     }
