@@ -37,12 +37,29 @@ BYTES_PER_GP_STEP = 280.0          # 35 FP64 field passes per pc2 step
 # over the three stages (k_tend reads 4 fields in stage 1 and 5 in stages
 # 2-3 and writes 3; k_gzf reads 1, writes 2; k_uvf reads 5, writes 4)
 KERNEL_BYTES_PER_GP = {"k_tend": 8.0 * (7 + 8 + 8) / 3, "k_gzf": 24.0, "k_uvf": 72.0}
-# HBM bytes per grid point of one launch measured with rocprofv3 --pmc
-# FETCH_SIZE (x 2.00) and WRITE_SIZE (x 1.00), the factors a 1 GiB device copy
-# reads; 1440 x 721 workload, profiles/sw025_step_kernels.md
-KERNEL_TRAFFIC_PER_GP = {"k_tend": (2 * 19976.8 + 24310.8) * 1024 / (1440 * 721),
-                         "k_gzf": (2 * 4395.4 + 16222.5) * 1024 / (1440 * 721),
-                         "k_uvf": (2 * 22127.6 + 32400.4) * 1024 / (1440 * 721)}
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def profiled_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload` as measured by rocprofv3
+    --pmc FETCH_SIZE / WRITE_SIZE, read from the `traffic-data:` line of the
+    profile summaries under profiles/ (per-launch means in KiB and the factors
+    a device copy of known size calibrates them with).  None when no summary
+    holds the pair."""
+    try:
+        names = sorted(f for f in os.listdir(PROFILES) if f.endswith(".md"))
+    except OSError:
+        return None
+    for name in names:
+        with open(os.path.join(PROFILES, name)) as f:
+            for line in f:
+                if not line.startswith("traffic-data:"):
+                    continue
+                kv = dict(t.split("=", 1) for t in line.split()[1:] if "=" in t)
+                if kv.get("workload") == workload and kv.get("kernel") == kernel:
+                    return 1024.0 * (float(kv["read_factor"]) * float(kv["fetch_kib"])
+                                     + float(kv["write_factor"]) * float(kv["write_kib"]))
+    return None
 
 # FFSL tracer advection, 0.5 degrees, 8 tracers, dt = 900 s, one day
 ADV_CONFIGS = {
@@ -241,8 +258,10 @@ def run_adv(args):
                                f"PPM mono, lon_halo_width {cfg['hw']}",
                    "nlon": nlon, "nlat": nlat, "ntracers": ntr, "dt": dt},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "scope": f"whole advection step: {ADV_BYTES_PER_TRACER_GP(ntr):.1f} B per tracer-grid-point"},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args.config, "step"),
+                     "scope": f"whole advection step: {ADV_BYTES_PER_TRACER_GP(ntr):.1f} B per tracer-grid-point "
+                              "algorithmic; traffic = HBM bytes of one step's launches from the profiler's "
+                              "byte counters"},
         "kernels": kernels,
         "mass_rel_drift_bell_tracers": drift,
         "min_tracer": qmin,
@@ -514,8 +533,8 @@ def run_adv_adj(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="sw025",
                     choices=sorted(CONFIGS) + sorted(ADV_CONFIGS) + sorted(ENS_CONFIGS) + sorted(ADJ_CONFIGS)
                     + sorted(ADV_ADJ_CONFIGS))
@@ -603,7 +622,7 @@ def main():
                         "bytes_per_launch": b, "gbs": b / (kt[k] * 1e-3) / 1e9 if kt[k] > 0 else 0.0})
     dom = max(kernels, key=lambda x: x["avg_us"])
     step_gbs = BYTES_PER_GP_STEP * gp / (ms_step * 1e-3) / 1e9
-    traffic = KERNEL_TRAFFIC_PER_GP.get(dom["name"]) if args.config == "sw025" else None
+    traffic = profiled_traffic(args.config, dom["name"])
     result = {
         "metric": "grid-point-steps/s (FP64 SW pc2 step, 1 GPU)",
         "value": gp / (ms_step * 1e-3),
@@ -623,7 +642,7 @@ def main():
                    "nlon": cfg["nlon"], "nlat": cfg["nlat"], "dt": cfg["dt"], "time_scheme": "pc2"},
         "roofline": {"bound": "hbm", "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": dom["gbs"] / HBM_PEAK_GBS,
-                     "traffic": traffic * gp if traffic else None,
+                     "traffic": traffic,
                      "kernel": dom["name"],
                      "scope": f"dominant kernel {dom['name']}: algorithmic bytes per launch "
                               f"{KERNEL_BYTES_PER_GP[dom['name']]:.2f} B/grid point over its mean "

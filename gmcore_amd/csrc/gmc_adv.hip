@@ -51,6 +51,17 @@ struct AdvGrid {
   double dlon, radius, dt;
 };
 
+// Element i of a row whose base pointer is uniform over the wavefront, with
+// a 32-bit byte offset: the load or store takes the scalar base and a vector
+// offset (no 64-bit address arithmetic per lane).  i * 8 < 2^32 for every
+// row of a field.
+__device__ __forceinline__ double ldu(const double *base, unsigned i) {
+  return *(const double *)((const char *)base + i * (unsigned)sizeof(double));
+}
+__device__ __forceinline__ void stu(double *base, unsigned i, double v) {
+  *(double *)((char *)base + i * (unsigned)sizeof(double)) = v;
+}
+
 __device__ __forceinline__ int wrap_any(int i, int n) {
   const int r = i % n;
   return r < 0 ? r + n : r;
@@ -110,11 +121,11 @@ __device__ __forceinline__ double run_sum(const double *mx, int k0, int cnt, int
   for (; j + 8 <= cnt; j += 8) {
     double v[8];
 #pragma unroll
-    for (int u = 0; u < 8; u++) v[u] = mx[wrap(k0 + j + u, n)];
+    for (int u = 0; u < 8; u++) v[u] = ldu(mx, wrap(k0 + j + u, n));
 #pragma unroll
     for (int u = 0; u < 8; u++) sum = sum + v[u];
   }
-  for (; j < cnt; j++) sum = sum + mx[wrap(k0 + j, n)];
+  for (; j < cnt; j++) sum = sum + ldu(mx, wrap(k0 + j, n));
   return sum;
 }
 
@@ -126,14 +137,14 @@ __device__ __forceinline__ double run_sum(const double *mx, int k0, int cnt, int
 // selected per lane.
 __device__ __forceinline__ double yflux(const AdvGrid &g, int lim, double c, double v, int h, int i,
                                         const double *my) {
-  const int ih = wrap(i + g.half, g.nlon), P = (int)g.pitch;  // a field has fewer than 2^31 elements
+  const unsigned ih = wrap(i + g.half, g.nlon), ic = i;
   double f[6];
 #pragma unroll
   for (int k = 0; k < 6; k++) {
     const int r = h - 2 + k;
     const bool mir = r < 0 || r > g.nlat - 1;
     const int rr = r < 0 ? -r : (r > g.nlat - 1 ? 2 * (g.nlat - 1) - r : r);
-    f[k] = my[rr * P + (mir ? ih : i)];
+    f[k] = ldu(my + rr * g.pitch, mir ? ih : ic);
   }
   // both wind directions on one path (no divergence within a wavefront):
   // c > 0: v * part(1 - c, 1) / c from row h; c < 0: -v * part(0, -c) / c
@@ -277,8 +288,8 @@ __device__ __forceinline__ double xflux_g(const AdvGrid &g, int lim, double c, d
   const bool up = c > 0;
   const int iu = wrap(up ? i - ci : i - ci + 1, n);
   const double sum = run_sum(row, up ? i + 1 - ci : i + 1, up ? ci : -ci, n);
-  const Ppm p = ppm(lim, row[wrap(iu - 2, n)], row[wrap(iu - 1, n)], row[iu], row[wrap(iu + 1, n)],
-                    row[wrap(iu + 2, n)]);
+  const Ppm p = ppm(lim, ldu(row, wrap(iu - 2, n)), ldu(row, wrap(iu - 1, n)), ldu(row, iu),
+                    ldu(row, wrap(iu + 1, n)), ldu(row, wrap(iu + 2, n)));
   const double fx = (up ? u : -u) * ppm_part_x(up ? 1 - cf : 0, up ? 1 : -cf, p.fl, p.df, p.f6, sum) / c;
   return c == 0 ? 0 : fx;
 }
@@ -297,21 +308,22 @@ __device__ __forceinline__ double xflux_vl(const AdvGrid &g, int lim, double c, 
   const bool up = c > 0;
   const int iu = wrap(up ? i - ci : i - ci + 1, n);
   const double sum = run_sum(row, up ? i + 1 - ci : i + 1, up ? ci : -ci, n);
-  const double dm = slope(lim, row[wrap(iu - 1, n)], row[iu], row[wrap(iu + 1, n)]);
-  const double a = up ? row[iu] + dm * 0.5 * (1 - cf) : row[iu] - dm * 0.5 * (1 + cf);
+  const double fu = ldu(row, iu);
+  const double dm = slope(lim, ldu(row, wrap(iu - 1, n)), fu, ldu(row, wrap(iu + 1, n)));
+  const double a = up ? fu + dm * 0.5 * (1 - cf) : fu - dm * 0.5 * (1 + cf);
   const double fx = u * (up ? cf * a + sum : cf * a - sum) / c;
   return c == 0 ? 0 : fx;
 }
 __device__ __forceinline__ double yflux_vl(const AdvGrid &g, int lim, double c, double v, int h, int i,
                                            const double *my) {
-  const int ih = wrap(i + g.half, g.nlon), P = (int)g.pitch;
+  const unsigned ih = wrap(i + g.half, g.nlon), ic = i;
   double f[4];   // full rows h-1 .. h+2, beyond a pole mirrored
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const int r = h - 1 + k;
     const bool mir = r < 0 || r > g.nlat - 1;
     const int rr = r < 0 ? -r : (r > g.nlat - 1 ? 2 * (g.nlat - 1) - r : r);
-    f[k] = my[rr * P + (mir ? ih : i)];
+    f[k] = ldu(my + rr * g.pitch, mir ? ih : ic);
   }
   const bool up = c > 0;   // upwind row h, else h + 1
   const double fu = up ? f[1] : f[2];
@@ -390,8 +402,11 @@ __device__ __forceinline__ double wave_row_sum(int n, F f) {
 
 // ffsl_calc_tracer_hflx_cell, inner part (ffsl_mod.F90:350-407): the
 // advective half-steps qx, qy of every cell
+// (NF, 8): eight wavefronts per SIMD (four workgroups per CU); without the
+// bound the kernel argument and row registers take more scalar registers
+// than that occupancy allows
 template <bool VL>
-__global__ void __launch_bounds__(NF) k_adv_inner(AdvGrid g, Mesh M, int lim, int ntr, InnerPtrs p) {
+__global__ void __launch_bounds__(NF, 8) k_adv_inner(AdvGrid g, Mesh M, int lim, int ntr, InnerPtrs p) {
   __shared__ double sfy[NF / kWave][kWave];
   __shared__ double bcast;
   const FluxItem it = flux_item(g, ntr);
@@ -417,12 +432,16 @@ __global__ void __launch_bounds__(NF) k_adv_inner(AdvGrid g, Mesh M, int lim, in
     return;
   }
   if (it.r < 0) return;   // padding item (uniform over the block)
-  const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), l = threadIdx.x % kWave;
+  // rows h and r are uniform over the wavefront: row pointers are scalar and
+  // the lanes index them with unsigned column offsets
   const int iw = wrap(it.c0 - 1 + l, n);   // column of the lane
+  const unsigned uiw = iw;
   const int h = it.r - 1 + w, r = it.r + w;
-  if (h <= g.nhalf - 1) sfy[w][l] = yflux_any<VL>(g, lim, p.cfly[h * P + iw], p.v[h * P + iw], h, iw, q);
+  if (h <= g.nhalf - 1)
+    sfy[w][l] = yflux_any<VL>(g, lim, ldu(p.cfly + h * P, uiw), ldu(p.v + h * P, uiw), h, iw, q);
   const bool row = w < kRowsPerItem && r <= g.nlat - 2;   // uniform over the wavefront
-  const double fx = row ? xflux<VL>(g, lim, p.cflx[r * P + iw], p.u[r * P + iw], iw, q + r * P) : 0.0;
+  const double fx = row ? xflux<VL>(g, lim, ldu(p.cflx + r * P, uiw), ldu(p.u + r * P, uiw), iw, q + r * P) : 0.0;
   const double fxw = __shfl_up(fx, 1, kWave);
   __syncthreads();
   const int i = it.c0 - 1 + l;   // cell of the lane
@@ -430,10 +449,11 @@ __global__ void __launch_bounds__(NF) k_adv_inner(AdvGrid g, Mesh M, int lim, in
     const double lel = M(GMC_LE_LON, r), ac = M(GMC_AREA_CELL, r);
     const double ls = M(GMC_LE_LAT, r - 1), ln = M(GMC_LE_LAT, r);
     const double fs = sfy[w][l], fn = sfy[w + 1][l];
-    const int64_t k = r * P + i;
-    const double qc = q[k];
-    p.qx[toff + k] = qc - 0.5 * ((fx - fxw) * lel / ac - p.divx[k] * qc) * dt;
-    p.qy[toff + k] = qc - 0.5 * ((fn * ln - fs * ls) / ac - p.divy[k] * qc) * dt;
+    const unsigned ui = i;
+    const int64_t ro = r * P;
+    const double qc = ldu(q + ro, ui);
+    stu(p.qx + toff + ro, ui, qc - 0.5 * ((fx - fxw) * lel / ac - ldu(p.divx + ro, ui) * qc) * dt);
+    stu(p.qy + toff + ro, ui, qc - 0.5 * ((fn * ln - fs * ls) / ac - ldu(p.divy + ro, ui) * qc) * dt);
   }
 }
 
@@ -448,7 +468,7 @@ struct OuterPtrs {
 // outer fluxes (ffsl_mod.F90:410) and adv_run's flux-form update and /m
 // (adv_mod.F90:183-235) of every cell, into qp
 template <bool VL>
-__global__ void __launch_bounds__(NF) k_adv_flux(AdvGrid g, Mesh M, int lim, int ntr, OuterPtrs p) {
+__global__ void __launch_bounds__(NF, 8) k_adv_flux(AdvGrid g, Mesh M, int lim, int ntr, OuterPtrs p) {
   __shared__ double sfy[NF / kWave][kWave];
   __shared__ double bcast;
   const FluxItem it = flux_item(g, ntr);
@@ -473,12 +493,14 @@ __global__ void __launch_bounds__(NF) k_adv_flux(AdvGrid g, Mesh M, int lim, int
     return;
   }
   if (it.r < 0) return;
-  const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), l = threadIdx.x % kWave;
   const int iw = wrap(it.c0 - 1 + l, n);
+  const unsigned uiw = iw;
   const int h = it.r - 1 + w, r = it.r + w;
-  if (h <= g.nhalf - 1) sfy[w][l] = yflux_any<VL>(g, lim, p.cfly[h * P + iw], p.mfy[h * P + iw], h, iw, qx);
+  if (h <= g.nhalf - 1)
+    sfy[w][l] = yflux_any<VL>(g, lim, ldu(p.cfly + h * P, uiw), ldu(p.mfy + h * P, uiw), h, iw, qx);
   const bool row = w < kRowsPerItem && r <= g.nlat - 2;
-  const double fx = row ? xflux<VL>(g, lim, p.cflx[r * P + iw], p.mfx[r * P + iw], iw, qy + r * P) : 0.0;
+  const double fx = row ? xflux<VL>(g, lim, ldu(p.cflx + r * P, uiw), ldu(p.mfx + r * P, uiw), iw, qy + r * P) : 0.0;
   const double fxw = __shfl_up(fx, 1, kWave);
   __syncthreads();
   const int i = it.c0 - 1 + l;
@@ -486,9 +508,10 @@ __global__ void __launch_bounds__(NF) k_adv_flux(AdvGrid g, Mesh M, int lim, int
     const double lel = M(GMC_LE_LON, r), ac = M(GMC_AREA_CELL, r);
     const double ls = M(GMC_LE_LAT, r - 1), ln = M(GMC_LE_LAT, r);
     const double fs = sfy[w][l], fn = sfy[w + 1][l];
-    const int64_t k = r * P + i;
-    const double qv = p.old_m[k] * q[k] - ((fx - fxw) * lel + (fn * ln - fs * ls)) / ac * dt;
-    qp[k] = qv / p.m[k];
+    const unsigned ui = i;
+    const int64_t ro = r * P;
+    const double qv = ldu(p.old_m + ro, ui) * ldu(q + ro, ui) - ((fx - fxw) * lel + (fn * ln - fs * ls)) / ac * dt;
+    stu(qp + ro, ui, qv / ldu(p.m + ro, ui));
   }
 }
 
@@ -1378,7 +1401,7 @@ __global__ void __launch_bounds__(NF) t_adv_inner(AdvGrid g, Mesh M, int lim, in
     return;
   }
   if (it.r < 0) return;   // padding item (uniform over the block)
-  const int wv = threadIdx.x / kWave, l = threadIdx.x % kWave;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), l = threadIdx.x % kWave;
   const int iw = wrap(it.c0 - 1 + l, n);   // column of the lane
   const int h = it.r - 1 + wv, r = it.r + wv;
   if (h <= g.nhalf - 1) sfy[wv][l] = yflux_tl(g, lim, b.cfly[h * P + iw], b.v[h * P + iw], h, iw, q, d);
@@ -1424,7 +1447,7 @@ __global__ void __launch_bounds__(NF) t_adv_outer(AdvGrid g, Mesh M, int lim, in
     return;
   }
   if (it.r < 0) return;
-  const int wv = threadIdx.x / kWave, l = threadIdx.x % kWave;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), l = threadIdx.x % kWave;
   const int iw = wrap(it.c0 - 1 + l, n);
   const int h = it.r - 1 + wv, r = it.r + wv;
   if (h <= g.nhalf - 1) sfy[wv][l] = yflux_tl(g, lim, b.cfly[h * P + iw], b.mfy[h * P + iw], h, iw, qx, dqx);
