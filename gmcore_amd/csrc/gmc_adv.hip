@@ -279,6 +279,12 @@ struct InnerPtrs {
 __device__ __forceinline__ double xflux_g(const AdvGrid &g, int lim, double c, double u, int i,
                                           const double *row) {
   const int n = g.nlon;
+  // cells i-2 .. i+3: the PPM stencil of either upwind cell when |c| < 1,
+  // read without waiting for c so that these loads and c's are in flight
+  // together
+  double w[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) w[k] = ldu(row, wrap(i - 2 + k, n));
   int ci = (int)c;
   const double cf = c - ci;
   ci = ci > g.hw ? g.hw : (ci < -g.hw ? -g.hw : ci);
@@ -286,10 +292,17 @@ __device__ __forceinline__ double xflux_g(const AdvGrid &g, int lim, double c, d
   // c > 0: upwind cell i - ci, whole cells i+1-ci .. i, u * part(1 - cf, 1) / c;
   // c < 0: upwind cell i - ci + 1, whole cells i+1 .. i-ci, -u * part(0, -cf) / c
   const bool up = c > 0;
-  const int iu = wrap(up ? i - ci : i - ci + 1, n);
-  const double sum = run_sum(row, up ? i + 1 - ci : i + 1, up ? ci : -ci, n);
-  const Ppm p = ppm(lim, ldu(row, wrap(iu - 2, n)), ldu(row, wrap(iu - 1, n)), ldu(row, iu),
-                    ldu(row, wrap(iu + 1, n)), ldu(row, wrap(iu + 2, n)));
+  double f[5], sum = 0;
+  if (ci == 0) {   // upwind cell i or i + 1, no whole cells
+#pragma unroll
+    for (int k = 0; k < 5; k++) f[k] = up ? w[k] : w[k + 1];
+  } else {
+    const int iu = wrap(up ? i - ci : i - ci + 1, n);
+    sum = run_sum(row, up ? i + 1 - ci : i + 1, up ? ci : -ci, n);
+#pragma unroll
+    for (int k = 0; k < 5; k++) f[k] = ldu(row, wrap(iu - 2 + k, n));
+  }
+  const Ppm p = ppm(lim, f[0], f[1], f[2], f[3], f[4]);
   const double fx = (up ? u : -u) * ppm_part_x(up ? 1 - cf : 0, up ? 1 : -cf, p.fl, p.df, p.f6, sum) / c;
   return c == 0 ? 0 : fx;
 }
@@ -405,8 +418,9 @@ __device__ __forceinline__ double wave_row_sum(int n, F f) {
 // (NF, 8): eight wavefronts per SIMD (four workgroups per CU); without the
 // bound the kernel argument and row registers take more scalar registers
 // than that occupancy allows
-template <bool VL>
-__global__ void __launch_bounds__(NF, 8) k_adv_inner(AdvGrid g, Mesh M, int lim, int ntr, InnerPtrs p) {
+template <bool VL, int LIM>
+__global__ void __launch_bounds__(NF, 8) k_adv_inner(AdvGrid g, Mesh M, int ntr, InnerPtrs p) {
+  constexpr int lim = LIM;   // the limiter's branches fold at compile time
   __shared__ double sfy[NF / kWave][kWave];
   __shared__ double bcast;
   const FluxItem it = flux_item(g, ntr);
@@ -438,22 +452,29 @@ __global__ void __launch_bounds__(NF, 8) k_adv_inner(AdvGrid g, Mesh M, int lim,
   const int iw = wrap(it.c0 - 1 + l, n);   // column of the lane
   const unsigned uiw = iw;
   const int h = it.r - 1 + w, r = it.r + w;
+  const bool row = w < kRowsPerItem && r <= g.nlat - 2;   // uniform over the wavefront
+  const int i = it.c0 - 1 + l;   // cell of the lane
+  const bool upd = row && l >= 1 && i < n;
+  const unsigned ui = i;
+  const int64_t ro = r * P;
+  // the update's own loads, issued ahead of the fluxes
+  double qc = 0.0, dvx = 0.0, dvy = 0.0;
+  if (upd) {
+    qc = ldu(q + ro, ui);
+    dvx = ldu(p.divx + ro, ui);
+    dvy = ldu(p.divy + ro, ui);
+  }
   if (h <= g.nhalf - 1)
     sfy[w][l] = yflux_any<VL>(g, lim, ldu(p.cfly + h * P, uiw), ldu(p.v + h * P, uiw), h, iw, q);
-  const bool row = w < kRowsPerItem && r <= g.nlat - 2;   // uniform over the wavefront
-  const double fx = row ? xflux<VL>(g, lim, ldu(p.cflx + r * P, uiw), ldu(p.u + r * P, uiw), iw, q + r * P) : 0.0;
+  const double fx = row ? xflux<VL>(g, lim, ldu(p.cflx + ro, uiw), ldu(p.u + ro, uiw), iw, q + ro) : 0.0;
   const double fxw = __shfl_up(fx, 1, kWave);
   __syncthreads();
-  const int i = it.c0 - 1 + l;   // cell of the lane
-  if (row && l >= 1 && i < n) {
+  if (upd) {
     const double lel = M(GMC_LE_LON, r), ac = M(GMC_AREA_CELL, r);
     const double ls = M(GMC_LE_LAT, r - 1), ln = M(GMC_LE_LAT, r);
     const double fs = sfy[w][l], fn = sfy[w + 1][l];
-    const unsigned ui = i;
-    const int64_t ro = r * P;
-    const double qc = ldu(q + ro, ui);
-    stu(p.qx + toff + ro, ui, qc - 0.5 * ((fx - fxw) * lel / ac - ldu(p.divx + ro, ui) * qc) * dt);
-    stu(p.qy + toff + ro, ui, qc - 0.5 * ((fn * ln - fs * ls) / ac - ldu(p.divy + ro, ui) * qc) * dt);
+    stu(p.qx + toff + ro, ui, qc - 0.5 * ((fx - fxw) * lel / ac - dvx * qc) * dt);
+    stu(p.qy + toff + ro, ui, qc - 0.5 * ((fn * ln - fs * ls) / ac - dvy * qc) * dt);
   }
 }
 
@@ -467,8 +488,9 @@ struct OuterPtrs {
 
 // outer fluxes (ffsl_mod.F90:410) and adv_run's flux-form update and /m
 // (adv_mod.F90:183-235) of every cell, into qp
-template <bool VL>
-__global__ void __launch_bounds__(NF, 8) k_adv_flux(AdvGrid g, Mesh M, int lim, int ntr, OuterPtrs p) {
+template <bool VL, int LIM>
+__global__ void __launch_bounds__(NF, 8) k_adv_flux(AdvGrid g, Mesh M, int ntr, OuterPtrs p) {
+  constexpr int lim = LIM;
   __shared__ double sfy[NF / kWave][kWave];
   __shared__ double bcast;
   const FluxItem it = flux_item(g, ntr);
@@ -497,22 +519,48 @@ __global__ void __launch_bounds__(NF, 8) k_adv_flux(AdvGrid g, Mesh M, int lim, 
   const int iw = wrap(it.c0 - 1 + l, n);
   const unsigned uiw = iw;
   const int h = it.r - 1 + w, r = it.r + w;
+  const bool row = w < kRowsPerItem && r <= g.nlat - 2;
+  const int i = it.c0 - 1 + l;
+  const bool upd = row && l >= 1 && i < n;
+  const unsigned ui = i;
+  const int64_t ro = r * P;
+  // the update's own loads, issued ahead of the fluxes
+  double om = 0.0, qo = 0.0, mn = 1.0;
+  if (upd) {
+    om = ldu(p.old_m + ro, ui);
+    qo = ldu(q + ro, ui);
+    mn = ldu(p.m + ro, ui);
+  }
   if (h <= g.nhalf - 1)
     sfy[w][l] = yflux_any<VL>(g, lim, ldu(p.cfly + h * P, uiw), ldu(p.mfy + h * P, uiw), h, iw, qx);
-  const bool row = w < kRowsPerItem && r <= g.nlat - 2;
-  const double fx = row ? xflux<VL>(g, lim, ldu(p.cflx + r * P, uiw), ldu(p.mfx + r * P, uiw), iw, qy + r * P) : 0.0;
+  const double fx = row ? xflux<VL>(g, lim, ldu(p.cflx + ro, uiw), ldu(p.mfx + ro, uiw), iw, qy + ro) : 0.0;
   const double fxw = __shfl_up(fx, 1, kWave);
   __syncthreads();
-  const int i = it.c0 - 1 + l;
-  if (row && l >= 1 && i < n) {
+  if (upd) {
     const double lel = M(GMC_LE_LON, r), ac = M(GMC_AREA_CELL, r);
     const double ls = M(GMC_LE_LAT, r - 1), ln = M(GMC_LE_LAT, r);
     const double fs = sfy[w][l], fn = sfy[w + 1][l];
-    const unsigned ui = i;
-    const int64_t ro = r * P;
-    const double qv = ldu(p.old_m + ro, ui) * ldu(q + ro, ui) - ((fx - fxw) * lel + (fn * ln - fs * ls)) / ac * dt;
-    stu(qp + ro, ui, qv / ldu(p.m + ro, ui));
+    const double qv = om * qo - ((fx - fxw) * lel + (fn * ln - fs * ls)) / ac * dt;
+    stu(qp + ro, ui, qv / mn);
   }
+}
+
+// the instances of the flux kernels for a run's flux type and limiter
+template <bool VL>
+void (*inner_kernel_lim(int lim))(AdvGrid, Mesh, int, InnerPtrs) {
+  return lim == GMC_LIMITER_PD ? k_adv_inner<VL, GMC_LIMITER_PD>
+       : lim == GMC_LIMITER_NONE ? k_adv_inner<VL, GMC_LIMITER_NONE> : k_adv_inner<VL, GMC_LIMITER_MONO>;
+}
+void (*inner_kernel(bool vl, int lim))(AdvGrid, Mesh, int, InnerPtrs) {
+  return vl ? inner_kernel_lim<true>(lim) : inner_kernel_lim<false>(lim);
+}
+template <bool VL>
+void (*outer_kernel_lim(int lim))(AdvGrid, Mesh, int, OuterPtrs) {
+  return lim == GMC_LIMITER_PD ? k_adv_flux<VL, GMC_LIMITER_PD>
+       : lim == GMC_LIMITER_NONE ? k_adv_flux<VL, GMC_LIMITER_NONE> : k_adv_flux<VL, GMC_LIMITER_MONO>;
+}
+void (*outer_kernel(bool vl, int lim))(AdvGrid, Mesh, int, OuterPtrs) {
+  return vl ? outer_kernel_lim<true>(lim) : outer_kernel_lim<false>(lim);
 }
 
 struct FixPtrs {
@@ -737,12 +785,10 @@ int run_once(gmc_adv *a, int64_t time_step, bool basis, double scale, char *tape
     auto T = [&](double *p) { return tape ? (double *)(tape + ((char *)p - (char *)a->slab)) : p; };
     double *qx = T(a->qx), *qy = T(a->qy), *qp = T(a->qp);
     InnerPtrs ip{tracer_ptr(a, a->old, 0), qx, qy, a->u, a->v, a->cflx, a->cfly, a->divx, a->divy, ts};
-    hipLaunchKernelGGL(vl ? k_adv_inner<true> : k_adv_inner<false>, grid, dim3(NF), 0, a->stream, g, M,
-                       a->cfg.limiter, ntr, ip);
+    hipLaunchKernelGGL(inner_kernel(vl, a->cfg.limiter), grid, dim3(NF), 0, a->stream, g, M, ntr, ip);
     mark(a, KA_INNER);
     OuterPtrs op{qx, qy, tracer_ptr(a, a->old, 0), qp, a->mfx, a->mfy, a->cflx, a->cfly, a->old_m, a->sm, ts};
-    hipLaunchKernelGGL(vl ? k_adv_flux<true> : k_adv_flux<false>, grid, dim3(NF), 0, a->stream, g, M,
-                       a->cfg.limiter, ntr, op);
+    hipLaunchKernelGGL(outer_kernel(vl, a->cfg.limiter), grid, dim3(NF), 0, a->stream, g, M, ntr, op);
     mark(a, KA_FLUX);
     if (tape) {
       // the rest of the step's base state: m, the winds, CFL numbers, split
