@@ -1,8 +1,8 @@
 """Build the native parts in-tree.
 
-- libgmcore_hip.so: the gfx950 HIP library (hipcc), C ABI of include/gmcore_hip.h
-  include/gmcore_adv.h, include/gmcore_ens.h, include/gmcore_tlad.h and
-  include/gmcore_io.h
+- libgmcore_hip.so: the gfx950 HIP library (hipcc), C ABI of include/gmcore_hip.h,
+  include/gmcore_adv.h, include/gmcore_ens.h, include/gmcore_tlad.h,
+  include/gmcore_io.h and include/gmcore_baro.h
 - libgmcore_host.so: the Fortran host setup (amdflang): mesh metrics, filter tables
 - gmcore_swm_gpu, gmcore_adv_gpu: the Fortran drivers (amdflang) on top of both
 
@@ -22,7 +22,8 @@ LIB = os.path.join(HERE, "lib")
 BUILD = os.path.join(HERE, "_build")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-HIP_SOURCES = ["gmc_capi.hip", "gmc_ops.hip", "gmc_fused.hip", "gmc_adv.hip", "gmc_tlad.hip", "gmc_io.hip"]
+HIP_SOURCES = ["gmc_capi.hip", "gmc_ops.hip", "gmc_fused.hip", "gmc_adv.hip", "gmc_tlad.hip", "gmc_io.hip",
+               "gmc_baro.hip"]
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall"]
 F_FLAGS = ["-O2", "-fPIC", "-ffp-contract=off"]

@@ -90,6 +90,28 @@ class GmcH0Record(C.Structure):
                 ("pv", C.POINTER(C.c_double)), ("div", C.POINTER(C.c_double)), ("zs", C.POINTER(C.c_double))]
 
 
+class GmcVcoord(C.Structure):
+    """gmc_vcoord of include/gmcore_baro.h"""
+    _fields_ = [("nlev", C.c_int32), ("reserved", C.c_int32), ("p0", C.c_double), ("ptop", C.c_double),
+                ("hyai", C.POINTER(C.c_double)), ("hybi", C.POINTER(C.c_double))]
+
+
+class GmcThermo(C.Structure):
+    """gmc_thermo of include/gmcore_baro.h"""
+    _fields_ = [("Rd", C.c_double), ("cpd", C.c_double), ("p0", C.c_double)]
+
+
+# gmc_baro_field: (levels: 0 one, 1 full, 2 half; V-point rows)
+BARO_FIELDS = {
+    "phs": (0, False), "gzs": (0, False), "dphs": (0, False), "pt": (1, False), "ph_lev": (2, False),
+    "ph_exn_lev": (2, False), "ph": (1, False), "t": (1, False), "gz_lev": (2, False), "gz": (1, False),
+    "dmfdlon": (1, False), "dmfdlat": (1, False), "we_lev": (2, False), "we_lev_lon": (2, False),
+    "we_lev_lat": (2, True), "u_f": (1, False), "v_f": (1, True), "m_lon": (1, False), "m_lat": (1, True),
+    "wedudlev": (1, False), "wedvdlev": (1, True), "pgf_lon": (1, False), "pgf_lat": (1, True),
+}
+BARO_FIELD = {name: k for k, name in enumerate(BARO_FIELDS)}
+
+
 class GmcAdvConfig(C.Structure):
     """gmc_adv_config of include/gmcore_adv.h"""
     _fields_ = [("radius", C.c_double), ("dt_adv", C.c_double), ("nstep", C.c_int32),
@@ -178,6 +200,21 @@ def load_hip() -> C.CDLL:
         "gmc_restart_write": (C.c_int, [vp, C.c_char_p, C.c_int32, C.c_double, C.c_char_p, C.c_double,
                                         C.c_char_p]),
         "gmc_restart_read": (C.c_int, [vp, C.c_char_p, C.c_int32, P(C.c_double)]),
+        # hydrostatic column operators, include/gmcore_baro.h
+        "gmc_baro_create": (C.c_int, [P(GmcMesh), P(GmcVcoord), P(GmcThermo), P(vp)]),
+        "gmc_baro_destroy": (C.c_int, [vp]),
+        "gmc_baro_last_error": (C.c_char_p, [vp]),
+        "gmc_baro_set": (C.c_int, [vp, C.c_int32, P(C.c_double), C.c_int64]),
+        "gmc_baro_get": (C.c_int, [vp, C.c_int32, P(C.c_double), C.c_int64]),
+        "gmc_baro_calc_ph": (C.c_int, [vp]),
+        "gmc_baro_calc_t": (C.c_int, [vp]),
+        "gmc_baro_calc_gz_lev": (C.c_int, [vp]),
+        "gmc_baro_calc_dphsdt": (C.c_int, [vp]),
+        "gmc_baro_calc_we_lev": (C.c_int, [vp]),
+        "gmc_baro_calc_wedudlev_wedvdlev": (C.c_int, [vp]),
+        "gmc_baro_pgf_lin97": (C.c_int, [vp]),
+        "gmc_baro_synchronize": (C.c_int, [vp]),
+        "gmc_baro_stream": (C.c_int, [vp, P(vp)]),
         # batched ensembles, include/gmcore_ens.h
         "gmc_ens_create": (C.c_int, [P(vp), C.c_int32, P(vp)]),
         "gmc_ens_destroy": (C.c_int, [vp]),

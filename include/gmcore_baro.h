@@ -1,0 +1,130 @@
+/*
+ * gmcore_baro.h -- hydrostatic column operators of GMCORE's baroclinic
+ * dynamics on the MI355X (gfx950): the vertical-coordinate pressures, the
+ * temperature, the geopotential on half levels, the surface-pressure
+ * tendency, the vertical mass flux and its edge interpolations, the vertical
+ * advection of momentum and the Lin (1997) pressure gradient.
+ *
+ * Each entry point is one reference subroutine of operators_mod / pgf_lin97_mod
+ * (cited per function), run over every column of the block.  The horizontal
+ * operators of a level (calc_mf, calc_ke, calc_vor, ...) are those of the
+ * shallow-water library (gmcore_hip.h) applied per level.
+ *
+ * Conventions are those of gmcore_hip.h (rows counted from the south pole,
+ * longitude fastest, interior only) plus a level index: full levels
+ * k = 0 .. nlev-1 are the reference's k = 1 .. nlev counted from the model
+ * top, half levels k = 0 .. nlev are the reference's 1 .. nlev+1 (half level
+ * k is the top face of full level k).  A host buffer of a field holds all its
+ * levels: element (i, r, k) at host[(k * rows + r) * ld + i], rows = nlat for
+ * cell and U-point fields, nlat - 1 for V-point fields.
+ */
+#ifndef GMCORE_BARO_H
+#define GMCORE_BARO_H
+
+#include <stdint.h>
+#include "gmcore_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The hybrid sigma-pressure coordinate (hybrid_coord_mod.F90:146-171):
+ * ph_lev(k) = hyai(k) (p0 - ptop) + hybi(k) (phs - ptop) + ptop and
+ * dphdt_lev(k) = hybi(k) dphs.  p0 is hybrid_coord_p0 (namelist_mod.F90:54),
+ * ptop the coordinate's local_ptop (0 unless the template sets it,
+ * hybrid_coord_mod.F90:27, 76-105).  A sigma coordinate is the case
+ * hyai = 0. */
+typedef struct gmc_vcoord {
+  int32_t nlev;          /* num_full_lev, at least 2                    */
+  int32_t reserved;      /* 0                                           */
+  double p0;             /* hybrid_coord_p0                             */
+  double ptop;           /* local_ptop                                  */
+  const double *hyai;    /* nlev + 1 half-level coefficients            */
+  const double *hybi;    /* nlev + 1                                    */
+} gmc_vcoord;
+
+/* Thermodynamic constants of const_mod (const_mod.F90:56-83). */
+typedef struct gmc_thermo {
+  double Rd;             /* J kg-1 K-1                                  */
+  double cpd;            /* J kg-1 K-1; Rd_o_cpd = Rd / cpd             */
+  double p0;             /* reference pressure of temperature()         */
+} gmc_thermo;
+
+/* state_type / tend_type / static fields of the column operators */
+enum gmc_baro_field {
+  GMB_PHS = 0,       /* state%phs_f       cells, 1 level                 */
+  GMB_GZS,           /* static%gzs        cells, 1 level                 */
+  GMB_DPHS,          /* tend%dphs         cells, 1 level                 */
+  GMB_PT,            /* state%pt          cells, full levels             */
+  GMB_PH_LEV,        /* state%ph_lev      cells, half levels             */
+  GMB_PH_EXN_LEV,    /* state%ph_exn_lev  cells, half levels             */
+  GMB_PH,            /* state%ph          cells, full levels             */
+  GMB_T,             /* state%t           cells, full levels             */
+  GMB_GZ_LEV,        /* state%gz_lev      cells, half levels             */
+  GMB_GZ,            /* state%gz          cells, full levels             */
+  GMB_DMFDLON,       /* tend%dmfdlon      cells, full levels             */
+  GMB_DMFDLAT,       /* tend%dmfdlat      cells, full levels             */
+  GMB_WE_LEV,        /* state%we_lev      cells, half levels             */
+  GMB_WE_LEV_LON,    /* state%we_lev_lon  U points, half levels          */
+  GMB_WE_LEV_LAT,    /* state%we_lev_lat  V points, half levels          */
+  GMB_U_F,           /* state%u_f         U points, full levels          */
+  GMB_V_F,           /* state%v_f         V points, full levels          */
+  GMB_M_LON,         /* state%m_lon       U points, full levels          */
+  GMB_M_LAT,         /* state%m_lat       V points, full levels          */
+  GMB_WEDUDLEV,      /* tend%wedudlev     U points, full levels          */
+  GMB_WEDVDLEV,      /* tend%wedvdlev     V points, full levels          */
+  GMB_PGF_LON,       /* tend%pgf_lon      U points, full levels          */
+  GMB_PGF_LAT,       /* tend%pgf_lat      V points, full levels          */
+  GMB_NFIELDS
+};
+
+typedef struct gmc_baro gmc_baro;
+
+/* One block of nlev levels on the current device, every field zero. */
+int gmc_baro_create(const gmc_mesh *mesh, const gmc_vcoord *vc, const gmc_thermo *th, gmc_baro **out);
+int gmc_baro_destroy(gmc_baro *b);
+const char *gmc_baro_last_error(const gmc_baro *b);
+
+/* All levels of a field <-> host (layout above); ld >= nlon. */
+int gmc_baro_set(gmc_baro *b, int32_t field, const double *host, int64_t ld);
+int gmc_baro_get(gmc_baro *b, int32_t field, double *host, int64_t ld);
+
+/* calc_ph (operators_mod.F90:136-168): ph_lev and ph_exn_lev = ph_lev **
+ * Rd_o_cpd on the half levels from phs, ph on the full levels. */
+int gmc_baro_calc_ph(gmc_baro *b);
+/* calc_t (operators_mod.F90:170-190): t = temperature(pt, ph)
+ * (formula_mod.F90:24-31), pt (ph / p0) ** Rd_o_cpd. */
+int gmc_baro_calc_t(gmc_baro *b);
+/* calc_gz_lev (operators_mod.F90:433-469): gz_lev on half levels 0 ..
+ * nlev-1 from gzs, t and ph_lev (each level's sum from that level down, in
+ * ascending order), gz on the full levels.  The bottom half level keeps its
+ * value: gmcore_run sets it to gzs once (gmcore_mod.F90:109). */
+int gmc_baro_calc_gz_lev(gmc_baro *b);
+/* calc_dphsdt (operators_mod.F90:1120-1143): dphs = -sum over the levels
+ * of dmfdlon + dmfdlat. */
+int gmc_baro_calc_dphsdt(gmc_baro *b);
+/* calc_we_lev (operators_mod.F90:192-232) without the advection batch's
+ * accumulation: we_lev on the interior half levels from dphs and the
+ * column's mass-flux divergence above each level, 0 at the top and the
+ * bottom; we_lev_lon and we_lev_lat by the area-weighted edge interpolations
+ * with no wind given (interp_mod.F90:360-369, :425-434; the U points of the
+ * pole rows are not written). */
+int gmc_baro_calc_we_lev(gmc_baro *b);
+/* calc_wedudlev_wedvdlev (operators_mod.F90:1145-1216): the SB81 vertical
+ * advection of u_f and v_f by we_lev_lon / we_lev_lat over m_lon / m_lat
+ * (U points off the pole rows, every V point). */
+int gmc_baro_calc_wedudlev_wedvdlev(gmc_baro *b);
+/* pgf_lin97_run, hydrostatic branch (pgf_lin97_mod.F90:20-105): pgf_lon on
+ * the U points off the pole rows, pgf_lat on every V point, from ph_exn_lev
+ * and gz_lev. */
+int gmc_baro_pgf_lin97(gmc_baro *b);
+
+/* Wait for the block's queued work; its stream (hipStream_t). */
+int gmc_baro_synchronize(gmc_baro *b);
+int gmc_baro_stream(gmc_baro *b, void **stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GMCORE_BARO_H */

@@ -341,3 +341,100 @@ class CoupledOracle:
         self.time_step += 1
         return rc
 
+
+
+def equal_eta(nlev: int):
+    """hyai, hybi of hybrid_coord_equal_eta (hybrid_coord_test_mod.F90:1330-1352)."""
+    L = lib()
+    L.gmo_baro_equal_eta.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    a, b = np.zeros(nlev + 1), np.zeros(nlev + 1)
+    L.gmo_baro_equal_eta(nlev, dptr(a), dptr(b))
+    return a, b
+
+
+class BaroOracle:
+    """The hydrostatic column operators of baro.c on [levels, rows, nlon]
+    arrays (the layout of gmcore_baro.h).  Outputs the reference leaves
+    unwritten (U points of the pole rows, the bottom half level of gz_lev)
+    come back as given or zero."""
+
+    def __init__(self, metrics, nlon: int, nlat: int, hyai, hybi, p0: float = 1.0e5, ptop: float = 0.0,
+                 Rd: float = 287.04, cpd: float = 1004.0, tp0: float = 1.0e5):
+        self.m = np.ascontiguousarray(metrics, dtype=np.float64)
+        self.nlon, self.nlat = nlon, nlat
+        self.hyai = np.ascontiguousarray(hyai, dtype=np.float64)
+        self.hybi = np.ascontiguousarray(hybi, dtype=np.float64)
+        self.nlev = self.hyai.size - 1
+        self.p0, self.ptop, self.Rd, self.cpd, self.tp0 = p0, ptop, Rd, cpd, tp0
+        self.L = lib()
+        P, i, d = C.POINTER(C.c_double), C.c_int, C.c_double
+        L = self.L
+        L.gmo_baro_calc_ph.argtypes = [i, i, i, P, P, d, d, d, d, P, P, P, P]
+        L.gmo_baro_calc_t.argtypes = [i, i, i, d, d, d, P, P, P]
+        L.gmo_baro_calc_gz_lev.argtypes = [i, i, i, d, P, P, P, P, P]
+        L.gmo_baro_calc_dphsdt.argtypes = [i, i, i, P, P, P]
+        L.gmo_baro_calc_we_lev.argtypes = [i, i, i, P, P, P, P, P, P, P, P]
+        L.gmo_baro_calc_wedudlev_wedvdlev.argtypes = [i, i, i] + [P] * 8
+        L.gmo_baro_pgf_lin97.argtypes = [i, i, i, P, P, P, P, P]
+
+    def _a(self, x, levels, vrows=False):
+        shape = (levels, self.nlat - 1 if vrows else self.nlat, self.nlon)
+        return np.ascontiguousarray(np.broadcast_to(x, shape), dtype=np.float64)
+
+    def _z(self, levels, vrows=False):
+        return np.zeros((levels, self.nlat - 1 if vrows else self.nlat, self.nlon))
+
+    def calc_ph(self, phs):
+        n = self.nlev
+        phs = self._a(phs, 1)
+        ph_lev, exn, ph = self._z(n + 1), self._z(n + 1), self._z(n)
+        self.L.gmo_baro_calc_ph(self.nlon, self.nlat, n, dptr(self.hyai), dptr(self.hybi), self.p0, self.ptop,
+                                self.Rd, self.cpd, dptr(phs), dptr(ph_lev), dptr(exn), dptr(ph))
+        return ph_lev, exn, ph
+
+    def calc_t(self, pt, ph):
+        n = self.nlev
+        pt, ph = self._a(pt, n), self._a(ph, n)
+        t = self._z(n)
+        self.L.gmo_baro_calc_t(self.nlon, self.nlat, n, self.Rd, self.cpd, self.tp0, dptr(pt), dptr(ph), dptr(t))
+        return t
+
+    def calc_gz_lev(self, gzs, t, ph_lev, gz_lev_in=None):
+        n = self.nlev
+        gzs, t, ph_lev = self._a(gzs, 1), self._a(t, n), self._a(ph_lev, n + 1)
+        gz_lev = self._z(n + 1) if gz_lev_in is None else self._a(gz_lev_in, n + 1).copy()
+        gz = self._z(n)
+        self.L.gmo_baro_calc_gz_lev(self.nlon, self.nlat, n, self.Rd, dptr(gzs), dptr(t), dptr(ph_lev),
+                                    dptr(gz_lev), dptr(gz))
+        return gz_lev, gz
+
+    def calc_dphsdt(self, dmfdlon, dmfdlat):
+        n = self.nlev
+        a, b = self._a(dmfdlon, n), self._a(dmfdlat, n)
+        dphs = self._z(1)
+        self.L.gmo_baro_calc_dphsdt(self.nlon, self.nlat, n, dptr(a), dptr(b), dptr(dphs))
+        return dphs
+
+    def calc_we_lev(self, dmfdlon, dmfdlat, dphs):
+        n = self.nlev
+        a, b, d = self._a(dmfdlon, n), self._a(dmfdlat, n), self._a(dphs, 1)
+        we, wlon, wlat = self._z(n + 1), self._z(n + 1), self._z(n + 1, True)
+        self.L.gmo_baro_calc_we_lev(self.nlon, self.nlat, n, dptr(self.m), dptr(self.hybi), dptr(a), dptr(b),
+                                    dptr(d), dptr(we), dptr(wlon), dptr(wlat))
+        return we, wlon, wlat
+
+    def calc_wedudlev_wedvdlev(self, u, v, m_lon, m_lat, we_lon, we_lat):
+        n = self.nlev
+        u, m_lon, we_lon = self._a(u, n), self._a(m_lon, n), self._a(we_lon, n + 1)
+        v, m_lat, we_lat = self._a(v, n, True), self._a(m_lat, n, True), self._a(we_lat, n + 1, True)
+        du, dv = self._z(n), self._z(n, True)
+        self.L.gmo_baro_calc_wedudlev_wedvdlev(self.nlon, self.nlat, n, dptr(u), dptr(v), dptr(m_lon), dptr(m_lat),
+                                               dptr(we_lon), dptr(we_lat), dptr(du), dptr(dv))
+        return du, dv
+
+    def pgf_lin97(self, ph_exn_lev, gz_lev):
+        n = self.nlev
+        e, z = self._a(ph_exn_lev, n + 1), self._a(gz_lev, n + 1)
+        pl, pt = self._z(n), self._z(n, True)
+        self.L.gmo_baro_pgf_lin97(self.nlon, self.nlat, n, dptr(self.m), dptr(e), dptr(z), dptr(pl), dptr(pt))
+        return pl, pt

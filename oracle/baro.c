@@ -1,0 +1,176 @@
+/*
+ * baro.c -- CPU restatement of GMCORE's hydrostatic column operators
+ * (TEST INFRASTRUCTURE ONLY: the checker of gmc_baro_* in the HIP library).
+ *
+ * Arrays use the host layout of include/gmcore_baro.h: element (i, r, k) of a
+ * field at a[(k * rows + r) * nlon + i], rows = nlat for cells and U points,
+ * nlat - 1 for V points; full levels 0 .. nlev-1, half levels 0 .. nlev from
+ * the model top.  Each function follows the reference loop nest it cites,
+ * with the reference's operation order; the build uses -ffp-contract=off and
+ * libm's pow and log.
+ */
+#include <math.h>
+#include <stddef.h>
+
+#include "gmc_oracle.h"
+
+#define MX(f, r) (metrics[(size_t)(f) * nlat + (r)])
+
+/* calc_ph, operators_mod.F90:136-168, with hybrid_coord_calc_ph_lev
+ * (hybrid_coord_mod.F90:155-162) and Rd_o_cpd = Rd / cpd (const_mod.F90:81) */
+void gmo_baro_calc_ph(int nlon, int nlat, int nlev, const double *hyai, const double *hybi, double p0,
+                      double ptop, double Rd, double cpd, const double *phs, double *ph_lev, double *ph_exn_lev,
+                      double *ph) {
+  const size_t L = (size_t)nlat * nlon;
+  const double rd_o_cpd = Rd / cpd;
+  for (int k = 0; k <= nlev; k++)
+    for (size_t c = 0; c < L; c++) {
+      ph_lev[k * L + c] = hyai[k] * (p0 - ptop) + hybi[k] * (phs[c] - ptop) + ptop;
+      ph_exn_lev[k * L + c] = pow(ph_lev[k * L + c], rd_o_cpd);
+    }
+  for (int k = 0; k < nlev; k++)
+    for (size_t c = 0; c < L; c++) ph[k * L + c] = 0.5 * (ph_lev[k * L + c] + ph_lev[(k + 1) * L + c]);
+}
+
+/* calc_t, operators_mod.F90:170-190; temperature(), formula_mod.F90:24-31 */
+void gmo_baro_calc_t(int nlon, int nlat, int nlev, double Rd, double cpd, double p0, const double *pt,
+                     const double *ph, double *t) {
+  const size_t n = (size_t)nlev * nlat * nlon;
+  const double rd_o_cpd = Rd / cpd;
+  for (size_t c = 0; c < n; c++) t[c] = pt[c] * pow(ph[c] / p0, rd_o_cpd);
+}
+
+/* calc_gz_lev, operators_mod.F90:433-469: half levels 0 .. nlev-1 (the
+ * bottom one keeps its value), then the full-level means */
+void gmo_baro_calc_gz_lev(int nlon, int nlat, int nlev, double Rd, const double *gzs, const double *t,
+                          const double *ph_lev, double *gz_lev, double *gz) {
+  const size_t L = (size_t)nlat * nlon;
+  for (int k = 0; k < nlev; k++)
+    for (size_t c = 0; c < L; c++) {
+      double dgz = 0.0;
+      for (int l = k; l < nlev; l++) dgz = dgz + Rd * t[l * L + c] * log(ph_lev[(l + 1) * L + c] / ph_lev[l * L + c]);
+      gz_lev[k * L + c] = gzs[c] + dgz;
+    }
+  for (int k = 0; k < nlev; k++)
+    for (size_t c = 0; c < L; c++) gz[k * L + c] = 0.5 * (gz_lev[k * L + c] + gz_lev[(k + 1) * L + c]);
+}
+
+/* calc_dphsdt, operators_mod.F90:1120-1143 */
+void gmo_baro_calc_dphsdt(int nlon, int nlat, int nlev, const double *dmfdlon, const double *dmfdlat,
+                          double *dphs) {
+  const size_t L = (size_t)nlat * nlon;
+  for (size_t c = 0; c < L; c++) dphs[c] = 0.0;
+  for (int k = 0; k < nlev; k++)
+    for (size_t c = 0; c < L; c++) dphs[c] = dphs[c] - dmfdlon[k * L + c] - dmfdlat[k * L + c];
+}
+
+/* calc_we_lev, operators_mod.F90:192-232 (the advection batch's
+ * accum_we_lev aside): every interior half level re-adds the divergence of
+ * the levels above it from 0, as the reference does; then
+ * interp_lev_edge_to_lev_lon_edge / _lat_edge with no wind
+ * (interp_mod.F90:360-369, :425-434) */
+void gmo_baro_calc_we_lev(int nlon, int nlat, int nlev, const double *metrics, const double *hybi,
+                          const double *dmfdlon, const double *dmfdlat, const double *dphs, double *we_lev,
+                          double *we_lev_lon, double *we_lev_lat) {
+  const size_t L = (size_t)nlat * nlon, LV = (size_t)(nlat - 1) * nlon;
+  for (int k = 1; k < nlev; k++)
+    for (size_t c = 0; c < L; c++) {
+      double mf = 0.0;
+      for (int l = 0; l < k; l++) mf = mf + dmfdlon[l * L + c] + dmfdlat[l * L + c];
+      we_lev[k * L + c] = -(hybi[k] * dphs[c]) - mf;
+    }
+  for (size_t c = 0; c < L; c++) {
+    we_lev[c] = 0.0;
+    we_lev[(size_t)nlev * L + c] = 0.0;
+  }
+  for (int k = 0; k <= nlev; k++) {
+    const double *x = we_lev + k * L;
+    for (int r = 1; r < nlat - 1; r++)
+      for (int i = 0; i < nlon; i++) {
+        const int ie = (i + 1) % nlon;
+        we_lev_lon[k * L + (size_t)r * nlon + i] =
+            (MX(GMC_AREA_LON_WEST, r) * x[(size_t)r * nlon + i] + MX(GMC_AREA_LON_EAST, r) * x[(size_t)r * nlon + ie]) /
+            MX(GMC_AREA_LON, r);
+      }
+    for (int r = 0; r < nlat - 1; r++)
+      for (int i = 0; i < nlon; i++)
+        we_lev_lat[k * LV + (size_t)r * nlon + i] =
+            (MX(GMC_AREA_LAT_NORTH, r) * x[(size_t)(r + 1) * nlon + i] + MX(GMC_AREA_LAT_SOUTH, r) * x[(size_t)r * nlon + i]) /
+            MX(GMC_AREA_LAT, r);
+  }
+}
+
+/* calc_wedudlev_wedvdlev, operators_mod.F90:1145-1216 (SB81) */
+void gmo_baro_calc_wedudlev_wedvdlev(int nlon, int nlat, int nlev, const double *u, const double *v,
+                                     const double *m_lon, const double *m_lat, const double *we_lev_lon,
+                                     const double *we_lev_lat, double *wedudlev, double *wedvdlev) {
+  const size_t L = (size_t)nlat * nlon, LV = (size_t)(nlat - 1) * nlon;
+  for (int k = 1; k < nlev - 1; k++)
+    for (int r = 1; r < nlat - 1; r++)
+      for (int i = 0; i < nlon; i++) {
+        const size_t c = (size_t)r * nlon + i;
+        wedudlev[k * L + c] = (we_lev_lon[(k + 1) * L + c] * (u[(k + 1) * L + c] - u[k * L + c]) +
+                               we_lev_lon[k * L + c] * (u[k * L + c] - u[(k - 1) * L + c])) /
+                              m_lon[k * L + c] / 2.0;
+      }
+  for (int r = 1; r < nlat - 1; r++)
+    for (int i = 0; i < nlon; i++) {
+      const size_t c = (size_t)r * nlon + i, kb = nlev - 1;
+      wedudlev[c] = (we_lev_lon[L + c] * (u[L + c] - u[c])) / m_lon[c] / 2.0;
+      wedudlev[kb * L + c] = (we_lev_lon[kb * L + c] * (u[kb * L + c] - u[(kb - 1) * L + c])) / m_lon[kb * L + c] / 2.0;
+    }
+  for (int k = 1; k < nlev - 1; k++)
+    for (size_t c = 0; c < LV; c++)
+      wedvdlev[k * LV + c] = (we_lev_lat[(k + 1) * LV + c] * (v[(k + 1) * LV + c] - v[k * LV + c]) +
+                              we_lev_lat[k * LV + c] * (v[k * LV + c] - v[(k - 1) * LV + c])) /
+                             m_lat[k * LV + c] / 2.0;
+  for (size_t c = 0; c < LV; c++) {
+    const size_t kb = nlev - 1;
+    wedvdlev[c] = (we_lev_lat[LV + c] * (v[LV + c] - v[c])) / m_lat[c] / 2.0;
+    wedvdlev[kb * LV + c] = (we_lev_lat[kb * LV + c] * (v[kb * LV + c] - v[(kb - 1) * LV + c])) / m_lat[kb * LV + c] / 2.0;
+  }
+}
+
+/* pgf_lin97_run, hydrostatic branch, pgf_lin97_mod.F90:59-105 */
+void gmo_baro_pgf_lin97(int nlon, int nlat, int nlev, const double *metrics, const double *ph_exn_lev,
+                        const double *gz_lev, double *pgf_lon, double *pgf_lat) {
+  const size_t L = (size_t)nlat * nlon, LV = (size_t)(nlat - 1) * nlon;
+  const double *e = ph_exn_lev, *z = gz_lev;
+  for (int k = 0; k < nlev; k++) {
+    const size_t t = k * L, b = (k + 1) * L;   /* the full level's top and bottom half levels */
+    for (int r = 1; r < nlat - 1; r++)
+      for (int i = 0; i < nlon; i++) {
+        const size_t c = (size_t)r * nlon + i, ce = (size_t)r * nlon + (i + 1) % nlon;
+        const double dph1 = e[b + ce] - e[t + c];
+        const double dph2 = e[b + c] - e[t + ce];
+        const double dgz1 = z[b + c] - z[t + ce];
+        const double dgz2 = z[t + c] - z[b + ce];
+        pgf_lon[k * L + c] = -(dph1 * dgz1 + dph2 * dgz2) / MX(GMC_DE_LON, r) / (dph1 + dph2);
+      }
+    for (int r = 0; r < nlat - 1; r++)
+      for (int i = 0; i < nlon; i++) {
+        const size_t c = (size_t)r * nlon + i, cn = (size_t)(r + 1) * nlon + i;
+        const double dph1 = e[b + cn] - e[t + c];
+        const double dph2 = e[b + c] - e[t + cn];
+        const double dgz1 = z[b + c] - z[t + cn];
+        const double dgz2 = z[t + c] - z[b + cn];
+        pgf_lat[k * LV + c] = -(dph1 * dgz1 + dph2 * dgz2) / MX(GMC_DE_LAT, r) / (dph1 + dph2);
+      }
+  }
+}
+
+/* hybrid_coord_equal_eta (hybrid_coord_test_mod.F90:1330-1352): nlev + 1
+ * half-level coefficients; etac is the default-real literal 0.2 widened. */
+void gmo_baro_equal_eta(int nlev, double *hyai, double *hybi) {
+  const double deta = 1.0 / nlev, etac = (double)0.2f;
+  const double d3 = (1 - etac) * (1 - etac) * (1 - etac);
+  const double c1 = 2 * (etac * etac) / d3;
+  const double c2 = -etac * (4 + etac + etac * etac) / d3;
+  const double c3 = 2 * (1 + etac + etac * etac) / d3;
+  const double c4 = -(1 + etac) / d3;
+  for (int k = 0; k <= nlev; k++) {
+    const double eta = k * deta;
+    hybi[k] = eta < etac ? 0.0 : c1 + c2 * eta + c3 * (eta * eta) + c4 * (eta * eta * eta);
+    hyai[k] = eta - hybi[k];
+  }
+}

@@ -113,6 +113,28 @@ void gmo_ppm_jac(int limiter, const double f[5], double J[3][5]);
 /* ppm (ffsl_mod.F90:1147-1173) of f[0..4]: out = fl, df, f6 */
 void gmo_ppm(int limiter, const double f[5], double out[3]);
 
+/* Hydrostatic column operators (baro.c), arrays in the host layout of
+ * include/gmcore_baro.h without padding (ld = nlon). */
+void gmo_baro_calc_ph(int nlon, int nlat, int nlev, const double *hyai, const double *hybi, double p0,
+                      double ptop, double Rd, double cpd, const double *phs, double *ph_lev, double *ph_exn_lev,
+                      double *ph);
+void gmo_baro_calc_t(int nlon, int nlat, int nlev, double Rd, double cpd, double p0, const double *pt,
+                     const double *ph, double *t);
+void gmo_baro_calc_gz_lev(int nlon, int nlat, int nlev, double Rd, const double *gzs, const double *t,
+                          const double *ph_lev, double *gz_lev, double *gz);
+void gmo_baro_calc_dphsdt(int nlon, int nlat, int nlev, const double *dmfdlon, const double *dmfdlat,
+                          double *dphs);
+void gmo_baro_calc_we_lev(int nlon, int nlat, int nlev, const double *metrics, const double *hybi,
+                          const double *dmfdlon, const double *dmfdlat, const double *dphs, double *we_lev,
+                          double *we_lev_lon, double *we_lev_lat);
+void gmo_baro_calc_wedudlev_wedvdlev(int nlon, int nlat, int nlev, const double *u, const double *v,
+                                     const double *m_lon, const double *m_lat, const double *we_lev_lon,
+                                     const double *we_lev_lat, double *wedudlev, double *wedvdlev);
+void gmo_baro_pgf_lin97(int nlon, int nlat, int nlev, const double *metrics, const double *ph_exn_lev,
+                        const double *gz_lev, double *pgf_lon, double *pgf_lat);
+/* hybrid_coord_equal_eta (hybrid_coord_test_mod.F90:1330-1352) */
+void gmo_baro_equal_eta(int nlev, double *hyai, double *hybi);
+
 #ifdef __cplusplus
 }
 #endif

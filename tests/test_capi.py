@@ -17,7 +17,7 @@ def declared(header):
     return sorted(set(re.findall(r"\b(gmc_[a-z_0-9]+)\s*\(", text)))
 
 
-HEADERS = ["gmcore_hip.h", "gmcore_adv.h", "gmcore_ens.h", "gmcore_tlad.h", "gmcore_io.h"]
+HEADERS = ["gmcore_hip.h", "gmcore_adv.h", "gmcore_ens.h", "gmcore_tlad.h", "gmcore_io.h", "gmcore_baro.h"]
 
 
 def test_every_header_is_checked():

@@ -1,0 +1,186 @@
+"""Pins of the oracle's hydrostatic column operators (oracle/baro.c) by what
+the mathematics fixes: closed forms of isothermal and linear columns, exact
+cancellations and the column mass budget.  CPU only."""
+import numpy as np
+import pytest
+
+import oracle
+from gmcore_amd.capi import MESH
+
+NLON, NLAT, NLEV = 36, 19, 12
+RD, CPD, P0, PTOP = 287.04, 1004.0, 1.0e5, 219.4
+
+
+@pytest.fixture(scope="module")
+def setup():
+    m = oracle.mesh_metrics(NLON, NLAT, 6371220.0, 7.292e-5)
+    hyai, hybi = oracle.equal_eta(NLEV)
+    return m, hyai, hybi, oracle.BaroOracle(m, NLON, NLAT, hyai, hybi, P0, PTOP, RD, CPD, P0)
+
+
+def surface_pressure(seed=0):
+    rng = np.random.default_rng(seed)
+    return 1.0e5 + 2000.0 * rng.standard_normal((1, NLAT, NLON))
+
+
+def test_equal_eta_coordinate():
+    """hybrid_coord_equal_eta: eta = k / nlev split into a pure-pressure top
+    (hybi = 0 for eta < 0.2) and a cubic hybi reaching 1 at the surface."""
+    a, b = oracle.equal_eta(NLEV)
+    eta = np.arange(NLEV + 1) / NLEV
+    np.testing.assert_allclose(a + b, eta, rtol=0, atol=2e-16)
+    assert np.all(b[eta < 0.2] == 0) and np.all(b[eta >= 0.2] > -1e-16)
+    assert b[-1] == pytest.approx(1.0, abs=1e-15) and a[-1] == pytest.approx(0.0, abs=1e-15)
+    assert np.all(np.diff(b) >= 0)
+
+
+def test_calc_ph_faces(setup):
+    """ph_lev: ptop at the model top, phs at the surface, the means on the
+    full levels, and the Exner power."""
+    m, hyai, hybi, o = setup
+    phs = surface_pressure()
+    ph_lev, exn, ph = o.calc_ph(phs)
+    assert np.all(ph_lev[0] == PTOP)
+    np.testing.assert_allclose(ph_lev[-1], phs[0], rtol=1e-15)
+    np.testing.assert_array_equal(ph, 0.5 * (ph_lev[:-1] + ph_lev[1:]))
+    np.testing.assert_allclose(exn, ph_lev ** (RD / CPD), rtol=2e-16)
+    assert np.all(np.diff(ph_lev, axis=0) > 0)
+
+
+def test_sigma_coordinate():
+    """hyai = 0 is the sigma coordinate: ph_lev = sigma (phs - ptop) + ptop."""
+    m = oracle.mesh_metrics(NLON, NLAT, 6371220.0, 7.292e-5)
+    sig = np.linspace(0.0, 1.0, NLEV + 1)
+    o = oracle.BaroOracle(m, NLON, NLAT, np.zeros(NLEV + 1), sig, P0, PTOP, RD, CPD, P0)
+    phs = surface_pressure(1)
+    ph_lev, _, _ = o.calc_ph(phs)
+    np.testing.assert_allclose(ph_lev, sig[:, None, None] * (phs - PTOP) + PTOP, rtol=1e-15)
+
+
+def test_isothermal_column(setup):
+    """An isothermal column: pt = T0 (p0 / ph)^(Rd/cpd) gives t = T0, and the
+    hydrostatic sums telescope, gz_lev(k) = gzs + Rd T0 log(phs / ph_lev(k))."""
+    m, hyai, hybi, o = setup
+    phs = surface_pressure(2)
+    gzs = 500.0 * np.random.default_rng(3).random((1, NLAT, NLON))
+    ph_lev, _, ph = o.calc_ph(phs)
+    T0 = 250.0
+    t = o.calc_t(T0 * (P0 / ph) ** (RD / CPD), ph)
+    np.testing.assert_allclose(t, T0, rtol=1e-13)
+    bottom = np.zeros((NLEV + 1, NLAT, NLON))
+    bottom[-1] = gzs[0]
+    gz_lev, gz = o.calc_gz_lev(gzs, np.full((NLEV, NLAT, NLON), T0), ph_lev, bottom)
+    expect = gzs + RD * T0 * np.log(ph_lev[-1] / ph_lev)
+    np.testing.assert_allclose(gz_lev[:-1], expect[:-1], rtol=1e-12)
+    assert np.array_equal(gz_lev[-1], gzs[0])   # the bottom half level keeps its value
+    np.testing.assert_array_equal(gz, 0.5 * (gz_lev[:-1] + gz_lev[1:]))
+    assert np.all(np.diff(gz_lev, axis=0) < 0)   # geopotential falls with depth
+
+
+def test_gz_lev_sums_each_level_from_its_own_layer(setup):
+    """Each half level's sum starts at 0 from its own layer downwards: a layer
+    whose term is zero (t = 0) leaves every level above it unchanged."""
+    m, hyai, hybi, o = setup
+    ph_lev, _, _ = o.calc_ph(surface_pressure(4))
+    t = np.full((NLEV, NLAT, NLON), 280.0)
+    t[5] = 0.0
+    a, _ = o.calc_gz_lev(np.zeros((1, NLAT, NLON)), t, ph_lev)
+    t2 = t.copy()
+    t2[5] = 0.0
+    t2[2] = 300.0
+    b, _ = o.calc_gz_lev(np.zeros((1, NLAT, NLON)), t2, ph_lev)
+    np.testing.assert_array_equal(a[3:NLEV], b[3:NLEV])    # levels below the changed layer
+    assert np.all(b[:3] > a[:3])                           # levels above it see the warmer layer
+
+
+def test_dphsdt_and_we_lev_single_layer(setup):
+    """Divergence d in one layer l0 only: dphs = -d, and we_lev is hybi(k) d
+    above the layer's bottom face and (hybi(k) - 1) d below it; zero at the
+    model top and the surface."""
+    m, hyai, hybi, o = setup
+    l0 = 7
+    d = np.random.default_rng(5).standard_normal((NLAT, NLON))
+    dmfdlon = np.zeros((NLEV, NLAT, NLON))
+    dmfdlon[l0] = d
+    dmfdlat = np.zeros_like(dmfdlon)
+    dphs = o.calc_dphsdt(dmfdlon, dmfdlat)
+    np.testing.assert_array_equal(dphs[0], -d)
+    we, _, _ = o.calc_we_lev(dmfdlon, dmfdlat, dphs)
+    assert np.all(we[0] == 0) and np.all(we[-1] == 0)
+    for k in range(1, NLEV):
+        expect = hybi[k] * d if k <= l0 else (hybi[k] - 1.0) * d
+        np.testing.assert_allclose(we[k], expect, rtol=1e-14, atol=1e-300)
+
+
+def test_we_lev_column_budget(setup):
+    """we_lev(k) + sum of the divergence above k equals -hybi(k) dphs, and
+    dphs is minus the column total (to rounding)."""
+    m, hyai, hybi, o = setup
+    rng = np.random.default_rng(6)
+    a, b = rng.standard_normal((NLEV, NLAT, NLON)), rng.standard_normal((NLEV, NLAT, NLON))
+    dphs = o.calc_dphsdt(a, b)
+    np.testing.assert_allclose(dphs[0], -(a + b).sum(axis=0), rtol=1e-12, atol=1e-12)
+    we, _, _ = o.calc_we_lev(a, b, dphs)
+    above = np.cumsum(a + b, axis=0)
+    for k in range(1, NLEV):
+        np.testing.assert_allclose(we[k] + above[k - 1], -hybi[k] * dphs[0], rtol=1e-11, atol=1e-11)
+
+
+def test_we_edges_of_uniform_field(setup):
+    """The area-weighted edge means of a horizontally uniform we_lev return it
+    (west + east sub-areas make the U cell, north + south the V cell)."""
+    m, hyai, hybi, o = setup
+    c = np.linspace(-1.0, 1.0, NLEV)
+    dmfdlon = np.broadcast_to(c[:, None, None], (NLEV, NLAT, NLON)).copy()
+    dphs = np.full((1, NLAT, NLON), 0.37)
+    we, wlon, wlat = o.calc_we_lev(dmfdlon, np.zeros_like(dmfdlon), dphs)
+    np.testing.assert_allclose(wlon[:, 1:-1], we[:, 1:-1], rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(wlat, we[:, :-1], rtol=1e-13, atol=1e-13)
+    assert np.all(wlon[:, 0] == 0) and np.all(wlon[:, -1] == 0)   # pole rows not written
+
+
+def test_wedudlev_linear_profile(setup):
+    """u linear in the level index, we and m uniform: the SB81 form gives
+    we b / m inside and half of it on the top and bottom levels."""
+    m, hyai, hybi, o = setup
+    W, M, a, b = 0.3, 2.0e4, 5.0, 1.5
+    k = np.arange(NLEV, dtype=float)[:, None, None]
+    u = np.broadcast_to(a + b * k, (NLEV, NLAT, NLON))
+    v = np.broadcast_to(a + b * k, (NLEV, NLAT - 1, NLON))
+    du, dv = o.calc_wedudlev_wedvdlev(u, v, M, M, W, W)
+    np.testing.assert_allclose(du[1:-1, 1:-1], W * b / M, rtol=1e-14)
+    np.testing.assert_allclose(du[[0, -1]][:, 1:-1], W * b / M / 2, rtol=1e-14)
+    np.testing.assert_allclose(dv[1:-1], W * b / M, rtol=1e-14)
+    np.testing.assert_allclose(dv[[0, -1]], W * b / M / 2, rtol=1e-14)
+    assert np.all(du[:, [0, -1]] == 0)
+
+
+def test_pgf_lin97_uniform_state_is_exactly_zero(setup):
+    """A horizontally uniform column state has no pressure gradient: the
+    two quadrilateral products cancel exactly."""
+    m, hyai, hybi, o = setup
+    ph_lev, exn, _ = o.calc_ph(np.full((1, NLAT, NLON), 1.0e5))
+    gz = np.broadcast_to(np.linspace(3.0e5, 0.0, NLEV + 1)[:, None, None], (NLEV + 1, NLAT, NLON))
+    pl, pt = o.pgf_lin97(exn, gz)
+    assert np.all(pl == 0) and np.all(pt == 0)
+
+
+def test_pgf_lin97_linear_geopotential(setup):
+    """gz_lev rising linearly eastwards on level-independent Exner faces of
+    each column gives pgf_lon = dgz/dx (the slope over de_lon: pgf is the
+    gradient, as pgf_swm's), pgf_lat = 0; likewise northwards for pgf_lat."""
+    m, hyai, hybi, o = setup
+    ph_lev, exn, _ = o.calc_ph(np.full((1, NLAT, NLON), 1.0e5))
+    base = np.linspace(3.0e5, 0.0, NLEV + 1)[:, None, None]
+    slope = 7.0
+    i = np.arange(NLON, dtype=float)[None, None, :]
+    gz = base + slope * i + 0 * np.zeros((NLEV + 1, NLAT, NLON))
+    pl, pt = o.pgf_lin97(exn, gz)
+    de_lon = m[MESH["de_lon"]]
+    np.testing.assert_allclose(pl[:, 1:-1, :-1], slope / de_lon[None, 1:-1, None] + 0 * pl[:, 1:-1, :-1], rtol=1e-9)
+    np.testing.assert_allclose(pt, 0.0, atol=1e-12)
+    r = np.arange(NLAT, dtype=float)[None, :, None]
+    pl2, pt2 = o.pgf_lin97(exn, base + slope * r + 0 * gz)
+    de_lat = m[MESH["de_lat"], :NLAT - 1]
+    np.testing.assert_allclose(pt2, slope / de_lat[None, :, None] + 0 * pt2, rtol=1e-9)
+    np.testing.assert_allclose(pl2[:, 1:-1], 0.0, atol=1e-12)
