@@ -1,7 +1,7 @@
 """Hydrostatic column operators of the baroclinic dynamics on the GPU
 (include/gmcore_baro.h): the Python mirror of operators_mod's calc_ph,
-calc_t, calc_gz_lev, calc_dphsdt, calc_we_lev, calc_wedudlev_wedvdlev and
-pgf_lin97_run (hydrostatic branch).
+calc_t, calc_gz_lev, calc_dphsdt, calc_we_lev, calc_wedudlev_wedvdlev, the
+vertical part of calc_grad_ptf and pgf_lin97_run (hydrostatic branch).
 
 Fields are numpy arrays [levels, rows, nlon]: levels nlev (full), nlev + 1
 (half) or 1; rows nlat, or nlat - 1 for V-point fields (capi.BARO_FIELDS).
@@ -87,6 +87,10 @@ class Baroclinic:
 
     def pgf_lin97(self):
         self._check(self.lib.gmc_baro_pgf_lin97(self.h), "pgf_lin97")
+
+    def calc_ptf_lev(self, limiter: str = "mono"):
+        """The vertical FFSL flux of pt (calc_grad_ptf's vertical part)."""
+        self._check(self.lib.gmc_baro_calc_ptf_lev(self.h, capi.LIMITERS[limiter]), "calc_ptf_lev")
 
     def synchronize(self):
         self._check(self.lib.gmc_baro_synchronize(self.h), "synchronize")

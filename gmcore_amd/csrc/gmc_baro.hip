@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "gmc_internal.hpp"
+#include "gmc_ppm.hpp"
 #include "../../include/gmcore_baro.h"
 
 namespace gmc {
@@ -40,7 +41,8 @@ constexpr FieldShape kShape[GMB_NFIELDS] = {
     {1, false}, {1, false},                              // dmfdlon, dmfdlat
     {2, false}, {2, false}, {2, true},                   // we_lev, we_lev_lon, we_lev_lat
     {1, false}, {1, true}, {1, false}, {1, true},        // u_f, v_f, m_lon, m_lat
-    {1, false}, {1, true}, {1, false}, {1, true}};       // wedudlev, wedvdlev, pgf_lon, pgf_lat
+    {1, false}, {1, true}, {1, false}, {1, true},        // wedudlev, wedvdlev, pgf_lon, pgf_lat
+    {2, false}, {2, false}, {1, false}};                 // cflz, ptf_lev, dptfdlev
 
 // calc_ph (operators_mod.F90:136-168): the half-level pressures of the
 // hybrid coordinate (hybrid_coord_mod.F90:155-162), their Exner powers, and
@@ -222,6 +224,65 @@ __global__ void __launch_bounds__(NB) k_pgf_lin97(BaroGrid g, Mesh M, const doub
   }
 }
 
+// the vertical part of calc_grad_ptf (operators_mod.F90:1083-1101) with
+// vflx_ppm_cell (ffsl_mod.F90:883-941).  Work arrays of the column: ext, pt
+// on levels -2 .. nlev+1 (ext level e = k + 2, stride sc, nlev + 4 levels);
+// ml, dm, m6, the PPM coefficients of the column's cells.  flag[0] is set
+// where a Courant number reaches outside the column (the index is then
+// clamped so that every access stays in the arrays).
+__global__ void __launch_bounds__(NB) k_vflx_pt(BaroGrid g, int lim, const double *__restrict__ pt,
+                                                const double *__restrict__ we, const double *__restrict__ cflz,
+                                                double *ext, double *ml, double *dm, double *m6, double *ptf,
+                                                double *dptf, int *flag) {
+  const int i = blockIdx.x * NB + threadIdx.x, r = blockIdx.y;
+  if (i >= g.nlon) return;
+  const int64_t o = r * g.pitch + i, s = g.sc;
+  const int n = g.nlev;
+  for (int k = 0; k < n; k++) ext[(k + 2) * s + o] = pt[k * s + o];
+  ext[1 * s + o] = 2 * ext[2 * s + o] - ext[3 * s + o];
+  ext[o] = 2 * ext[1 * s + o] - ext[2 * s + o];
+  ext[(n + 2) * s + o] = 2 * ext[(n + 1) * s + o] - ext[n * s + o];
+  ext[(n + 3) * s + o] = 2 * ext[(n + 2) * s + o] - ext[(n + 1) * s + o];
+  for (int k = 0; k < n; k++) {
+    const double *e = ext + (k + 2) * s + o;
+    const Ppm p = ppm(lim, e[-2 * s], e[-s], e[0], e[s], e[2 * s]);
+    ml[k * s + o] = p.fl;
+    dm[k * s + o] = p.df;
+    m6[k * s + o] = p.f6;
+  }
+  bool bad = false;
+  for (int k = 1; k < n; k++) {   // interior half levels
+    const double c = cflz[k * s + o];
+    int ci = (int)c;
+    const double cf = c - ci;
+    ci = ci > n + 4 ? n + 4 : (ci < -n - 4 ? -n - 4 : ci);   // beyond the column either way: flagged below
+    double f = 0.0;
+    if (c != 0) {
+      const bool down = c > 0;
+      // c > 0: upwind cell k - ci - 1, whole cells k - ci .. k - 1;
+      // c < 0: upwind cell k - ci, whole cells k .. k - ci - 1
+      int ku = down ? k - ci - 1 : k - ci;
+      int j0 = down ? k - ci : k, j1 = down ? k - 1 : k - ci - 1;
+      if (ku < 0 || ku > n - 1 || j0 < -2 || j1 > n + 1) {
+        bad = true;
+        ku = ku < 0 ? 0 : (ku > n - 1 ? n - 1 : ku);
+        j0 = j0 < -2 ? -2 : j0;
+        j1 = j1 > n + 1 ? n + 1 : j1;
+      }
+      double sum = 0.0;
+      for (int j = j0; j <= j1; j++) sum = sum + ext[(j + 2) * s + o];
+      const double s1 = down ? 1 - cf : 0.0, s2 = down ? 1.0 : -cf;
+      const double ds1 = s2 - s1, ds2 = s2 * s2 - s1 * s1, ds3 = s2 * s2 * s2 - s1 * s1 * s1;
+      const double w = we[k * s + o];
+      f = (down ? w : -w) *
+          (sum + ml[ku * s + o] * ds1 + 0.5 * dm[ku * s + o] * ds2 + m6[ku * s + o] * (ds2 / 2.0 - ds3 / 3.0)) / c;
+    }
+    ptf[k * s + o] = f;
+  }
+  for (int k = 0; k < n; k++) dptf[k * s + o] = ptf[(k + 1) * s + o] - ptf[k * s + o];
+  if (bad) flag[0] = 1;
+}
+
 }  // namespace
 }  // namespace gmc
 
@@ -234,6 +295,8 @@ struct gmc_baro {
   double *slab = nullptr;
   double *f[GMB_NFIELDS] = {};
   double *d_mesh = nullptr, *d_hyai = nullptr, *d_hybi = nullptr;
+  double *w_ext = nullptr, *w_ml = nullptr, *w_dm = nullptr, *w_m6 = nullptr;   // calc_ptf_lev work
+  int *d_flag = nullptr;
   hipStream_t stream = nullptr;
   HostStage stage;
   std::string err;
@@ -293,6 +356,7 @@ int gmc_baro_create(const gmc_mesh *mesh, const gmc_vcoord *vc, const gmc_thermo
   size_t n = 0;
   for (int f = 0; f < GMB_NFIELDS; f++) n += (size_t)levels_of(b, f) * rows_of(b, f) * g.pitch;
   n += GMC_MESH_NFIELDS * (size_t)nlat + 2 * (size_t)(nlev + 1);
+  n += (size_t)(nlev + 4 + 3 * nlev) * g.sc + 4;   // column work of calc_ptf_lev, the flag
   int rc = bhip(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream");
   if (!rc) rc = bhip(b, hipMalloc((void **)&b->slab, n * sizeof(double)), "alloc");
   if (!rc) rc = bhip(b, hipMemset(b->slab, 0, n * sizeof(double)), "memset");
@@ -309,6 +373,16 @@ int gmc_baro_create(const gmc_mesh *mesh, const gmc_vcoord *vc, const gmc_thermo
   p += GMC_MESH_NFIELDS * (size_t)nlat;
   b->d_hyai = p;
   b->d_hybi = p + nlev + 1;
+  p += 2 * (size_t)(nlev + 1);
+  b->w_ext = p;
+  p += (size_t)(nlev + 4) * g.sc;
+  b->w_ml = p;
+  p += (size_t)nlev * g.sc;
+  b->w_dm = p;
+  p += (size_t)nlev * g.sc;
+  b->w_m6 = p;
+  p += (size_t)nlev * g.sc;
+  b->d_flag = (int *)p;
   rc = bhip(b, hipMemcpy(b->d_mesh, mesh->metrics, sizeof(double) * GMC_MESH_NFIELDS * nlat, hipMemcpyHostToDevice),
             "mesh");
   if (!rc) rc = bhip(b, hipMemcpy(b->d_hyai, vc->hyai, sizeof(double) * (nlev + 1), hipMemcpyHostToDevice), "hyai");
@@ -406,6 +480,23 @@ int gmc_baro_pgf_lin97(gmc_baro *b) {
   hipLaunchKernelGGL(k_pgf_lin97, columns(g, 2 * g.nlat - 3), dim3(NB), 0, b->stream, g, Mesh{b->d_mesh, g.nlat},
                      b->f[GMB_PH_EXN_LEV], b->f[GMB_GZ_LEV], b->f[GMB_PGF_LON], b->f[GMB_PGF_LAT]);
   return launched(b, "pgf_lin97");
+}
+
+int gmc_baro_calc_ptf_lev(gmc_baro *b, int32_t limiter) {
+  if (!b) return GMC_EINVAL;
+  if (limiter < GMC_LIMITER_MONO || limiter > GMC_LIMITER_NONE) return berr(b, GMC_EINVAL, "bad limiter");
+  const BaroGrid &g = b->g;
+  int rc = bhip(b, hipMemsetAsync(b->d_flag, 0, sizeof(int), b->stream), "flag clear");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_vflx_pt, columns(g, g.nlat), dim3(NB), 0, b->stream, g, (int)limiter, b->f[GMB_PT],
+                     b->f[GMB_WE_LEV], b->f[GMB_CFLZ], b->w_ext, b->w_ml, b->w_dm, b->w_m6, b->f[GMB_PTF_LEV],
+                     b->f[GMB_DPTFDLEV], b->d_flag);
+  rc = launched(b, "calc_ptf_lev");
+  int h = 0;
+  if (!rc) rc = bhip(b, hipMemcpyAsync(&h, b->d_flag, sizeof(int), hipMemcpyDeviceToHost, b->stream), "flag");
+  if (!rc) rc = bhip(b, hipStreamSynchronize(b->stream), "sync");
+  if (!rc && h) rc = berr(b, GMC_ENUMERIC, "cflz reaches outside the column");
+  return rc;
 }
 
 int gmc_baro_synchronize(gmc_baro *b) {

@@ -75,6 +75,9 @@ enum gmc_baro_field {
   GMB_WEDVDLEV,      /* tend%wedvdlev     V points, full levels          */
   GMB_PGF_LON,       /* tend%pgf_lon      U points, full levels          */
   GMB_PGF_LAT,       /* tend%pgf_lat      V points, full levels          */
+  GMB_CFLZ,          /* batch%cflz        cells, half levels             */
+  GMB_PTF_LEV,       /* state%ptf_lev     cells, half levels             */
+  GMB_DPTFDLEV,      /* tend%dptfdlev     cells, full levels             */
   GMB_NFIELDS
 };
 
@@ -118,6 +121,17 @@ int gmc_baro_calc_wedudlev_wedvdlev(gmc_baro *b);
  * the U points off the pole rows, pgf_lat on every V point, from ph_exn_lev
  * and gz_lev. */
 int gmc_baro_pgf_lin97(gmc_baro *b);
+
+/* The vertical part of calc_grad_ptf (operators_mod.F90:1084-1098): pt's
+ * two levels beyond the top and the bottom extrapolated linearly, the
+ * vertical FFSL flux of pt (vflx_ppm_cell, ffsl_mod.F90:883-941, PPM with
+ * the limiter GMC_LIMITER_* of include/gmcore_adv.h) through the interior
+ * half levels with the Courant numbers cflz and the mass flux we_lev (the
+ * advection batch's we), into ptf_lev, and dptfdlev = ptf_lev(k+1) -
+ * ptf_lev(k).  ptf_lev's top and bottom faces are not written.  A Courant
+ * number whose upwind cell falls outside the column gives GMC_ENUMERIC (the
+ * reference reads beyond its arrays there); the call waits for the result. */
+int gmc_baro_calc_ptf_lev(gmc_baro *b, int32_t limiter);
 
 /* Wait for the block's queued work; its stream (hipStream_t). */
 int gmc_baro_synchronize(gmc_baro *b);

@@ -376,6 +376,7 @@ class BaroOracle:
         L.gmo_baro_calc_we_lev.argtypes = [i, i, i, P, P, P, P, P, P, P, P]
         L.gmo_baro_calc_wedudlev_wedvdlev.argtypes = [i, i, i] + [P] * 8
         L.gmo_baro_pgf_lin97.argtypes = [i, i, i, P, P, P, P, P]
+        L.gmo_baro_calc_ptf_lev.argtypes = [i, i, i, i, P, P, P, P, P]
 
     def _a(self, x, levels, vrows=False):
         shape = (levels, self.nlat - 1 if vrows else self.nlat, self.nlon)
@@ -438,3 +439,12 @@ class BaroOracle:
         pl, pt = self._z(n), self._z(n, True)
         self.L.gmo_baro_pgf_lin97(self.nlon, self.nlat, n, dptr(self.m), dptr(e), dptr(z), dptr(pl), dptr(pt))
         return pl, pt
+
+    def calc_ptf_lev(self, pt, we_lev, cflz, limiter: int = 0):
+        """ptf_lev, dptfdlev and the count of faces reaching outside the column."""
+        n = self.nlev
+        pt, we, c = self._a(pt, n), self._a(we_lev, n + 1), self._a(cflz, n + 1)
+        ptf, dptf = self._z(n + 1), self._z(n)
+        bad = self.L.gmo_baro_calc_ptf_lev(self.nlon, self.nlat, n, limiter, dptr(pt), dptr(we), dptr(c),
+                                           dptr(ptf), dptr(dptf))
+        return ptf, dptf, bad

@@ -174,3 +174,56 @@ void gmo_baro_equal_eta(int nlev, double *hyai, double *hybi) {
     hyai[k] = eta - hybi[k];
   }
 }
+
+/* The vertical part of calc_grad_ptf (operators_mod.F90:1084-1098): the two
+ * levels beyond the top and the bottom extrapolated linearly, then
+ * vflx_ppm_cell (ffsl_mod.F90:883-941) through the interior half levels and
+ * dptfdlev.  ptf_lev's top and bottom faces are left as given.  Returns the
+ * number of faces whose upwind cell or whole cells fall outside the column
+ * (their flux is left as given). */
+int gmo_baro_calc_ptf_lev(int nlon, int nlat, int nlev, int limiter, const double *pt, const double *we,
+                          const double *cflz, double *ptf_lev, double *dptfdlev) {
+  const size_t L = (size_t)nlat * nlon;
+  const int ne = nlev + 4;
+  double ext[ne], ml[nlev], dm[nlev], m6[nlev];
+  int bad = 0;
+  for (size_t c = 0; c < L; c++) {
+    for (int k = 0; k < nlev; k++) ext[k + 2] = pt[k * L + c];
+    for (int k = -1; k >= -2; k--) ext[k + 2] = 2 * ext[k + 3] - ext[k + 4];
+    for (int k = nlev; k <= nlev + 1; k++) ext[k + 2] = 2 * ext[k + 1] - ext[k];
+    for (int k = 0; k < nlev; k++) {
+      const double f[5] = {ext[k], ext[k + 1], ext[k + 2], ext[k + 3], ext[k + 4]};
+      double out[3];
+      gmo_ppm(limiter, f, out);
+      ml[k] = out[0];
+      dm[k] = out[1];
+      m6[k] = out[2];
+    }
+    for (int k = 1; k < nlev; k++) {
+      const double cf_all = cflz[k * L + c];
+      const int ci = (int)cf_all;
+      const double cf = cf_all - ci;
+      if (cf_all > 0) {
+        const int ku = k - ci - 1;
+        if (ku < 0 || ku > nlev - 1 || k - ci < -2) { bad++; continue; }
+        double sum = 0;
+        for (int j = k - ci; j <= k - 1; j++) sum = sum + ext[j + 2];
+        const double s1 = 1 - cf, s2 = 1;
+        const double ds1 = s2 - s1, ds2 = s2 * s2 - s1 * s1, ds3 = s2 * s2 * s2 - s1 * s1 * s1;
+        ptf_lev[k * L + c] = we[k * L + c] * (sum + ml[ku] * ds1 + 0.5 * dm[ku] * ds2 + m6[ku] * (ds2 / 2.0 - ds3 / 3.0)) / cf_all;
+      } else if (cf_all < 0) {
+        const int ku = k - ci;
+        if (ku < 0 || ku > nlev - 1 || k - ci - 1 > nlev + 1) { bad++; continue; }
+        double sum = 0;
+        for (int j = k; j <= k - ci - 1; j++) sum = sum + ext[j + 2];
+        const double s1 = 0, s2 = -cf;
+        const double ds1 = s2 - s1, ds2 = s2 * s2 - s1 * s1, ds3 = s2 * s2 * s2 - s1 * s1 * s1;
+        ptf_lev[k * L + c] = -we[k * L + c] * (sum + ml[ku] * ds1 + 0.5 * dm[ku] * ds2 + m6[ku] * (ds2 / 2.0 - ds3 / 3.0)) / cf_all;
+      } else {
+        ptf_lev[k * L + c] = 0;
+      }
+    }
+    for (int k = 0; k < nlev; k++) dptfdlev[k * L + c] = ptf_lev[(k + 1) * L + c] - ptf_lev[k * L + c];
+  }
+  return bad;
+}

@@ -132,6 +132,11 @@ void gmo_baro_calc_wedudlev_wedvdlev(int nlon, int nlat, int nlev, const double 
                                      const double *we_lev_lat, double *wedudlev, double *wedvdlev);
 void gmo_baro_pgf_lin97(int nlon, int nlat, int nlev, const double *metrics, const double *ph_exn_lev,
                         const double *gz_lev, double *pgf_lon, double *pgf_lat);
+/* the vertical part of calc_grad_ptf: ptf_lev (interior half levels) and
+ * dptfdlev; returns the number of faces whose Courant number reaches
+ * outside the column */
+int gmo_baro_calc_ptf_lev(int nlon, int nlat, int nlev, int limiter, const double *pt, const double *we,
+                          const double *cflz, double *ptf_lev, double *dptfdlev);
 /* hybrid_coord_equal_eta (hybrid_coord_test_mod.F90:1330-1352) */
 void gmo_baro_equal_eta(int nlev, double *hyai, double *hybi);
 

@@ -169,3 +169,39 @@ def test_rejects_bad_arguments(env):
     g = gpu(env)
     with pytest.raises(Exception):
         g.set("pt", np.zeros((NLEV - 1, NLAT, NLON)))
+
+
+def courant_numbers(seed):
+    r = rng(seed)
+    c = r.uniform(-0.95, 0.95, (NLEV + 1, NLAT, NLON))
+    c[3:NLEV - 2] = r.uniform(-2.9, 2.9, (NLEV - 5, NLAT, NLON))
+    return c
+
+
+@pytest.mark.parametrize("limiter", ["mono", "pd", "none"])
+def test_calc_ptf_lev_bitwise(env, limiter):
+    o = env[3]
+    g = gpu(env)
+    _, _, _, _, theta, _ = column_state(o, 8)
+    w = rng(9).standard_normal((NLEV + 1, NLAT, NLON))
+    c = courant_numbers(10)
+    for name, a in (("pt", theta), ("we_lev", w), ("cflz", c)):
+        g.set(name, a)
+    g.calc_ptf_lev(limiter)
+    ptf, dptf, bad = o.calc_ptf_lev(theta, w, c, {"mono": 0, "pd": 1, "none": 2}[limiter])
+    assert bad == 0
+    np.testing.assert_array_equal(g.get("ptf_lev"), ptf)
+    np.testing.assert_array_equal(g.get("dptfdlev"), dptf)
+
+
+def test_calc_ptf_lev_reports_courant_beyond_column(env):
+    """An upwind cell above the model top is refused (GMC_ENUMERIC) with every
+    access kept inside the arrays."""
+    g = gpu(env)
+    c = np.zeros((NLEV + 1, NLAT, NLON))
+    c[1, 5, 7] = 1.5
+    g.set("pt", np.full((NLEV, NLAT, NLON), 300.0))
+    g.set("we_lev", np.ones_like(c))
+    g.set("cflz", c)
+    with pytest.raises(Exception, match="outside the column"):
+        g.calc_ptf_lev()

@@ -184,3 +184,52 @@ def test_pgf_lin97_linear_geopotential(setup):
     de_lat = m[MESH["de_lat"], :NLAT - 1]
     np.testing.assert_allclose(pt2, slope / de_lat[None, :, None] + 0 * pt2, rtol=1e-9)
     np.testing.assert_allclose(pl2[:, 1:-1], 0.0, atol=1e-12)
+
+
+def courant_numbers(seed, nlev=NLEV):
+    """|cflz| < 0.95, and up to 2.9 on the levels where the upwind cell and
+    the whole cells crossed stay in the column (3 .. nlev-3)."""
+    r = np.random.default_rng(seed)
+    c = r.uniform(-0.95, 0.95, (nlev + 1, NLAT, NLON))
+    c[3:nlev - 2] = r.uniform(-2.9, 2.9, (nlev - 5, NLAT, NLON))
+    return c
+
+
+@pytest.mark.parametrize("limiter", [0, 1, 2])
+def test_vertical_flux_of_uniform_column(setup, limiter):
+    """A uniform pt is carried unchanged: ptf_lev = we pt on every interior
+    face, whatever the Courant number; the top and bottom faces are left."""
+    m, hyai, hybi, o = setup
+    w = np.random.default_rng(7).standard_normal((NLEV + 1, NLAT, NLON))
+    ptf, dptf, bad = o.calc_ptf_lev(np.full((NLEV, NLAT, NLON), 300.0), w, courant_numbers(8), limiter)
+    assert bad == 0
+    np.testing.assert_allclose(ptf[1:-1], 300.0 * w[1:-1], rtol=1e-13)
+    assert np.all(ptf[0] == 0) and np.all(ptf[-1] == 0)
+    np.testing.assert_array_equal(dptf, ptf[1:] - ptf[:-1])
+
+
+def test_vertical_flux_of_linear_column(setup):
+    """pt = a + b j on the cells j (the extrapolated levels continue the
+    line, PPM reproduces it): the flux through face k is we times the mean
+    of pt over the swept segment, a + b (k - c / 2 - 1/2), for either sign
+    of c and with whole cells crossed."""
+    m, hyai, hybi, o = setup
+    a, b = 250.0, 3.0
+    j = np.arange(NLEV, dtype=float)[:, None, None]
+    pt = np.broadcast_to(a + b * j, (NLEV, NLAT, NLON))
+    c = courant_numbers(9)
+    w = 1.0 + np.random.default_rng(10).random((NLEV + 1, NLAT, NLON))
+    ptf, _, bad = o.calc_ptf_lev(pt, w, c, 0)
+    assert bad == 0
+    k = np.arange(NLEV + 1, dtype=float)[:, None, None]
+    expect = w * (a + b * (k - c / 2 - 0.5))
+    np.testing.assert_allclose(ptf[1:-1], expect[1:-1], rtol=1e-12)
+
+
+def test_vertical_flux_reports_out_of_column(setup):
+    """A Courant number whose upwind cell lies above the model top."""
+    m, hyai, hybi, o = setup
+    c = np.zeros((NLEV + 1, NLAT, NLON))
+    c[1, 0, 0] = 1.5
+    _, _, bad = o.calc_ptf_lev(np.full((NLEV, NLAT, NLON), 300.0), np.ones_like(c), c, 0)
+    assert bad == 1
