@@ -1,7 +1,8 @@
 """Hydrostatic column operators of the baroclinic dynamics on the GPU
 (include/gmcore_baro.h): the Python mirror of operators_mod's calc_ph,
 calc_t, calc_gz_lev, calc_dphsdt, calc_we_lev, calc_wedudlev_wedvdlev, the
-vertical part of calc_grad_ptf and pgf_lin97_run (hydrostatic branch).
+vertical part of calc_grad_ptf, pgf_lin97_run (hydrostatic branch) and the
+small_filter2 step of update_state.
 
 Fields are numpy arrays [levels, rows, nlon]: levels nlev (full), nlev + 1
 (half) or 1; rows nlat, or nlat - 1 for V-point fields (capi.BARO_FIELDS).
@@ -91,6 +92,12 @@ class Baroclinic:
     def calc_ptf_lev(self, limiter: str = "mono"):
         """The vertical FFSL flux of pt (calc_grad_ptf's vertical part)."""
         self._check(self.lib.gmc_baro_calc_ptf_lev(self.h, capi.LIMITERS[limiter]), "calc_ptf_lev")
+
+    def small_filter2(self, table):
+        """update_state's small_filter2 step on u_lon and v_lat; table: a
+        host.Filter of kind 2."""
+        f = table.struct()
+        self._check(self.lib.gmc_baro_small_filter2(self.h, C.byref(f)), "small_filter2")
 
     def synchronize(self):
         self._check(self.lib.gmc_baro_synchronize(self.h), "synchronize")

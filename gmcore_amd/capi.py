@@ -108,7 +108,7 @@ BARO_FIELDS = {
     "dmfdlon": (1, False), "dmfdlat": (1, False), "we_lev": (2, False), "we_lev_lon": (2, False),
     "we_lev_lat": (2, True), "u_f": (1, False), "v_f": (1, True), "m_lon": (1, False), "m_lat": (1, True),
     "wedudlev": (1, False), "wedvdlev": (1, True), "pgf_lon": (1, False), "pgf_lat": (1, True),
-    "cflz": (2, False), "ptf_lev": (2, False), "dptfdlev": (1, False),
+    "cflz": (2, False), "ptf_lev": (2, False), "dptfdlev": (1, False), "u_lon": (1, False), "v_lat": (1, True),
 }
 BARO_FIELD = {name: k for k, name in enumerate(BARO_FIELDS)}
 
@@ -215,6 +215,7 @@ def load_hip() -> C.CDLL:
         "gmc_baro_calc_wedudlev_wedvdlev": (C.c_int, [vp]),
         "gmc_baro_pgf_lin97": (C.c_int, [vp]),
         "gmc_baro_calc_ptf_lev": (C.c_int, [vp, C.c_int32]),
+        "gmc_baro_small_filter2": (C.c_int, [vp, P(GmcFilter)]),
         "gmc_baro_synchronize": (C.c_int, [vp]),
         "gmc_baro_stream": (C.c_int, [vp, P(vp)]),
         # batched ensembles, include/gmcore_ens.h

@@ -9,12 +9,14 @@
 // nlat - 1 rows per level.  The arithmetic is the reference's, operation by
 // operation and in its order (IEEE division, no contraction); pow and log are
 // the device library's, within a few ulp of the host's.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "gmc_internal.hpp"
+#include "gmc_point.hpp"
 #include "gmc_ppm.hpp"
 #include "../../include/gmcore_baro.h"
 
@@ -42,7 +44,8 @@ constexpr FieldShape kShape[GMB_NFIELDS] = {
     {2, false}, {2, false}, {2, true},                   // we_lev, we_lev_lon, we_lev_lat
     {1, false}, {1, true}, {1, false}, {1, true},        // u_f, v_f, m_lon, m_lat
     {1, false}, {1, true}, {1, false}, {1, true},        // wedudlev, wedvdlev, pgf_lon, pgf_lat
-    {2, false}, {2, false}, {1, false}};                 // cflz, ptf_lev, dptfdlev
+    {2, false}, {2, false}, {1, false},                  // cflz, ptf_lev, dptfdlev
+    {1, false}, {1, true}};                              // u_lon, v_lat
 
 // calc_ph (operators_mod.F90:136-168): the half-level pressures of the
 // hybrid coordinate (hybrid_coord_mod.F90:155-162), their Exner powers, and
@@ -283,6 +286,29 @@ __global__ void __launch_bounds__(NB) k_vflx_pt(BaroGrid g, int lim, const doubl
   if (bad) flag[0] = 1;
 }
 
+// filter_on_lon_edge / filter_on_lat_edge (filter_mod.F90:93-163) in place:
+// block (row, level) stages its row and the halo of its taps in LDS, then
+// writes the filtered row (filter_point: products and additions in tap order)
+__global__ void __launch_bounds__(NB) k_filter_levels(BaroGrid g, int rows, int64_t lstride,
+                                                      const int32_t *__restrict__ ngrid,
+                                                      const double *__restrict__ wgt, int max_taps, double *x) {
+  extern __shared__ double row[];
+  const int r = blockIdx.x, k = blockIdx.y;
+  if (r >= rows) return;
+  const int n = ngrid[r];
+  if (n <= 1) return;
+  const int hn = (n - 1) / 2, len = g.nlon + 2 * hn;
+  double *xr = x + k * lstride + r * g.pitch;
+  for (int t = threadIdx.x; t < len; t += NB) {
+    int c = (t - hn) % g.nlon;
+    if (c < 0) c += g.nlon;
+    row[t] = xr[c];
+  }
+  __syncthreads();
+  const double *w = wgt + (int64_t)r * max_taps;
+  for (int i = threadIdx.x; i < g.nlon; i += NB) xr[i] = filter_point(w, n, row + i);
+}
+
 }  // namespace
 }  // namespace gmc
 
@@ -297,6 +323,8 @@ struct gmc_baro {
   double *d_mesh = nullptr, *d_hyai = nullptr, *d_hybi = nullptr;
   double *w_ext = nullptr, *w_ml = nullptr, *w_dm = nullptr, *w_m6 = nullptr;   // calc_ptf_lev work
   int *d_flag = nullptr;
+  void *d_filter = nullptr;          // small_filter2's table on the device
+  size_t filter_bytes = 0;
   hipStream_t stream = nullptr;
   HostStage stage;
   std::string err;
@@ -399,6 +427,7 @@ int gmc_baro_destroy(gmc_baro *b) {
   if (!b) return GMC_EINVAL;
   if (b->stream) hip_note(hipStreamSynchronize(b->stream), "baro sync");
   if (b->slab) hip_note(hipFree(b->slab), "baro free");
+  if (b->d_filter) hip_note(hipFree(b->d_filter), "baro filter free");
   stage_free(b->stage);
   if (b->stream) hip_note(hipStreamDestroy(b->stream), "baro stream");
   delete b;
@@ -497,6 +526,53 @@ int gmc_baro_calc_ptf_lev(gmc_baro *b, int32_t limiter) {
   if (!rc) rc = bhip(b, hipStreamSynchronize(b->stream), "sync");
   if (!rc && h) rc = berr(b, GMC_ENUMERIC, "cflz reaches outside the column");
   return rc;
+}
+
+int gmc_baro_small_filter2(gmc_baro *b, const gmc_filter *f) {
+  if (!b) return GMC_EINVAL;
+  const BaroGrid &g = b->g;
+  const int nlat = g.nlat, nhalf = nlat - 1;
+  if (!f || !f->ngrid_lon || !f->ngrid_lat || !f->wgt_lon || !f->wgt_lat || f->max_taps < 1)
+    return berr(b, GMC_EINVAL, "bad filter table");
+  int hmax = 0;
+  for (int r = 0; r < nlat; r++) {
+    if (f->ngrid_lon[r] > g.nlon || f->ngrid_lon[r] > f->max_taps) return berr(b, GMC_EINVAL, "bad filter taps");
+    hmax = std::max(hmax, (f->ngrid_lon[r] - 1) / 2);
+  }
+  for (int r = 0; r < nhalf; r++) {
+    if (f->ngrid_lat[r] > g.nlon || f->ngrid_lat[r] > f->max_taps) return berr(b, GMC_EINVAL, "bad filter taps");
+    hmax = std::max(hmax, (f->ngrid_lat[r] - 1) / 2);
+  }
+  const size_t ni = sizeof(int32_t) * (nlat + nhalf), nw = sizeof(double) * (size_t)f->max_taps * (nlat + nhalf);
+  const size_t need = (ni + 255) / 256 * 256 + nw;
+  int rc = GMC_OK;
+  if (need > b->filter_bytes) {
+    if (b->d_filter) {
+      rc = bhip(b, hipStreamSynchronize(b->stream), "sync");
+      if (!rc) rc = bhip(b, hipFree(b->d_filter), "filter free");
+      b->d_filter = nullptr;
+      b->filter_bytes = 0;
+    }
+    if (!rc) rc = bhip(b, hipMalloc(&b->d_filter, need), "filter alloc");
+    if (rc) return rc;
+    b->filter_bytes = need;
+  }
+  int32_t *ng = (int32_t *)b->d_filter;
+  double *w = (double *)((char *)b->d_filter + (ni + 255) / 256 * 256);
+  const size_t wl = sizeof(double) * (size_t)f->max_taps * nlat;
+  rc = bhip(b, hipMemcpy(ng, f->ngrid_lon, sizeof(int32_t) * nlat, hipMemcpyHostToDevice), "filter copy");
+  if (!rc) rc = bhip(b, hipMemcpy(ng + nlat, f->ngrid_lat, sizeof(int32_t) * nhalf, hipMemcpyHostToDevice), "filter copy");
+  if (!rc) rc = bhip(b, hipMemcpy(w, f->wgt_lon, wl, hipMemcpyHostToDevice), "filter copy");
+  if (!rc) rc = bhip(b, hipMemcpy((char *)w + wl, f->wgt_lat, sizeof(double) * (size_t)f->max_taps * nhalf,
+                                  hipMemcpyHostToDevice), "filter copy");
+  if (rc) return rc;
+  const size_t lds = sizeof(double) * (g.nlon + 2 * (size_t)hmax);
+  if (lds > 65536) return berr(b, GMC_EINVAL, "filter row does not fit in LDS");
+  hipLaunchKernelGGL(k_filter_levels, dim3(nlat, g.nlev), dim3(NB), lds, b->stream, g, nlat, g.sc, ng, w,
+                     f->max_taps, b->f[GMB_U_LON]);
+  hipLaunchKernelGGL(k_filter_levels, dim3(nhalf, g.nlev), dim3(NB), lds, b->stream, g, nhalf, g.sv, ng + nlat,
+                     (const double *)((const char *)w + wl), f->max_taps, b->f[GMB_V_LAT]);
+  return launched(b, "small_filter2");
 }
 
 int gmc_baro_synchronize(gmc_baro *b) {

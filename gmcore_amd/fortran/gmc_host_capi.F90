@@ -36,16 +36,17 @@ contains
     integer(c_int), value :: nlat, kind
     real(c_double), intent(in) :: metrics(nlat, mesh_field_count)
     real(c_double), value :: dt
-    real(c_double), intent(in) :: coefs(6)
+    real(c_double), intent(in) :: coefs(*)   ! 6 values; kind 2 reads a 7th, filter_coef_e
     integer(c_int32_t), intent(out) :: ng_lon(nlat), ng_lat(nlat-1)
     integer(c_int32_t), intent(inout) :: maxn
     real(c_double), intent(out) :: wl(*), wt(*)
     real(r8) w_lon(nlat), w_lat(nlat-1)
     integer ngl(nlat), ngt(nlat-1)
     rc = 1
-    if (kind /= 0 .and. kind /= 1) return
+    if (kind < 0 .or. kind > 2) return
     max_wave_speed = coefs(1); max_cfl = coefs(2); filter_coef_a = coefs(3)
     filter_coef_b = coefs(4); filter_coef_c = coefs(5); filter_coef_d = coefs(6)
+    if (kind == 2) filter_coef_e = coefs(7)
     call filter_taps(nlat, metrics, dt, kind, ngl, ngt, w_lon, w_lat)
     ng_lon = ngl; ng_lat = ngt
     if (maxn <= 0) then

@@ -40,6 +40,7 @@ module gmc_kinds_mod
   real(r8) :: filter_coef_b       = 0.4
   real(r8) :: filter_coef_c       = 0.2
   real(r8) :: filter_coef_d       = 0.375
+  real(r8) :: filter_coef_e       = 0.16
   logical  :: use_div_damp        = .false.
   integer  :: div_damp_order      = 2
   real(r8) :: div_damp_coef2      = 1.0_r8 / 128.0_r8
@@ -53,7 +54,7 @@ module gmc_kinds_mod
 
   namelist /gmcore_control/ num_lon, num_lat, num_lev, dt_dyn, dt_adv, run_steps, &
     time_scheme, pv_scheme, pv_pole_stokes, upwind_order_pv, upwind_wgt_pv, coriolis_scheme, &
-    max_wave_speed, max_cfl, filter_coef_a, filter_coef_b, filter_coef_c, filter_coef_d, &
+    max_wave_speed, max_cfl, filter_coef_a, filter_coef_b, filter_coef_c, filter_coef_d, filter_coef_e, &
     use_div_damp, div_damp_order, div_damp_coef2, div_damp_coef4, tvd_limiter_type, initial_file, &
     lon_halo_width, limiter_type, ffsl_flux_type
 

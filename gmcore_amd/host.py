@@ -56,12 +56,13 @@ class Mesh:
 
 
 class Filter:
-    """filter_type tables (filter_types_mod.F90:49-147); kind 0 big, 1 small_filter1."""
+    """filter_type tables (filter_types_mod.F90:49-147); kind 0 big, 1 small_filter1,
+    2 small_filter2."""
 
     def __init__(self, mesh: Mesh, dt: float, nl: Namelist, kind: int):
         nlat = mesh.nlat
         coefs = np.array([nl.max_wave_speed, nl.max_cfl, nl.filter_coef_a, nl.filter_coef_b,
-                          nl.filter_coef_c, nl.filter_coef_d])
+                          nl.filter_coef_c, nl.filter_coef_d, nl.filter_coef_e])
         self.ngrid_lon = np.zeros(nlat, np.int32)
         self.ngrid_lat = np.zeros(nlat - 1, np.int32)
         mx = np.zeros(1, np.int32)

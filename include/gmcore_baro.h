@@ -78,6 +78,8 @@ enum gmc_baro_field {
   GMB_CFLZ,          /* batch%cflz        cells, half levels             */
   GMB_PTF_LEV,       /* state%ptf_lev     cells, half levels             */
   GMB_DPTFDLEV,      /* tend%dptfdlev     cells, full levels             */
+  GMB_U_LON,         /* state%u_lon       U points, full levels          */
+  GMB_V_LAT,         /* state%v_lat       V points, full levels          */
   GMB_NFIELDS
 };
 
@@ -132,6 +134,15 @@ int gmc_baro_pgf_lin97(gmc_baro *b);
  * number whose upwind cell falls outside the column gives GMC_ENUMERIC (the
  * reference reads beyond its arrays there); the call waits for the result. */
 int gmc_baro_calc_ptf_lev(gmc_baro *b, int32_t limiter);
+
+/* The baroclinic branch of update_state's small_filter2 step
+ * (time_schemes_mod.F90:213-221): filter_on_lon_edge / filter_on_lat_edge
+ * (filter_mod.F90:93-163) with the table `small_filter2` (filter_init's
+ * 'small_filter2', filter_types_mod.F90:124-143: the big filter's widths
+ * times exp_two_values(filter_coef_e, filter_coef_d, lat0, 80, |lat|)), in
+ * place on every level of u_lon and v_lat; rows with at most one tap are
+ * left as they are. */
+int gmc_baro_small_filter2(gmc_baro *b, const gmc_filter *small_filter2);
 
 /* Wait for the block's queued work; its stream (hipStream_t). */
 int gmc_baro_synchronize(gmc_baro *b);

@@ -16,7 +16,7 @@ import os
 
 import numpy as np
 
-from gmcore_amd.capi import (GMC_MESH_NFIELDS, FIELD, GmcAdvConfig, GmcConfig, GmcMesh, dptr,
+from gmcore_amd.capi import (GMC_MESH_NFIELDS, FIELD, GmcAdvConfig, GmcConfig, GmcFilter, GmcMesh, dptr,
                              field_rows, iptr)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -37,7 +37,7 @@ def lib(omp: bool = False) -> C.CDLL:
         L.gmo_mesh_init.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, P(C.c_double)]
         L.gmo_filter_init.argtypes = [C.c_int, C.c_int, P(C.c_double), C.c_double, C.c_double,
                                       C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
-                                      C.c_int, P(C.c_int32), P(C.c_int32), P(C.c_int32),
+                                      C.c_double, C.c_int, P(C.c_int32), P(C.c_int32), P(C.c_int32),
                                       P(C.c_double), P(C.c_double)]
         L.gmo_sw_create.restype = vp
         L.gmo_sw_create.argtypes = [P(GmcMesh), P(GmcConfig)]
@@ -92,12 +92,13 @@ def mesh_metrics(nlon: int, nlat: int, radius: float, omega: float) -> np.ndarra
 
 
 def filter_table(nlon, nlat, metrics, dt, nl, kind):
-    """(ngrid_lon[nlat], wgt_lon[nlat, max], ngrid_lat[nlat-1], wgt_lat[nlat-1, max])"""
+    """(ngrid_lon[nlat], wgt_lon[nlat, max], ngrid_lat[nlat-1], wgt_lat[nlat-1, max]);
+    kind 0 big_filter, 1 small_filter1, 2 small_filter2"""
     ng_lon = np.zeros(nlat, np.int32)
     ng_lat = np.zeros(nlat - 1, np.int32)
     mx = np.zeros(1, np.int32)
     args = (nlon, nlat, dptr(metrics), dt, nl.max_wave_speed, nl.max_cfl, nl.filter_coef_a,
-            nl.filter_coef_b, nl.filter_coef_c, nl.filter_coef_d, kind)
+            nl.filter_coef_b, nl.filter_coef_c, nl.filter_coef_d, nl.filter_coef_e, kind)
     lib().gmo_filter_init(*args, iptr(ng_lon), iptr(ng_lat), iptr(mx), None, None)
     w_lon = np.zeros((nlat, int(mx[0])))
     w_lat = np.zeros((nlat - 1, int(mx[0])))
@@ -377,6 +378,7 @@ class BaroOracle:
         L.gmo_baro_calc_wedudlev_wedvdlev.argtypes = [i, i, i] + [P] * 8
         L.gmo_baro_pgf_lin97.argtypes = [i, i, i, P, P, P, P, P]
         L.gmo_baro_calc_ptf_lev.argtypes = [i, i, i, i, P, P, P, P, P]
+        L.gmo_baro_small_filter2.argtypes = [i, i, i, C.POINTER(GmcFilter), P, P]
 
     def _a(self, x, levels, vrows=False):
         shape = (levels, self.nlat - 1 if vrows else self.nlat, self.nlon)
@@ -448,3 +450,12 @@ class BaroOracle:
         bad = self.L.gmo_baro_calc_ptf_lev(self.nlon, self.nlat, n, limiter, dptr(pt), dptr(we), dptr(c),
                                            dptr(ptf), dptr(dptf))
         return ptf, dptf, bad
+
+    def small_filter2(self, u_lon, v_lat, table):
+        """u_lon, v_lat after small_filter2; table = filter_table(..., kind=2)."""
+        n = self.nlev
+        u, v = self._a(u_lon, n).copy(), self._a(v_lat, n, True).copy()
+        ng_lon, w_lon, ng_lat, w_lat = (np.ascontiguousarray(a) for a in table)
+        f = GmcFilter(w_lon.shape[1], iptr(ng_lon), dptr(w_lon), iptr(ng_lat), dptr(w_lat))
+        self.L.gmo_baro_small_filter2(self.nlon, self.nlat, n, C.byref(f), dptr(u), dptr(v))
+        return u, v

@@ -227,3 +227,33 @@ int gmo_baro_calc_ptf_lev(int nlon, int nlat, int nlev, int limiter, const doubl
   }
   return bad;
 }
+
+/* update_state's small_filter2 step, baroclinic branch
+ * (time_schemes_mod.F90:213-221): filter_on_lon_edge / filter_on_lat_edge
+ * (filter_mod.F90:93-163) in place on every level, each output the sum of
+ * its taps in order */
+void gmo_baro_small_filter2(int nlon, int nlat, int nlev, const gmc_filter *f, double *u_lon, double *v_lat) {
+  const size_t L = (size_t)nlat * nlon, LV = (size_t)(nlat - 1) * nlon;
+  double tmp[nlon];
+  for (int pass = 0; pass < 2; pass++) {
+    const int rows = pass == 0 ? nlat : nlat - 1;
+    const int32_t *ng = pass == 0 ? f->ngrid_lon : f->ngrid_lat;
+    const double *wg = pass == 0 ? f->wgt_lon : f->wgt_lat;
+    double *x = pass == 0 ? u_lon : v_lat;
+    const size_t ls = pass == 0 ? L : LV;
+    for (int k = 0; k < nlev; k++)
+      for (int r = 0; r < rows; r++) {
+        const int n = ng[r];
+        if (n <= 1) continue;
+        const int hn = (n - 1) / 2;
+        const double *w = wg + (size_t)r * f->max_taps;
+        double *row = x + k * ls + (size_t)r * nlon;
+        for (int i = 0; i < nlon; i++) {
+          double acc = 0.0;
+          for (int t = 0; t < n; t++) acc = acc + w[t] * row[((i - hn + t) % nlon + nlon) % nlon];
+          tmp[i] = acc;
+        }
+        for (int i = 0; i < nlon; i++) row[i] = tmp[i];
+      }
+  }
+}

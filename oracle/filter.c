@@ -36,13 +36,19 @@ static void row_width(double dx, double dt, double lat_deg, double mws, double m
   }
 }
 
+/* exp_two_values (math_mod.F90:146-159) */
+static double exp_two_values(double val1, double val2, double x0, double x1, double x) {
+  const double w = exp((x - x0) * (x - x0) * log(1.0e-3) / ((x1 - x0) * (x1 - x0)));
+  return w * val1 + (1 - w) * val2;
+}
+
 int gmo_filter_init(int nlon, int nlat, const double *metrics, double dt,
                     double max_wave_speed, double max_cfl, double coef_a,
-                    double coef_b, double coef_c, double coef_d, int kind,
+                    double coef_b, double coef_c, double coef_d, double coef_e, int kind,
                     int32_t *ngrid_lon, int32_t *ngrid_lat, int32_t *max_taps,
                     double *wgt_lon, double *wgt_lat) {
   (void)nlon;
-  if (!metrics || !ngrid_lon || !ngrid_lat || !max_taps || (kind != 0 && kind != 1)) return GMC_EINVAL;
+  if (!metrics || !ngrid_lon || !ngrid_lat || !max_taps || kind < 0 || kind > 2) return GMC_EINVAL;
   const int nhalf = nlat - 1;
   double wlon[nlat], wlat[nlat];
   memset(wlon, 0, sizeof wlon); memset(wlat, 0, sizeof wlat);
@@ -78,6 +84,26 @@ int gmo_filter_init(int nlon, int nlat, const double *metrics, double dt,
         ngrid_lat[r] = n; wlat[r] = w * wlat[r];
       }
   }
+  if (kind == 2) {
+    /* small_filter2: the factor goes from coef_e next to the poles to about
+     * coef_d at 80 degrees and beyond (filter_types_mod.F90:124-143) */
+    double lat0 = -metrics[GMC_FULL_LAT_DEG * nlat + 1];
+    for (int r = 1; r < nlat - 1; r++)
+      if (ngrid_lon[r] > 1) {
+        const double w = exp_two_values(coef_e, coef_d, lat0, 80.0, fabs(metrics[GMC_FULL_LAT_DEG * nlat + r]));
+        int n = (int)(w * ngrid_lon[r]);
+        if (n % 2 == 0) n = n + 1;
+        ngrid_lon[r] = n; wlon[r] = w * wlon[r];
+      }
+    lat0 = -metrics[GMC_HALF_LAT_DEG * nlat + 0];
+    for (int r = 0; r < nhalf; r++)
+      if (ngrid_lat[r] > 1) {
+        const double w = exp_two_values(coef_e, coef_d, lat0, 80.0, fabs(metrics[GMC_HALF_LAT_DEG * nlat + r]));
+        int n = (int)(w * ngrid_lat[r]);
+        if (n % 2 == 0) n = n + 1;
+        ngrid_lat[r] = n; wlat[r] = w * wlat[r];
+      }
+  }
   int mx = 1;
   for (int r = 0; r < nlat; r++) if (ngrid_lon[r] > mx) mx = ngrid_lon[r];
   for (int r = 0; r < nhalf; r++) if (ngrid_lat[r] > mx) mx = ngrid_lat[r];
@@ -86,13 +112,13 @@ int gmo_filter_init(int nlon, int nlat, const double *metrics, double dt,
   const int ld = *max_taps;
   memset(wgt_lon, 0, sizeof(double) * nlat * ld);
   memset(wgt_lat, 0, sizeof(double) * nhalf * ld);
-  /* the big filter weighs rows with n > 1; small_filter1 every row it
-   * resized (filter_types_mod.F90:93-124) */
+  /* the big filter weighs rows with n > 1; the small filters every row they
+   * resized (filter_types_mod.F90:93-143) */
   for (int r = 1; r < nlat - 1; r++)
-    if (ngrid_lon[r] > 1 || (kind == 1 && ngrid_lon[r] == 1 && wlon[r] > 0))
+    if (ngrid_lon[r] > 1 || (kind >= 1 && ngrid_lon[r] == 1 && wlon[r] > 0))
       gaussian_weight(wlon[r], ngrid_lon[r], coef_b, &wgt_lon[r * ld]);
   for (int r = 0; r < nhalf; r++)
-    if (ngrid_lat[r] > 1 || (kind == 1 && ngrid_lat[r] == 1 && wlat[r] > 0))
+    if (ngrid_lat[r] > 1 || (kind >= 1 && ngrid_lat[r] == 1 && wlat[r] > 0))
       gaussian_weight(wlat[r], ngrid_lat[r], coef_b, &wgt_lat[r * ld]);
   return GMC_OK;
 }

@@ -29,12 +29,12 @@ extern "C" {
 int gmo_mesh_init(int nlon, int nlat, double radius, double omega, double *metrics);
 
 /* filter_init (src/dynamics/filter_types_mod.F90:49-147).  kind 0 =
- * big_filter, 1 = small_filter1.  First call with wgt == NULL returns the
- * tap counts and *max_taps; then call again with wgt buffers sized
- * nlat * max_taps and (nlat-1) * max_taps. */
+ * big_filter, 1 = small_filter1, 2 = small_filter2.  First call with
+ * wgt == NULL returns the tap counts and *max_taps; then call again with wgt
+ * buffers sized nlat * max_taps and (nlat-1) * max_taps. */
 int gmo_filter_init(int nlon, int nlat, const double *metrics, double dt,
                     double max_wave_speed, double max_cfl, double coef_a,
-                    double coef_b, double coef_c, double coef_d, int kind,
+                    double coef_b, double coef_c, double coef_d, double coef_e, int kind,
                     int32_t *ngrid_lon, int32_t *ngrid_lat, int32_t *max_taps,
                     double *wgt_lon, double *wgt_lat);
 
@@ -137,6 +137,8 @@ void gmo_baro_pgf_lin97(int nlon, int nlat, int nlev, const double *metrics, con
  * outside the column */
 int gmo_baro_calc_ptf_lev(int nlon, int nlat, int nlev, int limiter, const double *pt, const double *we,
                           const double *cflz, double *ptf_lev, double *dptfdlev);
+/* small_filter2 of update_state's baroclinic branch, in place on u_lon and v_lat */
+void gmo_baro_small_filter2(int nlon, int nlat, int nlev, const gmc_filter *f, double *u_lon, double *v_lat);
 /* hybrid_coord_equal_eta (hybrid_coord_test_mod.F90:1330-1352) */
 void gmo_baro_equal_eta(int nlev, double *hyai, double *hybi);
 

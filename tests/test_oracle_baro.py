@@ -233,3 +233,30 @@ def test_vertical_flux_reports_out_of_column(setup):
     c[1, 0, 0] = 1.5
     _, _, bad = o.calc_ptf_lev(np.full((NLEV, NLAT, NLON), 300.0), np.ones_like(c), c, 0)
     assert bad == 1
+
+
+def test_small_filter2_damps_zonal_modes_by_its_response(setup):
+    """small_filter2 on u_lon / v_lat rows: a constant stays (weights sum to 1)
+    and cos(m lambda) comes back scaled by the row's symmetric response
+    sum_t w_t cos(m (t - hn) dlambda); rows with one tap are untouched."""
+    from gmcore_amd.namelist import Namelist
+    m, hyai, hybi, o = setup
+    nl = Namelist()
+    table = oracle.filter_table(NLON, NLAT, m, 7200.0, nl, 2)
+    ng_lon, w_lon, ng_lat, w_lat = table
+    lam = 2 * np.pi * np.arange(NLON) / NLON
+    mode = 3
+    u = np.broadcast_to(np.cos(mode * lam), (NLEV, NLAT, NLON)) + np.arange(NLEV)[:, None, None]
+    v = np.broadcast_to(np.cos(mode * lam), (NLEV, NLAT - 1, NLON)) + np.arange(NLEV)[:, None, None]
+    fu, fv = o.small_filter2(u, v, table)
+    for ng, w, x, fx in ((ng_lon, w_lon, u, fu), (ng_lat, w_lat, v, fv)):
+        for r in range(len(ng)):
+            n = int(ng[r])
+            if n <= 1:
+                assert np.array_equal(fx[:, r], x[:, r])
+                continue
+            t = np.arange(n) - (n - 1) // 2
+            resp = np.sum(w[r, :n] * np.cos(mode * t * 2 * np.pi / NLON))
+            want = np.arange(NLEV)[:, None] + resp * np.cos(mode * lam)[None, :]
+            np.testing.assert_allclose(fx[:, r], want, rtol=0, atol=1e-12 * NLEV)
+    assert (ng_lon > 1).sum() > 4 and (ng_lat > 1).sum() > 4

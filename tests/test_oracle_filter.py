@@ -4,7 +4,9 @@
   (widths from the CFL formula with the namelist defaults, including the
   single-precision literals 0.4 and 0.2 of namelist_mod.F90:92-93);
 - weights are the closed-form Gaussian exp(-x^2 / (2 s^2)), s = b w / 2,
-  normalised: sum 1, symmetric, ratio to the centre tap exact to 1e-15;
+  normalised: sum 1, symmetric, ratio to the centre tap exact to 1e-15,
+  with the widths of small_filter1 (x filter_coef_d) and small_filter2 (x
+  exp_two_values of filter_coef_e and filter_coef_d) and their tap counts;
 - the Fortran host and the C oracle build identical tables;
 - filtering a zonally constant field returns the constant (through the
   oracle's filter application at gmcore_run's set-up).
@@ -47,15 +49,24 @@ def test_namelist_single_precision_defaults():
     assert nl.filter_coef_d == 0.375 and nl.max_cfl == 0.5
 
 
-@pytest.mark.parametrize("kind", [0, 1])
+def small2_factor(nl, lat, x0):
+    """exp_two_values(filter_coef_e, filter_coef_d, x0, 80, |lat|) (math_mod.F90:146-159)"""
+    w = np.exp((abs(lat) - x0) ** 2 * np.log(1.0e-3) / (80.0 - x0) ** 2)
+    return w * nl.filter_coef_e + (1 - w) * nl.filter_coef_d
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
 def test_weights_closed_form(kind):
     nlon, nlat, dt = 360, 181, 300.0
     nl = Namelist()
     m = oracle.mesh_metrics(nlon, nlat, nl.radius, nl.omega)
     ng_lon, w_lon, ng_lat, w_lat = oracle.filter_table(nlon, nlat, m, dt, nl, kind)
+    big = oracle.filter_table(nlon, nlat, m, dt, nl, 0)
     checked = 0
-    for ng, w, dx, lat in ((ng_lon, w_lon, m[MESH["de_lon"]], m[MESH["full_lat_deg"]]),
-                           (ng_lat, w_lat, m[MESH["le_lat"], :nlat - 1], m[MESH["half_lat_deg"], :nlat - 1])):
+    for ng, w, dx, lat, x0, nb in ((ng_lon, w_lon, m[MESH["de_lon"]], m[MESH["full_lat_deg"]],
+                                    -m[MESH["full_lat_deg"], 1], big[0]),
+                                   (ng_lat, w_lat, m[MESH["le_lat"], :nlat - 1], m[MESH["half_lat_deg"], :nlat - 1],
+                                    -m[MESH["half_lat_deg"], 0], big[2])):
         for r in range(len(ng)):
             n = int(ng[r])
             if n <= 1:
@@ -68,6 +79,11 @@ def test_weights_closed_form(kind):
                 nl.filter_coef_c * (np.tanh(90 - abs(lat[r])) - 1) + 1)
             if kind == 1:
                 width *= nl.filter_coef_d
+            if kind == 2:
+                f = small2_factor(nl, lat[r], x0)
+                width *= f
+                want = int(f * nb[r])
+                assert n == (want + 1 if want % 2 == 0 else want)
             s = nl.filter_coef_b * width / 2
             x = np.arange(n) - (n - 1) // 2
             assert np.allclose(ww / ww[(n - 1) // 2], np.exp(-x ** 2 / (2 * s * s)), rtol=1e-13, atol=0)
@@ -75,7 +91,22 @@ def test_weights_closed_form(kind):
     assert checked > 50
 
 
-@pytest.mark.parametrize("kind", [0, 1])
+def test_small_filter2_factor_ends():
+    """small_filter2 shrinks the big filter by filter_coef_e next to the poles
+    and by about filter_coef_d from 80 degrees to the equator."""
+    nl = Namelist()
+    assert small2_factor(nl, 89.0, 89.0) == nl.filter_coef_e
+    assert small2_factor(nl, 80.0, 89.0) == pytest.approx(1e-3 * nl.filter_coef_e + 0.999 * nl.filter_coef_d,
+                                                          rel=1e-15)
+    nlon, nlat, dt = 360, 181, 300.0
+    m = oracle.mesh_metrics(nlon, nlat, nl.radius, nl.omega)
+    big = oracle.filter_table(nlon, nlat, m, dt, nl, 0)
+    s2 = oracle.filter_table(nlon, nlat, m, dt, nl, 2)
+    rows = np.nonzero(big[0] > 1)[0]
+    assert np.all(s2[0][rows] <= big[0][rows]) and np.all(s2[0][rows] % 2 == 1)
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
 def test_fortran_host_filter_bitwise(kind):
     nlon, nlat, dt = 180, 91, 600.0
     nl = Namelist()
