@@ -1,5 +1,6 @@
 ! Thin ISO_C_BINDING layer over the C ABI of libgmcore_hip.so
-! (include/gmcore_hip.h, include/gmcore_adv.h).  Every function returns 0 on success; gmc_check
+! (include/gmcore_hip.h, include/gmcore_adv.h, include/gmcore_baro.h).  Every function returns 0
+! on success; gmc_check
 ! turns a non-zero code into the message of gmc_last_error and stops, which
 ! keeps the reference's abort-on-error behaviour (log_error / process_stop,
 ! src/utils/process_mod.F90:54-68).
@@ -45,6 +46,24 @@ module gmc_hip_mod
     integer(c_int32_t) :: nstep, ntracers, lon_halo_width, limiter, lon_damp_passes, flux_type
     type(c_ptr) :: lon_damp_wgt
   end type gmc_adv_config
+
+  ! hydrostatic column operators (include/gmcore_baro.h)
+  type, bind(C) :: gmc_vcoord
+    integer(c_int32_t) :: nlev, reserved
+    real(c_double) :: p0, ptop
+    type(c_ptr) :: hyai, hybi
+  end type gmc_vcoord
+
+  type, bind(C) :: gmc_thermo
+    real(c_double) :: Rd, cpd, p0
+  end type gmc_thermo
+
+  integer(c_int32_t), parameter :: GMB_PHS = 0, GMB_GZS = 1, GMB_DPHS = 2, GMB_PT = 3, GMB_PH_LEV = 4, &
+    GMB_PH_EXN_LEV = 5, GMB_PH = 6, GMB_T = 7, GMB_GZ_LEV = 8, GMB_GZ = 9, GMB_DMFDLON = 10, &
+    GMB_DMFDLAT = 11, GMB_WE_LEV = 12, GMB_WE_LEV_LON = 13, GMB_WE_LEV_LAT = 14, GMB_U_F = 15, &
+    GMB_V_F = 16, GMB_M_LON = 17, GMB_M_LAT = 18, GMB_WEDUDLEV = 19, GMB_WEDVDLEV = 20, &
+    GMB_PGF_LON = 21, GMB_PGF_LAT = 22, GMB_CFLZ = 23, GMB_PTF_LEV = 24, GMB_DPTFDLEV = 25, &
+    GMB_U_LON = 26, GMB_V_LAT = 27
 
   interface
     integer(c_int) function gmc_create(mesh, cfg, ctx) bind(C, name='gmc_create')
@@ -163,6 +182,55 @@ module gmc_hip_mod
     integer(c_int) function gmc_adv_fix_count(adv, count) bind(C, name='gmc_adv_fix_count')
       import; type(c_ptr), value :: adv; integer(c_int64_t), intent(out) :: count
     end function
+    ! hydrostatic column operators
+    integer(c_int) function gmc_baro_create(mesh, vc, th, baro) bind(C, name='gmc_baro_create')
+      import; type(gmc_mesh), intent(in) :: mesh; type(gmc_vcoord), intent(in) :: vc
+      type(gmc_thermo), intent(in) :: th; type(c_ptr), intent(out) :: baro
+    end function
+    integer(c_int) function gmc_baro_destroy(baro) bind(C, name='gmc_baro_destroy')
+      import; type(c_ptr), value :: baro
+    end function
+    type(c_ptr) function gmc_baro_last_error(baro) bind(C, name='gmc_baro_last_error')
+      import; type(c_ptr), value :: baro
+    end function
+    integer(c_int) function gmc_baro_set(baro, field, host, ld) bind(C, name='gmc_baro_set')
+      import; type(c_ptr), value :: baro; integer(c_int32_t), value :: field
+      real(c_double), intent(in) :: host(*); integer(c_int64_t), value :: ld
+    end function
+    integer(c_int) function gmc_baro_get(baro, field, host, ld) bind(C, name='gmc_baro_get')
+      import; type(c_ptr), value :: baro; integer(c_int32_t), value :: field
+      real(c_double), intent(out) :: host(*); integer(c_int64_t), value :: ld
+    end function
+    integer(c_int) function gmc_baro_calc_ph(baro) bind(C, name='gmc_baro_calc_ph')
+      import; type(c_ptr), value :: baro
+    end function
+    integer(c_int) function gmc_baro_calc_t(baro) bind(C, name='gmc_baro_calc_t')
+      import; type(c_ptr), value :: baro
+    end function
+    integer(c_int) function gmc_baro_calc_gz_lev(baro) bind(C, name='gmc_baro_calc_gz_lev')
+      import; type(c_ptr), value :: baro
+    end function
+    integer(c_int) function gmc_baro_calc_dphsdt(baro) bind(C, name='gmc_baro_calc_dphsdt')
+      import; type(c_ptr), value :: baro
+    end function
+    integer(c_int) function gmc_baro_calc_we_lev(baro) bind(C, name='gmc_baro_calc_we_lev')
+      import; type(c_ptr), value :: baro
+    end function
+    integer(c_int) function gmc_baro_calc_wedudlev_wedvdlev(baro) bind(C, name='gmc_baro_calc_wedudlev_wedvdlev')
+      import; type(c_ptr), value :: baro
+    end function
+    integer(c_int) function gmc_baro_pgf_lin97(baro) bind(C, name='gmc_baro_pgf_lin97')
+      import; type(c_ptr), value :: baro
+    end function
+    integer(c_int) function gmc_baro_calc_ptf_lev(baro, limiter) bind(C, name='gmc_baro_calc_ptf_lev')
+      import; type(c_ptr), value :: baro; integer(c_int32_t), value :: limiter
+    end function
+    integer(c_int) function gmc_baro_small_filter2(baro, filter) bind(C, name='gmc_baro_small_filter2')
+      import; type(c_ptr), value :: baro; type(gmc_filter), intent(in) :: filter
+    end function
+    integer(c_int) function gmc_baro_synchronize(baro) bind(C, name='gmc_baro_synchronize')
+      import; type(c_ptr), value :: baro
+    end function
   end interface
 
 contains
@@ -178,6 +246,8 @@ contains
     if (c_associated(ctx)) then
       if (what(1:min(7, len(what))) == 'gmc_adv') then
         call c_f_pointer(gmc_adv_last_error(ctx), msg, [512])
+      else if (what(1:min(8, len(what))) == 'gmc_baro') then
+        call c_f_pointer(gmc_baro_last_error(ctx), msg, [512])
       else
         call c_f_pointer(gmc_last_error(ctx), msg, [512])
       end if

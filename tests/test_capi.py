@@ -56,3 +56,25 @@ def test_struct_layouts_match_header():
     # gmc_adv_config: 2 doubles, 6 int32, 1 pointer
     assert C.sizeof(capi.GmcAdvConfig) == 48
     assert capi.GmcAdvConfig.lon_damp_wgt.offset == 40
+
+
+def test_fortran_bindings_name_exported_symbols():
+    """Every bind(C, name=...) interface of the Fortran host module names a
+    function the library exports."""
+    text = open(os.path.join(ROOT, "gmcore_amd", "fortran", "gmc_hip_mod.F90")).read()
+    names = re.findall(r"bind\(C, name='(gmc_[a-z_0-9]+)'\)", text)
+    assert len(names) > 40
+    lib = C.CDLL(capi.LIB_HIP)
+    assert not [n for n in names if not hasattr(lib, n)]
+
+
+def test_fortran_baro_field_codes_match_header():
+    """The GMB_* parameters of the Fortran module are the enum of
+    include/gmcore_baro.h, in order."""
+    text = open(os.path.join(ROOT, "gmcore_amd", "fortran", "gmc_hip_mod.F90")).read()
+    fortran = {k: int(v) for k, v in re.findall(r"(GMB_[A-Z_]+) = (\d+)", text)}
+    header = open(os.path.join(ROOT, "include", "gmcore_baro.h")).read()
+    enum = re.search(r"enum gmc_baro_field \{(.*?)\};", header, re.S).group(1)
+    order = [m for m in re.findall(r"\b(GMB_[A-Z_]+)\b", re.sub(r"/\*.*?\*/", "", enum)) if m != "GMB_NFIELDS"]
+    assert fortran == {name: k for k, name in enumerate(order)}
+    assert list(capi.BARO_FIELD) == [n[4:].lower() for n in order]
