@@ -87,6 +87,19 @@ ADJ_CONFIGS = {
                  obs_seed=5, sigma_o=50.0, sigma_b=1.0, window=48, ckpt=8),
 }
 
+# hydrostatic column operators: 1 degree, 32 levels of the equal-eta
+# coordinate, a stratified column state
+BARO_CONFIGS = {
+    "baro1": dict(nlon=360, nlat=181, nlev=32, ptop=219.4, reps=20),
+}
+# algorithmic FP64 fields per column of each operator, as (full-level
+# passes, half-level passes, single-level passes), reads and writes
+BARO_FIELDS_PER_OP = {
+    "calc_ph": (1, 2, 1), "calc_t": (3, 0, 0), "calc_gz_lev": (3, 1, 1), "calc_dphsdt": (2, 0, 1),
+    "calc_we_lev": (2, 4, 1), "calc_wedudlev_wedvdlev": (6, 2, 0), "pgf_lin97": (2, 2, 0),
+    "calc_ptf_lev": (2, 3, 0), "small_filter2": (4, 0, 0),
+}
+
 CONFIGS = {
     "sw025": dict(nlon=1440, nlat=721, dt=60.0, lmax=60, seed=1),
     "sw2": dict(nlon=180, nlat=91, dt=600.0, lmax=20, seed=0),
@@ -530,6 +543,92 @@ def run_adv_adj(args):
     print(json.dumps(result))
 
 
+def run_baro(args):
+    """The hydrostatic column operators on a 1-degree, 32-level block: each
+    operator launched `reps` times back to back between two HIP events on
+    the block's stream; GB/s of its algorithmic bytes.  value = column-level
+    points per second of the sequence calc_ph, calc_t, calc_gz_lev,
+    pgf_lin97 (the hydrostatic pressure gradient of a backward pass)."""
+    import ctypes as C
+    import oracle
+    from gmcore_amd.baro import CPD, P0, RD, Baroclinic
+    from gmcore_amd.host import Filter, Mesh
+    from gmcore_amd.namelist import Namelist
+    cfg = BARO_CONFIGS[args.config]
+    nlon, nlat, nlev, reps = cfg["nlon"], cfg["nlat"], cfg["nlev"], cfg["reps"]
+    mesh = Mesh(nlon, nlat, 6371220.0, 7.292e-5)
+    hyai, hybi = oracle.equal_eta(nlev)
+    g = Baroclinic(mesh, hyai, hybi, P0, cfg["ptop"], RD, CPD, P0)
+    rng = np.random.default_rng(0)
+    col = np.linspace(1.0, 0.0, nlev)[:, None, None]
+    g.set("phs", 1.0e5 + 2000.0 * rng.standard_normal((1, nlat, nlon)))
+    g.set("pt", 300.0 + 60.0 * col + rng.standard_normal((nlev, nlat, nlon)))
+    gzs = 2000.0 * rng.random((1, nlat, nlon))
+    g.set("gzs", gzs)
+    bottom = np.zeros((nlev + 1, nlat, nlon))
+    bottom[-1] = gzs[0]
+    g.set("gz_lev", bottom)
+    for name in ("dmfdlon", "dmfdlat"):
+        g.set(name, 1e-3 * rng.standard_normal((nlev, nlat, nlon)))
+    g.set("u_f", 20 * rng.standard_normal((nlev, nlat, nlon)))
+    g.set("v_f", 20 * rng.standard_normal((nlev, nlat - 1, nlon)))
+    g.set("u_lon", 20 * rng.standard_normal((nlev, nlat, nlon)))
+    g.set("v_lat", 20 * rng.standard_normal((nlev, nlat - 1, nlon)))
+    g.set("m_lon", 1e3 + 4e3 * rng.random((nlev, nlat, nlon)))
+    g.set("m_lat", 1e3 + 4e3 * rng.random((nlev, nlat - 1, nlon)))
+    c = rng.uniform(-0.9, 0.9, (nlev + 1, nlat, nlon))
+    g.set("cflz", c)
+    table = Filter(mesh, 300.0, Namelist(num_lon=nlon, num_lat=nlat, dt_dyn=300.0), 2)
+    ops = [("calc_ph", g.calc_ph), ("calc_t", g.calc_t), ("calc_gz_lev", g.calc_gz_lev),
+           ("calc_dphsdt", g.calc_dphsdt), ("calc_we_lev", g.calc_we_lev),
+           ("calc_wedudlev_wedvdlev", g.calc_wedudlev_wedvdlev), ("pgf_lin97", g.pgf_lin97),
+           ("calc_ptf_lev", g.calc_ptf_lev), ("small_filter2", lambda: g.small_filter2(table))]
+    hip = C.CDLL("libamdhip64.so")
+    stream = C.c_void_p()
+    assert g.lib.gmc_baro_stream(g.h, C.byref(stream)) == 0
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    assert hip.hipEventCreate(C.byref(e0)) == 0 and hip.hipEventCreate(C.byref(e1)) == 0
+    for _, f in ops:
+        f()
+    g.synchronize()
+    cols = nlon * nlat
+    kernels = []
+    for name, f in ops:
+        best = 1e30
+        for _ in range(3):
+            hip.hipEventRecord(e0, stream)
+            for _ in range(reps):
+                f()
+            hip.hipEventRecord(e1, stream)
+            hip.hipEventSynchronize(e1)
+            ms = C.c_float()
+            hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+            best = min(best, ms.value / reps)
+        full, half, one = BARO_FIELDS_PER_OP[name]
+        nbytes = 8.0 * cols * (full * nlev + half * (nlev + 1) + one)
+        kernels.append({"name": name, "avg_us": best * 1e3, "bytes": nbytes,
+                        "gbs": nbytes / (best * 1e-3) / 1e9})
+    seq = ("calc_ph", "calc_t", "calc_gz_lev", "pgf_lin97")
+    t_seq = sum(k["avg_us"] for k in kernels if k["name"] in seq)
+    dom = max(kernels, key=lambda k: k["avg_us"])
+    result = {
+        "metric": "column-level-points/s (FP64 hydrostatic column operators, 1 GPU)",
+        "value": cols * nlev / (t_seq * 1e-6),
+        "unit": "column-level-points/s",
+        "n_gpus": 1, "steps": reps, "warmup": 1, "ms_per_step": t_seq * 1e-3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (equal-eta coordinate, stratified pt, random fluxes and winds)",
+        "config": {"workload": f"baroclinic column operators {args.config}: {nlon}x{nlat}x{nlev} levels, FP64",
+                   "nlon": nlon, "nlat": nlat, "nlev": nlev},
+        "roofline": {"bound": "hbm", "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": profiled_traffic(args.config, dom["name"]),
+                     "kernel": dom["name"],
+                     "scope": "slowest operator: algorithmic bytes over its mean time in back-to-back launches"},
+        "kernels": kernels,
+    }
+    print(json.dumps(result))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -537,7 +636,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="sw025",
                     choices=sorted(CONFIGS) + sorted(ADV_CONFIGS) + sorted(ENS_CONFIGS) + sorted(ADJ_CONFIGS)
-                    + sorted(ADV_ADJ_CONFIGS))
+                    + sorted(ADV_ADJ_CONFIGS) + sorted(BARO_CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -571,6 +670,9 @@ def main():
         return
     if args.config in ADV_ADJ_CONFIGS:
         run_adv_adj(args)
+        return
+    if args.config in BARO_CONFIGS:
+        run_baro(args)
         return
     cfg = CONFIGS[args.config]
     from gmcore_amd.host import ShallowWater
