@@ -582,7 +582,8 @@ def run_baro(args):
     ops = [("calc_ph", g.calc_ph), ("calc_t", g.calc_t), ("calc_gz_lev", g.calc_gz_lev),
            ("calc_dphsdt", g.calc_dphsdt), ("calc_we_lev", g.calc_we_lev),
            ("calc_wedudlev_wedvdlev", g.calc_wedudlev_wedvdlev), ("pgf_lin97", g.pgf_lin97),
-           ("calc_ptf_lev", g.calc_ptf_lev), ("small_filter2", lambda: g.small_filter2(table))]
+           ("calc_ptf_lev", lambda: g.calc_ptf_lev(check=False)), ("small_filter2", g.small_filter2)]
+    g.set_small_filter2(table)
     hip = C.CDLL("libamdhip64.so")
     stream = C.c_void_p()
     assert g.lib.gmc_baro_stream(g.h, C.byref(stream)) == 0

@@ -89,15 +89,26 @@ class Baroclinic:
     def pgf_lin97(self):
         self._check(self.lib.gmc_baro_pgf_lin97(self.h), "pgf_lin97")
 
-    def calc_ptf_lev(self, limiter: str = "mono"):
-        """The vertical FFSL flux of pt (calc_grad_ptf's vertical part)."""
+    def calc_ptf_lev(self, limiter: str = "mono", check: bool = True):
+        """The vertical FFSL flux of pt (calc_grad_ptf's vertical part).  With
+        check, wait for it and raise BaroError if a Courant number reached
+        outside the column; otherwise the next synchronize() reports it."""
         self._check(self.lib.gmc_baro_calc_ptf_lev(self.h, capi.LIMITERS[limiter]), "calc_ptf_lev")
+        if check:
+            self.synchronize()
 
-    def small_filter2(self, table):
-        """update_state's small_filter2 step on u_lon and v_lat; table: a
-        host.Filter of kind 2."""
-        f = table.struct()
-        self._check(self.lib.gmc_baro_small_filter2(self.h, C.byref(f)), "small_filter2")
+    def set_small_filter2(self, table):
+        """Copy the small_filter2 table (a host.Filter of kind 2) to the
+        device, once per run as filter_init builds it."""
+        self._table = table.struct()
+        self._check(self.lib.gmc_baro_set_small_filter2(self.h, C.byref(self._table)), "set_small_filter2")
+
+    def small_filter2(self, table=None):
+        """update_state's small_filter2 step on u_lon and v_lat with the table
+        set by set_small_filter2 (or given here, which sets it first)."""
+        if table is not None:
+            self.set_small_filter2(table)
+        self._check(self.lib.gmc_baro_small_filter2(self.h), "small_filter2")
 
     def synchronize(self):
         self._check(self.lib.gmc_baro_synchronize(self.h), "synchronize")

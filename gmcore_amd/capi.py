@@ -215,7 +215,8 @@ def load_hip() -> C.CDLL:
         "gmc_baro_calc_wedudlev_wedvdlev": (C.c_int, [vp]),
         "gmc_baro_pgf_lin97": (C.c_int, [vp]),
         "gmc_baro_calc_ptf_lev": (C.c_int, [vp, C.c_int32]),
-        "gmc_baro_small_filter2": (C.c_int, [vp, P(GmcFilter)]),
+        "gmc_baro_set_small_filter2": (C.c_int, [vp, P(GmcFilter)]),
+        "gmc_baro_small_filter2": (C.c_int, [vp]),
         "gmc_baro_synchronize": (C.c_int, [vp]),
         "gmc_baro_stream": (C.c_int, [vp, P(vp)]),
         # batched ensembles, include/gmcore_ens.h

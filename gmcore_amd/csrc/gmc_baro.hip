@@ -1,14 +1,17 @@
 // gmc_baro.hip -- hydrostatic column operators of the baroclinic dynamics
 // (include/gmcore_baro.h).
 //
-// Every kernel gives one thread one column (i, r) and walks the levels from
-// the model top, so the loads and stores of a level are coalesced along the
-// latitude row and the level recurrences (the geopotential's sums, the
-// vertical mass flux) stay in the thread that owns the column.  Fields are
-// [levels][rows][pitch] arrays in one device slab; V-point fields have
-// nlat - 1 rows per level.  The arithmetic is the reference's, operation by
-// operation and in its order (IEEE division, no contraction); pow and log are
-// the device library's, within a few ulp of the host's.
+// Fields are [levels][rows][pitch] arrays in one device slab; V-point fields
+// have nlat - 1 rows per level.  The level-local operators give a thread one
+// column and a chunk of LC consecutive levels (grid x over the points of a
+// level plane, grid y over the chunks), so the loads and stores of a level
+// are coalesced along the latitude row and a launch has LC-times fewer
+// threads than points but enough waves to fill the chip at 32 levels.  The
+// geopotential's sums stage a block of columns in LDS; the two level
+// recurrences (the surface-pressure tendency and the vertical mass flux)
+// keep one thread per column.  The arithmetic is the reference's, operation
+// by operation and in its order (IEEE division, no contraction); pow and log
+// are the device library's, within a few ulp of the host's.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -47,57 +50,134 @@ constexpr FieldShape kShape[GMB_NFIELDS] = {
     {2, false}, {2, false}, {1, false},                  // cflz, ptf_lev, dptfdlev
     {1, false}, {1, true}};                              // u_lon, v_lat
 
+constexpr int LC = 4;     // levels per thread of the level-chunked kernels
+
+// the point of this thread in a level-chunked launch over `rows` rows of a
+// level plane: o = r * pitch + i; false in the pitch padding and past the
+// last row
+__device__ __forceinline__ bool plane_point(const BaroGrid &g, int rows, int64_t &o, int &i, int &r) {
+  const uint32_t p = blockIdx.x * NB + threadIdx.x;
+  r = (int)(p / (uint32_t)g.pitch);
+  i = (int)(p - (uint32_t)r * (uint32_t)g.pitch);
+  o = p;
+  return r < rows && i < g.nlon;
+}
+
 // calc_ph (operators_mod.F90:136-168): the half-level pressures of the
 // hybrid coordinate (hybrid_coord_mod.F90:155-162), their Exner powers, and
-// the full levels' mean of the two faces
+// the full levels' mean of the two faces.  The chunk of full levels k0 .. k1-1
+// writes their top half levels; the last chunk also the bottom one.
+__device__ __forceinline__ double half_level_p(const double *hyai, const double *hybi, double p0, double ptop,
+                                               double ps, int k) {
+  return hyai[k] * (p0 - ptop) + hybi[k] * (ps - ptop) + ptop;
+}
+
 __global__ void __launch_bounds__(NB) k_calc_ph(BaroGrid g, const double *__restrict__ hyai,
                                                 const double *__restrict__ hybi, double p0, double ptop,
                                                 double rd_o_cpd, const double *__restrict__ phs, double *ph_lev,
                                                 double *ph_exn_lev, double *ph) {
-  const int i = blockIdx.x * NB + threadIdx.x, r = blockIdx.y;
-  if (i >= g.nlon) return;
-  const int64_t o = r * g.pitch + i;
+  int64_t o;
+  int i, r;
+  if (!plane_point(g, g.nlat, o, i, r)) return;
+  const int k0 = blockIdx.y * LC, k1 = min(k0 + LC, g.nlev);
   const double ps = phs[o];
-  double above = 0.0;
-  for (int k = 0; k <= g.nlev; k++) {
-    const double pl = hyai[k] * (p0 - ptop) + hybi[k] * (ps - ptop) + ptop;
-    ph_lev[k * g.sc + o] = pl;
-    ph_exn_lev[k * g.sc + o] = pow(pl, rd_o_cpd);
-    if (k > 0) ph[(k - 1) * g.sc + o] = 0.5 * (above + pl);
-    above = pl;
+  double above = half_level_p(hyai, hybi, p0, ptop, ps, k0);
+  for (int k = k0; k < k1; k++) {
+    const double below = half_level_p(hyai, hybi, p0, ptop, ps, k + 1);
+    ph_lev[k * g.sc + o] = above;
+    ph_exn_lev[k * g.sc + o] = pow(above, rd_o_cpd);
+    ph[k * g.sc + o] = 0.5 * (above + below);
+    above = below;
+  }
+  if (k1 == g.nlev) {
+    ph_lev[k1 * g.sc + o] = above;
+    ph_exn_lev[k1 * g.sc + o] = pow(above, rd_o_cpd);
   }
 }
 
 // calc_t (operators_mod.F90:170-190) with temperature() (formula_mod.F90:24-31)
 __global__ void __launch_bounds__(NB) k_calc_t(BaroGrid g, double p0, double rd_o_cpd, const double *__restrict__ pt,
                                                const double *__restrict__ ph, double *t) {
-  const int i = blockIdx.x * NB + threadIdx.x, r = blockIdx.y;
-  if (i >= g.nlon) return;
-  const int64_t o = r * g.pitch + i;
-  for (int k = 0; k < g.nlev; k++) t[k * g.sc + o] = pt[k * g.sc + o] * pow(ph[k * g.sc + o] / p0, rd_o_cpd);
+  int64_t o;
+  int i, r;
+  if (!plane_point(g, g.nlat, o, i, r)) return;
+  const int k0 = blockIdx.y * LC;
+  double a[LC], b[LC];
+#pragma unroll
+  for (int j = 0; j < LC; j++) {   // every load of the chunk in flight at once
+    const int k = min(k0 + j, g.nlev - 1);
+    a[j] = pt[k * g.sc + o];
+    b[j] = ph[k * g.sc + o];
+  }
+#pragma unroll
+  for (int j = 0; j < LC; j++)
+    if (k0 + j < g.nlev) t[(k0 + j) * g.sc + o] = a[j] * pow(b[j] / p0, rd_o_cpd);
 }
 
 // calc_gz_lev (operators_mod.F90:433-469).  Each half level's sum runs from
 // 0 over the levels below it in ascending order, as the reference's inner
-// loop does (so no running suffix sum: that would change the bits).  The
-// layer terms Rd t log(ph_lev(l+1) / ph_lev(l)) are formed once into the gz
-// array, which the full-level means overwrite at the end.
-__global__ void __launch_bounds__(NB) k_calc_gz_lev(BaroGrid g, double rd, const double *__restrict__ gzs,
+// loop does (so no running suffix sum: that would change the bits).  A block
+// holds cw columns (a power of two) x NB / cw level lanes: the layer terms
+// Rd t log(ph_lev(l+1) / ph_lev(l)) of its columns go to LDS (term l of
+// column c at lds[l * cw + c]), lane q sums the half levels q, q + NB / cw, ..
+// out of LDS, GZB of them at a time in one ascending pass over the terms (a
+// chain per half level, each in its own order), and the full-level means read
+// the half levels back from a second LDS array (the bottom half level, which
+// keeps its value, from HBM).
+constexpr int GZB = 8;
+
+__global__ void __launch_bounds__(NB) k_calc_gz_lev(BaroGrid g, int cw, double rd, const double *__restrict__ gzs,
                                                     const double *__restrict__ t,
                                                     const double *__restrict__ ph_lev, double *gz_lev, double *gz) {
-  const int i = blockIdx.x * NB + threadIdx.x, r = blockIdx.y;
-  if (i >= g.nlon) return;
-  const int64_t o = r * g.pitch + i;
-  const int nl = g.nlev;
-  for (int l = 0; l < nl; l++)
-    gz[l * g.sc + o] = rd * t[l * g.sc + o] * log(ph_lev[(l + 1) * g.sc + o] / ph_lev[l * g.sc + o]);
-  const double s = gzs[o];
-  for (int k = 0; k < nl; k++) {
-    double dgz = 0.0;
-    for (int l = k; l < nl; l++) dgz = dgz + gz[l * g.sc + o];
-    gz_lev[k * g.sc + o] = s + dgz;
+  extern __shared__ double lds[];
+  const int n = g.nlev, c = threadIdx.x & (cw - 1), q = threadIdx.x / cw, lanes = NB / cw;
+  double *term = lds, *glev = lds + (size_t)n * cw;
+  const uint32_t p = blockIdx.x * (uint32_t)cw + c;
+  const int r = (int)(p / (uint32_t)g.pitch), i = (int)(p - (uint32_t)r * (uint32_t)g.pitch);
+  const bool ok = r < g.nlat && i < g.nlon;
+  const int64_t o = ok ? (int64_t)p : 0;   // a column outside the block's rows reads column 0, unused
+  for (int l0 = q; l0 < n; l0 += GZB * lanes) {
+    double tv[GZB], pa[GZB], pb[GZB];
+#pragma unroll
+    for (int j = 0; j < GZB; j++) {   // the batch's loads in flight at once
+      const int l = min(l0 + j * lanes, n - 1);
+      tv[j] = t[l * g.sc + o];
+      pa[j] = ph_lev[l * g.sc + o];
+      pb[j] = ph_lev[(l + 1) * g.sc + o];
+    }
+#pragma unroll
+    for (int j = 0; j < GZB; j++) {
+      const int l = l0 + j * lanes;
+      if (l < n) term[l * cw + c] = ok ? rd * tv[j] * log(pb[j] / pa[j]) : 0.0;
+    }
   }
-  for (int k = 0; k < nl; k++) gz[k * g.sc + o] = 0.5 * (gz_lev[k * g.sc + o] + gz_lev[(k + 1) * g.sc + o]);
+  __syncthreads();
+  const double s = gzs[o];
+  for (int k0 = q; k0 < n; k0 += GZB * lanes) {
+    double acc[GZB];
+#pragma unroll
+    for (int j = 0; j < GZB; j++) acc[j] = 0.0;
+    for (int l = k0; l < n; l++) {
+      const double v = term[l * cw + c];
+#pragma unroll
+      for (int j = 0; j < GZB; j++) acc[j] = l >= k0 + j * lanes ? acc[j] + v : acc[j];
+    }
+#pragma unroll
+    for (int j = 0; j < GZB; j++) {
+      const int k = k0 + j * lanes;
+      if (k < n) {
+        const double v = s + acc[j];
+        glev[k * cw + c] = v;
+        if (ok) gz_lev[k * g.sc + o] = v;
+      }
+    }
+  }
+  __syncthreads();
+  if (!ok) return;
+  for (int k = q; k < n; k += lanes) {
+    const double below = k + 1 < n ? glev[(k + 1) * cw + c] : gz_lev[n * g.sc + o];
+    gz[k * g.sc + o] = 0.5 * (glev[k * cw + c] + below);
+  }
 }
 
 // calc_dphsdt (operators_mod.F90:1120-1143)
@@ -107,7 +187,18 @@ __global__ void __launch_bounds__(NB) k_calc_dphsdt(BaroGrid g, const double *__
   if (i >= g.nlon) return;
   const int64_t o = r * g.pitch + i;
   double d = 0.0;
-  for (int k = 0; k < g.nlev; k++) d = d - dmfdlon[k * g.sc + o] - dmfdlat[k * g.sc + o];
+  for (int k0 = 0; k0 < g.nlev; k0 += 8) {
+    double a[8], b[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {   // the batch's loads in flight at once
+      const int k = min(k0 + j, g.nlev - 1);
+      a[j] = dmfdlon[k * g.sc + o];
+      b[j] = dmfdlat[k * g.sc + o];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+      if (k0 + j < g.nlev) d = d - a[j] - b[j];
+  }
   dphs[o] = d;
 }
 
@@ -126,52 +217,102 @@ __global__ void __launch_bounds__(NB) k_calc_we_lev(BaroGrid g, const double *__
   const double dp = dphs[o];
   double mf = 0.0;
   we_lev[o] = 0.0;
-  for (int k = 1; k < g.nlev; k++) {
-    mf = mf + dmfdlon[(k - 1) * g.sc + o] + dmfdlat[(k - 1) * g.sc + o];
-    we_lev[k * g.sc + o] = -(hybi[k] * dp) - mf;
+  for (int k0 = 1; k0 < g.nlev; k0 += 8) {
+    double a[8], b[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {   // the batch's loads in flight at once
+      const int k = min(k0 + j, g.nlev - 1);
+      a[j] = dmfdlon[(k - 1) * g.sc + o];
+      b[j] = dmfdlat[(k - 1) * g.sc + o];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int k = k0 + j;
+      if (k < g.nlev) {
+        mf = mf + a[j] + b[j];
+        we_lev[k * g.sc + o] = -(hybi[k] * dp) - mf;
+      }
+    }
   }
   we_lev[g.nlev * g.sc + o] = 0.0;
 }
 
+// The U rows 1 .. nlat-2 and the V rows 0 .. nlat-2 of one launch: virtual
+// row y < nlat - 2 is U row y + 1, the rest V row y - (nlat - 2).
+__device__ __forceinline__ bool u_or_v_row(const BaroGrid &g, int y, int &r) {
+  const bool u = y < g.nlat - 2;
+  r = u ? y + 1 : y - (g.nlat - 2);
+  return u;
+}
+
 // interp_lev_edge_to_lev_lon_edge / _lat_edge with no wind
-// (interp_mod.F90:360-369, :425-434), every half level: blockIdx.y < nlat - 2
-// are the U rows 1 .. nlat-2, the rest the V rows 0 .. nlat-2
+// (interp_mod.F90:360-369, :425-434), the chunk's half levels
 __global__ void __launch_bounds__(NB) k_we_edges(BaroGrid g, Mesh M, const double *__restrict__ we_lev,
                                                  double *we_lon, double *we_lat) {
-  const int i = blockIdx.x * NB + threadIdx.x;
-  if (i >= g.nlon) return;
-  const int y = blockIdx.y;
-  if (y < g.nlat - 2) {
-    const int r = y + 1, ie = i + 1 == g.nlon ? 0 : i + 1;
+  int64_t o;
+  int i, y, r;
+  if (!plane_point(g, 2 * g.nlat - 3, o, i, y)) return;
+  const int k0 = blockIdx.y * LC, k1 = min(k0 + LC, g.nlev + 1);
+  if (u_or_v_row(g, y, r)) {
+    const int ie = i + 1 == g.nlon ? 0 : i + 1;
     const double aw = M(GMC_AREA_LON_WEST, r), ae = M(GMC_AREA_LON_EAST, r), a = M(GMC_AREA_LON, r);
-    const int64_t o = r * g.pitch;
-    for (int k = 0; k <= g.nlev; k++)
-      we_lon[k * g.sc + o + i] = (aw * we_lev[k * g.sc + o + i] + ae * we_lev[k * g.sc + o + ie]) / a;
+    const int64_t c = r * g.pitch;
+    double w[LC], e[LC];
+#pragma unroll
+    for (int j = 0; j < LC; j++) {
+      const int k = min(k0 + j, g.nlev);
+      w[j] = we_lev[k * g.sc + c + i];
+      e[j] = we_lev[k * g.sc + c + ie];
+    }
+#pragma unroll
+    for (int j = 0; j < LC; j++)
+      if (k0 + j < k1) we_lon[(k0 + j) * g.sc + c + i] = (aw * w[j] + ae * e[j]) / a;
   } else {
-    const int r = y - (g.nlat - 2);
     const double an = M(GMC_AREA_LAT_NORTH, r), as = M(GMC_AREA_LAT_SOUTH, r), a = M(GMC_AREA_LAT, r);
-    const int64_t o = r * g.pitch + i;
-    for (int k = 0; k <= g.nlev; k++)
-      we_lat[k * g.sv + o] = (an * we_lev[k * g.sc + o + g.pitch] + as * we_lev[k * g.sc + o]) / a;
+    const int64_t c = r * g.pitch + i;
+    double nn[LC], ss[LC];
+#pragma unroll
+    for (int j = 0; j < LC; j++) {
+      const int k = min(k0 + j, g.nlev);
+      nn[j] = we_lev[k * g.sc + c + g.pitch];
+      ss[j] = we_lev[k * g.sc + c];
+    }
+#pragma unroll
+    for (int j = 0; j < LC; j++)
+      if (k0 + j < k1) we_lat[(k0 + j) * g.sv + c] = (an * nn[j] + as * ss[j]) / a;
   }
 }
 
-// calc_wedudlev_wedvdlev (operators_mod.F90:1145-1216), one column of one
-// point kind: x the wind (u_f or v_f), we its half-level mass flux, m its
-// full-level mass; the top and bottom levels take their one face
-__device__ __forceinline__ void wedx_column(int nlev, int64_t s, const double *__restrict__ x,
+// calc_wedudlev_wedvdlev (operators_mod.F90:1145-1216) on full levels
+// k0 .. k0+LC-1 (those below nlev) of one column of one point kind: x the
+// wind (u_f or v_f), we its half-level mass flux, m its full-level mass; the
+// top and bottom levels take their one face.  The chunk's loads (x on
+// k0-1 .. k0+LC, we on k0 .. k0+LC, m, indexes clamped into the column) are
+// issued before any arithmetic.
+__device__ __forceinline__ void wedx_levels(int nlev, int k0, int64_t s, const double *__restrict__ x,
                                             const double *__restrict__ we, const double *__restrict__ m,
                                             double *out) {
-  double xm = 0.0, x0 = x[0], xp = x[s];
-  out[0] = (we[s] * (xp - x0)) / m[0] / 2.0;
-  for (int k = 1; k < nlev - 1; k++) {
-    xm = x0;
-    x0 = xp;
-    xp = x[(k + 1) * s];
-    out[k * s] = (we[(k + 1) * s] * (xp - x0) + we[k * s] * (x0 - xm)) / m[k * s] / 2.0;
+  double xv[LC + 2], wv[LC + 1], mv[LC];
+#pragma unroll
+  for (int j = 0; j < LC + 2; j++) xv[j] = x[min(max(k0 - 1 + j, 0), nlev - 1) * s];
+#pragma unroll
+  for (int j = 0; j < LC + 1; j++) wv[j] = we[min(k0 + j, nlev) * s];
+#pragma unroll
+  for (int j = 0; j < LC; j++) mv[j] = m[min(k0 + j, nlev - 1) * s];
+#pragma unroll
+  for (int j = 0; j < LC; j++) {
+    const int k = k0 + j;
+    if (k >= nlev) break;
+    const double xm = xv[j], xk = xv[j + 1], xp = xv[j + 2];
+    double v;
+    if (k == 0)
+      v = (wv[j + 1] * (xp - xk)) / mv[j] / 2.0;
+    else if (k == nlev - 1)
+      v = (wv[j] * (xk - xm)) / mv[j] / 2.0;
+    else
+      v = (wv[j + 1] * (xp - xk) + wv[j] * (xk - xm)) / mv[j] / 2.0;
+    out[k * s] = v;
   }
-  const int k = nlev - 1;
-  out[k * s] = (we[k * s] * (x[k * s] - x[(k - 1) * s])) / m[k * s] / 2.0;
 }
 
 __global__ void __launch_bounds__(NB) k_wedudlev(BaroGrid g, const double *__restrict__ u,
@@ -179,16 +320,16 @@ __global__ void __launch_bounds__(NB) k_wedudlev(BaroGrid g, const double *__res
                                                  double *wedudlev, const double *__restrict__ v,
                                                  const double *__restrict__ we_lat,
                                                  const double *__restrict__ m_lat, double *wedvdlev) {
-  const int i = blockIdx.x * NB + threadIdx.x;
-  if (i >= g.nlon) return;
-  const int y = blockIdx.y;
-  if (y < g.nlat - 2) {
-    const int64_t o = (y + 1) * g.pitch + i;   // U rows 1 .. nlat-2
-    wedx_column(g.nlev, g.sc, u + o, we_lon + o, m_lon + o, wedudlev + o);
-  } else {
-    const int64_t o = (y - (g.nlat - 2)) * g.pitch + i;   // V rows 0 .. nlat-2
-    wedx_column(g.nlev, g.sv, v + o, we_lat + o, m_lat + o, wedvdlev + o);
-  }
+  int64_t o;
+  int i, y, r;
+  if (!plane_point(g, 2 * g.nlat - 3, o, i, y)) return;
+  const int k0 = blockIdx.y * LC;
+  const bool isu = u_or_v_row(g, y, r);
+  const int64_t c = r * g.pitch + i;
+  if (isu)
+    wedx_levels(g.nlev, k0, g.sc, u + c, we_lon + c, m_lon + c, wedudlev + c);
+  else
+    wedx_levels(g.nlev, k0, g.sv, v + c, we_lat + c, m_lat + c, wedvdlev + c);
 }
 
 // pgf_lin97_run, hydrostatic branch (pgf_lin97_mod.F90:59-105): corners 1 .. 4
@@ -204,85 +345,113 @@ __device__ __forceinline__ double lin97(double exn_a_top, double exn_a_bot, doub
   return -(dph1 * dgz1 + dph2 * dgz2) / de / (dph1 + dph2);
 }
 
+// full levels k0 .. k0+LC-1 (those below nlev) of one point: the lower half
+// level of level k is the upper one of level k + 1; the chunk's half levels
+// k0 .. k0+LC are loaded (clamped to the column) before any arithmetic
+__device__ __forceinline__ void lin97_levels(int nlev, int k0, int64_t s, int64_t a, int64_t b, double de,
+                                             const double *__restrict__ exn, const double *__restrict__ gz,
+                                             double *out, int64_t so) {
+  double ea[LC + 1], eb[LC + 1], ga[LC + 1], gb[LC + 1];
+#pragma unroll
+  for (int j = 0; j < LC + 1; j++) {
+    const int64_t k = min(k0 + j, nlev) * s;
+    ea[j] = exn[k + a];
+    eb[j] = exn[k + b];
+    ga[j] = gz[k + a];
+    gb[j] = gz[k + b];
+  }
+#pragma unroll
+  for (int j = 0; j < LC; j++)
+    if (k0 + j < nlev) out[(k0 + j) * so] = lin97(ea[j], ea[j + 1], eb[j], eb[j + 1], ga[j], ga[j + 1], gb[j], gb[j + 1], de);
+}
+
 __global__ void __launch_bounds__(NB) k_pgf_lin97(BaroGrid g, Mesh M, const double *__restrict__ exn,
                                                   const double *__restrict__ gz, double *pgf_lon, double *pgf_lat) {
-  const int i = blockIdx.x * NB + threadIdx.x;
-  if (i >= g.nlon) return;
-  const int y = blockIdx.y;
-  const int64_t s = g.sc;
-  if (y < g.nlat - 2) {
-    const int r = y + 1, ie = i + 1 == g.nlon ? 0 : i + 1;   // U rows 1 .. nlat-2
-    const double de = M(GMC_DE_LON, r);
-    const int64_t a = r * g.pitch + i, b = r * g.pitch + ie;
-    for (int k = 0; k < g.nlev; k++)
-      pgf_lon[k * s + a] = lin97(exn[k * s + a], exn[(k + 1) * s + a], exn[k * s + b], exn[(k + 1) * s + b],
-                                 gz[k * s + a], gz[(k + 1) * s + a], gz[k * s + b], gz[(k + 1) * s + b], de);
+  int64_t o;
+  int i, y, r;
+  if (!plane_point(g, 2 * g.nlat - 3, o, i, y)) return;
+  const int k0 = blockIdx.y * LC;
+  if (u_or_v_row(g, y, r)) {
+    const int ie = i + 1 == g.nlon ? 0 : i + 1;
+    const int64_t a = r * g.pitch + i;
+    lin97_levels(g.nlev, k0, g.sc, a, r * g.pitch + ie, M(GMC_DE_LON, r), exn, gz, pgf_lon + a, g.sc);
   } else {
-    const int r = y - (g.nlat - 2);   // V rows 0 .. nlat-2
-    const double de = M(GMC_DE_LAT, r);
-    const int64_t a = r * g.pitch + i, b = a + g.pitch, o = r * g.pitch + i;
-    for (int k = 0; k < g.nlev; k++)
-      pgf_lat[k * g.sv + o] = lin97(exn[k * s + a], exn[(k + 1) * s + a], exn[k * s + b], exn[(k + 1) * s + b],
-                                    gz[k * s + a], gz[(k + 1) * s + a], gz[k * s + b], gz[(k + 1) * s + b], de);
+    const int64_t a = r * g.pitch + i;
+    lin97_levels(g.nlev, k0, g.sc, a, a + g.pitch, M(GMC_DE_LAT, r), exn, gz, pgf_lat + a, g.sv);
   }
 }
 
-// the vertical part of calc_grad_ptf (operators_mod.F90:1083-1101) with
-// vflx_ppm_cell (ffsl_mod.F90:883-941).  Work arrays of the column: ext, pt
-// on levels -2 .. nlev+1 (ext level e = k + 2, stride sc, nlev + 4 levels);
-// ml, dm, m6, the PPM coefficients of the column's cells.  flag[0] is set
-// where a Courant number reaches outside the column (the index is then
-// clamped so that every access stays in the arrays).
+// pt on level j = -2 .. nlev+1 of a column: the levels themselves, and two
+// beyond the top and the bottom extrapolated linearly from the two nearest
+// (operators_mod.F90:1085-1088)
+__device__ __forceinline__ double ext_pt(const double *__restrict__ p, int64_t s, int n, int j) {
+  if (j >= 0 && j < n) return p[j * s];
+  if (j < 0) {
+    const double e1 = 2 * p[0] - p[s];
+    return j == -1 ? e1 : 2 * e1 - p[0];
+  }
+  const double en = 2 * p[(n - 1) * s] - p[(n - 2) * s];
+  return j == n ? en : 2 * en - p[(n - 1) * s];
+}
+
+// vflx_ppm_cell (ffsl_mod.F90:883-941) through interior half level k of a
+// column: the upwind cell's PPM partial plus the whole cells the Courant
+// number c crosses, times the mass flux w.  bad is set where c's upwind cell
+// falls outside the column (the indexes are then clamped so that every access
+// stays in the column's levels -2 .. nlev+1).
+__device__ __forceinline__ double pt_face_flux(int lim, const double *__restrict__ p, int64_t s, int n, int k,
+                                               double c, double w, bool &bad) {
+  if (c == 0) return 0.0;
+  int ci = (int)c;
+  const double cf = c - ci;
+  ci = ci > n + 4 ? n + 4 : (ci < -n - 4 ? -n - 4 : ci);   // beyond the column either way: flagged below
+  const bool down = c > 0;
+  // c > 0: upwind cell k - ci - 1, whole cells k - ci .. k - 1;
+  // c < 0: upwind cell k - ci, whole cells k .. k - ci - 1
+  int ku = down ? k - ci - 1 : k - ci;
+  int j0 = down ? k - ci : k, j1 = down ? k - 1 : k - ci - 1;
+  if (ku < 0 || ku > n - 1 || j0 < -2 || j1 > n + 1) {
+    bad = true;
+    ku = ku < 0 ? 0 : (ku > n - 1 ? n - 1 : ku);
+    j0 = j0 < -2 ? -2 : j0;
+    j1 = j1 > n + 1 ? n + 1 : j1;
+  }
+  double sum = 0.0;
+  for (int j = j0; j <= j1; j++) sum = sum + ext_pt(p, s, n, j);
+  const Ppm q = ppm(lim, ext_pt(p, s, n, ku - 2), ext_pt(p, s, n, ku - 1), ext_pt(p, s, n, ku),
+                    ext_pt(p, s, n, ku + 1), ext_pt(p, s, n, ku + 2));
+  const double s1 = down ? 1 - cf : 0.0, s2 = down ? 1.0 : -cf;
+  const double ds1 = s2 - s1, ds2 = s2 * s2 - s1 * s1, ds3 = s2 * s2 * s2 - s1 * s1 * s1;
+  return (down ? w : -w) * (sum + q.fl * ds1 + 0.5 * q.df * ds2 + q.f6 * (ds2 / 2.0 - ds3 / 3.0)) / c;
+}
+
+// the vertical part of calc_grad_ptf (operators_mod.F90:1083-1101): the
+// chunk's cells k0 .. k1-1 need their faces k0 .. k1; the interior ones are
+// formed here (face k0 also by the chunk above, which writes it), the top
+// and bottom faces are read as they are.  flag[0] is set where a Courant
+// number reaches outside the column.
 __global__ void __launch_bounds__(NB) k_vflx_pt(BaroGrid g, int lim, const double *__restrict__ pt,
                                                 const double *__restrict__ we, const double *__restrict__ cflz,
-                                                double *ext, double *ml, double *dm, double *m6, double *ptf,
-                                                double *dptf, int *flag) {
-  const int i = blockIdx.x * NB + threadIdx.x, r = blockIdx.y;
-  if (i >= g.nlon) return;
-  const int64_t o = r * g.pitch + i, s = g.sc;
-  const int n = g.nlev;
-  for (int k = 0; k < n; k++) ext[(k + 2) * s + o] = pt[k * s + o];
-  ext[1 * s + o] = 2 * ext[2 * s + o] - ext[3 * s + o];
-  ext[o] = 2 * ext[1 * s + o] - ext[2 * s + o];
-  ext[(n + 2) * s + o] = 2 * ext[(n + 1) * s + o] - ext[n * s + o];
-  ext[(n + 3) * s + o] = 2 * ext[(n + 2) * s + o] - ext[(n + 1) * s + o];
-  for (int k = 0; k < n; k++) {
-    const double *e = ext + (k + 2) * s + o;
-    const Ppm p = ppm(lim, e[-2 * s], e[-s], e[0], e[s], e[2 * s]);
-    ml[k * s + o] = p.fl;
-    dm[k * s + o] = p.df;
-    m6[k * s + o] = p.f6;
-  }
+                                                double *ptf, double *dptf, int *flag) {
+  int64_t o;
+  int i, r;
+  if (!plane_point(g, g.nlat, o, i, r)) return;
+  const int64_t s = g.sc;
+  const int n = g.nlev, k0 = blockIdx.y * LC, k1 = min(k0 + LC, n);
+  const double *col = pt + o;
   bool bad = false;
-  for (int k = 1; k < n; k++) {   // interior half levels
-    const double c = cflz[k * s + o];
-    int ci = (int)c;
-    const double cf = c - ci;
-    ci = ci > n + 4 ? n + 4 : (ci < -n - 4 ? -n - 4 : ci);   // beyond the column either way: flagged below
-    double f = 0.0;
-    if (c != 0) {
-      const bool down = c > 0;
-      // c > 0: upwind cell k - ci - 1, whole cells k - ci .. k - 1;
-      // c < 0: upwind cell k - ci, whole cells k .. k - ci - 1
-      int ku = down ? k - ci - 1 : k - ci;
-      int j0 = down ? k - ci : k, j1 = down ? k - 1 : k - ci - 1;
-      if (ku < 0 || ku > n - 1 || j0 < -2 || j1 > n + 1) {
-        bad = true;
-        ku = ku < 0 ? 0 : (ku > n - 1 ? n - 1 : ku);
-        j0 = j0 < -2 ? -2 : j0;
-        j1 = j1 > n + 1 ? n + 1 : j1;
-      }
-      double sum = 0.0;
-      for (int j = j0; j <= j1; j++) sum = sum + ext[(j + 2) * s + o];
-      const double s1 = down ? 1 - cf : 0.0, s2 = down ? 1.0 : -cf;
-      const double ds1 = s2 - s1, ds2 = s2 * s2 - s1 * s1, ds3 = s2 * s2 * s2 - s1 * s1 * s1;
-      const double w = we[k * s + o];
-      f = (down ? w : -w) *
-          (sum + ml[ku * s + o] * ds1 + 0.5 * dm[ku * s + o] * ds2 + m6[ku * s + o] * (ds2 / 2.0 - ds3 / 3.0)) / c;
+  double top = k0 == 0 ? ptf[o] : pt_face_flux(lim, col, s, n, k0, cflz[k0 * s + o], we[k0 * s + o], bad);
+  for (int k = k0; k < k1; k++) {
+    double bot;
+    if (k + 1 == n) {
+      bot = ptf[n * s + o];
+    } else {
+      bot = pt_face_flux(lim, col, s, n, k + 1, cflz[(k + 1) * s + o], we[(k + 1) * s + o], bad);
+      ptf[(k + 1) * s + o] = bot;
     }
-    ptf[k * s + o] = f;
+    dptf[k * s + o] = bot - top;
+    top = bot;
   }
-  for (int k = 0; k < n; k++) dptf[k * s + o] = ptf[(k + 1) * s + o] - ptf[k * s + o];
   if (bad) flag[0] = 1;
 }
 
@@ -321,10 +490,13 @@ struct gmc_baro {
   double *slab = nullptr;
   double *f[GMB_NFIELDS] = {};
   double *d_mesh = nullptr, *d_hyai = nullptr, *d_hybi = nullptr;
-  double *w_ext = nullptr, *w_ml = nullptr, *w_dm = nullptr, *w_m6 = nullptr;   // calc_ptf_lev work
-  int *d_flag = nullptr;
+  int *d_flag = nullptr;             // set by calc_ptf_lev where cflz leaves the column
+  bool flag_pending = false;         // a calc_ptf_lev launched since the flag was last read
   void *d_filter = nullptr;          // small_filter2's table on the device
   size_t filter_bytes = 0;
+  int32_t *f_ngrid = nullptr;        // its tap counts: U rows, then V rows
+  const double *f_wlon = nullptr, *f_wlat = nullptr;
+  int f_max_taps = 0, f_hmax = 0;    // 0: no table set
   hipStream_t stream = nullptr;
   HostStage stage;
   std::string err;
@@ -349,6 +521,20 @@ int levels_of(const gmc_baro *b, int f) {
 int rows_of(const gmc_baro *b, int f) { return kShape[f].vrows ? b->g.nlat - 1 : b->g.nlat; }
 
 dim3 columns(const BaroGrid &g, int rows) { return dim3((g.nlon + NB - 1) / NB, rows); }
+
+// a level-chunked launch: every point of `rows` rows of a level plane, the
+// levels in chunks of LC
+dim3 chunks(const BaroGrid &g, int rows, int levels) {
+  return dim3((unsigned)(((int64_t)rows * g.pitch + NB - 1) / NB), (levels + LC - 1) / LC);
+}
+
+// calc_gz_lev's columns per block: the widest power of two up to 64 whose
+// layer terms and half levels fit 64 KiB of LDS
+int gz_columns(int nlev) {
+  int cw = 64;
+  while (cw > 1 && 2 * (size_t)nlev * cw * sizeof(double) > 65536) cw /= 2;
+  return cw;
+}
 
 int launched(gmc_baro *b, const char *what) { return bhip(b, hipGetLastError(), what); }
 
@@ -384,7 +570,7 @@ int gmc_baro_create(const gmc_mesh *mesh, const gmc_vcoord *vc, const gmc_thermo
   size_t n = 0;
   for (int f = 0; f < GMB_NFIELDS; f++) n += (size_t)levels_of(b, f) * rows_of(b, f) * g.pitch;
   n += GMC_MESH_NFIELDS * (size_t)nlat + 2 * (size_t)(nlev + 1);
-  n += (size_t)(nlev + 4 + 3 * nlev) * g.sc + 4;   // column work of calc_ptf_lev, the flag
+  n += 4;                                          // calc_ptf_lev's flag
   int rc = bhip(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream");
   if (!rc) rc = bhip(b, hipMalloc((void **)&b->slab, n * sizeof(double)), "alloc");
   if (!rc) rc = bhip(b, hipMemset(b->slab, 0, n * sizeof(double)), "memset");
@@ -402,14 +588,6 @@ int gmc_baro_create(const gmc_mesh *mesh, const gmc_vcoord *vc, const gmc_thermo
   b->d_hyai = p;
   b->d_hybi = p + nlev + 1;
   p += 2 * (size_t)(nlev + 1);
-  b->w_ext = p;
-  p += (size_t)(nlev + 4) * g.sc;
-  b->w_ml = p;
-  p += (size_t)nlev * g.sc;
-  b->w_dm = p;
-  p += (size_t)nlev * g.sc;
-  b->w_m6 = p;
-  p += (size_t)nlev * g.sc;
   b->d_flag = (int *)p;
   rc = bhip(b, hipMemcpy(b->d_mesh, mesh->metrics, sizeof(double) * GMC_MESH_NFIELDS * nlat, hipMemcpyHostToDevice),
             "mesh");
@@ -455,7 +633,7 @@ int gmc_baro_get(gmc_baro *b, int32_t field, double *host, int64_t ld) {
 int gmc_baro_calc_ph(gmc_baro *b) {
   if (!b) return GMC_EINVAL;
   const BaroGrid &g = b->g;
-  hipLaunchKernelGGL(k_calc_ph, columns(g, g.nlat), dim3(NB), 0, b->stream, g, b->d_hyai, b->d_hybi, b->vp0, b->ptop,
+  hipLaunchKernelGGL(k_calc_ph, chunks(g, g.nlat, g.nlev), dim3(NB), 0, b->stream, g, b->d_hyai, b->d_hybi, b->vp0, b->ptop,
                      b->rd_o_cpd, b->f[GMB_PHS], b->f[GMB_PH_LEV], b->f[GMB_PH_EXN_LEV], b->f[GMB_PH]);
   return launched(b, "calc_ph");
 }
@@ -463,7 +641,7 @@ int gmc_baro_calc_ph(gmc_baro *b) {
 int gmc_baro_calc_t(gmc_baro *b) {
   if (!b) return GMC_EINVAL;
   const BaroGrid &g = b->g;
-  hipLaunchKernelGGL(k_calc_t, columns(g, g.nlat), dim3(NB), 0, b->stream, g, b->tp0, b->rd_o_cpd, b->f[GMB_PT],
+  hipLaunchKernelGGL(k_calc_t, chunks(g, g.nlat, g.nlev), dim3(NB), 0, b->stream, g, b->tp0, b->rd_o_cpd, b->f[GMB_PT],
                      b->f[GMB_PH], b->f[GMB_T]);
   return launched(b, "calc_t");
 }
@@ -471,8 +649,10 @@ int gmc_baro_calc_t(gmc_baro *b) {
 int gmc_baro_calc_gz_lev(gmc_baro *b) {
   if (!b) return GMC_EINVAL;
   const BaroGrid &g = b->g;
-  hipLaunchKernelGGL(k_calc_gz_lev, columns(g, g.nlat), dim3(NB), 0, b->stream, g, b->Rd, b->f[GMB_GZS], b->f[GMB_T],
-                     b->f[GMB_PH_LEV], b->f[GMB_GZ_LEV], b->f[GMB_GZ]);
+  const int cw = gz_columns(g.nlev);
+  const unsigned blocks = (unsigned)(((int64_t)g.nlat * g.pitch + cw - 1) / cw);
+  hipLaunchKernelGGL(k_calc_gz_lev, dim3(blocks), dim3(NB), 2 * (size_t)g.nlev * cw * sizeof(double), b->stream, g,
+                     cw, b->Rd, b->f[GMB_GZS], b->f[GMB_T], b->f[GMB_PH_LEV], b->f[GMB_GZ_LEV], b->f[GMB_GZ]);
   return launched(b, "calc_gz_lev");
 }
 
@@ -489,7 +669,7 @@ int gmc_baro_calc_we_lev(gmc_baro *b) {
   const BaroGrid &g = b->g;
   hipLaunchKernelGGL(k_calc_we_lev, columns(g, g.nlat), dim3(NB), 0, b->stream, g, b->d_hybi, b->f[GMB_DMFDLON],
                      b->f[GMB_DMFDLAT], b->f[GMB_DPHS], b->f[GMB_WE_LEV]);
-  hipLaunchKernelGGL(k_we_edges, columns(g, 2 * g.nlat - 3), dim3(NB), 0, b->stream, g, Mesh{b->d_mesh, g.nlat},
+  hipLaunchKernelGGL(k_we_edges, chunks(g, 2 * g.nlat - 3, g.nlev + 1), dim3(NB), 0, b->stream, g, Mesh{b->d_mesh, g.nlat},
                      b->f[GMB_WE_LEV], b->f[GMB_WE_LEV_LON], b->f[GMB_WE_LEV_LAT]);
   return launched(b, "calc_we_lev");
 }
@@ -497,7 +677,7 @@ int gmc_baro_calc_we_lev(gmc_baro *b) {
 int gmc_baro_calc_wedudlev_wedvdlev(gmc_baro *b) {
   if (!b) return GMC_EINVAL;
   const BaroGrid &g = b->g;
-  hipLaunchKernelGGL(k_wedudlev, columns(g, 2 * g.nlat - 3), dim3(NB), 0, b->stream, g, b->f[GMB_U_F],
+  hipLaunchKernelGGL(k_wedudlev, chunks(g, 2 * g.nlat - 3, g.nlev), dim3(NB), 0, b->stream, g, b->f[GMB_U_F],
                      b->f[GMB_WE_LEV_LON], b->f[GMB_M_LON], b->f[GMB_WEDUDLEV], b->f[GMB_V_F], b->f[GMB_WE_LEV_LAT],
                      b->f[GMB_M_LAT], b->f[GMB_WEDVDLEV]);
   return launched(b, "calc_wedudlev_wedvdlev");
@@ -506,7 +686,7 @@ int gmc_baro_calc_wedudlev_wedvdlev(gmc_baro *b) {
 int gmc_baro_pgf_lin97(gmc_baro *b) {
   if (!b) return GMC_EINVAL;
   const BaroGrid &g = b->g;
-  hipLaunchKernelGGL(k_pgf_lin97, columns(g, 2 * g.nlat - 3), dim3(NB), 0, b->stream, g, Mesh{b->d_mesh, g.nlat},
+  hipLaunchKernelGGL(k_pgf_lin97, chunks(g, 2 * g.nlat - 3, g.nlev), dim3(NB), 0, b->stream, g, Mesh{b->d_mesh, g.nlat},
                      b->f[GMB_PH_EXN_LEV], b->f[GMB_GZ_LEV], b->f[GMB_PGF_LON], b->f[GMB_PGF_LAT]);
   return launched(b, "pgf_lin97");
 }
@@ -515,20 +695,14 @@ int gmc_baro_calc_ptf_lev(gmc_baro *b, int32_t limiter) {
   if (!b) return GMC_EINVAL;
   if (limiter < GMC_LIMITER_MONO || limiter > GMC_LIMITER_NONE) return berr(b, GMC_EINVAL, "bad limiter");
   const BaroGrid &g = b->g;
-  int rc = bhip(b, hipMemsetAsync(b->d_flag, 0, sizeof(int), b->stream), "flag clear");
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_vflx_pt, columns(g, g.nlat), dim3(NB), 0, b->stream, g, (int)limiter, b->f[GMB_PT],
-                     b->f[GMB_WE_LEV], b->f[GMB_CFLZ], b->w_ext, b->w_ml, b->w_dm, b->w_m6, b->f[GMB_PTF_LEV],
-                     b->f[GMB_DPTFDLEV], b->d_flag);
-  rc = launched(b, "calc_ptf_lev");
-  int h = 0;
-  if (!rc) rc = bhip(b, hipMemcpyAsync(&h, b->d_flag, sizeof(int), hipMemcpyDeviceToHost, b->stream), "flag");
-  if (!rc) rc = bhip(b, hipStreamSynchronize(b->stream), "sync");
-  if (!rc && h) rc = berr(b, GMC_ENUMERIC, "cflz reaches outside the column");
+  hipLaunchKernelGGL(k_vflx_pt, chunks(g, g.nlat, g.nlev), dim3(NB), 0, b->stream, g, (int)limiter, b->f[GMB_PT],
+                     b->f[GMB_WE_LEV], b->f[GMB_CFLZ], b->f[GMB_PTF_LEV], b->f[GMB_DPTFDLEV], b->d_flag);
+  const int rc = launched(b, "calc_ptf_lev");
+  if (!rc) b->flag_pending = true;
   return rc;
 }
 
-int gmc_baro_small_filter2(gmc_baro *b, const gmc_filter *f) {
+int gmc_baro_set_small_filter2(gmc_baro *b, const gmc_filter *f) {
   if (!b) return GMC_EINVAL;
   const BaroGrid &g = b->g;
   const int nlat = g.nlat, nhalf = nlat - 1;
@@ -543,16 +717,17 @@ int gmc_baro_small_filter2(gmc_baro *b, const gmc_filter *f) {
     if (f->ngrid_lat[r] > g.nlon || f->ngrid_lat[r] > f->max_taps) return berr(b, GMC_EINVAL, "bad filter taps");
     hmax = std::max(hmax, (f->ngrid_lat[r] - 1) / 2);
   }
+  if (sizeof(double) * (g.nlon + 2 * (size_t)hmax) > 65536) return berr(b, GMC_EINVAL, "filter row does not fit in LDS");
   const size_t ni = sizeof(int32_t) * (nlat + nhalf), nw = sizeof(double) * (size_t)f->max_taps * (nlat + nhalf);
   const size_t need = (ni + 255) / 256 * 256 + nw;
-  int rc = GMC_OK;
+  // the launches that read the old table finish first
+  int rc = bhip(b, hipStreamSynchronize(b->stream), "sync");
+  if (rc) return rc;
+  b->f_max_taps = 0;
   if (need > b->filter_bytes) {
-    if (b->d_filter) {
-      rc = bhip(b, hipStreamSynchronize(b->stream), "sync");
-      if (!rc) rc = bhip(b, hipFree(b->d_filter), "filter free");
-      b->d_filter = nullptr;
-      b->filter_bytes = 0;
-    }
+    if (b->d_filter) rc = bhip(b, hipFree(b->d_filter), "filter free");
+    b->d_filter = nullptr;
+    b->filter_bytes = 0;
     if (!rc) rc = bhip(b, hipMalloc(&b->d_filter, need), "filter alloc");
     if (rc) return rc;
     b->filter_bytes = need;
@@ -566,18 +741,38 @@ int gmc_baro_small_filter2(gmc_baro *b, const gmc_filter *f) {
   if (!rc) rc = bhip(b, hipMemcpy((char *)w + wl, f->wgt_lat, sizeof(double) * (size_t)f->max_taps * nhalf,
                                   hipMemcpyHostToDevice), "filter copy");
   if (rc) return rc;
-  const size_t lds = sizeof(double) * (g.nlon + 2 * (size_t)hmax);
-  if (lds > 65536) return berr(b, GMC_EINVAL, "filter row does not fit in LDS");
-  hipLaunchKernelGGL(k_filter_levels, dim3(nlat, g.nlev), dim3(NB), lds, b->stream, g, nlat, g.sc, ng, w,
-                     f->max_taps, b->f[GMB_U_LON]);
-  hipLaunchKernelGGL(k_filter_levels, dim3(nhalf, g.nlev), dim3(NB), lds, b->stream, g, nhalf, g.sv, ng + nlat,
-                     (const double *)((const char *)w + wl), f->max_taps, b->f[GMB_V_LAT]);
+  b->f_ngrid = ng;
+  b->f_wlon = w;
+  b->f_wlat = (const double *)((const char *)w + wl);
+  b->f_hmax = hmax;
+  b->f_max_taps = f->max_taps;
+  return GMC_OK;
+}
+
+int gmc_baro_small_filter2(gmc_baro *b) {
+  if (!b) return GMC_EINVAL;
+  if (!b->f_max_taps) return berr(b, GMC_EINVAL, "no small_filter2 table set");
+  const BaroGrid &g = b->g;
+  const size_t lds = sizeof(double) * (g.nlon + 2 * (size_t)b->f_hmax);
+  hipLaunchKernelGGL(k_filter_levels, dim3(g.nlat, g.nlev), dim3(NB), lds, b->stream, g, g.nlat, g.sc, b->f_ngrid,
+                     b->f_wlon, b->f_max_taps, b->f[GMB_U_LON]);
+  hipLaunchKernelGGL(k_filter_levels, dim3(g.nlat - 1, g.nlev), dim3(NB), lds, b->stream, g, g.nlat - 1, g.sv,
+                     b->f_ngrid + g.nlat, b->f_wlat, b->f_max_taps, b->f[GMB_V_LAT]);
   return launched(b, "small_filter2");
 }
 
 int gmc_baro_synchronize(gmc_baro *b) {
   if (!b) return GMC_EINVAL;
-  return bhip(b, hipStreamSynchronize(b->stream), "synchronize");
+  int rc = bhip(b, hipStreamSynchronize(b->stream), "synchronize");
+  if (rc || !b->flag_pending) return rc;
+  int h = 0;
+  rc = bhip(b, hipMemcpy(&h, b->d_flag, sizeof(int), hipMemcpyDeviceToHost), "flag");
+  if (rc) return rc;
+  b->flag_pending = false;
+  if (!h) return GMC_OK;
+  const int zero = 0;
+  rc = bhip(b, hipMemcpy(b->d_flag, &zero, sizeof(int), hipMemcpyHostToDevice), "flag clear");
+  return rc ? rc : berr(b, GMC_ENUMERIC, "cflz reaches outside the column");
 }
 
 int gmc_baro_stream(gmc_baro *b, void **stream) {

@@ -225,8 +225,11 @@ module gmc_hip_mod
     integer(c_int) function gmc_baro_calc_ptf_lev(baro, limiter) bind(C, name='gmc_baro_calc_ptf_lev')
       import; type(c_ptr), value :: baro; integer(c_int32_t), value :: limiter
     end function
-    integer(c_int) function gmc_baro_small_filter2(baro, filter) bind(C, name='gmc_baro_small_filter2')
+    integer(c_int) function gmc_baro_set_small_filter2(baro, filter) bind(C, name='gmc_baro_set_small_filter2')
       import; type(c_ptr), value :: baro; type(gmc_filter), intent(in) :: filter
+    end function
+    integer(c_int) function gmc_baro_small_filter2(baro) bind(C, name='gmc_baro_small_filter2')
+      import; type(c_ptr), value :: baro
     end function
     integer(c_int) function gmc_baro_synchronize(baro) bind(C, name='gmc_baro_synchronize')
       import; type(c_ptr), value :: baro

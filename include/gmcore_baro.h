@@ -130,21 +130,28 @@ int gmc_baro_pgf_lin97(gmc_baro *b);
  * the limiter GMC_LIMITER_* of include/gmcore_adv.h) through the interior
  * half levels with the Courant numbers cflz and the mass flux we_lev (the
  * advection batch's we), into ptf_lev, and dptfdlev = ptf_lev(k+1) -
- * ptf_lev(k).  ptf_lev's top and bottom faces are not written.  A Courant
- * number whose upwind cell falls outside the column gives GMC_ENUMERIC (the
- * reference reads beyond its arrays there); the call waits for the result. */
+ * ptf_lev(k).  ptf_lev's top and bottom faces are not written.  The call
+ * only queues the work; a Courant number whose upwind cell falls outside the
+ * column (the reference reads beyond its arrays there) makes the next
+ * gmc_baro_synchronize return GMC_ENUMERIC. */
 int gmc_baro_calc_ptf_lev(gmc_baro *b, int32_t limiter);
 
+/* The table `small_filter2` of update_state's small_filter2 step (filter_init's
+ * 'small_filter2', filter_types_mod.F90:124-143: the big filter's widths
+ * times exp_two_values(filter_coef_e, filter_coef_d, lat0, 80, |lat|)),
+ * copied to the device once, as filter_init builds it once per run; a later
+ * call replaces it (after the queued work that reads it). */
+int gmc_baro_set_small_filter2(gmc_baro *b, const gmc_filter *small_filter2);
 /* The baroclinic branch of update_state's small_filter2 step
  * (time_schemes_mod.F90:213-221): filter_on_lon_edge / filter_on_lat_edge
- * (filter_mod.F90:93-163) with the table `small_filter2` (filter_init's
- * 'small_filter2', filter_types_mod.F90:124-143: the big filter's widths
- * times exp_two_values(filter_coef_e, filter_coef_d, lat0, 80, |lat|)), in
- * place on every level of u_lon and v_lat; rows with at most one tap are
- * left as they are. */
-int gmc_baro_small_filter2(gmc_baro *b, const gmc_filter *small_filter2);
+ * (filter_mod.F90:93-163) with the table set above, in place on every level
+ * of u_lon and v_lat; rows with at most one tap are left as they are.
+ * GMC_EINVAL when no table is set. */
+int gmc_baro_small_filter2(gmc_baro *b);
 
-/* Wait for the block's queued work; its stream (hipStream_t). */
+/* Wait for the block's queued work (GMC_ENUMERIC if a calc_ptf_lev since the
+ * last call met a Courant number beyond the column); its stream
+ * (hipStream_t). */
 int gmc_baro_synchronize(gmc_baro *b);
 int gmc_baro_stream(gmc_baro *b, void **stream);
 

@@ -205,6 +205,26 @@ def test_calc_ptf_lev_reports_courant_beyond_column(env):
     g.set("cflz", c)
     with pytest.raises(Exception, match="outside the column"):
         g.calc_ptf_lev()
+    g.synchronize()                   # reported once
+
+
+def test_calc_ptf_lev_error_reported_by_synchronize(env):
+    """Without the check the call only queues the work: the next synchronize
+    reports the Courant number below the model bottom, the one after is
+    clean, and a clean launch in between does not hide it."""
+    g = gpu(env)
+    c = np.zeros((NLEV + 1, NLAT, NLON))
+    c[NLEV - 1, 3, 4] = -1.5
+    g.set("pt", np.full((NLEV, NLAT, NLON), 300.0))
+    g.set("we_lev", np.ones_like(c))
+    g.set("cflz", c)
+    g.calc_ptf_lev(check=False)
+    g.set("cflz", np.zeros_like(c))
+    g.calc_ptf_lev(check=False)
+    with pytest.raises(Exception, match="outside the column"):
+        g.synchronize()
+    g.synchronize()
+    g.calc_ptf_lev()
 
 
 def test_small_filter2_bitwise(env):
@@ -223,3 +243,77 @@ def test_small_filter2_bitwise(env):
     np.testing.assert_array_equal(g.get("u_lon"), fu)
     np.testing.assert_array_equal(g.get("v_lat"), fv)
     assert not np.array_equal(fu, u)
+
+
+def test_small_filter2_needs_a_table(env):
+    g = gpu(env)
+    with pytest.raises(Exception, match="no small_filter2 table"):
+        g.small_filter2()
+
+
+def test_small_filter2_table_set_once(env):
+    """The table set once serves every later call, as filter_init's does."""
+    from gmcore_amd.host import Filter
+    from gmcore_amd.namelist import Namelist
+    mesh, _, _, o = env
+    table = Filter(mesh, 600.0, Namelist(num_lon=NLON, num_lat=NLAT, dt_dyn=600.0), 2)
+    g = gpu(env)
+    g.set_small_filter2(table)
+    r = rng(12)
+    u, v = 20 * r.standard_normal((NLEV, NLAT, NLON)), 20 * r.standard_normal((NLEV, NLAT - 1, NLON))
+    g.set("u_lon", u)
+    g.set("v_lat", v)
+    g.small_filter2()
+    g.small_filter2()
+    t = (table.ngrid_lon, table.wgt_lon, table.ngrid_lat, table.wgt_lat)
+    fu, fv = o.small_filter2(*o.small_filter2(u, v, t), t)
+    np.testing.assert_array_equal(g.get("u_lon"), fu)
+    np.testing.assert_array_equal(g.get("v_lat"), fv)
+
+
+@pytest.mark.parametrize("nlev", [2, 5, 137, 300])
+def test_level_counts(nlev):
+    """Level counts off the chunk size (2, 5), a 137-level column (the
+    geopotential sums' LDS block narrows to 16 columns) and 300 levels (8
+    columns): the hydrostatic chain and the vertical flux bit for bit or to
+    the chain's bar on a small mesh."""
+    nlon, nlat = 36, 19
+    mesh = Mesh(nlon, nlat, 6371220.0, 7.292e-5)
+    hyai, hybi = oracle.equal_eta(nlev)
+    o = oracle.BaroOracle(mesh.metrics, nlon, nlat, hyai, hybi, P0, PTOP, RD, CPD, P0)
+    g = Baroclinic(mesh, hyai, hybi, P0, PTOP, RD, CPD, P0)
+    r = rng(13)
+    phs = 1.0e5 + 3000.0 * r.standard_normal((1, nlat, nlon))
+    theta = 300.0 + 60.0 * np.linspace(1.0, 0.0, nlev)[:, None, None] + r.standard_normal((nlev, nlat, nlon))
+    gzs = 2000.0 * r.random((1, nlat, nlon))
+    bottom = np.zeros((nlev + 1, nlat, nlon))
+    bottom[-1] = gzs[0]
+    for name, a in (("phs", phs), ("pt", theta), ("gzs", gzs), ("gz_lev", bottom)):
+        g.set(name, a)
+    g.calc_ph()
+    g.calc_t()
+    g.calc_gz_lev()
+    g.pgf_lin97()
+    ph_lev, exn, ph = o.calc_ph(phs)
+    np.testing.assert_array_equal(g.get("ph_lev"), ph_lev)
+    np.testing.assert_array_equal(g.get("ph"), ph)
+    t = o.calc_t(theta, ph)
+    gz_lev, gz = o.calc_gz_lev(gzs, t, ph_lev, bottom)
+    pl, pt = o.pgf_lin97(exn, gz_lev)
+    for name, want in (("t", t), ("gz_lev", gz_lev), ("gz", gz), ("pgf_lon", pl), ("pgf_lat", pt)):
+        assert rel(g.get(name), want) < 1e-12, name
+    # the geopotential sums alone, on the oracle's own temperature: the
+    # ascending order of every half level's sum kept
+    g.set("t", t)
+    g.set("ph_lev", ph_lev)
+    g.calc_gz_lev()
+    assert rel(g.get("gz_lev"), gz_lev) < 1e-13
+    w = r.standard_normal((nlev + 1, nlat, nlon))
+    c = r.uniform(-0.95, 0.95, (nlev + 1, nlat, nlon))
+    for name, a in (("pt", theta), ("we_lev", w), ("cflz", c)):
+        g.set(name, a)
+    g.calc_ptf_lev()
+    ptf, dptf, bad = o.calc_ptf_lev(theta, w, c, 0)
+    assert bad == 0
+    np.testing.assert_array_equal(g.get("ptf_lev"), ptf)
+    np.testing.assert_array_equal(g.get("dptfdlev"), dptf)
