@@ -97,7 +97,7 @@ BARO_CONFIGS = {
 BARO_FIELDS_PER_OP = {
     "calc_ph": (1, 2, 1), "calc_t": (3, 0, 0), "calc_gz_lev": (3, 1, 1), "calc_dphsdt": (2, 0, 1),
     "calc_we_lev": (2, 4, 1), "calc_wedudlev_wedvdlev": (6, 2, 0), "pgf_lin97": (2, 2, 0),
-    "calc_ptf_lev": (2, 3, 0), "small_filter2": (4, 0, 0),
+    "calc_ptf_lev": (2, 3, 0), "calc_ptf_hflx": (13, 0, 0), "small_filter2": (4, 0, 0),
 }
 
 CONFIGS = {
@@ -578,11 +578,21 @@ def run_baro(args):
     g.set("m_lat", 1e3 + 4e3 * rng.random((nlev, nlat - 1, nlon)))
     c = rng.uniform(-0.9, 0.9, (nlev + 1, nlat, nlon))
     g.set("cflz", c)
+    # the mass batch of calc_grad_ptf's horizontal fluxes: |cflx| up to 2.9
+    # (whole cells crossed) within lon_halo_width 3, |cfly| < 0.9
+    shp, shv = (nlev, nlat, nlon), (nlev, nlat - 1, nlon)
+    for name, a in (("adv_u", 20 * rng.standard_normal(shp)), ("adv_v", 20 * rng.standard_normal(shv)),
+                    ("adv_mfx", 1e4 * rng.standard_normal(shp)), ("adv_mfy", 1e4 * rng.standard_normal(shv)),
+                    ("adv_cflx", rng.uniform(-2.9, 2.9, shp)), ("adv_cfly", rng.uniform(-0.9, 0.9, shv)),
+                    ("adv_divx", 1e-5 * rng.standard_normal(shp)), ("adv_divy", 1e-5 * rng.standard_normal(shp))):
+        g.set(name, a)
     table = Filter(mesh, 300.0, Namelist(num_lon=nlon, num_lat=nlat, dt_dyn=300.0), 2)
     ops = [("calc_ph", g.calc_ph), ("calc_t", g.calc_t), ("calc_gz_lev", g.calc_gz_lev),
            ("calc_dphsdt", g.calc_dphsdt), ("calc_we_lev", g.calc_we_lev),
            ("calc_wedudlev_wedvdlev", g.calc_wedudlev_wedvdlev), ("pgf_lin97", g.pgf_lin97),
-           ("calc_ptf_lev", lambda: g.calc_ptf_lev(check=False)), ("small_filter2", g.small_filter2)]
+           ("calc_ptf_lev", lambda: g.calc_ptf_lev(check=False)),
+           ("calc_ptf_hflx", lambda: g.calc_ptf_hflx(300.0, lon_halo_width=3, check=False)),
+           ("small_filter2", g.small_filter2)]
     g.set_small_filter2(table)
     hip = C.CDLL("libamdhip64.so")
     stream = C.c_void_p()
@@ -609,6 +619,7 @@ def run_baro(args):
         nbytes = 8.0 * cols * (full * nlev + half * (nlev + 1) + one)
         kernels.append({"name": name, "avg_us": best * 1e3, "bytes": nbytes,
                         "gbs": nbytes / (best * 1e-3) / 1e9})
+    g.synchronize()   # raises if a timed launch met a Courant number out of range
     seq = ("calc_ph", "calc_t", "calc_gz_lev", "pgf_lin97")
     t_seq = sum(k["avg_us"] for k in kernels if k["name"] in seq)
     dom = max(kernels, key=lambda k: k["avg_us"])

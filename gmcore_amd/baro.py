@@ -97,6 +97,24 @@ class Baroclinic:
         if check:
             self.synchronize()
 
+    def calc_ptf_hflx(self, dt: float, limiter: str = "mono", flux_type: str = "ppm", lon_halo_width: int = 1,
+                      check: bool = True):
+        """The horizontal part of calc_grad_ptf: ptf_lon, ptf_lat, dptfdlon,
+        dptfdlat from pt and the mass batch's adv_* fields.  With check,
+        wait and raise BaroError if a cflx exceeded lon_halo_width."""
+        self._check(self.lib.gmc_baro_calc_ptf_hflx(self.h, capi.LIMITERS[limiter], capi.FLUX_TYPES[flux_type],
+                                                    lon_halo_width, dt), "calc_ptf_hflx")
+        if check:
+            self.synchronize()
+
+    def calc_grad_ptf(self, dt: float, limiter: str = "mono", flux_type: str = "ppm", lon_halo_width: int = 1,
+                      check: bool = True):
+        """calc_grad_ptf: the horizontal fluxes, then the vertical ones."""
+        self._check(self.lib.gmc_baro_calc_grad_ptf(self.h, capi.LIMITERS[limiter], capi.FLUX_TYPES[flux_type],
+                                                    lon_halo_width, dt), "calc_grad_ptf")
+        if check:
+            self.synchronize()
+
     def set_small_filter2(self, table):
         """Copy the small_filter2 table (a host.Filter of kind 2) to the
         device, once per run as filter_init builds it."""

@@ -109,6 +109,9 @@ BARO_FIELDS = {
     "we_lev_lat": (2, True), "u_f": (1, False), "v_f": (1, True), "m_lon": (1, False), "m_lat": (1, True),
     "wedudlev": (1, False), "wedvdlev": (1, True), "pgf_lon": (1, False), "pgf_lat": (1, True),
     "cflz": (2, False), "ptf_lev": (2, False), "dptfdlev": (1, False), "u_lon": (1, False), "v_lat": (1, True),
+    "adv_u": (1, False), "adv_v": (1, True), "adv_mfx": (1, False), "adv_mfy": (1, True),
+    "adv_cflx": (1, False), "adv_cfly": (1, True), "adv_divx": (1, False), "adv_divy": (1, False),
+    "ptf_lon": (1, False), "ptf_lat": (1, True), "dptfdlon": (1, False), "dptfdlat": (1, False),
 }
 BARO_FIELD = {name: k for k, name in enumerate(BARO_FIELDS)}
 
@@ -215,6 +218,8 @@ def load_hip() -> C.CDLL:
         "gmc_baro_calc_wedudlev_wedvdlev": (C.c_int, [vp]),
         "gmc_baro_pgf_lin97": (C.c_int, [vp]),
         "gmc_baro_calc_ptf_lev": (C.c_int, [vp, C.c_int32]),
+        "gmc_baro_calc_ptf_hflx": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_double]),
+        "gmc_baro_calc_grad_ptf": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_double]),
         "gmc_baro_set_small_filter2": (C.c_int, [vp, P(GmcFilter)]),
         "gmc_baro_small_filter2": (C.c_int, [vp]),
         "gmc_baro_synchronize": (C.c_int, [vp]),

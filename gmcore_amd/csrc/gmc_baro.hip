@@ -20,6 +20,7 @@
 
 #include "gmc_internal.hpp"
 #include "gmc_point.hpp"
+#include "gmc_hflx.hpp"
 #include "gmc_ppm.hpp"
 #include "../../include/gmcore_baro.h"
 
@@ -48,9 +49,13 @@ constexpr FieldShape kShape[GMB_NFIELDS] = {
     {1, false}, {1, true}, {1, false}, {1, true},        // u_f, v_f, m_lon, m_lat
     {1, false}, {1, true}, {1, false}, {1, true},        // wedudlev, wedvdlev, pgf_lon, pgf_lat
     {2, false}, {2, false}, {1, false},                  // cflz, ptf_lev, dptfdlev
-    {1, false}, {1, true}};                              // u_lon, v_lat
+    {1, false}, {1, true},                               // u_lon, v_lat
+    {1, false}, {1, true}, {1, false}, {1, true},        // adv_u, adv_v, adv_mfx, adv_mfy
+    {1, false}, {1, true}, {1, false}, {1, false},       // adv_cflx, adv_cfly, adv_divx, adv_divy
+    {1, false}, {1, true}, {1, false}, {1, false}};      // ptf_lon, ptf_lat, dptfdlon, dptfdlat
 
 constexpr int LC = 4;     // levels per thread of the level-chunked kernels
+enum { kFlagCflz = 1, kFlagCflx = 2 };   // bits of the error flag: Courant numbers out of range
 
 // the point of this thread in a level-chunked launch over `rows` rows of a
 // level plane: o = r * pitch + i; false in the pitch padding and past the
@@ -452,7 +457,105 @@ __global__ void __launch_bounds__(NB) k_vflx_pt(BaroGrid g, int lim, const doubl
     dptf[k * s + o] = bot - top;
     top = bot;
   }
-  if (bad) flag[0] = 1;
+  if (bad) atomicOr(flag, kFlagCflz);
+}
+
+// The horizontal part of calc_grad_ptf (operators_mod.F90:1029-1080):
+// ffsl_calc_tracer_hflx_cell (ffsl_mod.F90:318-413) of pt with the mass
+// batch's fields on every level, then the flux divergence.  A thread owns one
+// point of one level (grid y: the level); the face fluxes are those of the
+// tracer advection (gmc_hflx.hpp).
+
+// x-fluxes of mx through the U points of rows 1 .. nlat-2 and y-fluxes of my
+// through every V point, with the winds u / v and Courant numbers cflx / cfly
+// (hflx_ppm_cell / hflx_van_leer_cell, ffsl_mod.F90:556-616, :778-881).  A
+// cflx beyond lon_halo_width sets kFlagCflx (adv_batch_mod.F90:284-287); its
+// whole-cell count is clamped to the bound so that every access stays in the
+// row.
+template <bool VL>
+__global__ void __launch_bounds__(NB) k_hflx_levels(BaroGrid g, AdvGrid ag, int lim, const double *__restrict__ u,
+                                                    const double *__restrict__ v, const double *__restrict__ cflx,
+                                                    const double *__restrict__ cfly, const double *__restrict__ mx,
+                                                    const double *__restrict__ my, double *fx, double *fy, int *flag) {
+  int64_t o;
+  int i, y, r;
+  if (!plane_point(g, 2 * g.nlat - 3, o, i, y)) return;
+  const int64_t k = blockIdx.y;
+  if (u_or_v_row(g, y, r)) {
+    const int64_t ro = k * g.sc + r * g.pitch;
+    const double c = cflx[ro + i];
+    if (!(fabs(c) <= ag.hw)) atomicOr(flag, kFlagCflx);
+    fx[ro + i] = xflux<VL>(ag, lim, c, u[ro + i], i, mx + ro);
+  } else {
+    const int64_t lo = k * g.sv + r * g.pitch + i;
+    fy[lo] = yflux_any<VL>(ag, lim, cfly[lo], v[lo], r, i, my + k * g.sc);
+  }
+}
+
+// the advective half-steps qx, qy of the cells off the poles
+// (ffsl_mod.F90:354-373) from the inner fluxes qmfx, qmfy
+__global__ void __launch_bounds__(NB) k_half_steps(BaroGrid g, Mesh M, double dt, const double *__restrict__ q,
+                                                   const double *__restrict__ qmfx, const double *__restrict__ qmfy,
+                                                   const double *__restrict__ divx, const double *__restrict__ divy,
+                                                   double *qx, double *qy) {
+  int64_t o;
+  int i, r;
+  if (!plane_point(g, g.nlat, o, i, r) || r == 0 || r == g.nlat - 1) return;
+  const int64_t c = blockIdx.y * g.sc + o, cv = blockIdx.y * g.sv + o;
+  const int iw = i == 0 ? g.nlon - 1 : i - 1;
+  const double qc = q[c];
+  const double fx = qmfx[c], fxw = qmfx[c - i + iw], fn = qmfy[cv], fs = qmfy[cv - g.pitch];
+  const double lel = M(GMC_LE_LON, r), ac = M(GMC_AREA_CELL, r), ln = M(GMC_LE_LAT, r), ls = M(GMC_LE_LAT, r - 1);
+  qx[c] = qc - 0.5 * ((fx - fxw) * lel / ac - divx[c] * qc) * dt;
+  qy[c] = qc - 0.5 * ((fn * ln - fs * ls) / ac - divy[c] * qc) * dt;
+}
+
+// the pole rows' half-steps (ffsl_mod.F90:374-406): qx = q, qy from the
+// zonal sum of the adjacent V row's inner flux; one wavefront per (pole,
+// level), the sum in the library's fixed order
+__global__ void __launch_bounds__(kWave) k_half_steps_pole(BaroGrid g, Mesh M, double dt,
+                                                           const double *__restrict__ q,
+                                                           const double *__restrict__ qmfy,
+                                                           const double *__restrict__ divy, double *qx, double *qy) {
+  const bool north = blockIdx.x == 1;
+  const int r = north ? g.nlat - 1 : 0, h = north ? g.nlat - 2 : 0;
+  const int64_t k = blockIdx.y;
+  const double *fy = qmfy + k * g.sv + h * g.pitch;
+  double pole = wave_zonal_sum(g.nlon, threadIdx.x, [&](int i) { return fy[i]; });
+  pole = pole * M(GMC_LE_LAT, h) / g.nlon / M(GMC_AREA_CELL, r);
+  for (int i = threadIdx.x; i < g.nlon; i += kWave) {
+    const int64_t c = k * g.sc + r * g.pitch + i;
+    const double qc = q[c];
+    qx[c] = qc;
+    qy[c] = north ? qc + 0.5 * (pole - divy[c] * qc) * dt : qc - 0.5 * (pole - divy[c] * qc) * dt;
+  }
+}
+
+// dptfdlon and dptfdlat of the cells off the poles (operators_mod.F90:1034-1051)
+__global__ void __launch_bounds__(NB) k_ptf_div(BaroGrid g, Mesh M, const double *__restrict__ fxl,
+                                                const double *__restrict__ fyl, double *dx, double *dy) {
+  int64_t o;
+  int i, r;
+  if (!plane_point(g, g.nlat, o, i, r) || r == 0 || r == g.nlat - 1) return;
+  const int64_t c = blockIdx.y * g.sc + o, cv = blockIdx.y * g.sv + o;
+  const int iw = i == 0 ? g.nlon - 1 : i - 1;
+  const double ac = M(GMC_AREA_CELL, r);
+  dx[c] = (fxl[c] - fxl[c - i + iw]) * M(GMC_LE_LON, r) / ac;
+  dy[c] = (fyl[cv] * M(GMC_LE_LAT, r) - fyl[cv - g.pitch] * M(GMC_LE_LAT, r - 1)) / ac;
+}
+
+// dptfdlat of the pole rows (operators_mod.F90:1052-1079): the zonal sum of
+// the adjacent ptf_lat row (negated at the north pole), one wavefront per
+// (pole, level)
+__global__ void __launch_bounds__(kWave) k_ptf_div_pole(BaroGrid g, Mesh M, const double *__restrict__ fyl,
+                                                        double *dy) {
+  const bool north = blockIdx.x == 1;
+  const int r = north ? g.nlat - 1 : 0, h = north ? g.nlat - 2 : 0;
+  const int64_t k = blockIdx.y;
+  const double *fy = fyl + k * g.sv + h * g.pitch;
+  double pole = wave_zonal_sum(g.nlon, threadIdx.x, [&](int i) { return north ? -fy[i] : fy[i]; });
+  pole = pole * M(GMC_LE_LAT, h) / g.nlon / M(GMC_AREA_CELL, r);
+  for (int i = threadIdx.x; i < g.nlon; i += kWave) dy[k * g.sc + r * g.pitch + i] = pole;
 }
 
 // filter_on_lon_edge / filter_on_lat_edge (filter_mod.F90:93-163) in place:
@@ -490,8 +593,9 @@ struct gmc_baro {
   double *slab = nullptr;
   double *f[GMB_NFIELDS] = {};
   double *d_mesh = nullptr, *d_hyai = nullptr, *d_hybi = nullptr;
-  int *d_flag = nullptr;             // set by calc_ptf_lev where cflz leaves the column
-  bool flag_pending = false;         // a calc_ptf_lev launched since the flag was last read
+  double *w_qmfx = nullptr, *w_qmfy = nullptr, *w_qx = nullptr, *w_qy = nullptr;   // calc_ptf_hflx work
+  int *d_flag = nullptr;             // error bits of calc_ptf_lev / calc_ptf_hflx
+  bool flag_pending = false;         // one of them launched since the flag was last read
   void *d_filter = nullptr;          // small_filter2's table on the device
   size_t filter_bytes = 0;
   int32_t *f_ngrid = nullptr;        // its tap counts: U rows, then V rows
@@ -570,7 +674,8 @@ int gmc_baro_create(const gmc_mesh *mesh, const gmc_vcoord *vc, const gmc_thermo
   size_t n = 0;
   for (int f = 0; f < GMB_NFIELDS; f++) n += (size_t)levels_of(b, f) * rows_of(b, f) * g.pitch;
   n += GMC_MESH_NFIELDS * (size_t)nlat + 2 * (size_t)(nlev + 1);
-  n += 4;                                          // calc_ptf_lev's flag
+  n += (size_t)nlev * (3 * g.sc + g.sv);          // calc_ptf_hflx's work: qmfx, qx, qy; qmfy
+  n += 4;                                          // the error flag
   int rc = bhip(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream");
   if (!rc) rc = bhip(b, hipMalloc((void **)&b->slab, n * sizeof(double)), "alloc");
   if (!rc) rc = bhip(b, hipMemset(b->slab, 0, n * sizeof(double)), "memset");
@@ -588,6 +693,14 @@ int gmc_baro_create(const gmc_mesh *mesh, const gmc_vcoord *vc, const gmc_thermo
   b->d_hyai = p;
   b->d_hybi = p + nlev + 1;
   p += 2 * (size_t)(nlev + 1);
+  b->w_qmfx = p;
+  p += (size_t)nlev * g.sc;
+  b->w_qx = p;
+  p += (size_t)nlev * g.sc;
+  b->w_qy = p;
+  p += (size_t)nlev * g.sc;
+  b->w_qmfy = p;
+  p += (size_t)nlev * g.sv;
   b->d_flag = (int *)p;
   rc = bhip(b, hipMemcpy(b->d_mesh, mesh->metrics, sizeof(double) * GMC_MESH_NFIELDS * nlat, hipMemcpyHostToDevice),
             "mesh");
@@ -702,6 +815,62 @@ int gmc_baro_calc_ptf_lev(gmc_baro *b, int32_t limiter) {
   return rc;
 }
 
+int gmc_baro_calc_ptf_hflx(gmc_baro *b, int32_t limiter, int32_t flux_type, int32_t lon_halo_width, double dt) {
+  if (!b) return GMC_EINVAL;
+  const BaroGrid &g = b->g;
+  if (limiter < GMC_LIMITER_MONO || limiter > GMC_LIMITER_NONE) return berr(b, GMC_EINVAL, "bad limiter");
+  if (flux_type != GMC_FLUX_PPM && flux_type != GMC_FLUX_VAN_LEER) return berr(b, GMC_EINVAL, "bad flux type");
+  if (lon_halo_width < 1 || 2 * lon_halo_width >= g.nlon) return berr(b, GMC_EINVAL, "bad lon_halo_width");
+  if (!std::isfinite(dt)) return berr(b, GMC_EINVAL, "bad dt");
+  AdvGrid ag{};
+  ag.nlon = g.nlon;
+  ag.nlat = g.nlat;
+  ag.nhalf = g.nlat - 1;
+  ag.half = g.nlon / 2;
+  ag.pitch = g.pitch;
+  ag.hw = lon_halo_width;
+  ag.dt = dt;
+  const Mesh M{b->d_mesh, g.nlat};
+  const dim3 faces = dim3(chunks(g, 2 * g.nlat - 3, 1).x, g.nlev), cells = dim3(chunks(g, g.nlat, 1).x, g.nlev);
+  const bool vl = flux_type == GMC_FLUX_VAN_LEER;
+  double **f = b->f;
+  // inner fluxes of pt with the winds, then the half-steps
+  if (vl)
+    hipLaunchKernelGGL(k_hflx_levels<true>, faces, dim3(NB), 0, b->stream, g, ag, (int)limiter, f[GMB_ADV_U],
+                       f[GMB_ADV_V], f[GMB_ADV_CFLX], f[GMB_ADV_CFLY], f[GMB_PT], f[GMB_PT], b->w_qmfx, b->w_qmfy,
+                       b->d_flag);
+  else
+    hipLaunchKernelGGL(k_hflx_levels<false>, faces, dim3(NB), 0, b->stream, g, ag, (int)limiter, f[GMB_ADV_U],
+                       f[GMB_ADV_V], f[GMB_ADV_CFLX], f[GMB_ADV_CFLY], f[GMB_PT], f[GMB_PT], b->w_qmfx, b->w_qmfy,
+                       b->d_flag);
+  hipLaunchKernelGGL(k_half_steps, cells, dim3(NB), 0, b->stream, g, M, dt, f[GMB_PT], b->w_qmfx, b->w_qmfy,
+                     f[GMB_ADV_DIVX], f[GMB_ADV_DIVY], b->w_qx, b->w_qy);
+  hipLaunchKernelGGL(k_half_steps_pole, dim3(2, g.nlev), dim3(kWave), 0, b->stream, g, M, dt, f[GMB_PT], b->w_qmfy,
+                     f[GMB_ADV_DIVY], b->w_qx, b->w_qy);
+  // outer fluxes of qy along x and qx along y with the mass fluxes
+  if (vl)
+    hipLaunchKernelGGL(k_hflx_levels<true>, faces, dim3(NB), 0, b->stream, g, ag, (int)limiter, f[GMB_ADV_MFX],
+                       f[GMB_ADV_MFY], f[GMB_ADV_CFLX], f[GMB_ADV_CFLY], b->w_qy, b->w_qx, f[GMB_PTF_LON],
+                       f[GMB_PTF_LAT], b->d_flag);
+  else
+    hipLaunchKernelGGL(k_hflx_levels<false>, faces, dim3(NB), 0, b->stream, g, ag, (int)limiter, f[GMB_ADV_MFX],
+                       f[GMB_ADV_MFY], f[GMB_ADV_CFLX], f[GMB_ADV_CFLY], b->w_qy, b->w_qx, f[GMB_PTF_LON],
+                       f[GMB_PTF_LAT], b->d_flag);
+  hipLaunchKernelGGL(k_ptf_div, cells, dim3(NB), 0, b->stream, g, M, f[GMB_PTF_LON], f[GMB_PTF_LAT],
+                     f[GMB_DPTFDLON], f[GMB_DPTFDLAT]);
+  hipLaunchKernelGGL(k_ptf_div_pole, dim3(2, g.nlev), dim3(kWave), 0, b->stream, g, M, f[GMB_PTF_LAT],
+                     f[GMB_DPTFDLAT]);
+  const int rc = launched(b, "calc_ptf_hflx");
+  if (!rc) b->flag_pending = true;
+  return rc;
+}
+
+int gmc_baro_calc_grad_ptf(gmc_baro *b, int32_t limiter, int32_t flux_type, int32_t lon_halo_width, double dt) {
+  int rc = gmc_baro_calc_ptf_hflx(b, limiter, flux_type, lon_halo_width, dt);
+  if (!rc) rc = gmc_baro_calc_ptf_lev(b, limiter);
+  return rc;
+}
+
 int gmc_baro_set_small_filter2(gmc_baro *b, const gmc_filter *f) {
   if (!b) return GMC_EINVAL;
   const BaroGrid &g = b->g;
@@ -772,7 +941,11 @@ int gmc_baro_synchronize(gmc_baro *b) {
   if (!h) return GMC_OK;
   const int zero = 0;
   rc = bhip(b, hipMemcpy(b->d_flag, &zero, sizeof(int), hipMemcpyHostToDevice), "flag clear");
-  return rc ? rc : berr(b, GMC_ENUMERIC, "cflz reaches outside the column");
+  if (rc) return rc;
+  std::string msg;
+  if (h & kFlagCflz) msg = "cflz reaches outside the column";
+  if (h & kFlagCflx) msg += std::string(msg.empty() ? "" : "; ") + "cflx exceeds lon_halo_width";
+  return berr(b, GMC_ENUMERIC, msg);
 }
 
 int gmc_baro_stream(gmc_baro *b, void **stream) {

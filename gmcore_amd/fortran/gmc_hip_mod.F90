@@ -63,7 +63,9 @@ module gmc_hip_mod
     GMB_DMFDLAT = 11, GMB_WE_LEV = 12, GMB_WE_LEV_LON = 13, GMB_WE_LEV_LAT = 14, GMB_U_F = 15, &
     GMB_V_F = 16, GMB_M_LON = 17, GMB_M_LAT = 18, GMB_WEDUDLEV = 19, GMB_WEDVDLEV = 20, &
     GMB_PGF_LON = 21, GMB_PGF_LAT = 22, GMB_CFLZ = 23, GMB_PTF_LEV = 24, GMB_DPTFDLEV = 25, &
-    GMB_U_LON = 26, GMB_V_LAT = 27
+    GMB_U_LON = 26, GMB_V_LAT = 27, GMB_ADV_U = 28, GMB_ADV_V = 29, GMB_ADV_MFX = 30, GMB_ADV_MFY = 31, &
+    GMB_ADV_CFLX = 32, GMB_ADV_CFLY = 33, GMB_ADV_DIVX = 34, GMB_ADV_DIVY = 35, GMB_PTF_LON = 36, &
+    GMB_PTF_LAT = 37, GMB_DPTFDLON = 38, GMB_DPTFDLAT = 39
 
   interface
     integer(c_int) function gmc_create(mesh, cfg, ctx) bind(C, name='gmc_create')
@@ -224,6 +226,16 @@ module gmc_hip_mod
     end function
     integer(c_int) function gmc_baro_calc_ptf_lev(baro, limiter) bind(C, name='gmc_baro_calc_ptf_lev')
       import; type(c_ptr), value :: baro; integer(c_int32_t), value :: limiter
+    end function
+    integer(c_int) function gmc_baro_calc_ptf_hflx(baro, limiter, flux_type, lon_halo_width, dt) &
+        bind(C, name='gmc_baro_calc_ptf_hflx')
+      import; type(c_ptr), value :: baro; integer(c_int32_t), value :: limiter, flux_type, lon_halo_width
+      real(c_double), value :: dt
+    end function
+    integer(c_int) function gmc_baro_calc_grad_ptf(baro, limiter, flux_type, lon_halo_width, dt) &
+        bind(C, name='gmc_baro_calc_grad_ptf')
+      import; type(c_ptr), value :: baro; integer(c_int32_t), value :: limiter, flux_type, lon_halo_width
+      real(c_double), value :: dt
     end function
     integer(c_int) function gmc_baro_set_small_filter2(baro, filter) bind(C, name='gmc_baro_set_small_filter2')
       import; type(c_ptr), value :: baro; type(gmc_filter), intent(in) :: filter

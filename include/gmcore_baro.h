@@ -80,6 +80,18 @@ enum gmc_baro_field {
   GMB_DPTFDLEV,      /* tend%dptfdlev     cells, full levels             */
   GMB_U_LON,         /* state%u_lon       U points, full levels          */
   GMB_V_LAT,         /* state%v_lat       V points, full levels          */
+  GMB_ADV_U,         /* adv_batch_mass%u     U points, full levels       */
+  GMB_ADV_V,         /* adv_batch_mass%v     V points, full levels       */
+  GMB_ADV_MFX,       /* adv_batch_mass%mfx   U points, full levels       */
+  GMB_ADV_MFY,       /* adv_batch_mass%mfy   V points, full levels       */
+  GMB_ADV_CFLX,      /* adv_batch_mass%cflx  U points, full levels       */
+  GMB_ADV_CFLY,      /* adv_batch_mass%cfly  V points, full levels       */
+  GMB_ADV_DIVX,      /* adv_batch_mass%divx  cells, full levels          */
+  GMB_ADV_DIVY,      /* adv_batch_mass%divy  cells, full levels          */
+  GMB_PTF_LON,       /* state%ptf_lon     U points, full levels          */
+  GMB_PTF_LAT,       /* state%ptf_lat     V points, full levels          */
+  GMB_DPTFDLON,      /* tend%dptfdlon     cells, full levels             */
+  GMB_DPTFDLAT,      /* tend%dptfdlat     cells, full levels             */
   GMB_NFIELDS
 };
 
@@ -136,6 +148,24 @@ int gmc_baro_pgf_lin97(gmc_baro *b);
  * gmc_baro_synchronize return GMC_ENUMERIC. */
 int gmc_baro_calc_ptf_lev(gmc_baro *b, int32_t limiter);
 
+/* The horizontal part of calc_grad_ptf (operators_mod.F90:1029-1080): on
+ * every level, ffsl_calc_tracer_hflx_cell (ffsl_mod.F90:318-413) of pt with
+ * the mass batch's fields GMB_ADV_* and the time step dt: the inner fluxes of
+ * pt with the winds adv_u / adv_v, the advective half-steps qx, qy (pole
+ * rows from zonal sums), the outer fluxes of qy along x and qx along y with
+ * the mass fluxes adv_mfx / adv_mfy into ptf_lon (the U points of the rows
+ * off the poles) and ptf_lat; then dptfdlon on the rows off the poles and
+ * dptfdlat, whose pole rows take the zonal sum of the adjacent ptf_lat row.
+ * limiter: GMC_LIMITER_*; flux_type: GMC_FLUX_* (include/gmcore_adv.h);
+ * lon_halo_width: the bound on |adv_cflx| (below nlon / 2).  The call only
+ * queues the work; a Courant number beyond the bound (adv_batch_mod.F90:
+ * 284-287) makes the next gmc_baro_synchronize return GMC_ENUMERIC. */
+int gmc_baro_calc_ptf_hflx(gmc_baro *b, int32_t limiter, int32_t flux_type, int32_t lon_halo_width, double dt);
+/* calc_grad_ptf (operators_mod.F90:1008-1118): gmc_baro_calc_ptf_hflx, then
+ * gmc_baro_calc_ptf_lev with the same limiter (the vertical fluxes use the
+ * batch's we and cflz, GMB_WE_LEV and GMB_CFLZ). */
+int gmc_baro_calc_grad_ptf(gmc_baro *b, int32_t limiter, int32_t flux_type, int32_t lon_halo_width, double dt);
+
 /* The table `small_filter2` of update_state's small_filter2 step (filter_init's
  * 'small_filter2', filter_types_mod.F90:124-143: the big filter's widths
  * times exp_two_values(filter_coef_e, filter_coef_d, lat0, 80, |lat|)),
@@ -149,9 +179,9 @@ int gmc_baro_set_small_filter2(gmc_baro *b, const gmc_filter *small_filter2);
  * GMC_EINVAL when no table is set. */
 int gmc_baro_small_filter2(gmc_baro *b);
 
-/* Wait for the block's queued work (GMC_ENUMERIC if a calc_ptf_lev since the
- * last call met a Courant number beyond the column); its stream
- * (hipStream_t). */
+/* Wait for the block's queued work (GMC_ENUMERIC if a calc_ptf_lev or
+ * calc_ptf_hflx since the last call met a Courant number out of range); its
+ * stream (hipStream_t). */
 int gmc_baro_synchronize(gmc_baro *b);
 int gmc_baro_stream(gmc_baro *b, void **stream);
 

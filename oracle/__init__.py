@@ -379,6 +379,7 @@ class BaroOracle:
         L.gmo_baro_pgf_lin97.argtypes = [i, i, i, P, P, P, P, P]
         L.gmo_baro_calc_ptf_lev.argtypes = [i, i, i, i, P, P, P, P, P]
         L.gmo_baro_small_filter2.argtypes = [i, i, i, C.POINTER(GmcFilter), P, P]
+        L.gmo_baro_calc_ptf_hflx.argtypes = [C.POINTER(GmcMesh), i, C.POINTER(GmcAdvConfig), d] + [P] * 13
 
     def _a(self, x, levels, vrows=False):
         shape = (levels, self.nlat - 1 if vrows else self.nlat, self.nlon)
@@ -450,6 +451,22 @@ class BaroOracle:
         bad = self.L.gmo_baro_calc_ptf_lev(self.nlon, self.nlat, n, limiter, dptr(pt), dptr(we), dptr(c),
                                            dptr(ptf), dptr(dptf))
         return ptf, dptf, bad
+
+    def calc_ptf_hflx(self, pt, u, v, mfx, mfy, cflx, cfly, divx, divy, dt: float, limiter: int = 0,
+                      flux_type: int = 0, lon_halo_width: int = 1):
+        """ptf_lon, ptf_lat, dptfdlon, dptfdlat of the horizontal part of
+        calc_grad_ptf; ValueError when a cflx exceeds lon_halo_width."""
+        n = self.nlev
+        a = [self._a(pt, n), self._a(u, n), self._a(v, n, True), self._a(mfx, n), self._a(mfy, n, True),
+             self._a(cflx, n), self._a(cfly, n, True), self._a(divx, n), self._a(divy, n)]
+        out = [self._z(n), self._z(n, True), self._z(n), self._z(n)]
+        mesh = GmcMesh(self.nlon, self.nlat, 2 * np.pi / self.nlon, dptr(self.m))
+        wgt = np.zeros(self.nlat)
+        cfg = GmcAdvConfig(6371220.0, dt, 1, 1, lon_halo_width, limiter, 0, flux_type, dptr(wgt))
+        rc = self.L.gmo_baro_calc_ptf_hflx(C.byref(mesh), n, C.byref(cfg), dt, *[dptr(x) for x in a + out])
+        if rc:
+            raise ValueError(f"gmo_baro_calc_ptf_hflx failed ({rc})")
+        return tuple(out)
 
     def small_filter2(self, u_lon, v_lat, table):
         """u_lon, v_lat after small_filter2; table = filter_table(..., kind=2)."""

@@ -375,9 +375,8 @@ static void hflx(gmo_adv *s, const double *u, const double *v, const double *mx,
 /* ffsl_calc_tracer_hflx_cell (ffsl_mod.F90:318-413): inner advective
  * half-steps qx, qy from the PPM fluxes of q, then the outer fluxes of qy
  * along x and qx along y. */
-static void tracer_hflx(gmo_adv *s, double *q) {
+static void tracer_hflx_dt(gmo_adv *s, double *q, double dt) {
   const int nlon = s->nlon, nlat = s->nlat, nhalf = s->nhalf;
-  const double dt = s->cfg.dt_adv;
   double *qmfx = s->qmf_lon, *qmfy = s->qmf_lat;
   hflx(s, s->u, s->v, q, q, qmfx, qmfy);
   fill_halo(s, qmfx, KF, 1, 1, 0, 0);
@@ -411,6 +410,7 @@ static void tracer_hflx(gmo_adv *s, double *q) {
   fill_halo(s, s->qy, KF, 1, 1, 0, 0);
   hflx(s, s->mfx, s->mfy, s->qy, s->qx, qmfx, qmfy);
 }
+static void tracer_hflx(gmo_adv *s, double *q) { tracer_hflx_dt(s, q, s->cfg.dt_adv); }
 
 /* lon_damp_1d, order 2 (lon_damp_mod.F90:200-241): one Jacobi sweep of
  * f + 1/4 (f_{i-1} - 2 f + f_{i+1}) blended with weight wgt */
@@ -614,6 +614,42 @@ double gmo_adv_mass(gmo_adv *s, int level, int tracer) {
   for (int j = 1; j <= s->nlat; j++)
     for (int i = 1; i <= s->nlon; i++) qm = qm + A(s, q, i, j) * MS(s, GMC_AREA_CELL, j);
   return qm;
+}
+
+/* ffsl_calc_tracer_hflx_cell (ffsl_mod.F90:318-413) of one field q (nlat
+ * rows, halo filled here) with the batch's u, mfx, cflx, divx, divy (nlat
+ * rows) and v, mfy, cfly (nlat - 1 rows) given and the time step dt (the
+ * optional dt argument, :338): the outer fluxes into qmfx (nlat rows; the
+ * pole rows are those the x-flux leaves, zero here) and qmfy (nlat - 1
+ * rows).  The batch's arrays of s are overwritten; its tracers are not
+ * touched. */
+int gmo_adv_hflx_field(gmo_adv *s, const double *q, const double *u, const double *v, const double *mfx,
+                       const double *mfy, const double *cflx, const double *cfly, const double *divx,
+                       const double *divy, double dt, double *qmfx, double *qmfy, int64_t ld) {
+  if (ld < s->nlon) return GMC_EINVAL;
+  for (int j = 0; j < s->nlat; j++)
+    for (int i = 0; i < s->nlon; i++) {
+      const double c = cflx[(size_t)j * ld + i];
+      if (!(fabs(c) <= s->cfg.lon_halo_width)) return GMC_ENUMERIC;   /* adv_batch_mod.F90:284-287 */
+    }
+  double *w = falloc(s);
+  put(s, w, s->nlat, q, ld);
+  fill_all(s, w, KF);
+  put(s, s->u, s->nlat, u, ld);
+  put(s, s->v, s->nhalf, v, ld);
+  put(s, s->mfx, s->nlat, mfx, ld);
+  put(s, s->mfy, s->nhalf, mfy, ld);
+  put(s, s->cflx, s->nlat, cflx, ld);
+  put(s, s->cfly, s->nhalf, cfly, ld);
+  put(s, s->divx, s->nlat, divx, ld);
+  put(s, s->divy, s->nlat, divy, ld);
+  memset(s->qmf_lon, 0, (size_t)s->ncol * s->nrow * sizeof(double));
+  memset(s->qmf_lat, 0, (size_t)s->ncol * s->nrow * sizeof(double));
+  tracer_hflx_dt(s, w, dt);
+  take(s, s->qmf_lon, s->nlat, qmfx, ld);
+  take(s, s->qmf_lat, s->nhalf, qmfy, ld);
+  free(w);
+  return 0;
 }
 
 /* interior of a batch array: 0 u, 1 v, 2 mfx, 3 mfy, 4 cflx, 5 cfly, 6 divx,

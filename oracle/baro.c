@@ -257,3 +257,54 @@ void gmo_baro_small_filter2(int nlon, int nlat, int nlev, const gmc_filter *f, d
       }
   }
 }
+
+/* The horizontal part of calc_grad_ptf (operators_mod.F90:1029-1080): on
+ * every level, ffsl_calc_tracer_hflx_cell of pt with the mass batch's
+ * fields and the dynamics time step dt (adv.c's restatement) into ptf_lon
+ * (U points of the rows off the poles) and ptf_lat (V points); dptfdlon on
+ * the rows off the poles and dptfdlat, whose pole rows take the zonal sum of
+ * the nearest ptf_lat row.  cfg: the batch's limiter, flux type and
+ * lon_halo_width (ntracers >= 1, nstep >= 1, a damping table).  Returns 0,
+ * or GMC_ENUMERIC when a cflx exceeds lon_halo_width (nothing written from
+ * that level on). */
+int gmo_baro_calc_ptf_hflx(const gmc_mesh *mesh, int nlev, const gmc_adv_config *cfg, double dt, const double *pt,
+                           const double *u, const double *v, const double *mfx, const double *mfy,
+                           const double *cflx, const double *cfly, const double *divx, const double *divy,
+                           double *ptf_lon, double *ptf_lat, double *dptfdlon, double *dptfdlat) {
+  const int nlon = mesh->nlon, nlat = mesh->nlat, nhalf = nlat - 1;
+  const double *metrics = mesh->metrics;
+  const size_t L = (size_t)nlat * nlon, Lv = (size_t)nhalf * nlon;
+  gmo_adv *s = gmo_adv_create(mesh, cfg);
+  if (!s) return GMC_EINVAL;
+  double work[nlon];
+  for (int k = 0; k < nlev; k++) {
+    const int rc = gmo_adv_hflx_field(s, pt + k * L, u + k * L, v + k * Lv, mfx + k * L, mfy + k * Lv, cflx + k * L,
+                                      cfly + k * Lv, divx + k * L, divy + k * L, dt, ptf_lon + k * L,
+                                      ptf_lat + k * Lv, nlon);
+    if (rc) {
+      gmo_adv_destroy(s);
+      return rc;
+    }
+    const double *fx = ptf_lon + k * L, *fy = ptf_lat + k * Lv;
+    double *dx = dptfdlon + k * L, *dy = dptfdlat + k * L;
+    for (int j = 1; j < nlat - 1; j++)
+      for (int i = 0; i < nlon; i++) {
+        const int iw = i == 0 ? nlon - 1 : i - 1;
+        dx[j * nlon + i] = (fx[j * nlon + i] - fx[j * nlon + iw]) * MX(GMC_LE_LON, j) / MX(GMC_AREA_CELL, j);
+      }
+    for (int j = 1; j < nlat - 1; j++)
+      for (int i = 0; i < nlon; i++)
+        dy[j * nlon + i] = (fy[j * nlon + i] * MX(GMC_LE_LAT, j) - fy[(j - 1) * nlon + i] * MX(GMC_LE_LAT, j - 1)) /
+                           MX(GMC_AREA_CELL, j);
+    for (int i = 0; i < nlon; i++) work[i] = fy[i];
+    double pole = gmo_zonal_sum(work, nlon, 1);
+    pole = pole * MX(GMC_LE_LAT, 0) / nlon / MX(GMC_AREA_CELL, 0);
+    for (int i = 0; i < nlon; i++) dy[i] = pole;
+    for (int i = 0; i < nlon; i++) work[i] = -fy[(nhalf - 1) * nlon + i];
+    pole = gmo_zonal_sum(work, nlon, 1);
+    pole = pole * MX(GMC_LE_LAT, nhalf - 1) / nlon / MX(GMC_AREA_CELL, nlat - 1);
+    for (int i = 0; i < nlon; i++) dy[(nlat - 1) * nlon + i] = pole;
+  }
+  gmo_adv_destroy(s);
+  return 0;
+}

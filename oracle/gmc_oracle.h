@@ -103,6 +103,9 @@ double gmo_adv_mass(gmo_adv *s, int level, int tracer);
 /* interior of a batch array: 0 u, 1 v, 2 mfx, 3 mfy, 4 cflx, 5 cfly, 6 divx,
  * 7 divy, 8 qx, 9 qy, 10 qmf_lon, 11 qmf_lat, 12 old_m */
 int gmo_adv_get_internal(gmo_adv *s, int index, double *host, int64_t ld);
+int gmo_adv_hflx_field(gmo_adv *s, const double *q, const double *u, const double *v, const double *mfx,
+                       const double *mfy, const double *cflx, const double *cfly, const double *divx,
+                       const double *divy, double dt, double *qmfx, double *qmfy, int64_t ld);
 /* tangent linear and adjoint of one adv_run step with respect to the
  * tracers (winds held): dq [ntracers][nlat][ld] old -> new level, lam new ->
  * old level; either may be NULL.  Advances the base like gmo_adv_run. */
@@ -137,6 +140,12 @@ void gmo_baro_pgf_lin97(int nlon, int nlat, int nlev, const double *metrics, con
  * outside the column */
 int gmo_baro_calc_ptf_lev(int nlon, int nlat, int nlev, int limiter, const double *pt, const double *we,
                           const double *cflz, double *ptf_lev, double *dptfdlev);
+/* the horizontal part of calc_grad_ptf: ptf_lon, ptf_lat, dptfdlon, dptfdlat
+ * of pt on every level with the mass batch's fields; 0 or GMC_ENUMERIC */
+int gmo_baro_calc_ptf_hflx(const gmc_mesh *mesh, int nlev, const gmc_adv_config *cfg, double dt, const double *pt,
+                           const double *u, const double *v, const double *mfx, const double *mfy,
+                           const double *cflx, const double *cfly, const double *divx, const double *divy,
+                           double *ptf_lon, double *ptf_lat, double *dptfdlon, double *dptfdlat);
 /* small_filter2 of update_state's baroclinic branch, in place on u_lon and v_lat */
 void gmo_baro_small_filter2(int nlon, int nlat, int nlev, const gmc_filter *f, double *u_lon, double *v_lat);
 /* hybrid_coord_equal_eta (hybrid_coord_test_mod.F90:1330-1352) */

@@ -317,3 +317,80 @@ def test_level_counts(nlev):
     assert bad == 0
     np.testing.assert_array_equal(g.get("ptf_lev"), ptf)
     np.testing.assert_array_equal(g.get("dptfdlev"), dptf)
+
+
+HW = 3
+
+
+def mass_batch(seed):
+    """Random mass-batch fields: |cflx| up to 2.9 (whole cells crossed),
+    |cfly| < 0.9."""
+    r = rng(seed)
+    shp, shv = (NLEV, NLAT, NLON), (NLEV, NLAT - 1, NLON)
+    return {"adv_u": 20 * r.standard_normal(shp), "adv_v": 20 * r.standard_normal(shv),
+            "adv_mfx": 1e4 * r.standard_normal(shp), "adv_mfy": 1e4 * r.standard_normal(shv),
+            "adv_cflx": r.uniform(-2.9, 2.9, shp), "adv_cfly": r.uniform(-0.9, 0.9, shv),
+            "adv_divx": 1e-5 * r.standard_normal(shp), "adv_divy": 1e-5 * r.standard_normal(shp)}
+
+
+def oracle_hflx(o, pt, f, dt, limiter, flux_type):
+    return o.calc_ptf_hflx(pt, f["adv_u"], f["adv_v"], f["adv_mfx"], f["adv_mfy"], f["adv_cflx"], f["adv_cfly"],
+                           f["adv_divx"], f["adv_divy"], dt, limiter, flux_type, HW)
+
+
+@pytest.mark.parametrize("limiter,flux_type", [("mono", "ppm"), ("pd", "ppm"), ("none", "ppm"),
+                                               ("mono", "van_leer")])
+def test_calc_ptf_hflx_bitwise(env, limiter, flux_type):
+    """calc_grad_ptf's horizontal fluxes and their divergence, every level,
+    bit for bit against the oracle (pole rows through the zonal sums)."""
+    o = env[3]
+    g = gpu(env)
+    _, _, _, _, theta, _ = column_state(o, 14)
+    f = mass_batch(15)
+    g.set("pt", theta)
+    for k, a in f.items():
+        g.set(k, a)
+    g.calc_ptf_hflx(600.0, limiter, flux_type, HW)
+    want = oracle_hflx(o, theta, f, 600.0, {"mono": 0, "pd": 1, "none": 2}[limiter],
+                       {"ppm": 0, "van_leer": 1}[flux_type])
+    for name, w in zip(("ptf_lon", "ptf_lat", "dptfdlon", "dptfdlat"), want):
+        np.testing.assert_array_equal(g.get(name), w, err_msg=name)
+
+
+def test_calc_grad_ptf_is_both_parts(env):
+    """calc_grad_ptf = the horizontal fluxes, then the vertical ones, each
+    against the oracle."""
+    o = env[3]
+    g = gpu(env)
+    _, _, _, _, theta, _ = column_state(o, 16)
+    f = mass_batch(17)
+    w = rng(18).standard_normal((NLEV + 1, NLAT, NLON))
+    c = courant_numbers(19)
+    g.set("pt", theta)
+    for k, a in list(f.items()) + [("we_lev", w), ("cflz", c)]:
+        g.set(k, a)
+    g.calc_grad_ptf(300.0, "mono", "ppm", HW)
+    want = oracle_hflx(o, theta, f, 300.0, 0, 0)
+    for name, x in zip(("ptf_lon", "ptf_lat", "dptfdlon", "dptfdlat"), want):
+        np.testing.assert_array_equal(g.get(name), x, err_msg=name)
+    ptf, dptf, bad = o.calc_ptf_lev(theta, w, c, 0)
+    assert bad == 0
+    np.testing.assert_array_equal(g.get("ptf_lev"), ptf)
+    np.testing.assert_array_equal(g.get("dptfdlev"), dptf)
+
+
+def test_calc_ptf_hflx_reports_cflx_beyond_halo(env):
+    """A cflx beyond lon_halo_width is reported at the next synchronize (the
+    reference stops there); the accesses stay in the row meanwhile."""
+    g = gpu(env)
+    f = mass_batch(20)
+    f["adv_cflx"][4, 10, 11] = HW + 0.25
+    g.set("pt", np.full((NLEV, NLAT, NLON), 300.0))
+    for k, a in f.items():
+        g.set(k, a)
+    g.calc_ptf_hflx(600.0, lon_halo_width=HW, check=False)
+    with pytest.raises(Exception, match="cflx exceeds lon_halo_width"):
+        g.synchronize()
+    g.synchronize()
+    with pytest.raises(Exception, match="lon_halo_width"):
+        g.calc_ptf_hflx(600.0, lon_halo_width=NLON // 2)   # refused before any launch

@@ -260,3 +260,156 @@ def test_small_filter2_damps_zonal_modes_by_its_response(setup):
             want = np.arange(NLEV)[:, None] + resp * np.cos(mode * lam)[None, :]
             np.testing.assert_allclose(fx[:, r], want, rtol=0, atol=1e-12 * NLEV)
     assert (ng_lon > 1).sum() > 4 and (ng_lat > 1).sum() > 4
+
+
+# --- the horizontal part of calc_grad_ptf ---------------------------------
+HW = 3        # lon_halo_width of the tests
+DT = 600.0
+
+
+def hflx_fields(m, seed, nlev=NLEV, scale=1.0):
+    """Random mass-batch fields of the FFSL fluxes: winds, mass fluxes,
+    Courant numbers (|cflx| up to 2.9 on the rows off the poles, |cfly| <
+    0.9) and split divergences."""
+    r = np.random.default_rng(seed)
+    u = scale * r.standard_normal((nlev, NLAT, NLON))
+    v = scale * r.standard_normal((nlev, NLAT - 1, NLON))
+    mfx = 1e3 * scale * r.standard_normal((nlev, NLAT, NLON))
+    mfy = 1e3 * scale * r.standard_normal((nlev, NLAT - 1, NLON))
+    cflx = r.uniform(-2.9, 2.9, (nlev, NLAT, NLON))
+    cfly = r.uniform(-0.9, 0.9, (nlev, NLAT - 1, NLON))
+    divx = 1e-5 * r.standard_normal((nlev, NLAT, NLON))
+    divy = 1e-5 * r.standard_normal((nlev, NLAT, NLON))
+    return dict(u=u, v=v, mfx=mfx, mfy=mfy, cflx=cflx, cfly=cfly, divx=divx, divy=divy)
+
+
+def hflx(o, pt, f, limiter=0, flux_type=0):
+    return o.calc_ptf_hflx(pt, f["u"], f["v"], f["mfx"], f["mfy"], f["cflx"], f["cfly"], f["divx"], f["divy"],
+                           DT, limiter, flux_type, HW)
+
+
+def consistent_divergences(m, u, v):
+    """divx, divy that the winds u, v would give (the split divergences of
+    the batch): the inner half-steps of a uniform field are then the field."""
+    le_lon, le_lat, area = m[MESH["le_lon"]], m[MESH["le_lat"]], m[MESH["area_cell"]]
+    divx = (u - np.roll(u, 1, axis=-1)) * le_lon[None, :, None] / area[None, :, None]
+    divy = np.zeros_like(divx)
+    lv = le_lat[:NLAT - 1][None, :, None]      # the V rows' edge lengths
+    divy[:, 1:-1] = (v[:, 1:] * lv[:, 1:] - v[:, :-1] * lv[:, :-1]) / area[None, 1:-1, None]
+    divy[:, 0] = v[:, 0].sum(-1)[:, None] * le_lat[0] / NLON / area[0]
+    divy[:, -1] = -v[:, -1].sum(-1)[:, None] * le_lat[NLAT - 2] / NLON / area[-1]
+    return divx, divy
+
+
+@pytest.mark.parametrize("flux_type", [0, 1])
+def test_ptf_hflx_of_uniform_pt_is_the_mass_flux(setup, flux_type):
+    """A uniform pt is carried unchanged by the inner half-steps when the
+    split divergences are those of the winds, so the outer fluxes are the
+    mass fluxes times pt and dptfdlon / dptfdlat the mass-flux divergence
+    times pt; a swap of the winds and the mass fluxes, or of divx and divy,
+    breaks it."""
+    m, _, _, o = setup
+    f = hflx_fields(m, 1)
+    f["divx"], f["divy"] = consistent_divergences(m, f["u"], f["v"])
+    th = 300.0
+    fx, fy, dx, dy = hflx(o, np.full((NLEV, NLAT, NLON), th), f, 0, flux_type)
+    np.testing.assert_allclose(fx[:, 1:-1], th * f["mfx"][:, 1:-1], rtol=1e-12, atol=1e-9)
+    np.testing.assert_allclose(fy, th * f["mfy"], rtol=1e-12, atol=1e-9)
+    assert np.all(fx[:, [0, -1]] == 0)        # the pole rows' U points are never written
+    mdx, mdy = consistent_divergences(m, f["mfx"], f["mfy"])
+    scale = np.max(np.abs(th * mdx))
+    np.testing.assert_allclose(dx[:, 1:-1], th * mdx[:, 1:-1], rtol=0, atol=1e-11 * scale)
+    np.testing.assert_allclose(dy, th * mdy, rtol=0, atol=1e-11 * scale)
+
+
+def quad_means(n, a, b, c):
+    """cell means of q(x) = a + b x + c x^2 over the cells [i - 1/2, i + 1/2]"""
+    i = np.arange(n, dtype=np.float64)
+    return a + b * i + c * (i * i + 1.0 / 12.0)
+
+
+def quad_integral(lo, hi, a, b, c):
+    F = lambda x: a * x + b * x * x / 2 + c * x ** 3 / 3
+    return F(hi) - F(lo)
+
+
+def test_ptf_hflx_x_flux_is_exact_on_a_quadratic(setup):
+    """With no meridional wind (v = mfy = cfly = divy = 0) qy is pt itself,
+    and the outer x-flux through the east face x = i + 1/2 of cell i is
+    mfx / c times the integral of pt's parabola over [i + 1/2 - c, i + 1/2]:
+    the unlimited PPM reconstruction of quadratic cell means is the
+    quadratic, and the whole cells crossed add their means.  Checked for
+    Courant numbers of both signs up to 2.9 on faces away from the
+    longitude seam."""
+    m, _, _, o = setup
+    f = hflx_fields(m, 2)
+    for k in ("v", "mfy", "cfly", "divy"):
+        f[k][:] = 0.0
+    a, b, c = 300.0, 0.7, -0.02
+    q = np.broadcast_to(quad_means(NLON, a, b, c), (NLEV, NLAT, NLON)).copy()
+    fx, fy, _, _ = hflx(o, q, f, limiter=2)
+    assert np.all(fy == 0)
+    i = np.arange(NLON, dtype=np.float64)
+    cx = f["cflx"]
+    want = f["mfx"] / cx * quad_integral(i + 0.5 - cx, i + 0.5, a, b, c)
+    sel = np.zeros(NLON, bool)
+    sel[HW + 3:NLON - HW - 3] = True         # stencils clear of the seam
+    # rounding of the swept integral grows like 1 / |c| (1e-12 at the
+    # smallest Courant numbers drawn); a wrong stencil or sign is off by 1e-3
+    np.testing.assert_allclose(fx[:, 1:-1][..., sel], want[:, 1:-1][..., sel], rtol=1e-10)
+
+
+def test_ptf_hflx_y_flux_is_exact_on_a_quadratic(setup):
+    """With no zonal wind (u = mfx = cflx = divx = 0) qx is pt itself, and
+    the outer y-flux through V row h (the face y = h + 1/2 between cell
+    rows h and h + 1) is mfy / c times the integral of pt's parabola in the
+    row index over [h + 1/2 - c, h + 1/2], |c| < 1; V rows whose stencil stays
+    off the poles."""
+    m, _, _, o = setup
+    f = hflx_fields(m, 3)
+    for k in ("u", "mfx", "cflx", "divx"):
+        f[k][:] = 0.0
+    a, b, c = 280.0, 1.3, 0.05
+    q = np.broadcast_to(quad_means(NLAT, a, b, c)[None, :, None], (NLEV, NLAT, NLON)).copy()
+    fx, fy, _, _ = hflx(o, q, f, limiter=2)
+    assert np.all(fx == 0)
+    h = np.arange(NLAT - 1, dtype=np.float64)[None, :, None]
+    cy = f["cfly"]
+    want = f["mfy"] / cy * quad_integral(h + 0.5 - cy, h + 0.5, a, b, c)
+    np.testing.assert_allclose(fy[:, 3:NLAT - 5], want[:, 3:NLAT - 5], rtol=1e-10)
+
+
+@pytest.mark.parametrize("limiter,flux_type", [(0, 0), (1, 0), (2, 1)])
+def test_ptf_divergence_integrates_to_zero(setup, limiter, flux_type):
+    """The flux form: the area integral of dptfdlon + dptfdlat over the
+    sphere vanishes on every level (each row's x-differences telescope, the
+    y-differences telescope into the pole rows' zonal sums)."""
+    m, _, _, o = setup
+    f = hflx_fields(m, 4)
+    pt = 300.0 + 20.0 * np.random.default_rng(5).standard_normal((NLEV, NLAT, NLON))
+    fx, fy, dx, dy = hflx(o, pt, f, limiter, flux_type)
+    area = m[MESH["area_cell"]][None, :, None]
+    total = ((dx + dy) * area).sum(axis=(1, 2))
+    scale = (np.abs(dy) * area).sum(axis=(1, 2))
+    assert np.all(np.abs(total) < 1e-12 * scale)
+    assert np.all(scale > 0)
+
+
+def test_ptf_hflx_levels_are_independent(setup):
+    """Each level sees only its own fields: reversing the levels of every
+    input reverses the outputs, bit for bit."""
+    m, _, _, o = setup
+    f = hflx_fields(m, 6)
+    pt = 300.0 + 20.0 * np.random.default_rng(7).standard_normal((NLEV, NLAT, NLON))
+    out = hflx(o, pt, f)
+    rev = hflx(o, pt[::-1], {k: v[::-1] for k, v in f.items()})
+    for a, b in zip(out, rev):
+        np.testing.assert_array_equal(a, b[::-1])
+
+
+def test_ptf_hflx_refuses_cflx_beyond_the_halo(setup):
+    m, _, _, o = setup
+    f = hflx_fields(m, 8)
+    f["cflx"][2, 5, 7] = HW + 0.5
+    with pytest.raises(ValueError):
+        hflx(o, np.full((NLEV, NLAT, NLON), 300.0), f)
