@@ -57,11 +57,29 @@ constexpr FieldShape kShape[GMB_NFIELDS] = {
 constexpr int LC = 4;     // levels per thread of the level-chunked kernels
 enum { kFlagCflz = 1, kFlagCflx = 2 };   // bits of the error flag: Courant numbers out of range
 
-// the point of this thread in a level-chunked launch over `rows` rows of a
-// level plane: o = r * pitch + i; false in the pitch padding and past the
-// last row
-__device__ __forceinline__ bool plane_point(const BaroGrid &g, int rows, int64_t &o, int &i, int &r) {
-  const uint32_t p = blockIdx.x * NB + threadIdx.x;
+// How the blocks of a launch over (plane blocks, levels or level chunks) are
+// dealt to the XCDs, each with its own L2 (xcd_item over the linear block
+// index): kByRows gives an XCD a contiguous run of a level's plane blocks
+// (neighbouring rows of a level share its L2: horizontal stencils), kByColumns
+// a run of column blocks with all their levels (neighbouring levels of a
+// column share it: vertical stencils), kAsLaunched the hardware's
+// round-robin (pointwise work).
+enum Deal { kAsLaunched, kByRows, kByColumns };
+
+// The point of this thread in a launch over `rows` rows of a level plane
+// (grid x) and the levels or level chunks (grid y): o = r * pitch + i, lev
+// the level or chunk; false in the pitch padding and past the last row.
+template <Deal D = kAsLaunched>
+__device__ __forceinline__ bool plane_point(const BaroGrid &g, int rows, int64_t &o, int &i, int &r, int &lev) {
+  const int gx = gridDim.x, gy = gridDim.y;
+  int bx = blockIdx.x;
+  lev = blockIdx.y;
+  if (D != kAsLaunched) {
+    const int it = xcd_item(blockIdx.x + blockIdx.y * gx, gx * gy);
+    bx = D == kByRows ? it % gx : it / gy;
+    lev = D == kByRows ? it / gx : it % gy;
+  }
+  const uint32_t p = (uint32_t)bx * NB + threadIdx.x;
   r = (int)(p / (uint32_t)g.pitch);
   i = (int)(p - (uint32_t)r * (uint32_t)g.pitch);
   o = p;
@@ -82,9 +100,9 @@ __global__ void __launch_bounds__(NB) k_calc_ph(BaroGrid g, const double *__rest
                                                 double rd_o_cpd, const double *__restrict__ phs, double *ph_lev,
                                                 double *ph_exn_lev, double *ph) {
   int64_t o;
-  int i, r;
-  if (!plane_point(g, g.nlat, o, i, r)) return;
-  const int k0 = blockIdx.y * LC, k1 = min(k0 + LC, g.nlev);
+  int i, r, ch;
+  if (!plane_point(g, g.nlat, o, i, r, ch)) return;
+  const int k0 = ch * LC, k1 = min(k0 + LC, g.nlev);
   const double ps = phs[o];
   double above = half_level_p(hyai, hybi, p0, ptop, ps, k0);
   for (int k = k0; k < k1; k++) {
@@ -104,9 +122,9 @@ __global__ void __launch_bounds__(NB) k_calc_ph(BaroGrid g, const double *__rest
 __global__ void __launch_bounds__(NB) k_calc_t(BaroGrid g, double p0, double rd_o_cpd, const double *__restrict__ pt,
                                                const double *__restrict__ ph, double *t) {
   int64_t o;
-  int i, r;
-  if (!plane_point(g, g.nlat, o, i, r)) return;
-  const int k0 = blockIdx.y * LC;
+  int i, r, ch;
+  if (!plane_point(g, g.nlat, o, i, r, ch)) return;
+  const int k0 = ch * LC;
   double a[LC], b[LC];
 #pragma unroll
   for (int j = 0; j < LC; j++) {   // every load of the chunk in flight at once
@@ -251,41 +269,34 @@ __device__ __forceinline__ bool u_or_v_row(const BaroGrid &g, int y, int &r) {
 }
 
 // interp_lev_edge_to_lev_lon_edge / _lat_edge with no wind
-// (interp_mod.F90:360-369, :425-434), the chunk's half levels
+// (interp_mod.F90:360-369, :425-434), the chunk's half levels.  The thread
+// of cell (i, r), r = 0 .. nlat-2, forms the V point north of it and, off the
+// south pole row, the U point east of it: both read the cell's values.
 __global__ void __launch_bounds__(NB) k_we_edges(BaroGrid g, Mesh M, const double *__restrict__ we_lev,
                                                  double *we_lon, double *we_lat) {
   int64_t o;
-  int i, y, r;
-  if (!plane_point(g, 2 * g.nlat - 3, o, i, y)) return;
-  const int k0 = blockIdx.y * LC, k1 = min(k0 + LC, g.nlev + 1);
-  if (u_or_v_row(g, y, r)) {
-    const int ie = i + 1 == g.nlon ? 0 : i + 1;
-    const double aw = M(GMC_AREA_LON_WEST, r), ae = M(GMC_AREA_LON_EAST, r), a = M(GMC_AREA_LON, r);
-    const int64_t c = r * g.pitch;
-    double w[LC], e[LC];
+  int i, r, ch;
+  if (!plane_point<kByRows>(g, g.nlat - 1, o, i, r, ch)) return;
+  const int k0 = ch * LC, k1 = min(k0 + LC, g.nlev + 1);
+  const int64_t oe = o - i + (i + 1 == g.nlon ? 0 : i + 1);
+  double w[LC], e[LC], nn[LC];
 #pragma unroll
-    for (int j = 0; j < LC; j++) {
-      const int k = min(k0 + j, g.nlev);
-      w[j] = we_lev[k * g.sc + c + i];
-      e[j] = we_lev[k * g.sc + c + ie];
-    }
-#pragma unroll
-    for (int j = 0; j < LC; j++)
-      if (k0 + j < k1) we_lon[(k0 + j) * g.sc + c + i] = (aw * w[j] + ae * e[j]) / a;
-  } else {
-    const double an = M(GMC_AREA_LAT_NORTH, r), as = M(GMC_AREA_LAT_SOUTH, r), a = M(GMC_AREA_LAT, r);
-    const int64_t c = r * g.pitch + i;
-    double nn[LC], ss[LC];
-#pragma unroll
-    for (int j = 0; j < LC; j++) {
-      const int k = min(k0 + j, g.nlev);
-      nn[j] = we_lev[k * g.sc + c + g.pitch];
-      ss[j] = we_lev[k * g.sc + c];
-    }
-#pragma unroll
-    for (int j = 0; j < LC; j++)
-      if (k0 + j < k1) we_lat[(k0 + j) * g.sv + c] = (an * nn[j] + as * ss[j]) / a;
+  for (int j = 0; j < LC; j++) {   // the chunk's loads in flight at once
+    const int64_t k = min(k0 + j, g.nlev) * g.sc;
+    w[j] = we_lev[k + o];
+    e[j] = we_lev[k + oe];
+    nn[j] = we_lev[k + o + g.pitch];
   }
+  if (r >= 1) {
+    const double aw = M(GMC_AREA_LON_WEST, r), ae = M(GMC_AREA_LON_EAST, r), a = M(GMC_AREA_LON, r);
+#pragma unroll
+    for (int j = 0; j < LC; j++)
+      if (k0 + j < k1) we_lon[(k0 + j) * g.sc + o] = (aw * w[j] + ae * e[j]) / a;
+  }
+  const double an = M(GMC_AREA_LAT_NORTH, r), as = M(GMC_AREA_LAT_SOUTH, r), a = M(GMC_AREA_LAT, r);
+#pragma unroll
+  for (int j = 0; j < LC; j++)
+    if (k0 + j < k1) we_lat[(k0 + j) * g.sv + o] = (an * nn[j] + as * w[j]) / a;
 }
 
 // calc_wedudlev_wedvdlev (operators_mod.F90:1145-1216) on full levels
@@ -326,9 +337,9 @@ __global__ void __launch_bounds__(NB) k_wedudlev(BaroGrid g, const double *__res
                                                  const double *__restrict__ we_lat,
                                                  const double *__restrict__ m_lat, double *wedvdlev) {
   int64_t o;
-  int i, y, r;
-  if (!plane_point(g, 2 * g.nlat - 3, o, i, y)) return;
-  const int k0 = blockIdx.y * LC;
+  int i, y, r, ch;
+  if (!plane_point<kByColumns>(g, 2 * g.nlat - 3, o, i, y, ch)) return;
+  const int k0 = ch * LC;
   const bool isu = u_or_v_row(g, y, r);
   const int64_t c = r * g.pitch + i;
   if (isu)
@@ -350,40 +361,42 @@ __device__ __forceinline__ double lin97(double exn_a_top, double exn_a_bot, doub
   return -(dph1 * dgz1 + dph2 * dgz2) / de / (dph1 + dph2);
 }
 
-// full levels k0 .. k0+LC-1 (those below nlev) of one point: the lower half
-// level of level k is the upper one of level k + 1; the chunk's half levels
-// k0 .. k0+LC are loaded (clamped to the column) before any arithmetic
-__device__ __forceinline__ void lin97_levels(int nlev, int k0, int64_t s, int64_t a, int64_t b, double de,
-                                             const double *__restrict__ exn, const double *__restrict__ gz,
-                                             double *out, int64_t so) {
-  double ea[LC + 1], eb[LC + 1], ga[LC + 1], gb[LC + 1];
-#pragma unroll
-  for (int j = 0; j < LC + 1; j++) {
-    const int64_t k = min(k0 + j, nlev) * s;
-    ea[j] = exn[k + a];
-    eb[j] = exn[k + b];
-    ga[j] = gz[k + a];
-    gb[j] = gz[k + b];
-  }
-#pragma unroll
-  for (int j = 0; j < LC; j++)
-    if (k0 + j < nlev) out[(k0 + j) * so] = lin97(ea[j], ea[j + 1], eb[j], eb[j + 1], ga[j], ga[j + 1], gb[j], gb[j + 1], de);
-}
-
+// The thread of cell (i, r), r = 0 .. nlat-2, forms the V point north of it
+// and, off the south pole row, the U point east of it on full levels
+// k0 .. k0+LC-1 (those below nlev): the lower half level of level k is the
+// upper one of level k + 1, and the chunk's half levels k0 .. k0+LC of the
+// cell and its two neighbours are loaded (clamped to the column) before any
+// arithmetic.
 __global__ void __launch_bounds__(NB) k_pgf_lin97(BaroGrid g, Mesh M, const double *__restrict__ exn,
                                                   const double *__restrict__ gz, double *pgf_lon, double *pgf_lat) {
   int64_t o;
-  int i, y, r;
-  if (!plane_point(g, 2 * g.nlat - 3, o, i, y)) return;
-  const int k0 = blockIdx.y * LC;
-  if (u_or_v_row(g, y, r)) {
-    const int ie = i + 1 == g.nlon ? 0 : i + 1;
-    const int64_t a = r * g.pitch + i;
-    lin97_levels(g.nlev, k0, g.sc, a, r * g.pitch + ie, M(GMC_DE_LON, r), exn, gz, pgf_lon + a, g.sc);
-  } else {
-    const int64_t a = r * g.pitch + i;
-    lin97_levels(g.nlev, k0, g.sc, a, a + g.pitch, M(GMC_DE_LAT, r), exn, gz, pgf_lat + a, g.sv);
+  int i, r, ch;
+  if (!plane_point<kByRows>(g, g.nlat - 1, o, i, r, ch)) return;
+  const int k0 = ch * LC;
+  const int64_t oe = o - i + (i + 1 == g.nlon ? 0 : i + 1), on = o + g.pitch;
+  double ea[LC + 1], ga[LC + 1], ee[LC + 1], ge[LC + 1], en[LC + 1], gn[LC + 1];
+#pragma unroll
+  for (int j = 0; j < LC + 1; j++) {
+    const int64_t k = min(k0 + j, g.nlev) * g.sc;
+    ea[j] = exn[k + o];
+    ga[j] = gz[k + o];
+    ee[j] = exn[k + oe];
+    ge[j] = gz[k + oe];
+    en[j] = exn[k + on];
+    gn[j] = gz[k + on];
   }
+  if (r >= 1) {
+    const double de = M(GMC_DE_LON, r);
+#pragma unroll
+    for (int j = 0; j < LC; j++)
+      if (k0 + j < g.nlev)
+        pgf_lon[(k0 + j) * g.sc + o] = lin97(ea[j], ea[j + 1], ee[j], ee[j + 1], ga[j], ga[j + 1], ge[j], ge[j + 1], de);
+  }
+  const double de = M(GMC_DE_LAT, r);
+#pragma unroll
+  for (int j = 0; j < LC; j++)
+    if (k0 + j < g.nlev)
+      pgf_lat[(k0 + j) * g.sv + o] = lin97(ea[j], ea[j + 1], en[j], en[j + 1], ga[j], ga[j + 1], gn[j], gn[j + 1], de);
 }
 
 // pt on level j = -2 .. nlev+1 of a column: the levels themselves, and two
@@ -434,15 +447,18 @@ __device__ __forceinline__ double pt_face_flux(int lim, const double *__restrict
 // chunk's cells k0 .. k1-1 need their faces k0 .. k1; the interior ones are
 // formed here (face k0 also by the chunk above, which writes it), the top
 // and bottom faces are read as they are.  flag[0] is set where a Courant
-// number reaches outside the column.
-__global__ void __launch_bounds__(NB) k_vflx_pt(BaroGrid g, int lim, const double *__restrict__ pt,
+// number reaches outside the column.  LIM: the limiter, whose branches fold
+// at compile time.
+template <int LIM>
+__global__ void __launch_bounds__(NB) k_vflx_pt(BaroGrid g, const double *__restrict__ pt,
                                                 const double *__restrict__ we, const double *__restrict__ cflz,
                                                 double *ptf, double *dptf, int *flag) {
+  constexpr int lim = LIM;
   int64_t o;
-  int i, r;
-  if (!plane_point(g, g.nlat, o, i, r)) return;
+  int i, r, ch;
+  if (!plane_point(g, g.nlat, o, i, r, ch)) return;
   const int64_t s = g.sc;
-  const int n = g.nlev, k0 = blockIdx.y * LC, k1 = min(k0 + LC, n);
+  const int n = g.nlev, k0 = ch * LC, k1 = min(k0 + LC, n);
   const double *col = pt + o;
   bool bad = false;
   double top = k0 == 0 ? ptf[o] : pt_face_flux(lim, col, s, n, k0, cflz[k0 * s + o], we[k0 * s + o], bad);
@@ -468,29 +484,42 @@ __global__ void __launch_bounds__(NB) k_vflx_pt(BaroGrid g, int lim, const doubl
 
 // x-fluxes of mx through the U points of rows 1 .. nlat-2 and y-fluxes of my
 // through every V point, with the winds u / v and Courant numbers cflx / cfly
-// (hflx_ppm_cell / hflx_van_leer_cell, ffsl_mod.F90:556-616, :778-881).  A
-// cflx beyond lon_halo_width sets kFlagCflx (adv_batch_mod.F90:284-287); its
-// whole-cell count is clamped to the bound so that every access stays in the
-// row.
-template <bool VL>
-__global__ void __launch_bounds__(NB) k_hflx_levels(BaroGrid g, AdvGrid ag, int lim, const double *__restrict__ u,
+// (hflx_ppm_cell / hflx_van_leer_cell, ffsl_mod.F90:556-616, :778-881): the
+// thread of cell (i, r), r = 0 .. nlat-2, forms the V point north of it and,
+// off the south pole row, the U point east of it.  A cflx beyond
+// lon_halo_width sets kFlagCflx (adv_batch_mod.F90:284-287); its whole-cell
+// count is clamped to the bound so that every access stays in the row.
+template <bool VL, int LIM>
+__global__ void __launch_bounds__(NB) k_hflx_levels(BaroGrid g, AdvGrid ag, const double *__restrict__ u,
                                                     const double *__restrict__ v, const double *__restrict__ cflx,
                                                     const double *__restrict__ cfly, const double *__restrict__ mx,
                                                     const double *__restrict__ my, double *fx, double *fy, int *flag) {
+  constexpr int lim = LIM;
   int64_t o;
-  int i, y, r;
-  if (!plane_point(g, 2 * g.nlat - 3, o, i, y)) return;
-  const int64_t k = blockIdx.y;
-  if (u_or_v_row(g, y, r)) {
-    const int64_t ro = k * g.sc + r * g.pitch;
-    const double c = cflx[ro + i];
-    if (!(fabs(c) <= ag.hw)) atomicOr(flag, kFlagCflx);
-    fx[ro + i] = xflux<VL>(ag, lim, c, u[ro + i], i, mx + ro);
-  } else {
-    const int64_t lo = k * g.sv + r * g.pitch + i;
-    fy[lo] = yflux_any<VL>(ag, lim, cfly[lo], v[lo], r, i, my + k * g.sc);
+  int i, r, k;
+  if (!plane_point<kByRows>(g, g.nlat - 1, o, i, r, k)) return;
+  const int64_t c = k * g.sc + o, cv = k * g.sv + o;
+  if (r >= 1) {
+    const double cx = cflx[c];
+    if (!(fabs(cx) <= ag.hw)) atomicOr(flag, kFlagCflx);
+    fx[c] = xflux<VL>(ag, lim, cx, u[c], i, mx + (c - i));
   }
+  fy[cv] = yflux_any<VL>(ag, lim, cfly[cv], v[cv], r, i, my + k * g.sc);
 }
+
+using VflxKernel = void (*)(BaroGrid, const double *, const double *, const double *, double *, double *, int *);
+VflxKernel vflx_kernel(int lim) {
+  return lim == GMC_LIMITER_PD ? k_vflx_pt<GMC_LIMITER_PD>
+       : lim == GMC_LIMITER_NONE ? k_vflx_pt<GMC_LIMITER_NONE> : k_vflx_pt<GMC_LIMITER_MONO>;
+}
+using HflxKernel = void (*)(BaroGrid, AdvGrid, const double *, const double *, const double *, const double *,
+                            const double *, const double *, double *, double *, int *);
+template <bool VL>
+HflxKernel hflx_kernel_lim(int lim) {
+  return lim == GMC_LIMITER_PD ? k_hflx_levels<VL, GMC_LIMITER_PD>
+       : lim == GMC_LIMITER_NONE ? k_hflx_levels<VL, GMC_LIMITER_NONE> : k_hflx_levels<VL, GMC_LIMITER_MONO>;
+}
+HflxKernel hflx_kernel(bool vl, int lim) { return vl ? hflx_kernel_lim<true>(lim) : hflx_kernel_lim<false>(lim); }
 
 // the advective half-steps qx, qy of the cells off the poles
 // (ffsl_mod.F90:354-373) from the inner fluxes qmfx, qmfy
@@ -499,9 +528,9 @@ __global__ void __launch_bounds__(NB) k_half_steps(BaroGrid g, Mesh M, double dt
                                                    const double *__restrict__ divx, const double *__restrict__ divy,
                                                    double *qx, double *qy) {
   int64_t o;
-  int i, r;
-  if (!plane_point(g, g.nlat, o, i, r) || r == 0 || r == g.nlat - 1) return;
-  const int64_t c = blockIdx.y * g.sc + o, cv = blockIdx.y * g.sv + o;
+  int i, r, k;
+  if (!plane_point<kByRows>(g, g.nlat, o, i, r, k) || r == 0 || r == g.nlat - 1) return;
+  const int64_t c = k * g.sc + o, cv = k * g.sv + o;
   const int iw = i == 0 ? g.nlon - 1 : i - 1;
   const double qc = q[c];
   const double fx = qmfx[c], fxw = qmfx[c - i + iw], fn = qmfy[cv], fs = qmfy[cv - g.pitch];
@@ -535,9 +564,9 @@ __global__ void __launch_bounds__(kWave) k_half_steps_pole(BaroGrid g, Mesh M, d
 __global__ void __launch_bounds__(NB) k_ptf_div(BaroGrid g, Mesh M, const double *__restrict__ fxl,
                                                 const double *__restrict__ fyl, double *dx, double *dy) {
   int64_t o;
-  int i, r;
-  if (!plane_point(g, g.nlat, o, i, r) || r == 0 || r == g.nlat - 1) return;
-  const int64_t c = blockIdx.y * g.sc + o, cv = blockIdx.y * g.sv + o;
+  int i, r, k;
+  if (!plane_point<kByRows>(g, g.nlat, o, i, r, k) || r == 0 || r == g.nlat - 1) return;
+  const int64_t c = k * g.sc + o, cv = k * g.sv + o;
   const int iw = i == 0 ? g.nlon - 1 : i - 1;
   const double ac = M(GMC_AREA_CELL, r);
   dx[c] = (fxl[c] - fxl[c - i + iw]) * M(GMC_LE_LON, r) / ac;
@@ -782,7 +811,7 @@ int gmc_baro_calc_we_lev(gmc_baro *b) {
   const BaroGrid &g = b->g;
   hipLaunchKernelGGL(k_calc_we_lev, columns(g, g.nlat), dim3(NB), 0, b->stream, g, b->d_hybi, b->f[GMB_DMFDLON],
                      b->f[GMB_DMFDLAT], b->f[GMB_DPHS], b->f[GMB_WE_LEV]);
-  hipLaunchKernelGGL(k_we_edges, chunks(g, 2 * g.nlat - 3, g.nlev + 1), dim3(NB), 0, b->stream, g, Mesh{b->d_mesh, g.nlat},
+  hipLaunchKernelGGL(k_we_edges, chunks(g, g.nlat - 1, g.nlev + 1), dim3(NB), 0, b->stream, g, Mesh{b->d_mesh, g.nlat},
                      b->f[GMB_WE_LEV], b->f[GMB_WE_LEV_LON], b->f[GMB_WE_LEV_LAT]);
   return launched(b, "calc_we_lev");
 }
@@ -799,7 +828,7 @@ int gmc_baro_calc_wedudlev_wedvdlev(gmc_baro *b) {
 int gmc_baro_pgf_lin97(gmc_baro *b) {
   if (!b) return GMC_EINVAL;
   const BaroGrid &g = b->g;
-  hipLaunchKernelGGL(k_pgf_lin97, chunks(g, 2 * g.nlat - 3, g.nlev), dim3(NB), 0, b->stream, g, Mesh{b->d_mesh, g.nlat},
+  hipLaunchKernelGGL(k_pgf_lin97, chunks(g, g.nlat - 1, g.nlev), dim3(NB), 0, b->stream, g, Mesh{b->d_mesh, g.nlat},
                      b->f[GMB_PH_EXN_LEV], b->f[GMB_GZ_LEV], b->f[GMB_PGF_LON], b->f[GMB_PGF_LAT]);
   return launched(b, "pgf_lin97");
 }
@@ -808,7 +837,7 @@ int gmc_baro_calc_ptf_lev(gmc_baro *b, int32_t limiter) {
   if (!b) return GMC_EINVAL;
   if (limiter < GMC_LIMITER_MONO || limiter > GMC_LIMITER_NONE) return berr(b, GMC_EINVAL, "bad limiter");
   const BaroGrid &g = b->g;
-  hipLaunchKernelGGL(k_vflx_pt, chunks(g, g.nlat, g.nlev), dim3(NB), 0, b->stream, g, (int)limiter, b->f[GMB_PT],
+  hipLaunchKernelGGL(vflx_kernel(limiter), chunks(g, g.nlat, g.nlev), dim3(NB), 0, b->stream, g, b->f[GMB_PT],
                      b->f[GMB_WE_LEV], b->f[GMB_CFLZ], b->f[GMB_PTF_LEV], b->f[GMB_DPTFDLEV], b->d_flag);
   const int rc = launched(b, "calc_ptf_lev");
   if (!rc) b->flag_pending = true;
@@ -831,31 +860,19 @@ int gmc_baro_calc_ptf_hflx(gmc_baro *b, int32_t limiter, int32_t flux_type, int3
   ag.hw = lon_halo_width;
   ag.dt = dt;
   const Mesh M{b->d_mesh, g.nlat};
-  const dim3 faces = dim3(chunks(g, 2 * g.nlat - 3, 1).x, g.nlev), cells = dim3(chunks(g, g.nlat, 1).x, g.nlev);
-  const bool vl = flux_type == GMC_FLUX_VAN_LEER;
+  const dim3 faces = dim3(chunks(g, g.nlat - 1, 1).x, g.nlev), cells = dim3(chunks(g, g.nlat, 1).x, g.nlev);
+  const HflxKernel hk = hflx_kernel(flux_type == GMC_FLUX_VAN_LEER, limiter);
   double **f = b->f;
   // inner fluxes of pt with the winds, then the half-steps
-  if (vl)
-    hipLaunchKernelGGL(k_hflx_levels<true>, faces, dim3(NB), 0, b->stream, g, ag, (int)limiter, f[GMB_ADV_U],
-                       f[GMB_ADV_V], f[GMB_ADV_CFLX], f[GMB_ADV_CFLY], f[GMB_PT], f[GMB_PT], b->w_qmfx, b->w_qmfy,
-                       b->d_flag);
-  else
-    hipLaunchKernelGGL(k_hflx_levels<false>, faces, dim3(NB), 0, b->stream, g, ag, (int)limiter, f[GMB_ADV_U],
-                       f[GMB_ADV_V], f[GMB_ADV_CFLX], f[GMB_ADV_CFLY], f[GMB_PT], f[GMB_PT], b->w_qmfx, b->w_qmfy,
-                       b->d_flag);
+  hipLaunchKernelGGL(hk, faces, dim3(NB), 0, b->stream, g, ag, f[GMB_ADV_U], f[GMB_ADV_V], f[GMB_ADV_CFLX],
+                     f[GMB_ADV_CFLY], f[GMB_PT], f[GMB_PT], b->w_qmfx, b->w_qmfy, b->d_flag);
   hipLaunchKernelGGL(k_half_steps, cells, dim3(NB), 0, b->stream, g, M, dt, f[GMB_PT], b->w_qmfx, b->w_qmfy,
                      f[GMB_ADV_DIVX], f[GMB_ADV_DIVY], b->w_qx, b->w_qy);
   hipLaunchKernelGGL(k_half_steps_pole, dim3(2, g.nlev), dim3(kWave), 0, b->stream, g, M, dt, f[GMB_PT], b->w_qmfy,
                      f[GMB_ADV_DIVY], b->w_qx, b->w_qy);
   // outer fluxes of qy along x and qx along y with the mass fluxes
-  if (vl)
-    hipLaunchKernelGGL(k_hflx_levels<true>, faces, dim3(NB), 0, b->stream, g, ag, (int)limiter, f[GMB_ADV_MFX],
-                       f[GMB_ADV_MFY], f[GMB_ADV_CFLX], f[GMB_ADV_CFLY], b->w_qy, b->w_qx, f[GMB_PTF_LON],
-                       f[GMB_PTF_LAT], b->d_flag);
-  else
-    hipLaunchKernelGGL(k_hflx_levels<false>, faces, dim3(NB), 0, b->stream, g, ag, (int)limiter, f[GMB_ADV_MFX],
-                       f[GMB_ADV_MFY], f[GMB_ADV_CFLX], f[GMB_ADV_CFLY], b->w_qy, b->w_qx, f[GMB_PTF_LON],
-                       f[GMB_PTF_LAT], b->d_flag);
+  hipLaunchKernelGGL(hk, faces, dim3(NB), 0, b->stream, g, ag, f[GMB_ADV_MFX], f[GMB_ADV_MFY], f[GMB_ADV_CFLX],
+                     f[GMB_ADV_CFLY], b->w_qy, b->w_qx, f[GMB_PTF_LON], f[GMB_PTF_LAT], b->d_flag);
   hipLaunchKernelGGL(k_ptf_div, cells, dim3(NB), 0, b->stream, g, M, f[GMB_PTF_LON], f[GMB_PTF_LAT],
                      f[GMB_DPTFDLON], f[GMB_DPTFDLAT]);
   hipLaunchKernelGGL(k_ptf_div_pole, dim3(2, g.nlev), dim3(kWave), 0, b->stream, g, M, f[GMB_PTF_LAT],
