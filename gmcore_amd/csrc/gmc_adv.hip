@@ -564,9 +564,10 @@ int copy_rows(gmc_adv *a, double *dst, const double *host, int rows, int64_t ld)
 
 // adv_run: accumulate, maybe advance, copy_old_m (adv_mod.F90:152-312).
 // With tape != null an advancing step leaves its base state (what its
-// tangent and adjoint read) in the tape slot, at the offsets the fields have
-// in the slab: qx, qy and qp are written there, the rest is copied after the
-// flux kernel; *taped_old is the old level, 0 when the step does not advance.
+// tangent and adjoint read: the slab's fields from sm to the end of qy) in the
+// tape slot, at their offsets from sm (tape_bytes): qx, qy and qp are written
+// there, the rest is copied after the flux kernel; *taped_old is the old
+// level, 0 when the step does not advance.
 // before_fix(old level), when given, is called after an advancing step's
 // flux kernel and before its fixer, while the slab holds the step's base
 // state (the new level's seams included).
@@ -596,7 +597,7 @@ int run_once(gmc_adv *a, int64_t time_step, bool basis, double scale, char *tape
     const int ntr = a->cfg.ntracers;
     const dim3 grid(flux_blocks(g, ntr));
     // when taping, qx, qy and qp go straight to the tape slot's copies
-    auto T = [&](double *p) { return tape ? (double *)(tape + ((char *)p - (char *)a->slab)) : p; };
+    auto T = [&](double *p) { return tape ? (double *)(tape + ((char *)p - (char *)a->sm)) : p; };
     double *qx = T(a->qx), *qy = T(a->qy), *qp = T(a->qp);
     InnerPtrs ip{tracer_ptr(a, a->old, 0), qx, qy, a->u, a->v, a->cflx, a->cfly, a->divx, a->divy, ts};
     hipLaunchKernelGGL(inner_kernel(vl, a->cfg.limiter), grid, dim3(NF), 0, a->stream, g, M, ntr, ip);
@@ -607,9 +608,8 @@ int run_once(gmc_adv *a, int64_t time_step, bool basis, double scale, char *tape
     if (tape) {
       // the rest of the step's base state: m, the winds, CFL numbers, split
       // divergences and old_m, and q of both levels (the new level's seams)
-      const size_t off = (char *)a->sm - (char *)a->slab;
       const size_t len = (char *)(a->q + 2 * ts * ntr) - (char *)a->sm;
-      const int rc = ahip(a, hipMemcpyAsync(tape + off, a->sm, len, hipMemcpyDeviceToDevice, a->stream), "tape");
+      const int rc = ahip(a, hipMemcpyAsync(tape, a->sm, len, hipMemcpyDeviceToDevice, a->stream), "tape");
       if (rc) return rc;
       if (taped_old) *taped_old = a->old;
     }
@@ -1566,10 +1566,17 @@ void host_restore(gmc_adv *a, const HostCk &c) {
 
 size_t slot_bytes(const gmc_adv *a) { return (a->slab_bytes + 255) / 256 * 256; }
 size_t tl_fields(const gmc_adv *a) { return (size_t)a->cfg.ntracers * a->g.nlat * a->g.pitch; }
+// a tape slot: the slab's fields from sm to the end of qy (m, the accumulated
+// winds, CFL numbers, split divergences, old_m, q of both levels, qx, qp,
+// qy), not the state winds, the mesh and the counters
+size_t tape_bytes(const gmc_adv *a) {
+  return ((size_t)((const char *)(a->qy + tl_fields(a)) - (const char *)a->sm) + 255) / 256 * 256;
+}
 
 // work fields (5 per tracer: two tangent / adjoint levels, dqx, dqy, dqp,
-// then 4 pole sums per tracer) and nslots slab copies
-int tl_alloc(gmc_adv *a, int nslots) {
+// then 4 pole sums per tracer), nck slab copies (checkpoints) and ntape tape
+// slots after them
+int tl_alloc(gmc_adv *a, int nck, int ntape) {
   int rc = GMC_OK;
   if (!a->tl_f) {
     const size_t bytes = (5 * tl_fields(a) + 4 * (size_t)a->cfg.ntracers) * sizeof(double);
@@ -1580,7 +1587,7 @@ int tl_alloc(gmc_adv *a, int nslots) {
     if (!rc) rc = ahip(a, hipFuncSetAttribute((const void *)a_adv_fix, hipFuncAttributeMaxDynamicSharedMemorySize, lds), "lds");
     if (rc) return rc;
   }
-  const size_t need = (size_t)nslots * slot_bytes(a);
+  const size_t need = (size_t)nck * slot_bytes(a) + (size_t)ntape * tape_bytes(a);
   if (need > a->tl_ws_bytes) {
     if (a->tl_ws) {
       (void)hipStreamSynchronize(a->stream);
@@ -1594,6 +1601,7 @@ int tl_alloc(gmc_adv *a, int nslots) {
   return rc;
 }
 char *tl_slot(gmc_adv *a, int k) { return a->tl_ws + (size_t)k * slot_bytes(a); }
+char *tape_slot(gmc_adv *a, int nck, int j) { return a->tl_ws + (size_t)nck * slot_bytes(a) + (size_t)j * tape_bytes(a); }
 int slab_to(gmc_adv *a, char *dst) {
   return ahip(a, hipMemcpyAsync(dst, a->slab, a->slab_bytes, hipMemcpyDeviceToDevice, a->stream), "checkpoint");
 }
@@ -1614,8 +1622,9 @@ int adj_stream(gmc_adv *a) {
   return rc;
 }
 
+// the fields of a tape slot, or of the live slab with slot = (char *)a->sm
 TBase tape_base(gmc_adv *a, const char *slot, int old) {
-  auto S = [&](const double *p) { return (const double *)(slot + ((const char *)p - (const char *)a->slab)); };
+  auto S = [&](const double *p) { return (const double *)(slot + ((const char *)p - (const char *)a->sm)); };
   return TBase{S(tracer_ptr(a, old, 0)), S(tracer_ptr(a, 3 - old, 0)), S(a->qx), S(a->qy), S(a->qp),
                S(a->u), S(a->v), S(a->mfx), S(a->mfy), S(a->cflx), S(a->cfly), S(a->divx), S(a->divy),
                S(a->old_m), S(a->sm)};
@@ -1682,7 +1691,7 @@ extern "C" {
 int gmc_adv_tl_drive(gmc_adv *a, int64_t time_step0, const double *scale, int32_t nsteps, double *dq, int64_t ld) {
   if (!a) return GMC_EINVAL;
   int rc = tl_check(a, scale, nsteps, time_step0, dq, ld);
-  if (!rc) rc = tl_alloc(a, 0);
+  if (!rc) rc = tl_alloc(a, 0, 0);
   if (rc) return rc;
   const size_t F = tl_fields(a);
   double *cur = a->tl_f, *nxt = a->tl_f + F;
@@ -1690,7 +1699,7 @@ int gmc_adv_tl_drive(gmc_adv *a, int64_t time_step0, const double *scale, int32_
   // each advancing step's tangent runs on the live slab, between the step's
   // flux kernel and its fixer
   const auto tangent = [&](int old) {
-    tl_step(a, (const char *)a->slab, old, cur, nxt);
+    tl_step(a, (const char *)a->sm, old, cur, nxt);
     std::swap(cur, nxt);
   };
   for (int s = 0; s < nsteps && !rc; s++) {
@@ -1720,12 +1729,27 @@ int adv_adjoint(gmc_adv *a, int64_t time_step0, const double *scale, int32_t nst
   const int K = std::min<int>(ckpt_every, std::max<int>(nsteps, 1));
   const int nseg = (nsteps + K - 1) / K;
   // slots: checkpoints 0 .. nseg (the last is the window's end), then two
-  // tape buffers of K slots (one when there is a single segment); segment
-  // seg is taped into buffer seg % 2
-  rc = tl_alloc(a, nseg + 1 + (nseg > 1 ? 2 : 1) * K);
+  // tape buffers (one when there is a single segment) of as many slots as a
+  // segment has advancing steps (adv_run advances at time steps k nstep,
+  // k > 0); segment seg is taped into buffer seg % 2
+  const auto advances = [&](int s) { const int64_t t = time_step0 + s; return t > 0 && t % a->cfg.nstep == 0; };
+  int amax = 0;
+  for (int seg = 0; seg < nseg; seg++) {
+    int c = 0;
+    for (int s = seg * K; s < std::min(nsteps, (seg + 1) * K); s++) c += advances(s);
+    amax = std::max(amax, c);
+  }
+  rc = tl_alloc(a, nseg + 1, (nseg > 1 ? 2 : 1) * amax);
   if (!rc) rc = adj_stream(a);
   if (rc) return rc;
-  auto tape = [&](int seg, int k) { return tl_slot(a, nseg + 1 + (seg % 2) * K + k); };
+  // the tape slot of step s of segment seg: its rank among the segment's
+  // advancing steps (none for a step that does not advance)
+  auto tape = [&](int seg, int s) -> char * {
+    if (!advances(s)) return nullptr;
+    int j = 0;
+    for (int q = seg * K; q < s; q++) j += advances(q);
+    return tape_slot(a, nseg + 1, (seg % 2) * amax + j);
+  };
   std::vector<HostCk> hck(nseg + 1);
   std::vector<int> olds[2] = {std::vector<int>(K), std::vector<int>(K)};
   const int slast = (nseg - 1) * K;  // the last segment is taped on the way
@@ -1735,7 +1759,7 @@ int adv_adjoint(gmc_adv *a, int64_t time_step0, const double *scale, int32_t nst
       rc = slab_to(a, tl_slot(a, s / K));
     }
     if (!rc && s >= slast)
-      rc = run_once(a, time_step0 + s, true, scale[s], tape(nseg - 1, s - slast), &olds[(nseg - 1) % 2][s - slast]);
+      rc = run_once(a, time_step0 + s, true, scale[s], tape(nseg - 1, s), &olds[(nseg - 1) % 2][s - slast]);
     else if (!rc)
       rc = run_once(a, time_step0 + s, true, scale[s]);
   }
@@ -1777,14 +1801,14 @@ int adv_adjoint(gmc_adv *a, int64_t time_step0, const double *scale, int32_t nst
       host_restore(a, hck[seg - 1]);
       if (!rc) rc = fields_from(a, tl_slot(a, seg - 1));
       for (int s = p0; s < p1 && !rc; s++)
-        rc = run_once(a, time_step0 + s, true, scale[s], tape(seg - 1, s - p0), &olds[1 - b][s - p0]);
+        rc = run_once(a, time_step0 + s, true, scale[s], tape(seg - 1, s), &olds[1 - b][s - p0]);
       if (!rc) rc = ahip(a, hipEventRecord(a->ev_ready[1 - b], a->stream), "event");
     }
     if (!rc && seg < nseg - 1) rc = ahip(a, hipStreamWaitEvent(a->astream, a->ev_ready[b], 0), "wait");
     const int s0 = seg * K, s1 = std::min(nsteps, s0 + K);
     for (int s = s1 - 1; s >= s0 && !rc; s--) {
       if (!olds[b][s - s0]) continue;
-      ad_step(a, tape(seg, s - s0), olds[b][s - s0], cur, nxt, a->astream);
+      ad_step(a, tape(seg, s), olds[b][s - s0], cur, nxt, a->astream);
       std::swap(cur, nxt);
       rc = ahip(a, hipGetLastError(), "adjoint launch");
     }

@@ -128,10 +128,13 @@ int gmc_adv_drive(gmc_adv *adv, int64_t time_step0, const double *scale, int32_t
  * adjoint of the old level at entry (dJ/dq_0), in place.  It keeps the slab
  * every ckpt_every steps, then, last segment first, recomputes the segment
  * from its checkpoint keeping every step's base state and runs the adjoint
- * steps back over it.  Memory: nsteps / ckpt_every + 1 checkpoints and two
- * tape buffers of ckpt_every steps (one when ckpt_every >= nsteps), each a
- * copy of the object's device slab, kept between calls (the tangent keeps none:
- * it runs inside each forward step).  The object ends as after
+ * steps back over it.  Memory: nsteps / ckpt_every + 1 checkpoints (copies of
+ * the object's device slab) and two tape buffers (one when ckpt_every >=
+ * nsteps) of one slot per advancing step of a segment (a step advances at time
+ * steps k nstep, k > 0), each slot the step's base fields (m, the accumulated
+ * winds, Courant numbers, split divergences, old_m, the tracers of both
+ * levels, qx, qy, qp), kept between calls (the tangent keeps none: it runs
+ * inside each forward step).  The object ends as after
  * gmc_adv_drive.  The adjoint adds with atomics: reproducible to rounding, not
  * bit for bit. */
 int gmc_adv_tl_drive(gmc_adv *adv, int64_t time_step0, const double *scale, int32_t nsteps, double *dq, int64_t ld);
