@@ -17,6 +17,9 @@ rocprofv3 counters.  step_roofline: the algorithmic bytes of the whole step
 restatement of the reference, gcc -O3, one thread as the reference runs) timed
 on a bounded sample of the same workload; cpu_baseline_all_cores: the same
 code with its row loops on OpenMP threads over the usable host CPUs.
+targets: the two additions' target figures from the same run (the adjoint
+gradients' cost over their forward windows, the batched ensemble's speedup
+over its members run one at a time); skipped with --no-cpu or --no-targets.
 """
 from __future__ import annotations
 
@@ -356,7 +359,7 @@ def run_ens(args):
                      "scope": "whole ensemble step, 280 B per member grid point"},
     }
     ens.close()
-    print(json.dumps(result))
+    return result
 
 
 def run_adj(args):
@@ -454,7 +457,7 @@ def run_adj(args):
     }
     m.close()
     m2.close()
-    print(json.dumps(result))
+    return result
 
 
 def run_adv_adj(args):
@@ -540,7 +543,39 @@ def run_adv_adj(args):
         "gradient_ms_window_taped": t_grad_full * 1e3, "ratio_window_taped": t_grad_full / t_fwd,
         "window_taped_grad_rel_diff": same,
     }
-    print(json.dumps(result))
+    return result
+
+
+def target_figures():
+    """The target figures of the two additions, measured in the same run as
+    the headline: the SW adjoint gradient's cost over its forward window
+    (adj1, target <= 3, dot-product test to 1e-12), the FFSL adjoint's
+    (adjffsl05) and the batched ensemble's member-steps/s over the same
+    members run one at a time (ens64, target >= 2, identical states).  Each
+    is the full line of its own --config, summarised; a failure is reported
+    in place of its figures."""
+    out = {}
+    for name, fn in (("adj1", run_adj), ("adjffsl05", run_adv_adj), ("ens64", run_ens)):
+        ns = argparse.Namespace(config=name, steps=0, warmup=1, steps_given=False, no_cpu=True, cpu_budget=0.0)
+        try:
+            r = fn(ns)
+        except Exception as e:   # keep the headline line whatever happens here
+            out[name] = {"error": f"{type(e).__name__}: {e}"}
+            continue
+        if name == "ens64":
+            out[name] = {"workload": r["config"]["workload"], "member_steps_per_s": r["value"],
+                         "plain_member_steps_per_s": r["plain_member_steps_per_s"],
+                         "speedup_vs_plain": r["speedup_vs_plain"], "identical_to_plain": r["identical_to_plain"],
+                         "target": "speedup >= 2, identical results"}
+        else:
+            out[name] = {"workload": r["config"]["workload"], "gradient_over_forward": r["value"],
+                         "gradient_ms": r["gradient_ms"], "forward_ms": r["forward_ms"],
+                         "dot_product_rel_err": r["dot_product_rel_err"],
+                         "target": "<= 3, dot-product test to 1e-12" if name == "adj1"
+                                   else "dot-product test to 1e-12"}
+            if "ratio_window_taped" in r:
+                out[name]["gradient_over_forward_window_taped"] = r["ratio_window_taped"]
+    return out
 
 
 def run_baro(args):
@@ -650,7 +685,11 @@ def main():
     ap.add_argument("--config", default="sw025",
                     choices=sorted(CONFIGS) + sorted(ADV_CONFIGS) + sorted(ENS_CONFIGS) + sorted(ADJ_CONFIGS)
                     + sorted(ADV_ADJ_CONFIGS) + sorted(BARO_CONFIGS))
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU baseline leg (and the additions' target figures: the GPU leg of the "
+                         "headline workload only, as profiling runs want it)")
+    ap.add_argument("--no-targets", action="store_true",
+                    help="skip the additions' target figures of the default line")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 1 GiB five times (for scaling the HBM "
@@ -676,13 +715,13 @@ def main():
         run_adv(args)
         return
     if args.config in ENS_CONFIGS:
-        run_ens(args)
+        print(json.dumps(run_ens(args)))
         return
     if args.config in ADJ_CONFIGS:
-        run_adj(args)
+        print(json.dumps(run_adj(args)))
         return
     if args.config in ADV_ADJ_CONFIGS:
-        run_adv_adj(args)
+        print(json.dumps(run_adv_adj(args)))
         return
     if args.config in BARO_CONFIGS:
         run_baro(args)
@@ -773,6 +812,9 @@ def main():
         result["cpu_baseline"] = cpu_baseline(st, ic, cfg, args.cpu_budget)
         result["cpu_baseline_all_cores"] = cpu_baseline(st, ic, cfg, args.cpu_budget, omp=True)
     m.close()
+    if not args.no_cpu and not args.no_targets:
+        log("target figures of the additions (adj1, adjffsl05, ens64)")
+        result["targets"] = target_figures()
     print(json.dumps(result))
 
 
