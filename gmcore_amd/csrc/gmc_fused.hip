@@ -32,8 +32,11 @@ constexpr int HXL = 3;                  // column halo
 constexpr int TX = LC - 2 * HXL - 0;    // 58 output columns (q = 3 .. 60)
 // launch-shape parameters (the defaults are the measured best on MI355X;
 // the macros let a sweep build other shapes)
+#ifndef GMC_RPW_MAIN
+#define GMC_RPW_MAIN 1
+#endif
 #ifndef GMC_TY_MAIN
-#define GMC_TY_MAIN 11
+#define GMC_TY_MAIN (GMC_RPW_MAIN == 2 ? 27 : 11)
 #endif
 #ifndef GMC_FILTER_THREADS
 #define GMC_FILTER_THREADS 256
@@ -45,17 +48,23 @@ constexpr int TX = LC - 2 * HXL - 0;    // 58 output columns (q = 3 .. 60)
 #define GMC_NARROW_CHUNK (2 * GMC_FILTER_THREADS)
 #endif
 constexpr int TY_MAIN = GMC_TY_MAIN;    // output rows of the single-member k_tend tile
+constexpr int RPW_MAIN = GMC_RPW_MAIN;  // its tile rows per wavefront
 constexpr int NF = GMC_FILTER_THREADS;  // threads per workgroup of the filter kernels
 constexpr int NXCD = 8;                 // XCDs of an MI355X
 // A k_tend tile with TY output rows: LR = TY + 5 tile rows (r = R0-2 ..
-// R0+TY+2), one wavefront per tile row.
-template <int TY_> struct Tile {
+// R0+TY+2), RPW tile rows per wavefront: wavefront w owns the tile rows
+// w, w + NW, .. (NW = LR / RPW wavefronts).
+template <int TY_, int RPW_ = 1> struct Tile {
   static constexpr int TY = TY_;
+  static constexpr int RPW = RPW_;
   static constexpr int LR = TY + 5;
-  static constexpr int NT = LR * kWave;
+  static_assert(LR % RPW == 0, "the tile rows split evenly over the wavefronts");
+  static constexpr int NW = LR / RPW;
+  static constexpr int NT = NW * kWave;
   static constexpr int NE = LR * LC;
   static constexpr int SH_TILES = 7 * NE;                 // LDS field tiles
   static constexpr int SH_MESH = LR * kRowRec;            // metric row records of the tile
+  static constexpr int NREC = (SH_MESH + NT - 1) / NT;    // record words per thread
   static constexpr int SH_ALL = SH_TILES + SH_MESH + 8;
   static constexpr int kMaxPoleLon = SH_TILES / 3;        // the pole sums keep 3 row terms in the tile area
 };
@@ -143,31 +152,43 @@ struct MeshTile {
   }
 };
 
-// The global loads of one tile, per lane: gz, gzs, u_f, v_f of its tile
-// point, one metric record word, and gz_old of its output point.
-struct TileIn {
-  double gz, gzs, uf, vf, rec, gz_old;
+// The global loads of one tile, per lane and tile row of its wavefront: gz,
+// gzs, u_f, v_f of its tile point and gz_old of its output point; and its
+// words of the metric records.
+template <class S> struct TileIn {
+  double gz[S::RPW], gzs[S::RPW], uf[S::RPW], vf[S::RPW], gz_old[S::RPW];
+  double rec[S::NREC];
 };
 
 template <class S, class PT, class RT>
-__device__ __forceinline__ TileIn tile_fetch(const Grid &g, RT mrows, const PT &P, int C0, int R0) {
+__device__ __forceinline__ TileIn<S> tile_fetch(const Grid &g, RT mrows, const PT &P, int C0, int R0) {
   const int64_t p = g.pitch;
-  const int k = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int q = threadIdx.x % kWave;
-  const int r = R0 - 2 + k;
   const int i = wrap(C0 - HXL + q, g.nlon);
   const int ic = C0 + q - HXL;
-  TileIn in{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  if (k >= 2 && k < S::TY + 2 && q >= HXL && q < HXL + TX && ic < g.nlon && r < g.nlat) in.gz_old = P.gz_old[r * p + ic];
-  if (r >= 0 && r < g.nlat) {
-    in.gz = P.gz_star[r * p + i];
-    in.gzs = P.gzs[r * p + i];
-    in.uf = P.u_f[r * p + i];
+  TileIn<S> in;
+#pragma unroll
+  for (int h = 0; h < S::RPW; h++) {
+    const int k = w + h * S::NW, r = R0 - 2 + k;
+    in.gz[h] = in.gzs[h] = in.uf[h] = in.vf[h] = in.gz_old[h] = 0.0;
+    if (k >= 2 && k < S::TY + 2 && q >= HXL && q < HXL + TX && ic < g.nlon && r < g.nlat)
+      in.gz_old[h] = P.gz_old[r * p + ic];
+    if (r >= 0 && r < g.nlat) {
+      in.gz[h] = P.gz_star[r * p + i];
+      in.gzs[h] = P.gzs[r * p + i];
+      in.uf[h] = P.u_f[r * p + i];
+    }
+    if (r >= 0 && r < g.nhalf) in.vf[h] = P.v_f[r * p + i];
   }
-  if (r >= 0 && r < g.nhalf) in.vf = P.v_f[r * p + i];
-  if (threadIdx.x < S::SH_MESH) {
-    const int rr = R0 - 2 + (int)threadIdx.x / kRowRec;
-    if (rr >= 0 && rr < g.nlat) in.rec = mrows[(int64_t)(rr + 1) * kRowRec + threadIdx.x % kRowRec];
+#pragma unroll
+  for (int j = 0; j < S::NREC; j++) {
+    const int t = threadIdx.x + j * S::NT;
+    in.rec[j] = 0.0;
+    if (t < S::SH_MESH) {
+      const int rr = R0 - 2 + t / kRowRec;
+      if (rr >= 0 && rr < g.nlat) in.rec[j] = mrows[(int64_t)(rr + 1) * kRowRec + t % kRowRec];
+    }
   }
   return in;
 }
@@ -202,7 +223,7 @@ __device__ __forceinline__ void tape_upwind(double *const *dst, int64_t o, const
 template <class S, bool TAPE = false, class MT, class PT>
 __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double grav, double rgrav,
                                           double dt, int stokes, int pv_scheme, int order, double wgt, int coriolis,
-                                          const PT &P, int C0, int R0, double *sh, const TileIn &io,
+                                          const PT &P, int C0, int R0, double *sh, const TileIn<S> &io,
                                           const TapeOut *tape = nullptr) {
   double(*sm)[LC] = (double(*)[LC])(sh + 0 * S::NE);
   double(*suf)[LC] = (double(*)[LC])(sh + 1 * S::NE);
@@ -218,25 +239,34 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double gr
 
   const int64_t p = g.pitch;
   const int nlat = g.nlat, nhalf = g.nhalf;
-  const int k = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);   // tile row of this wavefront
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);   // wavefront: tile rows w, w + NW, ..
   const int q = threadIdx.x % kWave;                                   // tile column of this lane
-  const int r = R0 - 2 + k;                                            // global row (wave-uniform)
   const int i = wrap(C0 - HXL + q, g.nlon);                            // global column
-
-  // output point of this lane (phase E)
-  const int ic = C0 + q - HXL;
+  const int ic = C0 + q - HXL;                                         // output column of this lane (phase E)
+  // tile row k of the wavefront's h-th row, its global row (wave-uniform)
+  // and whether the lane holds an output point there
+#define TILE_ROW(h)                                   \
+  const int k = w + (h) * S::NW;                      \
+  const int r = R0 - 2 + k;                           \
   const bool out_pt = k >= 2 && k < S::TY + 2 && q >= HXL && q < HXL + TX && ic < g.nlon && r < nlat;
-  const double gz_old = io.gz_old;
 
-  // A: m, u_f, v_f of tile row k and the tile's metric rows
-  sm[k][q] = (r >= 0 && r < nlat) ? div_rn(io.gz - io.gzs, grav, rgrav) : 0.0;   // m_of
-  suf[k][q] = io.uf;
-  svf[k][q] = io.vf;
-  if (TAPE && out_pt && tape->u_f) {
-    tape->u_f[r * p + ic] = io.uf;
-    if (r < nhalf) tape->v_f[r * p + ic] = io.vf;
+  // A: m, u_f, v_f of the wavefront's tile rows and the tile's metric rows
+#pragma unroll
+  for (int h = 0; h < S::RPW; h++) {
+    TILE_ROW(h)
+    sm[k][q] = (r >= 0 && r < nlat) ? div_rn(io.gz[h] - io.gzs[h], grav, rgrav) : 0.0;   // m_of
+    suf[k][q] = io.uf[h];
+    svf[k][q] = io.vf[h];
+    if (TAPE && out_pt && tape->u_f) {
+      tape->u_f[r * p + ic] = io.uf[h];
+      if (r < nhalf) tape->v_f[r * p + ic] = io.vf[h];
+    }
   }
-  if (threadIdx.x < S::SH_MESH) sme[threadIdx.x] = io.rec;
+#pragma unroll
+  for (int j = 0; j < S::NREC; j++) {
+    const int t = threadIdx.x + j * S::NT;
+    if (t < S::SH_MESH) sme[t] = io.rec[j];
+  }
   const MeshTile M{sme, R0 - 2};
   __syncthreads();
 
@@ -245,6 +275,9 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double gr
   // so that the compiler interleaves the three dependency chains; the lanes
   // that block adds (mass fluxes on lane 1, ke on lanes 1, 2 and 62) write
   // tile elements that no later phase reads.
+#pragma unroll
+  for (int h = 0; h < S::RPW; h++) {
+  TILE_ROW(h)
   if (!TAPE && k >= 2 && k < S::TY + 3 && r >= 1 && r <= nlat - 3) {
     if (q >= 1 && q < 63) {
       const double m0 = sm[k][q], m1 = sm[k][q + 1], m2 = sm[k + 1][q], m3 = sm[k + 1][q + 1];
@@ -290,12 +323,17 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double gr
     ske[k][q] = v;
   }
   }
+  }
   __syncthreads();
 
   // D: tangential winds and pv on the edges.  v_lon (U rows 2 .. TY+2) and
   // u_lat (V rows 1 .. TY+1) are each read only by the lane that forms them,
   // so they stay in registers; pv_lon replaces u_f and pv_lat replaces v_f
-  // element by element (each lane reads only its own u_f / v_f here).
+  // element by element (each lane reads only its own u_f / v_f here, in each
+  // of its rows).
+#pragma unroll
+  for (int h = 0; h < S::RPW; h++) {
+  TILE_ROW(h)
   double pl = 0.0, pt = 0.0;
   if (!TAPE && k >= 2 && k < S::TY + 2 && r >= 1 && r <= nlat - 2) {
     // both edges in one block (interleaved chains); lane 2 forms a pv_lat
@@ -348,10 +386,15 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double gr
   }
   spl[k][q] = pl;
   spt[k][q] = pt;
+  }
   __syncthreads();
 
   // E: du, dv (forward pass) and gz_new = gz_old + dt dgz on the output rows;
   // rows away from the poles in one block
+#pragma unroll
+  for (int h = 0; h < S::RPW; h++) {
+  TILE_ROW(h)
+  const double gz_old = io.gz_old[h];
   if (r >= 1 && r <= nlat - 2) {
     if (out_pt) {
       const int64_t o = r * p + ic;
@@ -373,7 +416,7 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double gr
       P.dv[o] = -qhu - dkv;
       P.gz_new[o] = gz_old + dt * dgz;
     }
-    return;
+    continue;
   }
   // the pole rows: v on the southern pole edge row, gz from the pole sums
   if (out_pt) {
@@ -391,22 +434,26 @@ __device__ __forceinline__ void tend_tile(const Grid &g, const MT &Mg, double gr
     const double dgz = -(dmfdlon + dmfdlat) * grav;
     P.gz_new[o] = gz_old + dt * dgz;
   }
+  }
+#undef TILE_ROW
 }
 
-// Forward tendencies and the unfiltered gz update on a 58 x 11 tile.
-// Wavefront w owns tile row k = w (global row R0 - 2 + k) in every phase and
-// lane q the column C0 - 3 + q, so metric terms are wave-uniform (broadcast
-// LDS reads of the tile's metric rows) and each phase is one pass over a row.
-// Blocks are dealt to the XCDs round-robin; each XCD gets a contiguous run of
-// tiles (row-major), so vertically adjacent tiles share that XCD's L2.  The
-// XCD holding the north pole walks its run backwards so that the polar tiles,
-// which also form the pole sums, start first.
+// Forward tendencies and the unfiltered gz update on a 58 x TY_MAIN tile.
+// Wavefront w owns tile rows k = w, w + NW, .. (global row R0 - 2 + k) in
+// every phase and lane q the column C0 - 3 + q, so metric terms are
+// wave-uniform (broadcast LDS reads of the tile's metric rows) and each phase
+// is one pass over each of its rows.  Blocks are dealt to the XCDs
+// round-robin; each XCD gets a contiguous run of tiles (row-major), so
+// vertically adjacent tiles share that XCD's L2.  The XCD holding the north
+// pole walks its run backwards so that the polar tiles, which also form the
+// pole sums, start first.
+constexpr int kTendThreads = Tile<TY_MAIN, RPW_MAIN>::NT;
 template <bool TAPE>
-__global__ void __launch_bounds__(Tile<TY_MAIN>::NT, 8) k_tend(Grid g, Mesh Mg, const double *__restrict__ mrows,
+__global__ void __launch_bounds__(kTendThreads, 8 / RPW_MAIN) k_tend(Grid g, Mesh Mg, const double *__restrict__ mrows,
                                                              double grav, double rgrav, double dt, int stokes,
                                                              int pv_scheme, int order, double wgt, int coriolis,
                                                              StagePtrs P, TapeOut tape) {
-  using S = Tile<TY_MAIN>;
+  using S = Tile<TY_MAIN, RPW_MAIN>;
   __shared__ double sh[S::SH_ALL];
   // tile of this block
   const int nx = (g.nlon + TX - 1) / TX, ny = (g.nlat + S::TY - 1) / S::TY, T = nx * ny;
@@ -418,7 +465,7 @@ __global__ void __launch_bounds__(Tile<TY_MAIN>::NT, 8) k_tend(Grid g, Mesh Mg, 
   const int C0 = (t % nx) * TX, R0 = (t / nx) * S::TY;
 
   tile_poles<S>(g, Mg, grav, rgrav, P, R0, sh);
-  TileIn io = tile_fetch<S>(g, mrows, P, C0, R0);
+  const TileIn<S> io = tile_fetch<S>(g, mrows, P, C0, R0);
   tend_tile<S, TAPE>(g, Mg, grav, rgrav, dt, stokes, pv_scheme, order, wgt, coriolis, P, C0, R0, sh, io, &tape);
 }
 
@@ -637,7 +684,7 @@ __global__ void __launch_bounds__(S::NT, 8) k_tend_ens(const EnsMember *__restri
     const int C0 = (lt % d.nx) * TX, R0 = (lt / d.nx) * S::TY;
     const StagePtrsG sp = glb(d.sp);
     tile_poles<S>(d.g, glb(d.M), d.grav, d.rgrav, sp, R0, sh);
-    TileIn io = tile_fetch<S>(d.g, glb(d.mrows), sp, C0, R0);
+    const TileIn<S> io = tile_fetch<S>(d.g, glb(d.mrows), sp, C0, R0);
     tend_tile<S>(d.g, glb(d.M), d.grav, d.rgrav, d.dt, d.stokes, d.pv_scheme, d.order, d.wgt, d.coriolis, sp, C0,
                  R0, sh, io);
     __syncthreads();
@@ -760,7 +807,7 @@ static int build_plan(gmc_ctx *c) {
     if (nb_lon[r] > g.nlon) return set_error(c, GMC_EINVAL, "filter wider than a latitude circle");
   for (int r = 0; r < g.nhalf; r++)
     if (nb_lat[r] > g.nlon) return set_error(c, GMC_EINVAL, "filter wider than a latitude circle");
-  if (g.nlon > Tile<TY_MAIN>::kMaxPoleLon) return set_error(c, GMC_EINVAL, "nlon too large for the fused stage kernels");
+  if (g.nlon > Tile<TY_MAIN, RPW_MAIN>::kMaxPoleLon) return set_error(c, GMC_EINVAL, "nlon too large for the fused stage kernels");
   std::vector<Item> items;
   FusedPlan *P = new FusedPlan();
 
@@ -833,7 +880,7 @@ static void fused_stage(gmc_ctx *c, int o, int s, int n, double dt, bool tape = 
   TapeOut T{S[S_M_LON], S[S_M_LAT], S[S_M_VTX], S[S_MFX_LON], S[S_MFY_LAT], S[S_MFX_LAT], S[S_MFY_LON],
             S[S_U_LAT], S[S_V_LON], S[S_PV], S[S_PV_LON], S[S_PV_LAT], nullptr, nullptr};
   if (dst) T = *dst;
-  hipLaunchKernelGGL(tape ? k_tend<true> : k_tend<false>, dim3(tiles), dim3(Tile<TY_MAIN>::NT), 0, c->stream, g, M,
+  hipLaunchKernelGGL(tape ? k_tend<true> : k_tend<false>, dim3(tiles), dim3(Tile<TY_MAIN, RPW_MAIN>::NT), 0, c->stream, g, M,
                      c->d_mrows, c->cfg.g, 1.0 / c->cfg.g, dt, c->cfg.pv_pole_stokes, c->cfg.pv_scheme,
                      pv_order(c->cfg), c->cfg.upwind_wgt_pv, c->cfg.coriolis_scheme, sp, T);
   prof_mark(c);
