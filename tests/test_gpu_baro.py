@@ -394,3 +394,84 @@ def test_calc_ptf_hflx_reports_cflx_beyond_halo(env):
     g.synchronize()
     with pytest.raises(Exception, match="lon_halo_width"):
         g.calc_ptf_hflx(600.0, lon_halo_width=NLON // 2)   # refused before any launch
+
+
+def test_every_operator_on_a_grid_whose_rows_straddle_wavefronts():
+    """72 x 37 (row pitch 96 doubles: a wavefront spans two latitude rows, so
+    the U/V branches of a cell's thread and the row tests diverge inside a
+    wavefront) with 7 levels (a chunk of 3): every operator against the
+    oracle, bit for bit or to its stated bar."""
+    from gmcore_amd.host import Filter
+    from gmcore_amd.namelist import Namelist
+    nlon, nlat, nlev, hw = 72, 37, 7, 3
+    mesh = Mesh(nlon, nlat, 6371220.0, 7.292e-5)
+    hyai, hybi = oracle.equal_eta(nlev)
+    o = oracle.BaroOracle(mesh.metrics, nlon, nlat, hyai, hybi, P0, PTOP, RD, CPD, P0)
+    g = Baroclinic(mesh, hyai, hybi, P0, PTOP, RD, CPD, P0)
+    r = rng(30)
+    shp, shv, shh = (nlev, nlat, nlon), (nlev, nlat - 1, nlon), (nlev + 1, nlat, nlon)
+    phs = 1.0e5 + 3000.0 * r.standard_normal((1, nlat, nlon))
+    theta = 300.0 + 60.0 * np.linspace(1.0, 0.0, nlev)[:, None, None] + r.standard_normal(shp)
+    gzs = 2000.0 * r.random((1, nlat, nlon))
+    bottom = np.zeros(shh)
+    bottom[-1] = gzs[0]
+    for name, a in (("phs", phs), ("pt", theta), ("gzs", gzs), ("gz_lev", bottom)):
+        g.set(name, a)
+    g.calc_ph()
+    g.calc_t()
+    g.calc_gz_lev()
+    g.pgf_lin97()
+    ph_lev, exn, ph = o.calc_ph(phs)
+    t = o.calc_t(theta, ph)
+    gz_lev, gz = o.calc_gz_lev(gzs, t, ph_lev, bottom)
+    np.testing.assert_array_equal(g.get("ph_lev"), ph_lev)
+    for name, want in (("t", t), ("gz_lev", gz_lev), ("gz", gz)):
+        assert rel(g.get(name), want) < 1e-12, name
+    g.set("ph_exn_lev", exn)
+    g.set("gz_lev", gz_lev)
+    g.pgf_lin97()
+    pl, pt_ = o.pgf_lin97(exn, gz_lev)
+    np.testing.assert_array_equal(g.get("pgf_lon"), pl)
+    np.testing.assert_array_equal(g.get("pgf_lat"), pt_)
+    a, b = 1e-3 * r.standard_normal(shp), 1e-3 * r.standard_normal(shp)
+    g.set("dmfdlon", a)
+    g.set("dmfdlat", b)
+    g.calc_dphsdt()
+    g.calc_we_lev()
+    dphs = o.calc_dphsdt(a, b)
+    we, wlon, wlat = o.calc_we_lev(a, b, dphs)
+    for name, want in (("dphs", dphs), ("we_lev", we), ("we_lev_lon", wlon), ("we_lev_lat", wlat)):
+        np.testing.assert_array_equal(g.get(name), want, err_msg=name)
+    u, v = 30 * r.standard_normal(shp), 30 * r.standard_normal(shv)
+    ml, mt = 1e3 + 4e3 * r.random(shp), 1e3 + 4e3 * r.random(shv)
+    for name, x in (("u_f", u), ("v_f", v), ("m_lon", ml), ("m_lat", mt)):
+        g.set(name, x)
+    g.calc_wedudlev_wedvdlev()
+    du, dv = o.calc_wedudlev_wedvdlev(u, v, ml, mt, wlon, wlat)
+    np.testing.assert_array_equal(g.get("wedudlev"), du)
+    np.testing.assert_array_equal(g.get("wedvdlev"), dv)
+    f = {"adv_u": 20 * r.standard_normal(shp), "adv_v": 20 * r.standard_normal(shv),
+         "adv_mfx": 1e4 * r.standard_normal(shp), "adv_mfy": 1e4 * r.standard_normal(shv),
+         "adv_cflx": r.uniform(-2.9, 2.9, shp), "adv_cfly": r.uniform(-0.9, 0.9, shv),
+         "adv_divx": 1e-5 * r.standard_normal(shp), "adv_divy": 1e-5 * r.standard_normal(shp)}
+    w = r.standard_normal(shh)
+    c = r.uniform(-0.95, 0.95, shh)
+    for k, x in list(f.items()) + [("we_lev", w), ("cflz", c)]:
+        g.set(k, x)
+    g.calc_grad_ptf(600.0, "pd", "ppm", hw)
+    want = o.calc_ptf_hflx(theta, f["adv_u"], f["adv_v"], f["adv_mfx"], f["adv_mfy"], f["adv_cflx"],
+                           f["adv_cfly"], f["adv_divx"], f["adv_divy"], 600.0, 1, 0, hw)
+    for name, x in zip(("ptf_lon", "ptf_lat", "dptfdlon", "dptfdlat"), want):
+        np.testing.assert_array_equal(g.get(name), x, err_msg=name)
+    ptf, dptf, bad = o.calc_ptf_lev(theta, w, c, 1)
+    assert bad == 0
+    np.testing.assert_array_equal(g.get("ptf_lev"), ptf)
+    np.testing.assert_array_equal(g.get("dptfdlev"), dptf)
+    table = Filter(mesh, 3600.0, Namelist(num_lon=nlon, num_lat=nlat, dt_dyn=3600.0), 2)
+    u2, v2 = 20 * r.standard_normal(shp), 20 * r.standard_normal(shv)
+    g.set("u_lon", u2)
+    g.set("v_lat", v2)
+    g.small_filter2(table)
+    fu, fv = o.small_filter2(u2, v2, (table.ngrid_lon, table.wgt_lon, table.ngrid_lat, table.wgt_lat))
+    np.testing.assert_array_equal(g.get("u_lon"), fu)
+    np.testing.assert_array_equal(g.get("v_lat"), fv)
