@@ -62,25 +62,40 @@ struct AccumPtrs {
   const double *ub, *vb;
 };
 
-// grid (column chunks of NA, rows): one column per thread
+// grid (column chunks of NA, rows): one column per thread; the samples and
+// the running sums of the four fields are loaded before any of them is
+// stored (with a basis, u_f and mfx_lon share the basis ub, v_f and mfy_lat
+// the basis vb)
 __global__ void __launch_bounds__(NA) k_adv_accum(AdvGrid g, AccumPtrs p, int mode, int nstep, double scale) {
   const int r = blockIdx.y, i = blockIdx.x * NA + threadIdx.x;
-  if (i < g.nlon) {
-    const int64_t k = r * g.pitch + i;
-    for (int f = 0; f < 4; f++) {
-      if (r >= (f & 1 ? g.nhalf : g.nlat)) continue;
-      double *acc = f == 0 ? p.u : f == 1 ? p.v : f == 2 ? p.mfx : p.mfy;
-      double x;
-      if (p.ub) {
-        x = (f & 1 ? p.vb[k] : p.ub[k]) * scale;
-      } else {
-        const double *s = f == 0 ? p.su : f == 1 ? p.sv : f == 2 ? p.smfx : p.smfy;
-        x = s[k];
-      }
-      if (mode == 0) acc[k] = x;
-      else if (mode == 1) acc[k] = (acc[k] + x) / (double)(nstep + 1);
-      else acc[k] = acc[k] + x;
-    }
+  if (i >= g.nlon) return;
+  const int64_t k = r * g.pitch + i;
+  const bool vrow = r < g.nhalf;   // the V-point fields have nlat - 1 rows
+  double *acc[4] = {p.u, p.v, p.mfx, p.mfy};
+  double x[4], a[4] = {0.0, 0.0, 0.0, 0.0};
+  if (p.ub) {
+    const double xu = p.ub[k] * scale, xv = vrow ? p.vb[k] * scale : 0.0;
+    x[0] = xu;
+    x[1] = xv;
+    x[2] = xu;
+    x[3] = xv;
+  } else {
+    x[0] = p.su[k];
+    x[1] = vrow ? p.sv[k] : 0.0;
+    x[2] = p.smfx[k];
+    x[3] = vrow ? p.smfy[k] : 0.0;
+  }
+  if (mode != 0) {
+#pragma unroll
+    for (int f = 0; f < 4; f++)
+      if (!(f & 1) || vrow) a[f] = acc[f][k];
+  }
+#pragma unroll
+  for (int f = 0; f < 4; f++) {
+    if ((f & 1) && !vrow) continue;
+    if (mode == 0) acc[f][k] = x[f];
+    else if (mode == 1) acc[f][k] = (a[f] + x[f]) / (double)(nstep + 1);
+    else acc[f][k] = a[f] + x[f];
   }
 }
 
