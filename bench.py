@@ -10,8 +10,8 @@ captured HIP graph, inputs resident in HBM.
 
 Prints one JSON line.  value = grid-point-steps per second (nlon * nlat per
 step).  roofline: the dominant kernel's algorithmic bytes per launch over its
-mean launch time from gmc_kernel_times (HIP events around every launch on the
-library's stream) against 8 TB/s, with the HBM traffic per launch measured by
+mean launch time from gmc_kernel_times (a start and an end HIP event per
+launch on the library's stream, stamped by the dispatch) against 8 TB/s, with the HBM traffic per launch measured by
 rocprofv3 counters.  step_roofline: the algorithmic bytes of the whole step
 (35 FP64 field passes per grid point, 280 B/gp) over the step time.  cpu_baseline: the oracle (a scalar C
 restatement of the reference, gcc -O3, one thread as the reference runs) timed
@@ -800,7 +800,7 @@ def main():
                      "kernel": dom["name"],
                      "scope": f"dominant kernel {dom['name']}: algorithmic bytes per launch "
                               f"{KERNEL_BYTES_PER_GP[dom['name']]:.2f} B/grid point over its mean "
-                              "launch time (HIP events on the library stream)"},
+                              "launch time (start and end HIP events of each launch on the library stream)"},
         "step_roofline": {"bound": "hbm", "achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": step_gbs / HBM_PEAK_GBS,
                           "scope": "whole pc2 step (one graph launch): 280 B/grid point algorithmic"},

@@ -25,6 +25,7 @@
 // Longitude neighbours wrap; rows beyond a pole are the mirrored rows with a
 // half-circle shift (the reference's fill_halo on one rank).  Every formula
 // keeps the reference's operation order; the build uses -ffp-contract=off.
+#include <hip/hip_ext.h>
 #include <algorithm>
 #include <functional>
 #include <cmath>
@@ -565,10 +566,21 @@ double *tracer_ptr(gmc_adv *a, int level, int l) {
   return a->q + ((int64_t)(level - 1) * a->cfg.ntracers + l) * a->g.nlat * a->g.pitch;
 }
 
-void mark(gmc_adv *a, int kind) {
-  if (a->prof_ev && a->prof_n < a->prof_cap) {
-    (void)hipEventRecord(a->prof_ev[a->prof_n++], a->stream);
+// A launch of one of the step's kernel classes.  Under gmc_adv_kernel_times
+// the next two profiling events take the kernel's own start and end stamps
+// (hipExtLaunchKernelGGL fills them from the dispatch); every argument is
+// converted to the kernel's parameter type first, since the extended launch
+// passes the arguments' bytes as given.
+template <class... P, class... A>
+void adv_launch(gmc_adv *a, int kind, void (*k)(P...), dim3 grid, dim3 block, uint32_t lds, A &&...args) {
+  static_assert(sizeof...(P) == sizeof...(A), "one argument per kernel parameter");
+  if (a->prof_ev && a->prof_n + 2 <= a->prof_cap) {
+    hipExtLaunchKernelGGL(k, grid, block, lds, a->stream, a->prof_ev[a->prof_n], a->prof_ev[a->prof_n + 1], 0,
+                          static_cast<P>(args)...);
+    a->prof_n += 2;
     a->prof_kind.push_back(kind);
+  } else {
+    hipLaunchKernelGGL(k, grid, block, lds, a->stream, static_cast<P>(args)...);
   }
 }
 
@@ -598,14 +610,12 @@ int run_once(gmc_adv *a, int64_t time_step, bool basis, double scale, char *tape
   AccumPtrs ap{a->u, a->v, a->mfx, a->mfy, a->su, a->sv, a->smfx, a->smfy,
                basis ? a->ub : nullptr, basis ? a->vb : nullptr};
   const dim3 rows((g.nlon + NA - 1) / NA, g.nlat);
-  hipLaunchKernelGGL(k_adv_accum, rows, dim3(NA), 0, a->stream, g, ap, mode, ns, scale);
-  mark(a, KA_ACCUM);
+  adv_launch(a, KA_ACCUM, k_adv_accum, rows, dim3(NA), 0, g, ap, mode, ns, scale);
   a->uv_step = mode == 0 ? 1 : (mode == 1 ? 0 : a->uv_step + 1);
   a->mf_step = a->uv_step;
   if (a->uv_step == 0) {
     CflPtrs cp{a->u, a->v, a->cflx, a->cfly, a->divx, a->divy, a->d_flags};
-    hipLaunchKernelGGL(k_adv_cfl, rows, dim3(NA), 0, a->stream, g, M, cp);
-    mark(a, KA_CFL);
+    adv_launch(a, KA_CFL, k_adv_cfl, rows, dim3(NA), 0, g, M, cp);
   }
   if (time_step > 0 && time_step % ns == 0) {
     const int64_t ts = (int64_t)g.nlat * g.pitch;
@@ -615,11 +625,9 @@ int run_once(gmc_adv *a, int64_t time_step, bool basis, double scale, char *tape
     auto T = [&](double *p) { return tape ? (double *)(tape + ((char *)p - (char *)a->sm)) : p; };
     double *qx = T(a->qx), *qy = T(a->qy), *qp = T(a->qp);
     InnerPtrs ip{tracer_ptr(a, a->old, 0), qx, qy, a->u, a->v, a->cflx, a->cfly, a->divx, a->divy, ts};
-    hipLaunchKernelGGL(inner_kernel(vl, a->cfg.limiter), grid, dim3(NF), 0, a->stream, g, M, ntr, ip);
-    mark(a, KA_INNER);
+    adv_launch(a, KA_INNER, inner_kernel(vl, a->cfg.limiter), grid, dim3(NF), 0, g, M, ntr, ip);
     OuterPtrs op{qx, qy, tracer_ptr(a, a->old, 0), qp, a->mfx, a->mfy, a->cflx, a->cfly, a->old_m, a->sm, ts};
-    hipLaunchKernelGGL(outer_kernel(vl, a->cfg.limiter), grid, dim3(NF), 0, a->stream, g, M, ntr, op);
-    mark(a, KA_FLUX);
+    adv_launch(a, KA_FLUX, outer_kernel(vl, a->cfg.limiter), grid, dim3(NF), 0, g, M, ntr, op);
     if (tape) {
       // the rest of the step's base state: m, the winds, CFL numbers, split
       // divergences and old_m, and q of both levels (the new level's seams)
@@ -630,8 +638,7 @@ int run_once(gmc_adv *a, int64_t time_step, bool basis, double scale, char *tape
     }
     if (before_fix) before_fix(a->old);
     FixPtrs fp{qp, tracer_ptr(a, a->new_, 0), a->sm, a->d_wgt, ts, a->d_flags, a->d_fixes, a->cfg.lon_damp_passes};
-    hipLaunchKernelGGL(k_adv_fix, dim3(g.nlat, ntr), dim3(NA), a->lds_bytes, a->stream, g, fp);
-    mark(a, KA_FIX);
+    adv_launch(a, KA_FIX, k_adv_fix, dim3(g.nlat, ntr), dim3(NA), a->lds_bytes, g, fp);
     std::swap(a->old, a->new_);
   }
   int rc = ahip(a, hipGetLastError(), "advection launch");
@@ -953,7 +960,7 @@ int gmc_adv_kernel_times(gmc_adv *a, int64_t time_step0, const double *scale, in
   if (!a || !out_ms || !n || cap < 1) return GMC_EINVAL;
   if (!a->have_basis) return aerr(a, GMC_ESTATE, "gmc_adv_set_wind_basis must come first");
   if (!scale || nsteps < 1) return aerr(a, GMC_EINVAL, "bad timing arguments");
-  std::vector<hipEvent_t> ev((size_t)nsteps * 5 + 1);
+  std::vector<hipEvent_t> ev((size_t)nsteps * 5 * 2);   // a start and an end stamp per launch
   int rc = GMC_OK;
   for (auto &e : ev)
     if (!rc) rc = ahip(a, hipEventCreate(&e), "event");
@@ -962,15 +969,14 @@ int gmc_adv_kernel_times(gmc_adv *a, int64_t time_step0, const double *scale, in
   a->prof_cap = (int)ev.size();
   a->prof_n = 0;
   a->prof_kind.clear();
-  mark(a, -1);
   for (int s = 0; s < nsteps && !rc; s++) rc = run_once(a, time_step0 + s, true, scale[s]);
   a->prof_ev = nullptr;
   if (!rc) rc = check_flags(a);
   double acc[KA_NCLASS] = {};
   int cnt[KA_NCLASS] = {};
-  for (int k = 1; !rc && k < a->prof_n; k++) {
+  for (int k = 0; !rc && 2 * k + 1 < a->prof_n; k++) {
     float ms = 0.f;
-    rc = ahip(a, hipEventElapsedTime(&ms, ev[k - 1], ev[k]), "elapsed");
+    rc = ahip(a, hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]), "elapsed");
     acc[a->prof_kind[k]] += ms;
     cnt[a->prof_kind[k]]++;
   }

@@ -311,7 +311,7 @@ int gmc_kernel_times(gmc_ctx *c, double dt, int32_t nsteps, double *out_ms, int3
   int rc = fused_prepare(c);
   if (rc) return rc;
   const int stages = c->cfg.time_scheme == GMC_TIME_EULER ? 1 : 3;
-  const int per_stage = kKernelClasses + 1;   // events per stage
+  const int per_stage = 2 * kKernelClasses;   // a start and an end stamp per launch
   std::vector<hipEvent_t> ev((size_t)nsteps * stages * per_stage);
   for (auto &e : ev) {
     rc = check_hip(c, hipEventCreate(&e), "event create");
@@ -332,7 +332,7 @@ int gmc_kernel_times(gmc_ctx *c, double dt, int32_t nsteps, double *out_ms, int3
   for (int b = 0; !rc && b + per_stage <= c->prof_n; b += per_stage, cnt++)
     for (int k = 0; k < kKernelClasses; k++) {
       float ms = 0.f;
-      rc = check_hip(c, hipEventElapsedTime(&ms, ev[b + k], ev[b + k + 1]), "event elapsed");
+      rc = check_hip(c, hipEventElapsedTime(&ms, ev[b + 2 * k], ev[b + 2 * k + 1]), "event elapsed");
       acc[k] += ms;
     }
   for (auto &e : ev) (void)hipEventDestroy(e);

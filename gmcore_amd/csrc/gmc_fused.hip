@@ -15,6 +15,8 @@
 // items so that polar rows with hundreds of taps are spread over many
 // workgroups.  Every expression is the same point formula as the
 // operator-level kernels (gmc_point.hpp), so both paths give the same bits.
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <vector>
 
@@ -860,8 +862,21 @@ void fused_release(gmc_ctx *c) {
 
 const char *const kKernelNames[kKernelClasses] = {"k_tend", "k_gzf", "k_uvf"};
 
-static void prof_mark(gmc_ctx *c) {
-  if (c->prof_ev && c->prof_n < c->prof_cap) (void)hipEventRecord(c->prof_ev[c->prof_n++], c->stream);
+// A stage launch.  Under gmc_kernel_times the next two profiling events take
+// the kernel's own start and end stamps (hipExtLaunchKernelGGL fills them from
+// the dispatch itself, so no event packet sits between them and the kernel);
+// every argument is converted to the kernel's parameter type first, since the
+// extended launch passes the arguments' bytes as given.
+template <class... P, class... A>
+static void stage_launch(gmc_ctx *c, void (*k)(P...), dim3 grid, dim3 block, uint32_t lds, A &&...a) {
+  static_assert(sizeof...(P) == sizeof...(A), "one argument per kernel parameter");
+  if (c->prof_ev && c->prof_n + 2 <= c->prof_cap) {
+    hipExtLaunchKernelGGL(k, grid, block, lds, c->stream, c->prof_ev[c->prof_n], c->prof_ev[c->prof_n + 1], 0,
+                          static_cast<P>(a)...);
+    c->prof_n += 2;
+  } else {
+    hipLaunchKernelGGL(k, grid, block, lds, c->stream, static_cast<P>(a)...);
+  }
 }
 
 // one step_forward_backward stage: old -> new with the star state `star`;
@@ -874,23 +889,18 @@ static void fused_stage(gmc_ctx *c, int o, int s, int n, double dt, bool tape = 
   double **O = c->state[o], **S = c->state[s], **N = c->state[n];
   double *du = c->tend[o][T_DU], *dv = c->tend[o][T_DV];
   double *gz_new = c->tend[o][T_DGZ];   // the unfiltered gz of the new state
-  prof_mark(c);
   StagePtrs sp{S[S_GZ], c->gzs, S[S_U_F], S[S_V_F], O[S_GZ], gz_new, du, dv};
   const int tiles = ((g.nlon + TX - 1) / TX) * ((g.nlat + TY_MAIN - 1) / TY_MAIN);
   TapeOut T{S[S_M_LON], S[S_M_LAT], S[S_M_VTX], S[S_MFX_LON], S[S_MFY_LAT], S[S_MFX_LAT], S[S_MFY_LON],
             S[S_U_LAT], S[S_V_LON], S[S_PV], S[S_PV_LON], S[S_PV_LAT], nullptr, nullptr};
   if (dst) T = *dst;
-  hipLaunchKernelGGL(tape ? k_tend<true> : k_tend<false>, dim3(tiles), dim3(Tile<TY_MAIN, RPW_MAIN>::NT), 0, c->stream, g, M,
-                     c->d_mrows, c->cfg.g, 1.0 / c->cfg.g, dt, c->cfg.pv_pole_stokes, c->cfg.pv_scheme,
-                     pv_order(c->cfg), c->cfg.upwind_wgt_pv, c->cfg.coriolis_scheme, sp, T);
-  prof_mark(c);
-  hipLaunchKernelGGL(k_gzf, dim3(PL->n_gz), dim3(NF), PL->lds_gz, c->stream, g, c->big, c->small1, PL->d_items,
-                     gz_new, N[S_GZ_F], N[S_GZ]);
-  prof_mark(c);
+  stage_launch(c, tape ? k_tend<true> : k_tend<false>, dim3(tiles), dim3(Tile<TY_MAIN, RPW_MAIN>::NT), 0, g, M,
+               c->d_mrows, c->cfg.g, 1.0 / c->cfg.g, dt, c->cfg.pv_pole_stokes, c->cfg.pv_scheme,
+               pv_order(c->cfg), c->cfg.upwind_wgt_pv, c->cfg.coriolis_scheme, sp, T);
+  stage_launch(c, k_gzf, dim3(PL->n_gz), dim3(NF), PL->lds_gz, g, c->big, c->small1, PL->d_items, gz_new,
+               N[S_GZ_F], N[S_GZ]);
   UVPtrs up{O[S_U_LON], O[S_V_LAT], du, dv, N[S_GZ_F], N[S_U_LON], N[S_V_LAT], N[S_U_F], N[S_V_F]};
-  hipLaunchKernelGGL(k_uvf, dim3(PL->n_uv), dim3(NF), PL->lds_uv, c->stream, g, M, dt, c->big,
-                     PL->d_items + PL->n_gz, up);
-  prof_mark(c);
+  stage_launch(c, k_uvf, dim3(PL->n_uv), dim3(NF), PL->lds_uv, g, M, dt, c->big, PL->d_items + PL->n_gz, up);
 }
 
 int fused_prepare(gmc_ctx *c) { return c->fused_plan ? GMC_OK : build_plan(c); }

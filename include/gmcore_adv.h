@@ -164,7 +164,8 @@ int gmc_adv_fix_count(gmc_adv *adv, int64_t *count);
 int gmc_adv_synchronize(gmc_adv *adv);
 int gmc_adv_stream(gmc_adv *adv, void **stream);
 
-/* Per-kernel timing: runs gmc_adv_drive with HIP events around every launch;
+/* Per-kernel timing: runs gmc_adv_drive with a start and an end HIP event per
+ * launch, both stamped by the dispatch itself (hipExtLaunchKernelGGL);
  * out_ms[k] = mean duration (ms) of kernel class k, gmc_adv_kernel_name(k). */
 int gmc_adv_kernel_times(gmc_adv *adv, int64_t time_step0, const double *scale, int32_t nsteps,
                          double *out_ms, int32_t cap, int32_t *n);

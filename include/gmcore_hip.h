@@ -214,8 +214,10 @@ int gmc_keep_derived(gmc_ctx *ctx, int32_t on);
 int gmc_synchronize(gmc_ctx *ctx);
 
 /* Per-kernel timing of the fused step: runs nsteps steps exactly as gmc_step
- * does (the state advances), launched one by one with HIP events around every
- * launch on the context's stream.  out_ms[k] receives the mean duration in
+ * does (the state advances), launched one by one on the context's stream,
+ * each launch with a start and an end HIP event that the dispatch itself
+ * stamps (hipExtLaunchKernelGGL), so the figure is the kernel's own duration
+ * without the gap before it.  out_ms[k] receives the mean duration in
  * milliseconds of kernel class k (at most cap classes, *n set to the count);
  * gmc_kernel_name(k) names them. */
 int gmc_kernel_times(gmc_ctx *ctx, double dt, int32_t nsteps, double *out_ms, int32_t cap, int32_t *n);
