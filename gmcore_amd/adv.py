@@ -145,12 +145,23 @@ class Advection:
                     "ad_drive")
         return b
 
-    def gradient(self, time_step0: int, scales, y, ckpt_every: int = 8):
+    def gradient(self, time_step0: int, scales, y, ckpt_every: int = 8, out=None):
         """(J, dJ/dq_0) of J = 1/2 sum (q_N - y)^2 over the window drive()
-        would run; y has shape (ntracers, nlat, nlon)."""
+        would run; y has shape (ntracers, nlat, nlon).  out: a C-contiguous
+        float64 array of that shape that receives the gradient (a
+        minimisation reuses it from one call to the next); a new array when
+        None."""
         s = np.ascontiguousarray(scales, dtype=np.float64)
-        yb = self._tlad_buf(y)
-        grad = np.empty_like(yb)
+        shape = (self.ntracers, self.nlat, self.nlon)
+        yb = np.ascontiguousarray(y, dtype=np.float64)   # read only: no copy when already C-ordered float64
+        if yb.shape != shape:
+            raise ValueError(f"expected shape {shape}, got {yb.shape}")
+        if out is None:
+            grad = np.empty_like(yb)
+        else:
+            grad = out
+            if grad.dtype != np.float64 or grad.shape != shape or not grad.flags.c_contiguous:
+                raise ValueError(f"out: a C-contiguous float64 array of shape {shape}")
         cost = C.c_double()
         self._check(self.lib.gmc_adv_gradient(self.h, time_step0, dptr(s), len(s), ckpt_every, dptr(yb),
                                               dptr(grad), C.byref(cost), self.nlon), "gradient")

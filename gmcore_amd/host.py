@@ -331,13 +331,22 @@ class TLAD:
     def ad_step(self, nsteps=1, dt=None):
         self._check(self.lib.gmc_ad_step(self.h, self.model.setup.nl.dt_dyn if dt is None else dt, nsteps), "ad_step")
 
-    def gradient(self, nsteps, ckpt_every, obs_cell, obs_value, sigma_o, xb, sigma_b, dt=None):
-        """(J, (dJ/du0, dJ/dv0, dJ/dgz0)); xb = (u_b, v_b, gz_b)"""
+    def gradient(self, nsteps, ckpt_every, obs_cell, obs_value, sigma_o, xb, sigma_b, dt=None, out=None):
+        """(J, (dJ/du0, dJ/dv0, dJ/dgz0)); xb = (u_b, v_b, gz_b).  out: three
+        C-contiguous float64 arrays shaped like xb that receive the gradient
+        (a minimisation reuses them from one call to the next); new arrays
+        when None."""
         m = self.model
         cell = np.ascontiguousarray(obs_cell, dtype=np.int64)
         y = np.ascontiguousarray(obs_value, dtype=np.float64)
         ub, vb, gb = (np.ascontiguousarray(a, dtype=np.float64) for a in xb)
-        gu, gv, gg = np.empty_like(ub), np.empty_like(vb), np.empty_like(gb)   # every element is written
+        if out is None:
+            gu, gv, gg = np.empty_like(ub), np.empty_like(vb), np.empty_like(gb)   # every element is written
+        else:
+            gu, gv, gg = out
+            for o, a in zip(out, (ub, vb, gb)):
+                if o.dtype != np.float64 or o.shape != a.shape or not o.flags.c_contiguous:
+                    raise ValueError("out: C-contiguous float64 arrays shaped like xb")
         J = C.c_double()
         self._check(self.lib.gmc_gradient(self.h, m.setup.nl.dt_dyn if dt is None else dt, nsteps, ckpt_every,
                                           len(cell), cell.ctypes.data_as(C.POINTER(C.c_int64)), dptr(y), sigma_o,
