@@ -286,11 +286,16 @@ int gmc_step(gmc_ctx *c, double dt, int32_t nsteps) {
   if (!c) return GMC_EINVAL;
   if (!c->prepared) return set_error(c, GMC_ESTATE, "gmc_prepare must run before gmc_step");
   if (nsteps < 0) return set_error(c, GMC_EINVAL, "nsteps < 0");
-  for (int n = 0; n < nsteps; n++) {
-    int rc = fused_step_graph(c, c->old_level, c->new_level, dt);
+  constexpr int kStepBlock = 8;   // steps per graph of a long run (even: the levels end where they began)
+  for (int n = 0; n < nsteps;) {
+    const int block = nsteps - n >= kStepBlock ? kStepBlock : 1;
+    int rc = fused_step_graph(c, c->old_level, c->new_level, dt, block);
     if (rc) return rc;
-    derived_after_step(c, c->old_level, c->new_level);
-    std::swap(c->old_level, c->new_level);
+    for (int s = 0; s < block; s++) {
+      derived_after_step(c, c->old_level, c->new_level);
+      std::swap(c->old_level, c->new_level);
+    }
+    n += block;
   }
   return GMC_OK;
 }

@@ -959,25 +959,32 @@ void derived_forget(gmc_ctx *c) {
   for (int &k : c->derived_kept) k = 0;
 }
 
-// gmc_step: replay a captured graph of one fused step
-int fused_step_graph(gmc_ctx *c, int o, int n, double dt) {
+// gmc_step: replay a captured graph of `steps` fused steps from old level o
+// (the levels swap after every step).  Kernels inside one graph follow each
+// other with no gap, while consecutive graph launches leave one (about 9 us at
+// 1440 x 721), so long runs replay graphs of several steps.
+int fused_step_graph(gmc_ctx *c, int o, int n, double dt, int steps) {
   int rc = fused_prepare(c);
   if (rc) return rc;
   hipGraphExec_t exec = nullptr;
   for (auto &sg : c->graphs)
-    if (sg.o == o && sg.n == n && sg.dt == dt && sg.keep == c->keep_derived) exec = sg.exec;
+    if (sg.o == o && sg.n == n && sg.dt == dt && sg.keep == c->keep_derived && sg.steps == steps) exec = sg.exec;
   if (!exec) {
     hipGraph_t graph = nullptr;
     rc = check_hip(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture");
     if (rc) return rc;
-    int rc2 = fused_integrate(c, o, n, dt);
+    int rc2 = GMC_OK, a = o, b = n;
+    for (int s = 0; s < steps && !rc2; s++) {
+      rc2 = fused_integrate(c, a, b, dt);
+      std::swap(a, b);
+    }
     rc = check_hip(c, hipStreamEndCapture(c->stream, &graph), "end capture");
     if (rc2) return rc2;
     if (rc) return rc;
     rc = check_hip(c, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
     (void)hipGraphDestroy(graph);
     if (rc) return rc;
-    c->graphs.push_back({o, n, dt, c->keep_derived, exec});
+    c->graphs.push_back({o, n, dt, c->keep_derived, steps, exec});
   }
   return check_hip(c, hipGraphLaunch(exec, c->stream), "graph launch");
 }

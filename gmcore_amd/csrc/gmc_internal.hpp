@@ -163,9 +163,10 @@ struct gmc_ctx {
   int derived_kept[4] = {0, 0, 0, 0};
   void *fused_plan = nullptr;  // work lists of the fused stage (gmc_fused.hip)
   // graphs of one fused step, keyed by (old, new, dt, keep_derived)
-  struct StepGraph { int o, n; double dt; int keep; hipGraphExec_t exec; };
+  struct StepGraph { int o, n; double dt; int keep, steps; hipGraphExec_t exec; };
   std::vector<StepGraph> graphs;
-  // kernel timing: when non-null, fused_stage records an event after every launch
+  // kernel timing: when non-null, each fused_stage launch takes the next two
+  // events as its start and end stamps
   hipEvent_t *prof_ev = nullptr;
   int prof_n = 0, prof_cap = 0;
   std::string err;
@@ -204,7 +205,7 @@ int fused_plan_info(gmc_ctx *ctx, int32_t out[4]);
 void derived_after_step(gmc_ctx *ctx, int old_l, int new_l);
 // a path that steps without keeping derived fields (ensemble, TL/AD)
 void derived_forget(gmc_ctx *ctx);
-int fused_step_graph(gmc_ctx *ctx, int old_l, int new_l, double dt);
+int fused_step_graph(gmc_ctx *ctx, int old_l, int new_l, double dt, int steps = 1);
 // Destinations of the intermediates a taping stage writes for the adjoint:
 // the fields operators_prepare forms from the star state (mfx_lat and
 // mfy_lon only where not null); when u_f is not null the star state's u_f

@@ -169,8 +169,9 @@ int gmc_time_integrate(gmc_ctx *ctx, int32_t old_level, int32_t new_level, doubl
 /* `time_integrator` (time_schemes_mod.F90:244-288) with step =>
  * step_forward_backward, followed by damp_run (damp_mod.F90:49-63), then the
  * old/new swap of time_advance; repeated nsteps times.  Uses the fused
- * stage kernels, replayed from a HIP graph captured on first use for each
- * (old, new, dt); device resident, nothing crosses PCIe. */
+ * stage kernels, replayed from HIP graphs captured on first use for each
+ * (old, new, dt): graphs of 8 steps while 8 or more remain, then graphs of
+ * one step; device resident, nothing crosses PCIe. */
 int gmc_step(gmc_ctx *ctx, double dt, int32_t nsteps);
 
 /* Current time level index of `old` (1..3) after gmc_step's swaps. */

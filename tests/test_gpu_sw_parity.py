@@ -126,3 +126,33 @@ def test_diagnose_fresh_fields():
     assert abs(dg[0] - do[0]) <= 1e-13 * abs(do[0])       # tm
     for k in (1, 2, 3):
         assert abs(dg[k] - do[k]) <= 1e-12 * abs(do[k]), (k, dg[k], do[k])
+
+
+def test_long_step_runs_equal_single_steps():
+    """gmc_step(ctx, dt, n) replays graphs of 8 steps while 8 or more remain,
+    then graphs of one step: 19 steps in one call (8 + 8 + 3), and 8 after a
+    single step (the block starting from the other level), give the same bits
+    as the same steps taken one call at a time, with keep_derived on for the
+    last steps so that both graph keys are used."""
+    st = Setup(namelist())
+    ic = seeded_ic(st)
+    a, b = make_gpu(st, ic), make_gpu(st, ic)
+    for m in (a, b):
+        m.prepare(1)
+    a.step(19)
+    for _ in range(19):
+        b.step(1)
+    assert a.level == b.level
+    for f in STATE:
+        assert np.array_equal(a.download(a.level, f), b.download(b.level, f)), f
+    for m in (a, b):
+        m.step(1)
+        m.keep_derived(True)
+    a.step(8)
+    for _ in range(8):
+        b.step(1)
+    assert a.level == b.level
+    for f in STATE + ("u_f", "v_f", "gz_f"):
+        assert np.array_equal(a.download(a.level, f), b.download(b.level, f)), f
+    assert np.array_equal(a.download(3, "pv"), b.download(3, "pv"))   # the star level's kept pv
+    assert np.array_equal(a.diagnose(), b.diagnose())
