@@ -1036,7 +1036,8 @@ struct gmc_ens {
   int total_tiles = 0;
   std::vector<double> dt;           // dt the tables were built for
   int built_shape = -1;
-  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  // graphs of one step and of kEnsBlock steps, by the parity they start from
+  hipGraphExec_t graph[2] = {nullptr, nullptr}, graph_block[2] = {nullptr, nullptr};
   hipStream_t stream = nullptr;
   std::string err;
 };
@@ -1054,8 +1055,8 @@ int ehip(gmc_ens *e, hipError_t x, const char *what) {
 }
 
 void drop_graphs(gmc_ens *e) {
-  for (auto &g : e->graph)
-    if (g) { hip_note(hipGraphExecDestroy(g), "ensemble graph destroy"); g = nullptr; }
+  for (hipGraphExec_t *g : {&e->graph[0], &e->graph[1], &e->graph_block[0], &e->graph_block[1]})
+    if (*g) { hip_note(hipGraphExecDestroy(*g), "ensemble graph destroy"); *g = nullptr; }
 }
 
 // descriptor tables for the current shape and dts
@@ -1133,18 +1134,25 @@ void ens_launch_step(gmc_ens *e, int par) {
   }
 }
 
-int ens_graph(gmc_ens *e, int par, hipGraphExec_t *out) {
-  if (e->graph[par]) { *out = e->graph[par]; return GMC_OK; }
+// steps per graph of a long run (even: the parity ends where it began); as
+// gmc_step's, since consecutive graph launches leave a gap that kernels
+// inside one graph do not
+constexpr int kEnsBlock = 8;
+
+// the graph of `steps` (1 or kEnsBlock) ensemble steps from parity par
+int ens_graph(gmc_ens *e, int par, int steps, hipGraphExec_t *out) {
+  hipGraphExec_t &slot = steps == 1 ? e->graph[par] : e->graph_block[par];
+  if (slot) { *out = slot; return GMC_OK; }
   hipGraph_t graph = nullptr;
   int rc = ehip(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal), "begin capture");
   if (rc) return rc;
-  ens_launch_step(e, par);
+  for (int s = 0; s < steps; s++) ens_launch_step(e, (par + s) % 2);
   const hipError_t le = hipGetLastError();
   rc = ehip(e, hipStreamEndCapture(e->stream, &graph), "end capture");
   if (!rc) rc = ehip(e, le, "ensemble launch");
-  if (!rc) rc = ehip(e, hipGraphInstantiate(&e->graph[par], graph, nullptr, nullptr, 0), "graph instantiate");
+  if (!rc) rc = ehip(e, hipGraphInstantiate(&slot, graph, nullptr, nullptr, 0), "graph instantiate");
   if (graph) (void)hipGraphDestroy(graph);
-  if (!rc) *out = e->graph[par];
+  if (!rc) *out = slot;
   return rc;
 }
 
@@ -1163,13 +1171,16 @@ int ens_run(gmc_ens *e, const double *dt, int nsteps) {
     int rc = build_tables(e);
     if (rc) return rc;
   }
-  for (int s = 0; s < nsteps; s++) {
+  for (int s = 0; s < nsteps;) {
     const int par = e->m[0]->old_level == 1 ? 0 : 1;
+    const int steps = nsteps - s >= kEnsBlock ? kEnsBlock : 1;
     hipGraphExec_t g = nullptr;
-    int rc = ens_graph(e, par, &g);
+    int rc = ens_graph(e, par, steps, &g);
     if (!rc) rc = ehip(e, hipGraphLaunch(g, e->stream), "graph launch");
     if (rc) return rc;
-    for (gmc_ctx *c : e->m) std::swap(c->old_level, c->new_level);
+    for (int k = 0; k < steps; k++)
+      for (gmc_ctx *c : e->m) std::swap(c->old_level, c->new_level);
+    s += steps;
   }
   return GMC_OK;
 }

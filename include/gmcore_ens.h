@@ -33,8 +33,9 @@ int gmc_ens_create(gmc_ctx *const *members, int32_t n, gmc_ens **out);
 int gmc_ens_destroy(gmc_ens *ens);
 const char *gmc_ens_last_error(const gmc_ens *ens);
 
-/* nsteps steps of every member, member k with time step dt[k]; replays one
- * captured graph per old/new parity and synchronises at the end. */
+/* nsteps steps of every member, member k with time step dt[k]; replays
+ * captured graphs (per old/new parity: of 8 steps while 8 or more remain,
+ * then of one step) and synchronises at the end. */
 int gmc_ens_step(gmc_ens *ens, const double *dt, int32_t nsteps);
 
 /* Launch shapes: the rows of the tendency kernel's tile and the size of its

@@ -53,6 +53,14 @@ def test_ensemble_equals_plain_runs(scheme):
     for m in plain:
         m.step(2)
     _assert_same(ens_m, plain, "step 5")
+    # long runs replay graphs of 8 steps: from either parity, and the
+    # members' own gmc_step runs in graphs of 8 steps as well
+    for n in (10, 9):
+        ens.step(n)
+        for m in plain:
+            for _ in range(n):
+                m.step(1)
+        _assert_same(ens_m, plain, f"{n} more steps")
     ens.close()
 
 
